@@ -1,0 +1,176 @@
+"""ctypes front end of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: tests/, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of bench.py load this module as the checker.  The HIP
+product path never imports it.
+
+Every function mirrors a reference function of src/fluid_core.rs (cited on the
+C side in oracle/sheet.c) and works in place on numpy arrays, like the
+reference's ``&mut [f32]`` slices.  ``precision`` 32 is the reference's FP32
+arithmetic restated; 64 is the same algorithm in FP64.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+CONFINE = {"reference": 0, "jacobi": 1, "gauss_seidel": 2}
+GRAIN = {"clay": 0, "silt": 1, "sand": 2, "water": 3, "other": 4}
+
+_lib = None
+
+
+def build() -> None:
+    """Compile oracle/liboracle.so with the committed Makefile."""
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        _lib = C.CDLL(LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+_P = C.c_void_p
+_Z = C.c_size_t
+
+
+def _declare(L):
+    for sfx, R in (("_f32", C.c_float), ("_f64", C.c_double)):
+        f = getattr(L, "oracle_step_sheet_ex" + sfx)
+        f.argtypes = [_P, _P, _P, _P, _P, _P, _Z, R, R, C.c_int, R, R, R, R, C.c_int, R,
+                      C.c_int, C.c_int, C.c_int, _P, _P]
+        f.restype = C.c_int
+        for name in ("oracle_transport_open_ref", "oracle_transport_open_rows"):
+            f = getattr(L, name + sfx)
+            f.argtypes = [_P, _P, _P, _Z, R, R, _Z, _Z, _Z, _Z, C.c_int, _P, R, _P, _P]
+            f.restype = None
+        for name in ("oracle_sheet_work_boxes", "oracle_sheet_work_boxes_flood"):
+            f = getattr(L, name + sfx)
+            f.argtypes = [_P, _Z, R, C.c_int, _P, _Z]
+            f.restype = _Z
+        f = getattr(L, "oracle_awake_work_boxes" + sfx)
+        f.argtypes = [_P, _P, _P, _P, _Z, R, R, _P, _Z]
+        f.restype = _Z
+        f = getattr(L, "oracle_dynamic_viscosity" + sfx)
+        f.argtypes = [C.c_int, R]
+        f.restype = R
+        for name in ("oracle_sheet_mobility", "oracle_sheet_damp"):
+            f = getattr(L, name + sfx)
+            f.argtypes = [R]
+            f.restype = R
+        f = getattr(L, "oracle_column_sum" + sfx)
+        f.argtypes = [_P, _Z, R]
+        f.restype = R
+
+
+def _dt(precision):
+    return np.float32 if precision == 32 else np.float64
+
+
+def _sfx(precision):
+    if precision not in (32, 64):
+        raise ValueError("precision must be 32 or 64")
+    return "_f32" if precision == 32 else "_f64"
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+def _check(a, dtype, n, name):
+    if not isinstance(a, np.ndarray) or a.dtype != dtype or not a.flags.c_contiguous or a.size < n:
+        raise TypeError(f"{name}: need a C-contiguous {np.dtype(dtype).name} array of >= {n} items")
+
+
+def step_sheet(h, u, v, bed, nu, gate, res, cell, dt, drive=("gravity", 0.0), *, eps=1.0e-6,
+               banks=False, sub_cap=8.0, pace=0, inner_par=True, confine="reference",
+               precision=32, want_boxes=False):
+    """Reference ``step_sheet_eps`` (src/fluid_core.rs:856-889) on numpy planes.
+
+    ``drive`` is ``("gravity", vorticity)`` or ``("wind", x, z, vorticity)``.
+    ``inner_par`` False is the layer stepped beside the other one inside
+    ``FluidCore::step_pair`` (row splits off, so the coarse confinement is the
+    in-place Gauss-Seidel sweep).  Returns the substep count (and the boxes).
+    """
+    dtype = _dt(precision)
+    n = res * res
+    for a, nm in ((h, "h"), (u, "u"), (v, "v"), (bed, "bed"), (nu, "nu")):
+        _check(a, dtype, n, nm)
+    _check(gate, np.uint8, n, "gate")
+    if drive[0] == "gravity":
+        kind, wx, wz, vort = 0, 0.0, 0.0, float(drive[1])
+    elif drive[0] == "wind":
+        kind, wx, wz, vort = 1, float(drive[1]), float(drive[2]), float(drive[3])
+    else:
+        raise ValueError(drive)
+    boxes = np.zeros(32, dtype=np.uintp)
+    nb = C.c_size_t(0)
+    f = getattr(lib(), "oracle_step_sheet_ex" + _sfx(precision))
+    nsub = f(_ptr(h), _ptr(u), _ptr(v), _ptr(bed), _ptr(nu), _ptr(gate), res, cell, dt, kind, wx,
+             wz, vort, eps, int(banks), sub_cap, int(pace), int(inner_par), CONFINE[confine],
+             _ptr(boxes), C.byref(nb))
+    if want_boxes:
+        k = min(nb.value, 8)
+        return nsub, [tuple(int(x) for x in boxes[4 * i:4 * i + 4]) for i in range(k)]
+    return nsub
+
+
+def transport(h, vel, gate, res, cell, dt, i0, i1, j0, j1, along_x, bed=None, frac_cap=0.45,
+              form="rows", precision=32):
+    """One open transport pass; returns (removed, added) scratch planes.
+
+    ``form`` "serial" is the reference's edge loop (transport_open_ref),
+    "rows" the branch-free row form (transport_open_rows_body).
+    """
+    dtype = _dt(precision)
+    n = res * res
+    removed = np.zeros(n, dtype=dtype)
+    added = np.zeros(n, dtype=dtype)
+    name = "oracle_transport_open_ref" if form == "serial" else "oracle_transport_open_rows"
+    f = getattr(lib(), name + _sfx(precision))
+    f(_ptr(h), _ptr(vel), _ptr(gate), res, cell, dt, i0, i1, j0, j1, int(along_x), _ptr(bed),
+      frac_cap, _ptr(removed), _ptr(added))
+    return removed, added
+
+
+def work_boxes(h, res, eps, allow_split=True, form="runs", precision=32):
+    out = np.zeros(4 * 4096, dtype=np.uintp)
+    name = "oracle_sheet_work_boxes" if form == "runs" else "oracle_sheet_work_boxes_flood"
+    f = getattr(lib(), name + _sfx(precision))
+    k = f(_ptr(h), res, eps, int(allow_split), _ptr(out), 4096)
+    return [tuple(int(x) for x in out[4 * i:4 * i + 4]) for i in range(min(k, 4096))]
+
+
+def awake_boxes(h, u, v, bed, res, cell, eps, precision=32):
+    out = np.zeros(4 * 4096, dtype=np.uintp)
+    f = getattr(lib(), "oracle_awake_work_boxes" + _sfx(precision))
+    k = f(_ptr(h), _ptr(u), _ptr(v), _ptr(bed), res, cell, eps, _ptr(out), 4096)
+    return [tuple(int(x) for x in out[4 * i:4 * i + 4]) for i in range(min(k, 4096))]
+
+
+def dynamic_viscosity(grain, moisture, precision=32):
+    g = GRAIN[grain] if isinstance(grain, str) else int(grain)
+    return getattr(lib(), "oracle_dynamic_viscosity" + _sfx(precision))(g, moisture)
+
+
+def sheet_mobility(nu, precision=32):
+    return getattr(lib(), "oracle_sheet_mobility" + _sfx(precision))(nu)
+
+
+def sheet_damp(nu, precision=32):
+    return getattr(lib(), "oracle_sheet_damp" + _sfx(precision))(nu)
+
+
+def column_sum(h, cell, precision=32):
+    return getattr(lib(), "oracle_column_sum" + _sfx(precision))(_ptr(h), h.size, cell)

@@ -1,0 +1,1045 @@
+/*
+ * oracle/sheet.c -- CPU restatement of the reference's open sheet step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker for the HIP path: only
+ * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load the
+ * library built from it.  The product path (dirty-gen_amd/) never calls it.
+ *
+ * It is compiled twice (see oracle/Makefile): REAL=float with
+ * -ffp-contract=off restates the reference FP32 arithmetic operation by
+ * operation (same order, no fused multiply-add, IEEE division, glibc
+ * expf/powf/hypotf like Rust's std on Linux); REAL=double runs the same
+ * algorithm in FP64 with the reference's f32 constants widened, so it measures
+ * the FP32 rounding error of any implementation.
+ *
+ * Reference: src/fluid_core.rs of x3-team/Dirty-gen ("silt").  Each function
+ * cites the lines it follows.  Only the open sheet (no hydro solids) is
+ * restated: every caller on the sheet-step path passes solid = None.
+ *
+ * Scratch planes start at zero on every call.  The reference keeps them in a
+ * thread-local that outlives calls, but its stale values are only ever read at
+ * cells that are neither wet nor neighbours of wet cells (box halo >= 3), so
+ * the state a step produces does not depend on them.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#ifndef REAL
+#define REAL float
+#define SUFFIX _f32
+#endif
+
+typedef REAL real;
+
+#define PASTE_(a, b) a##b
+#define PASTE(a, b) PASTE_(a, b)
+#define F(name) PASTE(name, SUFFIX)
+
+/* Reference constants are f32; the FP64 build widens the f32 value. */
+#define K(x) ((real)(float)(x))
+
+#if defined(REAL_IS_DOUBLE)
+#define R_EXP exp
+#define R_POW pow
+#define R_SQRT sqrt
+#define R_HYPOT hypot
+#define R_FLOOR floor
+#define R_CEIL ceil
+#define R_FABS fabs
+#define R_FMIN fmin
+#define R_FMAX fmax
+#define R_ISFINITE(x) isfinite(x)
+#else
+#define R_EXP expf
+#define R_POW powf
+#define R_SQRT sqrtf
+#define R_HYPOT hypotf
+#define R_FLOOR floorf
+#define R_CEIL ceilf
+#define R_FABS fabsf
+#define R_FMIN fminf
+#define R_FMAX fmaxf
+#define R_ISFINITE(x) isfinite(x)
+#endif
+
+static const float GAUSS_F[5] = {0.06f, 0.24f, 0.40f, 0.24f, 0.06f};
+static const float STEEP_F[5] = {0.02f, 0.08f, 0.80f, 0.08f, 0.02f};
+#define GRAVITY K(9.81)
+#define FINE_CELL_M K(0.02)
+#define FILM_SPEED_CAP K(1.6)
+
+/* Rust f32::clamp: NaN passes through. */
+static inline real clampr(real x, real lo, real hi) {
+    if (x < lo) x = lo;
+    if (x > hi) x = hi;
+    return x;
+}
+
+static inline size_t clampi(long x, long lo, long hi) {
+    if (x < lo) x = lo;
+    if (x > hi) x = hi;
+    return (size_t)x;
+}
+
+/* ---- constitutive laws: src/fluid_core.rs:150-192 ------------------------ */
+
+/* dynamic_viscosity, src/fluid_core.rs:150-162 */
+real F(oracle_dynamic_viscosity)(int grain, real moisture) {
+    real x = clampr((clampr(moisture, K(0.0), K(1.0)) - K(0.22)) / K(0.70), K(0.0), K(1.0));
+    real thin = x * x * (K(3.0) - K(2.0) * x);
+    real stiff, fluid;
+    switch (grain) {
+    case 1: stiff = K(220.0); fluid = K(4.0); break;   /* Silt */
+    case 2: stiff = K(28.0); fluid = K(0.55); break;   /* Sand */
+    case 3: stiff = K(0.9); fluid = K(0.9); break;     /* Water */
+    case 4: stiff = K(1.5); fluid = K(0.08); break;    /* Other */
+    default: stiff = K(420.0); fluid = K(0.35); break; /* Clay */
+    }
+    return stiff + (fluid - stiff) * thin;
+}
+
+/* sheet_mobility_for, src/fluid_core.rs:169-180 */
+real F(oracle_sheet_mobility)(real nu) {
+    if (nu < K(0.0)) return -nu;
+    real boost = K(0.0);
+    if (nu < K(12.0)) {
+        real t = clampr((K(12.0) - R_FMAX(nu, K(0.0))) / K(12.0), K(0.0), K(1.0));
+        boost = t * t * K(1.6);
+    }
+    return K(0.042) * (K(1.0) + boost);
+}
+
+/* sheet_damp, src/fluid_core.rs:184-192 */
+real F(oracle_sheet_damp)(real nu) {
+    real rel = clampr(R_FMAX(nu, K(0.05)) / K(231.0), K(0.002), K(12.0));
+    if (rel >= K(0.42)) return K(1.70) * R_POW(rel, K(0.30));
+    real at_silt = K(1.70) * R_POW(K(0.42), K(0.30));
+    real t = clampr(rel / K(0.42), K(0.0), K(1.0));
+    return at_silt * R_POW(t, K(1.15));
+}
+
+/* finite_speed, src/fluid_core.rs:3473-3479 */
+static inline real finite_speed(real v, real lim) {
+    if (!R_ISFINITE(v)) return K(0.0);
+    return clampr(v, -lim, lim);
+}
+
+/* wet_sum, src/fluid_core.rs:1305-1319: eight lanes, then the remainder. */
+static real wet_sum(const real *h, size_t n, real eps) {
+    real lanes[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    size_t full = n / 8 * 8;
+    for (size_t c = 0; c < full; c += 8)
+        for (int k = 0; k < 8; k++) lanes[k] += (h[c + k] > eps) ? h[c + k] : K(0.0);
+    real sum = -K(0.0);
+    for (int k = 0; k < 8; k++) sum += lanes[k];
+    for (size_t c = full; c < n; c++) sum += (h[c] > eps) ? h[c] : K(0.0);
+    return sum;
+}
+
+/* ---- work boxes: src/fluid_core.rs:1321-1486, 1527-1657 ----------------- */
+
+typedef struct { size_t i0, i1, j0, j1; } Box;
+
+static int boxes_touch(Box a, Box b) {
+    /* saturating +-1 as in boxes_touch, src/fluid_core.rs:1449-1456 */
+    size_t ai0 = a.i0 ? a.i0 - 1 : 0, bi0 = b.i0 ? b.i0 - 1 : 0;
+    size_t aj0 = a.j0 ? a.j0 - 1 : 0, bj0 = b.j0 ? b.j0 - 1 : 0;
+    return ai0 <= b.i1 + 1 && bi0 <= a.i1 + 1 && aj0 <= b.j1 + 1 && bj0 <= a.j1 + 1;
+}
+
+/* merge_boxes, src/fluid_core.rs:1458-1486: pairwise merge with swap_remove. */
+static void merge_boxes(Box *boxes, size_t *count) {
+    for (;;) {
+        int did = 0;
+        size_t i = 0;
+        while (i < *count) {
+            size_t j = i + 1;
+            while (j < *count) {
+                if (boxes_touch(boxes[i], boxes[j])) {
+                    Box a = boxes[i], b = boxes[j];
+                    boxes[i].i0 = a.i0 < b.i0 ? a.i0 : b.i0;
+                    boxes[i].i1 = a.i1 > b.i1 ? a.i1 : b.i1;
+                    boxes[i].j0 = a.j0 < b.j0 ? a.j0 : b.j0;
+                    boxes[i].j1 = a.j1 > b.j1 ? a.j1 : b.j1;
+                    boxes[j] = boxes[*count - 1]; /* swap_remove */
+                    (*count)--;
+                    did = 1;
+                } else {
+                    j++;
+                }
+            }
+            i++;
+        }
+        if (!did) break;
+    }
+}
+
+static uint32_t uf_root(uint32_t *parent, uint32_t x) {
+    while (parent[x] != x) {
+        uint32_t up = parent[parent[x]];
+        parent[x] = up;
+        x = up;
+    }
+    return x;
+}
+
+typedef struct { size_t first; Box bb; } Body;
+
+static int body_cmp(const void *a, const void *b) {
+    const Body *x = (const Body *)a, *y = (const Body *)b;
+    return (x->first > y->first) - (x->first < y->first);
+}
+
+/*
+ * sheet_work_boxes, src/fluid_core.rs:1325-1447.  Wet = h > eps.  A solid
+ * wet rectangle (or allow_split == 0) is one box with halo 3; otherwise
+ * 4-connected wet bodies by row runs, each grown by 6, merged; one merged box
+ * or more than 8 fall back to the halo-3 rectangle.  Writes at most `cap`
+ * boxes into out, returns the count.
+ */
+size_t F(oracle_sheet_work_boxes)(const real *h, size_t res, real eps, int allow_split,
+                                  size_t *out, size_t cap) {
+    size_t i0 = res, i1 = 0, j0 = res, j1 = 0, wet = 0;
+    for (size_t j = 0; j < res; j++) {
+        const real *row = h + j * res;
+        size_t c = 0, first = res, last = 0;
+        for (size_t i = 0; i < res; i++)
+            if (row[i] > eps) {
+                c++;
+                if (first == res) first = i;
+                last = i;
+            }
+        if (c == 0) continue;
+        wet += c;
+        if (first < i0) i0 = first;
+        if (last > i1) i1 = last;
+        if (j < j0) j0 = j;
+        if (j > j1) j1 = j;
+    }
+    if (wet == 0) return 0;
+    Box halo3 = {i0 >= 3 ? i0 - 3 : 0, (i1 + 3 < res - 1) ? i1 + 3 : res - 1,
+                 j0 >= 3 ? j0 - 3 : 0, (j1 + 3 < res - 1) ? j1 + 3 : res - 1};
+    int solid = wet == (i1 - i0 + 1) * (j1 - j0 + 1);
+    Box *result = NULL;
+    size_t nres = 0;
+    if (!allow_split || solid) {
+        result = &halo3;
+        nres = 1;
+    }
+    size_t nruns = 0, runcap = 64;
+    size_t (*runs)[2] = NULL;
+    uint32_t *parent = NULL;
+    Box *bbox = NULL;
+    size_t *firsts = NULL;
+    Body *bodies = NULL;
+    Box *boxes = NULL;
+    if (!result) {
+        runs = malloc(runcap * sizeof *runs);
+        parent = malloc(runcap * sizeof *parent);
+        bbox = malloc(runcap * sizeof *bbox);
+        firsts = malloc(runcap * sizeof *firsts);
+        size_t prev0 = 0, prev1 = 0;
+        for (size_t j = j0; j <= j1; j++) {
+            const real *row = h + j * res;
+            size_t start = nruns;
+            size_t i = i0;
+            while (i <= i1) {
+                if (!(row[i] > eps)) {
+                    i++;
+                    continue;
+                }
+                size_t a = i;
+                while (i <= i1 && row[i] > eps) i++;
+                size_t b = i - 1;
+                if (nruns == runcap) {
+                    runcap *= 2;
+                    runs = realloc(runs, runcap * sizeof *runs);
+                    parent = realloc(parent, runcap * sizeof *parent);
+                    bbox = realloc(bbox, runcap * sizeof *bbox);
+                    firsts = realloc(firsts, runcap * sizeof *firsts);
+                }
+                uint32_t id = (uint32_t)nruns;
+                runs[nruns][0] = a;
+                runs[nruns][1] = b;
+                parent[nruns] = id;
+                bbox[nruns] = (Box){a, b, j, j};
+                firsts[nruns] = j * res + a;
+                nruns++;
+                for (size_t k = prev0; k < prev1; k++) {
+                    size_t pa = runs[k][0], pb = runs[k][1];
+                    if (pa <= b && a <= pb) {
+                        uint32_t ra = uf_root(parent, (uint32_t)k);
+                        uint32_t rb = uf_root(parent, id);
+                        if (ra != rb) {
+                            uint32_t keep = ra < rb ? ra : rb, gone = ra < rb ? rb : ra;
+                            parent[gone] = keep;
+                            Box g = bbox[gone];
+                            Box *kb = &bbox[keep];
+                            if (g.i0 < kb->i0) kb->i0 = g.i0;
+                            if (g.i1 > kb->i1) kb->i1 = g.i1;
+                            if (g.j0 < kb->j0) kb->j0 = g.j0;
+                            if (g.j1 > kb->j1) kb->j1 = g.j1;
+                            if (firsts[gone] < firsts[keep]) firsts[keep] = firsts[gone];
+                        }
+                    }
+                }
+            }
+            prev0 = start;
+            prev1 = nruns;
+        }
+        bodies = malloc((nruns ? nruns : 1) * sizeof *bodies);
+        size_t nb = 0;
+        for (size_t k = 0; k < nruns; k++)
+            if (parent[k] == (uint32_t)k) bodies[nb++] = (Body){firsts[k], bbox[k]};
+        qsort(bodies, nb, sizeof *bodies, body_cmp);
+        boxes = malloc((nb ? nb : 1) * sizeof *boxes);
+        for (size_t k = 0; k < nb; k++) {
+            Box b = bodies[k].bb;
+            boxes[k] = (Box){b.i0 >= 6 ? b.i0 - 6 : 0, (b.i1 + 6 < res - 1) ? b.i1 + 6 : res - 1,
+                             b.j0 >= 6 ? b.j0 - 6 : 0, (b.j1 + 6 < res - 1) ? b.j1 + 6 : res - 1};
+        }
+        size_t nbox = nb;
+        merge_boxes(boxes, &nbox);
+        if (nbox == 1 || nbox > 8) {
+            result = &halo3;
+            nres = 1;
+        } else {
+            result = boxes;
+            nres = nbox;
+        }
+    }
+    size_t w = 0;
+    for (size_t k = 0; k < nres && w < cap; k++, w++) {
+        out[4 * w + 0] = result[k].i0;
+        out[4 * w + 1] = result[k].i1;
+        out[4 * w + 2] = result[k].j0;
+        out[4 * w + 3] = result[k].j1;
+    }
+    free(runs);
+    free(parent);
+    free(bbox);
+    free(firsts);
+    free(bodies);
+    free(boxes);
+    return nres;
+}
+
+/*
+ * Flood-fill statement of the same boxes (the reference test's
+ * sheet_work_boxes_ref, src/fluid_core.rs:11134-11226).  Kept so the
+ * run-length version above can be checked against an independent form.
+ */
+size_t F(oracle_sheet_work_boxes_flood)(const real *h, size_t res, real eps, int allow_split,
+                                        size_t *out, size_t cap) {
+    size_t i0 = res, i1 = 0, j0 = res, j1 = 0, wet = 0;
+    for (size_t j = 0; j < res; j++)
+        for (size_t i = 0; i < res; i++)
+            if (h[j * res + i] > eps) {
+                wet++;
+                if (i < i0) i0 = i;
+                if (i > i1) i1 = i;
+                if (j < j0) j0 = j;
+                if (j > j1) j1 = j;
+            }
+    if (wet == 0) return 0;
+    Box halo3 = {i0 >= 3 ? i0 - 3 : 0, (i1 + 3 < res - 1) ? i1 + 3 : res - 1,
+                 j0 >= 3 ? j0 - 3 : 0, (j1 + 3 < res - 1) ? j1 + 3 : res - 1};
+    int solid = wet == (i1 - i0 + 1) * (j1 - j0 + 1);
+    if (!allow_split || solid) {
+        if (cap) {
+            out[0] = halo3.i0; out[1] = halo3.i1; out[2] = halo3.j0; out[3] = halo3.j1;
+        }
+        return 1;
+    }
+    uint8_t *mark = calloc(res * res, 1);
+    size_t *queue = malloc(res * res * sizeof *queue);
+    Box *boxes = malloc(res * res * sizeof *boxes);
+    size_t nbox = 0;
+    for (size_t j = j0; j <= j1; j++)
+        for (size_t i = i0; i <= i1; i++) {
+            size_t s = j * res + i;
+            if (h[s] <= eps || mark[s]) continue;
+            size_t ci0 = i, ci1 = i, cj0 = j, cj1 = j, qn = 0;
+            mark[s] = 1;
+            queue[qn++] = s;
+            while (qn) {
+                size_t cur = queue[--qn];
+                size_t cj = cur / res, ci = cur - cj * res;
+                if (ci < ci0) ci0 = ci;
+                if (ci > ci1) ci1 = ci;
+                if (cj < cj0) cj0 = cj;
+                if (cj > cj1) cj1 = cj;
+                long nb[4] = {ci > 0 ? (long)cur - 1 : -1, ci + 1 < res ? (long)cur + 1 : -1,
+                              cj > 0 ? (long)(cur - res) : -1, cj + 1 < res ? (long)(cur + res) : -1};
+                for (int q = 0; q < 4; q++)
+                    if (nb[q] >= 0 && !mark[nb[q]] && h[nb[q]] > eps) {
+                        mark[nb[q]] = 1;
+                        queue[qn++] = (size_t)nb[q];
+                    }
+            }
+            boxes[nbox++] = (Box){ci0 >= 6 ? ci0 - 6 : 0, (ci1 + 6 < res - 1) ? ci1 + 6 : res - 1,
+                                  cj0 >= 6 ? cj0 - 6 : 0, (cj1 + 6 < res - 1) ? cj1 + 6 : res - 1};
+        }
+    merge_boxes(boxes, &nbox);
+    size_t n = 0;
+    if (nbox == 1 || nbox > 8) {
+        if (cap) {
+            out[0] = halo3.i0; out[1] = halo3.i1; out[2] = halo3.j0; out[3] = halo3.j1;
+        }
+        n = 1;
+    } else {
+        for (size_t k = 0; k < nbox && k < cap; k++) {
+            out[4 * k + 0] = boxes[k].i0;
+            out[4 * k + 1] = boxes[k].i1;
+            out[4 * k + 2] = boxes[k].j0;
+            out[4 * k + 3] = boxes[k].j1;
+        }
+        n = nbox;
+    }
+    free(mark);
+    free(queue);
+    free(boxes);
+    return n;
+}
+
+/*
+ * awake_work_boxes, src/fluid_core.rs:1537-1657.  8x8 tiles are awake when a
+ * wet cell moves faster than 0.05 m/s or its free surface steps more than
+ * 0.02*cell to the +x or +z neighbour; awake tiles grow by one tile and each
+ * 4-connected group of grown tiles becomes one box (first tile in raster order).
+ */
+size_t F(oracle_awake_work_boxes)(const real *h, const real *u, const real *v, const real *bed,
+                                  size_t res, real cell, real eps, size_t *out, size_t cap) {
+    const size_t T = 8;
+    size_t tiles = (res + T - 1) / T, ntile = tiles * tiles;
+    uint8_t *hot = calloc(ntile, 1), *grown = calloc(ntile, 1), *seen = calloc(ntile, 1);
+    size_t *stack = malloc(ntile * sizeof *stack);
+    for (size_t tj = 0; tj < tiles; tj++)
+        for (size_t ti = 0; ti < tiles; ti++) {
+            size_t i0 = ti * T, j0 = tj * T;
+            size_t i1 = i0 + T < res ? i0 + T : res, j1 = j0 + T < res ? j0 + T : res;
+            int awake = 0;
+            for (size_t j = j0; j < j1 && !awake; j++) {
+                size_t row = j * res;
+                for (size_t i = i0; i < i1; i++) {
+                    size_t idx = row + i;
+                    if (h[idx] <= eps) continue;
+                    if (R_HYPOT(u[idx], v[idx]) > K(0.05)) {
+                        awake = 1;
+                        break;
+                    }
+                    real eta = bed[idx] + h[idx];
+                    size_t ip = i + 1 < res ? row + i + 1 : idx;
+                    size_t jp = j + 1 < res ? (j + 1) * res + i : idx;
+                    real dex = R_FABS(bed[ip] + h[ip] - eta);
+                    real dez = R_FABS(bed[jp] + h[jp] - eta);
+                    if (R_FMAX(dex, dez) > K(0.02) * cell) {
+                        awake = 1;
+                        break;
+                    }
+                }
+            }
+            if (awake) hot[tj * tiles + ti] = 1;
+        }
+    for (size_t tj = 0; tj < tiles; tj++)
+        for (size_t ti = 0; ti < tiles; ti++) {
+            if (!hot[tj * tiles + ti]) continue;
+            for (int dj = -1; dj <= 1; dj++)
+                for (int di = -1; di <= 1; di++) {
+                    long ni = (long)ti + di, nj = (long)tj + dj;
+                    if (ni >= 0 && nj >= 0 && ni < (long)tiles && nj < (long)tiles)
+                        grown[nj * tiles + ni] = 1;
+                }
+        }
+    size_t nbox = 0;
+    for (size_t s = 0; s < ntile; s++) {
+        if (!grown[s] || seen[s]) continue;
+        seen[s] = 1;
+        size_t sp = 0;
+        stack[sp++] = s;
+        size_t ti0 = s % tiles, ti1 = ti0, tj0 = s / tiles, tj1 = tj0;
+        while (sp) {
+            size_t cur = stack[--sp];
+            size_t tj = cur / tiles, ti = cur - tj * tiles;
+            if (ti < ti0) ti0 = ti;
+            if (ti > ti1) ti1 = ti;
+            if (tj < tj0) tj0 = tj;
+            if (tj > tj1) tj1 = tj;
+            size_t nb[4];
+            int nn = 0;
+            if (ti > 0) nb[nn++] = tj * tiles + ti - 1;
+            if (ti + 1 < tiles) nb[nn++] = tj * tiles + ti + 1;
+            if (tj > 0) nb[nn++] = (tj - 1) * tiles + ti;
+            if (tj + 1 < tiles) nb[nn++] = (tj + 1) * tiles + ti;
+            for (int q = 0; q < nn; q++)
+                if (grown[nb[q]] && !seen[nb[q]]) {
+                    seen[nb[q]] = 1;
+                    stack[sp++] = nb[q];
+                }
+        }
+        if (nbox < cap) {
+            size_t e_i = (ti1 + 1) * T < res ? (ti1 + 1) * T : res;
+            size_t e_j = (tj1 + 1) * T < res ? (tj1 + 1) * T : res;
+            out[4 * nbox + 0] = ti0 * T;
+            out[4 * nbox + 1] = e_i - 1;
+            out[4 * nbox + 2] = tj0 * T;
+            out[4 * nbox + 3] = e_j - 1;
+        }
+        nbox++;
+    }
+    free(hot);
+    free(grown);
+    free(seen);
+    free(stack);
+    return nbox;
+}
+
+/* ---- stencil helpers: src/fluid_core.rs:3489-3539 ----------------------- */
+
+static inline real vorticity_at(const real *u, const real *v, size_t res, size_t i, size_t j,
+                                real cell) {
+    size_t ip = i + 1 < res ? i + 1 : res - 1, im = i ? i - 1 : 0;
+    size_t jp = j + 1 < res ? j + 1 : res - 1, jm = j ? j - 1 : 0;
+    real dvdx = (v[j * res + ip] - v[j * res + im]) / (K(2.0) * cell);
+    real dudz = (u[jp * res + i] - u[jm * res + i]) / (K(2.0) * cell);
+    return dvdx - dudz;
+}
+
+static inline void grad(const real *p, size_t res, size_t i, size_t j, real cell, real *dx,
+                        real *dz) {
+    size_t ip = i + 1 < res ? i + 1 : res - 1, im = i ? i - 1 : 0;
+    size_t jp = j + 1 < res ? j + 1 : res - 1, jm = j ? j - 1 : 0;
+    *dx = (p[j * res + ip] - p[j * res + im]) / (K(2.0) * cell);
+    *dz = (p[jp * res + i] - p[jm * res + i]) / (K(2.0) * cell);
+}
+
+static inline real lap(const real *p, size_t res, size_t i, size_t j, real cell) {
+    size_t idx = j * res + i;
+    size_t ip = i + 1 < res ? i + 1 : res - 1, im = i ? i - 1 : 0;
+    size_t jp = j + 1 < res ? j + 1 : res - 1, jm = j ? j - 1 : 0;
+    real s = p[j * res + ip] + p[j * res + im] + p[jp * res + i] + p[jm * res + i] - K(4.0) * p[idx];
+    return s / (cell * cell);
+}
+
+/*
+ * One vorticity-confinement kick at (i, j) (confine_one / confine_interior /
+ * confine_jacobi, src/fluid_core.rs:2402-2503, 3442-3471).  The vorticity is
+ * read from (su, sv); the kick is added to (u, v).  Gauss-Seidel passes the
+ * live planes as both; Jacobi passes a snapshot as (su, sv).  Clamped indices
+ * equal the interior stencil away from the grid edge.
+ */
+static inline void confine(const real *su, const real *sv, real *u, real *v, size_t res, size_t i,
+                           size_t j, real cell, real dt, real eps) {
+    real w = vorticity_at(su, sv, res, i, j, cell);
+    if (!R_ISFINITE(w) || R_FABS(w) < K(1.0e-4)) return;
+    real wxp = R_FABS(vorticity_at(su, sv, res, i + 1 < res ? i + 1 : res - 1, j, cell));
+    real wxm = R_FABS(vorticity_at(su, sv, res, i ? i - 1 : 0, j, cell));
+    real wzp = R_FABS(vorticity_at(su, sv, res, i, j + 1 < res ? j + 1 : res - 1, cell));
+    real wzm = R_FABS(vorticity_at(su, sv, res, i, j ? j - 1 : 0, cell));
+    real nx = (wxp - wxm) / (K(2.0) * cell);
+    real nz = (wzp - wzm) / (K(2.0) * cell);
+    real du = eps * dt * (-nz * w);
+    real dv = eps * dt * (nx * w);
+    if (R_ISFINITE(du) && R_ISFINITE(dv) && du * du + dv * dv <= K(0.0625)) {
+        size_t idx = j * res + i;
+        u[idx] += du;
+        v[idx] += dv;
+    }
+}
+
+/* accum5, src/fluid_core.rs:3681-3688: left to right, first product unadded. */
+static inline real accum5(real a0, real a1, real a2, real a3, real a4, const float *k) {
+    real a = a0 * K(k[0]);
+    a += a1 * K(k[1]);
+    a += a2 * K(k[2]);
+    a += a3 * K(k[3]);
+    a += a4 * K(k[4]);
+    return a;
+}
+
+/*
+ * blur_box, src/fluid_core.rs:3541-3569 with blur_axis_x/z (:3700-3757): a
+ * 5-tap pass along x on the box rows into tmp, then along z into dst.  Taps
+ * clamp at the grid edge, where the sum starts from 0.0 (accum5_clamped).
+ */
+static void blur_box(const real *src, real *dst, real *tmp, size_t res, Box b, const float *k) {
+    if (b.i0 > b.i1 || b.j0 > b.j1 || res == 0) return;
+    for (size_t j = b.j0; j <= b.j1; j++) {
+        const real *row = src + j * res;
+        for (size_t i = b.i0; i <= b.i1; i++) {
+            real a;
+            if (i >= 2 && i + 2 < res) {
+                a = accum5(row[i - 2], row[i - 1], row[i], row[i + 1], row[i + 2], k);
+            } else {
+                a = K(0.0);
+                for (int t = -2; t <= 2; t++) a += row[clampi((long)i + t, 0, (long)res - 1)] * K(k[t + 2]);
+            }
+            tmp[j * res + i] = a;
+        }
+    }
+    for (size_t j = b.j0; j <= b.j1; j++) {
+        for (size_t i = b.i0; i <= b.i1; i++) {
+            real a;
+            if (j >= 2 && j + 2 < res) {
+                a = accum5(tmp[(j - 2) * res + i], tmp[(j - 1) * res + i], tmp[j * res + i],
+                           tmp[(j + 1) * res + i], tmp[(j + 2) * res + i], k);
+            } else {
+                a = K(0.0);
+                for (int t = -2; t <= 2; t++)
+                    a += tmp[clampi((long)j + t, 0, (long)res - 1) * res + i] * K(k[t + 2]);
+            }
+            dst[j * res + i] = a;
+        }
+    }
+}
+
+/* ---- open transport: src/fluid_core.rs:2847-3215 ------------------------ */
+
+/* dry_bank, src/fluid_core.rs:2843-2845 */
+static inline int dry_bank(const real *h, const real *bed, size_t donor, size_t recv) {
+    return h[recv] <= K(1.0e-6) && bed[recv] > bed[donor] + h[donor] + K(1.0e-3);
+}
+
+/* flux_open, src/fluid_core.rs:3412-3440 */
+static inline void flux_open(const real *h, const uint8_t *gate, size_t a, size_t b, real vel,
+                             real cell, real dt, real frac_cap, real *removed, real *added) {
+    size_t donor = vel > K(0.0) ? a : b, recv = vel > K(0.0) ? b : a;
+    real room = h[donor] - removed[donor];
+    if (room <= K(1.0e-8)) return;
+    if (gate[recv] == 0 && h[recv] <= K(1.0e-6)) return;
+    real frac = R_FMIN(R_FABS(vel) * dt / cell, frac_cap);
+    real dh = R_FMIN(h[donor] * frac, room);
+    if (dh <= K(1.0e-8)) return;
+    removed[donor] += dh;
+    added[recv] += dh;
+}
+
+static void apply_moves(real *h, const real *removed, const real *added, size_t res, Box b) {
+    for (size_t j = b.j0; j <= b.j1; j++)
+        for (size_t i = b.i0; i <= b.i1; i++) {
+            size_t idx = j * res + i;
+            real r = removed[idx], a = added[idx];
+            real moved = R_FMAX(h[idx] - r + a, K(0.0));
+            h[idx] = (r > K(0.0) || a > K(0.0)) ? moved : h[idx];
+        }
+}
+
+/*
+ * Serial edge loop (transport_open_ref, src/fluid_core.rs:3139-3215; the
+ * rayon form transport_open_par :3219-3397 computes the same edges).  Edges
+ * interior to the box only: its outer faces act as walls.
+ */
+void F(oracle_transport_open_ref)(real *h, const real *vel, const uint8_t *gate, size_t res,
+                                  real cell, real dt, size_t i0, size_t i1, size_t j0, size_t j1,
+                                  int along_x, const real *bed, real frac_cap, real *removed,
+                                  real *added) {
+    Box b = {i0, i1, j0, j1};
+    for (size_t j = j0; j <= j1; j++)
+        for (size_t i = i0; i <= i1; i++) removed[j * res + i] = added[j * res + i] = K(0.0);
+    if (along_x) {
+        for (size_t j = j0; j <= j1; j++)
+            for (size_t i = i0; i < i1; i++) {
+                size_t a = j * res + i, bb = a + 1;
+                real speed = K(0.5) * (vel[a] + vel[bb]);
+                if (R_FABS(speed) < K(1.0e-5)) continue;
+                if (bed) {
+                    size_t d = speed > K(0.0) ? a : bb, r = speed > K(0.0) ? bb : a;
+                    if (dry_bank(h, bed, d, r)) continue;
+                }
+                flux_open(h, gate, a, bb, speed, cell, dt, frac_cap, removed, added);
+            }
+    } else {
+        for (size_t j = j0; j < j1; j++)
+            for (size_t i = i0; i <= i1; i++) {
+                size_t a = j * res + i, bb = a + res;
+                real speed = K(0.5) * (vel[a] + vel[bb]);
+                if (R_FABS(speed) < K(1.0e-5)) continue;
+                if (bed) {
+                    size_t d = speed > K(0.0) ? a : bb, r = speed > K(0.0) ? bb : a;
+                    if (dry_bank(h, bed, d, r)) continue;
+                }
+                flux_open(h, gate, a, bb, speed, cell, dt, frac_cap, removed, added);
+            }
+    }
+    apply_moves(h, removed, added, res, b);
+}
+
+/* open_edge_dh, src/fluid_core.rs:2887-2911 */
+static inline real open_edge_dh(real hd, real hr, real remd, uint8_t gate_r, int use_bed, real bed_d,
+                                real bed_r, real speed, real dt, real cell, real frac_cap) {
+    int live = !(R_FABS(speed) < K(1.0e-5));
+    int bank = use_bed && hr <= K(1.0e-6) && bed_r > bed_d + hd + K(1.0e-3);
+    real room = hd - remd;
+    int gate_shut = gate_r == 0 && hr <= K(1.0e-6);
+    real frac = R_FMIN(R_FABS(speed) * dt / cell, frac_cap);
+    real dh = R_FMIN(hd * frac, room);
+    int ok = live && !bank && !(room <= K(1.0e-8)) && !gate_shut && !(dh <= K(1.0e-8));
+    return ok ? dh : K(0.0);
+}
+
+/*
+ * Branch-free row form (transport_open_rows_body, src/fluid_core.rs:2998-3137):
+ * left-going fluxes first, then right-going, then each cell sums its two
+ * edges in the serial order.  Equal to the serial loop bit for bit.
+ */
+void F(oracle_transport_open_rows)(real *h, const real *vel, const uint8_t *gate, size_t res,
+                                   real cell, real dt, size_t i0, size_t i1, size_t j0, size_t j1,
+                                   int along_x, const real *bed, real frac_cap, real *removed,
+                                   real *added) {
+    size_t m = i1 + 1 - i0;
+    int ub = bed != NULL;
+    if (along_x) {
+        real *fl = calloc(m, sizeof *fl), *fr = calloc(m, sizeof *fr);
+        for (size_t j = j0; j <= j1; j++) {
+            size_t lo = j * res + i0;
+            real *hr = h + lo;
+            const real *vr = vel + lo;
+            const uint8_t *gr = gate + lo;
+            const real *br = ub ? bed + lo : NULL;
+            size_t edges = m - 1;
+            for (size_t k = 0; k < edges; k++) {
+                real speed = K(0.5) * (vr[k] + vr[k + 1]);
+                real dh = open_edge_dh(hr[k + 1], hr[k], K(0.0), gr[k], ub, ub ? br[k + 1] : 0,
+                                       ub ? br[k] : 0, speed, dt, cell, frac_cap);
+                fl[k] = speed > K(0.0) ? K(0.0) : dh;
+            }
+            for (size_t k = 0; k < edges; k++) {
+                real speed = K(0.5) * (vr[k] + vr[k + 1]);
+                real prev = k > 0 ? fl[k - 1] : K(0.0);
+                real dh = open_edge_dh(hr[k], hr[k + 1], K(0.0) + prev, gr[k + 1], ub,
+                                       ub ? br[k] : 0, ub ? br[k + 1] : 0, speed, dt, cell, frac_cap);
+                fr[k] = speed > K(0.0) ? dh : K(0.0);
+            }
+            fl[edges] = K(0.0);
+            fr[edges] = K(0.0);
+            real *rem = removed + lo, *add = added + lo;
+            rem[0] = (K(0.0) + K(0.0)) + fr[0];
+            add[0] = (K(0.0) + K(0.0)) + fl[0];
+            for (size_t c = 1; c < m; c++) {
+                rem[c] = (K(0.0) + fl[c - 1]) + fr[c];
+                add[c] = (K(0.0) + fr[c - 1]) + fl[c];
+            }
+            for (size_t c = 0; c < m; c++) {
+                real moved = R_FMAX(hr[c] - rem[c] + add[c], K(0.0));
+                hr[c] = (rem[c] > K(0.0) || add[c] > K(0.0)) ? moved : hr[c];
+            }
+        }
+        free(fl);
+        free(fr);
+        return;
+    }
+    for (size_t j = j0; j <= j1; j++)
+        for (size_t c = 0; c < m; c++) removed[j * res + i0 + c] = added[j * res + i0 + c] = K(0.0);
+    for (size_t j = j0; j < j1; j++) {
+        size_t lo = j * res + i0;
+        const real *ha = h + lo, *hb = h + lo + res, *va = vel + lo, *vb = vel + lo + res;
+        const uint8_t *ga = gate + lo, *gb = gate + lo + res;
+        const real *ba = ub ? bed + lo : NULL, *bb = ub ? bed + lo + res : NULL;
+        real *ra = removed + lo, *rb = removed + lo + res, *aa = added + lo, *ab = added + lo + res;
+        for (size_t i = 0; i < m; i++) {
+            real speed = K(0.5) * (va[i] + vb[i]);
+            real dh = open_edge_dh(hb[i], ha[i], K(0.0), ga[i], ub, ub ? bb[i] : 0, ub ? ba[i] : 0,
+                                   speed, dt, cell, frac_cap);
+            real up = speed > K(0.0) ? K(0.0) : dh;
+            rb[i] += up;
+            aa[i] += up;
+        }
+        for (size_t i = 0; i < m; i++) {
+            real speed = K(0.5) * (va[i] + vb[i]);
+            real dh = open_edge_dh(ha[i], hb[i], ra[i], gb[i], ub, ub ? ba[i] : 0, ub ? bb[i] : 0,
+                                   speed, dt, cell, frac_cap);
+            real down = speed > K(0.0) ? dh : K(0.0);
+            ra[i] += down;
+            ab[i] += down;
+        }
+    }
+    Box b = {i0, i1, j0, j1};
+    apply_moves(h, removed, added, res, b);
+}
+
+/* ---- the substep: src/fluid_core.rs:1879-2082 (solid = None) ------------- */
+
+typedef struct {
+    int wind;          /* 0 Gravity, 1 Wind */
+    real wx, wz;       /* wind velocity */
+    real vorticity;
+} Drive;
+
+typedef struct {
+    real *a, *b, *c, *d, *e, *removed, *added, *su, *sv;
+} Scratch;
+
+/* Confinement schedule: the reference's own choice, or Jacobi everywhere. */
+enum { CONFINE_REFERENCE = 0, CONFINE_JACOBI = 1, CONFINE_GAUSS_SEIDEL = 2 };
+
+/*
+ * Gravity (or wind blend) on the wet cells of the box, then one confinement
+ * sweep.  The reference sweeps in place (forward Gauss-Seidel) for a small
+ * box or a coarse sheet stepped beside the other layer, and from a snapshot
+ * (Jacobi) on the 2 cm sheet and on boxes of at least 48 x 32 when row
+ * splits are allowed: force_gravity_open :2204-2261, force_gravity_par
+ * :2263-2378, gravity_jacobi_serial :2533-2595, force_wind_open :2633-2682.
+ */
+static void force_open(const real *h, real *u, real *v, const real *nu, const real *p, size_t res,
+                       real cell, real dt, Drive drive, real eps, Box b, int inner_par,
+                       int confine_mode, Scratch *sc) {
+    int jacobi;
+    if (confine_mode == CONFINE_JACOBI) {
+        jacobi = 1;
+    } else if (confine_mode == CONFINE_GAUSS_SEIDEL) {
+        jacobi = 0;
+    } else {
+        int big = (b.j1 - b.j0 >= 47) && (b.i1 - b.i0 >= 31) && inner_par;
+        int fine = !drive.wind && cell <= FINE_CELL_M + K(1.0e-6);
+        jacobi = fine || big;
+    }
+    real two = K(2.0) * cell;
+    real blend = K(1.0) - R_EXP(-dt * K(1.6));
+    if (!jacobi) {
+        for (size_t j = b.j0; j <= b.j1; j++)
+            for (size_t i = b.i0; i <= b.i1; i++) {
+                size_t idx = j * res + i;
+                if (h[idx] <= eps) continue;
+                if (drive.wind) {
+                    u[idx] += (drive.wx - u[idx]) * blend;
+                    v[idx] += (drive.wz - v[idx]) * blend;
+                } else {
+                    real px, pz;
+                    if (i >= 2 && j >= 2 && i + 2 < res && j + 2 < res) {
+                        px = (p[idx + 1] - p[idx - 1]) / two;
+                        pz = (p[idx + res] - p[idx - res]) / two;
+                    } else {
+                        grad(p, res, i, j, cell, &px, &pz);
+                    }
+                    real mob = F(oracle_sheet_mobility)(nu[idx]);
+                    u[idx] += -GRAVITY * px * mob * dt;
+                    v[idx] += -GRAVITY * pz * mob * dt;
+                }
+                if (drive.vorticity > K(0.0)) confine(u, v, u, v, res, i, j, cell, dt, drive.vorticity);
+            }
+        return;
+    }
+    for (size_t j = b.j0; j <= b.j1; j++)
+        for (size_t i = b.i0; i <= b.i1; i++) {
+            size_t idx = j * res + i;
+            if (h[idx] <= eps) continue;
+            if (drive.wind) {
+                u[idx] += (drive.wx - u[idx]) * blend;
+                v[idx] += (drive.wz - v[idx]) * blend;
+            } else {
+                real px, pz;
+                if (i >= 2 && j >= 2 && i + 2 < res && j + 2 < res) {
+                    px = (p[idx + 1] - p[idx - 1]) / two;
+                    pz = (p[idx + res] - p[idx - res]) / two;
+                } else {
+                    grad(p, res, i, j, cell, &px, &pz);
+                }
+                real mob = F(oracle_sheet_mobility)(nu[idx]);
+                u[idx] += -GRAVITY * px * mob * dt;
+                v[idx] += -GRAVITY * pz * mob * dt;
+            }
+        }
+    if (drive.vorticity <= K(0.0)) return;
+    memcpy(sc->su, u, res * res * sizeof(real));
+    memcpy(sc->sv, v, res * res * sizeof(real));
+    for (size_t j = b.j0; j <= b.j1; j++)
+        for (size_t i = b.i0; i <= b.i1; i++) {
+            if (h[j * res + i] <= eps) continue;
+            confine(sc->su, sc->sv, u, v, res, i, j, cell, dt, drive.vorticity);
+        }
+}
+
+static void transport_rows(real *h, const real *vel, const uint8_t *gate, size_t res, real cell,
+                           real dt, Box b, int along_x, const real *bed, real frac_cap, Scratch *sc) {
+    F(oracle_transport_open_rows)(h, vel, gate, res, cell, dt, b.i0, b.i1, b.j0, b.j1, along_x, bed,
+                                  frac_cap, sc->removed, sc->added);
+}
+
+static void substep(real *h, real *u, real *v, const real *bed, const real *nu, const uint8_t *gate,
+                    size_t res, real cell, real dt, Drive drive, real eps, Box b, int banks,
+                    int do_flux, int inner_par, int confine_mode, Scratch *sc) {
+    size_t hi0 = b.i0 >= 2 ? b.i0 - 2 : 0, hi1 = (b.i1 + 2 < res - 1) ? b.i1 + 2 : res - 1;
+    size_t hj0 = b.j0 >= 2 ? b.j0 - 2 : 0, hj1 = (b.j1 + 2 < res - 1) ? b.j1 + 2 : res - 1;
+    for (size_t j = hj0; j <= hj1; j++)
+        for (size_t i = hi0; i <= hi1; i++) sc->a[j * res + i] = bed[j * res + i] + h[j * res + i];
+    int wind_open = drive.wind;
+    if (!wind_open) {
+        blur_box(sc->a, sc->b, sc->e, res, b, GAUSS_F);
+        blur_box(sc->a, sc->c, sc->e, res, b, STEEP_F);
+        for (size_t j = b.j0; j <= b.j1; j++)
+            for (size_t i = b.i0; i <= b.i1; i++) {
+                size_t idx = j * res + i;
+                real filtered = K(0.62) * sc->b[idx] + K(0.38) * sc->c[idx];
+                sc->d[idx] = K(0.45) * filtered + K(0.55) * sc->a[idx];
+            }
+    }
+    force_open(h, u, v, nu, sc->d, res, cell, dt, drive, eps, b, inner_par, confine_mode, sc);
+    int project = !drive.wind && cell > FINE_CELL_M + K(1.0e-6);
+    if (project) {
+        for (size_t j = b.j0; j <= b.j1; j++)
+            for (size_t i = b.i0; i <= b.i1; i++) {
+                size_t idx = j * res + i;
+                size_t ip = i + 1 < res ? i + 1 : res - 1, im = i ? i - 1 : 0;
+                size_t jp = j + 1 < res ? j + 1 : res - 1, jm = j ? j - 1 : 0;
+                real ux = (u[j * res + ip] - u[j * res + im]) / (K(2.0) * cell);
+                real vz = (v[jp * res + i] - v[jm * res + i]) / (K(2.0) * cell);
+                real div = ux + vz;
+                real lap_h = lap(sc->a, res, i, j, cell);
+                real overload = K(1.0) + clampr(nu[idx] / K(231.0), K(0.0), K(6.0));
+                sc->b[idx] = div * overload - K(0.015) * overload * lap_h;
+            }
+        blur_box(sc->b, sc->c, sc->e, res, b, GAUSS_F);
+        blur_box(sc->b, sc->a, sc->e, res, b, STEEP_F);
+        for (size_t j = b.j0; j <= b.j1; j++)
+            for (size_t i = b.i0; i <= b.i1; i++) {
+                size_t idx = j * res + i;
+                sc->d[idx] = K(0.62) * sc->c[idx] + K(0.38) * sc->a[idx];
+            }
+    }
+    /* Damping, memoised per viscosity bit pattern: :2014-2042 */
+    int memo_set = 0;
+    real memo_nu = K(0.0), memo_decay = K(1.0);
+    real lim = drive.wind ? R_FMAX(R_HYPOT(drive.wx, drive.wz), FILM_SPEED_CAP) : FILM_SPEED_CAP;
+    for (size_t j = b.j0; j <= b.j1; j++)
+        for (size_t i = b.i0; i <= b.i1; i++) {
+            size_t idx = j * res + i;
+            if (h[idx] <= eps) continue;
+            if (project) {
+                real qx, qz;
+                grad(sc->d, res, i, j, cell, &qx, &qz);
+                u[idx] += K(0.012) * dt * qx;
+                v[idx] += K(0.012) * dt * qz;
+            }
+            real nu_here = nu[idx];
+            if (!memo_set || memcmp(&nu_here, &memo_nu, sizeof(real)) != 0) {
+                memo_set = 1;
+                memo_nu = nu_here;
+                memo_decay = R_EXP(-F(oracle_sheet_damp)(nu_here) * dt);
+            }
+            u[idx] = finite_speed(u[idx] * memo_decay, lim);
+            v[idx] = finite_speed(v[idx] * memo_decay, lim);
+        }
+    if (!do_flux) return;
+    const real *open_bed = banks ? bed : NULL;
+    transport_rows(h, u, gate, res, cell, dt, b, 1, open_bed, K(0.45), sc);
+    transport_rows(h, v, gate, res, cell, dt, b, 0, open_bed, K(0.45), sc);
+    for (size_t j = b.j0; j <= b.j1; j++)
+        for (size_t i = b.i0; i <= b.i1; i++) {
+            size_t idx = j * res + i;
+            if (eps > K(0.0) && h[idx] < eps) {
+                h[idx] = K(0.0);
+                u[idx] = K(0.0);
+                v[idx] = K(0.0);
+            }
+        }
+}
+
+/* pace3_slices, src/fluid_core.rs:1835-1868 */
+static void pace3_slices(real *h, real *u, real *v, const uint8_t *gate, size_t res, real cell,
+                         real dts, uint32_t nsub, real eps, Box b, Scratch *sc) {
+    for (uint32_t s = 0; s < nsub; s++) {
+        transport_rows(h, u, gate, res, cell, dts, b, 1, NULL, K(0.45), sc);
+        transport_rows(h, v, gate, res, cell, dts, b, 0, NULL, K(0.45), sc);
+    }
+    if (eps > K(0.0))
+        for (size_t j = b.j0; j <= b.j1; j++)
+            for (size_t i = b.i0; i <= b.i1; i++) {
+                size_t idx = j * res + i;
+                if (h[idx] < eps) h[idx] = u[idx] = v[idx] = K(0.0);
+            }
+}
+
+/*
+ * The step: step_sheet_eps (:856-889) and step_sheet_scratch (:1659-1831).
+ * Returns the number of substeps taken (0 when the call is a no-op).
+ * boxes_out (optional, 8 x 4 entries) receives the work boxes.
+ */
+int F(oracle_step_sheet_ex)(real *h, real *u, real *v, const real *bed, const real *nu,
+                            const uint8_t *gate, size_t res, real cell, real dt, int drive_wind,
+                            real wind_x, real wind_z, real vorticity, real eps, int banks,
+                            real sub_cap, int pace, int inner_par, int confine_mode,
+                            size_t *boxes_out, size_t *nboxes_out) {
+    if (nboxes_out) *nboxes_out = 0;
+    if (!(dt > K(0.0)) || res == 0) return 0;
+    size_t n = res * res;
+    real sum = wet_sum(h, n, eps);
+    if (sum <= eps) return 0;
+    Drive drive = {drive_wind, wind_x, wind_z, vorticity};
+    int open_fine = !drive_wind && cell <= FINE_CELL_M + K(1.0e-6);
+    int fine_open = open_fine && pace == 0;
+    int whole_wet = open_fine && (pace == 2 || pace == 3);
+    int awake = open_fine && !whole_wet && !fine_open;
+    size_t raw[4 * 4096];
+    size_t nb;
+    if (fine_open || awake)
+        nb = F(oracle_awake_work_boxes)(h, u, v, bed, res, cell, eps, raw, 4096);
+    else
+        nb = F(oracle_sheet_work_boxes)(h, res, eps, 1, raw, 4096);
+    if (nb > 4096) nb = 4096;
+    if (nboxes_out) *nboxes_out = nb;
+    if (boxes_out)
+        for (size_t k = 0; k < nb && k < 8; k++)
+            for (int q = 0; q < 4; q++) boxes_out[4 * k + q] = raw[4 * k + q];
+    if (nb == 0) return 0;
+    real cap = R_ISFINITE(sub_cap) ? clampr(sub_cap, K(1.0), K(8.0)) : K(8.0);
+    uint32_t nsub;
+    if (fine_open) {
+        nsub = 1;
+    } else {
+        real max_s = K(0.15);
+        if (drive_wind) max_s = R_FMAX(max_s, R_HYPOT(wind_x, wind_z));
+        real max_s2 = K(0.0);
+        for (size_t k = 0; k < nb; k++)
+            for (size_t j = raw[4 * k + 2]; j <= raw[4 * k + 3]; j++)
+                for (size_t i = raw[4 * k + 0]; i <= raw[4 * k + 1]; i++) {
+                    real uu = u[j * res + i], vv = v[j * res + i];
+                    real s2 = uu * uu + vv * vv;
+                    if (s2 > max_s2) max_s2 = s2;
+                }
+        max_s = R_FMAX(max_s, R_SQRT(max_s2));
+        real courant = drive_wind ? K(0.85) : K(0.42);
+        real q = R_CEIL((dt * max_s) / (courant * R_FMAX(cell, K(1.0e-3))));
+        nsub = (uint32_t)clampr(q, K(1.0), cap);
+    }
+    real dts = dt / (real)nsub;
+    Scratch sc;
+    real *pool = calloc(9 * n, sizeof(real));
+    sc.a = pool; sc.b = pool + n; sc.c = pool + 2 * n; sc.d = pool + 3 * n; sc.e = pool + 4 * n;
+    sc.removed = pool + 5 * n; sc.added = pool + 6 * n; sc.su = pool + 7 * n; sc.sv = pool + 8 * n;
+    if (pace == 3 && whole_wet && nsub > 1) {
+        for (size_t k = 0; k < nb; k++) {
+            Box b = {raw[4 * k], raw[4 * k + 1], raw[4 * k + 2], raw[4 * k + 3]};
+            substep(h, u, v, bed, nu, gate, res, cell, dt, drive, eps, b, banks, 0, inner_par,
+                    confine_mode, &sc);
+            pace3_slices(h, u, v, gate, res, cell, dts, nsub, eps, b, &sc);
+        }
+    } else {
+        for (uint32_t s = 0; s < nsub; s++)
+            for (size_t k = 0; k < nb; k++) {
+                Box b = {raw[4 * k], raw[4 * k + 1], raw[4 * k + 2], raw[4 * k + 3]};
+                substep(h, u, v, bed, nu, gate, res, cell, dts, drive, eps, b, banks, 1, inner_par,
+                        confine_mode, &sc);
+            }
+    }
+    free(pool);
+    return (int)nsub;
+}
+
+/* step_sheet, src/fluid_core.rs:770-785: eps 1e-6, no banks, cap 8, pace 0. */
+int F(oracle_step_sheet)(real *h, real *u, real *v, const real *bed, const real *nu,
+                         const uint8_t *gate, size_t res, real cell, real dt, int drive_wind,
+                         real wind_x, real wind_z, real vorticity) {
+    return F(oracle_step_sheet_ex)(h, u, v, bed, nu, gate, res, cell, dt, drive_wind, wind_x, wind_z,
+                                   vorticity, K(1.0e-6), 0, K(8.0), 0, 1, CONFINE_REFERENCE, NULL,
+                                   NULL);
+}
+
+/* column_sum, src/fluid_core.rs:4617-4620 (sequential sum of max(h,0)). */
+real F(oracle_column_sum)(const real *h, size_t n, real cell) {
+    real area = cell * cell;
+    real s = -K(0.0);
+    for (size_t i = 0; i < n; i++) s += R_FMAX(h[i], K(0.0));
+    return s * area;
+}
