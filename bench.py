@@ -1,0 +1,132 @@
+"""Throughput of the two-layer sheet step on one MI355X.
+
+Workload: the plain two-layer sheet at 4096 x 4096 cells of 0.2 m (819 m):
+seeded fBm bed (5 octaves, 0.3 m), a 0.05 m water film with 64 Gaussian
+puddles, a 0.02 m slurry sheet at 0.5 Pa s, dt = 1/120 s.  One step is the
+reference's generic-res step_sheet on the water layer and on the slurry layer
+(src/fluid_core.rs:770-785 with the layer constants of step_pair), fused into
+one kernel launch.  Inputs are resident in HBM before the timed region.
+
+Prints one JSON line (the driver's contract).  ``value`` is cell-steps/s
+(one cell-step = both layers of one cell advanced one step).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "dirty-gen_amd"))
+
+import numpy as np  # noqa: E402
+
+METRIC = "sheet cell-steps/s + HBM GB/s; surrogate vs nested-sheet speedup at fixed err"
+HBM_PEAK_GBS = 8000.0
+# Algorithmic bytes per cell-step of the fused two-layer kernel: read bed (4),
+# water h, u, v (12), slurry h, u, v (12); write both layers' h, u, v (24).
+# The slurry viscosity is uniform in this workload, so no viscosity plane.
+ALG_BYTES_PER_CELL = 52
+
+
+def cpu_baseline(sc, dt, steps):
+    """The FP32 oracle (one core, the reference's arithmetic restated in C) on
+    ``steps`` full steps of the same workload from the same initial state."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as O
+
+    res, cell = sc["res"], sc["cell"]
+    n = res * res
+    gate = np.ones(n, np.uint8)
+    layers = [(sc["water"].copy(), np.full(n, sc["nu_water"], np.float32), 0.08),
+              (sc["slurry"].copy(), np.full(n, sc["nu_slurry"], np.float32), 0.05)]
+    state = [(h, np.zeros(n, np.float32), np.zeros(n, np.float32), nu, vort) for h, nu, vort in layers]
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        for h, u, v, nu, vort in state:
+            O.step_sheet(h, u, v, sc["bed"], nu, gate, res, cell, dt, ("gravity", vort))
+    el = time.perf_counter() - t0
+    return {"value": n * steps / el, "unit": "cell-steps/s", "cores": 1, "kind": "port",
+            "sample": f"{steps} full steps of the {res}^2 two-layer workload, FP32 oracle, 1 thread, {el:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--res", type=int, default=4096)
+    ap.add_argument("--path", default="fast", choices=["fast", "fused_exact", "exact"])
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported on this build "
+              "(the sheet's multi-GPU halo exchange is a design, not code)", file=sys.stderr)
+        sys.exit(2)
+
+    import silt_amd as S
+    from silt_amd import scenes
+
+    dt = 1.0 / 120.0
+    print(f"[bench] building the {args.res}^2 scene", file=sys.stderr, flush=True)
+    sc = scenes.large_scene(res=args.res)
+    res, cell = sc["res"], sc["cell"]
+    n = res * res
+    pair = S.SheetPair(res, cell, path=args.path)
+    z = np.zeros(n, np.float32)
+    pair.upload(sc["bed"], sc["water"], z, z, sc["slurry"], z, z, sc["nu_slurry"])
+    print(f"[bench] warmup {args.warmup} steps", file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        pair.step(dt)
+    import torch
+
+    torch.cuda.synchronize()
+    kernel_ms = []
+    fallbacks = 0
+    print(f"[bench] timing {args.steps} steps", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        pair.step(dt)
+        kernel_ms.append(pair.last_kernel_ms())
+        fallbacks += pair.last_fallback() != 0
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    led = pair.ledger()
+    ms = el * 1e3 / args.steps
+    kms = float(np.mean(kernel_ms))
+    achieved = ALG_BYTES_PER_CELL * n / (kms * 1e-3) / 1e9
+    out = {
+        "metric": METRIC,
+        "value": n * args.steps / el,
+        "unit": "cell-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded fBm bed, Gaussian puddles)",
+        "config": {"workload": f"plain two-layer sheet {res}x{res} @ {cell} m, dt 1/120 s, water nu 0.9 vort 0.08, "
+                               f"slurry nu 0.5 vort 0.05", "res": res, "cell_m": cell, "path": args.path,
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_sheet_stream (both layers, one launch)", "kernel_ms": kms,
+                     "alg_bytes_per_cell": ALG_BYTES_PER_CELL},
+        "ledger": {"water_gap_rel": led.water.gap_m3 / max(led.water.initial_m3, 1e-30),
+                   "slurry_gap_rel": led.slurry.gap_m3 / max(led.slurry.initial_m3, 1e-30),
+                   "total_m3": led.total_m3},
+        "host_plan_steps": fallbacks,
+    }
+    if not args.no_cpu:
+        print(f"[bench] CPU baseline, {args.cpu_steps} oracle steps", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(sc, dt, args.cpu_steps)
+        out["gpu_over_cpu"] = out["value"] / out["cpu_baseline"]["value"]
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,333 @@
+// Two-layer sheet handle (include/silt_pair.h): water and slurry over one
+// bed, each stepped as the reference's generic-res step_sheet
+// (src/fluid_core.rs:770-785) once per step, with the layer constants of
+// FluidCore::step_pair (:8036-8125): water Gravity{0.08} at the water
+// viscosity, slurry Gravity{0.05} at its own viscosity.  State stays on the
+// GPU between steps; the plan of each step is read from the statistics the
+// previous step's kernel left behind (a 64-byte read back), and the ledger is
+// kept in FP64 with explicit dry-cut and residue lines.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <string.h>
+
+#include <vector>
+
+#include "plan.hpp"
+#include "sheet_device.hpp"
+#include "sheet_fused.hpp"
+#include "silt_pair.h"
+#include "silt_sheet.h"
+#include "workspace.hpp"
+
+#define HIPCHK(x)                                 \
+  do {                                            \
+    if ((x) != hipSuccess) return SILT_EDEVICE;   \
+  } while (0)
+
+using namespace silt;
+
+struct silt_pair {
+  int res = 0;
+  float cell = 0.2f;
+  int path = SILT_PATH_FAST;
+  int device = 0;
+  size_t n = 0;
+  float* bed = nullptr;
+  float* planes[2][2][3] = {};  // [layer][buffer][h, u, v]
+  int cur[2] = {0, 0};
+  float* nu_slurry = nullptr;
+  bool nu_slurry_uniform = true;
+  float nu_slurry0 = 0.5f;
+  float nu_water = 0.9f;  // dynamic_viscosity(Grain::Water, 1.0), src/fluid_core.rs:642-646
+  float vort[2] = {0.08f, 0.05f};
+  SheetStats* stats = nullptr;     // [2] device
+  SheetTotals* totals = nullptr;   // [2] device
+  unsigned* boxmax = nullptr;      // [2] device
+  SheetStats* h_stats = nullptr;   // pinned host copy
+  unsigned* h_boxmax = nullptr;
+  double vol0[2] = {0, 0};
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  float last_kernel_ms = 0.0f;
+  int last_nsub[2] = {0, 0};
+  int last_fallback = 0;
+};
+
+namespace {
+
+const float kEps = 1.0e-6f;
+
+SheetPlanes layer_planes(silt_pair* p, int layer, int buf) {
+  SheetPlanes s{};
+  s.h = p->planes[layer][buf][0];
+  s.u = p->planes[layer][buf][1];
+  s.v = p->planes[layer][buf][2];
+  s.bed = p->bed;
+  s.nu = layer == 1 && !p->nu_slurry_uniform ? p->nu_slurry : nullptr;
+  s.gate = nullptr;
+  return s;
+}
+
+float layer_nu0(const silt_pair* p, int layer) { return layer == 0 ? p->nu_water : p->nu_slurry0; }
+bool layer_nu_uniform(const silt_pair* p, int layer) { return layer == 0 || p->nu_slurry_uniform; }
+
+// Plan of one layer from its statistics.  Returns false when the plan needs
+// the full planes on the host (a wet set that is not one solid rectangle, or
+// a fine cell with sleeping tiles).
+bool plan_from_stats(const silt_pair* p, const SheetStats& st, float dt, StepPlan* plan, bool* need_box_max) {
+  *plan = StepPlan{};
+  *need_box_max = false;
+  if (!(dt > 0.0f)) return true;
+  if (st.wet == 0) return true;  // wet_sum <= eps: the reference returns early
+  if (p->cell <= kFineCell + 1.0e-6f) return false;
+  long w = (long)(st.i1 - st.i0 + 1), hgt = (long)(st.j1 - st.j0 + 1);
+  if ((long long)st.wet != (long long)w * hgt) return false;
+  int res = p->res;
+  plan->boxes.push_back(Box{std::max(st.i0 - 3, 0), std::min(st.i1 + 3, res - 1), std::max(st.j0 - 3, 0),
+                            std::min(st.j1 + 3, res - 1)});
+  // nsub from the largest speed anywhere bounds the box's; when that bound
+  // is one substep it is exact, otherwise the box's own maximum is read.
+  float s2;
+  memcpy(&s2, &st.max_s2_bits, 4);
+  float max_s = fmaxf(0.15f, sqrtf(s2));
+  float q = ceilf((dt * max_s) / (0.42f * fmaxf(p->cell, 1.0e-3f)));
+  int bound = (int)clampf(q, 1.0f, 8.0f);
+  plan->nsub = bound;
+  if (bound > 1) *need_box_max = true;
+  return true;
+}
+
+int nsub_from_s2(float s2, float dt, float cell) {
+  float max_s = fmaxf(0.15f, sqrtf(s2));
+  float q = ceilf((dt * max_s) / (0.42f * fmaxf(cell, 1.0e-3f)));
+  return (int)clampf(q, 1.0f, 8.0f);
+}
+
+}  // namespace
+
+extern "C" {
+
+silt_pair* silt_pair_create(size_t res, float cell, int path) {
+  if (res < 1 || res > 65536 || !(cell > 0.0f)) return nullptr;
+  if (path != SILT_PATH_EXACT && path != SILT_PATH_FAST && path != SILT_PATH_FUSED_EXACT) return nullptr;
+  silt_pair* p = new silt_pair();
+  p->res = (int)res;
+  p->cell = cell;
+  p->path = path;
+  p->n = res * res;
+  size_t fb = p->n * sizeof(float);
+  bool ok = hipGetDevice(&p->device) == hipSuccess && hipMalloc(&p->bed, fb) == hipSuccess &&
+            hipMalloc(&p->nu_slurry, fb) == hipSuccess;
+  for (int l = 0; l < 2 && ok; l++)
+    for (int b = 0; b < 2 && ok; b++)
+      for (int k = 0; k < 3 && ok; k++) ok = hipMalloc(&p->planes[l][b][k], fb) == hipSuccess;
+  ok = ok && hipMalloc(&p->stats, 2 * sizeof(SheetStats)) == hipSuccess &&
+       hipMalloc(&p->totals, 2 * sizeof(SheetTotals)) == hipSuccess &&
+       hipMalloc(&p->boxmax, 2 * sizeof(unsigned)) == hipSuccess &&
+       hipHostMalloc(&p->h_stats, 2 * sizeof(SheetStats), hipHostMallocDefault) == hipSuccess &&
+       hipHostMalloc(&p->h_boxmax, 2 * sizeof(unsigned), hipHostMallocDefault) == hipSuccess &&
+       hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) == hipSuccess &&
+       hipEventCreate(&p->ev0) == hipSuccess && hipEventCreate(&p->ev1) == hipSuccess;
+  if (ok) ok = hipMemset(p->totals, 0, 2 * sizeof(SheetTotals)) == hipSuccess;
+  for (int l = 0; l < 2 && ok; l++)
+    for (int b = 0; b < 2 && ok; b++)
+      for (int k = 0; k < 3 && ok; k++) ok = hipMemset(p->planes[l][b][k], 0, fb) == hipSuccess;
+  if (!ok) {
+    silt_pair_destroy(p);
+    return nullptr;
+  }
+  stats_reset(p->stats, 2, p->stream);
+  (void)hipStreamSynchronize(p->stream);
+  return p;
+}
+
+void silt_pair_destroy(silt_pair* p) {
+  if (!p) return;
+  if (p->stream) (void)hipStreamSynchronize(p->stream);
+  (void)hipFree(p->bed);
+  (void)hipFree(p->nu_slurry);
+  for (int l = 0; l < 2; l++)
+    for (int b = 0; b < 2; b++)
+      for (int k = 0; k < 3; k++) (void)hipFree(p->planes[l][b][k]);
+  (void)hipFree(p->stats);
+  (void)hipFree(p->totals);
+  (void)hipFree(p->boxmax);
+  if (p->h_stats) (void)hipHostFree(p->h_stats);
+  if (p->h_boxmax) (void)hipHostFree(p->h_boxmax);
+  if (p->ev0) (void)hipEventDestroy(p->ev0);
+  if (p->ev1) (void)hipEventDestroy(p->ev1);
+  if (p->stream) (void)hipStreamDestroy(p->stream);
+  delete p;
+}
+
+int silt_pair_upload(silt_pair* p, const float* bed, const float* water, const float* uw, const float* vw,
+                     const float* slurry, const float* us, const float* vs, const float* nu_slurry,
+                     float nu_slurry_uniform) {
+  if (!p || !bed || !water || !uw || !vw || !slurry || !us || !vs) return SILT_EINVAL;
+  bool uniform = isfinite(nu_slurry_uniform);
+  if (!uniform && !nu_slurry) return SILT_EINVAL;
+  size_t fb = p->n * sizeof(float);
+  hipStream_t st = p->stream;
+  HIPCHK(hipMemcpyAsync(p->bed, bed, fb, hipMemcpyHostToDevice, st));
+  const float* src[2][3] = {{water, uw, vw}, {slurry, us, vs}};
+  for (int l = 0; l < 2; l++) {
+    p->cur[l] = 0;
+    for (int k = 0; k < 3; k++) HIPCHK(hipMemcpyAsync(p->planes[l][0][k], src[l][k], fb, hipMemcpyHostToDevice, st));
+  }
+  p->nu_slurry_uniform = uniform;
+  p->nu_slurry0 = uniform ? nu_slurry_uniform : 0.0f;
+  if (!uniform) HIPCHK(hipMemcpyAsync(p->nu_slurry, nu_slurry, fb, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemsetAsync(p->totals, 0, 2 * sizeof(SheetTotals), st));
+  for (int l = 0; l < 2; l++)
+    stats_compute(p->planes[l][0][0], p->planes[l][0][1], p->planes[l][0][2], p->res, kEps, p->stats + l, st);
+  HIPCHK(hipMemcpyAsync(p->h_stats, p->stats, 2 * sizeof(SheetStats), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  for (int l = 0; l < 2; l++) p->vol0[l] = p->h_stats[l].vol;
+  return SILT_OK;
+}
+
+int silt_pair_download(silt_pair* p, float* water, float* uw, float* vw, float* slurry, float* us, float* vs) {
+  if (!p || !water || !uw || !vw || !slurry || !us || !vs) return SILT_EINVAL;
+  size_t fb = p->n * sizeof(float);
+  float* dst[2][3] = {{water, uw, vw}, {slurry, us, vs}};
+  for (int l = 0; l < 2; l++)
+    for (int k = 0; k < 3; k++)
+      HIPCHK(hipMemcpyAsync(dst[l][k], p->planes[l][p->cur[l]][k], fb, hipMemcpyDeviceToHost, p->stream));
+  HIPCHK(hipStreamSynchronize(p->stream));
+  return SILT_OK;
+}
+
+int silt_pair_step(silt_pair* p, float dt) {
+  if (!p) return SILT_EINVAL;
+  if (!(dt > 0.0f)) return 0;
+  hipStream_t st = p->stream;
+  const int res = p->res;
+  // Statistics of the current planes, left by the previous step's kernel.
+  HIPCHK(hipMemcpyAsync(p->h_stats, p->stats, 2 * sizeof(SheetStats), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  StepPlan plans[2];
+  bool fast_plan[2], need_max[2];
+  p->last_fallback = 0;
+  for (int l = 0; l < 2; l++) {
+    fast_plan[l] = plan_from_stats(p, p->h_stats[l], dt, &plans[l], &need_max[l]);
+    if (!fast_plan[l]) {
+      // The wet set is not one solid rectangle: read the planes back and
+      // plan on the host (connected wet bodies, as sheet_work_boxes does).
+      p->last_fallback |= 1 << l;
+      std::vector<float> hh(p->n), uh(p->n), vh(p->n), bh(p->n);
+      size_t fb = p->n * sizeof(float);
+      float** pl = p->planes[l][p->cur[l]];
+      HIPCHK(hipMemcpyAsync(hh.data(), pl[0], fb, hipMemcpyDeviceToHost, st));
+      HIPCHK(hipMemcpyAsync(uh.data(), pl[1], fb, hipMemcpyDeviceToHost, st));
+      HIPCHK(hipMemcpyAsync(vh.data(), pl[2], fb, hipMemcpyDeviceToHost, st));
+      HIPCHK(hipMemcpyAsync(bh.data(), p->bed, fb, hipMemcpyDeviceToHost, st));
+      HIPCHK(hipStreamSynchronize(st));
+      plans[l] = plan_step(hh.data(), uh.data(), vh.data(), bh.data(), res, p->cell, dt, 0, 0.0f, 0.0f, kEps, 8.0f, 0);
+    } else if (need_max[l]) {
+      unsigned zero = 0;
+      HIPCHK(hipMemcpyAsync(p->boxmax + l, &zero, 4, hipMemcpyHostToDevice, st));
+      float** pl = p->planes[l][p->cur[l]];
+      box_max_s2(pl[1], pl[2], res, plans[l].boxes[0], p->boxmax + l, st);
+      HIPCHK(hipMemcpyAsync(p->h_boxmax + l, p->boxmax + l, 4, hipMemcpyDeviceToHost, st));
+      HIPCHK(hipStreamSynchronize(st));
+      float s2;
+      memcpy(&s2, p->h_boxmax + l, 4);
+      plans[l].nsub = nsub_from_s2(s2, dt, p->cell);
+    }
+  }
+  silt_sheet_opts o;
+  silt_sheet_opts_default(&o);
+  o.path = p->path;
+  HIPCHK(hipEventRecord(p->ev0, st));
+  // Layers whose plan the fused kernel covers and that take one substep run
+  // together in one launch; anything else runs per layer.
+  auto fused_ok = [&](int l) {
+    if (p->path == SILT_PATH_EXACT || plans[l].boxes.empty() || plans[l].pace3) return false;
+    for (const Box& b : plans[l].boxes)
+      if (!reference_uses_jacobi(b, p->cell, 0, true)) return false;
+    return true;
+  };
+  std::vector<FusedLayer> joint;
+  for (int l = 0; l < 2; l++) {
+    p->last_nsub[l] = plans[l].boxes.empty() ? 0 : plans[l].nsub;
+    if (plans[l].boxes.empty()) continue;
+    silt_drive d{SILT_DRIVE_GRAVITY, 0.0f, 0.0f, p->vort[l]};
+    float dts = dt / (float)plans[l].nsub;
+    if (fused_ok(l)) {
+      for (int k = 0; k < plans[l].nsub; k++) {
+        FusedLayer f{};
+        f.in = layer_planes(p, l, p->cur[l]);
+        int nb = 1 - p->cur[l];
+        f.ho = p->planes[l][nb][0];
+        f.uo = p->planes[l][nb][1];
+        f.vo = p->planes[l][nb][2];
+        f.a = make_args(res, p->cell, dts, d, kEps, 0, true, layer_nu_uniform(p, l), layer_nu0(p, l));
+        f.boxes = plans[l].boxes;
+        f.stats = p->stats + l;
+        f.totals = p->totals + l;
+        if (plans[l].nsub == 1) {
+          joint.push_back(f);
+          p->cur[l] = nb;
+        } else {
+          stats_reset(p->stats + l, 1, st);
+          int rc = fused_substep_layers(&f, 1, p->bed, res, p->path == SILT_PATH_FUSED_EXACT, st);
+          if (rc != 0) return rc;
+          p->cur[l] = nb;
+        }
+      }
+    } else {
+      // Stage kernels in place on the current planes (Gauss-Seidel boxes,
+      // overlapping boxes, or the exact path).
+      Workspace* ws = workspace_for(res);
+      if (!ws) return SILT_ENOMEM;
+      SheetPlanes s = ws->planes();
+      SheetPlanes cur = layer_planes(p, l, p->cur[l]);
+      s.h = cur.h;
+      s.u = cur.u;
+      s.v = cur.v;
+      s.bed = p->bed;
+      s.nu = cur.nu ? cur.nu : s.nu;
+      o.path = SILT_PATH_EXACT;
+      int rc = run_plan(s, plans[l], res, p->cell, dt, d, o, true, layer_nu_uniform(p, l), layer_nu0(p, l), st);
+      if (rc < 0) return rc;
+      o.path = p->path;
+      stats_compute(cur.h, cur.u, cur.v, res, kEps, p->stats + l, st);
+    }
+  }
+  if (!joint.empty()) {
+    for (const FusedLayer& f : joint) stats_reset(f.stats, 1, st);
+    // Time the joint launch alone when it is the step's only kernel work.
+    if (joint.size() == 2) HIPCHK(hipEventRecord(p->ev0, st));
+    int rc = fused_substep_layers(joint.data(), (int)joint.size(), p->bed, res, p->path == SILT_PATH_FUSED_EXACT, st);
+    if (rc != 0) return rc;
+  }
+  HIPCHK(hipEventRecord(p->ev1, st));
+  HIPCHK(hipEventSynchronize(p->ev1));
+  HIPCHK(hipEventElapsedTime(&p->last_kernel_ms, p->ev0, p->ev1));
+  return std::max(p->last_nsub[0], p->last_nsub[1]);
+}
+
+int silt_pair_ledger(silt_pair* p, silt_pair_ledger_t* out) {
+  if (!p || !out) return SILT_EINVAL;
+  SheetTotals t[2];
+  HIPCHK(hipMemcpyAsync(p->h_stats, p->stats, 2 * sizeof(SheetStats), hipMemcpyDeviceToHost, p->stream));
+  HIPCHK(hipMemcpyAsync(t, p->totals, sizeof t, hipMemcpyDeviceToHost, p->stream));
+  HIPCHK(hipStreamSynchronize(p->stream));
+  double area = (double)p->cell * (double)p->cell;
+  for (int l = 0; l < 2; l++) {
+    silt_layer_ledger_t& L = l == 0 ? out->water : out->slurry;
+    L.volume_m3 = p->h_stats[l].vol * area;
+    L.initial_m3 = p->vol0[l] * area;
+    L.cut_m3 = t[l].cut * area;
+    L.residue_m3 = t[l].residue * area;
+    L.gap_m3 = (p->h_stats[l].vol + t[l].cut - t[l].residue - p->vol0[l]) * area;
+  }
+  out->total_m3 = out->water.volume_m3 + out->slurry.volume_m3;
+  return SILT_OK;
+}
+
+float silt_pair_last_kernel_ms(const silt_pair* p) { return p ? p->last_kernel_ms : 0.0f; }
+
+int silt_pair_last_fallback(const silt_pair* p) { return p ? p->last_fallback : SILT_EINVAL; }
+
+}  // extern "C"

@@ -1,0 +1,414 @@
+// Exact sheet substep for gfx950: one kernel per stage of the reference
+// substep (src/fluid_core.rs:1879-2082, open sheet), each stage over one work
+// box plus its 2-cell rim.  Arithmetic follows the reference operation by
+// operation (no contraction: the file is built with -ffp-contract=off; IEEE
+// division), so on the same inputs it reproduces the FP32 reference bit for bit
+// wherever no libm transcendental enters (the damping factor is passed in when
+// the viscosity is uniform).  This is the parity anchor of the library; the
+// fused streaming kernel (sheet_fused.hip) is the throughput path.
+//
+// Scratch convention: inside the box every scratch value is the reference's;
+// on the 2-cell rim the filtered planes are zero, as if each box started from
+// a cleared scratch (the reference reuses a thread-local scratch whose stale
+// rim values never reach a cell that is wet or next to a wet cell).
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "sheet_device.hpp"
+
+namespace silt {
+
+namespace {
+
+constexpr float kGauss[5] = {0.06f, 0.24f, 0.40f, 0.24f, 0.06f};
+constexpr float kSteep[5] = {0.02f, 0.08f, 0.80f, 0.08f, 0.02f};
+
+struct Region {
+  int i0, i1, j0, j1;
+};
+
+__device__ inline bool in_box(const Box& b, int i, int j) {
+  return i >= b.i0 && i <= b.i1 && j >= b.j0 && j <= b.j1;
+}
+
+__device__ inline int clampi(int x, int lo, int hi) { return x < lo ? lo : (x > hi ? hi : x); }
+
+// accum5 / accum5_clamped (src/fluid_core.rs:3681-3698): interior taps start
+// from the first product, clamped taps from 0.0.
+__device__ inline float accum5(float a0, float a1, float a2, float a3, float a4, const float* k) {
+  float a = a0 * k[0];
+  a = a + a1 * k[1];
+  a = a + a2 * k[2];
+  a = a + a3 * k[3];
+  a = a + a4 * k[4];
+  return a;
+}
+
+__device__ inline float xblur_at(const float* src, int res, int i, int j, const float* k) {
+  const float* row = src + (size_t)j * res;
+  if (i >= 2 && i + 2 < res) return accum5(row[i - 2], row[i - 1], row[i], row[i + 1], row[i + 2], k);
+  float a = 0.0f;
+  for (int t = -2; t <= 2; t++) a = a + row[clampi(i + t, 0, res - 1)] * k[t + 2];
+  return a;
+}
+
+__device__ inline float zblur_at(const float* src, int res, int i, int j, const float* k) {
+  if (j >= 2 && j + 2 < res)
+    return accum5(src[(size_t)(j - 2) * res + i], src[(size_t)(j - 1) * res + i], src[(size_t)j * res + i],
+                  src[(size_t)(j + 1) * res + i], src[(size_t)(j + 2) * res + i], k);
+  float a = 0.0f;
+  for (int t = -2; t <= 2; t++) a = a + src[(size_t)clampi(j + t, 0, res - 1) * res + i] * k[t + 2];
+  return a;
+}
+
+__device__ inline void grad_at(const float* p, int res, int i, int j, float cell, float* dx, float* dz) {
+  int ip = i + 1 < res ? i + 1 : res - 1, im = i > 0 ? i - 1 : 0;
+  int jp = j + 1 < res ? j + 1 : res - 1, jm = j > 0 ? j - 1 : 0;
+  *dx = (p[(size_t)j * res + ip] - p[(size_t)j * res + im]) / (2.0f * cell);
+  *dz = (p[(size_t)jp * res + i] - p[(size_t)jm * res + i]) / (2.0f * cell);
+}
+
+__device__ inline float vort_at(const float* u, const float* v, int res, int i, int j, float cell) {
+  int ip = i + 1 < res ? i + 1 : res - 1, im = i > 0 ? i - 1 : 0;
+  int jp = j + 1 < res ? j + 1 : res - 1, jm = j > 0 ? j - 1 : 0;
+  float dvdx = (v[(size_t)j * res + ip] - v[(size_t)j * res + im]) / (2.0f * cell);
+  float dudz = (u[(size_t)jp * res + i] - u[(size_t)jm * res + i]) / (2.0f * cell);
+  return dvdx - dudz;
+}
+
+// Confinement kick from (su, sv) added to (u, v) at one cell
+// (confine_one / confine_jacobi, src/fluid_core.rs:2402-2427, 3442-3471).
+__device__ inline void confine_at(const float* su, const float* sv, float* u, float* v, int res, int i, int j,
+                                  float cell, float dt, float eps) {
+  float w = vort_at(su, sv, res, i, j, cell);
+  if (!isfinite(w) || fabsf(w) < 1.0e-4f) return;
+  float wxp = fabsf(vort_at(su, sv, res, i + 1 < res ? i + 1 : res - 1, j, cell));
+  float wxm = fabsf(vort_at(su, sv, res, i > 0 ? i - 1 : 0, j, cell));
+  float wzp = fabsf(vort_at(su, sv, res, i, j + 1 < res ? j + 1 : res - 1, cell));
+  float wzm = fabsf(vort_at(su, sv, res, i, j > 0 ? j - 1 : 0, cell));
+  float nx = (wxp - wxm) / (2.0f * cell);
+  float nz = (wzp - wzm) / (2.0f * cell);
+  float du = eps * dt * (-nz * w);
+  float dv = eps * dt * (nx * w);
+  if (isfinite(du) && isfinite(dv) && du * du + dv * dv <= 0.0625f) {
+    size_t idx = (size_t)j * res + i;
+    u[idx] = u[idx] + du;
+    v[idx] = v[idx] + dv;
+  }
+}
+
+// ---- stage kernels --------------------------------------------------------
+
+// Launch helper: a 2D grid over an inclusive region.
+__device__ inline bool region_cell(const Region& r, int* i, int* j) {
+  *i = r.i0 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  *j = r.j0 + (int)(blockIdx.y * blockDim.y + threadIdx.y);
+  return *i <= r.i1 && *j <= r.j1;
+}
+
+__global__ void k_eta(const float* __restrict__ h, const float* __restrict__ bed, float* __restrict__ eta,
+                      Region r, int res) {
+  int i, j;
+  if (!region_cell(r, &i, &j)) return;
+  size_t idx = (size_t)j * res + i;
+  eta[idx] = bed[idx] + h[idx];
+}
+
+// Both 5-tap filters along x; zero on the rim outside the box.
+__global__ void k_xblur2(const float* __restrict__ src, float* __restrict__ tg, float* __restrict__ ts,
+                         Region r, Box b, int res) {
+  int i, j;
+  if (!region_cell(r, &i, &j)) return;
+  size_t idx = (size_t)j * res + i;
+  if (in_box(b, i, j)) {
+    tg[idx] = xblur_at(src, res, i, j, kGauss);
+    ts[idx] = xblur_at(src, res, i, j, kSteep);
+  } else {
+    tg[idx] = 0.0f;
+    ts[idx] = 0.0f;
+  }
+}
+
+// Pressure d = 0.45 (0.62 G + 0.38 S) + 0.55 eta (src/fluid_core.rs:1923-1935).
+__global__ void k_zblur_pressure(const float* __restrict__ tg, const float* __restrict__ ts,
+                                 const float* __restrict__ eta, float* __restrict__ p, Region r, Box b, int res) {
+  int i, j;
+  if (!region_cell(r, &i, &j)) return;
+  size_t idx = (size_t)j * res + i;
+  if (in_box(b, i, j)) {
+    float g = zblur_at(tg, res, i, j, kGauss);
+    float s = zblur_at(ts, res, i, j, kSteep);
+    float filtered = 0.62f * g + 0.38f * s;
+    p[idx] = 0.45f * filtered + 0.55f * eta[idx];
+  } else {
+    p[idx] = 0.0f;
+  }
+}
+
+// Correction potential q = 0.62 G(b) + 0.38 S(b) (src/fluid_core.rs:2021-2028).
+__global__ void k_zblur_q(const float* __restrict__ tg, const float* __restrict__ ts, float* __restrict__ q,
+                          Region r, Box b, int res) {
+  int i, j;
+  if (!region_cell(r, &i, &j)) return;
+  size_t idx = (size_t)j * res + i;
+  if (in_box(b, i, j)) {
+    float g = zblur_at(tg, res, i, j, kGauss);
+    float s = zblur_at(ts, res, i, j, kSteep);
+    q[idx] = 0.62f * g + 0.38f * s;
+  } else {
+    q[idx] = 0.0f;
+  }
+}
+
+// Gravity (or wind blend) on wet box cells, then the Jacobi snapshot on the
+// box and its rim (force_gravity_par :2263-2319, gravity_jacobi_serial
+// :2533-2570, force_wind_par :2684-2716).
+__global__ void k_force(const float* __restrict__ h, float* __restrict__ u, float* __restrict__ v,
+                        const float* __restrict__ nu, const float* __restrict__ p, float* __restrict__ su,
+                        float* __restrict__ sv, Region r, Box b, StepArgs a) {
+  int i, j;
+  if (!region_cell(r, &i, &j)) return;
+  int res = a.res;
+  size_t idx = (size_t)j * res + i;
+  float uu = u[idx], vv = v[idx];
+  if (in_box(b, i, j) && h[idx] > a.eps) {
+    if (a.wind) {
+      uu = uu + (a.wx - uu) * a.blend;
+      vv = vv + (a.wz - vv) * a.blend;
+    } else {
+      float px, pz;
+      grad_at(p, res, i, j, a.cell, &px, &pz);
+      float mob = a.nu_uniform ? a.mob0 : sheet_mobility(nu[idx]);
+      uu = uu + -kGravity * px * mob * a.dt;
+      vv = vv + -kGravity * pz * mob * a.dt;
+    }
+    u[idx] = uu;
+    v[idx] = vv;
+  }
+  su[idx] = uu;
+  sv[idx] = vv;
+}
+
+__global__ void k_confine_jacobi(const float* __restrict__ h, float* __restrict__ u, float* __restrict__ v,
+                                 const float* __restrict__ su, const float* __restrict__ sv, Region r, StepArgs a) {
+  int i, j;
+  if (!region_cell(r, &i, &j)) return;
+  size_t idx = (size_t)j * a.res + i;
+  if (h[idx] <= a.eps) return;
+  confine_at(su, sv, u, v, a.res, i, j, a.cell, a.dt, a.vort);
+}
+
+// Forward Gauss-Seidel sweep of gravity + confinement over one box
+// (force_gravity_open :2236-2260, force_wind_open :2660-2681), scheduled as
+// wavefronts t = (i - i0) + 2 (j - j0): every cell a kick reads that comes
+// earlier in raster order has a smaller t, every later one a larger t, and
+// no two cells of one wavefront read each other.  One workgroup.
+__global__ void k_force_gauss_seidel(const float* __restrict__ h, float* u, float* v, const float* __restrict__ nu,
+                                     const float* __restrict__ p, Box b, StepArgs a) {
+  int res = a.res;
+  int W = b.i1 - b.i0 + 1, H = b.j1 - b.j0 + 1;
+  int tmax = (W - 1) + 2 * (H - 1);
+  for (int t = 0; t <= tmax; t++) {
+    // Cells with di + 2 dj = t: dj from max(0, ceil((t - (W-1))/2)) to min(H-1, t/2).
+    int dj_lo = t - (W - 1) > 0 ? (t - (W - 1) + 1) / 2 : 0;
+    int dj_hi = t / 2 < H - 1 ? t / 2 : H - 1;
+    for (int dj = dj_lo + (int)threadIdx.x; dj <= dj_hi; dj += (int)blockDim.x) {
+      int i = b.i0 + (t - 2 * dj), j = b.j0 + dj;
+      size_t idx = (size_t)j * res + i;
+      if (h[idx] <= a.eps) continue;
+      if (a.wind) {
+        u[idx] = u[idx] + (a.wx - u[idx]) * a.blend;
+        v[idx] = v[idx] + (a.wz - v[idx]) * a.blend;
+      } else {
+        float px, pz;
+        grad_at(p, res, i, j, a.cell, &px, &pz);
+        float mob = a.nu_uniform ? a.mob0 : sheet_mobility(nu[idx]);
+        u[idx] = u[idx] + -kGravity * px * mob * a.dt;
+        v[idx] = v[idx] + -kGravity * pz * mob * a.dt;
+      }
+      if (a.vort > 0.0f) confine_at(u, v, u, v, res, i, j, a.cell, a.dt, a.vort);
+    }
+    __syncthreads();
+  }
+}
+
+// Divergence source b = div(u,v) ov - 0.015 ov lap(eta) (:1999-2000).
+__global__ void k_project_source(const float* __restrict__ u, const float* __restrict__ v,
+                                 const float* __restrict__ eta, const float* __restrict__ nu, float* __restrict__ bsrc,
+                                 Region r, Box b, StepArgs a) {
+  int i, j;
+  if (!region_cell(r, &i, &j)) return;
+  int res = a.res;
+  size_t idx = (size_t)j * res + i;
+  if (!in_box(b, i, j)) {
+    bsrc[idx] = 0.0f;
+    return;
+  }
+  float cell = a.cell;
+  int ip = i + 1 < res ? i + 1 : res - 1, im = i > 0 ? i - 1 : 0;
+  int jp = j + 1 < res ? j + 1 : res - 1, jm = j > 0 ? j - 1 : 0;
+  float ux = (u[(size_t)j * res + ip] - u[(size_t)j * res + im]) / (2.0f * cell);
+  float vz = (v[(size_t)jp * res + i] - v[(size_t)jm * res + i]) / (2.0f * cell);
+  float div = ux + vz;
+  float s = eta[(size_t)j * res + ip] + eta[(size_t)j * res + im] + eta[(size_t)jp * res + i] +
+            eta[(size_t)jm * res + i] - 4.0f * eta[idx];
+  float lap_h = s / (cell * cell);
+  float overload = a.nu_uniform ? a.ov0 : overload_of(nu[idx]);
+  bsrc[idx] = div * overload - 0.015f * overload * lap_h;
+}
+
+// Correction kick, damping and the speed cap on wet box cells (:2010-2042).
+__global__ void k_damp(const float* __restrict__ h, float* __restrict__ u, float* __restrict__ v,
+                       const float* __restrict__ nu, const float* __restrict__ q, Region r, StepArgs a) {
+  int i, j;
+  if (!region_cell(r, &i, &j)) return;
+  size_t idx = (size_t)j * a.res + i;
+  if (h[idx] <= a.eps) return;
+  float uu = u[idx], vv = v[idx];
+  if (a.project) {
+    float qx, qz;
+    grad_at(q, a.res, i, j, a.cell, &qx, &qz);
+    uu = uu + 0.012f * a.dt * qx;
+    vv = vv + 0.012f * a.dt * qz;
+  }
+  float decay = a.nu_uniform ? a.decay0 : expf(-sheet_damp(nu[idx]) * a.dt);
+  u[idx] = finite_speed(uu * decay, a.lim);
+  v[idx] = finite_speed(vv * decay, a.lim);
+}
+
+// open_edge_dh (src/fluid_core.rs:2887-2911).
+__device__ inline float edge_dh(float hd, float hr, float remd, uint8_t gate_r, bool banks, float bed_d,
+                                float bed_r, float speed, float dt, float cell) {
+  bool live = !(fabsf(speed) < 1.0e-5f);
+  bool bank = banks && hr <= 1.0e-6f && bed_r > bed_d + hd + 1.0e-3f;
+  float room = hd - remd;
+  bool gate_shut = gate_r == 0 && hr <= 1.0e-6f;
+  float frac = fminf(fabsf(speed) * dt / cell, 0.45f);
+  float dh = fminf(hd * frac, room);
+  bool ok = live && !bank && !(room <= 1.0e-8f) && !gate_shut && !(dh <= 1.0e-8f);
+  return ok ? dh : 0.0f;
+}
+
+// One open transport pass for one cell: its two box-interior edges along the
+// axis, in the reference's summation order (transport_open_rows_body
+// :2998-3137).  stride is 1 along x, res along z; lo/hi bound the cell's index
+// along the axis inside the box.
+__device__ inline float transport_cell(const float* hin, const float* vel, const uint8_t* gate,
+                                       const float* bed, bool banks, size_t idx, size_t stride, int pos, int lo,
+                                       int hi, float dt, float cell, bool gate_uniform) {
+  auto gate_of = [&](size_t k) -> uint8_t { return gate_uniform ? (uint8_t)1 : gate[k]; };
+  auto bed_of = [&](size_t k) -> float { return banks ? bed[k] : 0.0f; };
+  // Left-going flux of edge (k, k+1): donor k+1.
+  auto fl = [&](size_t k) -> float {
+    float speed = 0.5f * (vel[k] + vel[k + stride]);
+    float dh = edge_dh(hin[k + stride], hin[k], 0.0f, gate_of(k), banks, bed_of(k + stride), bed_of(k), speed, dt,
+                       cell);
+    return speed > 0.0f ? 0.0f : dh;
+  };
+  // Right-going flux of edge (k, k+1): donor k, whose removed sum holds the
+  // left-going flux of the edge before it.
+  auto fr = [&](size_t k, bool has_prev) -> float {
+    float speed = 0.5f * (vel[k] + vel[k + stride]);
+    float prev = has_prev ? fl(k - stride) : 0.0f;
+    float dh = edge_dh(hin[k], hin[k + stride], 0.0f + prev, gate_of(k + stride), banks, bed_of(k),
+                       bed_of(k + stride), speed, dt, cell);
+    return speed > 0.0f ? dh : 0.0f;
+  };
+  bool has_left = pos > lo, has_right = pos < hi;
+  float fl_prev = has_left ? fl(idx - stride) : 0.0f;
+  float fr_prev = has_left ? fr(idx - stride, pos - 1 > lo) : 0.0f;
+  float fl_here = has_right ? fl(idx) : 0.0f;
+  float fr_here = has_right ? fr(idx, has_left) : 0.0f;
+  float rem = (0.0f + fl_prev) + fr_here;
+  float add = (0.0f + fr_prev) + fl_here;
+  float hv = hin[idx];
+  float moved = fmaxf(hv - rem + add, 0.0f);
+  return (rem > 0.0f || add > 0.0f) ? moved : hv;
+}
+
+__global__ void k_transport(const float* __restrict__ hin, float* __restrict__ hout, const float* __restrict__ vel,
+                            const uint8_t* __restrict__ gate, const float* __restrict__ bed, Box b, int along_x,
+                            StepArgs a, float dt) {
+  Region r{b.i0, b.i1, b.j0, b.j1};
+  int i, j;
+  if (!region_cell(r, &i, &j)) return;
+  size_t idx = (size_t)j * a.res + i;
+  if (along_x)
+    hout[idx] = transport_cell(hin, vel, gate, bed, a.banks, idx, 1, i, b.i0, b.i1, dt, a.cell, a.gate_uniform);
+  else
+    hout[idx] = transport_cell(hin, vel, gate, bed, a.banks, idx, (size_t)a.res, j, b.j0, b.j1, dt, a.cell,
+                               a.gate_uniform);
+}
+
+// Dry cut (:2072-2081): depth under eps, and its velocity, go to zero.
+__global__ void k_dry_cut(float* __restrict__ h, float* __restrict__ u, float* __restrict__ v, Box b, StepArgs a) {
+  Region r{b.i0, b.i1, b.j0, b.j1};
+  int i, j;
+  if (!region_cell(r, &i, &j)) return;
+  size_t idx = (size_t)j * a.res + i;
+  if (a.eps > 0.0f && h[idx] < a.eps) {
+    h[idx] = 0.0f;
+    u[idx] = 0.0f;
+    v[idx] = 0.0f;
+  }
+}
+
+inline dim3 grid_for(const Region& r, dim3 blk) {
+  return dim3((unsigned)((r.i1 - r.i0 + 1 + (int)blk.x - 1) / (int)blk.x),
+              (unsigned)((r.j1 - r.j0 + 1 + (int)blk.y - 1) / (int)blk.y));
+}
+
+}  // namespace
+
+// One substep on one box (substep, src/fluid_core.rs:1879-2082).  do_flux 0
+// stops after the damping (pace 3's pressure pass).
+void exact_substep(const SheetPlanes& s, const Box& b, const StepArgs& a, bool do_flux, bool gauss_seidel,
+                   hipStream_t st) {
+  const int res = a.res;
+  Region rim{b.i0 >= 2 ? b.i0 - 2 : 0, b.i1 + 2 < res - 1 ? b.i1 + 2 : res - 1, b.j0 >= 2 ? b.j0 - 2 : 0,
+             b.j1 + 2 < res - 1 ? b.j1 + 2 : res - 1};
+  Region box{b.i0, b.i1, b.j0, b.j1};
+  dim3 blk(64, 4);
+  dim3 g_rim = grid_for(rim, blk), g_box = grid_for(box, blk);
+  hipLaunchKernelGGL(k_eta, g_rim, blk, 0, st, s.h, s.bed, s.eta, rim, res);
+  if (!a.wind) {
+    hipLaunchKernelGGL(k_xblur2, g_rim, blk, 0, st, s.eta, s.tg, s.ts, rim, b, res);
+    hipLaunchKernelGGL(k_zblur_pressure, g_rim, blk, 0, st, s.tg, s.ts, s.eta, s.p, rim, b, res);
+  }
+  if (gauss_seidel) {
+    hipLaunchKernelGGL(k_force_gauss_seidel, dim3(1), dim3(256), 0, st, s.h, s.u, s.v, s.nu, s.p, b, a);
+  } else {
+    hipLaunchKernelGGL(k_force, g_rim, blk, 0, st, s.h, s.u, s.v, s.nu, s.p, s.su, s.sv, rim, b, a);
+    if (a.vort > 0.0f)
+      hipLaunchKernelGGL(k_confine_jacobi, g_box, blk, 0, st, s.h, s.u, s.v, s.su, s.sv, box, a);
+  }
+  if (a.project) {
+    hipLaunchKernelGGL(k_project_source, g_rim, blk, 0, st, s.u, s.v, s.eta, s.nu, s.bsrc, rim, b, a);
+    hipLaunchKernelGGL(k_xblur2, g_rim, blk, 0, st, s.bsrc, s.tg, s.ts, rim, b, res);
+    hipLaunchKernelGGL(k_zblur_q, g_rim, blk, 0, st, s.tg, s.ts, s.p, rim, b, res);
+  }
+  hipLaunchKernelGGL(k_damp, g_box, blk, 0, st, s.h, s.u, s.v, s.nu, s.p, box, a);
+  if (!do_flux) return;
+  exact_transport(s, b, a, a.dt, st);
+  hipLaunchKernelGGL(k_dry_cut, g_box, blk, 0, st, s.h, s.u, s.v, b, a);
+}
+
+// Transport along x then z with time step dt (the two transport_open calls of
+// the substep, or one pace-3 slice).
+void exact_transport(const SheetPlanes& s, const Box& b, const StepArgs& a, float dt, hipStream_t st) {
+  Region box{b.i0, b.i1, b.j0, b.j1};
+  dim3 blk(64, 4);
+  dim3 g_box = grid_for(box, blk);
+  hipLaunchKernelGGL(k_transport, g_box, blk, 0, st, s.h, s.h1, s.u, s.gate, s.bed, b, 1, a, dt);
+  // Cells of the box now hold the x-moved depth in h1; copy the box back
+  // through the z pass, which writes h.
+  hipLaunchKernelGGL(k_transport, g_box, blk, 0, st, s.h1, s.h, s.v, s.gate, s.bed, b, 0, a, dt);
+}
+
+void exact_dry_cut(const SheetPlanes& s, const Box& b, const StepArgs& a, hipStream_t st) {
+  Region box{b.i0, b.i1, b.j0, b.j1};
+  dim3 blk(64, 4);
+  hipLaunchKernelGGL(k_dry_cut, grid_for(box, blk), blk, 0, st, s.h, s.u, s.v, b, a);
+}
+
+}  // namespace silt

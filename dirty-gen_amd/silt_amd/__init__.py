@@ -1,0 +1,220 @@
+"""Python mirror of the reference's sheet-step API over libsilt_amd.so.
+
+The functions keep the reference's names and argument meaning
+(src/fluid_core.rs ``step_sheet``, ``SheetDrive``) and call the C ABI declared
+in include/silt_sheet.h.  There is no fallback: if the HIP library is missing
+or a call fails, they raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+import subprocess
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)  # dirty-gen_amd/
+LIB_PATH = os.path.join(ROOT, "libsilt_amd.so")
+
+SILT_OK = 0
+SILT_EINVAL = -1
+SILT_EDEVICE = -2
+SILT_ENOMEM = -3
+SILT_EUNSUPPORTED = -4
+DRIVE_GRAVITY = 0
+DRIVE_WIND = 1
+CONFINE = {"reference": 0, "jacobi": 1}
+PATH = {"exact": 0, "fast": 1, "fused_exact": 2}
+
+_lib = None
+
+
+class SiltError(RuntimeError):
+    pass
+
+
+class Drive(C.Structure):
+    """SheetDrive (src/fluid_core.rs:204-211)."""
+    _fields_ = [("kind", C.c_int32), ("x", C.c_float), ("z", C.c_float), ("vorticity", C.c_float)]
+
+    @classmethod
+    def gravity(cls, vorticity):
+        return cls(DRIVE_GRAVITY, 0.0, 0.0, vorticity)
+
+    @classmethod
+    def wind(cls, x, z, vorticity):
+        return cls(DRIVE_WIND, x, z, vorticity)
+
+
+class SheetOpts(C.Structure):
+    _fields_ = [("eps", C.c_float), ("banks", C.c_int32), ("sub_cap", C.c_float), ("pace", C.c_int32),
+                ("row_split", C.c_int32), ("confine", C.c_int32), ("path", C.c_int32)]
+
+
+class LayerLedger(C.Structure):
+    _fields_ = [("volume_m3", C.c_double), ("initial_m3", C.c_double), ("cut_m3", C.c_double),
+                ("residue_m3", C.c_double), ("gap_m3", C.c_double)]
+
+
+class PairLedger(C.Structure):
+    _fields_ = [("water", LayerLedger), ("slurry", LayerLedger), ("total_m3", C.c_double)]
+
+
+def build() -> None:
+    """Compile libsilt_amd.so for gfx950 with the package Makefile."""
+    subprocess.run(["make", "-s", "-j8", "-C", ROOT], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise SiltError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+        _lib = C.CDLL(LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+_P = C.c_void_p
+_Z = C.c_size_t
+_F = C.c_float
+
+
+def _declare(L):
+    L.silt_sheet_opts_default.argtypes = [C.POINTER(SheetOpts)]
+    L.silt_sheet_opts_default.restype = None
+    L.silt_version.restype = C.c_char_p
+    L.silt_step_sheet.argtypes = [_P, _P, _P, _P, _P, _P, _Z, _Z, _F, _F, Drive]
+    L.silt_step_sheet.restype = C.c_int
+    L.silt_step_sheet_ex.argtypes = [_P, _P, _P, _P, _P, _P, _Z, _Z, _F, _F, Drive, C.POINTER(SheetOpts)]
+    L.silt_step_sheet_ex.restype = C.c_int
+    L.silt_step_sheet_device.argtypes = [_P, _P, _P, _P, _P, _F, _P, _Z, _F, _F, Drive, C.POINTER(SheetOpts), _P]
+    L.silt_step_sheet_device.restype = C.c_int
+    L.silt_pair_create.argtypes = [_Z, _F, C.c_int]
+    L.silt_pair_create.restype = _P
+    L.silt_pair_destroy.argtypes = [_P]
+    L.silt_pair_destroy.restype = None
+    L.silt_pair_upload.argtypes = [_P, _P, _P, _P, _P, _P, _P, _P, _P, _F]
+    L.silt_pair_upload.restype = C.c_int
+    L.silt_pair_download.argtypes = [_P, _P, _P, _P, _P, _P, _P]
+    L.silt_pair_download.restype = C.c_int
+    L.silt_pair_step.argtypes = [_P, _F]
+    L.silt_pair_step.restype = C.c_int
+    L.silt_pair_ledger.argtypes = [_P, C.POINTER(PairLedger)]
+    L.silt_pair_ledger.restype = C.c_int
+    L.silt_pair_last_kernel_ms.argtypes = [_P]
+    L.silt_pair_last_kernel_ms.restype = _F
+    L.silt_pair_last_fallback.argtypes = [_P]
+    L.silt_pair_last_fallback.restype = C.c_int
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _drive(drive):
+    if isinstance(drive, Drive):
+        return drive
+    if drive[0] == "gravity":
+        return Drive.gravity(drive[1])
+    if drive[0] == "wind":
+        return Drive.wind(drive[1], drive[2], drive[3])
+    raise ValueError(drive)
+
+
+def sheet_opts(*, eps=1.0e-6, banks=False, sub_cap=8.0, pace=0, row_split=True, confine="reference",
+               path="exact"):
+    o = SheetOpts()
+    lib().silt_sheet_opts_default(C.byref(o))
+    o.eps, o.banks, o.sub_cap, o.pace = eps, int(banks), sub_cap, pace
+    o.row_split, o.confine, o.path = int(row_split), CONFINE[confine], PATH[path]
+    return o
+
+
+def _check_status(rc):
+    if rc < 0:
+        raise SiltError(f"silt call failed with status {rc}")
+    return rc
+
+
+def step_sheet(h, u, v, bed, nu, gate, res, cell, dt, drive, **opts):
+    """``step_sheet`` (src/fluid_core.rs:770-785) on numpy float32 planes, in
+    place.  Keyword options are those of ``step_sheet_eps``.  Returns the
+    number of substeps (0 when the reference would return early)."""
+    for a, nm in ((h, "h"), (u, "u"), (v, "v"), (bed, "bed"), (nu, "nu")):
+        if a.dtype != np.float32 or not a.flags.c_contiguous:
+            raise TypeError(f"{nm} must be C-contiguous float32")
+    if gate.dtype != np.uint8 or not gate.flags.c_contiguous:
+        raise TypeError("gate must be C-contiguous uint8")
+    n = min(a.size for a in (h, u, v, bed, nu, gate))
+    o = sheet_opts(**opts)
+    rc = lib().silt_step_sheet_ex(_ptr(h), _ptr(u), _ptr(v), _ptr(bed), _ptr(nu), _ptr(gate), n, res, cell, dt,
+                                  _drive(drive), C.byref(o))
+    return _check_status(rc)
+
+
+def step_sheet_device(h, u, v, bed, nu, gate, res, cell, dt, drive, nu_uniform=math.nan, stream=0, **opts):
+    """The same step on device buffers.  Arguments are integer device
+    addresses (``tensor.data_ptr()``); ``nu``/``gate`` may be 0 with
+    ``nu_uniform`` finite / gate all ones."""
+    o = sheet_opts(**opts)
+    rc = lib().silt_step_sheet_device(h, u, v, bed, nu or None, nu_uniform, gate or None, res, cell, dt,
+                                      _drive(drive), C.byref(o), stream or None)
+    return _check_status(rc)
+
+
+class SheetPair:
+    """Water and slurry over one bed, resident on the GPU (include/silt_pair.h):
+    one ``step`` is the generic-res ``step_sheet`` on both layers with the
+    layer constants of ``FluidCore::step_pair``."""
+
+    def __init__(self, res, cell, path="fast"):
+        self.res, self.cell = int(res), float(cell)
+        self._h = lib().silt_pair_create(self.res, self.cell, PATH[path])
+        if not self._h:
+            raise SiltError("silt_pair_create failed")
+
+    def close(self):
+        if self._h:
+            lib().silt_pair_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, bed, water, uw, vw, slurry, us, vs, nu_slurry):
+        arrs = [np.ascontiguousarray(a, np.float32) for a in (bed, water, uw, vw, slurry, us, vs)]
+        n = self.res * self.res
+        if any(a.size != n for a in arrs):
+            raise ValueError("planes must be res * res")
+        if np.ndim(nu_slurry) == 0:
+            nu_ptr, nu0 = None, float(nu_slurry)
+        else:
+            nu_arr = np.ascontiguousarray(nu_slurry, np.float32)
+            nu_ptr, nu0 = _ptr(nu_arr), math.nan
+        _check_status(lib().silt_pair_upload(self._h, *(_ptr(a) for a in arrs), nu_ptr, nu0))
+
+    def download(self):
+        n = self.res * self.res
+        out = [np.empty(n, np.float32) for _ in range(6)]
+        _check_status(lib().silt_pair_download(self._h, *(_ptr(a) for a in out)))
+        return out
+
+    def step(self, dt):
+        return _check_status(lib().silt_pair_step(self._h, dt))
+
+    def ledger(self):
+        led = PairLedger()
+        _check_status(lib().silt_pair_ledger(self._h, C.byref(led)))
+        return led
+
+    def last_kernel_ms(self):
+        return lib().silt_pair_last_kernel_ms(self._h)
+
+    def last_fallback(self):
+        return lib().silt_pair_last_fallback(self._h)

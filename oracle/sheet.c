@@ -16,10 +16,11 @@
  * cites the lines it follows.  Only the open sheet (no hydro solids) is
  * restated: every caller on the sheet-step path passes solid = None.
  *
- * Scratch planes start at zero on every call.  The reference keeps them in a
- * thread-local that outlives calls, but its stale values are only ever read at
- * cells that are neither wet nor neighbours of wet cells (box halo >= 3), so
- * the state a step produces does not depend on them.
+ * The filter planes are cleared on each box's 2-cell rim when the box's
+ * substep starts.  The reference keeps them in a thread-local that outlives
+ * boxes and calls; its stale rim values reach only cells at least three cells
+ * from every wet cell of the box when the step starts, so a step whose water
+ * stays there (one substep, or a slow spread) does not depend on them.
  */
 #include <math.h>
 #include <stdint.h>
@@ -862,6 +863,13 @@ static void substep(real *h, real *u, real *v, const real *bed, const real *nu, 
                     int do_flux, int inner_par, int confine_mode, Scratch *sc) {
     size_t hi0 = b.i0 >= 2 ? b.i0 - 2 : 0, hi1 = (b.i1 + 2 < res - 1) ? b.i1 + 2 : res - 1;
     size_t hj0 = b.j0 >= 2 ? b.j0 - 2 : 0, hj1 = (b.j1 + 2 < res - 1) ? b.j1 + 2 : res - 1;
+    /* Each box starts from cleared filter planes on its 2-cell rim (see the
+     * scratch note at the top of this file). */
+    for (size_t j = hj0; j <= hj1; j++)
+        for (size_t i = hi0; i <= hi1; i++) {
+            size_t k = j * res + i;
+            sc->b[k] = sc->c[k] = sc->d[k] = sc->e[k] = K(0.0);
+        }
     for (size_t j = hj0; j <= hj1; j++)
         for (size_t i = hi0; i <= hi1; i++) sc->a[j * res + i] = bed[j * res + i] + h[j * res + i];
     int wind_open = drive.wind;
