@@ -1,0 +1,999 @@
+/*
+ * oracle/core.c -- CPU restatement of the reference's two-layer FluidCore
+ * step on the sheet path: the 40 m coarse window and the 2 cm patch under the
+ * tires, water and slurry.
+ *
+ * TEST INFRASTRUCTURE ONLY (see oracle/sheet.c).  Compiled twice, FP32 and
+ * FP64, like sheet.c, whose step_sheet_ex it calls for every sheet.
+ *
+ * Reference: src/fluid_core.rs and src/fluid_core/near.rs of x3-team/Dirty-gen.
+ * Restated: FluidCore::step (:6034-6128) restricted to the open-sheet path --
+ * refine (:8358-8460), near field (near.rs:150-328), tire shoves
+ * (push_on :8169-8239, apply_pushes :8152-8167), step_pair (:7942-8150),
+ * spray peel (:8241-8314), restrict_fine (:8462-8555), soften_border
+ * (:9594-9620); the setters map_water_bed (:5329-5355), paint/pour
+ * (:5820-5855, :5424-5440), lay_slurry_mask (:5857-5880),
+ * read_substrate_grid (:5928-5962); ledger() (:6821-6883) for the two sheets,
+ * parked volume and spray; coarse_cell (:6998-7034).
+ * Not restated (no sheet-path configuration enables them): wet bodies, hydro
+ * solids, waves, open water, river reach, stream cycle, tracers, suspended
+ * solids, dust and fog, window moves.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#ifndef REAL
+#define REAL float
+#define SUFFIX _f32
+#endif
+
+typedef REAL real;
+
+#define PASTE_(a, b) a##b
+#define PASTE(a, b) PASTE_(a, b)
+#define F(name) PASTE(name, SUFFIX)
+#define K(x) ((real)(float)(x))
+
+#if defined(REAL_IS_DOUBLE)
+#define R_EXP exp
+#define R_CEIL ceil
+#define R_SQRT sqrt
+#define R_FLOOR floor
+#define R_FABS fabs
+#define R_FMIN fmin
+#define R_FMAX fmax
+#define R_HYPOT hypot
+#define R_COPYSIGN copysign
+#else
+#define R_EXP expf
+#define R_CEIL ceilf
+#define R_SQRT sqrtf
+#define R_FLOOR floorf
+#define R_FABS fabsf
+#define R_FMIN fminf
+#define R_FMAX fmaxf
+#define R_HYPOT hypotf
+#define R_COPYSIGN copysignf
+#endif
+
+#define WRES 200
+#define FRES 160
+#define WN (WRES * WRES)
+#define FN (FRES * FRES)
+#define COARSE K(0.20)
+#define FINE K(0.02)
+#define NEAR_SUB 10
+#define NEAR_SUB2 100
+#define MAX_PUSH 64
+
+/* From sheet.c (same precision). */
+int F(oracle_step_sheet_ex)(real *h, real *u, real *v, const real *bed, const real *nu, const uint8_t *gate,
+                            size_t res, real cell, real dt, int drive_wind, real wind_x, real wind_z,
+                            real vorticity, real eps, int banks, real sub_cap, int pace, int inner_par,
+                            int confine_mode, size_t *boxes_out, size_t *nboxes_out);
+real F(oracle_dynamic_viscosity)(int grain, real moisture);
+
+typedef struct {
+    real x, z, fx, fz, radius, width, speed, slip;
+} Push;
+
+/* One residual tile of the near field, keyed by the world 20 cm cell. */
+typedef struct {
+    int32_t kx, kz;
+    int used;
+    real w[NEAR_SUB2], s[NEAR_SUB2], uw[NEAR_SUB2], vw[NEAR_SUB2];
+    int fresh;
+    real idle_s, peak;
+} Tile;
+
+typedef struct {
+    real origin[2];
+    real *water, *slurry, *uw, *vw, *us, *vs, *bed, *moisture;
+    uint8_t *grain;
+    real fine_origin[2];
+    real *fw, *fs, *fuw, *fvw, *fus, *fvs, *fbed, *fbed_s, *fmoist;
+    uint8_t *fgrain;
+    uint8_t *hold;
+    int fine_on;
+    int fine_pace;
+    real fine_sub_cap;
+    int coarse_confine; /* confine mode of the coarse sheets (0 reference = Gauss-Seidel here) */
+    int near_on;
+    Push pushes[MAX_PUSH];
+    int npush;
+    real spray_m3;
+    real frozen_w, frozen_s; /* fine volume parked off the window, cubic metres */
+    /* near field: open addressing over world keys */
+    Tile *tiles;
+    size_t tcap;
+    size_t ntiles;
+    /* scratch */
+    real *nu_w, *nu_s;
+    uint8_t *gate;
+    int spray_count;
+} Core;
+
+static real *zalloc(size_t n) { return (real *)calloc(n, sizeof(real)); }
+
+Core *F(oracle_core_new)(real cx, real cz) {
+    Core *c = (Core *)calloc(1, sizeof(Core));
+    real extent = (real)WRES * COARSE;
+    /* snap_origin (src/fluid_core.rs:9888-9892) */
+    real x = cx - extent * K(0.5), z = cz - extent * K(0.5);
+    c->origin[0] = R_FLOOR(x / COARSE) * COARSE;
+    c->origin[1] = R_FLOOR(z / COARSE) * COARSE;
+    c->water = zalloc(WN); c->slurry = zalloc(WN); c->uw = zalloc(WN); c->vw = zalloc(WN);
+    c->us = zalloc(WN); c->vs = zalloc(WN); c->bed = zalloc(WN); c->moisture = zalloc(WN);
+    c->grain = (uint8_t *)malloc(WN);
+    memset(c->grain, 4, WN); /* Grain::Other */
+    c->fw = zalloc(FN); c->fs = zalloc(FN); c->fuw = zalloc(FN); c->fvw = zalloc(FN);
+    c->fus = zalloc(FN); c->fvs = zalloc(FN); c->fbed = zalloc(FN); c->fbed_s = zalloc(FN);
+    c->fmoist = zalloc(FN);
+    c->fgrain = (uint8_t *)malloc(FN);
+    memset(c->fgrain, 4, FN);
+    c->hold = (uint8_t *)calloc(WN, 1);
+    c->fine_pace = 0;
+    c->fine_sub_cap = K(8.0);
+    c->near_on = 1;
+    c->tcap = 1 << 14;
+    c->tiles = (Tile *)calloc(c->tcap, sizeof(Tile));
+    c->nu_w = zalloc(WN);
+    c->nu_s = zalloc(WN);
+    c->gate = (uint8_t *)malloc(WN);
+    memset(c->gate, 1, WN);
+    real nw = F(oracle_dynamic_viscosity)(3, K(1.0));
+    for (size_t k = 0; k < WN; k++) c->nu_w[k] = nw;
+    return c;
+}
+
+void F(oracle_core_free)(Core *c) {
+    if (!c) return;
+    real *planes[] = {c->water, c->slurry, c->uw, c->vw, c->us, c->vs, c->bed, c->moisture, c->fw, c->fs, c->fuw,
+                      c->fvw, c->fus, c->fvs, c->fbed, c->fbed_s, c->fmoist, c->nu_w, c->nu_s};
+    for (size_t k = 0; k < sizeof planes / sizeof planes[0]; k++) free(planes[k]);
+    free(c->grain); free(c->fgrain); free(c->hold); free(c->tiles); free(c->gate);
+    free(c);
+}
+
+void F(oracle_core_set_fine_pace)(Core *c, int pace) { c->fine_pace = pace > 3 ? 3 : pace; }
+void F(oracle_core_set_fine_sub_cap)(Core *c, real cap) {
+    c->fine_sub_cap = isfinite(cap) ? (cap < K(1.0) ? K(1.0) : (cap > K(8.0) ? K(8.0) : cap)) : K(8.0);
+}
+void F(oracle_core_set_coarse_confine)(Core *c, int mode) { c->coarse_confine = mode; }
+void F(oracle_core_set_near_field)(Core *c, int on) {
+    c->near_on = on;
+    if (!on) {
+        memset(c->tiles, 0, c->tcap * sizeof(Tile));
+        c->ntiles = 0;
+    }
+}
+
+/* ---- near field tile map --------------------------------------------------- */
+
+static size_t tile_hash(int32_t kx, int32_t kz, size_t cap) {
+    uint32_t h = (uint32_t)kx * 0x9E3779B1u ^ ((uint32_t)kz * 0x85EBCA6Bu + 0x27d4eb2fu);
+    h ^= h >> 15;
+    return (size_t)h & (cap - 1);
+}
+
+static Tile *tile_find(Core *c, int32_t kx, int32_t kz) {
+    size_t i = tile_hash(kx, kz, c->tcap);
+    for (size_t n = 0; n < c->tcap; n++, i = (i + 1) & (c->tcap - 1)) {
+        Tile *t = &c->tiles[i];
+        if (!t->used) return NULL;
+        if (t->used == 1 && t->kx == kx && t->kz == kz) return t;
+    }
+    return NULL;
+}
+
+static void tile_remove(Core *c, int32_t kx, int32_t kz) {
+    Tile *t = tile_find(c, kx, kz);
+    if (t) {
+        t->used = 2; /* tombstone */
+        c->ntiles--;
+    }
+}
+
+static Tile *tile_get_or_insert(Core *c, int32_t kx, int32_t kz) {
+    Tile *t = tile_find(c, kx, kz);
+    if (t) return t;
+    size_t i = tile_hash(kx, kz, c->tcap);
+    for (size_t n = 0; n < c->tcap; n++, i = (i + 1) & (c->tcap - 1)) {
+        Tile *s = &c->tiles[i];
+        if (s->used != 1) {
+            memset(s, 0, sizeof *s);
+            s->used = 1;
+            s->kx = kx;
+            s->kz = kz;
+            c->ntiles++;
+            return s;
+        }
+    }
+    return NULL;
+}
+
+size_t F(oracle_core_near_tiles)(Core *c) { return c->ntiles; }
+
+/* ---- geometry helpers ------------------------------------------------------ */
+
+/* coarse_axis (src/fluid_core.rs:9557-9568); -1 outside the window. */
+static int coarse_axis(real origin, real p) {
+    real i = R_FLOOR((p - origin) / COARSE);
+    if (i < K(0.0)) return -1;
+    if (i >= (real)WRES) return -1;
+    return (int)i;
+}
+
+static void coarse_xz(const Core *c, int i, int j, real *x, real *z) {
+    *x = c->origin[0] + ((real)i + K(0.5)) * COARSE;
+    *z = c->origin[1] + ((real)j + K(0.5)) * COARSE;
+}
+
+/* fine_coarse_axes (src/fluid_core.rs:9518-9534); -1 = off the window. */
+static void fine_axes(const Core *c, int col[FRES], int row[FRES]) {
+    for (int i = 0; i < FRES; i++)
+        col[i] = coarse_axis(c->origin[0], c->fine_origin[0] + ((real)i + K(0.5)) * FINE);
+    for (int j = 0; j < FRES; j++)
+        row[j] = coarse_axis(c->origin[1], c->fine_origin[1] + ((real)j + K(0.5)) * FINE);
+}
+
+/* world_key (src/fluid_core.rs:9894-9896) */
+static void world_key(real x, real z, int32_t *kx, int32_t *kz) {
+    *kx = (int32_t)R_FLOOR(x / COARSE);
+    *kz = (int32_t)R_FLOOR(z / COARSE);
+}
+
+/* index_span (src/fluid_core.rs:9877-9886) */
+static void index_span(real origin, real cell, int res, real center, real reach, int *lo, int *hi) {
+    if (!(cell > K(0.0)) || res == 0) {
+        *lo = *hi = 0;
+        return;
+    }
+    real l = R_FMAX(R_FLOOR((center - reach - origin) / cell - K(1.0)), K(0.0));
+    real h = R_FMAX(R_CEIL((center + reach - origin) / cell + K(2.0)), K(0.0));
+    long li = (long)l, hi_ = (long)h;
+    if (li > res) li = res;
+    if (hi_ > res) hi_ = res;
+    if (hi_ < li) hi_ = li;
+    *lo = (int)li;
+    *hi = (int)hi_;
+}
+
+/* dist2_segment (src/fluid_core.rs:9648-9662) */
+static real dist2_segment(real px, real pz, real ax, real az, real bx, real bz) {
+    real abx = bx - ax, abz = bz - az;
+    real ab2 = abx * abx + abz * abz;
+    real t;
+    if (ab2 < K(1.0e-8)) {
+        t = K(0.0);
+    } else {
+        t = ((px - ax) * abx + (pz - az) * abz) / ab2;
+        if (t < K(0.0)) t = K(0.0);
+        if (t > K(1.0)) t = K(1.0);
+    }
+    real dx = px - (ax + abx * t), dz = pz - (az + abz * t);
+    return dx * dx + dz * dz;
+}
+
+/* near_tire_zone (src/fluid_core.rs:9624-9646) */
+static int near_tire_zone(const Push *p, int n, real x, real z, real r2) {
+    for (int k = 0; k < n; k++) {
+        real dx = x - p[k].x, dz = z - p[k].z;
+        if (dx * dx + dz * dz <= r2) return 1;
+    }
+    for (int a = 0; a < n; a++)
+        for (int b = a + 1; b < n; b++) {
+            if (R_FABS(p[a].x - p[b].x) >= K(0.45)) continue;
+            if (dist2_segment(x, z, p[a].x, p[a].z, p[b].x, p[b].z) <= r2) return 1;
+        }
+    return 0;
+}
+
+static int fine_held_at(const Core *c, real x, real z) {
+    int i = coarse_axis(c->origin[0], x), j = coarse_axis(c->origin[1], z);
+    if (i < 0 || j < 0) return 0;
+    return c->hold[j * WRES + i] == 1;
+}
+
+/* cell_is_refined (src/fluid_core.rs:8337-8356) */
+static int cell_is_refined(const Core *c, real x, real z) {
+    if (!c->fine_on || !fine_held_at(c, x, z)) return 0;
+    int i = coarse_axis(c->origin[0], x), j = coarse_axis(c->origin[1], z);
+    real x0 = c->origin[0] + (real)i * COARSE, z0 = c->origin[1] + (real)j * COARSE;
+    real x1 = x0 + COARSE, z1 = z0 + COARSE;
+    real extent = (real)FRES * FINE;
+    real r0 = c->fine_origin[0], r1 = c->fine_origin[0] + extent;
+    real r2 = c->fine_origin[1], r3 = c->fine_origin[1] + extent;
+    return x0 >= r0 - K(1.0e-4) && z0 >= r2 - K(1.0e-4) && x1 <= r1 + K(1.0e-4) && z1 <= r3 + K(1.0e-4);
+}
+
+/* ---- near field: src/fluid_core/near.rs:114-328 ------------------------------ */
+
+typedef struct {
+    int cidx;
+    int32_t kx, kz;
+    int i0, j0;
+} NearCol;
+
+/* near_columns (near.rs:152-183): held columns covered by exactly 10 x 10 fine cells. */
+static int near_columns(const Core *c, NearCol *out) {
+    int col[FRES], row[FRES];
+    fine_axes(c, col, row);
+    int cr[FRES][2], nc = 0, rr[FRES][2], nr = 0;
+    for (int pass = 0; pass < 2; pass++) {
+        int *ax = pass == 0 ? col : row;
+        int(*dst)[2] = pass == 0 ? cr : rr;
+        int *cnt = pass == 0 ? &nc : &nr;
+        int i = 0;
+        while (i < FRES) {
+            int cc = ax[i], n = 1;
+            while (i + n < FRES && ax[i + n] == cc) n++;
+            if (cc != -1 && n == NEAR_SUB) {
+                dst[*cnt][0] = cc;
+                dst[*cnt][1] = i;
+                (*cnt)++;
+            }
+            i += n;
+        }
+    }
+    int n = 0;
+    for (int a = 0; a < nr; a++)
+        for (int b = 0; b < nc; b++) {
+            int cidx = rr[a][0] * WRES + cr[b][0];
+            if (c->hold[cidx] != 1) continue;
+            real x, z;
+            coarse_xz(c, cr[b][0], rr[a][0], &x, &z);
+            out[n].cidx = cidx;
+            world_key(x, z, &out[n].kx, &out[n].kz);
+            out[n].i0 = cr[b][1];
+            out[n].j0 = rr[a][1];
+            n++;
+        }
+    return n;
+}
+
+static int near_inner(const Core *c, int cidx) {
+    int i = cidx % WRES, j = cidx / WRES;
+    if (i == 0 || j == 0 || i + 1 >= WRES || j + 1 >= WRES) return 0;
+    return c->hold[cidx - 1] == 1 && c->hold[cidx + 1] == 1 && c->hold[cidx - WRES] == 1 &&
+           c->hold[cidx + WRES] == 1;
+}
+
+/* recentre (near.rs:114-121): the mean is taken in f64. */
+static void recentre(real *r) {
+    double m = 0.0;
+    for (int k = 0; k < NEAR_SUB2; k++) m += (double)r[k];
+    real mean = (real)(m / (double)NEAR_SUB2);
+    for (int k = 0; k < NEAR_SUB2; k++) r[k] -= mean;
+}
+
+/* abs_peak (near.rs:124-133) */
+static real abs_peak(const real *r) {
+    real hi[4] = {0, 0, 0, 0};
+    for (int k = 0; k < NEAR_SUB2; k += 4)
+        for (int l = 0; l < 4; l++) {
+            real a = R_FABS(r[k + l]);
+            hi[l] = a > hi[l] ? a : hi[l];
+        }
+    return R_FMAX(R_FMAX(R_FMAX(hi[0], hi[1]), hi[2]), hi[3]);
+}
+
+/* floor_scale (near.rs:136-148) */
+static real floor_scale(real base, const real *r) {
+    real low = K(0.0);
+    for (int k = 0; k < NEAR_SUB2; k++) low = R_FMIN(low, r[k]);
+    if (low >= K(0.0) || base + low >= K(0.0)) return K(1.0);
+    if (base <= K(0.0)) return K(0.0);
+    real s = base / -low;
+    return s < K(0.0) ? K(0.0) : (s > K(1.0) ? K(1.0) : s);
+}
+
+static real tile_left(const Tile *t) {
+    if (t->idle_s <= K(0.0)) return K(1.0);
+    return R_EXP(-t->idle_s / K(1.5));
+}
+
+/* near_capture (near.rs:199-261) */
+static void near_capture(Core *c) {
+    if (!c->fine_on || !c->near_on) return;
+    static NearCol cols[FN / NEAR_SUB2 + 4];
+    int n = near_columns(c, cols);
+    for (int q = 0; q < n; q++) {
+        NearCol nc = cols[q];
+        real sw = K(0.0), ss = K(0.0);
+        for (int b = 0; b < NEAR_SUB; b++) {
+            int f = (nc.j0 + b) * FRES + nc.i0;
+            real rw = -K(0.0), rs = -K(0.0);
+            for (int a = 0; a < NEAR_SUB; a++) {
+                rw += c->fw[f + a];
+                rs += c->fs[f + a];
+            }
+            sw += rw;
+            ss += rs;
+        }
+        if (sw <= K(0.0) && ss <= K(0.0)) {
+            tile_remove(c, nc.kx, nc.kz);
+            continue;
+        }
+        real cnt = (real)NEAR_SUB2;
+        real mw = sw / cnt, ms = ss / cnt;
+        real rw[NEAR_SUB2], rs[NEAR_SUB2];
+        for (int b = 0; b < NEAR_SUB; b++) {
+            int f = (nc.j0 + b) * FRES + nc.i0;
+            for (int a = 0; a < NEAR_SUB; a++) {
+                rw[b * NEAR_SUB + a] = c->fw[f + a] - mw;
+                rs[b * NEAR_SUB + a] = c->fs[f + a] - ms;
+            }
+        }
+        real peak = R_FMAX(abs_peak(rw), abs_peak(rs));
+        int inner = near_inner(c, nc.cidx);
+        if (peak <= K(1.0e-5) && !inner) {
+            tile_remove(c, nc.kx, nc.kz);
+            continue;
+        }
+        recentre(rw);
+        recentre(rs);
+        Tile *t = tile_get_or_insert(c, nc.kx, nc.kz);
+        if (!t) continue;
+        memcpy(t->w, rw, sizeof rw);
+        memcpy(t->s, rs, sizeof rs);
+        if (inner && sw > K(1.0e-10)) {
+            real mu = K(0.0), mv = K(0.0);
+            for (int b = 0; b < NEAR_SUB; b++) {
+                int f = (nc.j0 + b) * FRES + nc.i0;
+                for (int a = 0; a < NEAR_SUB; a++) {
+                    real w = c->fw[f + a];
+                    mu += w * c->fuw[f + a];
+                    mv += w * c->fvw[f + a];
+                }
+            }
+            real uw = mu / sw, vw = mv / sw;
+            for (int b = 0; b < NEAR_SUB; b++) {
+                int f = (nc.j0 + b) * FRES + nc.i0;
+                for (int a = 0; a < NEAR_SUB; a++) {
+                    int wet = c->fw[f + a] > K(1.0e-6);
+                    t->uw[b * NEAR_SUB + a] = wet ? c->fuw[f + a] - uw : K(0.0);
+                    t->vw[b * NEAR_SUB + a] = wet ? c->fvw[f + a] - vw : K(0.0);
+                }
+            }
+        } else {
+            memset(t->uw, 0, sizeof t->uw);
+            memset(t->vw, 0, sizeof t->vw);
+        }
+        t->peak = peak;
+        t->idle_s = K(0.0);
+        t->fresh = 1;
+    }
+}
+
+/* near_fade (near.rs:300-313) */
+static void near_fade(Core *c, real dt) {
+    if (!c->ntiles) return;
+    if (!(dt > K(0.0) && isfinite(dt))) dt = K(0.0);
+    for (size_t k = 0; k < c->tcap; k++) {
+        Tile *t = &c->tiles[k];
+        if (t->used != 1) continue;
+        if (t->fresh) {
+            t->fresh = 0;
+            continue;
+        }
+        t->idle_s += dt;
+        if (!(t->peak * tile_left(t) > K(1.0e-5))) {
+            t->used = 2;
+            c->ntiles--;
+        }
+    }
+}
+
+/* near_apply (near.rs:265-297) */
+static void near_apply(Core *c, real dt) {
+    if (c->ntiles) {
+        static NearCol cols[FN / NEAR_SUB2 + 4];
+        int n = near_columns(c, cols);
+        for (int q = 0; q < n; q++) {
+            NearCol nc = cols[q];
+            Tile *t = tile_find(c, nc.kx, nc.kz);
+            if (!t) continue;
+            int f0 = nc.j0 * FRES + nc.i0;
+            real bw = c->fw[f0], bs = c->fs[f0];
+            int inner = near_inner(c, nc.cidx);
+            real left = tile_left(t);
+            real aw = left * floor_scale(bw / R_FMAX(left, K(1.0e-6)), t->w);
+            real as = left * floor_scale(bs / R_FMAX(left, K(1.0e-6)), t->s);
+            for (int b = 0; b < NEAR_SUB; b++)
+                for (int a = 0; a < NEAR_SUB; a++) {
+                    int f = (nc.j0 + b) * FRES + nc.i0 + a;
+                    int k = b * NEAR_SUB + a;
+                    real w = R_FMAX(bw + aw * t->w[k], K(0.0));
+                    real s = R_FMAX(bs + as * t->s[k], K(0.0));
+                    c->fw[f] = w;
+                    c->fs[f] = s;
+                    if (inner && w > K(1.0e-6)) {
+                        c->fuw[f] += left * t->uw[k];
+                        c->fvw[f] += left * t->vw[k];
+                    }
+                }
+        }
+    }
+    near_fade(c, dt);
+}
+
+/* ---- refine / restrict: src/fluid_core.rs:8358-8555 -------------------------- */
+
+static void restrict_fine(Core *c) {
+    if (!c->fine_on) return;
+    near_capture(c);
+    const real fine_area = FINE * FINE, coarse_area = COARSE * COARSE;
+    int col[FRES], row[FRES];
+    fine_axes(c, col, row);
+    static real aw[WN], as[WN], auw[WN], avw[WN], aus[WN], avs[WN];
+    static uint32_t cnt[WN];
+    static int touch[WN];
+    int ntouch = 0;
+    for (int j = 0; j < FRES; j++) {
+        int cj = row[j];
+        for (int i = 0; i < FRES; i++) {
+            int ci = col[i];
+            int fidx = j * FRES + i;
+            if (cj == -1 || ci == -1) {
+                /* Off the coarse window: park the fine column's volume. */
+                c->frozen_w += c->fw[fidx] * fine_area / coarse_area * coarse_area;
+                c->frozen_s += c->fs[fidx] * fine_area / coarse_area * coarse_area;
+                continue;
+            }
+            int cidx = cj * WRES + ci;
+            if (cnt[cidx] == 0) touch[ntouch++] = cidx;
+            cnt[cidx]++;
+            real fw = c->fw[fidx], fs = c->fs[fidx];
+            if (fw == K(0.0) && fs == K(0.0)) continue;
+            aw[cidx] += c->fw[fidx] * fine_area;
+            as[cidx] += c->fs[fidx] * fine_area;
+            auw[cidx] += c->fuw[fidx] * c->fw[fidx] * fine_area;
+            avw[cidx] += c->fvw[fidx] * c->fw[fidx] * fine_area;
+            aus[cidx] += c->fus[fidx] * c->fs[fidx] * fine_area;
+            avs[cidx] += c->fvs[fidx] * c->fs[fidx] * fine_area;
+        }
+    }
+    const uint32_t full = 100;
+    for (int t = 0; t < ntouch; t++) {
+        int idx = touch[t];
+        int held = c->hold[idx] == 1;
+        real w = aw[idx] / coarse_area, s = as[idx] / coarse_area;
+        if (!held && cnt[idx] >= full) {
+            if (!(w <= K(0.0) && s <= K(0.0))) {
+                real old_w = c->water[idx], old_s = c->slurry[idx];
+                real new_w = old_w + w, new_s = old_s + s;
+                if (new_w > K(1.0e-6)) {
+                    c->uw[idx] = (old_w * c->uw[idx] * coarse_area + auw[idx]) / (new_w * coarse_area);
+                    c->vw[idx] = (old_w * c->vw[idx] * coarse_area + avw[idx]) / (new_w * coarse_area);
+                }
+                if (new_s > K(1.0e-6)) {
+                    c->us[idx] = (old_s * c->us[idx] * coarse_area + aus[idx]) / (new_s * coarse_area);
+                    c->vs[idx] = (old_s * c->vs[idx] * coarse_area + avs[idx]) / (new_s * coarse_area);
+                }
+                c->water[idx] = new_w;
+                c->slurry[idx] = new_s;
+            }
+        } else {
+            c->water[idx] = w;
+            c->slurry[idx] = s;
+            if (w > K(1.0e-6)) {
+                c->uw[idx] = auw[idx] / (w * coarse_area);
+                c->vw[idx] = avw[idx] / (w * coarse_area);
+            }
+            if (s > K(1.0e-6)) {
+                c->us[idx] = aus[idx] / (s * coarse_area);
+                c->vs[idx] = avs[idx] / (s * coarse_area);
+            }
+        }
+        aw[idx] = as[idx] = auw[idx] = avw[idx] = aus[idx] = avs[idx] = K(0.0);
+        cnt[idx] = 0;
+    }
+    memset(c->fw, 0, FN * sizeof(real));
+    memset(c->fs, 0, FN * sizeof(real));
+    c->fine_on = 0;
+}
+
+static void refine(Core *c, const Push *p, int np, real dt) {
+    restrict_fine(c);
+    if (np == 0) {
+        near_fade(c, dt);
+        return;
+    }
+    real cx = K(0.0), cz = K(0.0);
+    for (int k = 0; k < np; k++) {
+        cx += p[k].x;
+        cz += p[k].z;
+    }
+    cx /= (real)np;
+    cz /= (real)np;
+    real extent = (real)FRES * FINE;
+    c->fine_origin[0] = R_FLOOR((cx - extent * K(0.5)) / COARSE) * COARSE;
+    c->fine_origin[1] = R_FLOOR((cz - extent * K(0.5)) / COARSE) * COARSE;
+    memset(c->fw, 0, FN * sizeof(real));
+    memset(c->fs, 0, FN * sizeof(real));
+    memset(c->fuw, 0, FN * sizeof(real));
+    memset(c->fvw, 0, FN * sizeof(real));
+    memset(c->fus, 0, FN * sizeof(real));
+    memset(c->fvs, 0, FN * sizeof(real));
+    memset(c->hold, 0, WN);
+    const uint32_t full = 100;
+    real r2 = K(0.56) * K(0.56);
+    int col[FRES], row[FRES];
+    fine_axes(c, col, row);
+    static uint32_t cnt[WN];
+    static int touch[WN];
+    int ntouch = 0;
+    for (int j = 0; j < FRES; j++) {
+        if (row[j] == -1) continue;
+        for (int i = 0; i < FRES; i++) {
+            if (col[i] == -1) continue;
+            int cidx = row[j] * WRES + col[i];
+            if (cnt[cidx] == 0) touch[ntouch++] = cidx;
+            cnt[cidx]++;
+        }
+    }
+    for (int t = 0; t < ntouch; t++) {
+        int cidx = touch[t];
+        if (cnt[cidx] != full) continue;
+        real x, z;
+        coarse_xz(c, cidx % WRES, cidx / WRES, &x, &z);
+        if (near_tire_zone(p, np, x, z, r2)) c->hold[cidx] = 1;
+    }
+    for (int j = 0; j < FRES; j++) {
+        if (row[j] == -1) continue;
+        for (int i = 0; i < FRES; i++) {
+            if (col[i] == -1) continue;
+            int fidx = j * FRES + i, cidx = row[j] * WRES + col[i];
+            c->fmoist[fidx] = c->moisture[cidx];
+            c->fgrain[fidx] = c->grain[cidx];
+            if (c->hold[cidx] != 1) {
+                c->fbed[fidx] = c->bed[cidx] + c->water[cidx];
+                c->fbed_s[fidx] = c->bed[cidx] + c->slurry[cidx];
+                continue;
+            }
+            c->fbed[fidx] = c->bed[cidx];
+            c->fbed_s[fidx] = c->bed[cidx];
+            c->fw[fidx] = c->water[cidx];
+            c->fs[fidx] = c->slurry[cidx];
+            c->fuw[fidx] = c->uw[cidx];
+            c->fvw[fidx] = c->vw[cidx];
+            c->fus[fidx] = c->us[cidx];
+            c->fvs[fidx] = c->vs[cidx];
+        }
+    }
+    for (int t = 0; t < ntouch; t++) {
+        int cidx = touch[t];
+        if (c->hold[cidx] == 1)
+            c->water[cidx] = c->slurry[cidx] = c->uw[cidx] = c->vw[cidx] = c->us[cidx] = c->vs[cidx] = K(0.0);
+        cnt[cidx] = 0;
+    }
+    near_apply(c, dt);
+    c->fine_on = 1;
+}
+
+/* ---- tire shoves and spray -------------------------------------------------- */
+
+/* push_on (src/fluid_core.rs:8169-8239) */
+static void push_on(Core *c, const Push *p, int fine) {
+    int res = fine ? FRES : WRES;
+    real cell = fine ? FINE : COARSE;
+    const real *origin = fine ? c->fine_origin : c->origin;
+    real *water = fine ? c->fw : c->water, *slurry = fine ? c->fs : c->slurry;
+    real *uw = fine ? c->fuw : c->uw, *vw = fine ? c->fvw : c->vw;
+    real *us = fine ? c->fus : c->us, *vs = fine ? c->fvs : c->vs;
+    real len = R_FMAX(R_SQRT(p->fx * p->fx + p->fz * p->fz), K(1.0e-4));
+    real fx = p->fx / len, fz = p->fz / len;
+    real rx = fz, rz = -fx;
+    real speed = R_FMAX(p->speed, K(0.0));
+    real reach = R_HYPOT(p->radius * K(1.4), p->width * K(0.65) + cell) + cell;
+    int i0, i1, j0, j1;
+    index_span(origin[0], cell, res, p->x, reach, &i0, &i1);
+    index_span(origin[1], cell, res, p->z, reach, &j0, &j1);
+    for (int j = j0; j < j1; j++)
+        for (int i = i0; i < i1; i++) {
+            real x = origin[0] + ((real)i + K(0.5)) * cell;
+            real z = origin[1] + ((real)j + K(0.5)) * cell;
+            real dx = x - p->x, dz = z - p->z;
+            real along = dx * fx + dz * fz;
+            real lat = dx * rx + dz * rz;
+            if (R_FABS(lat) > p->width * K(0.65) + cell) continue;
+            if (along < -p->radius * K(1.4) || along > p->radius * K(1.1)) continue;
+            int idx = j * res + i;
+            real fl = R_FABS(lat) / (p->width * K(0.65) + cell);
+            real flank = fl < K(0.0) ? K(0.0) : (fl > K(1.0) ? K(1.0) : fl);
+            real shove = R_FMAX(speed, K(0.4)) * (K(1.0) - flank * flank);
+            real sg = R_COPYSIGN(K(1.0), lat);
+            if (water[idx] > K(0.004)) {
+                uw[idx] += fx * shove * K(0.85);
+                vw[idx] += fz * shove * K(0.85);
+                uw[idx] += rx * sg * shove * K(0.18) * flank;
+                vw[idx] += rz * sg * shove * K(0.18) * flank;
+            }
+            if (slurry[idx] > K(0.004)) {
+                real fore = along > K(0.0) ? K(1.0) : K(0.72);
+                us[idx] += fx * shove * fore;
+                vs[idx] += fz * shove * fore;
+                us[idx] += rx * sg * shove * K(0.28) * flank;
+                vs[idx] += rz * sg * shove * K(0.28) * flank;
+            }
+        }
+}
+
+/* coarse_held (src/fluid_core.rs:9664-9672) */
+static int coarse_held(const Core *c, real x, real z) {
+    real i = R_FLOOR((x - c->origin[0]) / COARSE), j = R_FLOOR((z - c->origin[1]) / COARSE);
+    if (i < K(0.0) || j < K(0.0)) return 0;
+    if (i >= (real)WRES || j >= (real)WRES) return 0;
+    return c->hold[(int)j * WRES + (int)i] == 1;
+}
+
+/* peel_spray (src/fluid_core.rs:8241-8314); returns the volume peeled. */
+static real peel_spray(Core *c, int fine, int skip_held, int *events) {
+    int res = fine ? FRES : WRES;
+    real cell = fine ? FINE : COARSE;
+    const real *origin = fine ? c->fine_origin : c->origin;
+    real *water = fine ? c->fw : c->water;
+    real area = cell * cell, total = K(0.0);
+    for (int q = 0; q < c->npush; q++) {
+        const Push *p = &c->pushes[q];
+        if (p->speed < K(0.8)) continue;
+        real reach = p->radius * K(1.3) + cell;
+        int i0, i1, j0, j1;
+        index_span(origin[0], cell, res, p->x, reach, &i0, &i1);
+        index_span(origin[1], cell, res, p->z, reach, &j0, &j1);
+        for (int j = j0; j < j1; j++)
+            for (int i = i0; i < i1; i++) {
+                real x = origin[0] + ((real)i + K(0.5)) * cell;
+                real z = origin[1] + ((real)j + K(0.5)) * cell;
+                if (skip_held && coarse_held(c, x, z)) continue;
+                real dx = x - p->x, dz = z - p->z;
+                if (dx * dx + dz * dz > (p->radius * K(1.3)) * (p->radius * K(1.3))) continue;
+                int idx = j * res + i;
+                if (water[idx] < K(0.01)) continue;
+                real take = R_FMIN(water[idx] * K(0.04), K(0.004));
+                if (take * area < K(1.0e-6)) continue;
+                water[idx] -= take;
+                total += take * area;
+                c->spray_m3 += take * area;
+                (*events)++;
+            }
+    }
+    return total;
+}
+
+/* soften_border (src/fluid_core.rs:9594-9620), no open reach. */
+static void soften_border(real *u, real *v, int res) {
+    int edge = res / 12 > 2 ? res / 12 : 2;
+    for (int j = 0; j < res; j++)
+        for (int i = 0; i < res; i++) {
+            int di = i < res - 1 - i ? i : res - 1 - i;
+            int dj = j < res - 1 - j ? j : res - 1 - j;
+            int d = di < dj ? di : dj;
+            if (d >= edge) continue;
+            real t = (real)d / (real)edge;
+            real w = t * t * (K(3.0) - K(2.0) * t);
+            u[j * res + i] *= w;
+            v[j * res + i] *= w;
+        }
+}
+
+/* wet_bounds (src/fluid_core.rs:794-819) then the slurry viscosity fill. */
+static void fill_nu_s(Core *c, const real *slurry, const real *moisture, const uint8_t *grain, int res) {
+    int i0 = res, i1 = 0, j0 = res, j1 = 0, any = 0;
+    for (int j = 0; j < res; j++)
+        for (int i = 0; i < res; i++)
+            if (slurry[j * res + i] > K(0.0)) {
+                any = 1;
+                if (i < i0) i0 = i;
+                if (i > i1) i1 = i;
+                if (j < j0) j0 = j;
+                if (j > j1) j1 = j;
+            }
+    if (!any) return;
+    i0 = i0 >= 8 ? i0 - 8 : 0;
+    j0 = j0 >= 8 ? j0 - 8 : 0;
+    i1 = i1 + 8 < res - 1 ? i1 + 8 : res - 1;
+    j1 = j1 + 8 < res - 1 ? j1 + 8 : res - 1;
+    for (int j = j0; j <= j1; j++)
+        for (int i = i0; i <= i1; i++)
+            c->nu_s[j * res + i] = F(oracle_dynamic_viscosity)(grain[j * res + i], moisture[j * res + i]);
+}
+
+/* step_pair (src/fluid_core.rs:7942-8150), no solids, lifts or reach. */
+static void step_pair(Core *c, real dt, int fine) {
+    if (fine && !c->fine_on) return;
+    int res = fine ? FRES : WRES;
+    real cell = fine ? FINE : COARSE;
+    real cap = fine ? c->fine_sub_cap : K(8.0);
+    int pace = fine ? c->fine_pace : 0;
+    int confine = fine ? 0 : c->coarse_confine;
+    real *bed = fine ? c->fbed : c->bed;
+    real *bed_s = fine ? c->fbed_s : c->bed;
+    if (!fine) soften_border(c->uw, c->vw, res);
+    F(oracle_step_sheet_ex)(fine ? c->fw : c->water, fine ? c->fuw : c->uw, fine ? c->fvw : c->vw, bed, c->nu_w,
+                            c->gate, res, cell, dt, 0, K(0.0), K(0.0), K(0.08), K(1.0e-6), 0, cap, pace, 0,
+                            confine, NULL, NULL);
+    fill_nu_s(c, fine ? c->fs : c->slurry, fine ? c->fmoist : c->moisture, fine ? c->fgrain : c->grain, res);
+    if (!fine) soften_border(c->us, c->vs, res);
+    F(oracle_step_sheet_ex)(fine ? c->fs : c->slurry, fine ? c->fus : c->us, fine ? c->fvs : c->vs, bed_s, c->nu_s,
+                            c->gate, res, cell, dt, 0, K(0.0), K(0.0), K(0.05), K(1.0e-6), 0, cap, pace, 0,
+                            confine, NULL, NULL);
+}
+
+/* ---- public oracle API ----------------------------------------------------- */
+
+/* Test hooks: the two halves of the patch hand-off on their own
+ * (near.rs:384-410 calls refine and restrict_fine directly). */
+void F(oracle_core_refine)(Core *c, real dt) { refine(c, c->pushes, c->npush, dt); }
+void F(oracle_core_restrict)(Core *c) { restrict_fine(c); }
+real F(oracle_floor_scale)(real base, const real *r) { return floor_scale(base, r); }
+void F(oracle_recentre)(real *r) { recentre(r); }
+int F(oracle_core_fine_on)(Core *c) { return c->fine_on; }
+void F(oracle_core_fine_get)(Core *c, real *fw, real *origin2) {
+    memcpy(fw, c->fw, FN * sizeof(real));
+    origin2[0] = c->fine_origin[0];
+    origin2[1] = c->fine_origin[1];
+}
+
+void F(oracle_core_push_tire)(Core *c, const real *p) {
+    if (c->npush >= MAX_PUSH) return;
+    Push q = {p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7]};
+    c->pushes[c->npush++] = q;
+}
+
+/* FluidCore::step on the sheet path.  Returns the spray events; *spray the
+ * volume peeled this step. */
+int F(oracle_core_step)(Core *c, real dt, real *spray) {
+    Push pushes[MAX_PUSH];
+    int np = c->npush;
+    memcpy(pushes, c->pushes, sizeof(Push) * (size_t)np);
+    int events = 0;
+    real vol = K(0.0);
+    refine(c, pushes, np, dt);
+    for (int k = 0; k < np; k++)
+        if (cell_is_refined(c, pushes[k].x, pushes[k].z)) push_on(c, &pushes[k], 1);
+    step_pair(c, dt, 1);
+    if (c->fine_on) vol += peel_spray(c, 1, 0, &events);
+    restrict_fine(c);
+    for (int k = 0; k < np; k++)
+        if (!fine_held_at(c, pushes[k].x, pushes[k].z)) push_on(c, &pushes[k], 0);
+    step_pair(c, dt, 0);
+    vol += peel_spray(c, 0, 1, &events);
+    c->npush = 0;
+    if (spray) *spray = vol;
+    return events;
+}
+
+/* map_water_bed with arrays sampled at cell centres (src/fluid_core.rs:5329-5355). */
+void F(oracle_core_map_water_bed)(Core *c, const real *bed, const real *water) {
+    restrict_fine(c);
+    for (int k = 0; k < WN; k++) {
+        c->bed[k] = isfinite(bed[k]) ? bed[k] : K(0.0);
+        real w = isfinite(water[k]) ? R_FMAX(water[k], K(0.0)) : K(0.0);
+        c->water[k] = w;
+        if (w > K(0.0)) {
+            c->moisture[k] = R_FMAX(c->moisture[k], K(0.8));
+            if (c->grain[k] == 4) c->grain[k] = 1;
+        }
+    }
+}
+
+/* paint (src/fluid_core.rs:5820-5855): pour_water / pour_slurry. */
+void F(oracle_core_paint)(Core *c, real x0, real x1, real z0, real z1, real depth, int water, int grain,
+                          real moisture) {
+    if (!(depth > K(0.0))) depth = K(0.0);
+    for (int j = 0; j < WRES; j++)
+        for (int i = 0; i < WRES; i++) {
+            real x, z;
+            coarse_xz(c, i, j, &x, &z);
+            if (x < x0 || x >= x1 || z < z0 || z >= z1) continue;
+            int idx = j * WRES + i;
+            if (water) {
+                c->water[idx] = depth;
+            } else {
+                c->slurry[idx] = depth;
+                c->grain[idx] = (uint8_t)grain;
+                c->moisture[idx] = moisture < K(0.0) ? K(0.0) : (moisture > K(1.0) ? K(1.0) : moisture);
+            }
+        }
+}
+
+/* lay_slurry_mask with a depth array (src/fluid_core.rs:5857-5880). */
+void F(oracle_core_lay_slurry)(Core *c, const real *depth, int grain, real moisture) {
+    restrict_fine(c);
+    real m = moisture < K(0.0) ? K(0.0) : (moisture > K(1.0) ? K(1.0) : moisture);
+    for (int k = 0; k < WN; k++) {
+        real d = depth[k];
+        if (!(isfinite(d) && d > K(0.0))) continue;
+        c->slurry[k] += d;
+        c->grain[k] = (uint8_t)grain;
+        c->moisture[k] = R_FMAX(c->moisture[k], m);
+    }
+}
+
+/* read_substrate_grid with arrays (src/fluid_core.rs:5928-5962), ambient
+ * moisture histogram aside (it only feeds the dust). */
+void F(oracle_core_read_substrate)(Core *c, const real *bed, const real *moisture, const uint8_t *grain) {
+    restrict_fine(c);
+    for (int k = 0; k < WN; k++) {
+        real m = moisture[k];
+        m = m < K(0.0) ? K(0.0) : (m > K(1.0) ? K(1.0) : m);
+        c->bed[k] = bed[k];
+        c->moisture[k] = m;
+        if (c->slurry[k] > K(1.0e-5)) c->grain[k] = grain[k];
+    }
+}
+
+/* Copies the coarse planes out (water, slurry, uw, vw, us, vs, bed, moisture). */
+void F(oracle_core_get)(Core *c, real *out8) {
+    real *planes[8] = {c->water, c->slurry, c->uw, c->vw, c->us, c->vs, c->bed, c->moisture};
+    for (int p = 0; p < 8; p++) memcpy(out8 + (size_t)p * WN, planes[p], WN * sizeof(real));
+}
+
+void F(oracle_core_set)(Core *c, const real *in8) {
+    real *planes[8] = {c->water, c->slurry, c->uw, c->vw, c->us, c->vs, c->bed, c->moisture};
+    for (int p = 0; p < 8; p++) memcpy(planes[p], in8 + (size_t)p * WN, WN * sizeof(real));
+}
+
+void F(oracle_core_origin)(Core *c, real *out2) {
+    out2[0] = c->origin[0];
+    out2[1] = c->origin[1];
+}
+
+/* ledger() for the sheet path (src/fluid_core.rs:6821-6883): out[0] water,
+ * [1] slurry, [2] frozen water, [3] frozen slurry, [4] spray, [5] total. */
+void F(oracle_core_ledger)(Core *c, real *out) {
+    real area = COARSE * COARSE;
+    real w = -K(0.0), s = -K(0.0);
+    for (int k = 0; k < WN; k++) {
+        w += R_FMAX(c->water[k], K(0.0));
+        s += R_FMAX(c->slurry[k], K(0.0));
+    }
+    w *= area;
+    s *= area;
+    if (c->fine_on) {
+        real fa = FINE * FINE, fw = -K(0.0), fs = -K(0.0);
+        for (int k = 0; k < FN; k++) {
+            fw += R_FMAX(c->fw[k], K(0.0));
+            fs += R_FMAX(c->fs[k], K(0.0));
+        }
+        w += fw * fa;
+        s += fs * fa;
+    }
+    out[0] = w;
+    out[1] = s;
+    out[2] = c->frozen_w;
+    out[3] = c->frozen_s;
+    out[4] = c->spray_m3;
+    out[5] = w + s + c->frozen_w + c->frozen_s + c->spray_m3;
+}
+
+/* The render-contract film and flow maps from coarse_cell
+ * (src/fluid_core.rs:6998-7034; src/render_contract.rs:468-496):
+ * film2 = (water, slurry), flow4 = (u, v, speed, turbidity 0). */
+void F(oracle_core_publish)(Core *c, real *film2, real *flow4) {
+    for (int k = 0; k < WN; k++) {
+        real water = isfinite(c->water[k]) ? c->water[k] : K(0.0);
+        real slurry = isfinite(c->slurry[k]) ? c->slurry[k] : K(0.0);
+        real film = water + slurry;
+        real u = K(0.0), v = K(0.0);
+        if (!(film < K(1.0e-5))) {
+            u = (c->uw[k] * water + c->us[k] * slurry) / film;
+            v = (c->vw[k] * water + c->vs[k] * slurry) / film;
+            if (!isfinite(u)) u = K(0.0);
+            if (!isfinite(v)) v = K(0.0);
+        }
+        int dry = !isfinite(water) || !isfinite(slurry) || water + slurry < K(1.0e-5);
+        real speed = dry ? K(0.0) : R_SQRT(u * u + v * v);
+        if (!isfinite(speed)) speed = K(0.0);
+        film2[2 * k] = water;
+        film2[2 * k + 1] = slurry;
+        flow4[4 * k] = dry ? K(0.0) : u;
+        flow4[4 * k + 1] = dry ? K(0.0) : v;
+        flow4[4 * k + 2] = speed;
+        flow4[4 * k + 3] = K(0.0);
+    }
+}

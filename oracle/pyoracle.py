@@ -174,3 +174,188 @@ def sheet_damp(nu, precision=32):
 
 def column_sum(h, cell, precision=32):
     return getattr(lib(), "oracle_column_sum" + _sfx(precision))(_ptr(h), h.size, cell)
+
+
+# --- FluidCore restatement (oracle/core.c) ------------------------------------
+
+WINDOW_RES = 200
+FINE_RES = 160
+
+
+def _declare_core(L):
+    for sfx, R in (("_f32", C.c_float), ("_f64", C.c_double)):
+        g = lambda n: getattr(L, n + sfx)  # noqa: E731
+        g("oracle_core_new").argtypes = [R, R]
+        g("oracle_core_new").restype = _P
+        g("oracle_core_free").argtypes = [_P]
+        for n, at in (("oracle_core_set_fine_pace", [_P, C.c_int]), ("oracle_core_set_fine_sub_cap", [_P, R]),
+                      ("oracle_core_set_coarse_confine", [_P, C.c_int]), ("oracle_core_set_near_field", [_P, C.c_int]),
+                      ("oracle_core_push_tire", [_P, _P]), ("oracle_core_map_water_bed", [_P, _P, _P]),
+                      ("oracle_core_paint", [_P, R, R, R, R, R, C.c_int, C.c_int, R]),
+                      ("oracle_core_lay_slurry", [_P, _P, C.c_int, R]),
+                      ("oracle_core_read_substrate", [_P, _P, _P, _P]), ("oracle_core_get", [_P, _P]),
+                      ("oracle_core_set", [_P, _P]), ("oracle_core_origin", [_P, _P]),
+                      ("oracle_core_ledger", [_P, _P]), ("oracle_core_publish", [_P, _P, _P]),
+                      ("oracle_core_refine", [_P, R]), ("oracle_core_restrict", [_P]),
+                      ("oracle_core_fine_get", [_P, _P, _P]), ("oracle_recentre", [_P])):
+            g(n).argtypes = at
+            g(n).restype = None
+        g("oracle_core_step").argtypes = [_P, R, _P]
+        g("oracle_core_step").restype = C.c_int
+        g("oracle_core_near_tiles").argtypes = [_P]
+        g("oracle_core_near_tiles").restype = _Z
+        g("oracle_core_fine_on").argtypes = [_P]
+        g("oracle_core_fine_on").restype = C.c_int
+        g("oracle_floor_scale").argtypes = [R, _P]
+        g("oracle_floor_scale").restype = R
+
+
+class Core:
+    """FluidCore on the sheet path (oracle/core.c).  Closures of the reference
+    API are arrays sampled at coarse cell centres."""
+
+    def __init__(self, center=(0.0, 0.0), precision=32):
+        self.precision = precision
+        self.dtype = _dt(precision)
+        self.sfx = _sfx(precision)
+        L = lib()
+        if not getattr(L, "_core_declared", False):
+            _declare_core(L)
+            L._core_declared = True
+        self._c = self._f("oracle_core_new")(center[0], center[1])
+
+    def _f(self, name):
+        return getattr(lib(), name + self.sfx)
+
+    def __del__(self):
+        try:
+            self._f("oracle_core_free")(self._c)
+        except Exception:
+            pass
+
+    def _arr(self, a):
+        a = np.ascontiguousarray(a, self.dtype)
+        return a
+
+    def origin(self):
+        o = np.zeros(2, self.dtype)
+        self._f("oracle_core_origin")(self._c, _ptr(o))
+        return o
+
+    def cell_centres(self):
+        o = self.origin().astype(np.float64)
+        c = (np.arange(WINDOW_RES) + 0.5) * 0.2
+        return np.meshgrid((o[0] + c).astype(self.dtype), (o[1] + c).astype(self.dtype))
+
+    def set_fine_pace(self, pace):
+        self._f("oracle_core_set_fine_pace")(self._c, pace)
+
+    def set_fine_sub_cap(self, cap):
+        self._f("oracle_core_set_fine_sub_cap")(self._c, cap)
+
+    def set_coarse_confine(self, mode):
+        self._f("oracle_core_set_coarse_confine")(self._c, CONFINE[mode])
+
+    def set_near_field(self, on):
+        self._f("oracle_core_set_near_field")(self._c, int(on))
+
+    def pour_water(self, x0, x1, z0, z1, depth):
+        self._f("oracle_core_paint")(self._c, x0, x1, z0, z1, depth, 1, 3, 1.0)
+
+    def pour_slurry(self, x0, x1, z0, z1, depth, grain, moisture):
+        g = GRAIN[grain] if isinstance(grain, str) else int(grain)
+        self._f("oracle_core_paint")(self._c, x0, x1, z0, z1, depth, 0, g, moisture)
+
+    def map_water_bed(self, bed, water):
+        b, w = self._arr(bed), self._arr(water)
+        self._f("oracle_core_map_water_bed")(self._c, _ptr(b), _ptr(w))
+
+    def lay_slurry(self, depth, grain, moisture):
+        d = self._arr(depth)
+        g = GRAIN[grain] if isinstance(grain, str) else int(grain)
+        self._f("oracle_core_lay_slurry")(self._c, _ptr(d), g, moisture)
+
+    def read_substrate(self, bed, moisture, grain):
+        b, m = self._arr(bed), self._arr(moisture)
+        g = np.ascontiguousarray(grain, np.uint8)
+        self._f("oracle_core_read_substrate")(self._c, _ptr(b), _ptr(m), _ptr(g))
+
+    def push_tire(self, x, z, forward_x, forward_z, radius, width, speed, slip):
+        p = np.array([x, z, forward_x, forward_z, radius, width, speed, slip], self.dtype)
+        self._f("oracle_core_push_tire")(self._c, _ptr(p))
+
+    def step(self, dt):
+        vol = np.zeros(1, self.dtype)
+        ev = self._f("oracle_core_step")(self._c, dt, _ptr(vol))
+        return ev, float(vol[0])
+
+    def planes(self):
+        out = np.zeros(8 * WINDOW_RES * WINDOW_RES, self.dtype)
+        self._f("oracle_core_get")(self._c, _ptr(out))
+        n = WINDOW_RES * WINDOW_RES
+        names = ("water", "slurry", "uw", "vw", "us", "vs", "bed", "moisture")
+        return {k: out[i * n:(i + 1) * n].copy() for i, k in enumerate(names)}
+
+    def set_planes(self, planes):
+        n = WINDOW_RES * WINDOW_RES
+        buf = np.zeros(8 * n, self.dtype)
+        for i, k in enumerate(("water", "slurry", "uw", "vw", "us", "vs", "bed", "moisture")):
+            buf[i * n:(i + 1) * n] = planes[k]
+        self._f("oracle_core_set")(self._c, _ptr(buf))
+
+    def ledger(self):
+        out = np.zeros(6, self.dtype)
+        self._f("oracle_core_ledger")(self._c, _ptr(out))
+        keys = ("water_m3", "slurry_m3", "frozen_water_m3", "frozen_slurry_m3", "spray_m3", "total_m3")
+        return {k: float(v) for k, v in zip(keys, out)}
+
+    def publish(self):
+        n = WINDOW_RES * WINDOW_RES
+        film = np.zeros(2 * n, self.dtype)
+        flow = np.zeros(4 * n, self.dtype)
+        self._f("oracle_core_publish")(self._c, _ptr(film), _ptr(flow))
+        return film, flow
+
+    def near_tiles(self):
+        return int(self._f("oracle_core_near_tiles")(self._c))
+
+    def refine(self, dt):
+        self._f("oracle_core_refine")(self._c, dt)
+
+    def restrict(self):
+        self._f("oracle_core_restrict")(self._c)
+
+    def fine_water(self):
+        fw = np.zeros(FINE_RES * FINE_RES, self.dtype)
+        o = np.zeros(2, self.dtype)
+        self._f("oracle_core_fine_get")(self._c, _ptr(fw), _ptr(o))
+        return fw, o
+
+    def sample_water(self, x, z):
+        """Coarse water depth under a world point (sample_coarse when the patch
+        is packed, src/fluid_core.rs:6806-6813)."""
+        o = self.origin().astype(np.float64)
+        i = int(np.floor((np.float32(x) - o[0]) / 0.2))
+        j = int(np.floor((np.float32(z) - o[1]) / 0.2))
+        if not (0 <= i < WINDOW_RES and 0 <= j < WINDOW_RES):
+            return 0.0
+        return float(self.planes()["water"][j * WINDOW_RES + i])
+
+
+def floor_scale(base, r, precision=32):
+    L = lib()
+    if not getattr(L, "_core_declared", False):
+        _declare_core(L)
+        L._core_declared = True
+    a = np.ascontiguousarray(r, _dt(precision))
+    return getattr(L, "oracle_floor_scale" + _sfx(precision))(base, _ptr(a))
+
+
+def recentre(r, precision=32):
+    L = lib()
+    if not getattr(L, "_core_declared", False):
+        _declare_core(L)
+        L._core_declared = True
+    a = np.ascontiguousarray(r, _dt(precision)).copy()
+    getattr(L, "oracle_recentre" + _sfx(precision))(_ptr(a))
+    return a
