@@ -84,6 +84,25 @@ int run_plan(const SheetPlanes& s, const StepPlan& plan, int res, float cell, fl
   return plan.nsub;
 }
 
+// A step of one layer whose planes are already on the device: the plan reads
+// depth, velocity and bed back to the host (blocking), then the kernels are
+// enqueued on st.  The workspace scratch planes in s must be valid.
+int step_device_planes(const SheetPlanes& s, int res, float cell, float dt, const silt_drive& d,
+                       const silt_sheet_opts& o, bool gate_uniform, bool nu_uniform, float nu0, hipStream_t st) {
+  size_t n = (size_t)res * res;
+  std::vector<float> hh(n), uh(n), vh(n), bh(n);
+  size_t fb = n * sizeof(float);
+  HIPCHK(hipMemcpyAsync(hh.data(), s.h, fb, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(uh.data(), s.u, fb, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(vh.data(), s.v, fb, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(bh.data(), s.bed, fb, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  StepPlan plan = plan_step(hh.data(), uh.data(), vh.data(), bh.data(), res, cell, dt, d.kind == SILT_DRIVE_WIND, d.x,
+                            d.z, o.eps, o.sub_cap, o.pace);
+  if (plan.boxes.empty()) return 0;
+  return run_plan(s, plan, res, cell, dt, d, o, gate_uniform, nu_uniform, nu0, st);
+}
+
 bool valid_opts(const silt_sheet_opts& o) {
   return (o.pace >= 0 && o.pace <= 255) && (o.confine == SILT_CONFINE_REFERENCE || o.confine == SILT_CONFINE_JACOBI) &&
          (o.path == SILT_PATH_EXACT || o.path == SILT_PATH_FAST || o.path == SILT_PATH_FUSED_EXACT);
@@ -167,22 +186,9 @@ int silt_step_sheet_device(float* h, float* u, float* v, const float* bed, const
     silt_sheet_opts_default(&o);
   if (!valid_opts(o)) return SILT_EINVAL;
   if (!(dt > 0.0f) || res == 0) return 0;
-  size_t n = res * res;
-  hipStream_t st = (hipStream_t)hip_stream;
   Workspace* ws = workspace_for((int)res);
   if (!ws) return SILT_ENOMEM;
   std::lock_guard<std::mutex> lock(ws->mu);
-  // The plan needs the depth, the velocity and the bed on the host.
-  std::vector<float> hh(n), uh(n), vh(n), bh(n);
-  size_t fb = n * sizeof(float);
-  HIPCHK(hipMemcpyAsync(hh.data(), h, fb, hipMemcpyDeviceToHost, st));
-  HIPCHK(hipMemcpyAsync(uh.data(), u, fb, hipMemcpyDeviceToHost, st));
-  HIPCHK(hipMemcpyAsync(vh.data(), v, fb, hipMemcpyDeviceToHost, st));
-  HIPCHK(hipMemcpyAsync(bh.data(), bed, fb, hipMemcpyDeviceToHost, st));
-  if (hipStreamSynchronize(st) != hipSuccess) return SILT_EDEVICE;
-  StepPlan plan = plan_step(hh.data(), uh.data(), vh.data(), bh.data(), (int)res, cell, dt,
-                            drive.kind == SILT_DRIVE_WIND, drive.x, drive.z, o.eps, o.sub_cap, o.pace);
-  if (plan.boxes.empty()) return 0;
   SheetPlanes s = ws->planes();
   s.h = h;
   s.u = u;
@@ -190,8 +196,10 @@ int silt_step_sheet_device(float* h, float* u, float* v, const float* bed, const
   s.bed = bed;
   s.nu = uniform ? s.nu : nu;
   s.gate = gate ? gate : s.gate;
-  int nsub = run_plan(s, plan, (int)res, cell, dt, drive, o, gate == nullptr, uniform, uniform ? nu_uniform : 0.0f, st);
-  if (hipStreamSynchronize(st) != hipSuccess) return SILT_EDEVICE;
+  int nsub = step_device_planes(s, (int)res, cell, dt, drive, o, gate == nullptr, uniform, uniform ? nu_uniform : 0.0f,
+                                (hipStream_t)hip_stream);
+  if (nsub < 0) return nsub;
+  if (hipStreamSynchronize((hipStream_t)hip_stream) != hipSuccess) return SILT_EDEVICE;
   if (hipGetLastError() != hipSuccess) return SILT_EDEVICE;
   return nsub;
 }

@@ -1,0 +1,99 @@
+// Device structures of the two-layer FluidCore (api_core.cpp, core_kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "sheet_device.hpp"
+
+namespace silt {
+
+constexpr int kWindowRes = 200;     // WINDOW_RES, src/fluid_core.rs:72
+constexpr int kFineRes = 160;       // FINE_RES, :76
+constexpr float kCoarseCell = 0.20f;  // COARSE_CELL_M, :70
+constexpr int kMaxPush = 64;
+constexpr unsigned long long kEmptyKey = 0xFFFFFFFFFFFFFFFFull;
+constexpr unsigned long long kTombKey = 0xFFFFFFFFFFFFFFFEull;
+
+struct GridGeo {
+  int res;
+  float cell;
+  float ox, oz;  // world origin of cell (0, 0)'s corner
+};
+
+// TirePush (src/fluid_core.rs:357-368).
+struct CorePush {
+  float x, z, fx, fz, radius, width, speed, slip;
+};
+
+struct PushList {
+  int n;
+  CorePush p[kMaxPush];
+};
+
+// SprayEvent (src/fluid_core.rs:371-382).
+struct SprayEvent {
+  float x, y, z, vx, vy, vz, volume;
+  int kind;
+};
+
+struct CorePlanes {
+  float *water, *slurry, *uw, *vw, *us, *vs, *bed, *moisture;
+  uint8_t* grain;
+};
+
+struct FinePlanes {
+  float *water, *slurry, *uw, *vw, *us, *vs, *bed, *bed_s, *moisture;
+  uint8_t* grain;
+};
+
+// Coarse column / row of every fine column / row, -1 off the window
+// (fine_coarse_axes, src/fluid_core.rs:9518-9534).
+struct FineAxes {
+  short col[kFineRes];
+  short row[kFineRes];
+};
+
+// The coarse cells the patch covers and, for each coarse column / row, its
+// first fine column / row and how many there are.
+struct CoarseSpans {
+  int ci0, cj0, ncol, nrow;
+  short col_start[24], col_count[24], row_start[24], row_count[24];
+};
+
+struct NearColumn {
+  int cidx, kx, kz, i0, j0, inner;
+};
+
+// Near-field residual tiles in an open-addressing table on the device.
+struct NearTable {
+  unsigned long long* keys;  // (kx, kz) packed, kEmptyKey / kTombKey
+  float* data;               // 400 floats per slot: w, s, uw, vw residuals
+  float* peak;
+  float* idle;
+  int* fresh;
+  unsigned* count;
+  unsigned cap;
+};
+
+void core_push_on(const GridGeo& g, float* water, float* slurry, float* uw, float* vw, float* us, float* vs,
+                  const PushList& pl, int i0, int j0, int w, int h, hipStream_t st);
+void core_peel(const GridGeo& g, float* water, const float* uw, const float* vw, const PushList& pl, int i0, int j0,
+               int w, int h, const uint8_t* hold, const GridGeo& cg, bool skip_held, SprayEvent* ev, unsigned* nev,
+               unsigned cap, double* vol, hipStream_t st);
+void core_soften(float* u, float* v, int res, hipStream_t st);
+void core_nu(float* nu, const uint8_t* grain, const float* moisture, int n, hipStream_t st);
+void core_fill(float* p, float val, int n, hipStream_t st);
+void core_refine_copy(const CorePlanes& c, const FinePlanes& f, const uint8_t* hold, const FineAxes& ax, int ci0,
+                      int cj0, int cw, int ch, hipStream_t st);
+void core_restrict(const CorePlanes& c, const FinePlanes& f, const uint8_t* hold, const CoarseSpans& sp,
+                   const FineAxes& ax, double* frozen, hipStream_t st);
+void core_near_capture(const FinePlanes& f, const NearTable& t, const NearColumn* cols, int ncols, hipStream_t st);
+void core_near_apply(const FinePlanes& f, const NearTable& t, const NearColumn* cols, int ncols, hipStream_t st);
+void core_near_fade(const NearTable& t, float dt, hipStream_t st);
+void core_near_clear(const NearTable& t, hipStream_t st);
+void core_near_rehash(const NearTable& from, const NearTable& to, hipStream_t st);
+void core_publish(const CorePlanes& c, const float* soil_bed, const float* soil_placed, const float* soil_rut,
+                  float* ground_rut4, float* film2, float* flow4, hipStream_t st);
+
+}  // namespace silt

@@ -1,0 +1,588 @@
+// Kernels of the two-layer FluidCore step on the sheet path (the 40 m coarse
+// window and the 2 cm patch under the tires): tire shoves, spray peel, the
+// soft window edge, the slurry viscosity field, the coarse -> fine copy and
+// the fine -> coarse sum, the near-field residual tiles, and the publish maps.
+// Each kernel restates one reference function (cited at the kernel) with the
+// same per-cell operation order, so a cell's arithmetic matches the
+// reference bit for bit; only the order of independent cells changes.
+#include "core.hpp"
+
+namespace silt {
+
+namespace {
+
+__device__ inline int span_lo(float origin, float cell, int res, float center, float reach) {
+  float lo = fmaxf(floorf((center - reach - origin) / cell - 1.0f), 0.0f);
+  int l = (int)lo;
+  return l < res ? l : res;
+}
+
+__device__ inline int span_hi(float origin, float cell, int res, float center, float reach, int lo) {
+  float hi = fmaxf(ceilf((center + reach - origin) / cell + 2.0f), 0.0f);
+  int h = (int)hi;
+  h = h < res ? h : res;
+  return h > lo ? h : lo;
+}
+
+// push_on (src/fluid_core.rs:8169-8239): every push, in order, on one cell.
+__global__ void k_push_on(GridGeo g, float* water, float* slurry, float* uw, float* vw, float* us, float* vs,
+                          PushList pl, int i0, int j0, int w, int h) {
+  int i = i0 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  int j = j0 + (int)(blockIdx.y * blockDim.y + threadIdx.y);
+  if (i >= i0 + w || j >= j0 + h) return;
+  size_t idx = (size_t)j * g.res + i;
+  const float cell = g.cell;
+  for (int q = 0; q < pl.n; q++) {
+    const CorePush& p = pl.p[q];
+    float len = fmaxf(sqrtf(p.fx * p.fx + p.fz * p.fz), 1.0e-4f);
+    float fx = p.fx / len, fz = p.fz / len;
+    float rx = fz, rz = -fx;
+    float speed = fmaxf(p.speed, 0.0f);
+    float reach = hypotf(p.radius * 1.4f, p.width * 0.65f + cell) + cell;
+    int a0 = span_lo(g.ox, cell, g.res, p.x, reach), a1 = span_hi(g.ox, cell, g.res, p.x, reach, a0);
+    int b0 = span_lo(g.oz, cell, g.res, p.z, reach), b1 = span_hi(g.oz, cell, g.res, p.z, reach, b0);
+    if (i < a0 || i >= a1 || j < b0 || j >= b1) continue;
+    float x = g.ox + ((float)i + 0.5f) * cell;
+    float z = g.oz + ((float)j + 0.5f) * cell;
+    float dx = x - p.x, dz = z - p.z;
+    float along = dx * fx + dz * fz;
+    float lat = dx * rx + dz * rz;
+    if (fabsf(lat) > p.width * 0.65f + cell) continue;
+    if (along < -p.radius * 1.4f || along > p.radius * 1.1f) continue;
+    float flank = clampf(fabsf(lat) / (p.width * 0.65f + cell), 0.0f, 1.0f);
+    float shove = fmaxf(speed, 0.4f) * (1.0f - flank * flank);
+    float sg = copysignf(1.0f, lat);
+    if (water[idx] > 0.004f) {
+      uw[idx] = uw[idx] + fx * shove * 0.85f;
+      vw[idx] = vw[idx] + fz * shove * 0.85f;
+      uw[idx] = uw[idx] + rx * sg * shove * 0.18f * flank;
+      vw[idx] = vw[idx] + rz * sg * shove * 0.18f * flank;
+    }
+    if (slurry[idx] > 0.004f) {
+      float fore = along > 0.0f ? 1.0f : 0.72f;
+      us[idx] = us[idx] + fx * shove * fore;
+      vs[idx] = vs[idx] + fz * shove * fore;
+      us[idx] = us[idx] + rx * sg * shove * 0.28f * flank;
+      vs[idx] = vs[idx] + rz * sg * shove * 0.28f * flank;
+    }
+  }
+}
+
+// peel_spray (src/fluid_core.rs:8241-8314) on one cell, every push in order.
+// Events go to a buffer in arbitrary order; the peeled volume is summed in
+// FP64 (the reference sums it in f32 in event order).
+__global__ void k_peel(GridGeo g, float* water, const float* uw, const float* vw, PushList pl, int i0, int j0, int w,
+                       int h, const uint8_t* hold, GridGeo cg, int skip_held, SprayEvent* ev, unsigned* nev,
+                       unsigned cap, double* vol) {
+  int i = i0 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  int j = j0 + (int)(blockIdx.y * blockDim.y + threadIdx.y);
+  if (i >= i0 + w || j >= j0 + h) return;
+  size_t idx = (size_t)j * g.res + i;
+  const float cell = g.cell, area = cell * cell;
+  float x = g.ox + ((float)i + 0.5f) * cell;
+  float z = g.oz + ((float)j + 0.5f) * cell;
+  if (skip_held) {
+    // coarse_held (src/fluid_core.rs:9664-9672)
+    float ci = floorf((x - cg.ox) / 0.2f), cj = floorf((z - cg.oz) / 0.2f);
+    if (ci >= 0.0f && cj >= 0.0f && ci < (float)cg.res && cj < (float)cg.res &&
+        hold[(size_t)cj * cg.res + (size_t)ci] == 1)
+      return;
+  }
+  for (int q = 0; q < pl.n; q++) {
+    const CorePush& p = pl.p[q];
+    if (p.speed < 0.8f) continue;
+    float reach = p.radius * 1.3f + cell;
+    int a0 = span_lo(g.ox, cell, g.res, p.x, reach), a1 = span_hi(g.ox, cell, g.res, p.x, reach, a0);
+    int b0 = span_lo(g.oz, cell, g.res, p.z, reach), b1 = span_hi(g.oz, cell, g.res, p.z, reach, b0);
+    if (i < a0 || i >= a1 || j < b0 || j >= b1) continue;
+    float dx = x - p.x, dz = z - p.z;
+    if (dx * dx + dz * dz > (p.radius * 1.3f) * (p.radius * 1.3f)) continue;
+    float wv = water[idx];
+    if (wv < 0.01f) continue;
+    float take = fminf(wv * 0.04f, 0.004f);
+    if (take * area < 1.0e-6f) continue;
+    water[idx] = wv - take;
+    atomicAdd(vol, (double)(take * area));
+    unsigned k = atomicAdd(nev, 1u);
+    if (k < cap) {
+      SprayEvent e;
+      e.x = x;
+      e.y = 0.35f;
+      e.z = z;
+      e.vx = uw[idx] * 0.3f;
+      e.vy = 1.4f + p.speed * 0.25f;
+      e.vz = vw[idx] * 0.3f;
+      e.volume = take * area;
+      e.kind = 0;
+      ev[k] = e;
+    }
+  }
+}
+
+// soften_border (src/fluid_core.rs:9594-9620), no open reach.
+__global__ void k_soften(float* u, float* v, int res) {
+  int i = (int)(blockIdx.x * blockDim.x + threadIdx.x), j = (int)(blockIdx.y * blockDim.y + threadIdx.y);
+  if (i >= res || j >= res) return;
+  int edge = res / 12 > 2 ? res / 12 : 2;
+  int di = min(i, res - 1 - i), dj = min(j, res - 1 - j);
+  int d = min(di, dj);
+  if (d >= edge) return;
+  float t = (float)d / (float)edge;
+  float w = t * t * (3.0f - 2.0f * t);
+  size_t idx = (size_t)j * res + i;
+  u[idx] = u[idx] * w;
+  v[idx] = v[idx] * w;
+}
+
+// The slurry viscosity plane from grain and moisture (step_pair :8090-8098).
+__global__ void k_nu(float* nu, const uint8_t* grain, const float* moisture, int n) {
+  int k = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (k >= n) return;
+  nu[k] = dynamic_viscosity(grain[k], moisture[k]);
+}
+
+// Coarse -> fine copy of refine (src/fluid_core.rs:8418-8444), one fine cell.
+__global__ void k_refine(CorePlanes c, FinePlanes f, const uint8_t* hold, FineAxes ax) {
+  int i = (int)(blockIdx.x * blockDim.x + threadIdx.x), j = (int)(blockIdx.y * blockDim.y + threadIdx.y);
+  if (i >= kFineRes || j >= kFineRes) return;
+  int ci = ax.col[i], cj = ax.row[j];
+  if (ci < 0 || cj < 0) return;
+  size_t fidx = (size_t)j * kFineRes + i, cidx = (size_t)cj * kWindowRes + ci;
+  f.moisture[fidx] = c.moisture[cidx];
+  f.grain[fidx] = c.grain[cidx];
+  if (hold[cidx] != 1) {
+    f.bed[fidx] = c.bed[cidx] + c.water[cidx];
+    f.bed_s[fidx] = c.bed[cidx] + c.slurry[cidx];
+    return;
+  }
+  f.bed[fidx] = c.bed[cidx];
+  f.bed_s[fidx] = c.bed[cidx];
+  f.water[fidx] = c.water[cidx];
+  f.slurry[fidx] = c.slurry[cidx];
+  f.uw[fidx] = c.uw[cidx];
+  f.vw[fidx] = c.vw[cidx];
+  f.us[fidx] = c.us[cidx];
+  f.vs[fidx] = c.vs[cidx];
+}
+
+__global__ void k_zero_held(CorePlanes c, const uint8_t* hold, int ci0, int cj0, int w, int h) {
+  int i = ci0 + (int)(blockIdx.x * blockDim.x + threadIdx.x), j = cj0 + (int)(blockIdx.y * blockDim.y + threadIdx.y);
+  if (i >= ci0 + w || j >= cj0 + h) return;
+  size_t idx = (size_t)j * kWindowRes + i;
+  if (hold[idx] != 1) return;
+  c.water[idx] = 0.0f;
+  c.slurry[idx] = 0.0f;
+  c.uw[idx] = 0.0f;
+  c.vw[idx] = 0.0f;
+  c.us[idx] = 0.0f;
+  c.vs[idx] = 0.0f;
+}
+
+// Fine -> coarse sum of restrict_fine (src/fluid_core.rs:8462-8549) for one
+// coarse cell: its fine cells in the fine raster order, then the hand-back.
+__global__ void k_restrict(CorePlanes c, FinePlanes f, const uint8_t* hold, CoarseSpans sp) {
+  int a = (int)(blockIdx.x * blockDim.x + threadIdx.x), b = (int)(blockIdx.y * blockDim.y + threadIdx.y);
+  if (a >= sp.ncol || b >= sp.nrow) return;
+  int ci = sp.ci0 + a, cj = sp.cj0 + b;
+  int i0 = sp.col_start[a], ni = sp.col_count[a];
+  int j0 = sp.row_start[b], nj = sp.row_count[b];
+  if (ni == 0 || nj == 0) return;
+  const float fine_area = kFineCell * kFineCell, coarse_area = kCoarseCell * kCoarseCell;
+  float w = 0.0f, s = 0.0f, auw = 0.0f, avw = 0.0f, aus = 0.0f, avs = 0.0f;
+  for (int j = j0; j < j0 + nj; j++)
+    for (int i = i0; i < i0 + ni; i++) {
+      size_t fidx = (size_t)j * kFineRes + i;
+      float fw = f.water[fidx], fs = f.slurry[fidx];
+      if (fw == 0.0f && fs == 0.0f) continue;
+      w = w + fw * fine_area;
+      s = s + fs * fine_area;
+      auw = auw + f.uw[fidx] * fw * fine_area;
+      avw = avw + f.vw[fidx] * fw * fine_area;
+      aus = aus + f.us[fidx] * fs * fine_area;
+      avs = avs + f.vs[fidx] * fs * fine_area;
+    }
+  size_t idx = (size_t)cj * kWindowRes + ci;
+  bool held = hold[idx] == 1;
+  float wd = w / coarse_area, sd = s / coarse_area;
+  if (!held && ni * nj >= 100) {
+    if (wd <= 0.0f && sd <= 0.0f) return;
+    float old_w = c.water[idx], old_s = c.slurry[idx];
+    float new_w = old_w + wd, new_s = old_s + sd;
+    if (new_w > 1.0e-6f) {
+      c.uw[idx] = (old_w * c.uw[idx] * coarse_area + auw) / (new_w * coarse_area);
+      c.vw[idx] = (old_w * c.vw[idx] * coarse_area + avw) / (new_w * coarse_area);
+    }
+    if (new_s > 1.0e-6f) {
+      c.us[idx] = (old_s * c.us[idx] * coarse_area + aus) / (new_s * coarse_area);
+      c.vs[idx] = (old_s * c.vs[idx] * coarse_area + avs) / (new_s * coarse_area);
+    }
+    c.water[idx] = new_w;
+    c.slurry[idx] = new_s;
+    return;
+  }
+  c.water[idx] = wd;
+  c.slurry[idx] = sd;
+  if (wd > 1.0e-6f) {
+    c.uw[idx] = auw / (wd * coarse_area);
+    c.vw[idx] = avw / (wd * coarse_area);
+  }
+  if (sd > 1.0e-6f) {
+    c.us[idx] = aus / (sd * coarse_area);
+    c.vs[idx] = avs / (sd * coarse_area);
+  }
+}
+
+// Fine columns off the coarse window are parked (restrict_fine :8476-8485);
+// their volume is booked as frozen, in FP64.
+__global__ void k_park(FinePlanes f, FineAxes ax, double* frozen) {
+  int i = (int)(blockIdx.x * blockDim.x + threadIdx.x), j = (int)(blockIdx.y * blockDim.y + threadIdx.y);
+  if (i >= kFineRes || j >= kFineRes) return;
+  if (ax.col[i] >= 0 && ax.row[j] >= 0) return;
+  size_t fidx = (size_t)j * kFineRes + i;
+  const float fa = kFineCell * kFineCell;
+  float w = f.water[fidx], s = f.slurry[fidx];
+  if (w != 0.0f) atomicAdd(&frozen[0], (double)(w * fa));
+  if (s != 0.0f) atomicAdd(&frozen[1], (double)(s * fa));
+}
+
+__global__ void k_fill(float* p, float val, int n) {
+  int k = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (k < n) p[k] = val;
+}
+
+// ---- near field tiles (src/fluid_core/near.rs) ------------------------------
+
+__device__ inline unsigned long long pack_key(int kx, int kz) {
+  return ((unsigned long long)(unsigned)kx << 32) | (unsigned long long)(unsigned)kz;
+}
+
+__device__ inline unsigned tile_hash(int kx, int kz, unsigned cap) {
+  unsigned h = (unsigned)kx * 0x9E3779B1u ^ ((unsigned)kz * 0x85EBCA6Bu + 0x27d4eb2fu);
+  h ^= h >> 15;
+  return h & (cap - 1);
+}
+
+__device__ int tile_find(const NearTable& t, int kx, int kz) {
+  unsigned long long key = pack_key(kx, kz);
+  unsigned s = tile_hash(kx, kz, t.cap);
+  for (unsigned n = 0; n < t.cap; n++, s = (s + 1) & (t.cap - 1)) {
+    unsigned long long k = t.keys[s];
+    if (k == kEmptyKey) return -1;
+    if (k == key) return (int)s;
+  }
+  return -1;
+}
+
+__device__ int tile_insert(NearTable& t, int kx, int kz) {
+  int found = tile_find(t, kx, kz);
+  if (found >= 0) return found;
+  unsigned long long key = pack_key(kx, kz);
+  unsigned s = tile_hash(kx, kz, t.cap);
+  for (unsigned n = 0; n < t.cap; n++, s = (s + 1) & (t.cap - 1)) {
+    unsigned long long k = t.keys[s];
+    if (k == kEmptyKey || k == kTombKey) {
+      unsigned long long prev = atomicCAS(&t.keys[s], k, key);
+      if (prev == k) {
+        atomicAdd(t.count, 1u);
+        return (int)s;
+      }
+    }
+  }
+  return -1;
+}
+
+__device__ void tile_remove(NearTable& t, int kx, int kz) {
+  int s = tile_find(t, kx, kz);
+  if (s < 0) return;
+  t.keys[s] = kTombKey;
+  atomicSub(t.count, 1u);
+}
+
+__device__ inline float tile_left(float idle) { return idle <= 0.0f ? 1.0f : expf(-idle / 1.5f); }
+
+__device__ inline float abs_peak(const float* r) {
+  float hi[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+  for (int k = 0; k < 100; k += 4)
+    for (int l = 0; l < 4; l++) {
+      float a = fabsf(r[k + l]);
+      hi[l] = a > hi[l] ? a : hi[l];
+    }
+  return fmaxf(fmaxf(fmaxf(hi[0], hi[1]), hi[2]), hi[3]);
+}
+
+__device__ inline float floor_scale(float base, const float* r) {
+  float low = 0.0f;
+  for (int k = 0; k < 100; k++) low = fminf(low, r[k]);
+  if (low >= 0.0f || base + low >= 0.0f) return 1.0f;
+  if (base <= 0.0f) return 0.0f;
+  return clampf(base / -low, 0.0f, 1.0f);
+}
+
+__device__ inline void recentre(float* r) {
+  double m = 0.0;
+  for (int k = 0; k < 100; k++) m += (double)r[k];
+  float mean = (float)(m / 100.0);
+  for (int k = 0; k < 100; k++) r[k] = r[k] - mean;
+}
+
+// near_capture (near.rs:199-261), one held column per thread.
+__global__ void k_near_capture(FinePlanes f, NearTable t, const NearColumn* cols, int ncols) {
+  int q = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (q >= ncols) return;
+  NearColumn nc = cols[q];
+  float sw = 0.0f, ss = 0.0f;
+  for (int b = 0; b < 10; b++) {
+    size_t fr = (size_t)(nc.j0 + b) * kFineRes + nc.i0;
+    float rw = -0.0f, rs = -0.0f;
+    for (int a = 0; a < 10; a++) {
+      rw = rw + f.water[fr + a];
+      rs = rs + f.slurry[fr + a];
+    }
+    sw = sw + rw;
+    ss = ss + rs;
+  }
+  if (sw <= 0.0f && ss <= 0.0f) {
+    tile_remove(t, nc.kx, nc.kz);
+    return;
+  }
+  float mw = sw / 100.0f, ms = ss / 100.0f;
+  float rw[100], rs[100];
+  for (int b = 0; b < 10; b++) {
+    size_t fr = (size_t)(nc.j0 + b) * kFineRes + nc.i0;
+    for (int a = 0; a < 10; a++) {
+      rw[b * 10 + a] = f.water[fr + a] - mw;
+      rs[b * 10 + a] = f.slurry[fr + a] - ms;
+    }
+  }
+  float peak = fmaxf(abs_peak(rw), abs_peak(rs));
+  if (peak <= 1.0e-5f && !nc.inner) {
+    tile_remove(t, nc.kx, nc.kz);
+    return;
+  }
+  recentre(rw);
+  recentre(rs);
+  int s = tile_insert(t, nc.kx, nc.kz);
+  if (s < 0) return;
+  float* d = t.data + (size_t)s * 400;
+  for (int k = 0; k < 100; k++) {
+    d[k] = rw[k];
+    d[100 + k] = rs[k];
+  }
+  if (nc.inner && sw > 1.0e-10f) {
+    float mu = 0.0f, mv = 0.0f;
+    for (int b = 0; b < 10; b++) {
+      size_t fr = (size_t)(nc.j0 + b) * kFineRes + nc.i0;
+      for (int a = 0; a < 10; a++) {
+        float w = f.water[fr + a];
+        mu = mu + w * f.uw[fr + a];
+        mv = mv + w * f.vw[fr + a];
+      }
+    }
+    float uw = mu / sw, vw = mv / sw;
+    for (int b = 0; b < 10; b++) {
+      size_t fr = (size_t)(nc.j0 + b) * kFineRes + nc.i0;
+      for (int a = 0; a < 10; a++) {
+        bool wet = f.water[fr + a] > 1.0e-6f;
+        d[200 + b * 10 + a] = wet ? f.uw[fr + a] - uw : 0.0f;
+        d[300 + b * 10 + a] = wet ? f.vw[fr + a] - vw : 0.0f;
+      }
+    }
+  } else {
+    for (int k = 0; k < 100; k++) {
+      d[200 + k] = 0.0f;
+      d[300 + k] = 0.0f;
+    }
+  }
+  t.peak[s] = peak;
+  t.idle[s] = 0.0f;
+  t.fresh[s] = 1;
+}
+
+// near_apply's lay-down (near.rs:265-296), one held column per thread.
+__global__ void k_near_apply(FinePlanes f, NearTable t, const NearColumn* cols, int ncols) {
+  int q = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (q >= ncols) return;
+  NearColumn nc = cols[q];
+  int s = tile_find(t, nc.kx, nc.kz);
+  if (s < 0) return;
+  const float* d = t.data + (size_t)s * 400;
+  size_t f0 = (size_t)nc.j0 * kFineRes + nc.i0;
+  float bw = f.water[f0], bs = f.slurry[f0];
+  float left = tile_left(t.idle[s]);
+  float aw = left * floor_scale(bw / fmaxf(left, 1.0e-6f), d);
+  float as = left * floor_scale(bs / fmaxf(left, 1.0e-6f), d + 100);
+  for (int b = 0; b < 10; b++)
+    for (int a = 0; a < 10; a++) {
+      size_t fi = (size_t)(nc.j0 + b) * kFineRes + nc.i0 + a;
+      int k = b * 10 + a;
+      float w = fmaxf(bw + aw * d[k], 0.0f);
+      float sv = fmaxf(bs + as * d[100 + k], 0.0f);
+      f.water[fi] = w;
+      f.slurry[fi] = sv;
+      if (nc.inner && w > 1.0e-6f) {
+        f.uw[fi] = f.uw[fi] + left * d[200 + k];
+        f.vw[fi] = f.vw[fi] + left * d[300 + k];
+      }
+    }
+}
+
+// near_fade (near.rs:300-313), one table slot per thread.
+__global__ void k_near_fade(NearTable t, float dt) {
+  unsigned s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= t.cap) return;
+  unsigned long long k = t.keys[s];
+  if (k == kEmptyKey || k == kTombKey) return;
+  if (t.fresh[s]) {
+    t.fresh[s] = 0;
+    return;
+  }
+  t.idle[s] = t.idle[s] + dt;
+  if (!(t.peak[s] * tile_left(t.idle[s]) > 1.0e-5f)) {
+    t.keys[s] = kTombKey;
+    atomicSub(t.count, 1u);
+  }
+}
+
+// Live tiles move into a fresh table (drops tombstones).
+__global__ void k_near_rehash(NearTable from, NearTable to) {
+  unsigned s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= from.cap) return;
+  unsigned long long k = from.keys[s];
+  if (k == kEmptyKey || k == kTombKey) return;
+  int kx = (int)(unsigned)(k >> 32), kz = (int)(unsigned)(k & 0xffffffffull);
+  int d = tile_insert(to, kx, kz);
+  if (d < 0) return;
+  for (int q = 0; q < 400; q++) to.data[(size_t)d * 400 + q] = from.data[(size_t)s * 400 + q];
+  to.peak[d] = from.peak[s];
+  to.idle[d] = from.idle[s];
+  to.fresh[d] = from.fresh[s];
+}
+
+__global__ void k_near_clear(NearTable t) {
+  unsigned s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= t.cap) return;
+  t.keys[s] = kEmptyKey;
+  t.fresh[s] = 0;
+  if (s == 0) *t.count = 0u;
+}
+
+// coarse_cell + publish (src/fluid_core.rs:6998-7034,
+// src/render_contract.rs:424-496): film (water, slurry), flow (u, v, speed,
+// turbidity) and, when soil planes are given, ground_rut
+// (bed + placed, rut, bed, placed).
+__global__ void k_publish(CorePlanes c, const float* soil_bed, const float* soil_placed, const float* soil_rut,
+                          float* ground_rut4, float* film2, float* flow4, int n) {
+  int k = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (k >= n) return;
+  auto fin = [](float x) { return isfinite(x) ? x : 0.0f; };
+  float water = fin(c.water[k]), slurry = fin(c.slurry[k]);
+  float film = water + slurry;
+  float u = 0.0f, v = 0.0f;
+  if (!(film < 1.0e-5f)) {
+    u = fin((c.uw[k] * water + c.us[k] * slurry) / film);
+    v = fin((c.vw[k] * water + c.vs[k] * slurry) / film);
+  }
+  bool dry = !isfinite(water) || !isfinite(slurry) || water + slurry < 1.0e-5f;
+  if (dry) u = v = 0.0f;
+  float speed = dry ? 0.0f : fin(sqrtf(u * u + v * v));
+  if (film2) {
+    film2[2 * k] = water;
+    film2[2 * k + 1] = slurry;
+  }
+  if (flow4) {
+    flow4[4 * k] = u;
+    flow4[4 * k + 1] = v;
+    flow4[4 * k + 2] = speed;
+    flow4[4 * k + 3] = 0.0f;
+  }
+  if (ground_rut4) {
+    float bed = soil_bed ? soil_bed[k] : c.bed[k];
+    float placed = soil_placed ? soil_placed[k] : 0.0f;
+    float rut = soil_rut ? soil_rut[k] : 0.0f;
+    ground_rut4[4 * k] = fin(bed + placed);
+    ground_rut4[4 * k + 1] = fin(rut);
+    ground_rut4[4 * k + 2] = fin(bed);
+    ground_rut4[4 * k + 3] = fin(placed);
+  }
+}
+
+inline dim3 grid2(int w, int h, dim3 b) {
+  return dim3((unsigned)((w + (int)b.x - 1) / (int)b.x), (unsigned)((h + (int)b.y - 1) / (int)b.y));
+}
+
+}  // namespace
+
+void core_push_on(const GridGeo& g, float* water, float* slurry, float* uw, float* vw, float* us, float* vs,
+                  const PushList& pl, int i0, int j0, int w, int h, hipStream_t st) {
+  if (w <= 0 || h <= 0 || pl.n <= 0) return;
+  dim3 b(32, 8);
+  hipLaunchKernelGGL(k_push_on, grid2(w, h, b), b, 0, st, g, water, slurry, uw, vw, us, vs, pl, i0, j0, w, h);
+}
+
+void core_peel(const GridGeo& g, float* water, const float* uw, const float* vw, const PushList& pl, int i0, int j0,
+               int w, int h, const uint8_t* hold, const GridGeo& cg, bool skip_held, SprayEvent* ev, unsigned* nev,
+               unsigned cap, double* vol, hipStream_t st) {
+  if (w <= 0 || h <= 0 || pl.n <= 0) return;
+  dim3 b(32, 8);
+  hipLaunchKernelGGL(k_peel, grid2(w, h, b), b, 0, st, g, water, uw, vw, pl, i0, j0, w, h, hold, cg, (int)skip_held, ev,
+                     nev, cap, vol);
+}
+
+void core_soften(float* u, float* v, int res, hipStream_t st) {
+  dim3 b(32, 8);
+  hipLaunchKernelGGL(k_soften, grid2(res, res, b), b, 0, st, u, v, res);
+}
+
+void core_nu(float* nu, const uint8_t* grain, const float* moisture, int n, hipStream_t st) {
+  hipLaunchKernelGGL(k_nu, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, nu, grain, moisture, n);
+}
+
+void core_fill(float* p, float val, int n, hipStream_t st) {
+  hipLaunchKernelGGL(k_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, val, n);
+}
+
+void core_refine_copy(const CorePlanes& c, const FinePlanes& f, const uint8_t* hold, const FineAxes& ax, int ci0,
+                      int cj0, int cw, int ch, hipStream_t st) {
+  dim3 b(32, 8);
+  hipLaunchKernelGGL(k_refine, grid2(kFineRes, kFineRes, b), b, 0, st, c, f, hold, ax);
+  if (cw > 0 && ch > 0) hipLaunchKernelGGL(k_zero_held, grid2(cw, ch, b), b, 0, st, c, hold, ci0, cj0, cw, ch);
+}
+
+void core_restrict(const CorePlanes& c, const FinePlanes& f, const uint8_t* hold, const CoarseSpans& sp,
+                   const FineAxes& ax, double* frozen, hipStream_t st) {
+  dim3 b(16, 16);
+  if (sp.ncol > 0 && sp.nrow > 0) hipLaunchKernelGGL(k_restrict, grid2(sp.ncol, sp.nrow, b), b, 0, st, c, f, hold, sp);
+  hipLaunchKernelGGL(k_park, grid2(kFineRes, kFineRes, b), b, 0, st, f, ax, frozen);
+}
+
+void core_near_capture(const FinePlanes& f, const NearTable& t, const NearColumn* cols, int ncols, hipStream_t st) {
+  if (ncols <= 0) return;
+  hipLaunchKernelGGL(k_near_capture, dim3((unsigned)((ncols + 63) / 64)), dim3(64), 0, st, f, t, cols, ncols);
+}
+
+void core_near_apply(const FinePlanes& f, const NearTable& t, const NearColumn* cols, int ncols, hipStream_t st) {
+  if (ncols <= 0) return;
+  hipLaunchKernelGGL(k_near_apply, dim3((unsigned)((ncols + 63) / 64)), dim3(64), 0, st, f, t, cols, ncols);
+}
+
+void core_near_fade(const NearTable& t, float dt, hipStream_t st) {
+  hipLaunchKernelGGL(k_near_fade, dim3((t.cap + 255) / 256), dim3(256), 0, st, t, dt);
+}
+
+void core_near_clear(const NearTable& t, hipStream_t st) {
+  hipLaunchKernelGGL(k_near_clear, dim3((t.cap + 255) / 256), dim3(256), 0, st, t);
+}
+
+void core_near_rehash(const NearTable& from, const NearTable& to, hipStream_t st) {
+  core_near_clear(to, st);
+  hipLaunchKernelGGL(k_near_rehash, dim3((from.cap + 255) / 256), dim3(256), 0, st, from, to);
+}
+
+void core_publish(const CorePlanes& c, const float* soil_bed, const float* soil_placed, const float* soil_rut,
+                  float* ground_rut4, float* film2, float* flow4, hipStream_t st) {
+  int n = kWindowRes * kWindowRes;
+  hipLaunchKernelGGL(k_publish, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, c, soil_bed, soil_placed, soil_rut,
+                     ground_rut4, film2, flow4, n);
+}
+
+}  // namespace silt

@@ -81,6 +81,21 @@ __host__ __device__ inline float sheet_damp(float nu) {
   return at_silt * powf(t, 1.15f);
 }
 
+// dynamic_viscosity, src/fluid_core.rs:150-162 (grain codes of Grain).
+__host__ __device__ inline float dynamic_viscosity(int grain, float moisture) {
+  float x = clampf((clampf(moisture, 0.0f, 1.0f) - 0.22f) / 0.70f, 0.0f, 1.0f);
+  float thin = x * x * (3.0f - 2.0f * x);
+  float stiff, fluid;
+  switch (grain) {
+    case 1: stiff = 220.0f; fluid = 4.0f; break;   // Silt
+    case 2: stiff = 28.0f; fluid = 0.55f; break;   // Sand
+    case 3: stiff = 0.9f; fluid = 0.9f; break;     // Water
+    case 4: stiff = 1.5f; fluid = 0.08f; break;    // Other
+    default: stiff = 420.0f; fluid = 0.35f; break; // Clay
+  }
+  return stiff + (fluid - stiff) * thin;
+}
+
 // overload of the divergence source, src/fluid_core.rs:1999.
 __host__ __device__ inline float overload_of(float nu) { return 1.0f + clampf(nu / 231.0f, 0.0f, 6.0f); }
 

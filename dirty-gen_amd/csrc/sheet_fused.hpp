@@ -72,6 +72,11 @@ int fused_step(const SheetPlanes& s, const StepPlan& plan, int res, float cell, 
 int run_plan(const SheetPlanes& s, const StepPlan& plan, int res, float cell, float dt, const silt_drive& d,
              const silt_sheet_opts& o, bool gate_uniform, bool nu_uniform, float nu0, hipStream_t st);
 
+// A step of one layer on device planes, planned on the host from a read
+// back of depth, velocity and bed (blocking); kernels enqueued on st.
+int step_device_planes(const SheetPlanes& s, int res, float cell, float dt, const silt_drive& d,
+                       const silt_sheet_opts& o, bool gate_uniform, bool nu_uniform, float nu0, hipStream_t st);
+
 // Output rows per workgroup (the autotuned chunk); 0 restores the default.
 int fused_chunk_rows(int res);
 void fused_set_chunk_rows(int rows);

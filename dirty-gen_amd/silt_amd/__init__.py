@@ -218,3 +218,144 @@ class SheetPair:
 
     def last_fallback(self):
         return lib().silt_pair_last_fallback(self._h)
+
+
+class TirePush(C.Structure):
+    """TirePush (src/fluid_core.rs:357-368)."""
+    _fields_ = [(n, C.c_float) for n in ("x", "z", "forward_x", "forward_z", "radius", "width", "speed", "slip")]
+
+
+class SprayEvent(C.Structure):
+    _fields_ = [(n, C.c_float) for n in ("x", "y", "z", "vx", "vy", "vz", "volume")] + [("kind", C.c_int32)]
+
+
+class FluidLedger(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("water_m3", "slurry_m3", "frozen_water_m3", "frozen_slurry_m3",
+                                          "spray_m3", "total_m3")]
+
+
+GRAIN = {"clay": 0, "silt": 1, "sand": 2, "water": 3, "other": 4}
+WINDOW_RES = 200
+
+
+def _declare_core(L):
+    L.silt_core_create.argtypes = [_F, _F, C.c_int]
+    L.silt_core_create.restype = _P
+    L.silt_core_destroy.argtypes = [_P]
+    L.silt_core_destroy.restype = None
+    for name, at in (("silt_core_set_fine_pace", [_P, C.c_int]), ("silt_core_set_fine_sub_cap", [_P, _F]),
+                     ("silt_core_set_near_field", [_P, C.c_int]), ("silt_core_set_coarse_confine", [_P, C.c_int]),
+                     ("silt_core_window_origin", [_P, _P]), ("silt_core_map_water_bed", [_P, _P, _P]),
+                     ("silt_core_pour", [_P, _F, _F, _F, _F, _F, C.c_int, C.c_int, _F]),
+                     ("silt_core_lay_slurry", [_P, _P, C.c_int, _F]),
+                     ("silt_core_read_substrate", [_P, _P, _P, _P]),
+                     ("silt_core_push_tire", [_P, C.POINTER(TirePush)]),
+                     ("silt_core_step", [_P, _F, C.POINTER(SprayEvent), _Z, C.POINTER(C.c_size_t)]),
+                     ("silt_core_ledger", [_P, C.POINTER(FluidLedger)]),
+                     ("silt_core_get_planes", [_P] + [_P] * 8), ("silt_core_set_planes", [_P] + [_P] * 8),
+                     ("silt_core_publish", [_P] + [_P] * 6)):
+        getattr(L, name).argtypes = at
+        getattr(L, name).restype = C.c_int
+    L.silt_core_near_tiles.argtypes = [_P]
+    L.silt_core_near_tiles.restype = C.c_long
+
+
+class FluidCore:
+    """``FluidCore`` on the sheet path (include/silt_core.h), state on the GPU.
+    Reference closures are arrays sampled at coarse cell centres."""
+
+    PLANES = ("water", "slurry", "uw", "vw", "us", "vs", "bed", "moisture")
+
+    def __init__(self, center=(0.0, 0.0), path="exact"):
+        L = lib()
+        if not getattr(L, "_core_declared", False):
+            _declare_core(L)
+            L._core_declared = True
+        self._c = L.silt_core_create(center[0], center[1], PATH[path])
+        if not self._c:
+            raise SiltError("silt_core_create failed")
+
+    def close(self):
+        if self._c:
+            lib().silt_core_destroy(self._c)
+            self._c = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_fine_pace(self, pace):
+        _check_status(lib().silt_core_set_fine_pace(self._c, pace))
+
+    def set_fine_sub_cap(self, cap):
+        _check_status(lib().silt_core_set_fine_sub_cap(self._c, cap))
+
+    def set_near_field(self, on):
+        _check_status(lib().silt_core_set_near_field(self._c, int(on)))
+
+    def set_coarse_confine(self, mode):
+        _check_status(lib().silt_core_set_coarse_confine(self._c, CONFINE[mode]))
+
+    def origin(self):
+        o = np.zeros(2, np.float32)
+        _check_status(lib().silt_core_window_origin(self._c, _ptr(o)))
+        return o
+
+    def pour_water(self, x0, x1, z0, z1, depth):
+        _check_status(lib().silt_core_pour(self._c, x0, x1, z0, z1, depth, 1, 3, 1.0))
+
+    def pour_slurry(self, x0, x1, z0, z1, depth, grain, moisture):
+        g = GRAIN[grain] if isinstance(grain, str) else int(grain)
+        _check_status(lib().silt_core_pour(self._c, x0, x1, z0, z1, depth, 0, g, moisture))
+
+    def map_water_bed(self, bed, water):
+        b, w = (np.ascontiguousarray(a, np.float32) for a in (bed, water))
+        _check_status(lib().silt_core_map_water_bed(self._c, _ptr(b), _ptr(w)))
+
+    def lay_slurry(self, depth, grain, moisture):
+        d = np.ascontiguousarray(depth, np.float32)
+        g = GRAIN[grain] if isinstance(grain, str) else int(grain)
+        _check_status(lib().silt_core_lay_slurry(self._c, _ptr(d), g, moisture))
+
+    def read_substrate(self, bed, moisture, grain):
+        b, m = (np.ascontiguousarray(a, np.float32) for a in (bed, moisture))
+        g = np.ascontiguousarray(grain, np.uint8)
+        _check_status(lib().silt_core_read_substrate(self._c, _ptr(b), _ptr(m), _ptr(g)))
+
+    def push_tire(self, x, z, forward_x, forward_z, radius, width, speed, slip):
+        p = TirePush(x, z, forward_x, forward_z, radius, width, speed, slip)
+        _check_status(lib().silt_core_push_tire(self._c, C.byref(p)))
+
+    def step(self, dt, max_events=4096):
+        ev = (SprayEvent * max_events)()
+        n = C.c_size_t(0)
+        _check_status(lib().silt_core_step(self._c, dt, ev, max_events, C.byref(n)))
+        return [ev[k] for k in range(min(n.value, max_events))], n.value
+
+    def ledger(self):
+        led = FluidLedger()
+        _check_status(lib().silt_core_ledger(self._c, C.byref(led)))
+        return {k: getattr(led, k) for k, _ in FluidLedger._fields_}
+
+    def planes(self):
+        n = WINDOW_RES * WINDOW_RES
+        out = {k: np.empty(n, np.float32) for k in self.PLANES}
+        _check_status(lib().silt_core_get_planes(self._c, *(_ptr(out[k]) for k in self.PLANES)))
+        return out
+
+    def set_planes(self, planes):
+        arrs = [None if planes.get(k) is None else np.ascontiguousarray(planes[k], np.float32) for k in self.PLANES]
+        _check_status(lib().silt_core_set_planes(self._c, *(None if a is None else _ptr(a) for a in arrs)))
+
+    def publish(self, soil_bed=None, soil_placed=None, soil_rut=None):
+        n = WINDOW_RES * WINDOW_RES
+        gr, film, flow = np.empty(4 * n, np.float32), np.empty(2 * n, np.float32), np.empty(4 * n, np.float32)
+        soils = [None if a is None else np.ascontiguousarray(a, np.float32) for a in (soil_bed, soil_placed, soil_rut)]
+        _check_status(lib().silt_core_publish(self._c, *(None if a is None else _ptr(a) for a in soils),
+                                              _ptr(gr), _ptr(film), _ptr(flow)))
+        return gr, film, flow
+
+    def near_tiles(self):
+        return int(_check_status(lib().silt_core_near_tiles(self._c)))

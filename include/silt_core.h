@@ -1,0 +1,105 @@
+/*
+ * silt_core.h -- the two-layer FluidCore of libsilt_amd.so.
+ *
+ * Replaces FluidCore (src/fluid_core.rs:4808-4955) on the sheet path: the
+ * 40 m coarse window (200 x 200 cells of 0.2 m) and the 2 cm patch
+ * (160 x 160 cells) laid under the tires each step.  State lives on the GPU.
+ * Closures of the reference API (map_water_bed, lay_slurry_mask,
+ * read_substrate) become arrays sampled at coarse cell centres
+ * origin + (i + 0.5) * 0.2, row-major idx = j * 200 + i.
+ * A handle is not thread-safe; calls on one handle are serialised on its own
+ * HIP stream.  Negative returns are SILT_E* codes (silt_sheet.h).
+ */
+#ifndef SILT_CORE_H
+#define SILT_CORE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "silt_sheet.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SILT_WINDOW_RES 200
+#define SILT_FINE_RES 160
+
+typedef struct silt_core silt_core;
+
+/* TirePush, src/fluid_core.rs:357-368. */
+typedef struct silt_tire_push {
+    float x, z, forward_x, forward_z, radius, width, speed, slip;
+} silt_tire_push;
+
+/* SprayEvent, src/fluid_core.rs:371-382 (kind 0 = drop). */
+typedef struct silt_spray_event {
+    float x, y, z, vx, vy, vz, volume;
+    int32_t kind;
+} silt_spray_event;
+
+/* FluidLedger (src/fluid_core.rs:483-509) restricted to the sheet path, FP64. */
+typedef struct silt_fluid_ledger {
+    double water_m3, slurry_m3, frozen_water_m3, frozen_slurry_m3, spray_m3, total_m3;
+} silt_fluid_ledger;
+
+/* FluidCore::new(center) (src/fluid_core.rs:4958-5060).  path: SILT_PATH_*
+ * for the sheet steps.  NULL on failure. */
+silt_core *silt_core_create(float center_x, float center_z, int path);
+void silt_core_destroy(silt_core *c);
+
+/* set_fine_pace / set_fine_sub_cap (src/fluid_core.rs:5072-5091),
+ * set_near_field (src/fluid_core/near.rs:332-338).  coarse_confine picks the
+ * coarse sheets' confinement: SILT_CONFINE_REFERENCE (in-place sweep, as the
+ * reference steps a layer beside the other) or SILT_CONFINE_JACOBI. */
+int silt_core_set_fine_pace(silt_core *c, int pace);
+int silt_core_set_fine_sub_cap(silt_core *c, float cap);
+int silt_core_set_near_field(silt_core *c, int on);
+int silt_core_set_coarse_confine(silt_core *c, int confine);
+
+/* window_bounds origin (src/fluid_core.rs:6977-6990): out2 = corner x, z. */
+int silt_core_window_origin(const silt_core *c, float *out2);
+
+/* map_water_bed (src/fluid_core.rs:5329-5355): bed and water per cell. */
+int silt_core_map_water_bed(silt_core *c, const float *bed, const float *water);
+/* pour_water / pour_slurry (src/fluid_core.rs:5424-5440 via paint :5820-5855).
+ * water != 0 pours water; otherwise slurry of the grain code (Grain) and moisture. */
+int silt_core_pour(silt_core *c, float x0, float x1, float z0, float z1, float depth, int water, int grain,
+                   float moisture);
+/* lay_slurry_mask (src/fluid_core.rs:5857-5880). */
+int silt_core_lay_slurry(silt_core *c, const float *depth, int grain, float moisture);
+/* read_substrate_grid (src/fluid_core.rs:5928-5962): bed, moisture, grain per cell. */
+int silt_core_read_substrate(silt_core *c, const float *bed, const float *moisture, const uint8_t *grain);
+/* push_tire (src/fluid_core.rs:5964-5966); at most 64 per step. */
+int silt_core_push_tire(silt_core *c, const silt_tire_push *push);
+
+/* FluidCore::step (src/fluid_core.rs:6034-6128) on the sheet path.  Up to
+ * cap spray events are copied to events; *nevents gets the count the step
+ * produced (may exceed cap).  Returns SILT_OK. */
+int silt_core_step(silt_core *c, float dt, silt_spray_event *events, size_t cap, size_t *nevents);
+
+/* ledger() (src/fluid_core.rs:6821-6883) for the sheets, parked fine volume and spray. */
+int silt_core_ledger(silt_core *c, silt_fluid_ledger *out);
+
+/* Coarse planes in and out, 200 x 200 each (any pointer may be NULL to skip). */
+int silt_core_get_planes(silt_core *c, float *water, float *slurry, float *uw, float *vw, float *us, float *vs,
+                         float *bed, float *moisture);
+int silt_core_set_planes(silt_core *c, const float *water, const float *slurry, const float *uw, const float *vw,
+                         const float *us, const float *vs, const float *bed, const float *moisture);
+
+/* publish maps 1/3/4 of the render contract (src/render_contract.rs:424-496,
+ * docs/RENDER_CONTRACT.md:35-38) on the 200 x 200 window: film rg32f
+ * (water, slurry), flow rgba32f (u, v, speed, turbidity), ground_rut rgba32f
+ * (bed + placed, rut, bed, placed).  soil_* (host, 200 x 200) may be NULL:
+ * then the sheet bed is used with no placed soil and no rut.  Host outputs. */
+int silt_core_publish(silt_core *c, const float *soil_bed, const float *soil_placed, const float *soil_rut,
+                      float *ground_rut4, float *film2, float *flow4);
+
+/* Near-field residual tiles alive (FluidCore::near_tiles, near.rs:316-318). */
+long silt_core_near_tiles(silt_core *c);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SILT_CORE_H */
