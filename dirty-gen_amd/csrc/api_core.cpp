@@ -12,6 +12,7 @@
 
 #include "core.hpp"
 #include "plan.hpp"
+#include "rut.hpp"
 #include "sheet_device.hpp"
 #include "sheet_fused.hpp"
 #include "silt_core.h"
@@ -57,6 +58,11 @@ struct silt_core {
   unsigned* d_nev = nullptr;
   double* d_acc = nullptr;  // [0] spray volume this step, [1..2] frozen water / slurry
   double spray_m3 = 0.0;
+  // Ground under the sheet (SoilField bed / placed / initial height on the
+  // window grid), set by silt_core_set_ground and cut by the rut stamp.
+  bool ground = false;
+  float *soil_bed = nullptr, *soil_placed = nullptr, *soil_init = nullptr, *soil_rut = nullptr;
+  RutContact* d_contacts = nullptr;
   hipStream_t st = nullptr;
   std::vector<void*> allocs;
 };
@@ -362,7 +368,8 @@ silt_core* silt_core_create(float center_x, float center_z, int path) {
   ok = ok && dalloc(c, &c->c.grain, WN) && dalloc(c, &c->f.grain, FN) && dalloc(c, &c->d_hold, WN) &&
        dalloc(c, &c->nu_c, WN) && dalloc(c, &c->nu_f, FN) && near_alloc(c, &c->near[0]) && near_alloc(c, &c->near[1]) &&
        dalloc(c, &c->d_cols, 1024) && dalloc(c, &c->d_ev, kSprayCap) && dalloc(c, &c->d_nev, 1) &&
-       dalloc(c, &c->d_acc, 4);
+       dalloc(c, &c->d_acc, 4) && dalloc(c, &c->soil_bed, WN) && dalloc(c, &c->soil_placed, WN) &&
+       dalloc(c, &c->soil_init, WN) && dalloc(c, &c->soil_rut, WN) && dalloc(c, &c->d_contacts, kMaxPush);
   if (ok) {
     // Grain::Other everywhere (FluidCore::new, src/fluid_core.rs:4979-4980).
     ok = hipMemset(c->c.grain, 4, WN) == hipSuccess && hipMemset(c->f.grain, 4, FN) == hipSuccess;
@@ -624,9 +631,60 @@ int silt_core_ledger(silt_core* c, silt_fluid_ledger* out) {
   return SILT_OK;
 }
 
+int silt_core_set_ground(silt_core* c, const float* bed, const float* placed) {
+  if (!c || !bed) return SILT_EINVAL;
+  if (restrict_fine(c)) return SILT_EDEVICE;
+  HIPCHK(hipMemcpyAsync(c->soil_bed, bed, WN * 4, hipMemcpyHostToDevice, c->st));
+  HIPCHK(hipMemcpyAsync(c->soil_init, bed, WN * 4, hipMemcpyHostToDevice, c->st));
+  if (placed)
+    HIPCHK(hipMemcpyAsync(c->soil_placed, placed, WN * 4, hipMemcpyHostToDevice, c->st));
+  else
+    core_fill(c->soil_placed, 0.0f, WN, c->st);
+  core_ground(c->soil_bed, c->soil_placed, c->soil_init, c->c.bed, c->soil_rut, WN, c->st);
+  c->ground = true;
+  HIPCHK(hipStreamSynchronize(c->st));
+  return SILT_OK;
+}
+
+int silt_core_stamp_ruts(silt_core* c, const silt_tire_contact* contacts, size_t n, float dt) {
+  if (!c || (!contacts && n)) return SILT_EINVAL;
+  if (!c->ground) return SILT_EINVAL;
+  if (n > (size_t)kMaxPush) return SILT_EINVAL;
+  if (n == 0 || !(dt > 0.0f)) return SILT_OK;
+  if (restrict_fine(c)) return SILT_EDEVICE;
+  int rc = rut_stamp(c->soil_bed, c->soil_placed, c->soil_init, kWindowRes, kCoarseCell, c->origin[0], c->origin[1],
+                     contacts, n, dt, c->d_contacts, c->st);
+  if (rc) return rc;
+  core_ground(c->soil_bed, c->soil_placed, c->soil_init, c->c.bed, c->soil_rut, WN, c->st);
+  HIPCHK(hipStreamSynchronize(c->st));
+  return SILT_OK;
+}
+
+int silt_core_get_ground(silt_core* c, float* bed, float* placed, float* rut) {
+  if (!c) return SILT_EINVAL;
+  if (!c->ground) return SILT_EINVAL;
+  if (bed) HIPCHK(hipMemcpyAsync(bed, c->soil_bed, WN * 4, hipMemcpyDeviceToHost, c->st));
+  if (placed) HIPCHK(hipMemcpyAsync(placed, c->soil_placed, WN * 4, hipMemcpyDeviceToHost, c->st));
+  if (rut) HIPCHK(hipMemcpyAsync(rut, c->soil_rut, WN * 4, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  return SILT_OK;
+}
+
 int silt_core_publish(silt_core* c, const float* soil_bed, const float* soil_placed, const float* soil_rut,
                       float* ground_rut4, float* film2, float* flow4) {
   if (!c) return SILT_EINVAL;
+  if (c->ground && !soil_bed && !soil_placed && !soil_rut) {
+    // The core's own ground: publish it without a round trip through the host.
+    float* d_out = nullptr;
+    if (hipMalloc(&d_out, (size_t)WN * 10 * sizeof(float)) != hipSuccess) return SILT_ENOMEM;
+    core_publish(c->c, c->soil_bed, c->soil_placed, c->soil_rut, d_out, d_out + 4 * WN, d_out + 6 * WN, c->st);
+    if (ground_rut4) HIPCHK(hipMemcpyAsync(ground_rut4, d_out, 4 * WN * 4, hipMemcpyDeviceToHost, c->st));
+    if (film2) HIPCHK(hipMemcpyAsync(film2, d_out + 4 * WN, 2 * WN * 4, hipMemcpyDeviceToHost, c->st));
+    if (flow4) HIPCHK(hipMemcpyAsync(flow4, d_out + 6 * WN, 4 * WN * 4, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    (void)hipFree(d_out);
+    return SILT_OK;
+  }
   float *d_out = nullptr, *d_soil = nullptr;
   if (hipMalloc(&d_out, (size_t)WN * 10 * sizeof(float)) != hipSuccess) return SILT_ENOMEM;
   if (hipMalloc(&d_soil, (size_t)WN * 3 * sizeof(float)) != hipSuccess) {

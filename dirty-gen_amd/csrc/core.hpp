@@ -93,6 +93,8 @@ void core_near_apply(const FinePlanes& f, const NearTable& t, const NearColumn* 
 void core_near_fade(const NearTable& t, float dt, hipStream_t st);
 void core_near_clear(const NearTable& t, hipStream_t st);
 void core_near_rehash(const NearTable& from, const NearTable& to, hipStream_t st);
+void core_ground(const float* soil_bed, const float* soil_placed, const float* soil_init, float* sheet_bed, float* rut,
+                 int n, hipStream_t st);
 void core_publish(const CorePlanes& c, const float* soil_bed, const float* soil_placed, const float* soil_rut,
                   float* ground_rut4, float* film2, float* flow4, hipStream_t st);
 

@@ -506,6 +506,16 @@ __global__ void k_publish(CorePlanes c, const float* soil_bed, const float* soil
   }
 }
 
+// pull_substrate (src/world.rs:5511-5537): the sheet's bed is the pressed
+// ground, bed + placed; the rut map is the cut below the initial ground.
+__global__ void k_ground(const float* soil_bed, const float* soil_placed, const float* soil_init, float* sheet_bed,
+                         float* rut, int n) {
+  int k = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (k >= n) return;
+  sheet_bed[k] = soil_bed[k] + soil_placed[k];
+  if (rut) rut[k] = fmaxf(soil_init[k] - soil_bed[k], 0.0f);
+}
+
 inline dim3 grid2(int w, int h, dim3 b) {
   return dim3((unsigned)((w + (int)b.x - 1) / (int)b.x), (unsigned)((h + (int)b.y - 1) / (int)b.y));
 }
@@ -576,6 +586,12 @@ void core_near_clear(const NearTable& t, hipStream_t st) {
 void core_near_rehash(const NearTable& from, const NearTable& to, hipStream_t st) {
   core_near_clear(to, st);
   hipLaunchKernelGGL(k_near_rehash, dim3((from.cap + 255) / 256), dim3(256), 0, st, from, to);
+}
+
+void core_ground(const float* soil_bed, const float* soil_placed, const float* soil_init, float* sheet_bed, float* rut,
+                 int n, hipStream_t st) {
+  hipLaunchKernelGGL(k_ground, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, soil_bed, soil_placed, soil_init,
+                     sheet_bed, rut, n);
 }
 
 void core_publish(const CorePlanes& c, const float* soil_bed, const float* soil_placed, const float* soil_rut,

@@ -258,6 +258,10 @@ def _declare_core(L):
         getattr(L, name).restype = C.c_int
     L.silt_core_near_tiles.argtypes = [_P]
     L.silt_core_near_tiles.restype = C.c_long
+    L.silt_core_set_ground.argtypes = [_P, _P, _P]
+    L.silt_core_set_ground.restype = C.c_int
+    L.silt_core_get_ground.argtypes = [_P, _P, _P, _P]
+    L.silt_core_get_ground.restype = C.c_int
 
 
 class FluidCore:
@@ -359,3 +363,62 @@ class FluidCore:
 
     def near_tiles(self):
         return int(_check_status(lib().silt_core_near_tiles(self._c)))
+
+    def set_ground(self, bed, placed=None):
+        b = np.ascontiguousarray(bed, np.float32)
+        p = None if placed is None else np.ascontiguousarray(placed, np.float32)
+        _check_status(lib().silt_core_set_ground(self._c, _ptr(b), None if p is None else _ptr(p)))
+
+    def stamp_ruts(self, contacts, dt):
+        L = _rut_lib()
+        if not getattr(L, "_core_rut_declared", False):
+            L.silt_core_stamp_ruts.argtypes = [_P, C.POINTER(TireContact), _Z, _F]
+            L.silt_core_stamp_ruts.restype = C.c_int
+            L._core_rut_declared = True
+        arr = (TireContact * max(len(contacts), 1))(*[TireContact(**c) for c in contacts])
+        _check_status(L.silt_core_stamp_ruts(self._c, arr, len(contacts), dt))
+
+    def ground(self):
+        n = WINDOW_RES * WINDOW_RES
+        out = [np.empty(n, np.float32) for _ in range(3)]
+        _check_status(lib().silt_core_get_ground(self._c, *(_ptr(a) for a in out)))
+        return out
+
+
+class TireContact(C.Structure):
+    """silt_tire_contact (include/silt_rut.h)."""
+    _fields_ = [(n, C.c_float) for n in ("x", "z", "hub_y", "forward_x", "forward_z", "right_x", "right_z", "radius",
+                                         "width", "sinkage", "normal", "push", "k_eq", "n")]
+
+
+def _declare_rut(L):
+    L.silt_bekker_load.argtypes = [_F] * 5
+    L.silt_bekker_load.restype = _F
+    L.silt_rut_depth.argtypes = [_F] * 6
+    L.silt_rut_depth.restype = _F
+    L.silt_rut_stamp_device.argtypes = [_P, _P, _P, _Z, _F, _F, _F, C.POINTER(TireContact), _Z, _F, _P]
+    L.silt_rut_stamp_device.restype = C.c_int
+
+
+def _rut_lib():
+    L = lib()
+    if not getattr(L, "_rut_declared", False):
+        _declare_rut(L)
+        L._rut_declared = True
+    return L
+
+
+def bekker_load(sinkage, radius, width, k_eq, n):
+    return _rut_lib().silt_bekker_load(sinkage, radius, width, k_eq, n)
+
+
+def rut_depth(normal, push, radius, width, k_eq, n):
+    return _rut_lib().silt_rut_depth(normal, push, radius, width, k_eq, n)
+
+
+def rut_stamp_device(bed, placed, initial, res, cell, origin, contacts, dt, stream=0):
+    """Stamp contacts (dicts of TireContact fields) into device planes given
+    as integer addresses."""
+    arr = (TireContact * len(contacts))(*[TireContact(**c) for c in contacts])
+    return _check_status(_rut_lib().silt_rut_stamp_device(bed, placed, initial, res, cell, origin[0], origin[1], arr,
+                                                          len(contacts), dt, stream or None))

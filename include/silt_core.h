@@ -16,6 +16,7 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include "silt_rut.h"
 #include "silt_sheet.h"
 
 #ifdef __cplusplus
@@ -91,9 +92,21 @@ int silt_core_set_planes(silt_core *c, const float *water, const float *slurry, 
  * docs/RENDER_CONTRACT.md:35-38) on the 200 x 200 window: film rg32f
  * (water, slurry), flow rgba32f (u, v, speed, turbidity), ground_rut rgba32f
  * (bed + placed, rut, bed, placed).  soil_* (host, 200 x 200) may be NULL:
- * then the sheet bed is used with no placed soil and no rut.  Host outputs. */
+ * then the core's ground (silt_core_set_ground) is published, or, with none
+ * set, the sheet bed with no placed soil and no rut.  Host outputs. */
 int silt_core_publish(silt_core *c, const float *soil_bed, const float *soil_placed, const float *soil_rut,
                       float *ground_rut4, float *film2, float *flow4);
+
+/* The ground under the sheet on the window grid (SoilField bed and placed
+ * soil sampled at the cell centres).  The initial ground is this bed; the
+ * sheet bed becomes bed + placed (pull_substrate, src/world.rs:5511-5537).
+ * placed may be NULL (no placed soil). */
+int silt_core_set_ground(silt_core *c, const float *bed, const float *placed);
+/* Bekker ruts of the contacts (at most 64) cut into that ground
+ * (include/silt_rut.h), then the sheet bed is pulled from it again. */
+int silt_core_stamp_ruts(silt_core *c, const silt_tire_contact *contacts, size_t n, float dt);
+/* The ground planes out: bed, placed, rut = max(initial - bed, 0). */
+int silt_core_get_ground(silt_core *c, float *bed, float *placed, float *rut);
 
 /* Near-field residual tiles alive (FluidCore::near_tiles, near.rs:316-318). */
 long silt_core_near_tiles(silt_core *c);

@@ -359,3 +359,71 @@ def recentre(r, precision=32):
     a = np.ascontiguousarray(r, _dt(precision)).copy()
     getattr(L, "oracle_recentre" + _sfx(precision))(_ptr(a))
     return a
+
+
+# --- Bekker rut coupling (oracle/rut.c) ----------------------------------------
+
+def _declare_rut(L):
+    for sfx, R in (("_f32", C.c_float), ("_f64", C.c_double)):
+        g = lambda n: getattr(L, n + sfx)  # noqa: E731
+        g("oracle_bekker_load").argtypes = [R, R, R, R, R]
+        g("oracle_bekker_load").restype = R
+        g("oracle_sinkage_with").argtypes = [R, R, R, R, R]
+        g("oracle_sinkage_with").restype = R
+        g("oracle_rut_depth_from_force").argtypes = [R, R, R, R, R, R]
+        g("oracle_rut_depth_from_force").restype = R
+        g("oracle_contact_length").argtypes = [R, R]
+        g("oracle_contact_length").restype = R
+        g("oracle_terramech").argtypes = [R, _P, _P]
+        g("oracle_terramech").restype = None
+        g("oracle_softness_of").argtypes = [R, R, R]
+        g("oracle_softness_of").restype = R
+        g("oracle_stamp_tread").argtypes = [_P, _P, _P, C.c_int, R, R, R, _P, R]
+        g("oracle_stamp_tread").restype = R
+
+
+def _rut_lib():
+    L = lib()
+    if not getattr(L, "_rut_declared", False):
+        _declare_rut(L)
+        L._rut_declared = True
+    return L
+
+
+def bekker_load(sinkage, radius, width, k_eq, n, precision=32):
+    return getattr(_rut_lib(), "oracle_bekker_load" + _sfx(precision))(sinkage, radius, width, k_eq, n)
+
+
+def sinkage_with(load, radius, width, k_eq, n, precision=32):
+    return getattr(_rut_lib(), "oracle_sinkage_with" + _sfx(precision))(load, radius, width, k_eq, n)
+
+
+def rut_depth_from_force(normal, push, radius, width, k_eq, n, precision=32):
+    return getattr(_rut_lib(), "oracle_rut_depth_from_force" + _sfx(precision))(normal, push, radius, width, k_eq, n)
+
+
+def terramech(softness, precision=32):
+    dt = _dt(precision)
+    k = np.zeros(1, dt)
+    n = np.zeros(1, dt)
+    getattr(_rut_lib(), "oracle_terramech" + _sfx(precision))(softness, _ptr(k), _ptr(n))
+    return float(k[0]), float(n[0])
+
+
+def softness_of(wetness, loose, water, precision=32):
+    return getattr(_rut_lib(), "oracle_softness_of" + _sfx(precision))(wetness, loose, water)
+
+
+CONTACT_FIELDS = ("x", "z", "hub_y", "forward_x", "forward_z", "right_x", "right_z", "radius", "width", "sinkage",
+                  "normal", "push", "k_eq", "n")
+
+
+def stamp_tread(bed, placed, initial, res, cell, origin, contact, dt, precision=32):
+    """stamp_wheel's loaded lane tread branch on numpy planes, in place;
+    returns the cut volume (m^3)."""
+    dtype = _dt(precision)
+    for a in (bed, placed, initial):
+        _check(a, dtype, res * res, "plane")
+    cv = np.array([contact[k] for k in CONTACT_FIELDS], dtype)
+    return getattr(_rut_lib(), "oracle_stamp_tread" + _sfx(precision))(_ptr(bed), _ptr(placed), _ptr(initial), res,
+                                                                         cell, origin[0], origin[1], _ptr(cv), dt)
