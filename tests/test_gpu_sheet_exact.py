@@ -10,6 +10,7 @@ import pytest
 
 import pyoracle as O
 import silt_amd as S
+from hipmem import DeviceArray, synchronize
 from silt_amd import scenes
 
 pytestmark = pytest.mark.gpu
@@ -137,19 +138,18 @@ def test_wind_plane_matches_oracle():
 
 
 def test_device_pointer_entry_matches_host_entry():
-    torch = pytest.importorskip("torch")
     sc = scenes.window_scene(seed=4)
     res, cell = sc["res"], sc["cell"]
     n = res * res
     nu = np.full(n, 0.9, np.float32)
     gate = np.ones(n, np.uint8)
     host = [sc["water"].copy(), np.zeros(n, np.float32), np.zeros(n, np.float32)]
-    dev = [torch.from_numpy(a.copy()).cuda() for a in host]
-    bed_d = torch.from_numpy(sc["bed"]).cuda()
+    dev = [DeviceArray(a) for a in host]
+    bed_d = DeviceArray(sc["bed"])
     for _ in range(5):
         S.step_sheet(*host, sc["bed"], nu, gate, res, cell, 1.0 / 120.0, ("gravity", 0.08))
-        S.step_sheet_device(*(t.data_ptr() for t in dev), bed_d.data_ptr(), 0, 0, res, cell, 1.0 / 120.0,
+        S.step_sheet_device(*(d.ptr for d in dev), bed_d.ptr, 0, 0, res, cell, 1.0 / 120.0,
                             ("gravity", 0.08), nu_uniform=0.9)
-    torch.cuda.synchronize()
-    for a, t in zip(host, dev):
-        assert np.array_equal(a, t.cpu().numpy())
+    synchronize()
+    for a, d in zip(host, dev):
+        assert np.array_equal(a, d.numpy())
