@@ -708,6 +708,25 @@ int silt_core_publish(silt_core* c, const float* soil_bed, const float* soil_pla
   return SILT_OK;
 }
 
+int silt_core_fine_water_debug(silt_core* c, const silt_tire_push* pushes, size_t n, float* water, float* uw, float* vw,
+                               float* origin2) {
+  if (!c || (!pushes && n) || !water || !origin2 || n > (size_t)kMaxPush) return SILT_EINVAL;
+  std::vector<CorePush> p;
+  for (size_t k = 0; k < n; k++)
+    p.push_back(CorePush{pushes[k].x, pushes[k].z, pushes[k].forward_x, pushes[k].forward_z, pushes[k].radius,
+                         pushes[k].width, pushes[k].speed, pushes[k].slip});
+  int rc = refine(c, p, 0.0f);
+  if (rc) return rc;
+  size_t fb = (size_t)FN * sizeof(float);
+  HIPCHK(hipMemcpyAsync(water, c->f.water, fb, hipMemcpyDeviceToHost, c->st));
+  if (uw) HIPCHK(hipMemcpyAsync(uw, c->f.uw, fb, hipMemcpyDeviceToHost, c->st));
+  if (vw) HIPCHK(hipMemcpyAsync(vw, c->f.vw, fb, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  origin2[0] = c->fine_origin[0];
+  origin2[1] = c->fine_origin[1];
+  return restrict_fine(c);
+}
+
 long silt_core_near_tiles(silt_core* c) {
   if (!c) return SILT_EINVAL;
   unsigned n = 0;

@@ -252,9 +252,15 @@ __global__ void k_fill(float* p, float val, int n) {
 
 // ---- near field tiles (src/fluid_core/near.rs) ------------------------------
 
+// World keys are stored with the sign bit of each half flipped, so the keys
+// of the cells around the world origin ((-1, -1), (-1, -2)) do not land on
+// the all-ones sentinels.  The two keys that still do (kx = 2^31 - 1,
+// kz = 2^31 - 1 or 2^31 - 2, some 430,000 km out) keep no tile.
 __device__ inline unsigned long long pack_key(int kx, int kz) {
-  return ((unsigned long long)(unsigned)kx << 32) | (unsigned long long)(unsigned)kz;
+  return ((unsigned long long)((unsigned)kx ^ 0x80000000u) << 32) | (unsigned long long)((unsigned)kz ^ 0x80000000u);
 }
+
+__device__ inline bool reserved_key(unsigned long long key) { return key == kEmptyKey || key == kTombKey; }
 
 __device__ inline unsigned tile_hash(int kx, int kz, unsigned cap) {
   unsigned h = (unsigned)kx * 0x9E3779B1u ^ ((unsigned)kz * 0x85EBCA6Bu + 0x27d4eb2fu);
@@ -264,6 +270,7 @@ __device__ inline unsigned tile_hash(int kx, int kz, unsigned cap) {
 
 __device__ int tile_find(const NearTable& t, int kx, int kz) {
   unsigned long long key = pack_key(kx, kz);
+  if (reserved_key(key)) return -1;
   unsigned s = tile_hash(kx, kz, t.cap);
   for (unsigned n = 0; n < t.cap; n++, s = (s + 1) & (t.cap - 1)) {
     unsigned long long k = t.keys[s];
@@ -277,6 +284,7 @@ __device__ int tile_insert(NearTable& t, int kx, int kz) {
   int found = tile_find(t, kx, kz);
   if (found >= 0) return found;
   unsigned long long key = pack_key(kx, kz);
+  if (reserved_key(key)) return -1;
   unsigned s = tile_hash(kx, kz, t.cap);
   for (unsigned n = 0; n < t.cap; n++, s = (s + 1) & (t.cap - 1)) {
     unsigned long long k = t.keys[s];
@@ -449,7 +457,7 @@ __global__ void k_near_rehash(NearTable from, NearTable to) {
   if (s >= from.cap) return;
   unsigned long long k = from.keys[s];
   if (k == kEmptyKey || k == kTombKey) return;
-  int kx = (int)(unsigned)(k >> 32), kz = (int)(unsigned)(k & 0xffffffffull);
+  int kx = (int)((unsigned)(k >> 32) ^ 0x80000000u), kz = (int)((unsigned)(k & 0xffffffffull) ^ 0x80000000u);
   int d = tile_insert(to, kx, kz);
   if (d < 0) return;
   for (int q = 0; q < 400; q++) to.data[(size_t)d * 400 + q] = from.data[(size_t)s * 400 + q];

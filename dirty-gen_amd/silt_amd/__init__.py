@@ -257,6 +257,8 @@ def _declare_core(L):
         getattr(L, name).argtypes = at
         getattr(L, name).restype = C.c_int
     L.silt_core_near_tiles.argtypes = [_P]
+    L.silt_core_fine_water_debug.argtypes = [_P, C.POINTER(TirePush), _Z, _P, _P, _P, _P]
+    L.silt_core_fine_water_debug.restype = C.c_int
     L.silt_core_near_tiles.restype = C.c_long
     L.silt_core_set_ground.argtypes = [_P, _P, _P]
     L.silt_core_set_ground.restype = C.c_int
@@ -363,6 +365,17 @@ class FluidCore:
 
     def near_tiles(self):
         return int(_check_status(lib().silt_core_near_tiles(self._c)))
+
+    def fine_water_debug(self, pushes):
+        """fine_water_debug (src/fluid_core/near.rs:341-346): the 2 cm water,
+        u, v and origin of the patch laid under ``pushes`` (dicts of
+        push_tire's fields); the patch is restricted again afterwards."""
+        arr = (TirePush * max(len(pushes), 1))(*[TirePush(**q) for q in pushes])
+        fw, fu, fv = (np.zeros(160 * 160, np.float32) for _ in range(3))
+        o = np.zeros(2, np.float32)
+        _check_status(lib().silt_core_fine_water_debug(self._c, arr, len(pushes), _ptr(fw), _ptr(fu), _ptr(fv),
+                                                       _ptr(o)))
+        return fw, fu, fv, o
 
     def set_ground(self, bed, placed=None):
         b = np.ascontiguousarray(bed, np.float32)

@@ -108,6 +108,12 @@ int silt_core_stamp_ruts(silt_core *c, const silt_tire_contact *contacts, size_t
 /* The ground planes out: bed, placed, rut = max(initial - bed, 0). */
 int silt_core_get_ground(silt_core *c, float *bed, float *placed, float *rut);
 
+/* fine_water_debug (src/fluid_core/near.rs:341-346): lay the 2 cm patch
+ * under n pushes with dt = 0, copy its water (and, when not NULL, its u and v;
+ * 160 x 160 each) and its world origin (out2), then restrict it again. */
+int silt_core_fine_water_debug(silt_core *c, const silt_tire_push *pushes, size_t n, float *water, float *uw,
+                               float *vw, float *origin2);
+
 /* Near-field residual tiles alive (FluidCore::near_tiles, near.rs:316-318). */
 long silt_core_near_tiles(silt_core *c);
 

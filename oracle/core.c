@@ -838,6 +838,26 @@ void F(oracle_core_fine_get)(Core *c, real *fw, real *origin2) {
     origin2[1] = c->fine_origin[1];
 }
 
+/* fine_water_debug (near.rs:341-346): lay the patch under the given pushes
+ * with dt = 0, read it, then restrict it again.  fuw and fvw may be NULL. */
+void F(oracle_core_fine_water_debug)(Core *c, const real *p8, int np, real *fw, real *fuw, real *fvw,
+                                     real *origin2) {
+    Push q[MAX_PUSH];
+    if (np > MAX_PUSH) np = MAX_PUSH;
+    for (int k = 0; k < np; k++) {
+        const real *p = p8 + 8 * k;
+        Push one = {p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7]};
+        q[k] = one;
+    }
+    refine(c, q, np, K(0.0));
+    memcpy(fw, c->fw, FN * sizeof(real));
+    if (fuw) memcpy(fuw, c->fuw, FN * sizeof(real));
+    if (fvw) memcpy(fvw, c->fvw, FN * sizeof(real));
+    origin2[0] = c->fine_origin[0];
+    origin2[1] = c->fine_origin[1];
+    restrict_fine(c);
+}
+
 void F(oracle_core_push_tire)(Core *c, const real *p) {
     if (c->npush >= MAX_PUSH) return;
     Push q = {p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7]};

@@ -197,7 +197,8 @@ def _declare_core(L):
                       ("oracle_core_set", [_P, _P]), ("oracle_core_origin", [_P, _P]),
                       ("oracle_core_ledger", [_P, _P]), ("oracle_core_publish", [_P, _P, _P]),
                       ("oracle_core_refine", [_P, R]), ("oracle_core_restrict", [_P]),
-                      ("oracle_core_fine_get", [_P, _P, _P]), ("oracle_recentre", [_P])):
+                      ("oracle_core_fine_get", [_P, _P, _P]), ("oracle_recentre", [_P]),
+                      ("oracle_core_fine_water_debug", [_P, _P, C.c_int, _P, _P, _P, _P])):
             g(n).argtypes = at
             g(n).restype = None
         g("oracle_core_step").argtypes = [_P, R, _P]
@@ -324,6 +325,16 @@ class Core:
 
     def restrict(self):
         self._f("oracle_core_restrict")(self._c)
+
+    def fine_water_debug(self, pushes):
+        """fine_water_debug (near.rs:341-346): 2 cm water, u, v and origin of
+        the patch laid under ``pushes`` (dicts of push_tire's fields)."""
+        keys = ("x", "z", "forward_x", "forward_z", "radius", "width", "speed", "slip")
+        p = np.array([[q[k] for k in keys] for q in pushes], self.dtype).reshape(-1)
+        fw, fu, fv = (np.zeros(FINE_RES * FINE_RES, self.dtype) for _ in range(3))
+        o = np.zeros(2, self.dtype)
+        self._f("oracle_core_fine_water_debug")(self._c, _ptr(p), len(pushes), _ptr(fw), _ptr(fu), _ptr(fv), _ptr(o))
+        return fw, fu, fv, o
 
     def fine_water(self):
         fw = np.zeros(FINE_RES * FINE_RES, self.dtype)

@@ -93,6 +93,27 @@ def test_core_fine_pace3_and_still_tire_in_deep_water():
     assert abs(after - before) < 1e-3 * before
 
 
+def test_fine_patch_after_near_apply_matches_oracle():
+    """fine_water_debug (src/fluid_core/near.rs:341-346) after steps with a
+    tire at the world origin: the patch laid from the coarse sheet plus the
+    near-field residuals equals the oracle's, including the columns whose
+    world keys are (-1, -1) and (-1, -2)."""
+    g, o = _pair()
+    for c in (g, o):
+        c.pour_water(-6.0, 6.0, -6.0, 6.0, 0.40)
+    for k in range(3):
+        t = dict(x=0.0, z=0.08 * k, forward_x=0.0, forward_z=1.0, radius=0.38, width=0.28, speed=3.0, slip=0.2)
+        g.push_tire(**t)
+        o.push_tire(**t)
+        g.step(1.0 / 30.0)
+        o.step(1.0 / 30.0)
+        assert g.near_tiles() == o.near_tiles()
+        fg, fo = g.fine_water_debug([t]), o.fine_water_debug([t])
+        for a, b, name in zip(fg, fo, ("water", "uw", "vw", "origin")):
+            _eq(a, b, f"fine {name} after step {k}")
+        _compare(g, o, k)
+
+
 def test_core_publish_matches_oracle():
     g, o = _pair()
     for c in (g, o):

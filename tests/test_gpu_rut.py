@@ -40,7 +40,7 @@ def test_rut_stamp_matches_oracle_step_by_step():
         synchronize()
         assert np.array_equal(db.numpy(), ob), f"bed differs at step {k}"
         assert np.array_equal(dp.numpy(), op), f"placed differs at step {k}"
-    assert ob.min() < bed.min() - 0.01  # the tires cut a rut
+    assert (init - ob).max() > 0.01  # the tires cut a rut
 
 
 def test_core_ruts_lower_the_sheet_bed_and_publish():
