@@ -18,14 +18,22 @@
 //   S11 open transport along x -> h1                    row r-10
 //   S12 open transport along z, dry cut, store          row r-11
 //
-// Horizontal neighbours go through LDS row rings (one barrier between a
-// producer and a consumer); vertical neighbours of a lane's own column are
-// register windows or the same rings.  The reference clamps every stencil at
-// the grid edge; here the producer of a row writes its edge values into ghost
-// columns and ghost rows, so every read is a fixed offset.  Work boxes enter as
-// masks: filter planes are zero outside the boxes, forces act on wet box cells,
-// transport faces exist between cells of one box, and the dry cut runs inside
-// the boxes (the scratch convention of the exact path).
+// A value a later stage reads across lanes goes through LDS: one row for a
+// value read in the iteration that writes it, a two-slot ring (slot = row
+// parity) for one read in the next iteration.  The row loop is unrolled by
+// two, so the parity and with it every LDS offset is a compile-time constant.
+// A value a later stage reads only in its own column (the vertical
+// neighbours of every z difference, the lags between stages) stays in
+// registers: each lane keeps a short window of the rows it produced.
+//
+// The reference clamps every stencil at the grid edge.  Columns: the lane of
+// the first or last grid column also writes its value into two ghost columns.
+// Rows: S1 loads clamped rows, so its windows repeat the edge row; windows of
+// later stages select the centre value for the missing neighbour of the first
+// and last grid row.  Work boxes enter as masks: filter planes are zero
+// outside the boxes, forces act on wet box cells, transport faces exist
+// between cells of one box, and the dry cut runs inside the boxes (the
+// scratch convention of the exact path).
 //
 // EXACT = true keeps the reference's operation order with no fused
 // multiply-add and IEEE division, and reproduces the exact path bit for bit;
@@ -34,6 +42,7 @@
 
 #include <math.h>
 
+#include <type_traits>
 #include <vector>
 
 namespace silt {
@@ -48,24 +57,14 @@ constexpr int kLag = 11;                        // S12 trails S1 by 11 rows
 constexpr int kRow = 264;                       // LDS row stride (256 + ghosts)
 constexpr int kMaxBoxes = 8;
 
-// Ring depths (rows) and their offsets in the LDS row pool.
-constexpr int D_ETA = 5, D_LAPE = 7, D_P = 4, D_UG = 4, D_VG = 4, D_W = 4, D_UC = 5, D_VC = 6, D_B = 1, D_Q = 4,
-              D_UF = 2, D_HX = 1, D_GT = 4;
-constexpr int O_ETA = 0;
-constexpr int O_LAPE = O_ETA + D_ETA;
-constexpr int O_P = O_LAPE + D_LAPE;
-constexpr int O_UG = O_P + D_P;
-constexpr int O_VG = O_UG + D_UG;
-constexpr int O_W = O_VG + D_VG;
-constexpr int O_UC = O_W + D_W;
-constexpr int O_VC = O_UC + D_UC;
-constexpr int O_B = O_VC + D_VC;
-constexpr int O_Q = O_B + D_B;
-constexpr int O_UF = O_Q + D_Q;
-constexpr int O_HX = O_UF + D_UF;
-constexpr int O_GT = O_HX + D_HX;
-constexpr int kRows = O_GT + D_GT;
-constexpr int kBias = 420 * 16;  // multiple of every depth, keeps row + bias >= 0
+// LDS rows.  Two-slot rings: eta, p, vg, w, uc, q.  Single rows: b, uf, h, gate.
+constexpr int R_ETA = 0, R_P = 2, R_VG = 4, R_W = 6, R_UC = 8, R_Q = 10, R_B = 12, R_UF = 13, R_HX = 14,
+              R_GT = 15;
+constexpr int kRows = 16;
+
+// Ring slot of row r - k when r has parity P.
+template <int P, int K>
+constexpr int kSlot = ((P - K) % 2 + 2) % 2;
 
 struct BoxList {
   int n;
@@ -119,12 +118,18 @@ struct Ar {
   }
 };
 
-__device__ __forceinline__ int slot(int row, int depth) { return (row + kBias) % depth; }
+// Shift a register window one row on: w[0] drops, w[N-1] = v.
+template <int N>
+__device__ __forceinline__ void push(float (&w)[N], float v) {
+#pragma unroll
+  for (int k = 0; k < N - 1; k++) w[k] = w[k + 1];
+  w[N - 1] = v;
+}
 
 }  // namespace
 
 template <bool EXACT, bool PROJECT, bool WIND>
-__global__ __launch_bounds__(kLanes) void k_sheet_stream(FusedArgs fa) {
+__global__ __launch_bounds__(kLanes, 4) void k_sheet_stream(FusedArgs fa) {
   using A = Ar<EXACT>;
   __shared__ float lds[kRows * kRow];
 
@@ -150,11 +155,13 @@ __global__ __launch_bounds__(kLanes) void k_sheet_stream(FusedArgs fa) {
   const bool col_in = col >= 0 && col < res;
   const int colc = col < 0 ? 0 : (col >= res ? res - 1 : col);
   const bool left_edge = col == 0, right_edge = col == res - 1;
-  const int li = t + 2;  // this lane's LDS column
+  // Strips that reach past a grid side carry ghost columns and idle lanes.
+  const bool edge_strip = x0 - kHaloL < 0 || x0 - kHaloL + kLanes > res;
+  float* const my = lds + (t + 2);  // this lane's LDS column
   const float cell = a.cell;
   const float eps = a.eps;
 
-  // Box membership: bit k set when box k covers the column.
+  // Box membership: bit k set when box k covers the column / row.
   auto colbits = [&](int c) -> unsigned {
     unsigned m = 0;
     for (int k = 0; k < L.boxes.n; k++)
@@ -170,38 +177,28 @@ __global__ __launch_bounds__(kLanes) void k_sheet_stream(FusedArgs fa) {
     return m;
   };
 
-  auto ring = [&](int base, int depth, int row) -> float* { return lds + (base + slot(row, depth)) * kRow; };
-  // Store one lane value of row y into a ring, with the clamp ghosts.
-  auto put = [&](int base, int depth, int y, float val, bool ghost_rows) {
-    if (y < 0 || y >= res || !col_in) return;
-    float* r = ring(base, depth, y);
-    r[li] = val;
-    if (left_edge) {
-      r[li - 1] = val;
-      r[li - 2] = val;
-    }
-    if (right_edge) {
-      r[li + 1] = val;
-      r[li + 2] = val;
-    }
-    if (ghost_rows) {
-      if (y == 0) {
-        float* g = ring(base, depth, -1);
-        g[li] = val;
-        if (left_edge) g[li - 1] = g[li - 2] = val;
-        if (right_edge) g[li + 1] = g[li + 2] = val;
+  // Store this lane's value in LDS row `row` (with the edge ghosts).
+  auto put = [&](int row, float val) {
+    float* p = my + row * kRow;
+    if (!edge_strip) {
+      *p = val;
+    } else if (col_in) {
+      *p = val;
+      if (left_edge) {
+        p[-1] = val;
+        p[-2] = val;
       }
-      if (y == res - 1) {
-        float* g = ring(base, depth, res);
-        g[li] = val;
-        if (left_edge) g[li - 1] = g[li - 2] = val;
-        if (right_edge) g[li + 1] = g[li + 2] = val;
+      if (right_edge) {
+        p[1] = val;
+        p[2] = val;
       }
     }
   };
-  auto rowptr = [&](const float* p, int y) -> const float* {
+  auto at = [&](int row, int d) -> float { return my[row * kRow + d]; };
+  auto load = [&](const float* p, int y) -> float {
     int yc = y < 0 ? 0 : (y >= res ? res - 1 : y);
-    return p + (size_t)yc * res;
+    const float* rp = p + (size_t)yc * res;
+    return rp[colc];
   };
 
   const float inv2c = L.inv2c;
@@ -235,13 +232,30 @@ __global__ __launch_bounds__(kLanes) void k_sheet_stream(FusedArgs fa) {
     s = A::fma_(pick(2), k[4], s);
     return s;
   };
+  // Neighbours of row y in a window holding rows y-1, y, y+1 at m, c, p:
+  // the reference clamps the stencil at the first and last grid row.
+  auto below = [&](int y, float m, float c) -> float { return y == 0 ? c : m; };
+  auto above = [&](int y, float c, float p) -> float { return y == res - 1 ? c : p; };
 
-  // Register windows: x-filtered eta and b, rows y-2..y+2 of their z filter;
-  // transported depth h1 and final vf, rows y-2..y+1 of the z transport.
-  float gx[5] = {0, 0, 0, 0, 0}, sx[5] = {0, 0, 0, 0, 0};
-  float gbx[5] = {0, 0, 0, 0, 0}, sbx[5] = {0, 0, 0, 0, 0};
-  float h1w[4] = {0, 0, 0, 0}, vfw[4] = {0, 0, 0, 0};
-  unsigned rmw[4] = {0, 0, 0, 0};  // row box masks for rows y-2..y+1
+  // Register windows, oldest row first (rows relative to the iteration's r
+  // after its stage has run):
+  float E[4] = {0, 0, 0, 0};                               // eta      r-3 .. r
+  float GX[5] = {0, 0, 0, 0, 0}, SX[5] = {0, 0, 0, 0, 0};  // x filters of eta  r-5 .. r-1
+  float LW[7] = {0, 0, 0, 0, 0, 0, 0};                     // lap(eta) r-7 .. r-1
+  float PW[3] = {0, 0, 0};                                 // p        r-5 .. r-3
+  float UG[3] = {0, 0, 0}, VG[3] = {0, 0, 0};              // snapshot r-6 .. r-4
+  float WW[3] = {0, 0, 0};                                 // w        r-7 .. r-5
+  float UC[5] = {0, 0, 0, 0, 0}, VC[5] = {0, 0, 0, 0, 0};  // confined r-10 .. r-6
+  float GB[5] = {0, 0, 0, 0, 0}, SB[5] = {0, 0, 0, 0, 0};  // x filters of b    r-11 .. r-7
+  float QW[3] = {0, 0, 0};                                 // q        r-11 .. r-9
+  float UF[2] = {0, 0};                                    // final u  r-11 .. r-10
+  float VF[4] = {0, 0, 0, 0};                              // final v  r-13 .. r-10
+  float H1[4] = {0, 0, 0, 0};                              // x-moved h r-13 .. r-10
+  float GW[4] = {1, 1, 1, 1};                              // gate     r-13 .. r-10
+  unsigned RM[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};  // RM[k] = row mask of r-k
+  // z fluxes of the edge (y, y+1) of the previous S12 row y, which is the
+  // edge (y-1, y) of the next: up-going and down-going.
+  float c_fu_0 = 0.0f, c_fd_0 = 0.0f;
 
   // Epilogue accumulators over the cells this workgroup stores: FP64 volume,
   // dry-cut volume and transport residue (what FP32 rounding and the max(0)
@@ -250,81 +264,94 @@ __global__ __launch_bounds__(kLanes) void k_sheet_stream(FusedArgs fa) {
   const bool want_stats = L.stats != nullptr;
   double acc_vol = 0.0, acc_cut = 0.0, acc_res = 0.0;
   unsigned acc_wet = 0;
-  int acc_i0 = 0x7fffffff, acc_i1 = -1, acc_j0 = 0x7fffffff, acc_j1 = -1;
+  int acc_j0 = 0x7fffffff, acc_j1 = -1;
   float acc_s2 = 0.0f;
   const bool out_col = t >= kHaloL && t < kHaloL + kOutW && col_in;
 
-  const int r_begin = y0 - kHaloL;
-  const int r_end = y1 - 1 + kLag;
   const bool nu_uni = a.nu_uniform != 0;
   const bool gate_uni = a.gate_uniform != 0;
+  const bool vort = a.vort > 0.0f;
 
-  for (int r = r_begin; r <= r_end; r++) {
+  // Global loads run two rows ahead of their use: the loads of row r + 2
+  // are issued while row r computes, into the registers of r's parity.
+  //   pf_h, pf_b   h and bed of S1's row         (r)
+  //   pf_h4, pf_u4, pf_v4   h, u, v of S4's row  (r-4)
+  //   pf_h10       h of S10's row                (r-10)
+  // S6 reads h of row r-6, which S4 loaded two rows earlier (h6).
+  float pf_h[2], pf_b[2], pf_h4[2], pf_u4[2], pf_v4[2], pf_h10[2], h6[2] = {0.0f, 0.0f};
+  auto prefetch = [&](int q, int r) {
+    pf_h[q] = load(L.h, r);
+    pf_b[q] = load(fa.bed, r);
+    pf_h4[q] = load(L.h, r - 4);
+    pf_u4[q] = load(L.u, r - 4);
+    pf_v4[q] = load(L.v, r - 4);
+    pf_h10[q] = load(L.h, r - 10);
+  };
+
+  auto row = [&](auto par, int r) {
+    constexpr int P = decltype(par)::value;
+#pragma unroll
+    for (int k = 11; k > 1; k--) RM[k] = RM[k - 1];
+    RM[1] = rowmask(r - 1);
+
+    const float h4 = pf_h4[P], u4 = pf_u4[P], v4 = pf_v4[P], h10 = pf_h10[P];
     // ---- S1: eta at row r ------------------------------------------------
     {
-      const int y = r;
-      float hv = rowptr(L.h, y)[colc];
-      float bv = rowptr(fa.bed, y)[colc];
-      put(O_ETA, D_ETA, y, bv + hv, true);
+      const float ev = pf_b[P] + pf_h[P];
+      prefetch(P, r + 2);
+      put(R_ETA + kSlot<P, 0>, ev);
+      push(E, ev);
     }
     __syncthreads();
     // ---- S2: x filters of eta, lap(eta) at row r-1 -------------------------
     {
-      const int y = r - 1;
-      const float* e = ring(O_ETA, D_ETA, y);
+      constexpr int RE = R_ETA + kSlot<P, 1>;
       float g = 0.0f, s = 0.0f;
-      const unsigned rm = rowmask(y);
       if (!WIND) {
-        float e0 = e[li - 2], e1 = e[li - 1], e2 = e[li], e3 = e[li + 1], e4 = e[li + 2];
-        bool inside = (colmask & rm) != 0;
+        float e0 = at(RE, -2), e1 = at(RE, -1), e2 = at(RE, 0), e3 = at(RE, 1), e4 = at(RE, 2);
+        bool inside = (colmask & RM[1]) != 0;
         // accum5 (interior) and accum5_clamped (edges) differ only in a
         // leading 0.0 +, which leaves the value unchanged.
         g = inside ? acc5(e0, e1, e2, e3, e4, kG) : 0.0f;
         s = inside ? acc5(e0, e1, e2, e3, e4, kS) : 0.0f;
       }
-      for (int k = 0; k < 4; k++) {
-        gx[k] = gx[k + 1];
-        sx[k] = sx[k + 1];
-      }
-      gx[4] = g;
-      sx[4] = s;
+      push(GX, g);
+      push(SX, s);
       if (PROJECT) {
-        const float* en = ring(O_ETA, D_ETA, y + 1);
-        const float* ep = ring(O_ETA, D_ETA, y - 1);
-        float sum = e[li + 1] + e[li - 1] + en[li] + ep[li] - 4.0f * e[li];
-        float lp = A::div(sum, cell * cell, L.invc2);
-        put(O_LAPE, D_LAPE, y, lp, false);
+        // Rows r and r-2 of this column: E[3], E[1] (S1 loaded clamped rows).
+        float sum = at(RE, 1) + at(RE, -1) + E[3] + E[1] - 4.0f * E[2];
+        push(LW, A::div(sum, cell * cell, L.invc2));
       }
     }
     // ---- S3: pressure at row r-3 ------------------------------------------
     if (!WIND) {
       const int y = r - 3;
       float pv = 0.0f;
-      if ((colmask & rowmask(y)) != 0) {
-        float gz = zacc(gx, y, kG), sz = zacc(sx, y, kS);
+      if ((colmask & RM[3]) != 0) {
+        float gz = zacc(GX, y, kG), sz = zacc(SX, y, kS);
         float filtered = A::fma_(0.38f, sz, 0.62f * gz);
-        float ev = ring(O_ETA, D_ETA, y)[li];
-        pv = A::fma_(0.55f, ev, 0.45f * filtered);
+        pv = A::fma_(0.55f, E[0], 0.45f * filtered);
       }
-      put(O_P, D_P, y, pv, true);
+      put(R_P + kSlot<P, 3>, pv);
+      push(PW, pv);
     }
     __syncthreads();
     // ---- S4: gravity / wind blend at row r-4 -> snapshot ------------------
     {
       const int y = r - 4;
-      float hv = rowptr(L.h, y)[colc];
-      float uu = rowptr(L.u, y)[colc];
-      float vv = rowptr(L.v, y)[colc];
-      bool inside = (colmask & rowmask(y)) != 0;
+      float hv = h4;
+      float uu = u4;
+      float vv = v4;
+      bool inside = (colmask & RM[4]) != 0;
       if (inside && hv > eps) {
         if (WIND) {
           uu = A::fma_(a.wx - uu, a.blend, uu);
           vv = A::fma_(a.wz - vv, a.blend, vv);
         } else {
-          const float* pr = ring(O_P, D_P, y);
-          float px = A::div(pr[li + 1] - pr[li - 1], two, inv2c);
-          float pz = A::div(ring(O_P, D_P, y + 1)[li] - ring(O_P, D_P, y - 1)[li], two, inv2c);
-          float mob = nu_uni ? a.mob0 : sheet_mobility(rowptr(L.nu, y)[colc]);
+          constexpr int RP = R_P + kSlot<P, 4>;
+          float px = A::div(at(RP, 1) - at(RP, -1), two, inv2c);
+          float pz = A::div(above(y, PW[1], PW[2]) - below(y, PW[0], PW[1]), two, inv2c);
+          float mob = nu_uni ? a.mob0 : sheet_mobility(load(L.nu, y));
           if (EXACT) {
             uu = uu + -kGravity * px * mob * a.dt;
             vv = vv + -kGravity * pz * mob * a.dt;
@@ -335,33 +362,36 @@ __global__ __launch_bounds__(kLanes) void k_sheet_stream(FusedArgs fa) {
           }
         }
       }
-      put(O_UG, D_UG, y, uu, true);
-      put(O_VG, D_VG, y, vv, true);
+      put(R_VG + kSlot<P, 4>, vv);
+      push(UG, uu);
+      push(VG, vv);
     }
     __syncthreads();
     // ---- S5: vorticity of the snapshot at row r-5 -------------------------
-    if (a.vort > 0.0f) {
+    if (vort) {
       const int y = r - 5;
-      const float* vg = ring(O_VG, D_VG, y);
-      float dvdx = A::div(vg[li + 1] - vg[li - 1], two, inv2c);
-      float dudz = A::div(ring(O_UG, D_UG, y + 1)[li] - ring(O_UG, D_UG, y - 1)[li], two, inv2c);
-      put(O_W, D_W, y, dvdx - dudz, true);
+      constexpr int RV = R_VG + kSlot<P, 5>;
+      float dvdx = A::div(at(RV, 1) - at(RV, -1), two, inv2c);
+      float dudz = A::div(above(y, UG[1], UG[2]) - below(y, UG[0], UG[1]), two, inv2c);
+      float w = dvdx - dudz;
+      put(R_W + kSlot<P, 5>, w);
+      push(WW, w);
     }
     __syncthreads();
     // ---- S6: confinement at row r-6 ---------------------------------------
     {
       const int y = r - 6;
-      float uu = ring(O_UG, D_UG, y)[li];
-      float vv = ring(O_VG, D_VG, y)[li];
-      if (a.vort > 0.0f) {
-        float hv = rowptr(L.h, y)[colc];
-        bool inside = (colmask & rowmask(y)) != 0;
+      float uu = UG[0];
+      float vv = VG[0];
+      if (vort) {
+        float hv = h6[P];
+        bool inside = (colmask & RM[6]) != 0;
         if (inside && hv > eps) {
-          const float* wr = ring(O_W, D_W, y);
-          float w = wr[li];
+          float w = WW[1];
           if (isfinite(w) && !(fabsf(w) < 1.0e-4f)) {
-            float wxp = fabsf(wr[li + 1]), wxm = fabsf(wr[li - 1]);
-            float wzp = fabsf(ring(O_W, D_W, y + 1)[li]), wzm = fabsf(ring(O_W, D_W, y - 1)[li]);
+            constexpr int RW = R_W + kSlot<P, 6>;
+            float wxp = fabsf(at(RW, 1)), wxm = fabsf(at(RW, -1));
+            float wzp = fabsf(above(y, WW[1], WW[2])), wzm = fabsf(below(y, WW[0], WW[1]));
             float nx = A::div(wxp - wxm, two, inv2c);
             float nz = A::div(wzp - wzm, two, inv2c);
             float du, dv;
@@ -379,64 +409,59 @@ __global__ __launch_bounds__(kLanes) void k_sheet_stream(FusedArgs fa) {
           }
         }
       }
-      put(O_UC, D_UC, y, uu, false);
-      put(O_VC, D_VC, y, vv, true);
+      put(R_UC + kSlot<P, 6>, uu);
+      push(UC, uu);
+      push(VC, vv);
+      h6[P] = h4;  // S6's row two rows on
     }
     __syncthreads();
     // ---- S7: divergence source at row r-7 ---------------------------------
     if (PROJECT) {
       const int y = r - 7;
       float bv = 0.0f;
-      if ((colmask & rowmask(y)) != 0) {
-        const float* ucr = ring(O_UC, D_UC, y);
-        float ux = A::div(ucr[li + 1] - ucr[li - 1], two, inv2c);
-        float vz = A::div(ring(O_VC, D_VC, y + 1)[li] - ring(O_VC, D_VC, y - 1)[li], two, inv2c);
+      if ((colmask & RM[7]) != 0) {
+        constexpr int RU = R_UC + kSlot<P, 7>;
+        float ux = A::div(at(RU, 1) - at(RU, -1), two, inv2c);
+        float vz = A::div(above(y, VC[3], VC[4]) - below(y, VC[2], VC[3]), two, inv2c);
         float div = ux + vz;
-        float lp = ring(O_LAPE, D_LAPE, y)[li];
-        float ov = nu_uni ? a.ov0 : overload_of(rowptr(L.nu, y)[colc]);
+        float lp = LW[0];
+        float ov = nu_uni ? a.ov0 : overload_of(load(L.nu, y));
         if (EXACT)
           bv = div * ov - 0.015f * ov * lp;
         else
           bv = ov * __builtin_fmaf(-0.015f, lp, div);
       }
-      put(O_B, D_B, y, bv, false);
+      put(R_B, bv);
     }
     __syncthreads();
     // ---- S8: x filters of b at row r-7; S9: q at row r-9 -------------------
     if (PROJECT) {
-      const int y = r - 7;
-      const float* br = ring(O_B, D_B, y);
-      bool inside = (colmask & rowmask(y)) != 0;
-      float b0 = br[li - 2], b1 = br[li - 1], b2 = br[li], b3 = br[li + 1], b4 = br[li + 2];
-      float g = inside ? acc5(b0, b1, b2, b3, b4, kG) : 0.0f;
-      float s = inside ? acc5(b0, b1, b2, b3, b4, kS) : 0.0f;
-      for (int k = 0; k < 4; k++) {
-        gbx[k] = gbx[k + 1];
-        sbx[k] = sbx[k + 1];
-      }
-      gbx[4] = g;
-      sbx[4] = s;
+      bool inside = (colmask & RM[7]) != 0;
+      float b0 = at(R_B, -2), b1 = at(R_B, -1), b2 = at(R_B, 0), b3 = at(R_B, 1), b4 = at(R_B, 2);
+      push(GB, inside ? acc5(b0, b1, b2, b3, b4, kG) : 0.0f);
+      push(SB, inside ? acc5(b0, b1, b2, b3, b4, kS) : 0.0f);
       const int yq = r - 9;
       float qv = 0.0f;
-      if ((colmask & rowmask(yq)) != 0) {
-        float gz = zacc(gbx, yq, kG), sz = zacc(sbx, yq, kS);
+      if ((colmask & RM[9]) != 0) {
+        float gz = zacc(GB, yq, kG), sz = zacc(SB, yq, kS);
         qv = A::fma_(0.38f, sz, 0.62f * gz);
       }
-      put(O_Q, D_Q, yq, qv, true);
+      put(R_Q + kSlot<P, 9>, qv);
+      push(QW, qv);
     }
     __syncthreads();
     // ---- S10: correction, damping, cap at row r-10 -------------------------
     {
       const int y = r - 10;
-      float uu = ring(O_UC, D_UC, y)[li];
-      float vv = ring(O_VC, D_VC, y)[li];
-      float hv = rowptr(L.h, y)[colc];
-      bool inside = (colmask & rowmask(y)) != 0;
+      float uu = UC[0];
+      float vv = VC[0];
+      float hv = h10;
+      bool inside = (colmask & RM[10]) != 0;
       if (inside && hv > eps) {
         if (PROJECT) {
-          const float* qr = ring(O_Q, D_Q, y);
-          float qx = A::div(qr[li + 1] - qr[li - 1], two, inv2c);
-          float qz = A::div(ring(O_Q, D_Q, y + 1)[li] - ring(O_Q, D_Q, y - 1)[li], two, inv2c);
+          constexpr int RQ = R_Q + kSlot<P, 10>;
+          float qx = A::div(at(RQ, 1) - at(RQ, -1), two, inv2c);
+          float qz = A::div(above(y, QW[1], QW[2]) - below(y, QW[0], QW[1]), two, inv2c);
           if (EXACT) {
             uu = uu + 0.012f * a.dt * qx;
             vv = vv + 0.012f * a.dt * qz;
@@ -445,38 +470,36 @@ __global__ __launch_bounds__(kLanes) void k_sheet_stream(FusedArgs fa) {
             vv = __builtin_fmaf(L.kdt, qz, vv);
           }
         }
-        float decay = nu_uni ? a.decay0 : expf(-sheet_damp(rowptr(L.nu, y)[colc]) * a.dt);
+        float decay = nu_uni ? a.decay0 : expf(-sheet_damp(load(L.nu, y)) * a.dt);
         uu = finite_speed(uu * decay, a.lim);
         vv = finite_speed(vv * decay, a.lim);
       }
-      put(O_UF, D_UF, y, uu, false);
-      // vf stays in this lane: the z transport reads only its own column.
-      for (int k = 0; k < 3; k++) vfw[k] = vfw[k + 1];
-      vfw[3] = vv;
-      put(O_HX, D_HX, y, hv, false);
+      put(R_UF, uu);
+      push(UF, uu);
+      push(VF, vv);  // the z transport reads only this lane's column
+      put(R_HX, hv);
       if (!gate_uni) {
         int yc = y < 0 ? 0 : (y >= res ? res - 1 : y);
-        put(O_GT, D_GT, y, (float)L.gate[(size_t)yc * res + colc], false);
+        float gv = (float)L.gate[(size_t)yc * res + colc];
+        put(R_GT, gv);
+        push(GW, gv);
       }
     }
     __syncthreads();
     // ---- S11: transport along x at row r-10 --------------------------------
-    const int yx = r - 10;
-    const unsigned rmx = rowmask(yx);
     {
-      const float* hr = ring(O_HX, D_HX, yx);
-      const float* ur = ring(O_UF, D_UF, yx);
-      const float* gr = ring(O_GT, D_GT, yx);
-      auto gate_at = [&](int d) -> bool { return gate_uni ? true : gr[li + d] != 0.0f; };
+      const int yx = r - 10;
+      const unsigned rmx = RM[10];
+      auto gate_at = [&](int d) -> bool { return gate_uni ? true : at(R_GT, d) != 0.0f; };
       // Edge (c, c+1) exists when both cells lie in one box.
       const bool e_m2 = (colmask_l2 & colmask_l1 & rmx) != 0;  // edge (col-2, col-1)
       const bool e_m1 = (colmask_l1 & colmask & rmx) != 0;     // edge (col-1, col)
-      const bool e_0 = (colmask & colmask_r & rmx) != 0;  // edge (col, col+1)
+      const bool e_0 = (colmask & colmask_r & rmx) != 0;       // edge (col, col+1)
       const float dtc = L.dtc;
       // Left-going flux of edge (k, k+1) (donor k+1), k = col + d.
       auto fl = [&](int d) -> float {
-        float speed = 0.5f * (ur[li + d] + ur[li + d + 1]);
-        float hd = hr[li + d + 1], hrv = hr[li + d];
+        float speed = 0.5f * (at(R_UF, d) + at(R_UF, d + 1));
+        float hd = at(R_HX, d + 1), hrv = at(R_HX, d);
         bool live = !(fabsf(speed) < 1.0e-5f);
         bool gate_shut = !gate_at(d) && hrv <= 1.0e-6f;
         float frac = fminf(EXACT ? fabsf(speed) * a.dt / cell : fabsf(speed) * dtc, 0.45f);
@@ -487,8 +510,8 @@ __global__ __launch_bounds__(kLanes) void k_sheet_stream(FusedArgs fa) {
       // Right-going flux of edge (k, k+1) (donor k) with the donor's removed
       // sum holding the left-going flux of edge (k-1, k).
       auto fr = [&](int d, float prev) -> float {
-        float speed = 0.5f * (ur[li + d] + ur[li + d + 1]);
-        float hd = hr[li + d], hrv = hr[li + d + 1];
+        float speed = 0.5f * (at(R_UF, d) + at(R_UF, d + 1));
+        float hd = at(R_HX, d), hrv = at(R_HX, d + 1);
         bool live = !(fabsf(speed) < 1.0e-5f);
         float room = hd - (0.0f + prev);
         bool gate_shut = !gate_at(d + 1) && hrv <= 1.0e-6f;
@@ -504,34 +527,25 @@ __global__ __launch_bounds__(kLanes) void k_sheet_stream(FusedArgs fa) {
       float fr_0 = e_0 ? fr(0, fl_m1) : 0.0f;
       float rem = (0.0f + fl_m1) + fr_0;
       float add = (0.0f + fr_m1) + fl_0;
-      float hv = hr[li];
+      float hv = at(R_HX, 0);
       float moved = fmaxf(hv - rem + add, 0.0f);
       float h1 = (rem > 0.0f || add > 0.0f) ? moved : hv;
       if (want_stats && out_col && yx >= y0 && yx < y1)
         acc_res += ((double)h1 - (double)hv) - ((double)add - (double)rem);
-      for (int k = 0; k < 3; k++) {
-        h1w[k] = h1w[k + 1];
-        rmw[k] = rmw[k + 1];
-      }
-      h1w[3] = h1;
-      rmw[3] = rmx;
+      push(H1, h1);
     }
     // ---- S12: transport along z, dry cut, store at row r-11 ----------------
     {
       const int y = r - 11;
-      // Windows hold rows y-2..y+1 at indices 0..3.
-      const float* g_m1 = ring(O_GT, D_GT, y - 1);
-      const float* g_0 = ring(O_GT, D_GT, y);
-      const float* g_p1 = ring(O_GT, D_GT, y + 1);
-      const bool e_m2 = (colmask & rmw[0] & rmw[1]) != 0;  // edge (y-2, y-1)
-      const bool e_m1 = (colmask & rmw[1] & rmw[2]) != 0;  // edge (y-1, y)
-      const bool e_0 = (colmask & rmw[2] & rmw[3]) != 0;   // edge (y, y+1)
+      // Windows hold rows y-2..y+1 at indices 0..3; RM[11], RM[10] are the
+      // masks of rows y, y+1.
+      const bool e_0 = (colmask & RM[11] & RM[10]) != 0;  // edge (y, y+1)
       const float dtc = L.dtc;
-      auto gate_row = [&](const float* g) -> bool { return gate_uni ? true : g[li] != 0.0f; };
+      auto gate_ok = [&](float g) -> bool { return gate_uni ? true : g != 0.0f; };
       // Up-going flux of edge (k, k+1) (donor k+1 = window index k+1).
       auto fu = [&](int k, bool recv_gate) -> float {
-        float speed = 0.5f * (vfw[k] + vfw[k + 1]);
-        float hd = h1w[k + 1], hrv = h1w[k];
+        float speed = 0.5f * (VF[k] + VF[k + 1]);
+        float hd = H1[k + 1], hrv = H1[k];
         bool live = !(fabsf(speed) < 1.0e-5f);
         bool gate_shut = !recv_gate && hrv <= 1.0e-6f;
         float frac = fminf(EXACT ? fabsf(speed) * a.dt / cell : fabsf(speed) * dtc, 0.45f);
@@ -540,8 +554,8 @@ __global__ __launch_bounds__(kLanes) void k_sheet_stream(FusedArgs fa) {
         return ok ? dh : 0.0f;
       };
       auto fd = [&](int k, float prev, bool recv_gate) -> float {
-        float speed = 0.5f * (vfw[k] + vfw[k + 1]);
-        float hd = h1w[k], hrv = h1w[k + 1];
+        float speed = 0.5f * (VF[k] + VF[k + 1]);
+        float hd = H1[k], hrv = H1[k + 1];
         bool live = !(fabsf(speed) < 1.0e-5f);
         float room = hd - (0.0f + prev);
         bool gate_shut = !recv_gate && hrv <= 1.0e-6f;
@@ -550,19 +564,21 @@ __global__ __launch_bounds__(kLanes) void k_sheet_stream(FusedArgs fa) {
         bool ok = live && !(room <= 1.0e-8f) && !gate_shut && !(dh <= 1.0e-8f) && speed > 0.0f;
         return ok ? dh : 0.0f;
       };
-      float fu_m2 = e_m2 ? fu(0, gate_row(ring(O_GT, D_GT, y - 2))) : 0.0f;
-      float fu_m1 = e_m1 ? fu(1, gate_row(g_m1)) : 0.0f;
-      float fd_m1 = e_m1 ? fd(1, fu_m2, gate_row(g_0)) : 0.0f;
-      float fu_0 = e_0 ? fu(2, gate_row(g_0)) : 0.0f;
-      float fd_0 = e_0 ? fd(2, fu_m1, gate_row(g_p1)) : 0.0f;
+      // The fluxes of the two edges above come from the previous row, where
+      // they were the edges (y-1, y) and (y, y+1) of that row.
+      const float fu_m1 = c_fu_0, fd_m1 = c_fd_0;
+      const float fu_0 = e_0 ? fu(2, gate_ok(GW[2])) : 0.0f;
+      const float fd_0 = e_0 ? fd(2, fu_m1, gate_ok(GW[3])) : 0.0f;
+      c_fu_0 = fu_0;
+      c_fd_0 = fd_0;
       float rem = (0.0f + fu_m1) + fd_0;
       float add = (0.0f + fd_m1) + fu_0;
-      float hv = h1w[2];
+      float hv = H1[2];
       float moved = fmaxf(hv - rem + add, 0.0f);
       float h2 = (rem > 0.0f || add > 0.0f) ? moved : hv;
-      float uo = ring(O_UF, D_UF, y)[li];
-      float vo = vfw[2];
-      bool inside = (colmask & rmw[2]) != 0;
+      float uo = UF[0];
+      float vo = VF[2];
+      bool inside = (colmask & RM[11]) != 0;
       const bool store = out_col && y >= y0 && y < y1;
       if (want_stats && store) acc_res += ((double)h2 - (double)hv) - ((double)add - (double)rem);
       if (inside && eps > 0.0f && h2 < eps) {
@@ -580,8 +596,6 @@ __global__ __launch_bounds__(kLanes) void k_sheet_stream(FusedArgs fa) {
           acc_vol += (double)fmaxf(h2, 0.0f);
           if (h2 > eps) {
             acc_wet++;
-            acc_i0 = min(acc_i0, col);
-            acc_i1 = max(acc_i1, col);
             acc_j0 = min(acc_j0, y);
             acc_j1 = max(acc_j1, y);
           }
@@ -589,11 +603,26 @@ __global__ __launch_bounds__(kLanes) void k_sheet_stream(FusedArgs fa) {
         }
       }
     }
-    // No barrier here: the next iteration's first writes (eta, lap, p) go to
-    // ring slots no stage of this iteration still reads.
+    // No barrier here: the next row's first writes (eta) go to the ring slot
+    // of row r-1, which S2 read before this row's second barrier.
+  };
+
+  // Rows r_begin..r_end in pairs of (even, odd): a leading row of the wrong
+  // parity or a trailing extra row only runs the pipeline over rows that are
+  // never stored.
+  const int r_begin = (y0 - kHaloL) & ~1;
+  const int r_end = y1 - 1 + kLag;
+  prefetch(0, r_begin);
+  prefetch(1, r_begin + 1);
+  for (int r = r_begin; r <= r_end; r += 2) {
+    row(std::integral_constant<int, 0>{}, r);
+    row(std::integral_constant<int, 1>{}, r + 1);
   }
+
   if (want_stats) {
-    // Wavefront reduction by shuffles, then one atomic per wave.
+    // The wet columns of this lane span one column; the bounding box reduces
+    // over lanes with shuffles, then one atomic per wave.
+    int acc_i0 = acc_wet ? col : 0x7fffffff, acc_i1 = acc_wet ? col : -1;
     for (int off = 32; off > 0; off >>= 1) {
       acc_vol += __shfl_xor(acc_vol, off);
       acc_cut += __shfl_xor(acc_cut, off);
@@ -671,16 +700,42 @@ int g_chunk_rows = 0;  // 0: pick automatically
 
 }  // namespace
 
-int fused_chunk_rows(int res) {
+// Output rows per workgroup.  Every tile pays the 22-row pipeline fill, so
+// the chunk is the smallest that keeps all tiles of the launch resident at
+// once (one round of workgroups: no tail of late tiles).
+int fused_chunk_rows(int res, int strips, int layers, int resident) {
   if (g_chunk_rows > 0) return g_chunk_rows;
-  // Enough workgroups to give every CU several, few enough to keep the
-  // 22-row pipeline fill small.
-  if (res >= 2048) return 192;
-  if (res >= 512) return 64;
-  return 32;
+  int per_column = resident / (strips * layers);
+  if (per_column < 1) per_column = 1;
+  int chunk = (res + per_column - 1) / per_column;
+  return chunk < 8 ? 8 : chunk;
 }
 
 void fused_set_chunk_rows(int rows) { g_chunk_rows = rows; }
+
+template <bool EXACT, bool PROJECT, bool WIND>
+static int resident_tiles() {
+  // Workgroups of this instance the device holds at once (cached per device).
+  static int cached[64] = {0};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+  if (cached[dev] == 0) {
+    int per_cu = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sheet_stream<EXACT, PROJECT, WIND>, kLanes, 0) !=
+            hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || per_cu < 1 || cus < 1)
+      return 256;
+    cached[dev] = per_cu * cus;
+  }
+  return cached[dev];
+}
+
+template <bool EXACT>
+static int resident_for(bool project, bool wind) {
+  if (wind) return resident_tiles<EXACT, false, true>();
+  if (project) return resident_tiles<EXACT, true, false>();
+  return resident_tiles<EXACT, false, false>();
+}
 
 template <bool EXACT>
 static void launch(const FusedArgs& fa, int layers, bool project, bool wind, hipStream_t st) {
@@ -701,10 +756,11 @@ int fused_substep_layers(const FusedLayer* layers, int nlayers, const float* bed
   fa.bed = bed;
   fa.res = res;
   fa.strips = (res + kOutW - 1) / kOutW;
-  fa.chunk = fused_chunk_rows(res);
+  bool project = layers[0].a.project != 0, wind = layers[0].a.wind != 0;
+  int resident = exact ? resident_for<true>(project, wind) : resident_for<false>(project, wind);
+  fa.chunk = fused_chunk_rows(res, fa.strips, nlayers, resident);
   fa.chunks = (res + fa.chunk - 1) / fa.chunk;
   fa.tiles = fa.strips * fa.chunks;
-  bool project = layers[0].a.project != 0, wind = layers[0].a.wind != 0;
   for (int k = 0; k < nlayers; k++) {
     const FusedLayer& f = layers[k];
     if ((f.a.project != 0) != project || (f.a.wind != 0) != wind) return SILT_EINVAL;

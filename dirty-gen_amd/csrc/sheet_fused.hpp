@@ -77,8 +77,9 @@ int run_plan(const SheetPlanes& s, const StepPlan& plan, int res, float cell, fl
 int step_device_planes(const SheetPlanes& s, int res, float cell, float dt, const silt_drive& d,
                        const silt_sheet_opts& o, bool gate_uniform, bool nu_uniform, float nu0, hipStream_t st);
 
-// Output rows per workgroup (the autotuned chunk); 0 restores the default.
-int fused_chunk_rows(int res);
+// Output rows per workgroup: the smallest chunk that keeps every tile of the
+// launch resident, unless fused_set_chunk_rows pinned one (0 unpins).
+int fused_chunk_rows(int res, int strips, int layers, int resident);
 void fused_set_chunk_rows(int rows);
 
 }  // namespace silt
