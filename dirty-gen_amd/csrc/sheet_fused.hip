@@ -12,16 +12,18 @@
 //   S5  vorticity w of the snapshot                     row r-5
 //   S6  confinement -> uc, vc                           row r-6
 //   S7  divergence source b (projection)                row r-7
-//   S8  x filters of b                                  row r-7
-//   S9  z filters -> correction potential q             row r-9
-//   S10 correction kick, damping, speed cap -> uf, vf   row r-10
-//   S11 open transport along x -> h1                    row r-10
-//   S12 open transport along z, dry cut, store          row r-11
+//   S8  x filters of b                                  row r-8
+//   S9  z filters -> correction potential q             row r-10
+//   S10 correction kick, damping, speed cap -> uf, vf   row r-11
+//   S11 open transport along x -> h1                    row r-12
+//   S12 open transport along z, dry cut, store          row r-13
 //
-// A value a later stage reads across lanes goes through LDS: one row for a
-// value read in the iteration that writes it, a two-slot ring (slot = row
-// parity) for one read in the next iteration.  The row loop is unrolled by
-// two, so the parity and with it every LDS offset is a compile-time constant.
+// A value a later stage reads across lanes goes through LDS, in a two-slot
+// ring (slot = row parity), and is read one iteration after it is written.
+// So one barrier per iteration orders every exchange: it separates this
+// row's writes from the next row's reads, and this row's reads of a slot from
+// the next row's writes to it.  The row loop is unrolled by two, so the
+// parity and with it every LDS offset is a compile-time constant.
 // A value a later stage reads only in its own column (the vertical
 // neighbours of every z difference, the lags between stages) stays in
 // registers: each lane keeps a short window of the rows it produced.
@@ -53,14 +55,14 @@ constexpr int kLanes = 256;
 constexpr int kHaloL = 11;
 constexpr int kHaloR = 10;
 constexpr int kOutW = kLanes - kHaloL - kHaloR;  // 235
-constexpr int kLag = 11;                        // S12 trails S1 by 11 rows
+constexpr int kLag = 13;                        // S12 trails S1 by 13 rows
 constexpr int kRow = 264;                       // LDS row stride (256 + ghosts)
 constexpr int kMaxBoxes = 8;
 
-// LDS rows.  Two-slot rings: eta, p, vg, w, uc, q.  Single rows: b, uf, h, gate.
-constexpr int R_ETA = 0, R_P = 2, R_VG = 4, R_W = 6, R_UC = 8, R_Q = 10, R_B = 12, R_UF = 13, R_HX = 14,
-              R_GT = 15;
-constexpr int kRows = 16;
+// LDS rows: a two-slot ring for each value read across lanes.
+constexpr int R_ETA = 0, R_P = 2, R_VG = 4, R_W = 6, R_UC = 8, R_B = 10, R_Q = 12, R_UF = 14, R_HX = 16,
+              R_GT = 18;
+constexpr int kRows = 20;
 
 // Ring slot of row r - k when r has parity P.
 template <int P, int K>
@@ -245,14 +247,14 @@ __global__ __launch_bounds__(kLanes, 4) void k_sheet_stream(FusedArgs fa) {
   float PW[3] = {0, 0, 0};                                 // p        r-5 .. r-3
   float UG[3] = {0, 0, 0}, VG[3] = {0, 0, 0};              // snapshot r-6 .. r-4
   float WW[3] = {0, 0, 0};                                 // w        r-7 .. r-5
-  float UC[5] = {0, 0, 0, 0, 0}, VC[5] = {0, 0, 0, 0, 0};  // confined r-10 .. r-6
-  float GB[5] = {0, 0, 0, 0, 0}, SB[5] = {0, 0, 0, 0, 0};  // x filters of b    r-11 .. r-7
-  float QW[3] = {0, 0, 0};                                 // q        r-11 .. r-9
-  float UF[2] = {0, 0};                                    // final u  r-11 .. r-10
-  float VF[4] = {0, 0, 0, 0};                              // final v  r-13 .. r-10
-  float H1[4] = {0, 0, 0, 0};                              // x-moved h r-13 .. r-10
-  float GW[4] = {1, 1, 1, 1};                              // gate     r-13 .. r-10
-  unsigned RM[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};  // RM[k] = row mask of r-k
+  float UC[6] = {0, 0, 0, 0, 0, 0}, VC[6] = {0, 0, 0, 0, 0, 0};  // confined r-11 .. r-6
+  float GB[5] = {0, 0, 0, 0, 0}, SB[5] = {0, 0, 0, 0, 0};  // x filters of b    r-12 .. r-8
+  float QW[3] = {0, 0, 0};                                 // q        r-12 .. r-10
+  float UF[3] = {0, 0, 0};                                 // final u  r-13 .. r-11
+  float VF[5] = {0, 0, 0, 0, 0};                           // final v  r-15 .. r-11
+  float H1[4] = {0, 0, 0, 0};                              // x-moved h r-15 .. r-12
+  float GW[5] = {1, 1, 1, 1, 1};                           // gate     r-15 .. r-11
+  unsigned RM[14] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};  // RM[k] = row mask of r-k
   // z fluxes of the edge (y, y+1) of the previous S12 row y, which is the
   // edge (y-1, y) of the next: up-going and down-going.
   float c_fu_0 = 0.0f, c_fd_0 = 0.0f;
@@ -276,25 +278,25 @@ __global__ __launch_bounds__(kLanes, 4) void k_sheet_stream(FusedArgs fa) {
   // are issued while row r computes, into the registers of r's parity.
   //   pf_h, pf_b   h and bed of S1's row         (r)
   //   pf_h4, pf_u4, pf_v4   h, u, v of S4's row  (r-4)
-  //   pf_h10       h of S10's row                (r-10)
+  //   pf_h11       h of S10's row                (r-11)
   // S6 reads h of row r-6, which S4 loaded two rows earlier (h6).
-  float pf_h[2], pf_b[2], pf_h4[2], pf_u4[2], pf_v4[2], pf_h10[2], h6[2] = {0.0f, 0.0f};
+  float pf_h[2], pf_b[2], pf_h4[2], pf_u4[2], pf_v4[2], pf_h11[2], h6[2] = {0.0f, 0.0f};
   auto prefetch = [&](int q, int r) {
     pf_h[q] = load(L.h, r);
     pf_b[q] = load(fa.bed, r);
     pf_h4[q] = load(L.h, r - 4);
     pf_u4[q] = load(L.u, r - 4);
     pf_v4[q] = load(L.v, r - 4);
-    pf_h10[q] = load(L.h, r - 10);
+    pf_h11[q] = load(L.h, r - 11);
   };
 
   auto row = [&](auto par, int r) {
     constexpr int P = decltype(par)::value;
 #pragma unroll
-    for (int k = 11; k > 1; k--) RM[k] = RM[k - 1];
+    for (int k = 13; k > 1; k--) RM[k] = RM[k - 1];
     RM[1] = rowmask(r - 1);
 
-    const float h4 = pf_h4[P], u4 = pf_u4[P], v4 = pf_v4[P], h10 = pf_h10[P];
+    const float h4 = pf_h4[P], u4 = pf_u4[P], v4 = pf_v4[P], h11 = pf_h11[P];
     // ---- S1: eta at row r ------------------------------------------------
     {
       const float ev = pf_b[P] + pf_h[P];
@@ -302,7 +304,6 @@ __global__ __launch_bounds__(kLanes, 4) void k_sheet_stream(FusedArgs fa) {
       put(R_ETA + kSlot<P, 0>, ev);
       push(E, ev);
     }
-    __syncthreads();
     // ---- S2: x filters of eta, lap(eta) at row r-1 -------------------------
     {
       constexpr int RE = R_ETA + kSlot<P, 1>;
@@ -335,7 +336,6 @@ __global__ __launch_bounds__(kLanes, 4) void k_sheet_stream(FusedArgs fa) {
       put(R_P + kSlot<P, 3>, pv);
       push(PW, pv);
     }
-    __syncthreads();
     // ---- S4: gravity / wind blend at row r-4 -> snapshot ------------------
     {
       const int y = r - 4;
@@ -366,7 +366,6 @@ __global__ __launch_bounds__(kLanes, 4) void k_sheet_stream(FusedArgs fa) {
       push(UG, uu);
       push(VG, vv);
     }
-    __syncthreads();
     // ---- S5: vorticity of the snapshot at row r-5 -------------------------
     if (vort) {
       const int y = r - 5;
@@ -377,7 +376,6 @@ __global__ __launch_bounds__(kLanes, 4) void k_sheet_stream(FusedArgs fa) {
       put(R_W + kSlot<P, 5>, w);
       push(WW, w);
     }
-    __syncthreads();
     // ---- S6: confinement at row r-6 ---------------------------------------
     {
       const int y = r - 6;
@@ -414,7 +412,6 @@ __global__ __launch_bounds__(kLanes, 4) void k_sheet_stream(FusedArgs fa) {
       push(VC, vv);
       h6[P] = h4;  // S6's row two rows on
     }
-    __syncthreads();
     // ---- S7: divergence source at row r-7 ---------------------------------
     if (PROJECT) {
       const int y = r - 7;
@@ -422,7 +419,8 @@ __global__ __launch_bounds__(kLanes, 4) void k_sheet_stream(FusedArgs fa) {
       if ((colmask & RM[7]) != 0) {
         constexpr int RU = R_UC + kSlot<P, 7>;
         float ux = A::div(at(RU, 1) - at(RU, -1), two, inv2c);
-        float vz = A::div(above(y, VC[3], VC[4]) - below(y, VC[2], VC[3]), two, inv2c);
+        // VC holds rows r-11..r-6: y-1, y, y+1 at 3, 4, 5.
+        float vz = A::div(above(y, VC[4], VC[5]) - below(y, VC[3], VC[4]), two, inv2c);
         float div = ux + vz;
         float lp = LW[0];
         float ov = nu_uni ? a.ov0 : overload_of(load(L.nu, y));
@@ -431,35 +429,34 @@ __global__ __launch_bounds__(kLanes, 4) void k_sheet_stream(FusedArgs fa) {
         else
           bv = ov * __builtin_fmaf(-0.015f, lp, div);
       }
-      put(R_B, bv);
+      put(R_B + kSlot<P, 7>, bv);
     }
-    __syncthreads();
-    // ---- S8: x filters of b at row r-7; S9: q at row r-9 -------------------
+    // ---- S8: x filters of b at row r-8; S9: q at row r-10 ------------------
     if (PROJECT) {
-      bool inside = (colmask & RM[7]) != 0;
-      float b0 = at(R_B, -2), b1 = at(R_B, -1), b2 = at(R_B, 0), b3 = at(R_B, 1), b4 = at(R_B, 2);
+      constexpr int RB = R_B + kSlot<P, 8>;
+      bool inside = (colmask & RM[8]) != 0;
+      float b0 = at(RB, -2), b1 = at(RB, -1), b2 = at(RB, 0), b3 = at(RB, 1), b4 = at(RB, 2);
       push(GB, inside ? acc5(b0, b1, b2, b3, b4, kG) : 0.0f);
       push(SB, inside ? acc5(b0, b1, b2, b3, b4, kS) : 0.0f);
-      const int yq = r - 9;
+      const int yq = r - 10;
       float qv = 0.0f;
-      if ((colmask & RM[9]) != 0) {
+      if ((colmask & RM[10]) != 0) {
         float gz = zacc(GB, yq, kG), sz = zacc(SB, yq, kS);
         qv = A::fma_(0.38f, sz, 0.62f * gz);
       }
-      put(R_Q + kSlot<P, 9>, qv);
+      put(R_Q + kSlot<P, 10>, qv);
       push(QW, qv);
     }
-    __syncthreads();
-    // ---- S10: correction, damping, cap at row r-10 -------------------------
+    // ---- S10: correction, damping, cap at row r-11 -------------------------
     {
-      const int y = r - 10;
+      const int y = r - 11;
       float uu = UC[0];
       float vv = VC[0];
-      float hv = h10;
-      bool inside = (colmask & RM[10]) != 0;
+      float hv = h11;
+      bool inside = (colmask & RM[11]) != 0;
       if (inside && hv > eps) {
         if (PROJECT) {
-          constexpr int RQ = R_Q + kSlot<P, 10>;
+          constexpr int RQ = R_Q + kSlot<P, 11>;
           float qx = A::div(at(RQ, 1) - at(RQ, -1), two, inv2c);
           float qz = A::div(above(y, QW[1], QW[2]) - below(y, QW[0], QW[1]), two, inv2c);
           if (EXACT) {
@@ -474,23 +471,23 @@ __global__ __launch_bounds__(kLanes, 4) void k_sheet_stream(FusedArgs fa) {
         uu = finite_speed(uu * decay, a.lim);
         vv = finite_speed(vv * decay, a.lim);
       }
-      put(R_UF, uu);
+      put(R_UF + kSlot<P, 11>, uu);
       push(UF, uu);
       push(VF, vv);  // the z transport reads only this lane's column
-      put(R_HX, hv);
+      put(R_HX + kSlot<P, 11>, hv);
       if (!gate_uni) {
         int yc = y < 0 ? 0 : (y >= res ? res - 1 : y);
         float gv = (float)L.gate[(size_t)yc * res + colc];
-        put(R_GT, gv);
+        put(R_GT + kSlot<P, 11>, gv);
         push(GW, gv);
       }
     }
-    __syncthreads();
-    // ---- S11: transport along x at row r-10 --------------------------------
+    // ---- S11: transport along x at row r-12 --------------------------------
     {
-      const int yx = r - 10;
-      const unsigned rmx = RM[10];
-      auto gate_at = [&](int d) -> bool { return gate_uni ? true : at(R_GT, d) != 0.0f; };
+      const int yx = r - 12;
+      const unsigned rmx = RM[12];
+      constexpr int RUF = R_UF + kSlot<P, 12>, RHX = R_HX + kSlot<P, 12>, RGT = R_GT + kSlot<P, 12>;
+      auto gate_at = [&](int d) -> bool { return gate_uni ? true : at(RGT, d) != 0.0f; };
       // Edge (c, c+1) exists when both cells lie in one box.
       const bool e_m2 = (colmask_l2 & colmask_l1 & rmx) != 0;  // edge (col-2, col-1)
       const bool e_m1 = (colmask_l1 & colmask & rmx) != 0;     // edge (col-1, col)
@@ -498,8 +495,8 @@ __global__ __launch_bounds__(kLanes, 4) void k_sheet_stream(FusedArgs fa) {
       const float dtc = L.dtc;
       // Left-going flux of edge (k, k+1) (donor k+1), k = col + d.
       auto fl = [&](int d) -> float {
-        float speed = 0.5f * (at(R_UF, d) + at(R_UF, d + 1));
-        float hd = at(R_HX, d + 1), hrv = at(R_HX, d);
+        float speed = 0.5f * (at(RUF, d) + at(RUF, d + 1));
+        float hd = at(RHX, d + 1), hrv = at(RHX, d);
         bool live = !(fabsf(speed) < 1.0e-5f);
         bool gate_shut = !gate_at(d) && hrv <= 1.0e-6f;
         float frac = fminf(EXACT ? fabsf(speed) * a.dt / cell : fabsf(speed) * dtc, 0.45f);
@@ -510,8 +507,8 @@ __global__ __launch_bounds__(kLanes, 4) void k_sheet_stream(FusedArgs fa) {
       // Right-going flux of edge (k, k+1) (donor k) with the donor's removed
       // sum holding the left-going flux of edge (k-1, k).
       auto fr = [&](int d, float prev) -> float {
-        float speed = 0.5f * (at(R_UF, d) + at(R_UF, d + 1));
-        float hd = at(R_HX, d), hrv = at(R_HX, d + 1);
+        float speed = 0.5f * (at(RUF, d) + at(RUF, d + 1));
+        float hd = at(RHX, d), hrv = at(RHX, d + 1);
         bool live = !(fabsf(speed) < 1.0e-5f);
         float room = hd - (0.0f + prev);
         bool gate_shut = !gate_at(d + 1) && hrv <= 1.0e-6f;
@@ -527,19 +524,19 @@ __global__ __launch_bounds__(kLanes, 4) void k_sheet_stream(FusedArgs fa) {
       float fr_0 = e_0 ? fr(0, fl_m1) : 0.0f;
       float rem = (0.0f + fl_m1) + fr_0;
       float add = (0.0f + fr_m1) + fl_0;
-      float hv = at(R_HX, 0);
+      float hv = at(RHX, 0);
       float moved = fmaxf(hv - rem + add, 0.0f);
       float h1 = (rem > 0.0f || add > 0.0f) ? moved : hv;
       if (want_stats && out_col && yx >= y0 && yx < y1)
         acc_res += ((double)h1 - (double)hv) - ((double)add - (double)rem);
       push(H1, h1);
     }
-    // ---- S12: transport along z, dry cut, store at row r-11 ----------------
+    // ---- S12: transport along z, dry cut, store at row r-13 ----------------
     {
-      const int y = r - 11;
-      // Windows hold rows y-2..y+1 at indices 0..3; RM[11], RM[10] are the
-      // masks of rows y, y+1.
-      const bool e_0 = (colmask & RM[11] & RM[10]) != 0;  // edge (y, y+1)
+      const int y = r - 13;
+      // H1, VF and GW hold rows y-2..y+1 at indices 0..3; RM[13], RM[12] are
+      // the masks of rows y, y+1.
+      const bool e_0 = (colmask & RM[13] & RM[12]) != 0;  // edge (y, y+1)
       const float dtc = L.dtc;
       auto gate_ok = [&](float g) -> bool { return gate_uni ? true : g != 0.0f; };
       // Up-going flux of edge (k, k+1) (donor k+1 = window index k+1).
@@ -578,7 +575,7 @@ __global__ __launch_bounds__(kLanes, 4) void k_sheet_stream(FusedArgs fa) {
       float h2 = (rem > 0.0f || add > 0.0f) ? moved : hv;
       float uo = UF[0];
       float vo = VF[2];
-      bool inside = (colmask & RM[11]) != 0;
+      bool inside = (colmask & RM[13]) != 0;
       const bool store = out_col && y >= y0 && y < y1;
       if (want_stats && store) acc_res += ((double)h2 - (double)hv) - ((double)add - (double)rem);
       if (inside && eps > 0.0f && h2 < eps) {
@@ -603,14 +600,13 @@ __global__ __launch_bounds__(kLanes, 4) void k_sheet_stream(FusedArgs fa) {
         }
       }
     }
-    // No barrier here: the next row's first writes (eta) go to the ring slot
-    // of row r-1, which S2 read before this row's second barrier.
+    __syncthreads();
   };
 
   // Rows r_begin..r_end in pairs of (even, odd): a leading row of the wrong
   // parity or a trailing extra row only runs the pipeline over rows that are
   // never stored.
-  const int r_begin = (y0 - kHaloL) & ~1;
+  const int r_begin = (y0 - kLag) & ~1;
   const int r_end = y1 - 1 + kLag;
   prefetch(0, r_begin);
   prefetch(1, r_begin + 1);
