@@ -40,6 +40,8 @@
 // EXACT = true keeps the reference's operation order with no fused
 // multiply-add and IEEE division, and reproduces the exact path bit for bit;
 // EXACT = false fuses multiply-adds and multiplies by precomputed reciprocals.
+// FULL = true is the common plan of one box covering the whole grid: the box
+// masks reduce to the grid's own edges.
 #include "sheet_fused.hpp"
 
 #include <math.h>
@@ -130,7 +132,7 @@ __device__ __forceinline__ void push(float (&w)[N], float v) {
 
 }  // namespace
 
-template <bool EXACT, bool PROJECT, bool WIND>
+template <bool EXACT, bool PROJECT, bool WIND, bool FULL>
 __global__ __launch_bounds__(kLanes, 4) void k_sheet_stream(FusedArgs fa) {
   using A = Ar<EXACT>;
   __shared__ float lds[kRows * kRow];
@@ -170,8 +172,13 @@ __global__ __launch_bounds__(kLanes, 4) void k_sheet_stream(FusedArgs fa) {
       if (c >= L.boxes.i0[k] && c <= L.boxes.i1[k]) m |= 1u << k;
     return m;
   };
-  const unsigned colmask = colbits(col), colmask_r = colbits(col + 1);
-  const unsigned colmask_l1 = colbits(col - 1), colmask_l2 = colbits(col - 2);
+  // With one box over the whole grid every lane counts as inside (lanes off
+  // the grid are never stored and never written to LDS) and only the grid's
+  // sides end the x faces.
+  const unsigned colmask = FULL ? ~0u : colbits(col);
+  const unsigned colmask_r = FULL ? (col + 1 < res ? ~0u : 0u) : colbits(col + 1);
+  const unsigned colmask_l1 = FULL ? (col >= 1 ? ~0u : 0u) : colbits(col - 1);
+  const unsigned colmask_l2 = FULL ? (col >= 2 ? ~0u : 0u) : colbits(col - 2);
   auto rowmask = [&](int y) -> unsigned {
     unsigned m = 0;
     for (int k = 0; k < L.boxes.n; k++)
@@ -294,7 +301,7 @@ __global__ __launch_bounds__(kLanes, 4) void k_sheet_stream(FusedArgs fa) {
     constexpr int P = decltype(par)::value;
 #pragma unroll
     for (int k = 13; k > 1; k--) RM[k] = RM[k - 1];
-    RM[1] = rowmask(r - 1);
+    RM[1] = FULL ? ((unsigned)(r - 1) < (unsigned)res ? 1u : 0u) : rowmask(r - 1);
 
     const float h4 = pf_h4[P], u4 = pf_u4[P], v4 = pf_v4[P], h11 = pf_h11[P];
     // ---- S1: eta at row r ------------------------------------------------
@@ -709,7 +716,7 @@ int fused_chunk_rows(int res, int strips, int layers, int resident) {
 
 void fused_set_chunk_rows(int rows) { g_chunk_rows = rows; }
 
-template <bool EXACT, bool PROJECT, bool WIND>
+template <bool EXACT, bool PROJECT, bool WIND, bool FULL>
 static int resident_tiles() {
   // Workgroups of this instance the device holds at once (cached per device).
   static int cached[64] = {0};
@@ -717,8 +724,8 @@ static int resident_tiles() {
   if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
   if (cached[dev] == 0) {
     int per_cu = 0, cus = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sheet_stream<EXACT, PROJECT, WIND>, kLanes, 0) !=
-            hipSuccess ||
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sheet_stream<EXACT, PROJECT, WIND, FULL>, kLanes,
+                                                     0) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || per_cu < 1 || cus < 1)
       return 256;
     cached[dev] = per_cu * cus;
@@ -726,23 +733,31 @@ static int resident_tiles() {
   return cached[dev];
 }
 
-template <bool EXACT>
-static int resident_for(bool project, bool wind) {
-  if (wind) return resident_tiles<EXACT, false, true>();
-  if (project) return resident_tiles<EXACT, true, false>();
-  return resident_tiles<EXACT, false, false>();
-}
-
-template <bool EXACT>
-static void launch(const FusedArgs& fa, int layers, bool project, bool wind, hipStream_t st) {
-  dim3 grid((unsigned)(fa.tiles * layers));
+// The instance for a launch: its resident tile count, or the launch itself.
+template <bool EXACT, bool FULL>
+static int dispatch(const FusedArgs* fa, int layers, bool project, bool wind, hipStream_t st) {
+  if (!fa) {
+    if (wind) return resident_tiles<EXACT, false, true, FULL>();
+    if (project) return resident_tiles<EXACT, true, false, FULL>();
+    return resident_tiles<EXACT, false, false, FULL>();
+  }
+  dim3 grid((unsigned)(fa->tiles * layers));
   dim3 blk(kLanes);
   if (wind)
-    hipLaunchKernelGGL((k_sheet_stream<EXACT, false, true>), grid, blk, 0, st, fa);
+    hipLaunchKernelGGL((k_sheet_stream<EXACT, false, true, FULL>), grid, blk, 0, st, *fa);
   else if (project)
-    hipLaunchKernelGGL((k_sheet_stream<EXACT, true, false>), grid, blk, 0, st, fa);
+    hipLaunchKernelGGL((k_sheet_stream<EXACT, true, false, FULL>), grid, blk, 0, st, *fa);
   else
-    hipLaunchKernelGGL((k_sheet_stream<EXACT, false, false>), grid, blk, 0, st, fa);
+    hipLaunchKernelGGL((k_sheet_stream<EXACT, false, false, FULL>), grid, blk, 0, st, *fa);
+  return 0;
+}
+
+static int dispatch_any(const FusedArgs* fa, int layers, bool exact, bool full, bool project, bool wind,
+                        hipStream_t st) {
+  if (exact) return full ? dispatch<true, true>(fa, layers, project, wind, st)
+                         : dispatch<true, false>(fa, layers, project, wind, st);
+  return full ? dispatch<false, true>(fa, layers, project, wind, st)
+              : dispatch<false, false>(fa, layers, project, wind, st);
 }
 
 int fused_substep_layers(const FusedLayer* layers, int nlayers, const float* bed, int res, bool exact,
@@ -753,7 +768,12 @@ int fused_substep_layers(const FusedLayer* layers, int nlayers, const float* bed
   fa.res = res;
   fa.strips = (res + kOutW - 1) / kOutW;
   bool project = layers[0].a.project != 0, wind = layers[0].a.wind != 0;
-  int resident = exact ? resident_for<true>(project, wind) : resident_for<false>(project, wind);
+  bool full = true;
+  for (int k = 0; k < nlayers; k++) {
+    const std::vector<Box>& b = layers[k].boxes;
+    full = full && b.size() == 1 && b[0].i0 == 0 && b[0].j0 == 0 && b[0].i1 == res - 1 && b[0].j1 == res - 1;
+  }
+  int resident = dispatch_any(nullptr, nlayers, exact, full, project, wind, st);
   fa.chunk = fused_chunk_rows(res, fa.strips, nlayers, resident);
   fa.chunks = (res + fa.chunk - 1) / fa.chunk;
   fa.tiles = fa.strips * fa.chunks;
@@ -765,10 +785,7 @@ int fused_substep_layers(const FusedLayer* layers, int nlayers, const float* bed
     fa.layer[k].stats = f.stats;
     fa.layer[k].totals = f.totals;
   }
-  if (exact)
-    launch<true>(fa, nlayers, project, wind, st);
-  else
-    launch<false>(fa, nlayers, project, wind, st);
+  dispatch_any(&fa, nlayers, exact, full, project, wind, st);
   return hipGetLastError() == hipSuccess ? 0 : SILT_EDEVICE;
 }
 
