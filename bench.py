@@ -59,6 +59,7 @@ def main():
     ap.add_argument("--path", default="fast", choices=["fast", "fused_exact", "exact"])
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default rows per workgroup")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported on this build "
@@ -76,6 +77,13 @@ def main():
     pair = S.SheetPair(res, cell, path=args.path)
     z = np.zeros(n, np.float32)
     pair.upload(sc["bed"], sc["water"], z, z, sc["slurry"], z, z, sc["nu_slurry"])
+    chunk, tune_ms = 0, None
+    if not args.no_autotune and args.path != "exact":
+        # Tuning step before the run (state restored afterwards): rows per
+        # workgroup of the fused kernel for this sheet.
+        chunk, tune_ms = pair.autotune(dt)
+        print(f"[bench] autotuned rows per workgroup: {chunk or 'default'} ({tune_ms:.3f} ms)", file=sys.stderr,
+              flush=True)
     print(f"[bench] warmup {args.warmup} steps", file=sys.stderr, flush=True)
     for _ in range(args.warmup):
         pair.step(dt)
@@ -107,6 +115,7 @@ def main():
         "data": "synthetic (seeded fBm bed, Gaussian puddles)",
         "config": {"workload": f"plain two-layer sheet {res}x{res} @ {cell} m, dt 1/120 s, water nu 0.9 vort 0.08, "
                                f"slurry nu 0.5 vort 0.05", "res": res, "cell_m": cell, "path": args.path,
+                   "chunk_rows": chunk or "default",
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,

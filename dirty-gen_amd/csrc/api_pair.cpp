@@ -328,6 +328,75 @@ int silt_pair_ledger(silt_pair* p, silt_pair_ledger_t* out) {
 
 float silt_pair_last_kernel_ms(const silt_pair* p) { return p ? p->last_kernel_ms : 0.0f; }
 
+int silt_pair_autotune(silt_pair* p, float dt, const int* rows, size_t n, int reps, int* best_rows, float* best_ms) {
+  if (!p || !(dt > 0.0f) || reps < 1 || reps > 1000 || (n && !rows)) return SILT_EINVAL;
+  std::vector<int> cand;
+  if (rows && n) {
+    for (size_t k = 0; k < n; k++)
+      if (rows[k] >= 0) cand.push_back(rows[k]);
+  } else {
+    // The default (one round of resident tiles) and heights around it.
+    cand = {0, 48, 64, 96, 128, 160, 192, 256, 384, 512};
+  }
+  if (cand.empty()) return SILT_EINVAL;
+  const size_t fb = p->n * sizeof(float);
+  float* save = nullptr;
+  if (hipMalloc(&save, 6 * fb) != hipSuccess) return SILT_ENOMEM;
+  hipStream_t st = p->stream;
+  const int cur0[2] = {p->cur[0], p->cur[1]};
+  SheetStats stats0[2];
+  SheetTotals totals0[2];
+  auto restore = [&]() -> int {
+    p->cur[0] = cur0[0];
+    p->cur[1] = cur0[1];
+    for (int l = 0; l < 2; l++)
+      for (int k = 0; k < 3; k++)
+        HIPCHK(hipMemcpyAsync(p->planes[l][cur0[l]][k], save + (size_t)(3 * l + k) * p->n, fb,
+                              hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(p->stats, stats0, sizeof stats0, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(p->totals, totals0, sizeof totals0, hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return 0;
+  };
+  int rc = 0;
+  for (int l = 0; l < 2 && rc == 0; l++)
+    for (int k = 0; k < 3; k++)
+      if (hipMemcpyAsync(save + (size_t)(3 * l + k) * p->n, p->planes[l][cur0[l]][k], fb, hipMemcpyDeviceToDevice,
+                         st) != hipSuccess)
+        rc = SILT_EDEVICE;
+  if (rc == 0 && (hipMemcpyAsync(stats0, p->stats, sizeof stats0, hipMemcpyDeviceToHost, st) != hipSuccess ||
+                  hipMemcpyAsync(totals0, p->totals, sizeof totals0, hipMemcpyDeviceToHost, st) != hipSuccess ||
+                  hipStreamSynchronize(st) != hipSuccess))
+    rc = SILT_EDEVICE;
+  const int pinned0 = fused_pinned_chunk_rows(p->res);
+  float best = INFINITY;
+  int best_c = pinned0;
+  for (size_t c = 0; c < cand.size() && rc == 0; c++) {
+    fused_set_chunk_rows_for(p->res, cand[c]);
+    rc = silt_pair_step(p, dt);  // warm-up of this height
+    double sum = 0.0;
+    for (int k = 0; k < reps && rc >= 0; k++) {
+      rc = silt_pair_step(p, dt);
+      sum += p->last_kernel_ms;
+    }
+    int rr = restore();
+    if (rc >= 0) rc = rr;
+    if (rc < 0) break;
+    rc = 0;
+    float mean = (float)(sum / reps);
+    if (mean < best) {
+      best = mean;
+      best_c = cand[c];
+    }
+  }
+  fused_set_chunk_rows_for(p->res, rc == 0 ? best_c : pinned0);
+  (void)hipFree(save);
+  if (rc < 0) return rc;
+  if (best_rows) *best_rows = best_c;
+  if (best_ms) *best_ms = best;
+  return SILT_OK;
+}
+
 int silt_pair_last_fallback(const silt_pair* p) { return p ? p->last_fallback : SILT_EINVAL; }
 
 }  // extern "C"

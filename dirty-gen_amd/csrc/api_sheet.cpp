@@ -127,6 +127,17 @@ void silt_sheet_opts_default(silt_sheet_opts* o) {
 
 const char* silt_version(void) { return "silt_amd 0.1 gfx950"; }
 
+int silt_set_chunk_rows(size_t res, int rows) {
+  if (res < 1 || res > 65536 || rows < 0) return SILT_EINVAL;
+  fused_set_chunk_rows_for((int)res, rows);
+  return SILT_OK;
+}
+
+int silt_chunk_rows(size_t res) {
+  if (res < 1 || res > 65536) return SILT_EINVAL;
+  return fused_pinned_chunk_rows((int)res);
+}
+
 int silt_step_sheet_ex(float* h, float* u, float* v, const float* bed, const float* nu, const uint8_t* gate,
                        size_t len, size_t res, float cell, float dt, silt_drive drive, const silt_sheet_opts* opts) {
   if (!h || !u || !v || !bed || !nu || !gate) return SILT_EINVAL;

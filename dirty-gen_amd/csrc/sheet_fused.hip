@@ -15,15 +15,17 @@
 //   S8  x filters of b                                  row r-8
 //   S9  z filters -> correction potential q             row r-10
 //   S10 correction kick, damping, speed cap -> uf, vf   row r-11
-//   S11 open transport along x -> h1                    row r-12
-//   S12 open transport along z, dry cut, store          row r-13
+//   S11 open transport along x -> h1                    rows r-12 .. r-14
+//   S12 open transport along z, dry cut, store          row r-15
 //
 // A value a later stage reads across lanes goes through LDS, in a two-slot
 // ring (slot = row parity), and is read one iteration after it is written.
 // So one barrier per iteration orders every exchange: it separates this
 // row's writes from the next row's reads, and this row's reads of a slot from
 // the next row's writes to it.  The row loop is unrolled by two, so the
-// parity and with it every LDS offset is a compile-time constant.
+// parity and with it every LDS offset is a compile-time constant.  The x
+// transport is itself pipelined over three rows: each lane computes the two
+// fluxes of its own face (col, col+1) and takes its neighbour's from LDS.
 // A value a later stage reads only in its own column (the vertical
 // neighbours of every z difference, the lags between stages) stays in
 // registers: each lane keeps a short window of the rows it produced.
@@ -46,6 +48,8 @@
 
 #include <math.h>
 
+#include <map>
+#include <mutex>
 #include <type_traits>
 #include <vector>
 
@@ -57,14 +61,16 @@ constexpr int kLanes = 256;
 constexpr int kHaloL = 11;
 constexpr int kHaloR = 10;
 constexpr int kOutW = kLanes - kHaloL - kHaloR;  // 235
-constexpr int kLag = 13;                        // S12 trails S1 by 13 rows
+constexpr int kLag = 15;                        // S12 trails S1 by 15 rows
 constexpr int kRow = 264;                       // LDS row stride (256 + ghosts)
 constexpr int kMaxBoxes = 8;
 
-// LDS rows: a two-slot ring for each value read across lanes.
+// LDS rows: a two-slot ring for each value read across lanes (the x fluxes
+// fa, fb included), and an eight-slot ring of lap(eta) that each lane reads
+// back in its own column six rows later.
 constexpr int R_ETA = 0, R_P = 2, R_VG = 4, R_W = 6, R_UC = 8, R_B = 10, R_Q = 12, R_UF = 14, R_HX = 16,
-              R_GT = 18;
-constexpr int kRows = 20;
+              R_GT = 18, R_FA = 20, R_FB = 22, R_LW = 24;
+constexpr int kRows = 32;
 
 // Ring slot of row r - k when r has parity P.
 template <int P, int K>
@@ -178,7 +184,6 @@ __global__ __launch_bounds__(kLanes, 4) void k_sheet_stream(FusedArgs fa) {
   const unsigned colmask = FULL ? ~0u : colbits(col);
   const unsigned colmask_r = FULL ? (col + 1 < res ? ~0u : 0u) : colbits(col + 1);
   const unsigned colmask_l1 = FULL ? (col >= 1 ? ~0u : 0u) : colbits(col - 1);
-  const unsigned colmask_l2 = FULL ? (col >= 2 ? ~0u : 0u) : colbits(col - 2);
   auto rowmask = [&](int y) -> unsigned {
     unsigned m = 0;
     for (int k = 0; k < L.boxes.n; k++)
@@ -250,18 +255,24 @@ __global__ __launch_bounds__(kLanes, 4) void k_sheet_stream(FusedArgs fa) {
   // after its stage has run):
   float E[4] = {0, 0, 0, 0};                               // eta      r-3 .. r
   float GX[5] = {0, 0, 0, 0, 0}, SX[5] = {0, 0, 0, 0, 0};  // x filters of eta  r-5 .. r-1
-  float LW[7] = {0, 0, 0, 0, 0, 0, 0};                     // lap(eta) r-7 .. r-1
   float PW[3] = {0, 0, 0};                                 // p        r-5 .. r-3
   float UG[3] = {0, 0, 0}, VG[3] = {0, 0, 0};              // snapshot r-6 .. r-4
   float WW[3] = {0, 0, 0};                                 // w        r-7 .. r-5
   float UC[6] = {0, 0, 0, 0, 0, 0}, VC[6] = {0, 0, 0, 0, 0, 0};  // confined r-11 .. r-6
   float GB[5] = {0, 0, 0, 0, 0}, SB[5] = {0, 0, 0, 0, 0};  // x filters of b    r-12 .. r-8
   float QW[3] = {0, 0, 0};                                 // q        r-12 .. r-10
-  float UF[3] = {0, 0, 0};                                 // final u  r-13 .. r-11
-  float VF[5] = {0, 0, 0, 0, 0};                           // final v  r-15 .. r-11
-  float H1[4] = {0, 0, 0, 0};                              // x-moved h r-15 .. r-12
-  float GW[5] = {1, 1, 1, 1, 1};                           // gate     r-15 .. r-11
-  unsigned RM[14] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};  // RM[k] = row mask of r-k
+  float UF[5] = {0, 0, 0, 0, 0};                           // final u  r-15 .. r-11
+  float VF[7] = {0, 0, 0, 0, 0, 0, 0};                     // final v  r-17 .. r-11
+  float H1[4] = {0, 0, 0, 0};                              // x-moved h r-17 .. r-14
+  float GW[7] = {1, 1, 1, 1, 1, 1, 1};                     // gate     r-17 .. r-11
+  unsigned RM[16] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};  // RM[k] = row mask of r-k
+  // x transport carried between its three rows: this lane's left-going flux
+  // of face (col, col+1) and its depth (rows r-13, r-14), the parts of the
+  // right-going flux known before the neighbour's (row r-13), and the
+  // left-going flux of face (col-1, col) and the right-going flux of face
+  // (col, col+1) (row r-14).
+  float xa1 = 0.0f, xa2 = 0.0f, xh1 = 0.0f, xh2 = 0.0f, xhdf = 0.0f, xflm1 = 0.0f, xb = 0.0f;
+  bool xpre = false;
   // z fluxes of the edge (y, y+1) of the previous S12 row y, which is the
   // edge (y-1, y) of the next: up-going and down-going.
   float c_fu_0 = 0.0f, c_fd_0 = 0.0f;
@@ -281,33 +292,77 @@ __global__ __launch_bounds__(kLanes, 4) void k_sheet_stream(FusedArgs fa) {
   const bool gate_uni = a.gate_uniform != 0;
   const bool vort = a.vort > 0.0f;
 
-  // Global loads run two rows ahead of their use: the loads of row r + 2
-  // are issued while row r computes, into the registers of r's parity.
+  // Global loads run one row ahead of their use: the loads of row r + 1 are
+  // issued at the start of row r and consumed when row r + 1 starts.
   //   pf_h, pf_b   h and bed of S1's row         (r)
   //   pf_h4, pf_u4, pf_v4   h, u, v of S4's row  (r-4)
   //   pf_h11       h of S10's row                (r-11)
   // S6 reads h of row r-6, which S4 loaded two rows earlier (h6).
-  float pf_h[2], pf_b[2], pf_h4[2], pf_u4[2], pf_v4[2], pf_h11[2], h6[2] = {0.0f, 0.0f};
-  auto prefetch = [&](int q, int r) {
-    pf_h[q] = load(L.h, r);
-    pf_b[q] = load(fa.bed, r);
-    pf_h4[q] = load(L.h, r - 4);
-    pf_u4[q] = load(L.u, r - 4);
-    pf_v4[q] = load(L.v, r - 4);
-    pf_h11[q] = load(L.h, r - 11);
+  float pf_h, pf_b, pf_h4, pf_u4, pf_v4, pf_h11, h6[2] = {0.0f, 0.0f};
+  auto prefetch = [&](auto edge, int r) {
+    constexpr bool EDGE = decltype(edge)::value;
+    auto ld = [&](const float* p, int y) -> float {
+      if constexpr (EDGE)
+        return load(p, y);
+      else
+        return p[(size_t)y * res + colc];
+    };
+    pf_h = ld(L.h, r);
+    pf_b = ld(fa.bed, r);
+    pf_h4 = ld(L.h, r - 4);
+    pf_u4 = ld(L.u, r - 4);
+    pf_v4 = ld(L.v, r - 4);
+    pf_h11 = ld(L.h, r - 11);
   };
 
-  auto row = [&](auto par, int r) {
+  // One row of the pipeline.  EDGE = false is the body for rows whose
+  // stencils and loads all lie inside the grid: no clamps, no edge selects,
+  // and with FULL no row masks.
+  auto row = [&](auto par, auto edge, int r) {
     constexpr int P = decltype(par)::value;
+    constexpr bool EDGE = decltype(edge)::value;
+    if constexpr (!FULL) {
 #pragma unroll
-    for (int k = 13; k > 1; k--) RM[k] = RM[k - 1];
-    RM[1] = FULL ? ((unsigned)(r - 1) < (unsigned)res ? 1u : 0u) : rowmask(r - 1);
+      for (int k = 15; k > 1; k--) RM[k] = RM[k - 1];
+      RM[1] = rowmask(r - 1);
+    }
+    // Row mask of row r - k.
+    auto rmk = [&](int k) -> unsigned {
+      if constexpr (FULL)
+        return !EDGE || (unsigned)(r - k) < (unsigned)res ? 1u : 0u;
+      else
+        return RM[k];
+    };
+    auto ld = [&](const float* p, int y) -> float {
+      if constexpr (EDGE)
+        return load(p, y);
+      else
+        return p[(size_t)y * res + colc];
+    };
+    auto abv = [&](int y, float c, float p) -> float {
+      if constexpr (EDGE)
+        return above(y, c, p);
+      else
+        return p;
+    };
+    auto blw = [&](int y, float m, float c) -> float {
+      if constexpr (EDGE)
+        return below(y, m, c);
+      else
+        return m;
+    };
+    auto zac = [&](const float* w, int y, const float* k) -> float {
+      if constexpr (EDGE)
+        return zacc(w, y, k);
+      else
+        return acc5(w[0], w[1], w[2], w[3], w[4], k);
+    };
 
-    const float h4 = pf_h4[P], u4 = pf_u4[P], v4 = pf_v4[P], h11 = pf_h11[P];
+    const float h4 = pf_h4, u4 = pf_u4, v4 = pf_v4, h11 = pf_h11;
     // ---- S1: eta at row r ------------------------------------------------
     {
-      const float ev = pf_b[P] + pf_h[P];
-      prefetch(P, r + 2);
+      const float ev = pf_b + pf_h;
+      prefetch(edge, r + 1);
       put(R_ETA + kSlot<P, 0>, ev);
       push(E, ev);
     }
@@ -317,7 +372,7 @@ __global__ __launch_bounds__(kLanes, 4) void k_sheet_stream(FusedArgs fa) {
       float g = 0.0f, s = 0.0f;
       if (!WIND) {
         float e0 = at(RE, -2), e1 = at(RE, -1), e2 = at(RE, 0), e3 = at(RE, 1), e4 = at(RE, 2);
-        bool inside = (colmask & RM[1]) != 0;
+        bool inside = (colmask & rmk(1)) != 0;
         // accum5 (interior) and accum5_clamped (edges) differ only in a
         // leading 0.0 +, which leaves the value unchanged.
         g = inside ? acc5(e0, e1, e2, e3, e4, kG) : 0.0f;
@@ -328,47 +383,47 @@ __global__ __launch_bounds__(kLanes, 4) void k_sheet_stream(FusedArgs fa) {
       if (PROJECT) {
         // Rows r and r-2 of this column: E[3], E[1] (S1 loaded clamped rows).
         float sum = at(RE, 1) + at(RE, -1) + E[3] + E[1] - 4.0f * E[2];
-        push(LW, A::div(sum, cell * cell, L.invc2));
+        my[(R_LW + ((r - 1) & 7)) * kRow] = A::div(sum, cell * cell, L.invc2);
       }
     }
     // ---- S3: pressure at row r-3 ------------------------------------------
     if (!WIND) {
       const int y = r - 3;
-      float pv = 0.0f;
-      if ((colmask & RM[3]) != 0) {
-        float gz = zacc(GX, y, kG), sz = zacc(SX, y, kS);
-        float filtered = A::fma_(0.38f, sz, 0.62f * gz);
-        pv = A::fma_(0.55f, E[0], 0.45f * filtered);
-      }
+      float gz = zac(GX, y, kG), sz = zac(SX, y, kS);
+      float filtered = A::fma_(0.38f, sz, 0.62f * gz);
+      float pv = (colmask & rmk(3)) != 0 ? A::fma_(0.55f, E[0], 0.45f * filtered) : 0.0f;
       put(R_P + kSlot<P, 3>, pv);
       push(PW, pv);
     }
     // ---- S4: gravity / wind blend at row r-4 -> snapshot ------------------
     {
       const int y = r - 4;
-      float hv = h4;
+      // The update is computed for every lane and kept on wet box cells:
+      // selects, not branches.
+      const float hv = h4;
       float uu = u4;
       float vv = v4;
-      bool inside = (colmask & RM[4]) != 0;
-      if (inside && hv > eps) {
-        if (WIND) {
-          uu = A::fma_(a.wx - uu, a.blend, uu);
-          vv = A::fma_(a.wz - vv, a.blend, vv);
+      const bool act = (colmask & rmk(4)) != 0 && hv > eps;
+      float un, vn;
+      if (WIND) {
+        un = A::fma_(a.wx - uu, a.blend, uu);
+        vn = A::fma_(a.wz - vv, a.blend, vv);
+      } else {
+        constexpr int RP = R_P + kSlot<P, 4>;
+        float px = A::div(at(RP, 1) - at(RP, -1), two, inv2c);
+        float pz = A::div(abv(y, PW[1], PW[2]) - blw(y, PW[0], PW[1]), two, inv2c);
+        float mob = nu_uni ? a.mob0 : sheet_mobility(ld(L.nu, y));
+        if (EXACT) {
+          un = uu + -kGravity * px * mob * a.dt;
+          vn = vv + -kGravity * pz * mob * a.dt;
         } else {
-          constexpr int RP = R_P + kSlot<P, 4>;
-          float px = A::div(at(RP, 1) - at(RP, -1), two, inv2c);
-          float pz = A::div(above(y, PW[1], PW[2]) - below(y, PW[0], PW[1]), two, inv2c);
-          float mob = nu_uni ? a.mob0 : sheet_mobility(load(L.nu, y));
-          if (EXACT) {
-            uu = uu + -kGravity * px * mob * a.dt;
-            vv = vv + -kGravity * pz * mob * a.dt;
-          } else {
-            float k = -kGravity * mob * a.dt;
-            uu = __builtin_fmaf(k, px, uu);
-            vv = __builtin_fmaf(k, pz, vv);
-          }
+          float k = -kGravity * mob * a.dt;
+          un = __builtin_fmaf(k, px, uu);
+          vn = __builtin_fmaf(k, pz, vv);
         }
       }
+      uu = act ? un : uu;
+      vv = act ? vn : vv;
       put(R_VG + kSlot<P, 4>, vv);
       push(UG, uu);
       push(VG, vv);
@@ -378,7 +433,7 @@ __global__ __launch_bounds__(kLanes, 4) void k_sheet_stream(FusedArgs fa) {
       const int y = r - 5;
       constexpr int RV = R_VG + kSlot<P, 5>;
       float dvdx = A::div(at(RV, 1) - at(RV, -1), two, inv2c);
-      float dudz = A::div(above(y, UG[1], UG[2]) - below(y, UG[0], UG[1]), two, inv2c);
+      float dudz = A::div(abv(y, UG[1], UG[2]) - blw(y, UG[0], UG[1]), two, inv2c);
       float w = dvdx - dudz;
       put(R_W + kSlot<P, 5>, w);
       push(WW, w);
@@ -389,30 +444,25 @@ __global__ __launch_bounds__(kLanes, 4) void k_sheet_stream(FusedArgs fa) {
       float uu = UG[0];
       float vv = VG[0];
       if (vort) {
-        float hv = h6[P];
-        bool inside = (colmask & RM[6]) != 0;
-        if (inside && hv > eps) {
-          float w = WW[1];
-          if (isfinite(w) && !(fabsf(w) < 1.0e-4f)) {
-            constexpr int RW = R_W + kSlot<P, 6>;
-            float wxp = fabsf(at(RW, 1)), wxm = fabsf(at(RW, -1));
-            float wzp = fabsf(above(y, WW[1], WW[2])), wzm = fabsf(below(y, WW[0], WW[1]));
-            float nx = A::div(wxp - wxm, two, inv2c);
-            float nz = A::div(wzp - wzm, two, inv2c);
-            float du, dv;
-            if (EXACT) {
-              du = a.vort * a.dt * (-nz * w);
-              dv = a.vort * a.dt * (nx * w);
-            } else {
-              du = L.cdt * (-nz * w);
-              dv = L.cdt * (nx * w);
-            }
-            if (isfinite(du) && isfinite(dv) && A::fma_(du, du, dv * dv) <= 0.0625f) {
-              uu = uu + du;
-              vv = vv + dv;
-            }
-          }
+        const float hv = h6[P];
+        const float w = WW[1];
+        constexpr int RW = R_W + kSlot<P, 6>;
+        float wxp = fabsf(at(RW, 1)), wxm = fabsf(at(RW, -1));
+        float wzp = fabsf(abv(y, WW[1], WW[2])), wzm = fabsf(blw(y, WW[0], WW[1]));
+        float nx = A::div(wxp - wxm, two, inv2c);
+        float nz = A::div(wzp - wzm, two, inv2c);
+        float du, dv;
+        if (EXACT) {
+          du = a.vort * a.dt * (-nz * w);
+          dv = a.vort * a.dt * (nx * w);
+        } else {
+          du = L.cdt * (-nz * w);
+          dv = L.cdt * (nx * w);
         }
+        const bool ok = (colmask & rmk(6)) != 0 && hv > eps && isfinite(w) && !(fabsf(w) < 1.0e-4f) &&
+                        isfinite(du) && isfinite(dv) && A::fma_(du, du, dv * dv) <= 0.0625f;
+        uu = ok ? uu + du : uu;
+        vv = ok ? vv + dv : vv;
       }
       put(R_UC + kSlot<P, 6>, uu);
       push(UC, uu);
@@ -422,35 +472,31 @@ __global__ __launch_bounds__(kLanes, 4) void k_sheet_stream(FusedArgs fa) {
     // ---- S7: divergence source at row r-7 ---------------------------------
     if (PROJECT) {
       const int y = r - 7;
-      float bv = 0.0f;
-      if ((colmask & RM[7]) != 0) {
-        constexpr int RU = R_UC + kSlot<P, 7>;
-        float ux = A::div(at(RU, 1) - at(RU, -1), two, inv2c);
-        // VC holds rows r-11..r-6: y-1, y, y+1 at 3, 4, 5.
-        float vz = A::div(above(y, VC[4], VC[5]) - below(y, VC[3], VC[4]), two, inv2c);
-        float div = ux + vz;
-        float lp = LW[0];
-        float ov = nu_uni ? a.ov0 : overload_of(load(L.nu, y));
-        if (EXACT)
-          bv = div * ov - 0.015f * ov * lp;
-        else
-          bv = ov * __builtin_fmaf(-0.015f, lp, div);
-      }
+      constexpr int RU = R_UC + kSlot<P, 7>;
+      float ux = A::div(at(RU, 1) - at(RU, -1), two, inv2c);
+      // VC holds rows r-11..r-6: y-1, y, y+1 at 3, 4, 5.
+      float vz = A::div(abv(y, VC[4], VC[5]) - blw(y, VC[3], VC[4]), two, inv2c);
+      float div = ux + vz;
+      float lp = my[(R_LW + ((r - 7) & 7)) * kRow];
+      float ov = nu_uni ? a.ov0 : overload_of(ld(L.nu, y));
+      float bv;
+      if (EXACT)
+        bv = div * ov - 0.015f * ov * lp;
+      else
+        bv = ov * __builtin_fmaf(-0.015f, lp, div);
+      bv = (colmask & rmk(7)) != 0 ? bv : 0.0f;
       put(R_B + kSlot<P, 7>, bv);
     }
     // ---- S8: x filters of b at row r-8; S9: q at row r-10 ------------------
     if (PROJECT) {
       constexpr int RB = R_B + kSlot<P, 8>;
-      bool inside = (colmask & RM[8]) != 0;
+      bool inside = (colmask & rmk(8)) != 0;
       float b0 = at(RB, -2), b1 = at(RB, -1), b2 = at(RB, 0), b3 = at(RB, 1), b4 = at(RB, 2);
       push(GB, inside ? acc5(b0, b1, b2, b3, b4, kG) : 0.0f);
       push(SB, inside ? acc5(b0, b1, b2, b3, b4, kS) : 0.0f);
       const int yq = r - 10;
-      float qv = 0.0f;
-      if ((colmask & RM[10]) != 0) {
-        float gz = zacc(GB, yq, kG), sz = zacc(SB, yq, kS);
-        qv = A::fma_(0.38f, sz, 0.62f * gz);
-      }
+      float gz = zac(GB, yq, kG), sz = zac(SB, yq, kS);
+      float qv = (colmask & rmk(10)) != 0 ? A::fma_(0.38f, sz, 0.62f * gz) : 0.0f;
       put(R_Q + kSlot<P, 10>, qv);
       push(QW, qv);
     }
@@ -459,91 +505,92 @@ __global__ __launch_bounds__(kLanes, 4) void k_sheet_stream(FusedArgs fa) {
       const int y = r - 11;
       float uu = UC[0];
       float vv = VC[0];
-      float hv = h11;
-      bool inside = (colmask & RM[11]) != 0;
-      if (inside && hv > eps) {
-        if (PROJECT) {
-          constexpr int RQ = R_Q + kSlot<P, 11>;
-          float qx = A::div(at(RQ, 1) - at(RQ, -1), two, inv2c);
-          float qz = A::div(above(y, QW[1], QW[2]) - below(y, QW[0], QW[1]), two, inv2c);
-          if (EXACT) {
-            uu = uu + 0.012f * a.dt * qx;
-            vv = vv + 0.012f * a.dt * qz;
-          } else {
-            uu = __builtin_fmaf(L.kdt, qx, uu);
-            vv = __builtin_fmaf(L.kdt, qz, vv);
-          }
+      const float hv = h11;
+      const bool act = (colmask & rmk(11)) != 0 && hv > eps;
+      float un = uu, vn = vv;
+      if (PROJECT) {
+        constexpr int RQ = R_Q + kSlot<P, 11>;
+        float qx = A::div(at(RQ, 1) - at(RQ, -1), two, inv2c);
+        float qz = A::div(abv(y, QW[1], QW[2]) - blw(y, QW[0], QW[1]), two, inv2c);
+        if (EXACT) {
+          un = uu + 0.012f * a.dt * qx;
+          vn = vv + 0.012f * a.dt * qz;
+        } else {
+          un = __builtin_fmaf(L.kdt, qx, uu);
+          vn = __builtin_fmaf(L.kdt, qz, vv);
         }
-        float decay = nu_uni ? a.decay0 : expf(-sheet_damp(load(L.nu, y)) * a.dt);
-        uu = finite_speed(uu * decay, a.lim);
-        vv = finite_speed(vv * decay, a.lim);
       }
+      float decay = nu_uni ? a.decay0 : expf(-sheet_damp(ld(L.nu, y)) * a.dt);
+      un = finite_speed(un * decay, a.lim);
+      vn = finite_speed(vn * decay, a.lim);
+      uu = act ? un : uu;
+      vv = act ? vn : vv;
       put(R_UF + kSlot<P, 11>, uu);
       push(UF, uu);
       push(VF, vv);  // the z transport reads only this lane's column
       put(R_HX + kSlot<P, 11>, hv);
       if (!gate_uni) {
-        int yc = y < 0 ? 0 : (y >= res ? res - 1 : y);
+        int yc = EDGE ? (y < 0 ? 0 : (y >= res ? res - 1 : y)) : y;
         float gv = (float)L.gate[(size_t)yc * res + colc];
         put(R_GT + kSlot<P, 11>, gv);
         push(GW, gv);
       }
     }
-    // ---- S11: transport along x at row r-12 --------------------------------
+    // ---- S11: transport along x over rows r-12 .. r-14 ---------------------
     {
-      const int yx = r - 12;
-      const unsigned rmx = RM[12];
-      constexpr int RUF = R_UF + kSlot<P, 12>, RHX = R_HX + kSlot<P, 12>, RGT = R_GT + kSlot<P, 12>;
-      auto gate_at = [&](int d) -> bool { return gate_uni ? true : at(RGT, d) != 0.0f; };
-      // Edge (c, c+1) exists when both cells lie in one box.
-      const bool e_m2 = (colmask_l2 & colmask_l1 & rmx) != 0;  // edge (col-2, col-1)
-      const bool e_m1 = (colmask_l1 & colmask & rmx) != 0;     // edge (col-1, col)
-      const bool e_0 = (colmask & colmask_r & rmx) != 0;       // edge (col, col+1)
       const float dtc = L.dtc;
-      // Left-going flux of edge (k, k+1) (donor k+1), k = col + d.
-      auto fl = [&](int d) -> float {
-        float speed = 0.5f * (at(RUF, d) + at(RUF, d + 1));
-        float hd = at(RHX, d + 1), hrv = at(RHX, d);
-        bool live = !(fabsf(speed) < 1.0e-5f);
-        bool gate_shut = !gate_at(d) && hrv <= 1.0e-6f;
-        float frac = fminf(EXACT ? fabsf(speed) * a.dt / cell : fabsf(speed) * dtc, 0.45f);
-        float dh = fminf(hd * frac, hd);
-        bool ok = live && !(hd <= 1.0e-8f) && !gate_shut && !(dh <= 1.0e-8f) && !(speed > 0.0f);
-        return ok ? dh : 0.0f;
-      };
-      // Right-going flux of edge (k, k+1) (donor k) with the donor's removed
-      // sum holding the left-going flux of edge (k-1, k).
-      auto fr = [&](int d, float prev) -> float {
-        float speed = 0.5f * (at(RUF, d) + at(RUF, d + 1));
-        float hd = at(RHX, d), hrv = at(RHX, d + 1);
-        bool live = !(fabsf(speed) < 1.0e-5f);
-        float room = hd - (0.0f + prev);
-        bool gate_shut = !gate_at(d + 1) && hrv <= 1.0e-6f;
-        float frac = fminf(EXACT ? fabsf(speed) * a.dt / cell : fabsf(speed) * dtc, 0.45f);
-        float dh = fminf(hd * frac, room);
-        bool ok = live && !(room <= 1.0e-8f) && !gate_shut && !(dh <= 1.0e-8f) && speed > 0.0f;
-        return ok ? dh : 0.0f;
-      };
-      float fl_m2 = e_m2 ? fl(-2) : 0.0f;
-      float fl_m1 = e_m1 ? fl(-1) : 0.0f;
-      float fr_m1 = e_m1 ? fr(-1, fl_m2) : 0.0f;
-      float fl_0 = e_0 ? fl(0) : 0.0f;
-      float fr_0 = e_0 ? fr(0, fl_m1) : 0.0f;
-      float rem = (0.0f + fl_m1) + fr_0;
-      float add = (0.0f + fr_m1) + fl_0;
-      float hv = at(RHX, 0);
-      float moved = fmaxf(hv - rem + add, 0.0f);
-      float h1 = (rem > 0.0f || add > 0.0f) ? moved : hv;
-      if (want_stats && out_col && yx >= y0 && yx < y1)
-        acc_res += ((double)h1 - (double)hv) - ((double)add - (double)rem);
+      // X1, row r-12: the left-going flux of face (col, col+1) (donor col+1)
+      // goes to LDS; of its right-going flux (donor col) all but the room,
+      // which needs the neighbour's left-going flux.
+      constexpr int RUF = R_UF + kSlot<P, 12>, RHX = R_HX + kSlot<P, 12>, RGT = R_GT + kSlot<P, 12>;
+      const bool e_0 = (colmask & colmask_r & rmk(12)) != 0;
+      const float speed = 0.5f * (at(RUF, 0) + at(RUF, 1));
+      const float hc = at(RHX, 0), hr = at(RHX, 1);
+      const bool live = !(fabsf(speed) < 1.0e-5f);
+      const float frac = fminf(EXACT ? fabsf(speed) * a.dt / cell : fabsf(speed) * dtc, 0.45f);
+      const bool shut_l = !gate_uni && at(RGT, 0) == 0.0f && hc <= 1.0e-6f;
+      const bool shut_r = !gate_uni && at(RGT, 1) == 0.0f && hr <= 1.0e-6f;
+      const float dl = fminf(hr * frac, hr);
+      const bool ok_l = e_0 && live && !(hr <= 1.0e-8f) && !shut_l && !(dl <= 1.0e-8f) && !(speed > 0.0f);
+      const float fa = ok_l ? dl : 0.0f;
+      my[(R_FA + kSlot<P, 12>) * kRow] = fa;
+      const bool pre = e_0 && live && !shut_r && speed > 0.0f;
+      const float hdf = hc * frac;
+      // X2, row r-13: the right-going flux of face (col, col+1), whose donor
+      // has already lost the left-going flux of face (col-1, col).
+      const bool e_m1b = (colmask_l1 & colmask & rmk(13)) != 0;
+      const float fl_m1 = e_m1b ? at(R_FA + kSlot<P, 13>, -1) : 0.0f;
+      const float room = xh1 - (0.0f + fl_m1);
+      const float dr = fminf(xhdf, room);
+      const float fb = (xpre && !(room <= 1.0e-8f) && !(dr <= 1.0e-8f)) ? dr : 0.0f;
+      my[(R_FB + kSlot<P, 13>) * kRow] = fb;
+      // X3, row r-14: the depth after the x pass.
+      const int yx = r - 14;
+      const bool e_m1c = (colmask_l1 & colmask & rmk(14)) != 0;
+      const float fr_m1 = e_m1c ? at(R_FB + kSlot<P, 14>, -1) : 0.0f;
+      const float rem = (0.0f + xflm1) + xb;
+      const float add = (0.0f + fr_m1) + xa2;
+      const float hv = xh2;
+      const float moved = fmaxf(hv - rem + add, 0.0f);
+      const float h1 = (rem > 0.0f || add > 0.0f) ? moved : hv;
+      if (want_stats)
+        acc_res += out_col && yx >= y0 && yx < y1 ? ((double)h1 - (double)hv) - ((double)add - (double)rem) : 0.0;
       push(H1, h1);
+      xa2 = xa1;
+      xa1 = fa;
+      xh2 = xh1;
+      xh1 = hc;
+      xhdf = hdf;
+      xpre = pre;
+      xflm1 = fl_m1;
+      xb = fb;
     }
-    // ---- S12: transport along z, dry cut, store at row r-13 ----------------
+    // ---- S12: transport along z, dry cut, store at row r-15 ----------------
     {
-      const int y = r - 13;
-      // H1, VF and GW hold rows y-2..y+1 at indices 0..3; RM[13], RM[12] are
+      const int y = r - 15;
+      // H1, VF and GW hold rows y-2..y+1 at indices 0..3; rmk(15), rmk(14) are
       // the masks of rows y, y+1.
-      const bool e_0 = (colmask & RM[13] & RM[12]) != 0;  // edge (y, y+1)
+      const bool e_0 = (colmask & rmk(15) & rmk(14)) != 0;  // edge (y, y+1)
       const float dtc = L.dtc;
       auto gate_ok = [&](float g) -> bool { return gate_uni ? true : g != 0.0f; };
       // Up-going flux of edge (k, k+1) (donor k+1 = window index k+1).
@@ -571,8 +618,10 @@ __global__ __launch_bounds__(kLanes, 4) void k_sheet_stream(FusedArgs fa) {
       // The fluxes of the two edges above come from the previous row, where
       // they were the edges (y-1, y) and (y, y+1) of that row.
       const float fu_m1 = c_fu_0, fd_m1 = c_fd_0;
-      const float fu_0 = e_0 ? fu(2, gate_ok(GW[2])) : 0.0f;
-      const float fd_0 = e_0 ? fd(2, fu_m1, gate_ok(GW[3])) : 0.0f;
+      const float fu_c = fu(2, gate_ok(GW[2]));
+      const float fd_c = fd(2, fu_m1, gate_ok(GW[3]));
+      const float fu_0 = e_0 ? fu_c : 0.0f;
+      const float fd_0 = e_0 ? fd_c : 0.0f;
       c_fu_0 = fu_0;
       c_fd_0 = fd_0;
       float rem = (0.0f + fu_m1) + fd_0;
@@ -582,29 +631,31 @@ __global__ __launch_bounds__(kLanes, 4) void k_sheet_stream(FusedArgs fa) {
       float h2 = (rem > 0.0f || add > 0.0f) ? moved : hv;
       float uo = UF[0];
       float vo = VF[2];
-      bool inside = (colmask & RM[13]) != 0;
+      const bool inside = (colmask & rmk(15)) != 0;
       const bool store = out_col && y >= y0 && y < y1;
-      if (want_stats && store) acc_res += ((double)h2 - (double)hv) - ((double)add - (double)rem);
-      if (inside && eps > 0.0f && h2 < eps) {
-        if (want_stats && store) acc_cut += (double)h2;
-        h2 = 0.0f;
-        uo = 0.0f;
-        vo = 0.0f;
+      const bool cut = inside && eps > 0.0f && h2 < eps;
+      if (want_stats) {
+        // Adding +0.0 for the lanes and rows that are not stored keeps the
+        // sums unchanged without a branch.
+        acc_res += store ? ((double)h2 - (double)hv) - ((double)add - (double)rem) : 0.0;
+        acc_cut += store && cut ? (double)h2 : 0.0;
+      }
+      h2 = cut ? 0.0f : h2;
+      uo = cut ? 0.0f : uo;
+      vo = cut ? 0.0f : vo;
+      if (want_stats) {
+        acc_vol += store ? (double)fmaxf(h2, 0.0f) : 0.0;
+        const bool wet = store && h2 > eps;
+        acc_wet += wet ? 1u : 0u;
+        acc_j0 = wet ? min(acc_j0, y) : acc_j0;
+        acc_j1 = wet ? y : acc_j1;  // rows come in increasing order
+        acc_s2 = store ? fmaxf(acc_s2, uo * uo + vo * vo) : acc_s2;
       }
       if (store) {
         size_t idx = (size_t)y * res + col;
         L.ho[idx] = h2;
         L.uo[idx] = uo;
         L.vo[idx] = vo;
-        if (want_stats) {
-          acc_vol += (double)fmaxf(h2, 0.0f);
-          if (h2 > eps) {
-            acc_wet++;
-            acc_j0 = min(acc_j0, y);
-            acc_j1 = max(acc_j1, y);
-          }
-          acc_s2 = fmaxf(acc_s2, uo * uo + vo * vo);
-        }
       }
     }
     __syncthreads();
@@ -615,11 +666,28 @@ __global__ __launch_bounds__(kLanes, 4) void k_sheet_stream(FusedArgs fa) {
   // never stored.
   const int r_begin = (y0 - kLag) & ~1;
   const int r_end = y1 - 1 + kLag;
-  prefetch(0, r_begin);
-  prefetch(1, r_begin + 1);
-  for (int r = r_begin; r <= r_end; r += 2) {
-    row(std::integral_constant<int, 0>{}, r);
-    row(std::integral_constant<int, 1>{}, r + 1);
+  using Even = std::integral_constant<int, 0>;
+  using Odd = std::integral_constant<int, 1>;
+  using Edge = std::true_type;
+  using Inside = std::false_type;
+  prefetch(Edge{}, r_begin);
+  // Rows r-20 .. r+5 inside the grid cover every stage row, stencil
+  // neighbour and load of the pair of rows (r, r+1): those pairs take the
+  // interior body.  They form one run, so three loops: top edge, interior,
+  // bottom edge.
+  auto interior = [&](int r) { return r - 20 >= 0 && r + 5 < res; };
+  int r = r_begin;
+  for (; r <= r_end && !interior(r); r += 2) {
+    row(Even{}, Edge{}, r);
+    row(Odd{}, Edge{}, r + 1);
+  }
+  for (; r <= r_end && interior(r); r += 2) {
+    row(Even{}, Inside{}, r);
+    row(Odd{}, Inside{}, r + 1);
+  }
+  for (; r <= r_end; r += 2) {
+    row(Even{}, Edge{}, r);
+    row(Odd{}, Edge{}, r + 1);
   }
 
   if (want_stats) {
@@ -700,6 +768,9 @@ bool boxes_fit(const std::vector<Box>& boxes) {
 }
 
 int g_chunk_rows = 0;  // 0: pick automatically
+// Rows per workgroup pinned for one side length (by the autotuner).
+std::mutex g_chunk_mu;
+std::map<int, int> g_chunk_by_res;
 
 }  // namespace
 
@@ -708,6 +779,11 @@ int g_chunk_rows = 0;  // 0: pick automatically
 // once (one round of workgroups: no tail of late tiles).
 int fused_chunk_rows(int res, int strips, int layers, int resident) {
   if (g_chunk_rows > 0) return g_chunk_rows;
+  {
+    std::lock_guard<std::mutex> lock(g_chunk_mu);
+    auto it = g_chunk_by_res.find(res);
+    if (it != g_chunk_by_res.end()) return it->second;
+  }
   int per_column = resident / (strips * layers);
   if (per_column < 1) per_column = 1;
   int chunk = (res + per_column - 1) / per_column;
@@ -715,6 +791,20 @@ int fused_chunk_rows(int res, int strips, int layers, int resident) {
 }
 
 void fused_set_chunk_rows(int rows) { g_chunk_rows = rows; }
+
+void fused_set_chunk_rows_for(int res, int rows) {
+  std::lock_guard<std::mutex> lock(g_chunk_mu);
+  if (rows > 0)
+    g_chunk_by_res[res] = rows;
+  else
+    g_chunk_by_res.erase(res);
+}
+
+int fused_pinned_chunk_rows(int res) {
+  std::lock_guard<std::mutex> lock(g_chunk_mu);
+  auto it = g_chunk_by_res.find(res);
+  return it == g_chunk_by_res.end() ? 0 : it->second;
+}
 
 template <bool EXACT, bool PROJECT, bool WIND, bool FULL>
 static int resident_tiles() {

@@ -81,5 +81,9 @@ int step_device_planes(const SheetPlanes& s, int res, float cell, float dt, cons
 // launch resident, unless fused_set_chunk_rows pinned one (0 unpins).
 int fused_chunk_rows(int res, int strips, int layers, int resident);
 void fused_set_chunk_rows(int rows);
+// Pin the rows per workgroup for sheets of side res (0 unpins); the pinned
+// value for res, or 0.
+void fused_set_chunk_rows_for(int res, int rows);
+int fused_pinned_chunk_rows(int res);
 
 }  // namespace silt

@@ -108,6 +108,13 @@ def _declare(L):
     L.silt_pair_last_kernel_ms.restype = _F
     L.silt_pair_last_fallback.argtypes = [_P]
     L.silt_pair_last_fallback.restype = C.c_int
+    L.silt_pair_autotune.argtypes = [_P, _F, C.POINTER(C.c_int), _Z, C.c_int, C.POINTER(C.c_int),
+                                     C.POINTER(C.c_float)]
+    L.silt_pair_autotune.restype = C.c_int
+    L.silt_set_chunk_rows.argtypes = [_Z, C.c_int]
+    L.silt_set_chunk_rows.restype = C.c_int
+    L.silt_chunk_rows.argtypes = [_Z]
+    L.silt_chunk_rows.restype = C.c_int
 
 
 def _ptr(a):
@@ -165,6 +172,16 @@ def step_sheet_device(h, u, v, bed, nu, gate, res, cell, dt, drive, nu_uniform=m
     return _check_status(rc)
 
 
+def set_chunk_rows(res, rows):
+    """Rows per workgroup of the fused kernel for sheets of side res (0: default)."""
+    _check_status(lib().silt_set_chunk_rows(res, rows))
+
+
+def chunk_rows(res):
+    """The rows pinned for side res, 0 when the default applies."""
+    return int(_check_status(lib().silt_chunk_rows(res)))
+
+
 class SheetPair:
     """Water and slurry over one bed, resident on the GPU (include/silt_pair.h):
     one ``step`` is the generic-res ``step_sheet`` on both layers with the
@@ -218,6 +235,16 @@ class SheetPair:
 
     def last_fallback(self):
         return lib().silt_pair_last_fallback(self._h)
+
+    def autotune(self, dt, rows=None, reps=5):
+        """Time candidate rows per workgroup of the fused kernel on this
+        handle's state (restored afterwards) and pin the fastest for this
+        side; returns (rows, mean kernel ms), rows 0 being the default."""
+        best, ms = C.c_int(0), C.c_float(0.0)
+        arr = None if not rows else (C.c_int * len(rows))(*rows)
+        _check_status(lib().silt_pair_autotune(self._h, dt, arr, len(rows) if rows else 0, reps, C.byref(best),
+                                               C.byref(ms)))
+        return best.value, ms.value
 
 
 class TirePush(C.Structure):

@@ -65,6 +65,17 @@ float silt_pair_last_kernel_ms(const silt_pair *p);
  * rectangle). */
 int silt_pair_last_fallback(const silt_pair *p);
 
+/* Autotune the fused kernel's rows per workgroup (silt_set_chunk_rows) for
+ * this handle's side: every candidate (n of them in rows; NULL or n = 0 for
+ * a built-in set around the default, 0 meaning the default itself) is timed
+ * over `reps` steps of dt from the handle's current state, which is restored
+ * after each candidate; the fastest is pinned.  best_rows / best_ms (mean
+ * kernel time of a step) may be NULL.  Allocates one temporary copy of the
+ * six planes.  No counterpart in the reference: a tuning step of the GPU
+ * path, run once before a long simulation. */
+int silt_pair_autotune(silt_pair *p, float dt, const int *rows, size_t n, int reps, int *best_rows,
+                       float *best_ms);
+
 #ifdef __cplusplus
 }
 #endif

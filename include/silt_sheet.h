@@ -96,6 +96,14 @@ int silt_step_sheet_device(float *h, float *u, float *v, const float *bed, const
 /* Library identity: "silt_amd <version> gfx950". */
 const char *silt_version(void);
 
+/* Rows of the grid each workgroup of the fused streaming kernel takes, for
+ * sheets of side res; 0 restores the default (the smallest height that keeps
+ * every tile of a launch resident at once).  Process-wide.  No counterpart in
+ * the reference: a tuning knob of the GPU path (see silt_pair_autotune). */
+int silt_set_chunk_rows(size_t res, int rows);
+/* The rows pinned for side res, 0 when the default applies. */
+int silt_chunk_rows(size_t res);
+
 #ifdef __cplusplus
 }
 #endif

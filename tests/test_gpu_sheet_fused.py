@@ -162,3 +162,53 @@ def test_fast_path_conserves_like_the_exact_path_over_many_steps():
     # Fields stay close in the L1 sense after 200 steps (threshold flips aside).
     l1 = float(np.abs(fa[0] - ex[0]).sum()) / float(np.abs(ex[0]).sum())
     assert l1 < 1e-3, l1
+
+
+def _pair_from(sc, path="fast"):
+    n = sc["res"] * sc["res"]
+    z = np.zeros(n, np.float32)
+    p = S.SheetPair(sc["res"], sc["cell"], path=path)
+    p.upload(sc["bed"], sc["water"], z, z, sc["slurry"], z, z, sc["nu_slurry"])
+    return p
+
+
+@pytest.mark.parametrize("path", ["fast", "fused_exact"])
+def test_results_do_not_depend_on_the_chunk_height(path):
+    """Tiles recompute their halo rows and columns with the same arithmetic,
+    so the rows per workgroup leave every value unchanged."""
+    sc = scenes.large_scene(res=700, seed=5, puddles=10)
+    res = sc["res"]
+    outs = []
+    try:
+        for rows in (0, 24, 97):
+            S.set_chunk_rows(res, rows)
+            p = _pair_from(sc, path)
+            for _ in range(3):
+                p.step(1.0 / 120.0)
+            outs.append(p.download())
+            p.close()
+    finally:
+        S.set_chunk_rows(res, 0)
+    for other in outs[1:]:
+        for a, b, name in zip(outs[0], other, ("water", "uw", "vw", "slurry", "us", "vs")):
+            _eq(a, b, name)
+
+
+def test_pair_autotune_restores_the_state_and_pins_the_fastest_height():
+    sc = scenes.large_scene(res=600, seed=6, puddles=8)
+    res = sc["res"]
+    p = _pair_from(sc)
+    try:
+        for _ in range(2):
+            p.step(1.0 / 120.0)
+        before, led0 = p.download(), p.ledger()
+        rows, ms = p.autotune(1.0 / 120.0, rows=[0, 32, 96], reps=2)
+        assert rows in (0, 32, 96) and ms > 0.0
+        assert S.chunk_rows(res) == rows
+        for a, b, name in zip(before, p.download(), ("water", "uw", "vw", "slurry", "us", "vs")):
+            _eq(a, b, f"{name} after autotune")
+        led1 = p.ledger()
+        assert led1.water.gap_m3 == led0.water.gap_m3 and led1.slurry.residue_m3 == led0.slurry.residue_m3
+    finally:
+        S.set_chunk_rows(res, 0)
+        p.close()
