@@ -27,6 +27,11 @@ HBM_PEAK_GBS = 8000.0
 # water h, u, v (12), slurry h, u, v (12); write both layers' h, u, v (24).
 # The slurry viscosity is uniform in this workload, so no viscosity plane.
 ALG_BYTES_PER_CELL = 52
+# HBM bytes per launch of the fused kernel measured with rocprofv3 --pmc
+# FETCH_SIZE and WRITE_SIZE on this workload (default rows per workgroup),
+# each scaled by the counter's reading for a 1 GiB copy (FETCH x 2, WRITE x 1);
+# profiles/k_sheet_stream_4096_two_layer.md holds the runs and commands.
+PMC_TRAFFIC_BYTES = {("fast", 4096): 1_316_000_666}
 
 
 def cpu_baseline(sc, dt, steps):
@@ -59,7 +64,9 @@ def main():
     ap.add_argument("--path", default="fast", choices=["fast", "fused_exact", "exact"])
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-autotune", action="store_true", help="keep the default rows per workgroup")
+    ap.add_argument("--autotune", action="store_true",
+                    help="time candidate rows per workgroup first and keep the fastest (default: the "
+                         "occupancy-derived height, so a kernel trace of the run holds one launch shape)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported on this build "
@@ -78,7 +85,7 @@ def main():
     z = np.zeros(n, np.float32)
     pair.upload(sc["bed"], sc["water"], z, z, sc["slurry"], z, z, sc["nu_slurry"])
     chunk, tune_ms = 0, None
-    if not args.no_autotune and args.path != "exact":
+    if args.autotune and args.path != "exact":
         # Tuning step before the run (state restored afterwards): rows per
         # workgroup of the fused kernel for this sheet.
         chunk, tune_ms = pair.autotune(dt)
@@ -118,7 +125,8 @@ def main():
                    "chunk_rows": chunk or "default",
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": None if args.autotune else PMC_TRAFFIC_BYTES.get((args.path, res)),
                      "kernel": "k_sheet_stream (both layers, one launch)", "kernel_ms": kms,
                      "alg_bytes_per_cell": ALG_BYTES_PER_CELL},
         "ledger": {"water_gap_rel": led.water.gap_m3 / max(led.water.initial_m3, 1e-30),

@@ -280,9 +280,13 @@ __global__ __launch_bounds__(kLanes, 4) void k_sheet_stream(FusedArgs fa) {
   // Epilogue accumulators over the cells this workgroup stores: FP64 volume,
   // dry-cut volume and transport residue (what FP32 rounding and the max(0)
   // clamp added beyond the fluxes), plus the wet bounding box, wet count and
-  // largest squared speed the next step's plan needs.
+  // largest squared speed the next step's plan needs.  A cell's residue is
+  // formed exactly in FP64; the residues and cut depths of one column (each
+  // below 1e-6 m) are summed in FP32, whose error on a column's few hundred
+  // terms is some 1e-13 m, and the column sums reduce in FP64.
   const bool want_stats = L.stats != nullptr;
-  double acc_vol = 0.0, acc_cut = 0.0, acc_res = 0.0;
+  double acc_vol = 0.0;
+  float acc_cut = 0.0f, acc_res = 0.0f;
   unsigned acc_wet = 0;
   int acc_j0 = 0x7fffffff, acc_j1 = -1;
   float acc_s2 = 0.0f;
@@ -574,7 +578,8 @@ __global__ __launch_bounds__(kLanes, 4) void k_sheet_stream(FusedArgs fa) {
       const float moved = fmaxf(hv - rem + add, 0.0f);
       const float h1 = (rem > 0.0f || add > 0.0f) ? moved : hv;
       if (want_stats)
-        acc_res += out_col && yx >= y0 && yx < y1 ? ((double)h1 - (double)hv) - ((double)add - (double)rem) : 0.0;
+        acc_res += out_col && yx >= y0 && yx < y1 ? (float)(((double)h1 - (double)hv) - ((double)add - (double)rem))
+                                                  : 0.0f;
       push(H1, h1);
       xa2 = xa1;
       xa1 = fa;
@@ -637,8 +642,8 @@ __global__ __launch_bounds__(kLanes, 4) void k_sheet_stream(FusedArgs fa) {
       if (want_stats) {
         // Adding +0.0 for the lanes and rows that are not stored keeps the
         // sums unchanged without a branch.
-        acc_res += store ? ((double)h2 - (double)hv) - ((double)add - (double)rem) : 0.0;
-        acc_cut += store && cut ? (double)h2 : 0.0;
+        acc_res += store ? (float)(((double)h2 - (double)hv) - ((double)add - (double)rem)) : 0.0f;
+        acc_cut += store && cut ? h2 : 0.0f;
       }
       h2 = cut ? 0.0f : h2;
       uo = cut ? 0.0f : uo;
@@ -694,10 +699,11 @@ __global__ __launch_bounds__(kLanes, 4) void k_sheet_stream(FusedArgs fa) {
     // The wet columns of this lane span one column; the bounding box reduces
     // over lanes with shuffles, then one atomic per wave.
     int acc_i0 = acc_wet ? col : 0x7fffffff, acc_i1 = acc_wet ? col : -1;
+    double sum_cut = acc_cut, sum_res = acc_res;
     for (int off = 32; off > 0; off >>= 1) {
       acc_vol += __shfl_xor(acc_vol, off);
-      acc_cut += __shfl_xor(acc_cut, off);
-      acc_res += __shfl_xor(acc_res, off);
+      sum_cut += __shfl_xor(sum_cut, off);
+      sum_res += __shfl_xor(sum_res, off);
       acc_wet += __shfl_xor(acc_wet, off);
       acc_i0 = min(acc_i0, __shfl_xor(acc_i0, off));
       acc_i1 = max(acc_i1, __shfl_xor(acc_i1, off));
@@ -716,8 +722,8 @@ __global__ __launch_bounds__(kLanes, 4) void k_sheet_stream(FusedArgs fa) {
       }
       atomicMax(&L.stats->max_s2_bits, __float_as_uint(acc_s2));
       if (L.totals) {
-        atomicAdd(&L.totals->cut, acc_cut);
-        atomicAdd(&L.totals->residue, acc_res);
+        atomicAdd(&L.totals->cut, sum_cut);
+        atomicAdd(&L.totals->residue, sum_res);
       }
     }
   }
