@@ -71,38 +71,6 @@ SheetPlanes layer_planes(silt_pair* p, int layer, int buf) {
 float layer_nu0(const silt_pair* p, int layer) { return layer == 0 ? p->nu_water : p->nu_slurry0; }
 bool layer_nu_uniform(const silt_pair* p, int layer) { return layer == 0 || p->nu_slurry_uniform; }
 
-// Plan of one layer from its statistics.  Returns false when the plan needs
-// the full planes on the host (a wet set that is not one solid rectangle, or
-// a fine cell with sleeping tiles).
-bool plan_from_stats(const silt_pair* p, const SheetStats& st, float dt, StepPlan* plan, bool* need_box_max) {
-  *plan = StepPlan{};
-  *need_box_max = false;
-  if (!(dt > 0.0f)) return true;
-  if (st.wet == 0) return true;  // wet_sum <= eps: the reference returns early
-  if (p->cell <= kFineCell + 1.0e-6f) return false;
-  long w = (long)(st.i1 - st.i0 + 1), hgt = (long)(st.j1 - st.j0 + 1);
-  if ((long long)st.wet != (long long)w * hgt) return false;
-  int res = p->res;
-  plan->boxes.push_back(Box{std::max(st.i0 - 3, 0), std::min(st.i1 + 3, res - 1), std::max(st.j0 - 3, 0),
-                            std::min(st.j1 + 3, res - 1)});
-  // nsub from the largest speed anywhere bounds the box's; when that bound
-  // is one substep it is exact, otherwise the box's own maximum is read.
-  float s2;
-  memcpy(&s2, &st.max_s2_bits, 4);
-  float max_s = fmaxf(0.15f, sqrtf(s2));
-  float q = ceilf((dt * max_s) / (0.42f * fmaxf(p->cell, 1.0e-3f)));
-  int bound = (int)clampf(q, 1.0f, 8.0f);
-  plan->nsub = bound;
-  if (bound > 1) *need_box_max = true;
-  return true;
-}
-
-int nsub_from_s2(float s2, float dt, float cell) {
-  float max_s = fmaxf(0.15f, sqrtf(s2));
-  float q = ceilf((dt * max_s) / (0.42f * fmaxf(cell, 1.0e-3f)));
-  return (int)clampf(q, 1.0f, 8.0f);
-}
-
 }  // namespace
 
 extern "C" {
@@ -209,7 +177,7 @@ int silt_pair_step(silt_pair* p, float dt) {
   bool fast_plan[2], need_max[2];
   p->last_fallback = 0;
   for (int l = 0; l < 2; l++) {
-    fast_plan[l] = plan_from_stats(p, p->h_stats[l], dt, &plans[l], &need_max[l]);
+    fast_plan[l] = plan_from_stats(p->h_stats[l], p->res, p->cell, dt, &plans[l], &need_max[l]);
     if (!fast_plan[l]) {
       // The wet set is not one solid rectangle: read the planes back and
       // plan on the host (connected wet bodies, as sheet_work_boxes does).

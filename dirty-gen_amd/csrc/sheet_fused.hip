@@ -49,6 +49,7 @@
 #include <math.h>
 
 #include <map>
+#include <string.h>
 #include <mutex>
 #include <type_traits>
 #include <vector>
@@ -81,7 +82,15 @@ struct BoxList {
   int i0[kMaxBoxes], i1[kMaxBoxes], j0[kMaxBoxes], j1[kMaxBoxes];
 };
 
-struct LayerArgs {
+// One window of a launch (a layer of a sheet, or one sheet of a batch):
+// its scalar fields, which the kernel copies into registers, and its boxes,
+// which it reads from memory.
+struct LayerHot {
+  const float* bed;
+  int res;
+  int strips;
+  int chunk;  // output rows per workgroup
+  int tile0;  // first tile of this window in the launch
   const float* h;
   const float* u;
   const float* v;
@@ -91,7 +100,6 @@ struct LayerArgs {
   float* uo;
   float* vo;
   StepArgs a;
-  BoxList boxes;
   SheetStats* stats;    // plan statistics of the output (zeroed before launch), or null
   SheetTotals* totals;  // running FP64 ledger lines, or null
   float inv2c;    // 1 / (2 cell)
@@ -101,14 +109,23 @@ struct LayerArgs {
   float cdt;      // vorticity dt
 };
 
+struct LayerArgs {
+  LayerHot hot;
+  BoxList boxes;
+};
+
+// The descriptors are read through the constant address space: they do not
+// change during a launch, so the compiler may load a field wherever it is
+// needed instead of holding every field in registers for the whole loop.
+using ConstLayer = const __attribute__((address_space(4))) LayerArgs;
+using ConstInt = const __attribute__((address_space(4))) int;
+
+// Kernel argument: the windows' descriptors in device memory and, for more
+// than two windows, the window of every tile.
 struct FusedArgs {
-  const float* bed;
-  int res;
-  int strips;
-  int chunk;   // output rows per workgroup
-  int chunks;
-  int tiles;   // strips * chunks per layer
-  LayerArgs layer[2];
+  ConstLayer* table;
+  ConstInt* tile_window;
+  int nwin;
 };
 
 template <bool EXACT>
@@ -143,23 +160,25 @@ __global__ __launch_bounds__(kLanes, 4) void k_sheet_stream(FusedArgs fa) {
   using A = Ar<EXACT>;
   __shared__ float lds[kRows * kRow];
 
-  const int res = fa.res;
   // XCD-aware tile order: the 8 XCDs take workgroups round robin, so when the
   // count divides evenly each XCD gets a contiguous run of tiles (neighbouring
   // strips and chunks share halo rows in that XCD's L2).
   const int total = (int)gridDim.x;
   const int wg = (int)blockIdx.x;
   const int tile_lin = (total % 8 == 0) ? (wg % 8) * (total / 8) + wg / 8 : wg;
-  const int per_layer = fa.tiles;
-  const int layer = tile_lin / per_layer;
-  const int tile = tile_lin - layer * per_layer;
-  const LayerArgs& L = fa.layer[layer];
-  const StepArgs& a = L.a;
-  const int strip = tile % fa.strips;
-  const int chunk = tile / fa.strips;
+  const int win =
+      fa.nwin <= 2 ? (fa.nwin == 2 && tile_lin >= fa.table[1].hot.tile0 ? 1 : 0) : fa.tile_window[tile_lin];
+  ConstLayer& LA = fa.table[win];
+  const auto& L = LA.hot;
+  const auto& boxes = LA.boxes;
+  const auto& a = L.a;
+  const int res = L.res;
+  const int tile = tile_lin - L.tile0;
+  const int strip = tile % L.strips;
+  const int chunk = tile / L.strips;
   const int x0 = strip * kOutW;              // first output column
-  const int y0 = chunk * fa.chunk;           // first output row
-  const int y1 = min(res, y0 + fa.chunk);    // one past the last output row
+  const int y0 = chunk * L.chunk;            // first output row
+  const int y1 = min(res, y0 + L.chunk);     // one past the last output row
   const int t = (int)threadIdx.x;
   const int col = x0 - kHaloL + t;           // this lane's grid column
   const bool col_in = col >= 0 && col < res;
@@ -174,8 +193,8 @@ __global__ __launch_bounds__(kLanes, 4) void k_sheet_stream(FusedArgs fa) {
   // Box membership: bit k set when box k covers the column / row.
   auto colbits = [&](int c) -> unsigned {
     unsigned m = 0;
-    for (int k = 0; k < L.boxes.n; k++)
-      if (c >= L.boxes.i0[k] && c <= L.boxes.i1[k]) m |= 1u << k;
+    for (int k = 0; k < boxes.n; k++)
+      if (c >= boxes.i0[k] && c <= boxes.i1[k]) m |= 1u << k;
     return m;
   };
   // With one box over the whole grid every lane counts as inside (lanes off
@@ -186,8 +205,8 @@ __global__ __launch_bounds__(kLanes, 4) void k_sheet_stream(FusedArgs fa) {
   const unsigned colmask_l1 = FULL ? (col >= 1 ? ~0u : 0u) : colbits(col - 1);
   auto rowmask = [&](int y) -> unsigned {
     unsigned m = 0;
-    for (int k = 0; k < L.boxes.n; k++)
-      if (y >= L.boxes.j0[k] && y <= L.boxes.j1[k]) m |= 1u << k;
+    for (int k = 0; k < boxes.n; k++)
+      if (y >= boxes.j0[k] && y <= boxes.j1[k]) m |= 1u << k;
     return m;
   };
 
@@ -312,7 +331,7 @@ __global__ __launch_bounds__(kLanes, 4) void k_sheet_stream(FusedArgs fa) {
         return p[(size_t)y * res + colc];
     };
     pf_h = ld(L.h, r);
-    pf_b = ld(fa.bed, r);
+    pf_b = ld(L.bed, r);
     pf_h4 = ld(L.h, r - 4);
     pf_u4 = ld(L.u, r - 4);
     pf_v4 = ld(L.v, r - 4);
@@ -734,15 +753,15 @@ namespace {
 LayerArgs make_layer(const SheetPlanes& s, float* ho, float* uo, float* vo, const StepArgs& a,
                      const std::vector<Box>& boxes) {
   LayerArgs L{};
-  L.h = s.h;
-  L.u = s.u;
-  L.v = s.v;
-  L.nu = s.nu;
-  L.gate = s.gate;
-  L.ho = ho;
-  L.uo = uo;
-  L.vo = vo;
-  L.a = a;
+  L.hot.h = s.h;
+  L.hot.u = s.u;
+  L.hot.v = s.v;
+  L.hot.nu = s.nu;
+  L.hot.gate = s.gate;
+  L.hot.ho = ho;
+  L.hot.uo = uo;
+  L.hot.vo = vo;
+  L.hot.a = a;
   L.boxes.n = (int)boxes.size();
   for (size_t k = 0; k < boxes.size() && k < (size_t)kMaxBoxes; k++) {
     L.boxes.i0[k] = boxes[k].i0;
@@ -750,13 +769,13 @@ LayerArgs make_layer(const SheetPlanes& s, float* ho, float* uo, float* vo, cons
     L.boxes.j0[k] = boxes[k].j0;
     L.boxes.j1[k] = boxes[k].j1;
   }
-  L.stats = nullptr;
-  L.totals = nullptr;
-  L.inv2c = 1.0f / (2.0f * a.cell);
-  L.invc2 = 1.0f / (a.cell * a.cell);
-  L.dtc = a.dt / a.cell;
-  L.kdt = 0.012f * a.dt;
-  L.cdt = a.vort * a.dt;
+  L.hot.stats = nullptr;
+  L.hot.totals = nullptr;
+  L.hot.inv2c = 1.0f / (2.0f * a.cell);
+  L.hot.invc2 = 1.0f / (a.cell * a.cell);
+  L.hot.dtc = a.dt / a.cell;
+  L.hot.kdt = 0.012f * a.dt;
+  L.hot.cdt = a.vort * a.dt;
   return L;
 }
 
@@ -830,14 +849,57 @@ static int resident_tiles() {
 }
 
 // The instance for a launch: its resident tile count, or the launch itself.
+namespace {
+
+// Device copies of the launch descriptors, one buffer per stream: launches on
+// one stream run in order, so a buffer is rewritten only after the kernel
+// that read it has finished.
+struct DescBuf {
+  LayerArgs* table = nullptr;
+  int* tile_window = nullptr;
+  size_t cap_win = 0, cap_tiles = 0;
+};
+std::mutex g_desc_mu;
+std::map<hipStream_t, DescBuf> g_desc;
+
+int upload_descs(const std::vector<LayerArgs>& wins, const std::vector<int>& tile_window, hipStream_t st,
+                 FusedArgs* fa) {
+  std::lock_guard<std::mutex> lock(g_desc_mu);
+  DescBuf& b = g_desc[st];
+  if (b.cap_win < wins.size()) {
+    if (b.table) (void)hipFree(b.table);
+    b.table = nullptr;
+    b.cap_win = 0;
+    if (hipMalloc(&b.table, wins.size() * sizeof(LayerArgs)) != hipSuccess) return SILT_ENOMEM;
+    b.cap_win = wins.size();
+  }
+  if (b.cap_tiles < tile_window.size()) {
+    if (b.tile_window) (void)hipFree(b.tile_window);
+    b.tile_window = nullptr;
+    b.cap_tiles = 0;
+    if (hipMalloc(&b.tile_window, tile_window.size() * sizeof(int)) != hipSuccess) return SILT_ENOMEM;
+    b.cap_tiles = tile_window.size();
+  }
+  // Pageable sources: the copies are staged before the calls return.
+  if (hipMemcpyAsync(b.table, wins.data(), wins.size() * sizeof(LayerArgs), hipMemcpyHostToDevice, st) != hipSuccess)
+    return SILT_EDEVICE;
+  if (!tile_window.empty() && hipMemcpyAsync(b.tile_window, tile_window.data(), tile_window.size() * sizeof(int),
+                                             hipMemcpyHostToDevice, st) != hipSuccess)
+    return SILT_EDEVICE;
+  fa->table = (ConstLayer*)b.table;
+  fa->tile_window = tile_window.empty() ? nullptr : (ConstInt*)b.tile_window;
+  fa->nwin = (int)wins.size();
+  return 0;
+}
+
 template <bool EXACT, bool FULL>
-static int dispatch(const FusedArgs* fa, int layers, bool project, bool wind, hipStream_t st) {
+static int dispatch(const FusedArgs* fa, int tiles, bool project, bool wind, hipStream_t st) {
   if (!fa) {
     if (wind) return resident_tiles<EXACT, false, true, FULL>();
     if (project) return resident_tiles<EXACT, true, false, FULL>();
     return resident_tiles<EXACT, false, false, FULL>();
   }
-  dim3 grid((unsigned)(fa->tiles * layers));
+  dim3 grid((unsigned)tiles);
   dim3 blk(kLanes);
   if (wind)
     hipLaunchKernelGGL((k_sheet_stream<EXACT, false, true, FULL>), grid, blk, 0, st, *fa);
@@ -848,41 +910,108 @@ static int dispatch(const FusedArgs* fa, int layers, bool project, bool wind, hi
   return 0;
 }
 
-static int dispatch_any(const FusedArgs* fa, int layers, bool exact, bool full, bool project, bool wind,
+static int dispatch_any(const FusedArgs* fa, int tiles, bool exact, bool full, bool project, bool wind,
                         hipStream_t st) {
-  if (exact) return full ? dispatch<true, true>(fa, layers, project, wind, st)
-                         : dispatch<true, false>(fa, layers, project, wind, st);
-  return full ? dispatch<false, true>(fa, layers, project, wind, st)
-              : dispatch<false, false>(fa, layers, project, wind, st);
+  if (exact) return full ? dispatch<true, true>(fa, tiles, project, wind, st)
+                         : dispatch<true, false>(fa, tiles, project, wind, st);
+  return full ? dispatch<false, true>(fa, tiles, project, wind, st)
+              : dispatch<false, false>(fa, tiles, project, wind, st);
+}
+
+}  // namespace
+
+size_t fused_desc_size() { return sizeof(LayerArgs); }
+
+int fused_build(const FusedLayer* wins, int n, bool exact, int rows, FusedLaunch* out) {
+  if (n < 1 || !out) return SILT_EINVAL;
+  bool project = wins[0].a.project != 0, wind = wins[0].a.wind != 0;
+  bool full = true;
+  for (int k = 0; k < n; k++) {
+    const FusedLayer& f = wins[k];
+    if ((f.a.project != 0) != project || (f.a.wind != 0) != wind || f.res < 1 || !f.in.bed) return SILT_EINVAL;
+    if (!boxes_fit(f.boxes)) return SILT_EUNSUPPORTED;
+    const std::vector<Box>& b = f.boxes;
+    full = full && b.size() == 1 && b[0].i0 == 0 && b[0].j0 == 0 && b[0].i1 == f.res - 1 && b[0].j1 == f.res - 1;
+  }
+  int resident = dispatch_any(nullptr, 0, exact, full, project, wind, nullptr);
+  // Rows per workgroup: pinned by the caller, else for one or two windows of
+  // one size the occupancy-derived height, else the height that gives about
+  // two tiles per resident slot over all windows.
+  int chunk = rows;
+  if (chunk <= 0) {
+    bool same = true;
+    for (int k = 1; k < n; k++) same = same && wins[k].res == wins[0].res;
+    if (n <= 2 && same) {
+      int strips = (wins[0].res + kOutW - 1) / kOutW;
+      chunk = fused_chunk_rows(wins[0].res, strips, n, resident);
+    } else {
+      long long work = 0;
+      for (int k = 0; k < n; k++) work += (long long)((wins[k].res + kOutW - 1) / kOutW) * wins[k].res;
+      long long target = 2LL * resident;
+      chunk = (int)((work + target - 1) / target);
+      if (chunk < 16) chunk = 16;
+    }
+  }
+  out->exact = exact;
+  out->full = full;
+  out->project = project;
+  out->wind = wind;
+  out->chunk = chunk;
+  out->nwin = n;
+  out->descs.assign((size_t)n * sizeof(LayerArgs), 0);
+  out->tile_window.clear();
+  int tiles = 0;
+  for (int k = 0; k < n; k++) {
+    const FusedLayer& f = wins[k];
+    LayerArgs L = make_layer(f.in, f.ho, f.uo, f.vo, f.a, f.boxes);
+    L.hot.bed = f.in.bed;
+    L.hot.res = f.res;
+    L.hot.strips = (f.res + kOutW - 1) / kOutW;
+    L.hot.chunk = chunk;
+    L.hot.tile0 = tiles;
+    L.hot.stats = f.stats;
+    L.hot.totals = f.totals;
+    int wt = L.hot.strips * ((f.res + chunk - 1) / chunk);
+    if (n > 2) out->tile_window.insert(out->tile_window.end(), (size_t)wt, k);
+    tiles += wt;
+    memcpy(out->descs.data() + (size_t)k * sizeof(LayerArgs), &L, sizeof(LayerArgs));
+  }
+  out->tiles = tiles;
+  return 0;
+}
+
+int fused_enqueue(const FusedLaunch& l, const void* dev_table, const int* dev_tile_window, hipStream_t st) {
+  if (!dev_table || (l.nwin > 2 && !dev_tile_window) || l.tiles < 1) return SILT_EINVAL;
+  FusedArgs fa{};
+  fa.table = (ConstLayer*)dev_table;
+  fa.tile_window = l.nwin > 2 ? (ConstInt*)dev_tile_window : nullptr;
+  fa.nwin = l.nwin;
+  dispatch_any(&fa, l.tiles, l.exact, l.full, l.project, l.wind, st);
+  return hipGetLastError() == hipSuccess ? 0 : SILT_EDEVICE;
+}
+
+int fused_substep_windows(const FusedLayer* wins, int n, bool exact, int rows, hipStream_t st) {
+  FusedLaunch l;
+  int rc = fused_build(wins, n, exact, rows, &l);
+  if (rc != 0) return rc;
+  std::vector<LayerArgs> descs((size_t)n);
+  memcpy(descs.data(), l.descs.data(), l.descs.size());
+  FusedArgs fa{};
+  rc = upload_descs(descs, l.tile_window, st, &fa);
+  if (rc != 0) return rc;
+  return fused_enqueue(l, (const void*)fa.table, (const int*)fa.tile_window, st);
 }
 
 int fused_substep_layers(const FusedLayer* layers, int nlayers, const float* bed, int res, bool exact,
                          hipStream_t st) {
   if (nlayers < 1 || nlayers > 2) return SILT_EINVAL;
-  FusedArgs fa{};
-  fa.bed = bed;
-  fa.res = res;
-  fa.strips = (res + kOutW - 1) / kOutW;
-  bool project = layers[0].a.project != 0, wind = layers[0].a.wind != 0;
-  bool full = true;
+  FusedLayer w[2];
   for (int k = 0; k < nlayers; k++) {
-    const std::vector<Box>& b = layers[k].boxes;
-    full = full && b.size() == 1 && b[0].i0 == 0 && b[0].j0 == 0 && b[0].i1 == res - 1 && b[0].j1 == res - 1;
+    w[k] = layers[k];
+    w[k].in.bed = bed;
+    w[k].res = res;
   }
-  int resident = dispatch_any(nullptr, nlayers, exact, full, project, wind, st);
-  fa.chunk = fused_chunk_rows(res, fa.strips, nlayers, resident);
-  fa.chunks = (res + fa.chunk - 1) / fa.chunk;
-  fa.tiles = fa.strips * fa.chunks;
-  for (int k = 0; k < nlayers; k++) {
-    const FusedLayer& f = layers[k];
-    if ((f.a.project != 0) != project || (f.a.wind != 0) != wind) return SILT_EINVAL;
-    if (!boxes_fit(f.boxes)) return SILT_EUNSUPPORTED;
-    fa.layer[k] = make_layer(f.in, f.ho, f.uo, f.vo, f.a, f.boxes);
-    fa.layer[k].stats = f.stats;
-    fa.layer[k].totals = f.totals;
-  }
-  dispatch_any(&fa, nlayers, exact, full, project, wind, st);
-  return hipGetLastError() == hipSuccess ? 0 : SILT_EDEVICE;
+  return fused_substep_windows(w, nlayers, exact, 0, st);
 }
 
 // One planned step of one layer: nsub fused substeps ping-ponging between

@@ -33,7 +33,8 @@ struct SheetTotals {
 
 // One layer of one fused substep: read planes in `in`, write ho/uo/vo.
 struct FusedLayer {
-  SheetPlanes in;
+  SheetPlanes in;  // in.bed is the window's bed
+  int res = 0;     // side of the window (fused_substep_windows)
   float* ho;
   float* uo;
   float* vo;
@@ -56,10 +57,36 @@ void box_max_s2(const float* u, const float* v, int res, const Box& b, unsigned*
 StepArgs make_args(int res, float cell, float dt, const silt_drive& d, float eps, int banks, bool gate_uniform,
                    bool nu_uniform, float nu0);
 
+// Plan of a gravity sheet from its statistics; false when the wet set is not
+// one solid rectangle (or the cell is 2 cm) and the planes must be read back.
+// need_box_max: nsub > 1 from the global maximum speed, to be refined with the
+// box's own maximum (box_max_s2).
+bool plan_from_stats(const SheetStats& st, int res, float cell, float dt, StepPlan* plan, bool* need_box_max);
+// Substep count of a gravity sheet whose boxes' largest squared speed is s2.
+int nsub_from_s2(float s2, float dt, float cell);
+
 // One substep of one or two layers (same res, drive kind and projection) in
 // a single launch.  exact selects the reference-order arithmetic.
 int fused_substep_layers(const FusedLayer* layers, int nlayers, const float* bed, int res, bool exact,
                          hipStream_t st);
+// One substep of n independent windows (each with its own side, bed and
+// planes; one drive kind and projection) in a single launch.  rows > 0 pins
+// the rows per workgroup.
+int fused_substep_windows(const FusedLayer* wins, int n, bool exact, int rows, hipStream_t st);
+
+// A prepared launch over n windows: the packed window descriptors (device
+// layout, fused_desc_size() bytes each) and, for more than two windows, the
+// window of every tile.  A caller that keeps its own device copies (a batch
+// replayed from a graph) uploads descs and tile_window and enqueues.
+struct FusedLaunch {
+  bool exact = false, full = false, project = false, wind = false;
+  int chunk = 0, nwin = 0, tiles = 0;
+  std::vector<unsigned char> descs;
+  std::vector<int> tile_window;
+};
+size_t fused_desc_size();
+int fused_build(const FusedLayer* wins, int n, bool exact, int rows, FusedLaunch* out);
+int fused_enqueue(const FusedLaunch& l, const void* dev_table, const int* dev_tile_window, hipStream_t st);
 
 // One planned step of one layer on the fused path.  Returns the substep count,
 // SILT_EUNSUPPORTED for plans the fused kernel does not cover (pace-3 slices,

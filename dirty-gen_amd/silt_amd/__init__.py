@@ -111,6 +111,22 @@ def _declare(L):
     L.silt_pair_autotune.argtypes = [_P, _F, C.POINTER(C.c_int), _Z, C.c_int, C.POINTER(C.c_int),
                                      C.POINTER(C.c_float)]
     L.silt_pair_autotune.restype = C.c_int
+    L.silt_batch_create.argtypes = [_Z, C.POINTER(C.c_size_t), C.POINTER(C.c_float), C.c_int]
+    L.silt_batch_create.restype = _P
+    L.silt_batch_destroy.argtypes = [_P]
+    L.silt_batch_destroy.restype = None
+    L.silt_batch_upload.argtypes = [_P, _Z, _P, _P, _P, _P, _F, _F]
+    L.silt_batch_download.argtypes = [_P, _Z, _P, _P, _P]
+    L.silt_batch_step.argtypes = [_P, _F]
+    L.silt_batch_ledger.argtypes = [_P, _Z, C.POINTER(LayerLedger)]
+    L.silt_batch_last_kernel_ms.argtypes = [_P]
+    L.silt_batch_last_kernel_ms.restype = C.c_float
+    for n in ("silt_batch_last_fallbacks", "silt_batch_graphs_captured"):
+        getattr(L, n).argtypes = [_P]
+    L.silt_batch_use_graph.argtypes = [_P, C.c_int]
+    L.silt_batch_set_rows.argtypes = [_P, C.c_int]
+    L.silt_batch_autotune.argtypes = [_P, _F, C.POINTER(C.c_int), _Z, C.c_int, C.POINTER(C.c_int),
+                                      C.POINTER(C.c_float)]
     L.silt_set_chunk_rows.argtypes = [_Z, C.c_int]
     L.silt_set_chunk_rows.restype = C.c_int
     L.silt_chunk_rows.argtypes = [_Z]
@@ -180,6 +196,73 @@ def set_chunk_rows(res, rows):
 def chunk_rows(res):
     """The rows pinned for side res, 0 when the default applies."""
     return int(_check_status(lib().silt_chunk_rows(res)))
+
+
+class Batch:
+    """Independent sheet windows stepped together (include/silt_batch.h):
+    window k is ``res[k]`` square cells of ``cell[k]`` metres, one layer each,
+    stepped as ``step_sheet`` with a Gravity drive."""
+
+    def __init__(self, res, cell, path="fast"):
+        self.res = [int(r) for r in res]
+        n = len(self.res)
+        cells = [float(c) for c in (cell if hasattr(cell, "__len__") else [cell] * n)]
+        self._h = lib().silt_batch_create(n, (C.c_size_t * n)(*self.res), (C.c_float * n)(*cells), PATH[path])
+        if not self._h:
+            raise SiltError("silt_batch_create failed")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().silt_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, k, bed, h, u, v, nu=0.9, vorticity=0.08):
+        arrs = [np.ascontiguousarray(a, np.float32) for a in (bed, h, u, v)]
+        n = self.res[k] ** 2
+        if any(a.size != n for a in arrs):
+            raise ValueError("plane size does not match the window")
+        _check_status(lib().silt_batch_upload(self._h, k, *(_ptr(a) for a in arrs), nu, vorticity))
+
+    def download(self, k):
+        out = [np.empty(self.res[k] ** 2, np.float32) for _ in range(3)]
+        _check_status(lib().silt_batch_download(self._h, k, *(_ptr(a) for a in out)))
+        return out
+
+    def step(self, dt):
+        return _check_status(lib().silt_batch_step(self._h, dt))
+
+    def ledger(self, k):
+        led = LayerLedger()
+        _check_status(lib().silt_batch_ledger(self._h, k, C.byref(led)))
+        return led
+
+    def last_kernel_ms(self):
+        return lib().silt_batch_last_kernel_ms(self._h)
+
+    def last_fallbacks(self):
+        return _check_status(lib().silt_batch_last_fallbacks(self._h))
+
+    def graphs_captured(self):
+        return _check_status(lib().silt_batch_graphs_captured(self._h))
+
+    def use_graph(self, on):
+        _check_status(lib().silt_batch_use_graph(self._h, 1 if on else 0))
+
+    def set_rows(self, rows):
+        _check_status(lib().silt_batch_set_rows(self._h, rows))
+
+    def autotune(self, dt, rows=None, reps=3):
+        best, ms = C.c_int(0), C.c_float(0.0)
+        arr = None if not rows else (C.c_int * len(rows))(*rows)
+        _check_status(lib().silt_batch_autotune(self._h, dt, arr, len(rows) if rows else 0, reps, C.byref(best),
+                                                C.byref(ms)))
+        return best.value, ms.value
 
 
 class SheetPair:

@@ -119,3 +119,26 @@ def large_scene(res=4096, cell=0.2, seed=7, puddles=64):
     slurry = np.full((res, res), 0.02, np.float32)
     return {"res": res, "cell": cell, "origin": origin, "bed": bed.ravel(), "water": water.ravel(),
             "slurry": slurry.ravel(), "nu_water": 0.9, "nu_slurry": 0.5}
+
+
+def batch_scene(n=64, seed=11, lo=128, hi=1024, cell=0.2, film=0.0):
+    """Independent sheet windows: side drawn uniformly from lo..hi cells
+    (seeded), each with its own fBm bed (seed + k) and 1 to 3 Gaussian
+    puddles (0.10 m, sigma 4 m) at seeded positions, on a film of ``film``
+    metres (0: puddles on a dry bed, whose wet set may be several bodies)."""
+    rng = np.random.default_rng(seed)
+    sides = rng.integers(lo, hi + 1, size=n)
+    out = []
+    for k, res in enumerate(sides):
+        res = int(res)
+        ext = res * cell
+        origin = (-ext / 2.0, -ext / 2.0)
+        bed = fbm_terrain(res, cell, origin, seed + 1 + k).ravel()
+        water = np.full(res * res, film, np.float32)
+        for _ in range(int(rng.integers(1, 4))):
+            cx, cz = rng.uniform(origin[0], origin[0] + ext, size=2)
+            water += gaussian(res, cell, origin, np.float32(cx), np.float32(cz), 0.10, 4.0).ravel()
+        water = water.astype(np.float32)
+        water[water < 1.0e-6] = 0.0
+        out.append({"res": res, "cell": cell, "origin": origin, "bed": bed, "water": water})
+    return out
