@@ -62,9 +62,7 @@ struct silt_batch {
   std::vector<Window> w;
   SheetStats* stats = nullptr;    // [n] device
   SheetTotals* totals = nullptr;  // [n] device
-  unsigned* boxmax = nullptr;     // [n] device
   SheetStats* h_stats = nullptr;  // [n] pinned
-  unsigned* h_boxmax = nullptr;   // [n] pinned
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   Slot slot[kMaxLaunches];
@@ -76,6 +74,7 @@ struct silt_batch {
   int rows = 0;
   float last_ms = 0.0f;
   int last_fallbacks = 0;
+  std::vector<PlanScratch> ps;  // [n]
 };
 
 namespace {
@@ -144,6 +143,7 @@ silt_batch* silt_batch_create(size_t n, const size_t* res, const float* cell, in
   silt_batch* b = new silt_batch();
   b->path = path;
   b->w.resize(n);
+  b->ps.resize(n);
   bool ok = true;
   for (size_t k = 0; k < n && ok; k++) {
     Window& w = b->w[k];
@@ -159,9 +159,7 @@ silt_batch* silt_batch_create(size_t n, const size_t* res, const float* cell, in
   }
   ok = ok && hipMalloc(&b->stats, n * sizeof(SheetStats)) == hipSuccess &&
        hipMalloc(&b->totals, n * sizeof(SheetTotals)) == hipSuccess &&
-       hipMalloc(&b->boxmax, n * sizeof(unsigned)) == hipSuccess &&
        hipHostMalloc(&b->h_stats, n * sizeof(SheetStats), hipHostMallocDefault) == hipSuccess &&
-       hipHostMalloc(&b->h_boxmax, n * sizeof(unsigned), hipHostMallocDefault) == hipSuccess &&
        hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) == hipSuccess &&
        hipEventCreate(&b->ev0) == hipSuccess && hipEventCreate(&b->ev1) == hipSuccess &&
        hipMemset(b->totals, 0, n * sizeof(SheetTotals)) == hipSuccess;
@@ -182,6 +180,7 @@ void silt_batch_destroy(silt_batch* b) {
   if (!b) return;
   if (b->stream) (void)hipStreamSynchronize(b->stream);
   drop_graph(b);
+  for (PlanScratch& p : b->ps) plan_scratch_free(p);
   for (Window& w : b->w) {
     (void)hipFree(w.bed);
     for (int q = 0; q < 2; q++)
@@ -195,9 +194,7 @@ void silt_batch_destroy(silt_batch* b) {
   }
   (void)hipFree(b->stats);
   (void)hipFree(b->totals);
-  (void)hipFree(b->boxmax);
   if (b->h_stats) (void)hipHostFree(b->h_stats);
-  if (b->h_boxmax) (void)hipHostFree(b->h_boxmax);
   if (b->ev0) (void)hipEventDestroy(b->ev0);
   if (b->ev1) (void)hipEventDestroy(b->ev1);
   if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -245,14 +242,31 @@ int silt_batch_step(silt_batch* b, float dt) {
   // step's kernels, or by the upload).
   HIPCHK(hipMemcpyAsync(b->h_stats, b->stats, n * sizeof(SheetStats), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
+  // Every window's labelling is enqueued before the first wait, and every
+  // box maximum before the second: two waits for the whole batch.
   std::vector<StepPlan> plans(n);
+  std::vector<char> host(n, 0);
   b->last_fallbacks = 0;
-  int nsub_max = 0;
+  bool wait = false;
   for (size_t k = 0; k < n; k++) {
     Window& w = b->w[k];
-    bool need_max = false;
-    if (!plan_from_stats(b->h_stats[k], w.res, w.cell, dt, &plans[k], &need_max)) {
-      // Several wet bodies: the planes go to the host for the flood fill.
+    int pr = plan_begin(w.planes[w.cur][0], w.res, w.cell, dt, kEps, b->h_stats[k], b->ps[k], &plans[k], st);
+    if (pr < 0) return pr;
+    host[k] = pr == 0;
+    wait = wait || b->ps[k].labelled;
+  }
+  if (wait) HIPCHK(hipStreamSynchronize(st));
+  for (size_t k = 0; k < n; k++) {
+    if (host[k]) continue;
+    int pr = plan_boxes(b->ps[k], b->w[k].res, &plans[k], st);
+    if (pr < 0) return pr;
+    host[k] = pr == 0;
+  }
+  wait = false;
+  for (size_t k = 0; k < n; k++) {
+    Window& w = b->w[k];
+    if (host[k]) {
+      // Too many wet bodies to read back: the planes go to the host.
       b->last_fallbacks++;
       std::vector<float> hh(w.n), uh(w.n), vh(w.n), bh(w.n);
       size_t fb = w.n * sizeof(float);
@@ -263,16 +277,17 @@ int silt_batch_step(silt_batch* b, float dt) {
       HIPCHK(hipStreamSynchronize(st));
       plans[k] = plan_step(hh.data(), uh.data(), vh.data(), bh.data(), w.res, w.cell, dt, 0, 0.0f, 0.0f, kEps,
                            8.0f, 0);
-    } else if (need_max) {
-      unsigned zero = 0;
-      HIPCHK(hipMemcpyAsync(b->boxmax + k, &zero, 4, hipMemcpyHostToDevice, st));
-      box_max_s2(w.planes[w.cur][1], w.planes[w.cur][2], w.res, plans[k].boxes[0], b->boxmax + k, st);
-      HIPCHK(hipMemcpyAsync(b->h_boxmax + k, b->boxmax + k, 4, hipMemcpyDeviceToHost, st));
-      HIPCHK(hipStreamSynchronize(st));
-      float s2;
-      memcpy(&s2, b->h_boxmax + k, 4);
-      plans[k].nsub = nsub_from_s2(s2, dt, w.cell);
+      continue;
     }
+    int rc = plan_nsub_begin(w.planes[w.cur][1], w.planes[w.cur][2], w.res, w.cell, dt, b->h_stats[k], b->ps[k],
+                             &plans[k], st);
+    if (rc < 0) return rc;
+    wait = wait || b->ps[k].maxs2_pending;
+  }
+  if (wait) HIPCHK(hipStreamSynchronize(st));
+  int nsub_max = 0;
+  for (size_t k = 0; k < n; k++) {
+    if (!host[k]) plan_nsub_end(b->ps[k], b->w[k].cell, dt, &plans[k]);
     if (!plans[k].boxes.empty()) nsub_max = std::max(nsub_max, plans[k].nsub);
   }
   if (nsub_max == 0) {

@@ -51,6 +51,7 @@ struct silt_pair {
   float last_kernel_ms = 0.0f;
   int last_nsub[2] = {0, 0};
   int last_fallback = 0;
+  PlanScratch ps;
 };
 
 namespace {
@@ -112,6 +113,7 @@ silt_pair* silt_pair_create(size_t res, float cell, int path) {
 void silt_pair_destroy(silt_pair* p) {
   if (!p) return;
   if (p->stream) (void)hipStreamSynchronize(p->stream);
+  plan_scratch_free(p->ps);
   (void)hipFree(p->bed);
   (void)hipFree(p->nu_slurry);
   for (int l = 0; l < 2; l++)
@@ -174,33 +176,23 @@ int silt_pair_step(silt_pair* p, float dt) {
   HIPCHK(hipMemcpyAsync(p->h_stats, p->stats, 2 * sizeof(SheetStats), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
   StepPlan plans[2];
-  bool fast_plan[2], need_max[2];
   p->last_fallback = 0;
   for (int l = 0; l < 2; l++) {
-    fast_plan[l] = plan_from_stats(p->h_stats[l], p->res, p->cell, dt, &plans[l], &need_max[l]);
-    if (!fast_plan[l]) {
-      // The wet set is not one solid rectangle: read the planes back and
-      // plan on the host (connected wet bodies, as sheet_work_boxes does).
+    float** pl = p->planes[l][p->cur[l]];
+    int pr = plan_sheet_device(pl[0], pl[1], pl[2], res, p->cell, dt, kEps, p->h_stats[l], p->ps, &plans[l], st);
+    if (pr < 0) return pr;
+    if (pr == 0) {
+      // A 2 cm sheet (sleeping tiles) or more wet bodies than are read back:
+      // the planes go to the host.
       p->last_fallback |= 1 << l;
       std::vector<float> hh(p->n), uh(p->n), vh(p->n), bh(p->n);
       size_t fb = p->n * sizeof(float);
-      float** pl = p->planes[l][p->cur[l]];
       HIPCHK(hipMemcpyAsync(hh.data(), pl[0], fb, hipMemcpyDeviceToHost, st));
       HIPCHK(hipMemcpyAsync(uh.data(), pl[1], fb, hipMemcpyDeviceToHost, st));
       HIPCHK(hipMemcpyAsync(vh.data(), pl[2], fb, hipMemcpyDeviceToHost, st));
       HIPCHK(hipMemcpyAsync(bh.data(), p->bed, fb, hipMemcpyDeviceToHost, st));
       HIPCHK(hipStreamSynchronize(st));
       plans[l] = plan_step(hh.data(), uh.data(), vh.data(), bh.data(), res, p->cell, dt, 0, 0.0f, 0.0f, kEps, 8.0f, 0);
-    } else if (need_max[l]) {
-      unsigned zero = 0;
-      HIPCHK(hipMemcpyAsync(p->boxmax + l, &zero, 4, hipMemcpyHostToDevice, st));
-      float** pl = p->planes[l][p->cur[l]];
-      box_max_s2(pl[1], pl[2], res, plans[l].boxes[0], p->boxmax + l, st);
-      HIPCHK(hipMemcpyAsync(p->h_boxmax + l, p->boxmax + l, 4, hipMemcpyDeviceToHost, st));
-      HIPCHK(hipStreamSynchronize(st));
-      float s2;
-      memcpy(&s2, p->h_boxmax + l, 4);
-      plans[l].nsub = nsub_from_s2(s2, dt, p->cell);
     }
   }
   silt_sheet_opts o;

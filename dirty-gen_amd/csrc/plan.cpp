@@ -124,12 +124,20 @@ std::vector<Box> sheet_work_boxes(const float* h, int res, float eps, bool allow
   for (size_t k = 0; k < runs.size(); k++)
     if (parent[k] == (uint32_t)k) bodies.push_back({first[k], bbox[k]});
   std::sort(bodies.begin(), bodies.end(), [](const auto& x, const auto& y) { return x.first < y.first; });
+  std::vector<Box> bb;
+  for (auto& bd : bodies) bb.push_back(bd.second);
+  return boxes_from_bodies(std::move(bb), res, halo3);
+}
+
+// The merged set does not depend on the bodies' order (merging only grows
+// boxes, so bodies whose boxes touch at any point end in one box whatever the
+// order); only the list order does, and boxes that do not touch are stepped
+// independently.
+std::vector<Box> boxes_from_bodies(std::vector<Box> bodies, int res, const Box& halo3) {
   std::vector<Box> boxes;
-  for (auto& bd : bodies) {
-    const Box& b = bd.second;
+  for (const Box& b : bodies)
     boxes.push_back(Box{std::max(b.i0 - 6, 0), std::min(b.i1 + 6, res - 1), std::max(b.j0 - 6, 0),
                         std::min(b.j1 + 6, res - 1)});
-  }
   merge(boxes);
   if (boxes.size() == 1 || boxes.size() > 8) return {halo3};
   return boxes;

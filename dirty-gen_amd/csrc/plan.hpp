@@ -21,6 +21,10 @@ struct StepPlan {
 
 // sheet_work_boxes (src/fluid_core.rs:1325-1447).
 std::vector<Box> sheet_work_boxes(const float* h, int res, float eps, bool allow_split);
+// The split step of sheet_work_boxes given the bounding boxes of the wet
+// bodies: each grown by 6 (clamped to the grid), touching boxes merged; one
+// box, or more than 8, gives halo3 (the wet bounding box grown by 3).
+std::vector<Box> boxes_from_bodies(std::vector<Box> bodies, int res, const Box& halo3);
 // awake_work_boxes (src/fluid_core.rs:1537-1657).
 std::vector<Box> awake_work_boxes(const float* h, const float* u, const float* v, const float* bed, int res,
                                   float cell, float eps);

@@ -111,30 +111,8 @@ void box_max_s2(const float* u, const float* v, int res, const Box& b, unsigned*
 
 namespace silt {
 
-// Plan of one layer of a gravity sheet from its statistics (sheet_work_boxes,
-// src/fluid_core.rs:1325-1486, and the substep count, :1689-1760, for the
-// case the statistics decide).  Returns false when the plan needs the planes
-// on the host: a wet set that is not one solid rectangle, or a 2 cm cell
-// (sleeping tiles).
-bool plan_from_stats(const SheetStats& st, int res, float cell, float dt, StepPlan* plan, bool* need_box_max) {
-  *plan = StepPlan{};
-  *need_box_max = false;
-  if (!(dt > 0.0f)) return true;
-  if (st.wet == 0) return true;  // wet_sum <= eps: the reference returns early
-  if (cell <= kFineCell + 1.0e-6f) return false;
-  long w = (long)(st.i1 - st.i0 + 1), hgt = (long)(st.j1 - st.j0 + 1);
-  if ((long long)st.wet != (long long)w * hgt) return false;
-  plan->boxes.push_back(Box{std::max(st.i0 - 3, 0), std::min(st.i1 + 3, res - 1), std::max(st.j0 - 3, 0),
-                            std::min(st.j1 + 3, res - 1)});
-  // nsub from the largest speed anywhere bounds the box's; when that bound
-  // is one substep it is exact, otherwise the box's own maximum is read.
-  float s2;
-  memcpy(&s2, &st.max_s2_bits, 4);
-  plan->nsub = nsub_from_s2(s2, dt, cell);
-  if (plan->nsub > 1) *need_box_max = true;
-  return true;
-}
-
+// Substep count from the largest squared speed over the boxes
+// (src/fluid_core.rs:1689-1760).
 int nsub_from_s2(float s2, float dt, float cell) {
   float max_s = fmaxf(0.15f, sqrtf(s2));
   float q = ceilf((dt * max_s) / (0.42f * fmaxf(cell, 1.0e-3f)));

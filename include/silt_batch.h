@@ -48,8 +48,8 @@ int silt_batch_ledger(silt_batch *b, size_t k, silt_layer_ledger_t *out);
 
 /* GPU time of the last step (HIP events around its launches). */
 float silt_batch_last_kernel_ms(const silt_batch *b);
-/* Windows whose last plan needed a read back of their planes (wet set not
- * one rectangle); the launch still ran on the GPU. */
+/* Windows whose last plan needed a read back of their planes (more than
+ * 4096 separate wet bodies); the launch still ran on the GPU. */
 int silt_batch_last_fallbacks(const silt_batch *b);
 /* Launch graphs captured so far (a new one each time the launch shape of a
  * step changes). */

@@ -61,8 +61,9 @@ int silt_pair_ledger(silt_pair *p, silt_pair_ledger_t *out);
 /* GPU time of the last step's kernels, milliseconds (HIP events on the
  * handle's stream). */
 float silt_pair_last_kernel_ms(const silt_pair *p);
-/* Bit l set when layer l's last plan needed the host (wet set not one solid
- * rectangle). */
+/* Bit l set when layer l's last plan read its planes back to the host (a
+ * 2 cm sheet, or more than 4096 separate wet bodies; other multi-body wet
+ * sets are labelled on the GPU). */
 int silt_pair_last_fallback(const silt_pair *p);
 
 /* Autotune the fused kernel's rows per workgroup (silt_set_chunk_rows) for

@@ -116,3 +116,40 @@ def test_batch_autotune_restores_the_state():
                 _eq(a, e, f"window {k} {name} after autotune")
     finally:
         b.close()
+
+
+def test_wet_bodies_labelled_on_the_gpu_give_the_host_boxes():
+    """Scattered puddles: the reference's host scan plans 4 split boxes,
+    more than 8 (so the wet bounding box), 7, and bodies whose grown boxes
+    merge into one.  The GPU labels the bodies (no plane read back); every
+    window must step exactly as the host-planned single-window path."""
+    rng = np.random.default_rng(8)
+    ws = []
+    shapes = [(120, 5, 0.8, 0.01), (200, 14, 0.6, 0.02), (200, 14, 0.6, 0.01), (90, 3, 2.0, 0.005)]
+    for k, (res, blobs, sigma, cut) in enumerate(shapes):
+        cell = 0.2
+        origin = (-res * cell / 2.0, -res * cell / 2.0)
+        bed = scenes.fbm_terrain(res, cell, origin, 40 + k).ravel()
+        water = np.zeros(res * res, np.float32)
+        for _ in range(blobs):
+            cx, cz = rng.uniform(origin[0] + 2, -origin[0] - 2, size=2)
+            water += scenes.gaussian(res, cell, origin, np.float32(cx), np.float32(cz), 0.05, sigma).ravel()
+        water[water < cut] = 0.0
+        ws.append({"res": res, "cell": cell, "bed": bed, "water": water.astype(np.float32)})
+    assert [len(O.work_boxes(w["water"], w["res"], 1.0e-6)) for w in ws] == [4, 1, 7, 1]
+    nus, vorts = [0.9] * 4, [0.08] * 4
+    b = _batch(ws, nus, vorts, "fused_exact")
+    state = [[w["water"].copy(), np.zeros(w["res"] ** 2, np.float32), np.zeros(w["res"] ** 2, np.float32)]
+             for w in ws]
+    try:
+        for step in range(3):
+            b.step(DT)
+            assert b.last_fallbacks() == 0
+            for k, w in enumerate(ws):
+                n = w["res"] ** 2
+                S.step_sheet(*state[k], w["bed"], np.full(n, 0.9, np.float32), np.ones(n, np.uint8), w["res"],
+                             w["cell"], DT, ("gravity", 0.08), path="exact")
+                for a, e, name in zip(b.download(k), state[k], "huv"):
+                    _eq(a, e, f"window {k} {name} step {step}")
+    finally:
+        b.close()
