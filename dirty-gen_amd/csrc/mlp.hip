@@ -1,0 +1,324 @@
+// MFMA kernels of the tire sub-grid network (mlp.hpp).
+//
+// One GEMM kernel serves every product of the forward and backward passes:
+// C(m, n) = sum_k A(m, k) B(n, k) with BF16 operands, each stored either
+// k-contiguous or transposed, and FP32 accumulation in
+// v_mfma_f32_16x16x16_bf16.  A 256-lane workgroup owns a 64 x 64 tile of C;
+// each of its four wavefronts owns 32 x 32 (2 x 2 MFMA tiles) and the K
+// dimension streams through LDS 32 at a time.  The epilogue is fused:
+//   EPI_PLAIN     C (+ bias)                          -> FP32
+//   EPI_HIDDEN    z = C + bias -> FP32, gelu(z)       -> BF16
+//   EPI_GELU_BWD  dz = C * gelu'(z)                   -> FP32 and BF16
+//
+// Forward (n samples):  z1 = x W1^T + b1, a1 = gelu(z1); z2 = a1 W2^T + b2,
+// a2 = gelu(z2); y = a2 W3^T + b3.  Backward of L = sum (y - t)^2 / (8 n):
+// dy = 2 (y - t) / (8 n); dW3 = dy^T a2; dz2 = (dy W3) * gelu'(z2);
+// dW2 = dz2^T a1; dz1 = (dz2 W2) * gelu'(z1); dW1 = dz1^T x; db = column sums.
+#include <math.h>
+#include <string.h>
+
+#include <vector>
+
+#include "mlp.hpp"
+#include "silt_sheet.h"
+
+namespace silt {
+
+namespace {
+
+typedef short v4s __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+enum { EPI_PLAIN = 0, EPI_HIDDEN = 1, EPI_GELU_BWD = 2 };
+
+struct Gemm {
+  int M, N, K;
+  const uint16_t* A;
+  int lda;
+  const uint16_t* B;
+  int ldb;
+  const float* bias;
+  float* out;
+  int ldo;
+  uint16_t* outb;
+  int ldob;
+  const float* z;
+  int ldz;
+};
+
+__device__ inline float gelu(float x) { return 0.5f * x * (1.0f + erff(x * 0.70710678118654752f)); }
+__device__ inline float gelu_grad(float x) {
+  float cdf = 0.5f * (1.0f + erff(x * 0.70710678118654752f));
+  float pdf = expf(-0.5f * x * x) * 0.39894228040143268f;
+  return cdf + x * pdf;
+}
+
+constexpr int kT = 64;    // tile side
+constexpr int kKs = 32;   // K per LDS stage
+constexpr int kLd = 40;   // LDS row stride (elements): 80 B keeps 8 B alignment
+
+// AT: A stored as [K][lda] (m contiguous); otherwise [M][lda] (k contiguous).
+// BT: B stored as [K][ldb] (n contiguous); otherwise [N][ldb].
+template <bool AT, bool BT, int EPI>
+__global__ __launch_bounds__(256) void k_gemm(Gemm g) {
+  __shared__ __attribute__((aligned(16))) uint16_t As[kT][kLd];
+  __shared__ __attribute__((aligned(16))) uint16_t Bs[kT][kLd];
+  const int tid = (int)threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int m0 = (int)blockIdx.y * kT, n0 = (int)blockIdx.x * kT;
+  v4f acc[2][2];
+  for (int a = 0; a < 2; a++)
+    for (int b = 0; b < 2; b++) acc[a][b] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+  for (int k0 = 0; k0 < g.K; k0 += kKs) {
+    // Stage the A and B tiles, reading along each operand's contiguous axis.
+    for (int e = tid; e < kT * kKs; e += 256) {
+      int r, c;
+      if (AT) {
+        c = e / kT;
+        r = e % kT;
+      } else {
+        r = e / kKs;
+        c = e % kKs;
+      }
+      const int m = m0 + r, k = k0 + c;
+      uint16_t v = 0;
+      if (m < g.M && k < g.K) v = AT ? g.A[(size_t)k * g.lda + m] : g.A[(size_t)m * g.lda + k];
+      As[r][c] = v;
+    }
+    for (int e = tid; e < kT * kKs; e += 256) {
+      int r, c;
+      if (BT) {
+        c = e / kT;
+        r = e % kT;
+      } else {
+        r = e / kKs;
+        c = e % kKs;
+      }
+      const int n = n0 + r, k = k0 + c;
+      uint16_t v = 0;
+      if (n < g.N && k < g.K) v = BT ? g.B[(size_t)k * g.ldb + n] : g.B[(size_t)n * g.ldb + k];
+      Bs[r][c] = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ks = 0; ks < kKs; ks += 16) {
+      // Lane l holds rows (l mod 16) of its 16-row slices, k = 4 (l / 16) + 0..3.
+      const int kk = ks + 4 * (lane >> 4);
+      v4s a[2], b[2];
+#pragma unroll
+      for (int t = 0; t < 2; t++) {
+        a[t] = *reinterpret_cast<const v4s*>(&As[wm * 32 + t * 16 + (lane & 15)][kk]);
+        b[t] = *reinterpret_cast<const v4s*>(&Bs[wn * 32 + t * 16 + (lane & 15)][kk]);
+      }
+#pragma unroll
+      for (int ti = 0; ti < 2; ti++)
+#pragma unroll
+        for (int tj = 0; tj < 2; tj++)
+          acc[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[ti], b[tj], acc[ti][tj], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  // Result layout: lane l holds C[4 (l / 16) + i][l mod 16] of each 16 x 16 tile.
+#pragma unroll
+  for (int ti = 0; ti < 2; ti++)
+#pragma unroll
+    for (int tj = 0; tj < 2; tj++)
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const int m = m0 + wm * 32 + ti * 16 + 4 * (lane >> 4) + i;
+        const int n = n0 + wn * 32 + tj * 16 + (lane & 15);
+        if (m >= g.M || n >= g.N) continue;
+        float v = acc[ti][tj][i];
+        if (EPI == EPI_PLAIN) {
+          if (g.bias) v += g.bias[n];
+          g.out[(size_t)m * g.ldo + n] = v;
+        } else if (EPI == EPI_HIDDEN) {
+          v += g.bias[n];
+          g.out[(size_t)m * g.ldo + n] = v;
+          g.outb[(size_t)m * g.ldob + n] = f2bf(gelu(v));
+        } else {
+          const float d = v * gelu_grad(g.z[(size_t)m * g.ldz + n]);
+          g.out[(size_t)m * g.ldo + n] = d;
+          g.outb[(size_t)m * g.ldob + n] = f2bf(d);
+        }
+      }
+}
+
+template <bool AT, bool BT, int EPI>
+void gemm(const Gemm& g, hipStream_t st) {
+  dim3 grid((unsigned)((g.N + kT - 1) / kT), (unsigned)((g.M + kT - 1) / kT));
+  hipLaunchKernelGGL((k_gemm<AT, BT, EPI>), grid, dim3(256), 0, st, g);
+}
+
+__global__ void k_to_bf16(const float* __restrict__ x, uint16_t* __restrict__ y, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = f2bf(x[i]);
+}
+
+// dy = 2 (y - t) / (8 n) (FP32 and BF16); sum of squared errors in FP64.
+__global__ __launch_bounds__(256) void k_loss(const float* __restrict__ y, const float* __restrict__ t, size_t count,
+                                              float scale, float* __restrict__ dyf, uint16_t* __restrict__ dy,
+                                              double* loss) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double s = 0.0;
+  if (i < count) {
+    const float d = y[i] - t[i];
+    const float g = 2.0f * d * scale;
+    dyf[i] = g;
+    dy[i] = f2bf(g);
+    s = (double)d * (double)d;
+  }
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+  if ((threadIdx.x & 63) == 0 && s != 0.0) atomicAdd(loss, s);
+}
+
+// Bias gradient: column sums of an M x N FP32 matrix.
+__global__ void k_colsum(const float* __restrict__ x, int M, int N, int ld, float* __restrict__ out) {
+  int n = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (n >= N) return;
+  float s = 0.0f;
+  for (int m = 0; m < M; m++) s += x[(size_t)m * ld + n];
+  out[n] = s;
+}
+
+__global__ void k_adam(float* __restrict__ w, const float* __restrict__ g, float* __restrict__ m1,
+                       float* __restrict__ m2, uint16_t* __restrict__ wb, size_t n, float lr, float c1, float c2) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float b1 = 0.9f, b2 = 0.999f;
+  float gi = g[i];
+  float a = b1 * m1[i] + (1.0f - b1) * gi;
+  float b = b2 * m2[i] + (1.0f - b2) * gi * gi;
+  m1[i] = a;
+  m2[i] = b;
+  float wi = w[i] - lr * (a * c1) / (sqrtf(b * c2) + 1.0e-8f);
+  w[i] = wi;
+  wb[i] = f2bf(wi);
+}
+
+__global__ void k_zero_f64(double* p) { *p = 0.0; }
+
+template <typename T>
+bool dalloc(T** p, size_t n) {
+  return hipMalloc(p, n * sizeof(T)) == hipSuccess;
+}
+
+}  // namespace
+
+int mlp_reserve(Mlp& m, size_t cap) {
+  if (cap <= m.cap) return 0;
+  void* old[] = {m.xb, m.z1, m.a1, m.z2, m.a2, m.y, m.tg, m.dyf, m.dy, m.dz2f, m.dz2, m.dz1f, m.dz1};
+  if (m.st) (void)hipStreamSynchronize(m.st);
+  for (void* p : old)
+    if (p) (void)hipFree(p);
+  m.cap = 0;
+  const size_t H = kMlpHidden, O = kMlpOut;
+  bool ok = dalloc(&m.xb, cap * kMlpIn) && dalloc(&m.z1, cap * H) && dalloc(&m.a1, cap * H) &&
+            dalloc(&m.z2, cap * H) && dalloc(&m.a2, cap * H) && dalloc(&m.y, cap * O) && dalloc(&m.tg, cap * O) &&
+            dalloc(&m.dyf, cap * O) && dalloc(&m.dy, cap * O) && dalloc(&m.dz2f, cap * H) &&
+            dalloc(&m.dz2, cap * H) && dalloc(&m.dz1f, cap * H) && dalloc(&m.dz1, cap * H);
+  if (!ok) return SILT_ENOMEM;
+  m.cap = cap;
+  return 0;
+}
+
+int mlp_init(Mlp& m, size_t cap, unsigned seed, hipStream_t st) {
+  m.st = st;
+  if (!dalloc(&m.master, kMlpParams) || !dalloc(&m.wb, kMlpParams) || !dalloc(&m.grad, kMlpParams) ||
+      !dalloc(&m.m1, kMlpParams) || !dalloc(&m.m2, kMlpParams) || !dalloc(&m.loss, 1))
+    return SILT_ENOMEM;
+  if (hipMemset(m.m1, 0, kMlpParams * 4) != hipSuccess || hipMemset(m.m2, 0, kMlpParams * 4) != hipSuccess ||
+      hipMemset(m.grad, 0, kMlpParams * 4) != hipSuccess)
+    return SILT_EDEVICE;
+  // Uniform He initialisation from a splitmix64 stream; the output layer is
+  // scaled down so an untrained network predicts small fluxes.
+  std::vector<float> w(kMlpParams, 0.0f);
+  uint64_t s = 0x9e3779b97f4a7c15ull ^ (uint64_t)seed;
+  auto next = [&]() {
+    uint64_t z = (s += 0x9e3779b97f4a7c15ull);
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    z ^= z >> 31;
+    return (float)((double)(z >> 11) * (1.0 / 9007199254740992.0)) * 2.0f - 1.0f;
+  };
+  auto fill = [&](size_t off, size_t count, int fan_in, float gain) {
+    float a = gain * sqrtf(6.0f / (float)fan_in);
+    for (size_t i = 0; i < count; i++) w[off + i] = a * next();
+  };
+  fill(kOffW1, (size_t)kMlpHidden * kMlpIn, kMlpIn, 1.0f);
+  fill(kOffW2, (size_t)kMlpHidden * kMlpHidden, kMlpHidden, 1.0f);
+  fill(kOffW3, (size_t)kMlpOut * kMlpHidden, kMlpHidden, 0.1f);
+  if (hipMemcpy(m.master, w.data(), kMlpParams * 4, hipMemcpyHostToDevice) != hipSuccess) return SILT_EDEVICE;
+  mlp_sync_bf16(m);
+  m.t = 0;
+  return mlp_reserve(m, cap);
+}
+
+void mlp_free(Mlp& m) {
+  if (m.st) (void)hipStreamSynchronize(m.st);
+  void* ps[] = {m.master, m.wb, m.grad, m.m1, m.m2, m.loss, m.xb, m.z1, m.a1, m.z2, m.a2,
+                m.y, m.tg, m.dyf, m.dy, m.dz2f, m.dz2, m.dz1f, m.dz1};
+  for (void* p : ps)
+    if (p) (void)hipFree(p);
+  hipStream_t st = m.st;
+  m = Mlp{};
+  m.st = st;
+}
+
+void mlp_sync_bf16(Mlp& m) {
+  hipLaunchKernelGGL(k_to_bf16, dim3((unsigned)((kMlpParams + 255) / 256)), dim3(256), 0, m.st, m.master, m.wb,
+                     kMlpParams);
+}
+
+void mlp_load_x(Mlp& m, const float* x, size_t n) {
+  size_t c = n * kMlpIn;
+  hipLaunchKernelGGL(k_to_bf16, dim3((unsigned)((c + 255) / 256)), dim3(256), 0, m.st, x, m.xb, c);
+}
+
+void mlp_forward(Mlp& m, size_t n) {
+  const int N = (int)n, H = kMlpHidden;
+  Gemm g{};
+  g = Gemm{N, H, kMlpIn, m.xb, kMlpIn, m.wb + kOffW1, kMlpIn, m.master + kOffB1, m.z1, H, m.a1, H, nullptr, 0};
+  gemm<false, false, EPI_HIDDEN>(g, m.st);
+  g = Gemm{N, H, H, m.a1, H, m.wb + kOffW2, H, m.master + kOffB2, m.z2, H, m.a2, H, nullptr, 0};
+  gemm<false, false, EPI_HIDDEN>(g, m.st);
+  g = Gemm{N, kMlpOut, H, m.a2, H, m.wb + kOffW3, H, m.master + kOffB3, m.y, kMlpOut, nullptr, 0, nullptr, 0};
+  gemm<false, false, EPI_PLAIN>(g, m.st);
+}
+
+void mlp_backward(Mlp& m, size_t n) {
+  const int N = (int)n, H = kMlpHidden, O = kMlpOut;
+  const size_t cnt = n * O;
+  hipLaunchKernelGGL(k_zero_f64, dim3(1), dim3(1), 0, m.st, m.loss);
+  hipLaunchKernelGGL(k_loss, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, m.st, m.y, m.tg, cnt,
+                     1.0f / (float)cnt, m.dyf, m.dy, m.loss);
+  Gemm g{};
+  // dW3 = dy^T a2  (8 x 128, K = n)
+  g = Gemm{O, H, N, m.dy, O, m.a2, H, nullptr, m.grad + kOffW3, H, nullptr, 0, nullptr, 0};
+  gemm<true, true, EPI_PLAIN>(g, m.st);
+  hipLaunchKernelGGL(k_colsum, dim3(1), dim3(64), 0, m.st, m.dyf, N, O, O, m.grad + kOffB3);
+  // dz2 = (dy W3) * gelu'(z2)  (n x 128, K = 8)
+  g = Gemm{N, H, O, m.dy, O, m.wb + kOffW3, H, nullptr, m.dz2f, H, m.dz2, H, m.z2, H};
+  gemm<false, true, EPI_GELU_BWD>(g, m.st);
+  // dW2 = dz2^T a1
+  g = Gemm{H, H, N, m.dz2, H, m.a1, H, nullptr, m.grad + kOffW2, H, nullptr, 0, nullptr, 0};
+  gemm<true, true, EPI_PLAIN>(g, m.st);
+  hipLaunchKernelGGL(k_colsum, dim3(2), dim3(64), 0, m.st, m.dz2f, N, H, H, m.grad + kOffB2);
+  // dz1 = (dz2 W2) * gelu'(z1)
+  g = Gemm{N, H, H, m.dz2, H, m.wb + kOffW2, H, nullptr, m.dz1f, H, m.dz1, H, m.z1, H};
+  gemm<false, true, EPI_GELU_BWD>(g, m.st);
+  // dW1 = dz1^T x  (128 x 1536)
+  g = Gemm{H, kMlpIn, N, m.dz1, H, m.xb, kMlpIn, nullptr, m.grad + kOffW1, kMlpIn, nullptr, 0, nullptr, 0};
+  gemm<true, true, EPI_PLAIN>(g, m.st);
+  hipLaunchKernelGGL(k_colsum, dim3(2), dim3(64), 0, m.st, m.dz1f, N, H, H, m.grad + kOffB1);
+}
+
+void mlp_adam(Mlp& m, float lr) {
+  m.t++;
+  const float c1 = 1.0f / (1.0f - powf(0.9f, (float)m.t));
+  const float c2 = 1.0f / (1.0f - powf(0.999f, (float)m.t));
+  hipLaunchKernelGGL(k_adam, dim3((unsigned)((kMlpParams + 255) / 256)), dim3(256), 0, m.st, m.master, m.grad, m.m1,
+                     m.m2, m.wb, kMlpParams, lr, c1, c2);
+}
+
+}  // namespace silt

@@ -131,6 +131,15 @@ def _declare(L):
     L.silt_set_chunk_rows.restype = C.c_int
     L.silt_chunk_rows.argtypes = [_Z]
     L.silt_chunk_rows.restype = C.c_int
+    L.silt_mlp_create.argtypes = [C.c_uint]
+    L.silt_mlp_create.restype = _P
+    L.silt_mlp_destroy.argtypes = [_P]
+    L.silt_mlp_destroy.restype = None
+    L.silt_mlp_get_params.argtypes = [_P, _P]
+    L.silt_mlp_set_params.argtypes = [_P, _P]
+    L.silt_mlp_forward.argtypes = [_P, _P, _Z, _P]
+    L.silt_mlp_gradients.argtypes = [_P, _P, _P, _Z, C.POINTER(C.c_double), _P]
+    L.silt_mlp_train_batch.argtypes = [_P, _P, _P, _Z, _F, C.POINTER(C.c_double)]
 
 
 def _ptr(a):
@@ -545,3 +554,65 @@ def rut_stamp_device(bed, placed, initial, res, cell, origin, contacts, dt, stre
     arr = (TireContact * len(contacts))(*[TireContact(**c) for c in contacts])
     return _check_status(_rut_lib().silt_rut_stamp_device(bed, placed, initial, res, cell, origin[0], origin[1], arr,
                                                           len(contacts), dt, stream or None))
+
+
+MLP_IN, MLP_HIDDEN, MLP_OUT, MLP_PARAMS = 1536, 128, 8, 214280
+
+
+class Mlp:
+    """The tire sub-grid network on its own (include/silt_mlp.h)."""
+
+    def __init__(self, seed=1):
+        self._h = lib().silt_mlp_create(int(seed))
+        if not self._h:
+            raise SiltError("silt_mlp_create failed")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().silt_mlp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def params(self):
+        out = np.empty(MLP_PARAMS, np.float32)
+        _check_status(lib().silt_mlp_get_params(self._h, _ptr(out)))
+        return out
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        if p.size != MLP_PARAMS:
+            raise ValueError("parameter vector size")
+        _check_status(lib().silt_mlp_set_params(self._h, _ptr(p)))
+
+    @staticmethod
+    def _x(x):
+        x = np.ascontiguousarray(x, np.float32)
+        if x.ndim != 2 or x.shape[1] != MLP_IN:
+            raise ValueError("x must be n x 1536")
+        return x
+
+    def forward(self, x):
+        x = self._x(x)
+        y = np.empty((x.shape[0], MLP_OUT), np.float32)
+        _check_status(lib().silt_mlp_forward(self._h, _ptr(x), x.shape[0], _ptr(y)))
+        return y
+
+    def gradients(self, x, t):
+        x = self._x(x)
+        t = np.ascontiguousarray(t, np.float32).reshape(x.shape[0], MLP_OUT)
+        g = np.empty(MLP_PARAMS, np.float32)
+        sse = C.c_double()
+        _check_status(lib().silt_mlp_gradients(self._h, _ptr(x), _ptr(t), x.shape[0], C.byref(sse), _ptr(g)))
+        return sse.value, g
+
+    def train_batch(self, x, t, lr=1e-3):
+        x = self._x(x)
+        t = np.ascontiguousarray(t, np.float32).reshape(x.shape[0], MLP_OUT)
+        sse = C.c_double()
+        _check_status(lib().silt_mlp_train_batch(self._h, _ptr(x), _ptr(t), x.shape[0], lr, C.byref(sse)))
+        return sse.value

@@ -72,6 +72,12 @@ def _declare(L):
         f = getattr(L, "oracle_column_sum" + sfx)
         f.argtypes = [_P, _Z, R]
         f.restype = R
+        f = getattr(L, "oracle_mlp_forward" + sfx)
+        f.argtypes = [_P, _P, _Z, C.c_int, _P, _P, _P, _P, _P]
+        f.restype = None
+        f = getattr(L, "oracle_mlp_gradients" + sfx)
+        f.argtypes = [_P, _P, _P, _Z, C.c_int, _P]
+        f.restype = C.c_double
 
 
 def _dt(precision):
@@ -438,3 +444,32 @@ def stamp_tread(bed, placed, initial, res, cell, origin, contact, dt, precision=
     cv = np.array([contact[k] for k in CONTACT_FIELDS], dtype)
     return getattr(_rut_lib(), "oracle_stamp_tread" + _sfx(precision))(_ptr(bed), _ptr(placed), _ptr(initial), res,
                                                                          cell, origin[0], origin[1], _ptr(cv), dt)
+
+
+# Tire sub-grid network (oracle/mlp.c).
+MLP_IN, MLP_HIDDEN, MLP_OUT = 1536, 128, 8
+MLP_PARAMS = MLP_HIDDEN * MLP_IN + MLP_HIDDEN + MLP_HIDDEN * MLP_HIDDEN + MLP_HIDDEN + MLP_OUT * MLP_HIDDEN + MLP_OUT
+
+
+def mlp_forward(params, x, bf16=True, precision=32):
+    """y[n][8] of the network with parameter vector params on x[n][1536]."""
+    dt = _dt(precision)
+    p = np.ascontiguousarray(params, dtype=dt)
+    xx = np.ascontiguousarray(x, dtype=dt)
+    n = xx.shape[0]
+    y = np.zeros((n, MLP_OUT), dtype=dt)
+    getattr(lib(), "oracle_mlp_forward" + _sfx(precision))(_ptr(p), _ptr(xx), n, int(bf16), _ptr(y), None, None,
+                                                            None, None)
+    return y
+
+
+def mlp_gradients(params, x, t, bf16=True, precision=32):
+    """(sum of squared errors, gradient vector) of the mean squared error."""
+    dt = _dt(precision)
+    p = np.ascontiguousarray(params, dtype=dt)
+    xx = np.ascontiguousarray(x, dtype=dt)
+    tt = np.ascontiguousarray(t, dtype=dt)
+    g = np.zeros(MLP_PARAMS, dtype=dt)
+    sse = getattr(lib(), "oracle_mlp_gradients" + _sfx(precision))(_ptr(p), _ptr(xx), _ptr(tt), xx.shape[0],
+                                                                   int(bf16), _ptr(g))
+    return sse, g
