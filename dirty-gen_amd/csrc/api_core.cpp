@@ -8,14 +8,18 @@
 #include <string.h>
 
 #include <algorithm>
+#include <unordered_map>
 #include <vector>
 
 #include "core.hpp"
+#include "mlp.hpp"
 #include "plan.hpp"
 #include "rut.hpp"
 #include "sheet_device.hpp"
 #include "sheet_fused.hpp"
 #include "silt_core.h"
+#include "silt_mlp.h"
+#include "surrogate.hpp"
 #include "workspace.hpp"
 
 #define HIPCHK(x)                                 \
@@ -36,6 +40,13 @@ constexpr unsigned kSprayCap = 1u << 16;
 
 struct silt_core {
   float origin[2] = {0, 0};
+  float center[2] = {0, 0};  // chase point (set_center)
+  float moisture_ambient = 0.0f;  // read_substrate's most common moisture
+  // Coarse cells parked off the window by recenter, by world cell key.
+  struct Parked {
+    float water, slurry, uw, vw, us, vs, moisture;
+  };
+  std::unordered_map<uint64_t, Parked> parked;
   float fine_origin[2] = {0, 0};
   int path = SILT_PATH_EXACT;
   int fine_pace = 0;
@@ -63,6 +74,30 @@ struct silt_core {
   bool ground = false;
   float *soil_bed = nullptr, *soil_placed = nullptr, *soil_init = nullptr, *soil_rut = nullptr;
   RutContact* d_contacts = nullptr;
+  // Tire model (SILT_TIRE_*) and the surrogate's state: the network, its
+  // context and limiter scratch, the judge's snapshot of the six planes and
+  // the surrogate's depths, and the events that time the tire block.
+  int tire_model = SILT_TIRE_NESTED;
+  bool net_ready = false;
+  Mlp net;
+  float* sur_ctx = nullptr;
+  float* sur_fac = nullptr;
+  float* snap[6] = {};
+  float* judged_h[2] = {};
+  unsigned* d_err = nullptr;
+  double* d_sq = nullptr;
+  int judge_every = 30;
+  float judge_tol = 1.0e-3f;
+  int fallback_left = 0;
+  long sur_steps = 0;
+  float last_judge_err = -1.0f;
+  float last_judge_rms = -1.0f;
+  int judged = 0, fallbacks = 0, last_mode = SILT_TIRE_NESTED;
+  float last_tire_ms = 0.0f;
+  // Tire block timing: events 0-1 bracket the patch (nested) or the
+  // prediction (surrogate), 2-3 the surrogate's flux apply.
+  hipEvent_t tev[4] = {nullptr, nullptr, nullptr, nullptr};
+  int timed = 0;  // 0 none, 1 nested, 2 surrogate
   hipStream_t st = nullptr;
   std::vector<void*> allocs;
 };
@@ -347,6 +382,176 @@ void shove(silt_core* c, const std::vector<CorePush>& v, bool fine) {
     core_push_on(g, c->c.water, c->c.slurry, c->c.uw, c->c.vw, c->c.us, c->c.vs, pl, 0, 0, g.res, g.res, c->st);
 }
 
+// The reference's step from the refine to the coarse spray (FluidCore::step,
+// src/fluid_core.rs:6034-6128): the 2 cm patch under the tires, then the
+// coarse pair.  timed: events 0-1 bracket the patch's own work.
+int step_nested(silt_core* c, const std::vector<CorePush>& pushes, const PushList& pl, float dt, bool timed) {
+  if (timed) HIPCHK(hipEventRecord(c->tev[0], c->st));
+  if (timed) c->timed = 1;
+  int rc = refine(c, pushes, dt);
+  if (rc) return rc;
+  std::vector<CorePush> fine_p, coarse_p;
+  for (const CorePush& p : pushes)
+    if (cell_is_refined(c, p.x, p.z)) fine_p.push_back(p);
+  shove(c, fine_p, true);
+  if ((rc = step_pair(c, dt, true))) return rc;
+  if (c->fine_on) peel(c, pl, true, false);
+  if ((rc = restrict_fine(c))) return rc;
+  if (timed) HIPCHK(hipEventRecord(c->tev[1], c->st));
+  for (const CorePush& p : pushes)
+    if (!fine_held_at(c, p.x, p.z)) coarse_p.push_back(p);
+  shove(c, coarse_p, false);
+  if ((rc = step_pair(c, dt, false))) return rc;
+  peel(c, pl, false, true);
+  return 0;
+}
+
+SurGrid core_grid(const silt_core* c) {
+  return SurGrid{c->c.water, c->c.slurry, c->c.uw, c->c.vw, c->c.bed, kWindowRes, kCoarseCell, c->origin[0],
+                 c->origin[1]};
+}
+
+// Region of the tire sub-grid network for these pushes.
+SurRegion sur_region(const silt_core* c, const std::vector<CorePush>& p) {
+  return sur_region_at(p.data(), (int)p.size(), c->origin[0], c->origin[1], kCoarseCell, kWindowRes);
+}
+
+// Coarse-only step: every tire shoves the coarse sheet, no patch.  With a
+// region, the network's fluxes (predicted from the state before the step)
+// are applied after it.
+int step_coarse(silt_core* c, const std::vector<CorePush>& pushes, const PushList& pl, float dt, bool fade,
+                bool surrogate, bool timed) {
+  int rc = restrict_fine(c);
+  if (rc) return rc;
+  if (fade && (rc = near_fade(c, dt))) return rc;
+  SurRegion r{0, 0, 0, 0};
+  if (surrogate) {
+    r = sur_region(c, pushes);
+    if (r.nx) {
+      if (timed) HIPCHK(hipEventRecord(c->tev[0], c->st));
+      if (timed) c->timed = 2;
+      sur_context(core_grid(c), r, pl, c->sur_ctx, c->st);
+      sur_gather_region(c->sur_ctx, r, c->net.xb, c->st);
+      mlp_forward(c->net, (size_t)r.nx * r.nz);
+      if (timed) HIPCHK(hipEventRecord(c->tev[1], c->st));
+    }
+  }
+  shove(c, pushes, false);
+  if ((rc = step_pair(c, dt, false))) return rc;
+  peel(c, pl, false, false);
+  if (r.nx) {
+    if (timed) HIPCHK(hipEventRecord(c->tev[2], c->st));
+    sur_apply(c->c.water, c->c.slurry, kWindowRes, c->net.y, r, c->sur_fac, c->st);
+    if (timed) HIPCHK(hipEventRecord(c->tev[3], c->st));
+  }
+  return 0;
+}
+
+// Milliseconds of the tire block of the last timed step (after a sync).
+float tire_ms(silt_core* c) {
+  float a = 0.0f, b = 0.0f;
+  if (c->timed >= 1 && hipEventElapsedTime(&a, c->tev[0], c->tev[1]) != hipSuccess) a = 0.0f;
+  if (c->timed == 2 && hipEventElapsedTime(&b, c->tev[2], c->tev[3]) != hipSuccess) b = 0.0f;
+  return a + b;
+}
+
+float* const* core_planes6(silt_core* c, float** out) {
+  out[0] = c->c.water;
+  out[1] = c->c.slurry;
+  out[2] = c->c.uw;
+  out[3] = c->c.vw;
+  out[4] = c->c.us;
+  out[5] = c->c.vs;
+  return out;
+}
+
+int sur_alloc(silt_core* c) {
+  if (c->sur_ctx) return 0;
+  bool ok = dalloc(c, &c->sur_ctx, kSurCtxFloats) && dalloc(c, &c->sur_fac, 2 * kSurMax * kSurMax) &&
+            dalloc(c, &c->d_err, 1) && dalloc(c, &c->d_sq, 1);
+  for (int k = 0; k < 6 && ok; k++) ok = dalloc(c, &c->snap[k], WN);
+  for (int k = 0; k < 2 && ok; k++) ok = dalloc(c, &c->judged_h[k], WN);
+  for (int k = 0; k < 4 && ok; k++) ok = hipEventCreate(&c->tev[k]) == hipSuccess;
+  return ok ? 0 : SILT_ENOMEM;
+}
+
+// A surrogate step, judged against the nested one every judge_every steps:
+// the surrogate runs from a snapshot, its depths are kept aside, the nested
+// step runs from the same snapshot and stays; the largest depth difference
+// is the surrogate's error.  Over the tolerance, the next judge_every steps
+// are nested only.
+int step_surrogate(silt_core* c, const std::vector<CorePush>& pushes, const PushList& pl, float dt) {
+  if (c->fallback_left > 0) {
+    // After a fallback the next surrogate step is judged again.
+    if (--c->fallback_left == 0) c->sur_steps = 0;
+    c->last_mode = SILT_TIRE_NESTED;
+    return step_nested(c, pushes, pl, dt, true);
+  }
+  const bool judge = c->judge_every > 0 && c->sur_steps % c->judge_every == 0 && !pushes.empty();
+  c->sur_steps++;
+  if (!judge) {
+    c->last_mode = SILT_TIRE_SURROGATE;
+    return step_coarse(c, pushes, pl, dt, true, true, true);
+  }
+  float* p6[6];
+  core_planes6(c, p6);
+  int rc = restrict_fine(c);
+  if (rc) return rc;
+  for (int k = 0; k < 6; k++)
+    HIPCHK(hipMemcpyAsync(c->snap[k], p6[k], WN * sizeof(float), hipMemcpyDeviceToDevice, c->st));
+  if ((rc = step_coarse(c, pushes, pl, dt, false, true, true))) return rc;
+  HIPCHK(hipEventSynchronize(c->tev[3]));
+  const float surrogate_ms = tire_ms(c);
+  c->timed = 0;
+  HIPCHK(hipMemcpyAsync(c->judged_h[0], c->c.water, WN * sizeof(float), hipMemcpyDeviceToDevice, c->st));
+  HIPCHK(hipMemcpyAsync(c->judged_h[1], c->c.slurry, WN * sizeof(float), hipMemcpyDeviceToDevice, c->st));
+  for (int k = 0; k < 6; k++)
+    HIPCHK(hipMemcpyAsync(p6[k], c->snap[k], WN * sizeof(float), hipMemcpyDeviceToDevice, c->st));
+  // The surrogate's spray is undone with its planes (the coarse spray of a
+  // step starts from zero; frozen volumes only move in a restrict).
+  HIPCHK(hipMemsetAsync(c->d_nev, 0, sizeof(unsigned), c->st));
+  HIPCHK(hipMemsetAsync(c->d_acc, 0, sizeof(double), c->st));
+  if ((rc = step_nested(c, pushes, pl, dt, false))) return rc;
+  HIPCHK(hipMemsetAsync(c->d_err, 0, sizeof(unsigned), c->st));
+  HIPCHK(hipMemsetAsync(c->d_sq, 0, sizeof(double), c->st));
+  sur_max_abs_diff(c->judged_h[0], c->c.water, WN, c->d_err, c->st);
+  sur_max_abs_diff(c->judged_h[1], c->c.slurry, WN, c->d_err, c->st);
+  const SurRegion r = sur_region(c, pushes);
+  if (r.nx) {
+    sur_region_sq_diff(c->judged_h[0], c->c.water, kWindowRes, r, c->d_sq, c->st);
+    sur_region_sq_diff(c->judged_h[1], c->c.slurry, kWindowRes, r, c->d_sq, c->st);
+  }
+  unsigned bits = 0;
+  double sq = 0.0;
+  HIPCHK(hipMemcpyAsync(&bits, c->d_err, sizeof bits, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipMemcpyAsync(&sq, c->d_sq, sizeof sq, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  memcpy(&c->last_judge_err, &bits, 4);
+  c->last_judge_rms = r.nx ? (float)sqrt(sq / (2.0 * r.nx * r.nz)) : 0.0f;
+  c->judged++;
+  if (!(c->last_judge_err <= c->judge_tol)) {
+    c->fallbacks++;
+    c->fallback_left = c->judge_every;
+  }
+  c->last_mode = SILT_TIRE_NESTED;
+  c->last_tire_ms = surrogate_ms;
+  return 0;
+}
+
+int step_by_model(silt_core* c, const std::vector<CorePush>& pushes, const PushList& pl, float dt) {
+  c->timed = 0;
+  switch (c->tire_model) {
+    case SILT_TIRE_COARSE:
+      c->last_mode = SILT_TIRE_COARSE;
+      return step_coarse(c, pushes, pl, dt, true, false, false);
+    case SILT_TIRE_SURROGATE:
+      return step_surrogate(c, pushes, pl, dt);
+    default:
+      c->last_mode = SILT_TIRE_NESTED;
+      return step_nested(c, pushes, pl, dt, c->tev[0] != nullptr);
+  }
+}
+
 }  // namespace
 
 extern "C" {
@@ -355,6 +560,8 @@ silt_core* silt_core_create(float center_x, float center_z, int path) {
   if (path != SILT_PATH_EXACT && path != SILT_PATH_FAST && path != SILT_PATH_FUSED_EXACT) return nullptr;
   silt_core* c = new silt_core();
   c->path = path;
+  c->center[0] = center_x;
+  c->center[1] = center_z;
   float extent = (float)kWindowRes * kCoarseCell;
   c->origin[0] = floorf((center_x - extent * 0.5f) / kCoarseCell) * kCoarseCell;
   c->origin[1] = floorf((center_z - extent * 0.5f) / kCoarseCell) * kCoarseCell;
@@ -377,6 +584,7 @@ silt_core* silt_core_create(float center_x, float center_z, int path) {
     core_near_clear(c->near[1], c->st);
     ok = ok && hipStreamSynchronize(c->st) == hipSuccess;
   }
+  ok = ok && sur_alloc(c) == 0;
   if (!ok) {
     silt_core_destroy(c);
     return nullptr;
@@ -387,6 +595,9 @@ silt_core* silt_core_create(float center_x, float center_z, int path) {
 void silt_core_destroy(silt_core* c) {
   if (!c) return;
   if (c->st) (void)hipStreamSynchronize(c->st);
+  if (c->net_ready) mlp_free(c->net);
+  for (hipEvent_t e : c->tev)
+    if (e) (void)hipEventDestroy(e);
   for (void* p : c->allocs) (void)hipFree(p);
   if (c->st) (void)hipStreamDestroy(c->st);
   delete c;
@@ -539,10 +750,25 @@ int silt_core_read_substrate(silt_core* c, const float* bed, const float* moistu
   HIPCHK(hipMemcpyAsync(s.data(), c->c.slurry, WN * 4, hipMemcpyDeviceToHost, c->st));
   HIPCHK(hipMemcpyAsync(g.data(), c->c.grain, WN, hipMemcpyDeviceToHost, c->st));
   HIPCHK(hipStreamSynchronize(c->st));
+  // The most common moisture, by 1 % buckets (rounded half away from zero),
+  // is the ambient moisture of cells that slide onto the window.
+  unsigned hist[101] = {};
   for (int k = 0; k < WN; k++) {
     m[k] = clampf(moisture[k], 0.0f, 1.0f);
+    const float pct = m[k] * 100.0f;
+    const int whole = (int)pct;
+    const int bucket = pct - (float)whole >= 0.5f ? whole + 1 : whole;
+    hist[bucket < 100 ? bucket : 100]++;
     if (s[k] > 1.0e-5f) g[k] = grain[k];
   }
+  int best = 0;
+  unsigned best_n = 0;
+  for (int k = 0; k <= 100; k++)
+    if (hist[k] > best_n) {
+      best = k;
+      best_n = hist[k];
+    }
+  c->moisture_ambient = (float)best * 0.01f;
   HIPCHK(hipMemcpyAsync(c->c.bed, bed, WN * 4, hipMemcpyHostToDevice, c->st));
   HIPCHK(hipMemcpyAsync(c->c.moisture, m.data(), WN * 4, hipMemcpyHostToDevice, c->st));
   HIPCHK(hipMemcpyAsync(c->c.grain, g.data(), WN, hipMemcpyHostToDevice, c->st));
@@ -565,25 +791,14 @@ int silt_core_step(silt_core* c, float dt, silt_spray_event* events, size_t cap,
   hipStream_t st = c->st;
   HIPCHK(hipMemsetAsync(c->d_nev, 0, sizeof(unsigned), st));
   HIPCHK(hipMemsetAsync(c->d_acc, 0, sizeof(double), st));
-  int rc = refine(c, pushes, dt);
+  int rc = step_by_model(c, pushes, pl, dt);
   if (rc) return rc;
-  std::vector<CorePush> fine_p, coarse_p;
-  for (const CorePush& p : pushes)
-    if (cell_is_refined(c, p.x, p.z)) fine_p.push_back(p);
-  shove(c, fine_p, true);
-  if ((rc = step_pair(c, dt, true))) return rc;
-  if (c->fine_on) peel(c, pl, true, false);
-  if ((rc = restrict_fine(c))) return rc;
-  for (const CorePush& p : pushes)
-    if (!fine_held_at(c, p.x, p.z)) coarse_p.push_back(p);
-  shove(c, coarse_p, false);
-  if ((rc = step_pair(c, dt, false))) return rc;
-  peel(c, pl, false, true);
   unsigned nev = 0;
   double vol = 0.0;
   HIPCHK(hipMemcpyAsync(&nev, c->d_nev, sizeof nev, hipMemcpyDeviceToHost, st));
   HIPCHK(hipMemcpyAsync(&vol, c->d_acc, sizeof vol, hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
+  if (c->timed) c->last_tire_ms = tire_ms(c);
   c->spray_m3 += vol;
   size_t take = std::min((size_t)std::min(nev, kSprayCap), cap);
   if (events && take) {
@@ -624,8 +839,13 @@ int silt_core_ledger(silt_core* c, silt_fluid_ledger* out) {
   double ca = (double)kCoarseCell * kCoarseCell, fa = (double)kFineCell * kFineCell;
   out->water_m3 = h[0].vol * ca + (c->fine_on ? h[2].vol * fa : 0.0);
   out->slurry_m3 = h[1].vol * ca + (c->fine_on ? h[3].vol * fa : 0.0);
-  out->frozen_water_m3 = acc[1];
-  out->frozen_slurry_m3 = acc[2];
+  double pw = 0.0, ps = 0.0;
+  for (const auto& kv : c->parked) {
+    pw += (double)kv.second.water;
+    ps += (double)kv.second.slurry;
+  }
+  out->frozen_water_m3 = acc[1] + pw * ca;
+  out->frozen_slurry_m3 = acc[2] + ps * ca;
   out->spray_m3 = c->spray_m3;
   out->total_m3 = out->water_m3 + out->slurry_m3 + out->frozen_water_m3 + out->frozen_slurry_m3 + out->spray_m3;
   return SILT_OK;
@@ -734,6 +954,653 @@ long silt_core_near_tiles(silt_core* c) {
     return SILT_EDEVICE;
   if (hipStreamSynchronize(c->st) != hipSuccess) return SILT_EDEVICE;
   return (long)n;
+}
+
+int silt_core_set_tire_model(silt_core* c, int model) {
+  if (!c || model < SILT_TIRE_NESTED || model > SILT_TIRE_SURROGATE) return SILT_EINVAL;
+  if (model == SILT_TIRE_SURROGATE && !c->net_ready) return SILT_EINVAL;
+  c->tire_model = model;
+  c->fallback_left = 0;
+  c->sur_steps = 0;
+  return SILT_OK;
+}
+
+int silt_core_set_surrogate(silt_core* c, const float* params, int judge_every, float tol) {
+  if (!c || !params || judge_every < 0 || !(tol >= 0.0f)) return SILT_EINVAL;
+  if (!c->net_ready) {
+    if (mlp_init(c->net, (size_t)kSurMax * kSurMax, 0u, c->st) != 0) return SILT_ENOMEM;
+    c->net_ready = true;
+  }
+  HIPCHK(hipMemcpyAsync(c->net.master, params, kMlpParams * sizeof(float), hipMemcpyHostToDevice, c->st));
+  mlp_sync_bf16(c->net);
+  HIPCHK(hipStreamSynchronize(c->st));
+  c->judge_every = judge_every;
+  c->judge_tol = tol;
+  c->fallback_left = 0;
+  c->sur_steps = 0;
+  return SILT_OK;
+}
+
+int silt_core_surrogate_stats(const silt_core* c, silt_surrogate_stats* out) {
+  if (!c || !out) return SILT_EINVAL;
+  out->last_tire_ms = c->last_tire_ms;
+  out->last_judge_err_m = c->last_judge_err;
+  out->last_judge_rms_m = c->last_judge_rms;
+  out->judged = c->judged;
+  out->fallbacks = c->fallbacks;
+  out->last_mode = c->last_mode;
+  return SILT_OK;
+}
+
+int silt_core_map_water_velocity(silt_core* c, const float* u, const float* v) {
+  if (!c || !u || !v) return SILT_EINVAL;
+  if (restrict_fine(c)) return SILT_EDEVICE;
+  std::vector<float> w(WN), uw(WN), vw(WN);
+  HIPCHK(hipMemcpyAsync(w.data(), c->c.water, WN * 4, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipMemcpyAsync(uw.data(), c->c.uw, WN * 4, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipMemcpyAsync(vw.data(), c->c.vw, WN * 4, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  for (int k = 0; k < WN; k++) {
+    if (w[k] <= 1.0e-6f) continue;
+    uw[k] = isfinite(u[k]) ? u[k] : 0.0f;
+    vw[k] = isfinite(v[k]) ? v[k] : 0.0f;
+  }
+  HIPCHK(hipMemcpyAsync(c->c.uw, uw.data(), WN * 4, hipMemcpyHostToDevice, c->st));
+  HIPCHK(hipMemcpyAsync(c->c.vw, vw.data(), WN * 4, hipMemcpyHostToDevice, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  return SILT_OK;
+}
+
+long silt_core_non_finite_count(silt_core* c) {
+  if (!c) return SILT_EINVAL;
+  if (restrict_fine(c)) return SILT_EDEVICE;
+  float* planes[8] = {c->c.water, c->c.slurry, c->c.uw, c->c.vw, c->c.us, c->c.vs, c->c.bed, c->c.moisture};
+  std::vector<float> h(WN);
+  long n = 0;
+  for (float* p : planes) {
+    HIPCHK(hipMemcpyAsync(h.data(), p, WN * 4, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    for (float x : h) n += isfinite(x) ? 0 : 1;
+  }
+  return n;
+}
+
+int silt_core_set_center(silt_core* c, float x, float z) {
+  if (!c) return SILT_EINVAL;
+  c->center[0] = x;
+  c->center[1] = z;
+  return SILT_OK;
+}
+
+int silt_core_center(const silt_core* c, float* out2) {
+  if (!c || !out2) return SILT_EINVAL;
+  out2[0] = c->center[0];
+  out2[1] = c->center[1];
+  return SILT_OK;
+}
+
+int silt_core_coarse_cell(silt_core* c, size_t i, size_t j, silt_coarse_cell* out) {
+  if (!c || !out) return SILT_EINVAL;
+  if (i >= (size_t)kWindowRes || j >= (size_t)kWindowRes) return 0;
+  if (restrict_fine(c)) return SILT_EDEVICE;
+  const size_t idx = j * kWindowRes + i;
+  float v[7];
+  float* planes[7] = {c->c.water, c->c.slurry, c->c.uw, c->c.us, c->c.vw, c->c.vs, c->c.bed};
+  for (int k = 0; k < 7; k++) HIPCHK(hipMemcpyAsync(&v[k], planes[k] + idx, 4, hipMemcpyDeviceToHost, c->st));
+  float m = 0.0f;
+  HIPCHK(hipMemcpyAsync(&m, c->c.moisture + idx, 4, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  auto fin = [](float x) { return isfinite(x) ? x : 0.0f; };
+  const float wm = fin(v[0]), sm = fin(v[1]), film = wm + sm;
+  out->x = c->origin[0] + ((float)i + 0.5f) * kCoarseCell;
+  out->z = c->origin[1] + ((float)j + 0.5f) * kCoarseCell;
+  out->bed_m = fin(v[6]);
+  out->water_m = wm;
+  out->slurry_m = sm;
+  out->moisture = fin(m);
+  out->u = film < 1.0e-5f ? 0.0f : fin((v[2] * wm + v[3] * sm) / film);
+  out->v = film < 1.0e-5f ? 0.0f : fin((v[4] * wm + v[5] * sm) / film);
+  out->settled_dust_m = 0.0f;
+  out->turbidity = 0.0f;
+  return 1;
+}
+
+int silt_core_sample(silt_core* c, float x, float z, silt_ground_sample* out) {
+  if (!c || !out) return SILT_EINVAL;
+  *out = silt_ground_sample{};
+  if (restrict_fine(c)) return SILT_EDEVICE;
+  const int i = coarse_axis(c->origin[0], x), j = coarse_axis(c->origin[1], z);
+  if (i < 0 || j < 0) return SILT_OK;
+  const size_t idx = (size_t)j * kWindowRes + i;
+  float* planes[6] = {c->c.water, c->c.slurry, c->c.us, c->c.vs, c->c.moisture, c->c.bed};
+  float v[6];
+  for (int k = 0; k < 6; k++) HIPCHK(hipMemcpyAsync(&v[k], planes[k] + idx, 4, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  out->water_m = v[0];
+  out->slurry_m = v[1];
+  out->u = v[2];
+  out->v = v[3];
+  out->settled_dust_m = 0.0f;
+  out->moisture = v[4];
+  out->bed_m = v[5];
+  return SILT_OK;
+}
+
+int silt_core_ledger_full(silt_core* c, silt_fluid_ledger_full* out) {
+  if (!out) return SILT_EINVAL;
+  silt_fluid_ledger l{};
+  int rc = silt_core_ledger(c, &l);
+  if (rc) return rc;
+  *out = silt_fluid_ledger_full{};
+  out->water_m3 = l.water_m3;
+  out->slurry_m3 = l.slurry_m3;
+  out->frozen_water_m3 = l.frozen_water_m3;
+  out->frozen_slurry_m3 = l.frozen_slurry_m3;
+  out->spray_m3 = l.spray_m3;
+  out->total_m3 = l.total_m3;
+  return SILT_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// column_sum (src/fluid_core.rs:4617-4620) over a host plane: the
+// reference's sequential f32 sum, so the volume restore_wiped puts back is
+// the reference's to the bit.
+float column_sum_host(const std::vector<float>& h) {
+  float s = -0.0f;
+  for (float x : h) s += fmaxf(x, 0.0f);
+  return s * (kCoarseCell * kCoarseCell);
+}
+
+// restore_wiped (:4622-4637): volume a step lost to its dry cut goes back
+// into the deepest cell (the last of equal ones, as max_by keeps).
+void restore_wiped_host(std::vector<float>& h, float before) {
+  const float lost = before - column_sum_host(h);
+  if (lost <= 1.0e-8f) return;
+  int best = -1;
+  for (int k = 0; k < (int)h.size(); k++)
+    if (isfinite(h[k]) && (best < 0 || !(h[k] < h[best]))) best = k;
+  if (best >= 0) h[best] += lost / (kCoarseCell * kCoarseCell);
+}
+
+float peak_speed_host(const std::vector<float>& h, const std::vector<float>& u, const std::vector<float>& v) {
+  float peak = 0.0f;
+  for (size_t k = 0; k < h.size(); k++)
+    if (h[k] > 1.0e-4f) peak = fmaxf(peak, fmaxf(fabsf(u[k]), fabsf(v[k])));
+  return peak;
+}
+
+}  // namespace
+
+extern "C" {
+
+// drain_over_bed (src/fluid_core.rs:5123-5214): settle a pour with gravity
+// steps (no confinement, pace 2, extra friction), restoring what the dry cut
+// wipes, until the sheet has moved and then stayed calm for 24 steps.
+float silt_core_drain_over_bed(silt_core* c, float dt, unsigned steps) {
+  if (!c) return NAN;
+  auto volume = [&]() -> float {
+    std::vector<float> w(WN);
+    if (hipMemcpyAsync(w.data(), c->c.water, WN * 4, hipMemcpyDeviceToHost, c->st) != hipSuccess ||
+        hipStreamSynchronize(c->st) != hipSuccess)
+      return NAN;
+    return column_sum_host(w);
+  };
+  if (!(isfinite(dt) && dt > 0.0f) || steps == 0) return volume();
+  if (restrict_fine(c)) return NAN;
+  Workspace* ws = workspace_for(kWindowRes);
+  if (!ws) return NAN;
+  unsigned calm = 0;
+  bool moved = false;
+  const float keep = expf(-0.55f * dt);
+  std::vector<float> hw(WN), hs(WN), a(WN), b(WN), d(WN), e(WN);
+  for (unsigned it = 0; it < steps; it++) {
+    if (hipMemcpyAsync(hw.data(), c->c.water, WN * 4, hipMemcpyDeviceToHost, c->st) != hipSuccess ||
+        hipMemcpyAsync(hs.data(), c->c.slurry, WN * 4, hipMemcpyDeviceToHost, c->st) != hipSuccess ||
+        hipStreamSynchronize(c->st) != hipSuccess)
+      return NAN;
+    const float wb = column_sum_host(hw), sb = column_sum_host(hs);
+    silt_sheet_opts o;
+    silt_sheet_opts_default(&o);
+    o.path = c->path;
+    o.row_split = 1;
+    o.pace = 2;
+    o.sub_cap = 8.0f;
+    o.banks = 1;
+    SheetPlanes sp = ws->planes();
+    core_soften(c->c.uw, c->c.vw, kWindowRes, c->st);
+    sp.h = c->c.water;
+    sp.u = c->c.uw;
+    sp.v = c->c.vw;
+    sp.bed = c->c.bed;
+    silt_drive dg{SILT_DRIVE_GRAVITY, 0.0f, 0.0f, 0.0f};
+    if (step_device_planes(sp, kWindowRes, kCoarseCell, dt, dg, o, true, true, dynamic_viscosity(3, 1.0f), c->st) < 0)
+      return NAN;
+    core_nu(c->nu_c, c->c.grain, c->c.moisture, WN, c->st);
+    core_soften(c->c.us, c->c.vs, kWindowRes, c->st);
+    sp.h = c->c.slurry;
+    sp.u = c->c.us;
+    sp.v = c->c.vs;
+    sp.nu = c->nu_c;
+    if (step_device_planes(sp, kWindowRes, kCoarseCell, dt, dg, o, true, false, 0.0f, c->st) < 0) return NAN;
+    float* vel[4] = {c->c.uw, c->c.vw, c->c.us, c->c.vs};
+    std::vector<float>* hv[4] = {&a, &b, &d, &e};
+    for (int k = 0; k < 4; k++)
+      if (hipMemcpyAsync(hv[k]->data(), vel[k], WN * 4, hipMemcpyDeviceToHost, c->st) != hipSuccess) return NAN;
+    if (hipMemcpyAsync(hw.data(), c->c.water, WN * 4, hipMemcpyDeviceToHost, c->st) != hipSuccess ||
+        hipMemcpyAsync(hs.data(), c->c.slurry, WN * 4, hipMemcpyDeviceToHost, c->st) != hipSuccess ||
+        hipStreamSynchronize(c->st) != hipSuccess)
+      return NAN;
+    for (int k = 0; k < 4; k++)
+      for (float& x : *hv[k]) x *= keep;
+    restore_wiped_host(hw, wb);
+    restore_wiped_host(hs, sb);
+    for (int k = 0; k < 4; k++)
+      if (hipMemcpyAsync(vel[k], hv[k]->data(), WN * 4, hipMemcpyHostToDevice, c->st) != hipSuccess) return NAN;
+    if (hipMemcpyAsync(c->c.water, hw.data(), WN * 4, hipMemcpyHostToDevice, c->st) != hipSuccess ||
+        hipMemcpyAsync(c->c.slurry, hs.data(), WN * 4, hipMemcpyHostToDevice, c->st) != hipSuccess ||
+        hipStreamSynchronize(c->st) != hipSuccess)
+      return NAN;
+    const float speed = fmaxf(peak_speed_host(hw, a, b), peak_speed_host(hs, d, e));
+    if (speed > 0.05f) moved = true;
+    if (moved && speed < 0.008f) {
+      if (++calm >= 24) break;
+    } else {
+      calm = 0;
+    }
+  }
+  return volume();
+}
+int silt_surrogate_targets_debug(const float* hn_w, const float* hc_w, const float* hn_s, const float* hc_s, int i0,
+                                 int j0, int nx, int nz, float* targets) {
+  if (!hn_w || !hc_w || !hn_s || !hc_s || !targets || nx < 2 || nz < 2 || nx > kSurMax || nz > kSurMax || i0 < 0 ||
+      j0 < 0 || i0 + nx > kWindowRes || j0 + nz > kWindowRes)
+    return SILT_EINVAL;
+  float* d = nullptr;
+  if (hipMalloc(&d, (4 * (size_t)WN + kSurTgtFloats) * sizeof(float)) != hipSuccess) return SILT_ENOMEM;
+  const float* src[4] = {hn_w, hc_w, hn_s, hc_s};
+  int rc = SILT_OK;
+  for (int k = 0; k < 4 && rc == SILT_OK; k++)
+    if (hipMemcpy(d + k * WN, src[k], WN * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) rc = SILT_EDEVICE;
+  if (rc == SILT_OK) {
+    sur_targets(d, d + WN, d + 2 * WN, d + 3 * WN, kWindowRes, SurRegion{i0, j0, nx, nz}, d + 4 * WN, nullptr);
+    if (hipMemcpy(targets, d + 4 * WN, (size_t)nx * nz * 8 * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
+      rc = SILT_EDEVICE;
+  }
+  (void)hipFree(d);
+  return rc;
+}
+
+int silt_surrogate_apply_debug(float* water, float* slurry, const float* y, int i0, int j0, int nx, int nz) {
+  if (!water || !slurry || !y || nx < 2 || nz < 2 || nx > kSurMax || nz > kSurMax || i0 < 0 || j0 < 0 ||
+      i0 + nx > kWindowRes || j0 + nz > kWindowRes)
+    return SILT_EINVAL;
+  float* d = nullptr;
+  const size_t ny = (size_t)nx * nz * 8;
+  if (hipMalloc(&d, (2 * (size_t)WN + ny + 2 * kSurMax * kSurMax) * sizeof(float)) != hipSuccess) return SILT_ENOMEM;
+  int rc = SILT_OK;
+  if (hipMemcpy(d, water, WN * 4, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(d + WN, slurry, WN * 4, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(d + 2 * WN, y, ny * 4, hipMemcpyHostToDevice) != hipSuccess)
+    rc = SILT_EDEVICE;
+  if (rc == SILT_OK) {
+    sur_apply(d, d + WN, kWindowRes, d + 2 * WN, SurRegion{i0, j0, nx, nz}, d + 2 * WN + ny, nullptr);
+    if (hipMemcpy(water, d, WN * 4, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(slurry, d + WN, WN * 4, hipMemcpyDeviceToHost) != hipSuccess)
+      rc = SILT_EDEVICE;
+  }
+  (void)hipFree(d);
+  return rc;
+}
+
+}  // extern "C"
+
+namespace {
+
+// world_key (src/fluid_core.rs:9894-9896) packed into 64 bits.
+uint64_t park_key(float x, float z) {
+  const int kx = (int)floorf(x / kCoarseCell), kz = (int)floorf(z / kCoarseCell);
+  return ((uint64_t)(uint32_t)kx << 32) | (uint64_t)(uint32_t)kz;
+}
+
+// park_slot (src/fluid_core.rs:4797-4805) for the planes this library holds.
+bool park_worth(const silt_core::Parked& p, float ambient) {
+  return p.water > 1.0e-8f || p.slurry > 1.0e-8f || fabsf(p.moisture - ambient) > 6.0e-3f;
+}
+
+// each_leaving (src/fluid_core.rs:9936-9986): cells that leave the window
+// when its origin moves by (di, dj) cells, in the reference's order.
+template <typename Fn>
+void each_leaving(int di, int dj, Fn f) {
+  const int n = kWindowRes;
+  if (di > 0) {
+    const int w = std::min(di, n);
+    for (int j = 0; j < n; j++)
+      for (int i = 0; i < w; i++) f(i, j);
+  } else if (di < 0) {
+    const int w = std::min(-di, n);
+    for (int j = 0; j < n; j++)
+      for (int i = n - w; i < n; i++) f(i, j);
+  }
+  auto covered = [&](int i) { return di > 0 ? i < di : di < 0 ? i >= n + di : false; };
+  if (dj > 0) {
+    const int h = std::min(dj, n);
+    for (int j = 0; j < h; j++)
+      for (int i = 0; i < n; i++)
+        if (!covered(i)) f(i, j);
+  } else if (dj < 0) {
+    const int h = std::min(-dj, n);
+    for (int j = n - h; j < n; j++)
+      for (int i = 0; i < n; i++)
+        if (!covered(i)) f(i, j);
+  }
+}
+
+// uniform_cell_shift (src/fluid_core.rs:9899-9934).
+bool uniform_cell_shift(const float* o, const float* nw, int* di, int* dj) {
+  auto key = [](const float* org, int i, int j, int* kx, int* kz) {
+    *kx = (int)floorf((org[0] + ((float)i + 0.5f) * kCoarseCell) / kCoarseCell);
+    *kz = (int)floorf((org[1] + ((float)j + 0.5f) * kCoarseCell) / kCoarseCell);
+  };
+  int o0x, o0z, n0x, n0z, o1x, o1z, n1x, n1z, omx, omz, nmx, nmz;
+  const int last = kWindowRes - 1, mid = kWindowRes / 2;
+  key(o, 0, 0, &o0x, &o0z);
+  key(nw, 0, 0, &n0x, &n0z);
+  *di = n0x - o0x;
+  *dj = n0z - o0z;
+  key(o, last, last, &o1x, &o1z);
+  key(nw, last, last, &n1x, &n1z);
+  if (o1x - o0x != last || o1z - o0z != last) return false;
+  if (n1x - n0x != last || n1z - n0z != last) return false;
+  if (n1x - o1x != *di || n1z - o1z != *dj) return false;
+  key(o, mid, mid, &omx, &omz);
+  key(nw, mid, mid, &nmx, &nmz);
+  return nmx - omx == *di && nmz - omz == *dj;
+}
+
+}  // namespace
+
+extern "C" {
+
+// recenter (src/fluid_core.rs:5409-5422) with shift_coarse and its scroll
+// and hash forms (:9004-9226): cells that slide off are parked by world cell,
+// cells that slide on come back from the parked set; bed and grain stay on
+// their array index (the caller reads the substrate again).
+int silt_core_recenter(silt_core* c, float x, float z) {
+  if (!c || !isfinite(x) || !isfinite(z)) return SILT_EINVAL;
+  if (restrict_fine(c)) return SILT_EDEVICE;
+  c->center[0] = x;
+  c->center[1] = z;
+  const float extent = (float)kWindowRes * kCoarseCell;
+  const float nw[2] = {floorf((x - extent * 0.5f) / kCoarseCell) * kCoarseCell,
+                       floorf((z - extent * 0.5f) / kCoarseCell) * kCoarseCell};
+  if (fabsf(nw[0] - c->origin[0]) < 1.0e-4f && fabsf(nw[1] - c->origin[1]) < 1.0e-4f) return SILT_OK;
+  float* dev[7] = {c->c.water, c->c.slurry, c->c.uw, c->c.vw, c->c.us, c->c.vs, c->c.moisture};
+  std::vector<float> h[7], o[7];
+  for (int k = 0; k < 7; k++) {
+    h[k].resize(WN);
+    HIPCHK(hipMemcpyAsync(h[k].data(), dev[k], WN * 4, hipMemcpyDeviceToHost, c->st));
+  }
+  HIPCHK(hipStreamSynchronize(c->st));
+  const float amb = c->moisture_ambient;
+  auto slot_at = [&](int idx) {
+    return silt_core::Parked{h[0][idx], h[1][idx], h[2][idx], h[3][idx], h[4][idx], h[5][idx], h[6][idx]};
+  };
+  auto park = [&](const float* org, int i, int j) {
+    const int idx = j * kWindowRes + i;
+    const uint64_t key = park_key(org[0] + ((float)i + 0.5f) * kCoarseCell, org[1] + ((float)j + 0.5f) * kCoarseCell);
+    const silt_core::Parked p = slot_at(idx);
+    if (park_worth(p, amb))
+      c->parked[key] = p;
+    else
+      c->parked.erase(key);
+  };
+  auto restore = [&](const float* org, int i, int j) {
+    const int idx = j * kWindowRes + i;
+    const uint64_t key = park_key(org[0] + ((float)i + 0.5f) * kCoarseCell, org[1] + ((float)j + 0.5f) * kCoarseCell);
+    auto it = c->parked.find(key);
+    if (it == c->parked.end()) return;
+    const silt_core::Parked& p = it->second;
+    const float v[7] = {p.water, p.slurry, p.uw, p.vw, p.us, p.vs, p.moisture};
+    for (int k = 0; k < 7; k++) o[k][idx] = v[k];
+    c->parked.erase(it);
+  };
+  for (int k = 0; k < 7; k++) o[k].assign(WN, k == 6 ? amb : 0.0f);
+  int di = 0, dj = 0;
+  if (uniform_cell_shift(c->origin, nw, &di, &dj)) {
+    const float old[2] = {c->origin[0], c->origin[1]};
+    each_leaving(di, dj, [&](int i, int j) { park(old, i, j); });
+    for (int j = 0; j < kWindowRes; j++)
+      for (int i = 0; i < kWindowRes; i++) {
+        const int si = i + di, sj = j + dj;
+        if (si < 0 || sj < 0 || si >= kWindowRes || sj >= kWindowRes) continue;
+        const int src = sj * kWindowRes + si, dst = j * kWindowRes + i;
+        if (!(h[0][src] > 1.0e-8f || h[1][src] > 1.0e-8f || h[6][src] > 1.0e-4f)) continue;
+        for (int k = 0; k < 7; k++) o[k][dst] = h[k][src];
+      }
+    c->origin[0] = nw[0];
+    c->origin[1] = nw[1];
+    each_leaving(-di, -dj, [&](int i, int j) { restore(c->origin, i, j); });
+  } else {
+    const float old[2] = {c->origin[0], c->origin[1]};
+    for (int j = 0; j < kWindowRes; j++)
+      for (int i = 0; i < kWindowRes; i++) park(old, i, j);
+    c->origin[0] = nw[0];
+    c->origin[1] = nw[1];
+    for (int j = 0; j < kWindowRes; j++)
+      for (int i = 0; i < kWindowRes; i++) restore(c->origin, i, j);
+  }
+  for (int k = 0; k < 7; k++) HIPCHK(hipMemcpyAsync(dev[k], o[k].data(), WN * 4, hipMemcpyHostToDevice, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  return SILT_OK;
+}
+
+// lay_slurry_on_water (src/fluid_core.rs:5236-5254): slurry of `depth` on
+// every cell holding more than 12 mm of water; returns the volume laid (the
+// reference's f32 sum).
+float silt_core_lay_slurry_on_water(silt_core* c, float depth, int grain) {
+  if (!c || !(isfinite(depth) && depth > 0.0f) || grain < 0 || grain > 4) return 0.0f;
+  if (restrict_fine(c)) return NAN;
+  std::vector<float> w(WN), sl(WN), m(WN);
+  std::vector<uint8_t> g(WN);
+  if (hipMemcpyAsync(w.data(), c->c.water, WN * 4, hipMemcpyDeviceToHost, c->st) != hipSuccess ||
+      hipMemcpyAsync(sl.data(), c->c.slurry, WN * 4, hipMemcpyDeviceToHost, c->st) != hipSuccess ||
+      hipMemcpyAsync(m.data(), c->c.moisture, WN * 4, hipMemcpyDeviceToHost, c->st) != hipSuccess ||
+      hipMemcpyAsync(g.data(), c->c.grain, WN, hipMemcpyDeviceToHost, c->st) != hipSuccess ||
+      hipStreamSynchronize(c->st) != hipSuccess)
+    return NAN;
+  const float area = kCoarseCell * kCoarseCell;
+  float vol = 0.0f;
+  for (int k = 0; k < WN; k++) {
+    if (w[k] <= 0.012f) continue;
+    sl[k] = depth;
+    g[k] = (uint8_t)grain;
+    m[k] = fmaxf(m[k], 0.90f);
+    vol += depth * area;
+  }
+  if (hipMemcpyAsync(c->c.slurry, sl.data(), WN * 4, hipMemcpyHostToDevice, c->st) != hipSuccess ||
+      hipMemcpyAsync(c->c.moisture, m.data(), WN * 4, hipMemcpyHostToDevice, c->st) != hipSuccess ||
+      hipMemcpyAsync(c->c.grain, g.data(), WN, hipMemcpyHostToDevice, c->st) != hipSuccess ||
+      hipStreamSynchronize(c->st) != hipSuccess)
+    return NAN;
+  return vol;
+}
+
+// window_bounds (src/fluid_core.rs:6977-6980): corner x, z and the extent.
+int silt_core_window_bounds(const silt_core* c, float* out3) {
+  if (!c || !out3) return SILT_EINVAL;
+  out3[0] = c->origin[0];
+  out3[1] = c->origin[1];
+  out3[2] = (float)kWindowRes * kCoarseCell;
+  return SILT_OK;
+}
+
+long silt_core_parked_cells(const silt_core* c) { return c ? (long)c->parked.size() : SILT_EINVAL; }
+
+}  // extern "C"
+
+// Training set and trainer of the tire sub-grid network (include/silt_mlp.h).
+struct silt_trainer {
+  size_t max_steps = 0;
+  unsigned seed = 0;
+  silt_core* twin = nullptr;
+  float* ctx = nullptr;  // [max_steps][kSurCtxFloats]
+  float* tgt = nullptr;  // [max_steps][kSurTgtFloats]
+  SurRegion* regions = nullptr;
+  std::vector<SurRegion> h_regions;
+  size_t nsteps = 0;
+  Mlp net;
+  hipStream_t st = nullptr;
+  uint32_t iter = 0;
+};
+
+extern "C" {
+
+silt_trainer* silt_trainer_create(size_t max_steps, unsigned seed) {
+  if (max_steps == 0 || max_steps > (1u << 20)) return nullptr;
+  silt_trainer* t = new silt_trainer();
+  t->max_steps = max_steps;
+  t->seed = seed;
+  bool ok = hipStreamCreateWithFlags(&t->st, hipStreamNonBlocking) == hipSuccess &&
+            hipMalloc(&t->ctx, max_steps * kSurCtxFloats * sizeof(float)) == hipSuccess &&
+            hipMalloc(&t->tgt, max_steps * kSurTgtFloats * sizeof(float)) == hipSuccess &&
+            hipMalloc(&t->regions, max_steps * sizeof(SurRegion)) == hipSuccess &&
+            mlp_init(t->net, 512, seed, t->st) == 0 && hipStreamSynchronize(t->st) == hipSuccess;
+  if (!ok) {
+    silt_trainer_destroy(t);
+    return nullptr;
+  }
+  t->h_regions.resize(max_steps);
+  return t;
+}
+
+void silt_trainer_destroy(silt_trainer* t) {
+  if (!t) return;
+  if (t->st) (void)hipStreamSynchronize(t->st);
+  mlp_free(t->net);
+  if (t->twin) silt_core_destroy(t->twin);
+  if (t->ctx) (void)hipFree(t->ctx);
+  if (t->tgt) (void)hipFree(t->tgt);
+  if (t->regions) (void)hipFree(t->regions);
+  if (t->st) (void)hipStreamDestroy(t->st);
+  delete t;
+}
+
+int silt_trainer_record(silt_trainer* t, silt_core* c, float dt, silt_spray_event* events, size_t cap,
+                        size_t* nevents) {
+  if (!t || !c) return SILT_EINVAL;
+  SurRegion r = sur_region(c, c->pushes);
+  if (!(dt > 0.0f) || !r.nx || t->nsteps >= t->max_steps || c->tire_model != SILT_TIRE_NESTED) {
+    int rc = silt_core_step(c, dt, events, cap, nevents);
+    return rc < 0 ? rc : 0;
+  }
+  if (!t->twin) {
+    t->twin = silt_core_create(c->origin[0] + 0.5f * kWindowRes * kCoarseCell,
+                               c->origin[1] + 0.5f * kWindowRes * kCoarseCell, c->path);
+    if (!t->twin) return SILT_ENOMEM;
+  }
+  silt_core* w = t->twin;
+  int rc = restrict_fine(c);
+  if (rc) return rc;
+  // The twin takes the core's state and tires and steps coarse-only.
+  w->origin[0] = c->origin[0];
+  w->origin[1] = c->origin[1];
+  w->coarse_confine = c->coarse_confine;
+  w->near_on = false;
+  w->tire_model = SILT_TIRE_COARSE;
+  float* src[8] = {c->c.water, c->c.slurry, c->c.uw, c->c.vw, c->c.us, c->c.vs, c->c.bed, c->c.moisture};
+  float* dst[8] = {w->c.water, w->c.slurry, w->c.uw, w->c.vw, w->c.us, w->c.vs, w->c.bed, w->c.moisture};
+  for (int k = 0; k < 8; k++)
+    HIPCHK(hipMemcpyAsync(dst[k], src[k], WN * sizeof(float), hipMemcpyDeviceToDevice, c->st));
+  HIPCHK(hipMemcpyAsync(w->c.grain, c->c.grain, WN, hipMemcpyDeviceToDevice, c->st));
+  const PushList pl = push_list(c->pushes);
+  sur_context(core_grid(c), r, pl, t->ctx + t->nsteps * kSurCtxFloats, c->st);
+  HIPCHK(hipStreamSynchronize(c->st));
+  w->pushes = c->pushes;
+  if ((rc = silt_core_step(w, dt, nullptr, 0, nullptr)) < 0) return rc;
+  if ((rc = silt_core_step(c, dt, events, cap, nevents)) < 0) return rc;
+  sur_targets(c->c.water, w->c.water, c->c.slurry, w->c.slurry, kWindowRes, r, t->tgt + t->nsteps * kSurTgtFloats,
+              c->st);
+  t->h_regions[t->nsteps] = r;
+  HIPCHK(hipMemcpyAsync(t->regions + t->nsteps, &t->h_regions[t->nsteps], sizeof(SurRegion),
+                        hipMemcpyHostToDevice, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  t->nsteps++;
+  return (int)t->nsteps;
+}
+
+long silt_trainer_steps(const silt_trainer* t) { return t ? (long)t->nsteps : SILT_EINVAL; }
+
+long silt_trainer_samples(const silt_trainer* t) {
+  if (!t) return SILT_EINVAL;
+  long n = 0;
+  for (size_t s = 0; s < t->nsteps; s++) n += (long)t->h_regions[s].nx * t->h_regions[s].nz;
+  return n;
+}
+
+int silt_trainer_fit(silt_trainer* t, size_t train_steps, int iterations, int batch, float lr, double* last_mse) {
+  if (!t || iterations < 0 || batch < 1 || batch > 8192 || !(lr > 0.0f)) return SILT_EINVAL;
+  if (train_steps == 0 || train_steps > t->nsteps) train_steps = t->nsteps;
+  if (train_steps == 0) return SILT_EINVAL;
+  int rc = mlp_reserve(t->net, (size_t)batch);
+  if (rc) return rc;
+  for (int it = 0; it < iterations; it++) {
+    sur_gather_random(t->ctx, t->tgt, t->regions, (int)train_steps, t->seed * 2654435761u + ++t->iter, batch,
+                      t->net.xb, t->net.tg, t->st);
+    mlp_forward(t->net, (size_t)batch);
+    mlp_backward(t->net, (size_t)batch);
+    mlp_adam(t->net, lr);
+  }
+  double sse = 0.0;
+  HIPCHK(hipMemcpyAsync(&sse, t->net.loss, sizeof sse, hipMemcpyDeviceToHost, t->st));
+  HIPCHK(hipStreamSynchronize(t->st));
+  if (last_mse) *last_mse = iterations ? sse / ((double)batch * kMlpOut) : 0.0;
+  return SILT_OK;
+}
+
+int silt_trainer_eval(silt_trainer* t, size_t first, size_t count, double* mse_net, double* mse_zero) {
+  if (!t || first + count > t->nsteps || count == 0) return SILT_EINVAL;
+  int rc = mlp_reserve(t->net, (size_t)kSurMax * kSurMax);
+  if (rc) return rc;
+  double se = 0.0, sz = 0.0;
+  size_t n = 0;
+  std::vector<float> y((size_t)kSurMax * kSurMax * kMlpOut), tg(y.size());
+  for (size_t s = first; s < first + count; s++) {
+    const SurRegion& r = t->h_regions[s];
+    const size_t cells = (size_t)r.nx * r.nz;
+    sur_gather_region(t->ctx + s * kSurCtxFloats, r, t->net.xb, t->st);
+    mlp_forward(t->net, cells);
+    HIPCHK(hipMemcpyAsync(y.data(), t->net.y, cells * kMlpOut * sizeof(float), hipMemcpyDeviceToHost, t->st));
+    HIPCHK(hipMemcpyAsync(tg.data(), t->tgt + s * kSurTgtFloats, cells * kMlpOut * sizeof(float),
+                          hipMemcpyDeviceToHost, t->st));
+    HIPCHK(hipStreamSynchronize(t->st));
+    for (size_t k = 0; k < cells * kMlpOut; k++) {
+      const double d = (double)y[k] - tg[k];
+      se += d * d;
+      sz += (double)tg[k] * tg[k];
+    }
+    n += cells * kMlpOut;
+  }
+  if (mse_net) *mse_net = se / (double)n;
+  if (mse_zero) *mse_zero = sz / (double)n;
+  return SILT_OK;
+}
+
+int silt_trainer_params(const silt_trainer* t, float* params) {
+  if (!t || !params) return SILT_EINVAL;
+  HIPCHK(hipMemcpyAsync(params, t->net.master, kMlpParams * sizeof(float), hipMemcpyDeviceToHost, t->st));
+  HIPCHK(hipStreamSynchronize(t->st));
+  return SILT_OK;
+}
+
+int silt_trainer_set_params(silt_trainer* t, const float* params) {
+  if (!t || !params) return SILT_EINVAL;
+  HIPCHK(hipMemcpyAsync(t->net.master, params, kMlpParams * sizeof(float), hipMemcpyHostToDevice, t->st));
+  mlp_sync_bf16(t->net);
+  HIPCHK(hipStreamSynchronize(t->st));
+  return SILT_OK;
 }
 
 }  // extern "C"

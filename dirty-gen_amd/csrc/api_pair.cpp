@@ -12,11 +12,16 @@
 
 #include <vector>
 
+#include "core.hpp"
+#include "mlp.hpp"
 #include "plan.hpp"
+#include "rut.hpp"
 #include "sheet_device.hpp"
 #include "sheet_fused.hpp"
+#include "silt_mlp.h"
 #include "silt_pair.h"
 #include "silt_sheet.h"
+#include "surrogate.hpp"
 #include "workspace.hpp"
 
 #define HIPCHK(x)                                 \
@@ -52,6 +57,20 @@ struct silt_pair {
   int last_nsub[2] = {0, 0};
   int last_fallback = 0;
   PlanScratch ps;
+  // Tires on the sheet (silt_pair_step_tires): the ground the ruts cut (the
+  // sheet bed is ground bed + placed), the world corner of cell (0, 0), the
+  // network and its per-vehicle context and limiter scratch.
+  float origin[2] = {0, 0};
+  float *soil_bed = nullptr, *soil_placed = nullptr, *soil_init = nullptr, *soil_rut = nullptr;
+  RutContact* d_contacts = nullptr;
+  size_t cap_contacts = 0;
+  bool net_ready = false;
+  Mlp net;
+  float* sur_ctx = nullptr;
+  float* sur_fac = nullptr;
+  size_t cap_groups = 0;
+  hipEvent_t tev[4] = {nullptr, nullptr, nullptr, nullptr};
+  float last_tire_ms = 0.0f;
 };
 
 namespace {
@@ -107,6 +126,8 @@ silt_pair* silt_pair_create(size_t res, float cell, int path) {
   }
   stats_reset(p->stats, 2, p->stream);
   (void)hipStreamSynchronize(p->stream);
+  p->origin[0] = -0.5f * (float)res * cell;
+  p->origin[1] = -0.5f * (float)res * cell;
   return p;
 }
 
@@ -114,6 +135,12 @@ void silt_pair_destroy(silt_pair* p) {
   if (!p) return;
   if (p->stream) (void)hipStreamSynchronize(p->stream);
   plan_scratch_free(p->ps);
+  if (p->net_ready) mlp_free(p->net);
+  for (void* q : {(void*)p->soil_bed, (void*)p->soil_placed, (void*)p->soil_init, (void*)p->soil_rut,
+                  (void*)p->d_contacts, (void*)p->sur_ctx, (void*)p->sur_fac})
+    if (q) (void)hipFree(q);
+  for (hipEvent_t e : p->tev)
+    if (e) (void)hipEventDestroy(e);
   (void)hipFree(p->bed);
   (void)hipFree(p->nu_slurry);
   for (int l = 0; l < 2; l++)
@@ -358,5 +385,165 @@ int silt_pair_autotune(silt_pair* p, float dt, const int* rows, size_t n, int re
 }
 
 int silt_pair_last_fallback(const silt_pair* p) { return p ? p->last_fallback : SILT_EINVAL; }
+
+int silt_pair_set_ground(silt_pair* p, const float* bed, float origin_x, float origin_z) {
+  if (!p || !bed || !isfinite(origin_x) || !isfinite(origin_z)) return SILT_EINVAL;
+  const size_t fb = p->n * sizeof(float);
+  if (!p->soil_bed) {
+    if (hipMalloc(&p->soil_bed, fb) != hipSuccess || hipMalloc(&p->soil_placed, fb) != hipSuccess ||
+        hipMalloc(&p->soil_init, fb) != hipSuccess || hipMalloc(&p->soil_rut, fb) != hipSuccess)
+      return SILT_ENOMEM;
+  }
+  HIPCHK(hipMemcpyAsync(p->soil_bed, bed, fb, hipMemcpyHostToDevice, p->stream));
+  HIPCHK(hipMemcpyAsync(p->soil_init, bed, fb, hipMemcpyHostToDevice, p->stream));
+  HIPCHK(hipMemsetAsync(p->soil_placed, 0, fb, p->stream));
+  core_ground(p->soil_bed, p->soil_placed, p->soil_init, p->bed, p->soil_rut, (int)p->n, p->stream);
+  HIPCHK(hipStreamSynchronize(p->stream));
+  p->origin[0] = origin_x;
+  p->origin[1] = origin_z;
+  return SILT_OK;
+}
+
+int silt_pair_get_ground(silt_pair* p, float* bed, float* placed, float* rut) {
+  if (!p || !p->soil_bed) return SILT_EINVAL;
+  const size_t fb = p->n * sizeof(float);
+  if (bed) HIPCHK(hipMemcpyAsync(bed, p->soil_bed, fb, hipMemcpyDeviceToHost, p->stream));
+  if (placed) HIPCHK(hipMemcpyAsync(placed, p->soil_placed, fb, hipMemcpyDeviceToHost, p->stream));
+  if (rut) HIPCHK(hipMemcpyAsync(rut, p->soil_rut, fb, hipMemcpyDeviceToHost, p->stream));
+  HIPCHK(hipStreamSynchronize(p->stream));
+  return SILT_OK;
+}
+
+int silt_pair_set_surrogate(silt_pair* p, const float* params) {
+  if (!p) return SILT_EINVAL;
+  if (!params) {
+    p->net_ready = false;
+    return SILT_OK;
+  }
+  if (!p->tev[0])
+    for (hipEvent_t& e : p->tev) HIPCHK(hipEventCreate(&e));
+  if (!p->net.master && mlp_init(p->net, (size_t)kSurMax * kSurMax, 0u, p->stream) != 0) return SILT_ENOMEM;
+  HIPCHK(hipMemcpyAsync(p->net.master, params, kMlpParams * sizeof(float), hipMemcpyHostToDevice, p->stream));
+  mlp_sync_bf16(p->net);
+  HIPCHK(hipStreamSynchronize(p->stream));
+  p->net_ready = true;
+  return SILT_OK;
+}
+
+float silt_pair_last_tire_ms(const silt_pair* p) { return p ? p->last_tire_ms : 0.0f; }
+
+int silt_pair_step_tires(silt_pair* p, float dt, const silt_tire_push* pushes, size_t npush, size_t per_vehicle,
+                         const silt_tire_contact* contacts, size_t ncontacts) {
+  if (!p || (npush && (!pushes || per_vehicle == 0 || npush % per_vehicle || per_vehicle > (size_t)kMaxPush)) ||
+      (ncontacts && !contacts))
+    return SILT_EINVAL;
+  if (!(dt > 0.0f)) return 0;
+  hipStream_t st = p->stream;
+  const int res = p->res;
+  // Ruts first: the contacts cut the ground, the sheet bed follows.
+  if (ncontacts && p->soil_bed) {
+    if (p->cap_contacts < ncontacts) {
+      HIPCHK(hipStreamSynchronize(st));
+      if (p->d_contacts) (void)hipFree(p->d_contacts);
+      p->d_contacts = nullptr;
+      p->cap_contacts = 0;
+      if (hipMalloc(&p->d_contacts, ncontacts * sizeof(RutContact)) != hipSuccess) return SILT_ENOMEM;
+      p->cap_contacts = ncontacts;
+    }
+    int rc = rut_stamp(p->soil_bed, p->soil_placed, p->soil_init, res, p->cell, p->origin[0], p->origin[1], contacts,
+                       ncontacts, dt, p->d_contacts, st);
+    if (rc) return rc;
+    core_ground(p->soil_bed, p->soil_placed, p->soil_init, p->bed, p->soil_rut, (int)p->n, st);
+  }
+  // Vehicles: groups of per_vehicle pushes.
+  const size_t groups = npush ? npush / per_vehicle : 0;
+  std::vector<PushList> lists(groups);
+  std::vector<SurRegion> regions(groups);
+  std::vector<size_t> offset(groups + 1, 0);
+  for (size_t v = 0; v < groups; v++) {
+    PushList& pl = lists[v];
+    pl.n = (int)per_vehicle;
+    for (size_t k = 0; k < per_vehicle; k++) {
+      const silt_tire_push& q = pushes[v * per_vehicle + k];
+      pl.p[k] = CorePush{q.x, q.z, q.forward_x, q.forward_z, q.radius, q.width, q.speed, q.slip};
+    }
+    regions[v] = p->net_ready ? sur_region_at(pl.p, pl.n, p->origin[0], p->origin[1], p->cell, res) : SurRegion{};
+    offset[v + 1] = offset[v] + (size_t)regions[v].nx * regions[v].nz;
+  }
+  const bool timed = p->tev[0] != nullptr;
+  if (timed) HIPCHK(hipEventRecord(p->tev[0], st));
+  const size_t cells = offset[groups];
+  float** wl = p->planes[0][p->cur[0]];
+  float** sl = p->planes[1][p->cur[1]];
+  if (p->net_ready && cells) {
+    if (p->cap_groups < groups) {
+      HIPCHK(hipStreamSynchronize(st));
+      if (p->sur_ctx) (void)hipFree(p->sur_ctx);
+      if (p->sur_fac) (void)hipFree(p->sur_fac);
+      p->sur_ctx = p->sur_fac = nullptr;
+      p->cap_groups = 0;
+      if (hipMalloc(&p->sur_ctx, groups * kSurCtxFloats * sizeof(float)) != hipSuccess ||
+          hipMalloc(&p->sur_fac, groups * 2 * kSurMax * kSurMax * sizeof(float)) != hipSuccess)
+        return SILT_ENOMEM;
+      p->cap_groups = groups;
+    }
+    int rc = mlp_reserve(p->net, cells);
+    if (rc) return rc;
+    const SurGrid g{wl[0], sl[0], wl[1], wl[2], p->bed, res, p->cell, p->origin[0], p->origin[1]};
+    for (size_t v = 0; v < groups; v++) {
+      if (!regions[v].nx) continue;
+      float* ctx = p->sur_ctx + v * kSurCtxFloats;
+      sur_context(g, regions[v], lists[v], ctx, st);
+      sur_gather_region(ctx, regions[v], p->net.xb + offset[v] * kMlpIn, st);
+    }
+    mlp_forward(p->net, cells);
+  }
+  // The pushes shove both layers around each vehicle (push_on, src/fluid_core.rs:8152-8239).
+  const GridGeo geo{res, p->cell, p->origin[0], p->origin[1]};
+  for (size_t v = 0; v < groups; v++) {
+    float x0 = 1e30f, x1 = -1e30f, z0 = 1e30f, z1 = -1e30f;
+    for (int k = 0; k < lists[v].n; k++) {
+      const CorePush& q = lists[v].p[k];
+      const float reach = q.radius + q.width + 0.5f;
+      x0 = fminf(x0, q.x - reach);
+      x1 = fmaxf(x1, q.x + reach);
+      z0 = fminf(z0, q.z - reach);
+      z1 = fmaxf(z1, q.z + reach);
+    }
+    int i0 = std::max(0, (int)floorf((x0 - p->origin[0]) / p->cell));
+    int j0 = std::max(0, (int)floorf((z0 - p->origin[1]) / p->cell));
+    int i1 = std::min(res, (int)ceilf((x1 - p->origin[0]) / p->cell) + 1);
+    int j1 = std::min(res, (int)ceilf((z1 - p->origin[1]) / p->cell) + 1);
+    if (i1 <= i0 || j1 <= j0) continue;
+    core_push_on(geo, wl[0], sl[0], wl[1], wl[2], sl[1], sl[2], lists[v], i0, j0, i1 - i0, j1 - j0, st);
+  }
+  if (timed) HIPCHK(hipEventRecord(p->tev[1], st));
+  // Fresh statistics for the plan (the shove changed speeds, the last
+  // fluxes depths), then the step.
+  if (groups) {
+    stats_compute(wl[0], wl[1], wl[2], res, kEps, p->stats, st);
+    stats_compute(sl[0], sl[1], sl[2], res, kEps, p->stats + 1, st);
+  }
+  int nsub = silt_pair_step(p, dt);
+  if (nsub < 0) return nsub;
+  if (timed) HIPCHK(hipEventRecord(p->tev[2], st));
+  if (p->net_ready && cells) {
+    float* wh = p->planes[0][p->cur[0]][0];
+    float* sh = p->planes[1][p->cur[1]][0];
+    for (size_t v = 0; v < groups; v++) {
+      if (!regions[v].nx) continue;
+      sur_apply(wh, sh, res, p->net.y + offset[v] * kMlpOut, regions[v], p->sur_fac + v * 2 * kSurMax * kSurMax, st);
+    }
+  }
+  if (timed) {
+    HIPCHK(hipEventRecord(p->tev[3], st));
+    HIPCHK(hipEventSynchronize(p->tev[3]));
+    float a = 0.0f, b = 0.0f;
+    HIPCHK(hipEventElapsedTime(&a, p->tev[0], p->tev[1]));
+    HIPCHK(hipEventElapsedTime(&b, p->tev[2], p->tev[3]));
+    p->last_tire_ms = a + b;
+  }
+  return nsub;
+}
 
 }  // extern "C"

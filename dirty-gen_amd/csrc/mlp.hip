@@ -9,6 +9,10 @@
 //   EPI_PLAIN     C (+ bias)                          -> FP32
 //   EPI_HIDDEN    z = C + bias -> FP32, gelu(z)       -> BF16
 //   EPI_GELU_BWD  dz = C * gelu'(z)                   -> FP32 and BF16
+//   EPI_PARTIAL   one K slice of C (split K)          -> FP32 slice, summed
+//                 in order by the next kernel
+// A product with few output tiles (the first layer at inference sizes, the
+// first layer's weight gradient) splits K over gridDim.z to fill the chip.
 //
 // Forward (n samples):  z1 = x W1^T + b1, a1 = gelu(z1); z2 = a1 W2^T + b2,
 // a2 = gelu(z2); y = a2 W3^T + b3.  Backward of L = sum (y - t)^2 / (8 n):
@@ -29,7 +33,7 @@ namespace {
 typedef short v4s __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 
-enum { EPI_PLAIN = 0, EPI_HIDDEN = 1, EPI_GELU_BWD = 2 };
+enum { EPI_PLAIN = 0, EPI_HIDDEN = 1, EPI_GELU_BWD = 2, EPI_PARTIAL = 3 };
 
 struct Gemm {
   int M, N, K;
@@ -44,6 +48,7 @@ struct Gemm {
   int ldob;
   const float* z;
   int ldz;
+  int kchunk;  // K per workgroup along gridDim.z (split K, EPI_PARTIAL); 0: all of K
 };
 
 __device__ inline float gelu(float x) { return 0.5f * x * (1.0f + erff(x * 0.70710678118654752f)); }
@@ -69,35 +74,41 @@ __global__ __launch_bounds__(256) void k_gemm(Gemm g) {
   v4f acc[2][2];
   for (int a = 0; a < 2; a++)
     for (int b = 0; b < 2; b++) acc[a][b] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
-  for (int k0 = 0; k0 < g.K; k0 += kKs) {
-    // Stage the A and B tiles, reading along each operand's contiguous axis.
-    for (int e = tid; e < kT * kKs; e += 256) {
-      int r, c;
-      if (AT) {
-        c = e / kT;
-        r = e % kT;
-      } else {
-        r = e / kKs;
-        c = e % kKs;
-      }
+  const int kb = g.kchunk ? (int)blockIdx.z * g.kchunk : 0;
+  const int ke = g.kchunk ? min(g.K, kb + g.kchunk) : g.K;
+  for (int k0 = kb; k0 < ke; k0 += kKs) {
+    // Stage the A and B tiles, reading along each operand's contiguous axis:
+    // a k-contiguous operand moves 8 values (16 B) per lane when the row is
+    // aligned and the chunk lies inside K.
+    if (!AT) {
+      const int r = tid >> 2, c = (tid & 3) * 8;
       const int m = m0 + r, k = k0 + c;
-      uint16_t v = 0;
-      if (m < g.M && k < g.K) v = AT ? g.A[(size_t)k * g.lda + m] : g.A[(size_t)m * g.lda + k];
-      As[r][c] = v;
-    }
-    for (int e = tid; e < kT * kKs; e += 256) {
-      int r, c;
-      if (BT) {
-        c = e / kT;
-        r = e % kT;
+      if (m < g.M && k + 8 <= ke && (g.lda & 7) == 0) {
+        *reinterpret_cast<uint4*>(&As[r][c]) = *reinterpret_cast<const uint4*>(g.A + (size_t)m * g.lda + k);
       } else {
-        r = e / kKs;
-        c = e % kKs;
+        for (int q = 0; q < 8; q++) As[r][c + q] = (m < g.M && k + q < ke) ? g.A[(size_t)m * g.lda + k + q] : 0;
       }
+    } else {
+      for (int e = tid; e < kT * kKs; e += 256) {
+        const int c = e / kT, r = e % kT;
+        const int m = m0 + r, k = k0 + c;
+        As[r][c] = (m < g.M && k < ke) ? g.A[(size_t)k * g.lda + m] : (uint16_t)0;
+      }
+    }
+    if (!BT) {
+      const int r = tid >> 2, c = (tid & 3) * 8;
       const int n = n0 + r, k = k0 + c;
-      uint16_t v = 0;
-      if (n < g.N && k < g.K) v = BT ? g.B[(size_t)k * g.ldb + n] : g.B[(size_t)n * g.ldb + k];
-      Bs[r][c] = v;
+      if (n < g.N && k + 8 <= ke && (g.ldb & 7) == 0) {
+        *reinterpret_cast<uint4*>(&Bs[r][c]) = *reinterpret_cast<const uint4*>(g.B + (size_t)n * g.ldb + k);
+      } else {
+        for (int q = 0; q < 8; q++) Bs[r][c + q] = (n < g.N && k + q < ke) ? g.B[(size_t)n * g.ldb + k + q] : 0;
+      }
+    } else {
+      for (int e = tid; e < kT * kKs; e += 256) {
+        const int c = e / kT, r = e % kT;
+        const int n = n0 + r, k = k0 + c;
+        Bs[r][c] = (n < g.N && k < ke) ? g.B[(size_t)k * g.ldb + n] : (uint16_t)0;
+      }
     }
     __syncthreads();
 #pragma unroll
@@ -129,7 +140,11 @@ __global__ __launch_bounds__(256) void k_gemm(Gemm g) {
         const int n = n0 + wn * 32 + tj * 16 + (lane & 15);
         if (m >= g.M || n >= g.N) continue;
         float v = acc[ti][tj][i];
-        if (EPI == EPI_PLAIN) {
+        if (EPI == EPI_PARTIAL) {
+          // Slice blockIdx.z of a [splits][M][ldo] buffer, summed in a fixed
+          // order afterwards (deterministic, unlike atomics).
+          g.out[((size_t)blockIdx.z * g.M + m) * g.ldo + n] = v;
+        } else if (EPI == EPI_PLAIN) {
           if (g.bias) v += g.bias[n];
           g.out[(size_t)m * g.ldo + n] = v;
         } else if (EPI == EPI_HIDDEN) {
@@ -146,8 +161,44 @@ __global__ __launch_bounds__(256) void k_gemm(Gemm g) {
 
 template <bool AT, bool BT, int EPI>
 void gemm(const Gemm& g, hipStream_t st) {
-  dim3 grid((unsigned)((g.N + kT - 1) / kT), (unsigned)((g.M + kT - 1) / kT));
+  const unsigned splits = g.kchunk ? (unsigned)((g.K + g.kchunk - 1) / g.kchunk) : 1u;
+  dim3 grid((unsigned)((g.N + kT - 1) / kT), (unsigned)((g.M + kT - 1) / kT), splits);
   hipLaunchKernelGGL((k_gemm<AT, BT, EPI>), grid, dim3(256), 0, st, g);
+}
+
+// Sum of the split-K slices in order, plus bias, into z (FP32) and gelu(z)
+// (BF16): the epilogue of a split hidden layer.
+__global__ void k_split_bias_gelu(const float* __restrict__ part, int splits, int M, int N,
+                                  const float* __restrict__ bias, float* __restrict__ z, uint16_t* __restrict__ a) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t cnt = (size_t)M * N;
+  if (i >= cnt) return;
+  float v = 0.0f;
+  for (int s = 0; s < splits; s++) v += part[(size_t)s * cnt + i];
+  v += bias[i % N];
+  z[i] = v;
+  a[i] = f2bf(gelu(v));
+}
+
+// Sum of the split-K slices in order into out.
+__global__ void k_split_sum(const float* __restrict__ part, int splits, size_t cnt, float* __restrict__ out) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= cnt) return;
+  float v = 0.0f;
+  for (int s = 0; s < splits; s++) v += part[(size_t)s * cnt + i];
+  out[i] = v;
+}
+
+// Split K so a product with few output tiles still fills the chip: about 256
+// workgroups, at most kMaxSplits slices of at least 64.
+int split_chunk(int M, int N, int K) {
+  const int tiles = ((M + kT - 1) / kT) * ((N + kT - 1) / kT);
+  int splits = 256 / (tiles > 0 ? tiles : 1);
+  if (splits > kMlpMaxSplits) splits = kMlpMaxSplits;
+  if (splits < 2 || K < 128) return 0;
+  int chunk = (K + splits - 1) / splits;
+  chunk = ((chunk + kKs - 1) / kKs) * kKs;
+  return chunk < 64 ? 64 : chunk;
 }
 
 __global__ void k_to_bf16(const float* __restrict__ x, uint16_t* __restrict__ y, size_t n) {
@@ -207,7 +258,7 @@ bool dalloc(T** p, size_t n) {
 
 int mlp_reserve(Mlp& m, size_t cap) {
   if (cap <= m.cap) return 0;
-  void* old[] = {m.xb, m.z1, m.a1, m.z2, m.a2, m.y, m.tg, m.dyf, m.dy, m.dz2f, m.dz2, m.dz1f, m.dz1};
+  void* old[] = {m.xb, m.z1, m.a1, m.z2, m.a2, m.y, m.tg, m.dyf, m.dy, m.dz2f, m.dz2, m.dz1f, m.dz1, m.part};
   if (m.st) (void)hipStreamSynchronize(m.st);
   for (void* p : old)
     if (p) (void)hipFree(p);
@@ -216,7 +267,8 @@ int mlp_reserve(Mlp& m, size_t cap) {
   bool ok = dalloc(&m.xb, cap * kMlpIn) && dalloc(&m.z1, cap * H) && dalloc(&m.a1, cap * H) &&
             dalloc(&m.z2, cap * H) && dalloc(&m.a2, cap * H) && dalloc(&m.y, cap * O) && dalloc(&m.tg, cap * O) &&
             dalloc(&m.dyf, cap * O) && dalloc(&m.dy, cap * O) && dalloc(&m.dz2f, cap * H) &&
-            dalloc(&m.dz2, cap * H) && dalloc(&m.dz1f, cap * H) && dalloc(&m.dz1, cap * H);
+            dalloc(&m.dz2, cap * H) && dalloc(&m.dz1f, cap * H) && dalloc(&m.dz1, cap * H) &&
+            dalloc(&m.part, (size_t)kMlpMaxSplits * (cap > (size_t)kMlpIn ? cap : (size_t)kMlpIn) * H);
   if (!ok) return SILT_ENOMEM;
   m.cap = cap;
   return 0;
@@ -257,7 +309,7 @@ int mlp_init(Mlp& m, size_t cap, unsigned seed, hipStream_t st) {
 void mlp_free(Mlp& m) {
   if (m.st) (void)hipStreamSynchronize(m.st);
   void* ps[] = {m.master, m.wb, m.grad, m.m1, m.m2, m.loss, m.xb, m.z1, m.a1, m.z2, m.a2,
-                m.y, m.tg, m.dyf, m.dy, m.dz2f, m.dz2, m.dz1f, m.dz1};
+                m.y, m.tg, m.dyf, m.dy, m.dz2f, m.dz2, m.dz1f, m.dz1, m.part};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   hipStream_t st = m.st;
@@ -278,11 +330,21 @@ void mlp_load_x(Mlp& m, const float* x, size_t n) {
 void mlp_forward(Mlp& m, size_t n) {
   const int N = (int)n, H = kMlpHidden;
   Gemm g{};
-  g = Gemm{N, H, kMlpIn, m.xb, kMlpIn, m.wb + kOffW1, kMlpIn, m.master + kOffB1, m.z1, H, m.a1, H, nullptr, 0};
+  const int c1 = split_chunk(N, H, kMlpIn);
+  if (c1) {
+    const int splits = (kMlpIn + c1 - 1) / c1;
+    g = Gemm{N, H, kMlpIn, m.xb, kMlpIn, m.wb + kOffW1, kMlpIn, nullptr, m.part, H, nullptr, 0, nullptr, 0, c1};
+    gemm<false, false, EPI_PARTIAL>(g, m.st);
+    const size_t cnt = (size_t)N * H;
+    hipLaunchKernelGGL(k_split_bias_gelu, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, m.st, m.part, splits, N,
+                       H, m.master + kOffB1, m.z1, m.a1);
+  } else {
+    g = Gemm{N, H, kMlpIn, m.xb, kMlpIn, m.wb + kOffW1, kMlpIn, m.master + kOffB1, m.z1, H, m.a1, H, nullptr, 0, 0};
+    gemm<false, false, EPI_HIDDEN>(g, m.st);
+  }
+  g = Gemm{N, H, H, m.a1, H, m.wb + kOffW2, H, m.master + kOffB2, m.z2, H, m.a2, H, nullptr, 0, 0};
   gemm<false, false, EPI_HIDDEN>(g, m.st);
-  g = Gemm{N, H, H, m.a1, H, m.wb + kOffW2, H, m.master + kOffB2, m.z2, H, m.a2, H, nullptr, 0};
-  gemm<false, false, EPI_HIDDEN>(g, m.st);
-  g = Gemm{N, kMlpOut, H, m.a2, H, m.wb + kOffW3, H, m.master + kOffB3, m.y, kMlpOut, nullptr, 0, nullptr, 0};
+  g = Gemm{N, kMlpOut, H, m.a2, H, m.wb + kOffW3, H, m.master + kOffB3, m.y, kMlpOut, nullptr, 0, nullptr, 0, 0};
   gemm<false, false, EPI_PLAIN>(g, m.st);
 }
 
@@ -294,22 +356,32 @@ void mlp_backward(Mlp& m, size_t n) {
                      1.0f / (float)cnt, m.dyf, m.dy, m.loss);
   Gemm g{};
   // dW3 = dy^T a2  (8 x 128, K = n)
-  g = Gemm{O, H, N, m.dy, O, m.a2, H, nullptr, m.grad + kOffW3, H, nullptr, 0, nullptr, 0};
+  g = Gemm{O, H, N, m.dy, O, m.a2, H, nullptr, m.grad + kOffW3, H, nullptr, 0, nullptr, 0, 0};
   gemm<true, true, EPI_PLAIN>(g, m.st);
   hipLaunchKernelGGL(k_colsum, dim3(1), dim3(64), 0, m.st, m.dyf, N, O, O, m.grad + kOffB3);
   // dz2 = (dy W3) * gelu'(z2)  (n x 128, K = 8)
-  g = Gemm{N, H, O, m.dy, O, m.wb + kOffW3, H, nullptr, m.dz2f, H, m.dz2, H, m.z2, H};
+  g = Gemm{N, H, O, m.dy, O, m.wb + kOffW3, H, nullptr, m.dz2f, H, m.dz2, H, m.z2, H, 0};
   gemm<false, true, EPI_GELU_BWD>(g, m.st);
   // dW2 = dz2^T a1
-  g = Gemm{H, H, N, m.dz2, H, m.a1, H, nullptr, m.grad + kOffW2, H, nullptr, 0, nullptr, 0};
+  g = Gemm{H, H, N, m.dz2, H, m.a1, H, nullptr, m.grad + kOffW2, H, nullptr, 0, nullptr, 0, 0};
   gemm<true, true, EPI_PLAIN>(g, m.st);
   hipLaunchKernelGGL(k_colsum, dim3(2), dim3(64), 0, m.st, m.dz2f, N, H, H, m.grad + kOffB2);
   // dz1 = (dz2 W2) * gelu'(z1)
-  g = Gemm{N, H, H, m.dz2, H, m.wb + kOffW2, H, nullptr, m.dz1f, H, m.dz1, H, m.z1, H};
+  g = Gemm{N, H, H, m.dz2, H, m.wb + kOffW2, H, nullptr, m.dz1f, H, m.dz1, H, m.z1, H, 0};
   gemm<false, true, EPI_GELU_BWD>(g, m.st);
   // dW1 = dz1^T x  (128 x 1536)
-  g = Gemm{H, kMlpIn, N, m.dz1, H, m.xb, kMlpIn, nullptr, m.grad + kOffW1, kMlpIn, nullptr, 0, nullptr, 0};
-  gemm<true, true, EPI_PLAIN>(g, m.st);
+  const int cw = split_chunk(H, kMlpIn, N);
+  if (cw) {
+    const int splits = (N + cw - 1) / cw;
+    g = Gemm{H, kMlpIn, N, m.dz1, H, m.xb, kMlpIn, nullptr, m.part, kMlpIn, nullptr, 0, nullptr, 0, cw};
+    gemm<true, true, EPI_PARTIAL>(g, m.st);
+    const size_t cnt = (size_t)H * kMlpIn;
+    hipLaunchKernelGGL(k_split_sum, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, m.st, m.part, splits, cnt,
+                       m.grad + kOffW1);
+  } else {
+    g = Gemm{H, kMlpIn, N, m.dz1, H, m.xb, kMlpIn, nullptr, m.grad + kOffW1, kMlpIn, nullptr, 0, nullptr, 0, 0};
+    gemm<true, true, EPI_PLAIN>(g, m.st);
+  }
   hipLaunchKernelGGL(k_colsum, dim3(2), dim3(64), 0, m.st, m.dz1f, N, H, H, m.grad + kOffB1);
 }
 

@@ -23,6 +23,7 @@ constexpr size_t kOffB2 = kOffW2 + (size_t)kMlpHidden * kMlpHidden;
 constexpr size_t kOffW3 = kOffB2 + kMlpHidden;
 constexpr size_t kOffB3 = kOffW3 + (size_t)kMlpOut * kMlpHidden;
 constexpr size_t kMlpParams = kOffB3 + kMlpOut;
+constexpr int kMlpMaxSplits = 16;  // split-K slices of one product
 
 struct Mlp {
   float* master = nullptr;  // [kMlpParams]
@@ -46,6 +47,7 @@ struct Mlp {
   uint16_t* dz2 = nullptr;
   float* dz1f = nullptr;
   uint16_t* dz1 = nullptr;
+  float* part = nullptr;   // split-K slices [kMlpMaxSplits][max(cap, 1536)][128]
   double* loss = nullptr;  // [1]
   hipStream_t st = nullptr;
 };

@@ -328,6 +328,44 @@ class SheetPair:
     def last_fallback(self):
         return lib().silt_pair_last_fallback(self._h)
 
+    def _tires_declared(self):
+        L = lib()
+        if not getattr(L, "_pair_tires_declared", False):
+            L.silt_pair_set_ground.argtypes = [_P, _P, _F, _F]
+            L.silt_pair_get_ground.argtypes = [_P, _P, _P, _P]
+            L.silt_pair_set_surrogate.argtypes = [_P, _P]
+            L.silt_pair_step_tires.argtypes = [_P, _F, C.POINTER(TirePush), _Z, _Z, C.POINTER(TireContact), _Z]
+            L.silt_pair_last_tire_ms.argtypes = [_P]
+            L.silt_pair_last_tire_ms.restype = _F
+            L._pair_tires_declared = True
+        return L
+
+    def set_ground(self, bed, origin):
+        b = np.ascontiguousarray(bed, np.float32)
+        _check_status(self._tires_declared().silt_pair_set_ground(self._h, _ptr(b), origin[0], origin[1]))
+
+    def ground(self):
+        n = self.res * self.res
+        out = [np.empty(n, np.float32) for _ in range(3)]
+        _check_status(self._tires_declared().silt_pair_get_ground(self._h, *(_ptr(a) for a in out)))
+        return out
+
+    def set_surrogate(self, params):
+        p = None if params is None else np.ascontiguousarray(params, np.float32)
+        _check_status(self._tires_declared().silt_pair_set_surrogate(self._h, None if p is None else _ptr(p)))
+
+    def step_tires(self, dt, pushes, per_vehicle=4, contacts=()):
+        """One step with vehicles (include/silt_pair.h): pushes are dicts of
+        push_tire's fields, per_vehicle consecutive ones per vehicle;
+        contacts are dicts of TireContact's fields."""
+        L = self._tires_declared()
+        parr = (TirePush * max(len(pushes), 1))(*[TirePush(**q) for q in pushes])
+        carr = (TireContact * max(len(contacts), 1))(*[TireContact(**c) for c in contacts])
+        return _check_status(L.silt_pair_step_tires(self._h, dt, parr, len(pushes), per_vehicle, carr, len(contacts)))
+
+    def last_tire_ms(self):
+        return self._tires_declared().silt_pair_last_tire_ms(self._h)
+
     def autotune(self, dt, rows=None, reps=5):
         """Time candidate rows per workgroup of the fused kernel on this
         handle's state (restored afterwards) and pin the fastest for this
@@ -383,6 +421,64 @@ def _declare_core(L):
     L.silt_core_set_ground.restype = C.c_int
     L.silt_core_get_ground.argtypes = [_P, _P, _P, _P]
     L.silt_core_get_ground.restype = C.c_int
+    L.silt_core_map_water_velocity.argtypes = [_P, _P, _P]
+    L.silt_core_non_finite_count.argtypes = [_P]
+    L.silt_core_non_finite_count.restype = C.c_long
+    L.silt_core_set_center.argtypes = [_P, _F, _F]
+    L.silt_core_center.argtypes = [_P, _P]
+    L.silt_core_coarse_cell.argtypes = [_P, _Z, _Z, C.POINTER(CoarseCell)]
+    L.silt_core_sample.argtypes = [_P, _F, _F, C.POINTER(GroundSample)]
+    L.silt_core_ledger_full.argtypes = [_P, C.POINTER(FluidLedgerFull)]
+    L.silt_core_recenter.argtypes = [_P, _F, _F]
+    L.silt_core_window_bounds.argtypes = [_P, _P]
+    L.silt_core_lay_slurry_on_water.argtypes = [_P, _F, C.c_int]
+    L.silt_core_lay_slurry_on_water.restype = _F
+    L.silt_core_parked_cells.argtypes = [_P]
+    L.silt_core_parked_cells.restype = C.c_long
+    L.silt_core_drain_over_bed.argtypes = [_P, _F, C.c_uint]
+    L.silt_core_drain_over_bed.restype = _F
+    L.silt_core_set_tire_model.argtypes = [_P, C.c_int]
+    L.silt_core_set_surrogate.argtypes = [_P, _P, C.c_int, _F]
+    L.silt_core_surrogate_stats.argtypes = [_P, C.POINTER(SurrogateStats)]
+    L.silt_trainer_create.argtypes = [_Z, C.c_uint]
+    L.silt_trainer_create.restype = _P
+    L.silt_trainer_destroy.argtypes = [_P]
+    L.silt_trainer_destroy.restype = None
+    L.silt_trainer_record.argtypes = [_P, _P, _F, C.POINTER(SprayEvent), _Z, C.POINTER(C.c_size_t)]
+    for n in ("silt_trainer_steps", "silt_trainer_samples"):
+        getattr(L, n).argtypes = [_P]
+        getattr(L, n).restype = C.c_long
+    L.silt_trainer_fit.argtypes = [_P, _Z, C.c_int, C.c_int, _F, C.POINTER(C.c_double)]
+    L.silt_trainer_eval.argtypes = [_P, _Z, _Z, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.silt_trainer_params.argtypes = [_P, _P]
+    L.silt_trainer_set_params.argtypes = [_P, _P]
+    L.silt_surrogate_targets_debug.argtypes = [_P, _P, _P, _P] + [C.c_int] * 4 + [_P]
+    L.silt_surrogate_apply_debug.argtypes = [_P, _P, _P] + [C.c_int] * 4
+
+
+TIRE_MODEL = {"nested": 0, "coarse": 1, "surrogate": 2}
+
+
+class CoarseCell(C.Structure):
+    _fields_ = [(n, C.c_float) for n in ("x", "z", "bed_m", "water_m", "slurry_m", "moisture", "u", "v",
+                                         "settled_dust_m", "turbidity")]
+
+
+class GroundSample(C.Structure):
+    _fields_ = [(n, C.c_float) for n in ("water_m", "slurry_m", "u", "v", "settled_dust_m", "moisture", "bed_m")]
+
+
+class FluidLedgerFull(C.Structure):
+    _fields_ = [(n, C.c_double) for n in (
+        "water_m3", "slurry_m3", "dust_air_m3", "dust_settled_m3", "frozen_water_m3", "frozen_slurry_m3",
+        "frozen_dust_m3", "spray_m3", "fog_air_m3", "suspended_m3", "settled_suspended_m3", "frozen_suspended_m3",
+        "entrained_m3", "reach_m3", "total_m3")]
+
+
+class SurrogateStats(C.Structure):
+    _fields_ = [("last_tire_ms", C.c_float), ("last_judge_err_m", C.c_float), ("last_judge_rms_m", C.c_float),
+                ("judged", C.c_int),
+                ("fallbacks", C.c_int), ("last_mode", C.c_int)]
 
 
 class FluidCore:
@@ -484,6 +580,78 @@ class FluidCore:
 
     def near_tiles(self):
         return int(_check_status(lib().silt_core_near_tiles(self._c)))
+
+    def map_water_velocity(self, u, v):
+        """map_water_velocity (src/fluid_core.rs:5358-5372) from per-cell arrays."""
+        uu, vv = (np.ascontiguousarray(a, np.float32) for a in (u, v))
+        _check_status(lib().silt_core_map_water_velocity(self._c, _ptr(uu), _ptr(vv)))
+
+    def non_finite_count(self):
+        return int(_check_status(lib().silt_core_non_finite_count(self._c)))
+
+    def set_center(self, x, z):
+        _check_status(lib().silt_core_set_center(self._c, x, z))
+
+    def center(self):
+        out = np.zeros(2, np.float32)
+        _check_status(lib().silt_core_center(self._c, _ptr(out)))
+        return out
+
+    def coarse_cell(self, i, j):
+        cc = CoarseCell()
+        if _check_status(lib().silt_core_coarse_cell(self._c, i, j, C.byref(cc))) == 0:
+            return None
+        return {k: getattr(cc, k) for k, _ in CoarseCell._fields_}
+
+    def sample(self, x, z):
+        gs = GroundSample()
+        _check_status(lib().silt_core_sample(self._c, x, z, C.byref(gs)))
+        return {k: getattr(gs, k) for k, _ in GroundSample._fields_}
+
+    def ledger_full(self):
+        led = FluidLedgerFull()
+        _check_status(lib().silt_core_ledger_full(self._c, C.byref(led)))
+        return {k: getattr(led, k) for k, _ in FluidLedgerFull._fields_}
+
+    def window_bounds(self):
+        out = np.zeros(3, np.float32)
+        _check_status(lib().silt_core_window_bounds(self._c, _ptr(out)))
+        return tuple(float(v) for v in out)
+
+    def lay_slurry_on_water(self, depth, grain):
+        v = lib().silt_core_lay_slurry_on_water(self._c, depth, GRAIN[grain])
+        if v != v:
+            raise SiltError("silt_core_lay_slurry_on_water failed")
+        return float(v)
+
+    def recenter(self, x, z):
+        """recenter (src/fluid_core.rs:5409-5422): move the window."""
+        _check_status(lib().silt_core_recenter(self._c, x, z))
+
+    def parked_cells(self):
+        return int(_check_status(lib().silt_core_parked_cells(self._c)))
+
+    def drain_over_bed(self, dt, steps):
+        v = lib().silt_core_drain_over_bed(self._c, dt, int(steps))
+        if v != v:
+            raise SiltError("silt_core_drain_over_bed failed")
+        return float(v)
+
+    def set_tire_model(self, model):
+        """"nested" (the reference's 2 cm patch), "coarse" (no patch) or
+        "surrogate" (coarse + the tire sub-grid network)."""
+        _check_status(lib().silt_core_set_tire_model(self._c, TIRE_MODEL[model]))
+
+    def set_surrogate(self, params, judge_every=30, tol=1.0e-3):
+        p = np.ascontiguousarray(params, np.float32)
+        if p.size != MLP_PARAMS:
+            raise ValueError("parameter vector size")
+        _check_status(lib().silt_core_set_surrogate(self._c, _ptr(p), int(judge_every), tol))
+
+    def surrogate_stats(self):
+        st = SurrogateStats()
+        _check_status(lib().silt_core_surrogate_stats(self._c, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in SurrogateStats._fields_}
 
     def fine_water_debug(self, pushes):
         """fine_water_debug (src/fluid_core/near.rs:341-346): the 2 cm water,
@@ -616,3 +784,88 @@ class Mlp:
         sse = C.c_double()
         _check_status(lib().silt_mlp_train_batch(self._h, _ptr(x), _ptr(t), x.shape[0], lr, C.byref(sse)))
         return sse.value
+
+
+class Trainer:
+    """Training set and trainer of the tire sub-grid network
+    (include/silt_mlp.h): records nested steps of a FluidCore beside a
+    coarse-only twin and fits the network on the GPU."""
+
+    def __init__(self, max_steps=4096, seed=1):
+        L = lib()
+        if not getattr(L, "_core_declared", False):
+            _declare_core(L)
+            L._core_declared = True
+        self._t = L.silt_trainer_create(int(max_steps), int(seed))
+        if not self._t:
+            raise SiltError("silt_trainer_create failed")
+
+    def close(self):
+        if getattr(self, "_t", None):
+            lib().silt_trainer_destroy(self._t)
+            self._t = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def record(self, core, dt, max_events=4096):
+        ev = (SprayEvent * max_events)()
+        n = C.c_size_t(0)
+        return int(_check_status(lib().silt_trainer_record(self._t, core._c, dt, ev, max_events, C.byref(n))))
+
+    def steps(self):
+        return int(_check_status(lib().silt_trainer_steps(self._t)))
+
+    def samples(self):
+        return int(_check_status(lib().silt_trainer_samples(self._t)))
+
+    def fit(self, iterations, batch=512, lr=1e-3, train_steps=0):
+        mse = C.c_double()
+        _check_status(lib().silt_trainer_fit(self._t, int(train_steps), int(iterations), int(batch), lr,
+                                             C.byref(mse)))
+        return mse.value
+
+    def evaluate(self, first, count):
+        a, b = C.c_double(), C.c_double()
+        _check_status(lib().silt_trainer_eval(self._t, int(first), int(count), C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def params(self):
+        out = np.empty(MLP_PARAMS, np.float32)
+        _check_status(lib().silt_trainer_params(self._t, _ptr(out)))
+        return out
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        _check_status(lib().silt_trainer_set_params(self._t, _ptr(p)))
+
+
+def surrogate_targets(hn_w, hc_w, hn_s, hc_s, region):
+    """Training targets of a nested / coarse-only pair of depth planes over a
+    region (i0, j0, nx, nz): [cell][E, W, N, S water, E, W, N, S slurry] mm."""
+    L = lib()
+    if not getattr(L, "_core_declared", False):
+        _declare_core(L)
+        L._core_declared = True
+    arrs = [np.ascontiguousarray(a, np.float32) for a in (hn_w, hc_w, hn_s, hc_s)]
+    i0, j0, nx, nz = region
+    out = np.empty((nx * nz, 8), np.float32)
+    _check_status(L.silt_surrogate_targets_debug(*(_ptr(a) for a in arrs), i0, j0, nx, nz, _ptr(out)))
+    return out
+
+
+def surrogate_apply(water, slurry, y, region):
+    """Apply predicted edge fluxes y ([cell][8] mm) over a region; returns the
+    new water and slurry planes."""
+    L = lib()
+    if not getattr(L, "_core_declared", False):
+        _declare_core(L)
+        L._core_declared = True
+    w, s_ = (np.array(a, np.float32, copy=True) for a in (water, slurry))
+    yy = np.ascontiguousarray(y, np.float32)
+    i0, j0, nx, nz = region
+    _check_status(L.silt_surrogate_apply_debug(_ptr(w), _ptr(s_), _ptr(yy), i0, j0, nx, nz))
+    return w, s_

@@ -44,6 +44,26 @@ typedef struct silt_fluid_ledger {
     double water_m3, slurry_m3, frozen_water_m3, frozen_slurry_m3, spray_m3, total_m3;
 } silt_fluid_ledger;
 
+/* FluidLedger (src/fluid_core.rs:483-509) with all 15 fields, FP64.  The
+ * dust, fog, suspended-solid and river-reach lines belong to subsystems this
+ * library does not build (the air grid, tracers, walled reaches) and are 0. */
+typedef struct silt_fluid_ledger_full {
+    double water_m3, slurry_m3, dust_air_m3, dust_settled_m3, frozen_water_m3, frozen_slurry_m3, frozen_dust_m3,
+        spray_m3, fog_air_m3, suspended_m3, settled_suspended_m3, frozen_suspended_m3, entrained_m3, reach_m3,
+        total_m3;
+} silt_fluid_ledger_full;
+
+/* CoarseCell (src/fluid_core.rs:450-463); settled dust and turbidity are 0
+ * (no air grid, no tracers here). */
+typedef struct silt_coarse_cell {
+    float x, z, bed_m, water_m, slurry_m, moisture, u, v, settled_dust_m, turbidity;
+} silt_coarse_cell;
+
+/* GroundSample (src/fluid_core.rs:440-448). */
+typedef struct silt_ground_sample {
+    float water_m, slurry_m, u, v, settled_dust_m, moisture, bed_m;
+} silt_ground_sample;
+
 /* FluidCore::new(center) (src/fluid_core.rs:4958-5060).  path: SILT_PATH_*
  * for the sheet steps.  NULL on failure. */
 silt_core *silt_core_create(float center_x, float center_z, int path);
@@ -114,8 +134,79 @@ int silt_core_get_ground(silt_core *c, float *bed, float *placed, float *rut);
 int silt_core_fine_water_debug(silt_core *c, const silt_tire_push *pushes, size_t n, float *water, float *uw,
                                float *vw, float *origin2);
 
+/* map_water_velocity (src/fluid_core.rs:5358-5372): water (u, v) per cell
+ * where the water is deeper than 1e-6 m; non-finite values become 0. */
+int silt_core_map_water_velocity(silt_core *c, const float *u, const float *v);
+/* non_finite_count (src/fluid_core.rs:5374-5395) over the planes this
+ * library holds (water, slurry, their velocities, bed, moisture). */
+long silt_core_non_finite_count(silt_core *c);
+/* recenter (src/fluid_core.rs:5409-5422, shift_coarse :9004-9226): move the
+ * window to centre (x, z) snapped to the coarse grid.  Cells that slide off
+ * are parked by world cell (water, slurry, velocities, moisture; when they
+ * hold anything), cells that slide on are restored from the parked set or
+ * start dry at the ambient moisture; bed and grain stay on their array index
+ * (read the substrate again).  The ledger's frozen lines include the parked
+ * volume.  silt_core_parked_cells: cells in the parked set. */
+int silt_core_recenter(silt_core *c, float x, float z);
+long silt_core_parked_cells(const silt_core *c);
+/* window_bounds (src/fluid_core.rs:6977-6980): out3 = corner x, z, extent. */
+int silt_core_window_bounds(const silt_core *c, float *out3);
+/* lay_slurry_on_water (src/fluid_core.rs:5236-5254): slurry `depth` deep,
+ * of grain (0 clay .. 4 other), on every cell with more than 12 mm of water;
+ * moisture raised to 0.9.  Returns the volume laid, m^3 (f32 sum). */
+float silt_core_lay_slurry_on_water(silt_core *c, float depth, int grain);
+/* set_center (src/fluid_core.rs:5403-5406): the chase point, the window does
+ * not move; silt_core_center reads it back. */
+int silt_core_set_center(silt_core *c, float x, float z);
+int silt_core_center(const silt_core *c, float *out2);
+/* coarse_cell (src/fluid_core.rs:6998-7034): 1 and *out set, or 0 off the
+ * window (the reference's None). */
+int silt_core_coarse_cell(silt_core *c, size_t i, size_t j, silt_coarse_cell *out);
+/* sample (src/fluid_core.rs:6806-6813, sample_coarse :9428-9454): the coarse
+ * cell under (x, z) (its slurry velocity, as the reference returns); zeros
+ * off the window (no parked cells here). */
+int silt_core_sample(silt_core *c, float x, float z, silt_ground_sample *out);
+/* ledger() with every FluidLedger field. */
+int silt_core_ledger_full(silt_core *c, silt_fluid_ledger_full *out);
+/* drain_over_bed (src/fluid_core.rs:5123-5214): up to `steps` gravity steps
+ * of both coarse layers (no confinement, dry banks, pace 2), velocities damped
+ * by exp(-0.55 dt), volume wiped by the dry cut put back on the deepest cell,
+ * stopping 24 calm steps after the sheet has moved.  Returns the water volume
+ * in m^3 (NaN on a device error); volumes are the reference's f32 sums. */
+float silt_core_drain_over_bed(silt_core *c, float dt, unsigned steps);
+
 /* Near-field residual tiles alive (FluidCore::near_tiles, near.rs:316-318). */
 long silt_core_near_tiles(silt_core *c);
+
+/* How a step treats the tires.  No counterpart in the reference, which always
+ * runs the nested patch (SILT_TIRE_NESTED, the default and the reference's
+ * step).  SILT_TIRE_COARSE: no patch, every tire shoves the coarse sheet (the
+ * coarse step without a sub-grid model).  SILT_TIRE_SURROGATE: the coarse
+ * step plus the tire sub-grid network's edge fluxes (include/silt_mlp.h) over
+ * the cells the patch would cover, grown by 2; needs silt_core_set_surrogate.
+ * For callers that trade the patch's detail for its cost. */
+#define SILT_TIRE_NESTED 0
+#define SILT_TIRE_COARSE 1
+#define SILT_TIRE_SURROGATE 2
+int silt_core_set_tire_model(silt_core *c, int model);
+/* The network's parameters (SILT_MLP_PARAMS floats, silt_mlp.h order) and its
+ * judge: every judge_every steps with tires (0: never) the nested step also
+ * runs from the same state and its result is kept; when the two depths differ
+ * by more than tol metres anywhere in the window, the next judge_every steps
+ * run nested (fallback), and the step after them is judged again. */
+int silt_core_set_surrogate(silt_core *c, const float *params, int judge_every, float tol);
+
+typedef struct silt_surrogate_stats {
+    float last_tire_ms;     /* GPU time of the last step's tire block: the 2 cm patch
+                               (refine, fine step, restrict) or the network and its fluxes */
+    float last_judge_err_m; /* largest depth difference at the last judged step; -1 before one */
+    float last_judge_rms_m; /* root mean square depth difference over the network's region, both
+                               layers, at the last judged step; -1 before one */
+    int judged;             /* judged steps so far */
+    int fallbacks;          /* judged steps over the tolerance */
+    int last_mode;          /* SILT_TIRE_* whose result the last step kept */
+} silt_surrogate_stats;
+int silt_core_surrogate_stats(const silt_core *c, silt_surrogate_stats *out);
 
 #ifdef __cplusplus
 }

@@ -19,6 +19,8 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include "silt_core.h"
+#include "silt_rut.h"
 #include "silt_sheet.h"
 
 #ifdef __cplusplus
@@ -76,6 +78,32 @@ int silt_pair_last_fallback(const silt_pair *p);
  * path, run once before a long simulation. */
 int silt_pair_autotune(silt_pair *p, float dt, const int *rows, size_t n, int reps, int *best_rows,
                        float *best_ms);
+
+/* Tires on the sheet.  No counterpart in the reference, whose tires act on
+ * the 40 m FluidCore window only (FluidCore::push_tire and the rut stamp,
+ * src/fluid_core.rs:5964, src/soil.rs:2526-2879): the same coupling on a
+ * sheet of any size, for callers that drive many vehicles over a large one.
+ *
+ * set_ground: the ground under the sheet (no placed soil) and the world
+ * corner of cell (0, 0) (default: the sheet centred on the origin); the sheet
+ * bed becomes ground bed + placed.  get_ground: bed, placed and rut depth
+ * (initial - bed, >= 0) out; any may be NULL. */
+int silt_pair_set_ground(silt_pair *p, const float *bed, float origin_x, float origin_z);
+int silt_pair_get_ground(silt_pair *p, float *bed, float *placed, float *rut);
+/* The tire sub-grid network (SILT_MLP_PARAMS floats, silt_mlp.h) each
+ * vehicle of silt_pair_step_tires uses; NULL: none (the tires only shove). */
+int silt_pair_set_surrogate(silt_pair *p, const float *params);
+/* One step with vehicles: the contacts cut Bekker ruts into the ground (when
+ * set) and the bed follows; the network reads each vehicle's region before
+ * the step (pushes in groups of per_vehicle, one vehicle each); the pushes
+ * shove both layers (push_on, src/fluid_core.rs:8152-8239); both layers step
+ * as silt_pair_step; the network's edge fluxes are applied.  Returns as
+ * silt_pair_step. */
+int silt_pair_step_tires(silt_pair *p, float dt, const silt_tire_push *pushes, size_t npush, size_t per_vehicle,
+                         const silt_tire_contact *contacts, size_t ncontacts);
+/* GPU time of the last silt_pair_step_tires' tire work (network inputs and
+ * forward, shoves, flux apply), milliseconds; 0 before a network is set. */
+float silt_pair_last_tire_ms(const silt_pair *p);
 
 #ifdef __cplusplus
 }

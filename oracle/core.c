@@ -74,6 +74,7 @@ int F(oracle_step_sheet_ex)(real *h, real *u, real *v, const real *bed, const re
                             real vorticity, real eps, int banks, real sub_cap, int pace, int inner_par,
                             int confine_mode, size_t *boxes_out, size_t *nboxes_out);
 real F(oracle_dynamic_viscosity)(int grain, real moisture);
+real F(oracle_column_sum)(const real *h, size_t n, real cell);
 
 typedef struct {
     real x, z, fx, fz, radius, width, speed, slip;
@@ -87,6 +88,12 @@ typedef struct {
     int fresh;
     real idle_s, peak;
 } Tile;
+
+struct ParkedSlot {
+    int32_t kx, kz;
+    int state; /* 0 empty, 1 live, 2 deleted */
+    real water, slurry, uw, vw, us, vs, moisture;
+};
 
 typedef struct {
     real origin[2];
@@ -113,6 +120,10 @@ typedef struct {
     real *nu_w, *nu_s;
     uint8_t *gate;
     int spray_count;
+    real moisture_ambient; /* read_substrate's most common moisture */
+    /* cells parked off the window by recenter: open addressing by world key */
+    struct ParkedSlot *parked;
+    size_t pcap, pcount, pused;
 } Core;
 
 static real *zalloc(size_t n) { return (real *)calloc(n, sizeof(real)); }
@@ -153,7 +164,7 @@ void F(oracle_core_free)(Core *c) {
     real *planes[] = {c->water, c->slurry, c->uw, c->vw, c->us, c->vs, c->bed, c->moisture, c->fw, c->fs, c->fuw,
                       c->fvw, c->fus, c->fvs, c->fbed, c->fbed_s, c->fmoist, c->nu_w, c->nu_s};
     for (size_t k = 0; k < sizeof planes / sizeof planes[0]; k++) free(planes[k]);
-    free(c->grain); free(c->fgrain); free(c->hold); free(c->tiles); free(c->gate);
+    free(c->grain); free(c->fgrain); free(c->hold); free(c->tiles); free(c->gate); free(c->parked);
     free(c);
 }
 
@@ -934,17 +945,31 @@ void F(oracle_core_lay_slurry)(Core *c, const real *depth, int grain, real moist
     }
 }
 
-/* read_substrate_grid with arrays (src/fluid_core.rs:5928-5962), ambient
- * moisture histogram aside (it only feeds the dust). */
+/* read_substrate_grid with arrays (src/fluid_core.rs:5928-5962), with the
+ * ambient moisture histogram (1 % buckets, half away from zero). */
 void F(oracle_core_read_substrate)(Core *c, const real *bed, const real *moisture, const uint8_t *grain) {
     restrict_fine(c);
+    unsigned hist[101];
+    memset(hist, 0, sizeof hist);
     for (int k = 0; k < WN; k++) {
         real m = moisture[k];
         m = m < K(0.0) ? K(0.0) : (m > K(1.0) ? K(1.0) : m);
         c->bed[k] = bed[k];
         c->moisture[k] = m;
+        float pct = (float)m * 100.0f;
+        int whole = (int)pct;
+        int bucket = pct - (float)whole >= 0.5f ? whole + 1 : whole;
+        hist[bucket < 100 ? bucket : 100]++;
         if (c->slurry[k] > K(1.0e-5)) c->grain[k] = grain[k];
     }
+    int best = 0;
+    unsigned best_n = 0;
+    for (int k = 0; k <= 100; k++)
+        if (hist[k] > best_n) {
+            best = k;
+            best_n = hist[k];
+        }
+    c->moisture_ambient = (real)((float)best * 0.01f);
 }
 
 /* Copies the coarse planes out (water, slurry, uw, vw, us, vs, bed, moisture). */
@@ -985,10 +1010,16 @@ void F(oracle_core_ledger)(Core *c, real *out) {
     }
     out[0] = w;
     out[1] = s;
-    out[2] = c->frozen_w;
-    out[3] = c->frozen_s;
+    double pw = 0.0, ps = 0.0;
+    for (size_t k = 0; k < c->pcap; k++)
+        if (c->parked[k].state == 1) {
+            pw += (double)c->parked[k].water;
+            ps += (double)c->parked[k].slurry;
+        }
+    out[2] = c->frozen_w + (real)(pw * (double)area);
+    out[3] = c->frozen_s + (real)(ps * (double)area);
     out[4] = c->spray_m3;
-    out[5] = w + s + c->frozen_w + c->frozen_s + c->spray_m3;
+    out[5] = w + s + out[2] + out[3] + c->spray_m3;
 }
 
 /* The render-contract film and flow maps from coarse_cell
@@ -1016,4 +1047,259 @@ void F(oracle_core_publish)(Core *c, real *film2, real *flow4) {
         flow4[4 * k + 2] = speed;
         flow4[4 * k + 3] = K(0.0);
     }
+}
+
+/* drain_over_bed (src/fluid_core.rs:5123-5214) with step_water_drain /
+ * step_slurry_drain (:5159-5214) and restore_wiped (:4622-4637). */
+static void restore_wiped(real *h, real before) {
+    real lost = before - F(oracle_column_sum)(h, WN, COARSE);
+    if (lost <= K(1.0e-8)) return;
+    int best = -1;
+    for (int k = 0; k < WN; k++)
+        if (isfinite((double)h[k]) && (best < 0 || !(h[k] < h[best]))) best = k;
+    if (best >= 0) h[best] += lost / (COARSE * COARSE);
+}
+
+static real peak_speed(const real *h, const real *u, const real *v) {
+    real peak = K(0.0);
+    for (int k = 0; k < WN; k++)
+        if (h[k] > K(1.0e-4)) {
+            real a = u[k] < 0 ? -u[k] : u[k], b = v[k] < 0 ? -v[k] : v[k];
+            if (a > peak) peak = a;
+            if (b > peak) peak = b;
+        }
+    return peak;
+}
+
+real F(oracle_core_drain_over_bed)(Core *c, real dt, unsigned steps) {
+    if (!(isfinite((double)dt) && dt > K(0.0)) || steps == 0) return F(oracle_column_sum)(c->water, WN, COARSE);
+    restrict_fine(c);
+    real *nu_w = zalloc(WN), *nu_s = zalloc(WN);
+    uint8_t *gate = (uint8_t *)malloc(WN);
+    memset(gate, 1, WN);
+    for (int k = 0; k < WN; k++) nu_w[k] = F(oracle_dynamic_viscosity)(3, K(1.0));
+    unsigned calm = 0;
+    int moved = 0;
+    for (unsigned it = 0; it < steps; it++) {
+        real wb = F(oracle_column_sum)(c->water, WN, COARSE);
+        real sb = F(oracle_column_sum)(c->slurry, WN, COARSE);
+        soften_border(c->uw, c->vw, WRES);
+        F(oracle_step_sheet_ex)(c->water, c->uw, c->vw, c->bed, nu_w, gate, WRES, COARSE, dt, 0, K(0.0), K(0.0),
+                                K(0.0), K(1.0e-6), 1, K(8.0), 2, 1, 0, NULL, NULL);
+        for (int k = 0; k < WN; k++) nu_s[k] = F(oracle_dynamic_viscosity)(c->grain[k], c->moisture[k]);
+        soften_border(c->us, c->vs, WRES);
+        F(oracle_step_sheet_ex)(c->slurry, c->us, c->vs, c->bed, nu_s, gate, WRES, COARSE, dt, 0, K(0.0), K(0.0),
+                                K(0.0), K(1.0e-6), 1, K(8.0), 2, 1, 0, NULL, NULL);
+        real keep = (real)expf((float)(K(-0.55) * dt));
+        for (int k = 0; k < WN; k++) {
+            c->uw[k] *= keep;
+            c->vw[k] *= keep;
+            c->us[k] *= keep;
+            c->vs[k] *= keep;
+        }
+        restore_wiped(c->water, wb);
+        restore_wiped(c->slurry, sb);
+        real sw = peak_speed(c->water, c->uw, c->vw), ss = peak_speed(c->slurry, c->us, c->vs);
+        real speed = sw > ss ? sw : ss;
+        if (speed > K(0.05)) moved = 1;
+        if (moved && speed < K(0.008)) {
+            if (++calm >= 24) break;
+        } else {
+            calm = 0;
+        }
+    }
+    free(nu_w);
+    free(nu_s);
+    free(gate);
+    return F(oracle_column_sum)(c->water, WN, COARSE);
+}
+
+/* ---- recenter (src/fluid_core.rs:5409-5422, shift_coarse :9004-9226) ------- */
+
+static size_t park_hash(int32_t kx, int32_t kz, size_t cap) {
+    uint64_t h = ((uint64_t)(uint32_t)kx * 0x9E3779B97F4A7C15ull) ^ ((uint64_t)(uint32_t)kz * 0xC2B2AE3D27D4EB4Full);
+    return (size_t)(h ^ (h >> 29)) & (cap - 1);
+}
+
+static struct ParkedSlot *park_find(Core *c, int32_t kx, int32_t kz) {
+    if (!c->pcap) return NULL;
+    for (size_t k = park_hash(kx, kz, c->pcap), n = 0; n < c->pcap; k = (k + 1) & (c->pcap - 1), n++) {
+        struct ParkedSlot *p = &c->parked[k];
+        if (p->state == 0) return NULL;
+        if (p->state == 1 && p->kx == kx && p->kz == kz) return p;
+    }
+    return NULL;
+}
+
+static void park_grow(Core *c) {
+    size_t cap = c->pcap ? c->pcap * 2 : 1u << 16;
+    struct ParkedSlot *old = c->parked;
+    size_t ocap = c->pcap;
+    c->parked = (struct ParkedSlot *)calloc(cap, sizeof(struct ParkedSlot));
+    c->pcap = cap;
+    c->pcount = c->pused = 0;
+    for (size_t k = 0; k < ocap; k++)
+        if (old[k].state == 1) {
+            size_t j = park_hash(old[k].kx, old[k].kz, cap);
+            while (c->parked[j].state) j = (j + 1) & (cap - 1);
+            c->parked[j] = old[k];
+            c->pcount++;
+            c->pused++;
+        }
+    free(old);
+}
+
+static void park_put(Core *c, int32_t kx, int32_t kz, const struct ParkedSlot *v) {
+    struct ParkedSlot *p = park_find(c, kx, kz);
+    if (!p) {
+        if ((c->pused + 1) * 2 > c->pcap) park_grow(c);
+        size_t k = park_hash(kx, kz, c->pcap);
+        while (c->parked[k].state == 1) k = (k + 1) & (c->pcap - 1);
+        p = &c->parked[k];
+        if (p->state == 0) c->pused++;
+        c->pcount++;
+    }
+    *p = *v;
+    p->kx = kx;
+    p->kz = kz;
+    p->state = 1;
+}
+
+static void park_remove(Core *c, struct ParkedSlot *p) {
+    p->state = 2;
+    c->pcount--;
+}
+
+static void world_key_c(real x, real z, int32_t *kx, int32_t *kz) {
+    *kx = (int32_t)R_FLOOR(x / COARSE);
+    *kz = (int32_t)R_FLOOR(z / COARSE);
+}
+
+static void park_cell(Core *c, const real *org, int i, int j, real *const *h) {
+    int idx = j * WRES + i;
+    int32_t kx, kz;
+    world_key_c(org[0] + ((real)i + K(0.5)) * COARSE, org[1] + ((real)j + K(0.5)) * COARSE, &kx, &kz);
+    struct ParkedSlot v = {0, 0, 1, h[0][idx], h[1][idx], h[2][idx], h[3][idx], h[4][idx], h[5][idx], h[6][idx]};
+    real dm = v.moisture - c->moisture_ambient;
+    if (dm < 0) dm = -dm;
+    if (v.water > K(1.0e-8) || v.slurry > K(1.0e-8) || dm > K(6.0e-3)) {
+        park_put(c, kx, kz, &v);
+    } else {
+        struct ParkedSlot *p = park_find(c, kx, kz);
+        if (p) park_remove(c, p);
+    }
+}
+
+static void restore_cell(Core *c, int i, int j, real *const *o) {
+    int idx = j * WRES + i;
+    int32_t kx, kz;
+    world_key_c(c->origin[0] + ((real)i + K(0.5)) * COARSE, c->origin[1] + ((real)j + K(0.5)) * COARSE, &kx, &kz);
+    struct ParkedSlot *p = park_find(c, kx, kz);
+    if (!p) return;
+    const real v[7] = {p->water, p->slurry, p->uw, p->vw, p->us, p->vs, p->moisture};
+    for (int k = 0; k < 7; k++) o[k][idx] = v[k];
+    park_remove(c, p);
+}
+
+typedef void (*cell_fn)(Core *, const real *, int, int, real *const *);
+
+static void park_cb(Core *c, const real *org, int i, int j, real *const *h) { park_cell(c, org, i, j, h); }
+static void restore_cb(Core *c, const real *org, int i, int j, real *const *o) {
+    (void)org;
+    restore_cell(c, i, j, o);
+}
+
+static void each_leaving_c(Core *c, int di, int dj, cell_fn f, const real *org, real *const *planes) {
+    const int n = WRES;
+    if (di > 0) {
+        int w = di < n ? di : n;
+        for (int j = 0; j < n; j++)
+            for (int i = 0; i < w; i++) f(c, org, i, j, planes);
+    } else if (di < 0) {
+        int w = -di < n ? -di : n;
+        for (int j = 0; j < n; j++)
+            for (int i = n - w; i < n; i++) f(c, org, i, j, planes);
+    }
+    if (dj != 0) {
+        int h = (dj > 0 ? dj : -dj) < n ? (dj > 0 ? dj : -dj) : n;
+        int jlo = dj > 0 ? 0 : n - h, jhi = dj > 0 ? h : n;
+        for (int j = jlo; j < jhi; j++)
+            for (int i = 0; i < n; i++) {
+                int covered = di > 0 ? i < di : (di < 0 ? i >= n + di : 0);
+                if (!covered) f(c, org, i, j, planes);
+            }
+    }
+}
+
+static int uniform_shift_c(const real *o, const real *nw, int *di, int *dj) {
+    int last = WRES - 1, mid = WRES / 2;
+    int32_t o0x, o0z, n0x, n0z, o1x, o1z, n1x, n1z, omx, omz, nmx, nmz;
+    world_key_c(o[0] + K(0.5) * COARSE, o[1] + K(0.5) * COARSE, &o0x, &o0z);
+    world_key_c(nw[0] + K(0.5) * COARSE, nw[1] + K(0.5) * COARSE, &n0x, &n0z);
+    *di = n0x - o0x;
+    *dj = n0z - o0z;
+    world_key_c(o[0] + ((real)last + K(0.5)) * COARSE, o[1] + ((real)last + K(0.5)) * COARSE, &o1x, &o1z);
+    world_key_c(nw[0] + ((real)last + K(0.5)) * COARSE, nw[1] + ((real)last + K(0.5)) * COARSE, &n1x, &n1z);
+    if (o1x - o0x != last || o1z - o0z != last) return 0;
+    if (n1x - n0x != last || n1z - n0z != last) return 0;
+    if (n1x - o1x != *di || n1z - o1z != *dj) return 0;
+    world_key_c(o[0] + ((real)mid + K(0.5)) * COARSE, o[1] + ((real)mid + K(0.5)) * COARSE, &omx, &omz);
+    world_key_c(nw[0] + ((real)mid + K(0.5)) * COARSE, nw[1] + ((real)mid + K(0.5)) * COARSE, &nmx, &nmz);
+    return nmx - omx == *di && nmz - omz == *dj;
+}
+
+void F(oracle_core_recenter)(Core *c, real x, real z) {
+    restrict_fine(c);
+    real extent = (real)WRES * COARSE;
+    real nw[2] = {R_FLOOR((x - extent * K(0.5)) / COARSE) * COARSE, R_FLOOR((z - extent * K(0.5)) / COARSE) * COARSE};
+    real d0 = nw[0] - c->origin[0], d1 = nw[1] - c->origin[1];
+    if ((d0 < 0 ? -d0 : d0) < K(1.0e-4) && (d1 < 0 ? -d1 : d1) < K(1.0e-4)) return;
+    real *h[7] = {c->water, c->slurry, c->uw, c->vw, c->us, c->vs, c->moisture};
+    real *o[7];
+    for (int k = 0; k < 7; k++) {
+        o[k] = zalloc(WN);
+        if (k == 6)
+            for (int q = 0; q < WN; q++) o[k][q] = c->moisture_ambient;
+    }
+    int di, dj;
+    real old[2] = {c->origin[0], c->origin[1]};
+    if (uniform_shift_c(c->origin, nw, &di, &dj)) {
+        each_leaving_c(c, di, dj, park_cb, old, h);
+        for (int j = 0; j < WRES; j++)
+            for (int i = 0; i < WRES; i++) {
+                int si = i + di, sj = j + dj;
+                if (si < 0 || sj < 0 || si >= WRES || sj >= WRES) continue;
+                int src = sj * WRES + si, dst = j * WRES + i;
+                if (!(h[0][src] > K(1.0e-8) || h[1][src] > K(1.0e-8) || h[6][src] > K(1.0e-4))) continue;
+                for (int k = 0; k < 7; k++) o[k][dst] = h[k][src];
+            }
+        c->origin[0] = nw[0];
+        c->origin[1] = nw[1];
+        each_leaving_c(c, -di, -dj, restore_cb, c->origin, o);
+    } else {
+        for (int j = 0; j < WRES; j++)
+            for (int i = 0; i < WRES; i++) park_cell(c, old, i, j, h);
+        c->origin[0] = nw[0];
+        c->origin[1] = nw[1];
+        for (int j = 0; j < WRES; j++)
+            for (int i = 0; i < WRES; i++) restore_cell(c, i, j, o);
+    }
+    for (int k = 0; k < 7; k++) {
+        memcpy(h[k], o[k], WN * sizeof(real));
+        free(o[k]);
+    }
+}
+
+long F(oracle_core_parked_cells)(Core *c) { return (long)c->pcount; }
+
+/* Parked volumes (cubic metres) for the ledger's frozen lines. */
+void F(oracle_core_parked_volume)(Core *c, double *out2) {
+    double w = 0.0, sl = 0.0;
+    for (size_t k = 0; k < c->pcap; k++)
+        if (c->parked[k].state == 1) {
+            w += (double)c->parked[k].water;
+            sl += (double)c->parked[k].slurry;
+        }
+    out2[0] = w * (double)COARSE * (double)COARSE;
+    out2[1] = sl * (double)COARSE * (double)COARSE;
 }

@@ -209,6 +209,12 @@ def _declare_core(L):
             g(n).restype = None
         g("oracle_core_step").argtypes = [_P, R, _P]
         g("oracle_core_step").restype = C.c_int
+        g("oracle_core_drain_over_bed").argtypes = [_P, R, C.c_uint]
+        g("oracle_core_drain_over_bed").restype = R
+        g("oracle_core_recenter").argtypes = [_P, R, R]
+        g("oracle_core_recenter").restype = None
+        g("oracle_core_parked_cells").argtypes = [_P]
+        g("oracle_core_parked_cells").restype = C.c_long
         g("oracle_core_near_tiles").argtypes = [_P]
         g("oracle_core_near_tiles").restype = _Z
         g("oracle_core_fine_on").argtypes = [_P]
@@ -347,6 +353,18 @@ class Core:
         o = np.zeros(2, self.dtype)
         self._f("oracle_core_fine_get")(self._c, _ptr(fw), _ptr(o))
         return fw, o
+
+    def recenter(self, x, z):
+        """recenter (src/fluid_core.rs:5409-5422): move the window, parking
+        cells that slide off and restoring those that slide on."""
+        self._f("oracle_core_recenter")(self._c, x, z)
+
+    def parked_cells(self):
+        return int(self._f("oracle_core_parked_cells")(self._c))
+
+    def drain_over_bed(self, dt, steps):
+        """drain_over_bed (src/fluid_core.rs:5123-5214); the water volume."""
+        return float(self._f("oracle_core_drain_over_bed")(self._c, dt, int(steps)))
 
     def sample_water(self, x, z):
         """Coarse water depth under a world point (sample_coarse when the patch
