@@ -31,7 +31,7 @@ ALG_BYTES_PER_CELL = 52
 # FETCH_SIZE and WRITE_SIZE on this workload (default rows per workgroup),
 # each scaled by the counter's reading for a 1 GiB copy (FETCH x 2, WRITE x 1);
 # profiles/k_sheet_stream_4096_two_layer.md holds the runs and commands.
-PMC_TRAFFIC_BYTES = {("fast", 4096): 1_316_000_666}
+PMC_TRAFFIC_BYTES = {("fast", 4096): 1_325_663_385}
 
 
 def cpu_baseline(sc, dt, steps):

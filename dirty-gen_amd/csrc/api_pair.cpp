@@ -450,8 +450,11 @@ int silt_pair_step_tires(silt_pair* p, float dt, const silt_tire_push* pushes, s
       if (hipMalloc(&p->d_contacts, ncontacts * sizeof(RutContact)) != hipSuccess) return SILT_ENOMEM;
       p->cap_contacts = ncontacts;
     }
+    // Contacts come in the pushes' vehicle groups; vehicles' footprints do
+    // not overlap, so each vehicle stamps in its own workgroup.
+    const int per = (npush && ncontacts == npush) ? (int)per_vehicle : 0;
     int rc = rut_stamp(p->soil_bed, p->soil_placed, p->soil_init, res, p->cell, p->origin[0], p->origin[1], contacts,
-                       ncontacts, dt, p->d_contacts, st);
+                       ncontacts, dt, p->d_contacts, st, per);
     if (rc) return rc;
     core_ground(p->soil_bed, p->soil_placed, p->soil_init, p->bed, p->soil_rut, (int)p->n, st);
   }

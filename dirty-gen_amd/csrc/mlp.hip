@@ -223,13 +223,24 @@ __global__ __launch_bounds__(256) void k_loss(const float* __restrict__ y, const
   if ((threadIdx.x & 63) == 0 && s != 0.0) atomicAdd(loss, s);
 }
 
-// Bias gradient: column sums of an M x N FP32 matrix.
-__global__ void k_colsum(const float* __restrict__ x, int M, int N, int ld, float* __restrict__ out) {
-  int n = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  if (n >= N) return;
+// Bias gradient: column sums of an M x N FP32 matrix.  A block takes 32
+// columns; its 8 row groups each sum every 8th row (coalesced 128 B rows),
+// then the groups add in a fixed order.
+__global__ __launch_bounds__(256) void k_colsum(const float* __restrict__ x, int M, int N, int ld,
+                                                float* __restrict__ out) {
+  __shared__ float part[8][32];
+  const int c = (int)(threadIdx.x & 31), g = (int)(threadIdx.x >> 5);
+  const int n = (int)blockIdx.x * 32 + c;
   float s = 0.0f;
-  for (int m = 0; m < M; m++) s += x[(size_t)m * ld + n];
-  out[n] = s;
+  if (n < N)
+    for (int m = g; m < M; m += 8) s += x[(size_t)m * ld + n];
+  part[g][c] = s;
+  __syncthreads();
+  if (g == 0 && n < N) {
+    float t = part[0][c];
+    for (int k = 1; k < 8; k++) t += part[k][c];
+    out[n] = t;
+  }
 }
 
 __global__ void k_adam(float* __restrict__ w, const float* __restrict__ g, float* __restrict__ m1,
@@ -358,14 +369,14 @@ void mlp_backward(Mlp& m, size_t n) {
   // dW3 = dy^T a2  (8 x 128, K = n)
   g = Gemm{O, H, N, m.dy, O, m.a2, H, nullptr, m.grad + kOffW3, H, nullptr, 0, nullptr, 0, 0};
   gemm<true, true, EPI_PLAIN>(g, m.st);
-  hipLaunchKernelGGL(k_colsum, dim3(1), dim3(64), 0, m.st, m.dyf, N, O, O, m.grad + kOffB3);
+  hipLaunchKernelGGL(k_colsum, dim3(1), dim3(256), 0, m.st, m.dyf, N, O, O, m.grad + kOffB3);
   // dz2 = (dy W3) * gelu'(z2)  (n x 128, K = 8)
   g = Gemm{N, H, O, m.dy, O, m.wb + kOffW3, H, nullptr, m.dz2f, H, m.dz2, H, m.z2, H, 0};
   gemm<false, true, EPI_GELU_BWD>(g, m.st);
   // dW2 = dz2^T a1
   g = Gemm{H, H, N, m.dz2, H, m.a1, H, nullptr, m.grad + kOffW2, H, nullptr, 0, nullptr, 0, 0};
   gemm<true, true, EPI_PLAIN>(g, m.st);
-  hipLaunchKernelGGL(k_colsum, dim3(2), dim3(64), 0, m.st, m.dz2f, N, H, H, m.grad + kOffB2);
+  hipLaunchKernelGGL(k_colsum, dim3(4), dim3(256), 0, m.st, m.dz2f, N, H, H, m.grad + kOffB2);
   // dz1 = (dz2 W2) * gelu'(z1)
   g = Gemm{N, H, H, m.dz2, H, m.wb + kOffW2, H, nullptr, m.dz1f, H, m.dz1, H, m.z1, H, 0};
   gemm<false, true, EPI_GELU_BWD>(g, m.st);
@@ -382,7 +393,7 @@ void mlp_backward(Mlp& m, size_t n) {
     g = Gemm{H, kMlpIn, N, m.dz1, H, m.xb, kMlpIn, nullptr, m.grad + kOffW1, kMlpIn, nullptr, 0, nullptr, 0, 0};
     gemm<true, true, EPI_PLAIN>(g, m.st);
   }
-  hipLaunchKernelGGL(k_colsum, dim3(2), dim3(64), 0, m.st, m.dz1f, N, H, H, m.grad + kOffB1);
+  hipLaunchKernelGGL(k_colsum, dim3(4), dim3(256), 0, m.st, m.dz1f, N, H, H, m.grad + kOffB1);
 }
 
 void mlp_adam(Mlp& m, float lr) {

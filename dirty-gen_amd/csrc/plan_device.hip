@@ -10,7 +10,10 @@
 //      the first column of their overlap, where one of the two cells starts
 //      its run) by a lock-free union of heads: links always point to a
 //      smaller index and only ever decrease (atomicMin), so finds need no
-//      locks and the forest has no cycles; finds halve their path as they go;
+//      locks and the forest has no cycles; finds halve their path as they go.
+//      Rows join inside blocks of 32 first, the heads are compressed, then
+//      the block boundaries join: a mostly wet sheet would otherwise chain
+//      one head per row and every find would walk the chain;
 //   3. heads are compressed to their root, then every cell reads its head's
 //      root; each root takes a body number and every cell widens its body's
 //      box with atomics.
@@ -99,11 +102,17 @@ __global__ __launch_bounds__(256) void k_label_runs(const float* __restrict__ h,
   }
 }
 
-// Step 2: join overlapping runs of rows j-1 and j.
-__global__ void k_label_join(const float* __restrict__ h, int* parent, int res, Box bb, float eps) {
+// Step 2: join overlapping runs of rows j-1 and j, in two phases so no find
+// walks a long chain: phase 0 joins rows inside blocks of kJoinBlock rows
+// (chains stay within a block), the heads are compressed, phase 1 joins the
+// block boundaries (chains of block roots).
+constexpr int kJoinBlock = 32;
+
+__global__ void k_label_join(const float* __restrict__ h, int* parent, int res, Box bb, float eps, int phase) {
   const int i = bb.i0 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
   const int j = bb.j0 + 1 + (int)(blockIdx.y * blockDim.y + threadIdx.y);
   if (i > bb.i1 || j > bb.j1) return;
+  if ((((j - bb.j0) % kJoinBlock) == 0) != (phase == 1)) return;
   const int c = j * res + i;
   if (!wet_at(h, c, eps) || !wet_at(h, c - res, eps)) return;
   const bool left_in = i > bb.i0;
@@ -230,8 +239,13 @@ int plan_begin(const float* h, int res, float cell, float dt, float eps, const S
   const dim3 blk(64, 4);
   hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, stream, d_count, 1);
   hipLaunchKernelGGL(k_label_runs, dim3((unsigned)((bh + 3) / 4)), dim3(256), 0, stream, h, ps.parent, res, bb, eps);
-  if (bh > 1)
-    hipLaunchKernelGGL(k_label_join, grid_over(bw, bh - 1, blk), blk, 0, stream, h, ps.parent, res, bb, eps);
+  if (bh > 1) {
+    hipLaunchKernelGGL(k_label_join, grid_over(bw, bh - 1, blk), blk, 0, stream, h, ps.parent, res, bb, eps, 0);
+    if (bh > kJoinBlock) {
+      hipLaunchKernelGGL(k_label_heads, grid_over(bw, bh, blk), blk, 0, stream, ps.parent, res, bb);
+      hipLaunchKernelGGL(k_label_join, grid_over(bw, bh - 1, blk), blk, 0, stream, h, ps.parent, res, bb, eps, 1);
+    }
+  }
   hipLaunchKernelGGL(k_label_heads, grid_over(bw, bh, blk), blk, 0, stream, ps.parent, res, bb);
   hipLaunchKernelGGL(k_label_cells, grid_over(bw, bh, blk), blk, 0, stream, ps.parent, res, bb);
   hipLaunchKernelGGL(k_label_roots, grid_over(bw, bh, blk), blk, 0, stream, ps.parent, d_list, d_count,

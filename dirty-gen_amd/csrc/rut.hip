@@ -67,12 +67,16 @@ namespace {
 
 constexpr int kMaxFoot = 64 * 64;
 
+// One workgroup applies a group of `per` consecutive contacts in order
+// (stamp_wheel per tire, src/soil.rs:2526-2879); groups whose footprints do
+// not overlap run in parallel, one workgroup each.
 __global__ __launch_bounds__(256) void k_stamp(float* bed, float* placed, const float* initial, int res, float cell,
-                                               float ox, float oz, const RutContact* cs, int nc) {
+                                               float ox, float oz, const RutContact* cs, int nc, int per) {
   __shared__ float drop_s[kMaxFoot];
   __shared__ unsigned char berm_s[kMaxFoot];
   const int t = (int)threadIdx.x;
-  for (int q = 0; q < nc; q++) {
+  const int q0 = (int)blockIdx.x * per, q1 = min(nc, q0 + per);
+  for (int q = q0; q < q1; q++) {
     const RutContact c = cs[q];
     if (c.skip) continue;
     const int W = c.ihi - c.ilo, H = c.jhi - c.jlo;
@@ -223,15 +227,18 @@ int prepare_contacts(const silt_tire_contact* in, size_t n, int res, float cell,
 }
 
 int rut_stamp(float* bed, float* placed, const float* initial, int res, float cell, float ox, float oz,
-              const silt_tire_contact* contacts, size_t n, float dt, RutContact* d_scratch, hipStream_t st) {
+              const silt_tire_contact* contacts, size_t n, float dt, RutContact* d_scratch, hipStream_t st,
+              int per_group) {
   std::vector<RutContact> cs;
   int rc = prepare_contacts(contacts, n, res, cell, ox, oz, dt, &cs);
   if (rc) return rc;
   if (cs.empty()) return 0;
   if (hipMemcpyAsync(d_scratch, cs.data(), cs.size() * sizeof(RutContact), hipMemcpyHostToDevice, st) != hipSuccess)
     return SILT_EDEVICE;
-  hipLaunchKernelGGL(k_stamp, dim3(1), dim3(256), 0, st, bed, placed, initial, res, cell, ox, oz, d_scratch,
-                     (int)cs.size());
+  const int nc = (int)cs.size();
+  const int per = per_group > 0 && per_group < nc ? per_group : nc;
+  hipLaunchKernelGGL(k_stamp, dim3((unsigned)((nc + per - 1) / per)), dim3(256), 0, st, bed, placed, initial, res,
+                     cell, ox, oz, d_scratch, nc, per);
   return hipGetLastError() == hipSuccess ? 0 : SILT_EDEVICE;
 }
 

@@ -25,8 +25,12 @@ float contact_length(float radius, float sinkage);
 
 int prepare_contacts(const silt_tire_contact* in, size_t n, int res, float cell, float ox, float oz, float dt,
                      std::vector<RutContact>* out);
-// Stamps the contacts in order on device planes; d_scratch holds >= n contacts.
+// Stamps the contacts in order on device planes; d_scratch holds >= n
+// contacts.  per_group > 0: consecutive groups of that many contacts whose
+// footprints do not overlap each other are stamped in parallel (in order
+// within a group); 0: all in one sequence.
 int rut_stamp(float* bed, float* placed, const float* initial, int res, float cell, float ox, float oz,
-              const silt_tire_contact* contacts, size_t n, float dt, RutContact* d_scratch, hipStream_t st);
+              const silt_tire_contact* contacts, size_t n, float dt, RutContact* d_scratch, hipStream_t st,
+              int per_group = 0);
 
 }  // namespace silt

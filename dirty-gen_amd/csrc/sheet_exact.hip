@@ -232,6 +232,111 @@ __global__ void k_force_gauss_seidel(const float* __restrict__ h, float* u, floa
   }
 }
 
+// The same sweep with the planes it reads held in LDS.  At wavefront t row
+// jj updates column c(jj, t) = i0 + t - 2 (jj - j0); every cell of that
+// wavefront reads row jj only within c +- 4 (gradient and vorticity stencils
+// reach 2 columns in their own row, 1 column one row away, the same column
+// two rows away).  So each row keeps a ring of 16 columns of u, v, p and h:
+// at step t it loads column c + 4 (not yet updated: its update comes 4
+// wavefronts later), and an updated cell goes to the ring and to memory.
+// One lane per ring row (box rows plus 2 above and below, clamped to the
+// grid); one barrier per wavefront (the slot a lane loads at t + 1 is one
+// no cell of wavefront t reads).  Same arithmetic in the same order as the
+// global-memory sweep.  Needs the box's rows + 4 <= 256.
+constexpr int kRing = 16;
+constexpr int kRingRows = 256;
+
+__global__ __launch_bounds__(256) void k_force_gauss_seidel_lds(const float* __restrict__ h, float* u, float* v,
+                                                                const float* __restrict__ nu,
+                                                                const float* __restrict__ p, Box b, StepArgs a) {
+  __shared__ float ru[kRingRows][kRing], rv[kRingRows][kRing], rp[kRingRows][kRing], rh[kRingRows][kRing];
+  const int res = a.res;
+  const int r_lo = b.j0 - 2 > 0 ? b.j0 - 2 : 0;
+  const int r_hi = b.j1 + 2 < res - 1 ? b.j1 + 2 : res - 1;
+  const int row = (int)threadIdx.x;
+  const int jj = r_lo + row;
+  const bool live = jj <= r_hi;
+  const int W = b.i1 - b.i0 + 1, H = b.j1 - b.j0 + 1;
+  const int tmax = (W - 1) + 2 * (H - 1);
+  const float cell = a.cell;
+  auto col_at = [&](int r, int t) { return b.i0 + t - 2 * (r - b.j0); };
+  auto load = [&](int c) {
+    if (!live || c < 0 || c >= res) return;
+    const size_t idx = (size_t)jj * res + c;
+    ru[row][c & (kRing - 1)] = u[idx];
+    rv[row][c & (kRing - 1)] = v[idx];
+    rp[row][c & (kRing - 1)] = p[idx];
+    rh[row][c & (kRing - 1)] = h[idx];
+  };
+  // Ring slot of grid cell (x, y), the reads clamped to the grid as the
+  // global-memory helpers clamp them.
+  auto at = [&](float (*r)[kRing], int x, int y) -> float {
+    x = x < 0 ? 0 : (x > res - 1 ? res - 1 : x);
+    y = y < 0 ? 0 : (y > res - 1 ? res - 1 : y);
+    return r[y - r_lo][x & (kRing - 1)];
+  };
+  auto vort = [&](int x, int y) -> float {
+    const int xp = x + 1 < res ? x + 1 : res - 1, xm = x > 0 ? x - 1 : 0;
+    const int yp = y + 1 < res ? y + 1 : res - 1, ym = y > 0 ? y - 1 : 0;
+    const float dvdx = (at(rv, xp, y) - at(rv, xm, y)) / (2.0f * cell);
+    const float dudz = (at(ru, x, yp) - at(ru, x, ym)) / (2.0f * cell);
+    return dvdx - dudz;
+  };
+  {
+    const int c0 = col_at(jj, 0);
+    for (int c = c0 - 4; c < c0 + 4; c++) load(c);
+  }
+  for (int t = 0; t <= tmax; t++) {
+    const int c = live ? col_at(jj, t) : 0;
+    load(c + 4);
+    __syncthreads();
+    if (live && jj >= b.j0 && jj <= b.j1 && c >= b.i0 && c <= b.i1) {
+      const int i = c, j = jj;
+      const size_t idx = (size_t)j * res + i;
+      if (at(rh, i, j) > a.eps) {
+        float uu = at(ru, i, j), vv = at(rv, i, j);
+        if (a.wind) {
+          uu = uu + (a.wx - uu) * a.blend;
+          vv = vv + (a.wz - vv) * a.blend;
+        } else {
+          const int ip = i + 1 < res ? i + 1 : res - 1, im = i > 0 ? i - 1 : 0;
+          const int jp = j + 1 < res ? j + 1 : res - 1, jm = j > 0 ? j - 1 : 0;
+          const float px = (at(rp, ip, j) - at(rp, im, j)) / (2.0f * cell);
+          const float pz = (at(rp, i, jp) - at(rp, i, jm)) / (2.0f * cell);
+          const float mob = a.nu_uniform ? a.mob0 : sheet_mobility(nu[idx]);
+          uu = uu + -kGravity * px * mob * a.dt;
+          vv = vv + -kGravity * pz * mob * a.dt;
+        }
+        ru[row][i & (kRing - 1)] = uu;
+        rv[row][i & (kRing - 1)] = vv;
+        if (a.vort > 0.0f) {
+          // confine_at on the ring: this cell's own kick is already in it.
+          const float w = vort(i, j);
+          if (isfinite(w) && !(fabsf(w) < 1.0e-4f)) {
+            const float wxp = fabsf(vort(i + 1 < res ? i + 1 : res - 1, j));
+            const float wxm = fabsf(vort(i > 0 ? i - 1 : 0, j));
+            const float wzp = fabsf(vort(i, j + 1 < res ? j + 1 : res - 1));
+            const float wzm = fabsf(vort(i, j > 0 ? j - 1 : 0));
+            const float nx = (wxp - wxm) / (2.0f * cell);
+            const float nz = (wzp - wzm) / (2.0f * cell);
+            const float du = a.vort * a.dt * (-nz * w);
+            const float dv = a.vort * a.dt * (nx * w);
+            if (isfinite(du) && isfinite(dv) && du * du + dv * dv <= 0.0625f) {
+              uu = uu + du;
+              vv = vv + dv;
+              ru[row][i & (kRing - 1)] = uu;
+              rv[row][i & (kRing - 1)] = vv;
+            }
+          }
+        }
+        u[idx] = uu;
+        v[idx] = vv;
+      }
+    }
+    __syncthreads();
+  }
+}
+
 // Divergence source b = div(u,v) ov - 0.015 ov lap(eta) (:1999-2000).
 __global__ void k_project_source(const float* __restrict__ u, const float* __restrict__ v,
                                  const float* __restrict__ eta, const float* __restrict__ nu, float* __restrict__ bsrc,
@@ -376,7 +481,10 @@ void exact_substep(const SheetPlanes& s, const Box& b, const StepArgs& a, bool d
     hipLaunchKernelGGL(k_zblur_pressure, g_rim, blk, 0, st, s.tg, s.ts, s.eta, s.p, rim, b, res);
   }
   if (gauss_seidel) {
-    hipLaunchKernelGGL(k_force_gauss_seidel, dim3(1), dim3(256), 0, st, s.h, s.u, s.v, s.nu, s.p, b, a);
+    if (b.j1 - b.j0 + 1 + 4 <= kRingRows)
+      hipLaunchKernelGGL(k_force_gauss_seidel_lds, dim3(1), dim3(256), 0, st, s.h, s.u, s.v, s.nu, s.p, b, a);
+    else
+      hipLaunchKernelGGL(k_force_gauss_seidel, dim3(1), dim3(256), 0, st, s.h, s.u, s.v, s.nu, s.p, b, a);
   } else {
     hipLaunchKernelGGL(k_force, g_rim, blk, 0, st, s.h, s.u, s.v, s.nu, s.p, s.su, s.sv, rim, b, a);
     if (a.vort > 0.0f)

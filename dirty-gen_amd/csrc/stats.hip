@@ -60,7 +60,35 @@ __global__ __launch_bounds__(256) void k_stats(const float* __restrict__ h, cons
     j1 = max(j1, __shfl_xor(j1, off));
     s2 = fmaxf(s2, __shfl_xor(s2, off));
   }
+  // The four wavefronts meet in LDS; one set of atomics per block (all
+  // blocks' atomics on the same seven words otherwise queue up behind each
+  // other).
+  __shared__ double s_vol[4];
+  __shared__ unsigned s_wet[4];
+  __shared__ int s_box[4][4];
+  __shared__ float s_s2[4];
+  const int w = (int)(threadIdx.x >> 6);
   if ((threadIdx.x & 63) == 0) {
+    s_vol[w] = vol;
+    s_wet[w] = wet;
+    s_box[w][0] = i0;
+    s_box[w][1] = i1;
+    s_box[w][2] = j0;
+    s_box[w][3] = j1;
+    s_s2[w] = s2;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int nw = (int)(blockDim.x >> 6);
+    for (int k = 1; k < nw; k++) {
+      vol += s_vol[k];
+      wet += s_wet[k];
+      i0 = min(i0, s_box[k][0]);
+      i1 = max(i1, s_box[k][1]);
+      j0 = min(j0, s_box[k][2]);
+      j1 = max(j1, s_box[k][3]);
+      s2 = fmaxf(s2, s_s2[k]);
+    }
     atomicAdd(&s->vol, vol);
     atomicAdd(&s->wet, (unsigned long long)wet);
     if (wet) {
@@ -85,7 +113,13 @@ __global__ __launch_bounds__(256) void k_box_max_s2(const float* __restrict__ u,
       s2 = q > s2 ? q : s2;
     }
   for (int off = 32; off > 0; off >>= 1) s2 = fmaxf(s2, __shfl_xor(s2, off));
-  if ((threadIdx.x & 63) == 0) atomicMax(out, __float_as_uint(s2));
+  __shared__ float s_s2[4];
+  if ((threadIdx.x & 63) == 0) s_s2[threadIdx.x >> 6] = s2;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < (int)(blockDim.x >> 6); k++) s2 = fmaxf(s2, s_s2[k]);
+    atomicMax(out, __float_as_uint(s2));
+  }
 }
 
 }  // namespace
@@ -97,13 +131,13 @@ void stats_reset(SheetStats* s, int n, hipStream_t st) {
 void stats_compute(const float* h, const float* u, const float* v, int res, float eps, SheetStats* s,
                    hipStream_t st) {
   stats_reset(s, 1, st);
-  int blocks = res < 2048 ? res : 2048;
+  int blocks = res < 1024 ? res : 1024;
   hipLaunchKernelGGL(k_stats, dim3((unsigned)blocks), dim3(256), 0, st, h, u, v, res, eps, s);
 }
 
 void box_max_s2(const float* u, const float* v, int res, const Box& b, unsigned* out, hipStream_t st) {
   int rows = b.j1 - b.j0 + 1;
-  int blocks = rows < 2048 ? rows : 2048;
+  int blocks = rows < 1024 ? rows : 1024;
   hipLaunchKernelGGL(k_box_max_s2, dim3((unsigned)blocks), dim3(256), 0, st, u, v, res, b, out);
 }
 

@@ -97,7 +97,10 @@ int silt_pair_set_surrogate(silt_pair *p, const float *params);
  * set) and the bed follows; the network reads each vehicle's region before
  * the step (pushes in groups of per_vehicle, one vehicle each); the pushes
  * shove both layers (push_on, src/fluid_core.rs:8152-8239); both layers step
- * as silt_pair_step; the network's edge fluxes are applied.  Returns as
+ * as silt_pair_step; the network's edge fluxes are applied.  When there are
+ * as many contacts as pushes, they come in the same vehicle groups, and
+ * vehicles' rut footprints must not overlap (each vehicle's ruts are cut by
+ * its own workgroup; a vehicle's own contacts stay in order).  Returns as
  * silt_pair_step. */
 int silt_pair_step_tires(silt_pair *p, float dt, const silt_tire_push *pushes, size_t npush, size_t per_vehicle,
                          const silt_tire_contact *contacts, size_t ncontacts);

@@ -122,9 +122,11 @@ def test_recenter_matches_the_oracle_and_restores_parked_cells():
             pg, po = g.planes(), o.planes()
             for name in ("water", "uw", "vw", "moisture"):
                 _eq(pg[name], po[name], f"{name} move {k}")
+        # The oracle sums the window in f32 as the reference does (good to about
+        # 2e-5 relative over 40 000 cells); the library sums in FP64.
         lg, lo = g.ledger(), o.ledger()
         for key in ("water_m3", "frozen_water_m3", "frozen_slurry_m3", "total_m3"):
-            assert abs(lg[key] - lo[key]) <= 1e-5 * max(abs(lo[key]), 1e-9), (key, lg[key], lo[key])
+            assert abs(lg[key] - lo[key]) <= 1e-4 * max(abs(lo[key]), 1e-9), (key, lg[key], lo[key])
     finally:
         g.close()
 
