@@ -359,6 +359,31 @@ void mlp_forward(Mlp& m, size_t n) {
   gemm<false, false, EPI_PLAIN>(g, m.st);
 }
 
+namespace {
+
+// A weight gradient dW = dZ^T A (both operands batch-major, K = batch): few
+// output tiles and a long K, so K is split into slices summed in order.
+void weight_grad(Gemm g, Mlp& m) {
+  const int c = split_chunk(g.M, g.N, g.K);
+  if (!c) {
+    gemm<true, true, EPI_PLAIN>(g, m.st);
+    return;
+  }
+  float* out = g.out;
+  const int ldo = g.ldo;
+  const int splits = (g.K + c - 1) / c;
+  g.out = m.part;
+  g.ldo = g.N;
+  g.kchunk = c;
+  gemm<true, true, EPI_PARTIAL>(g, m.st);
+  const size_t cnt = (size_t)g.M * g.N;
+  // The gradient rows are contiguous (ldo == N) for every weight matrix.
+  (void)ldo;
+  hipLaunchKernelGGL(k_split_sum, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, m.st, m.part, splits, cnt, out);
+}
+
+}  // namespace
+
 void mlp_backward(Mlp& m, size_t n) {
   const int N = (int)n, H = kMlpHidden, O = kMlpOut;
   const size_t cnt = n * O;
@@ -367,32 +392,20 @@ void mlp_backward(Mlp& m, size_t n) {
                      1.0f / (float)cnt, m.dyf, m.dy, m.loss);
   Gemm g{};
   // dW3 = dy^T a2  (8 x 128, K = n)
-  g = Gemm{O, H, N, m.dy, O, m.a2, H, nullptr, m.grad + kOffW3, H, nullptr, 0, nullptr, 0, 0};
-  gemm<true, true, EPI_PLAIN>(g, m.st);
+  weight_grad(Gemm{O, H, N, m.dy, O, m.a2, H, nullptr, m.grad + kOffW3, H, nullptr, 0, nullptr, 0, 0}, m);
   hipLaunchKernelGGL(k_colsum, dim3(1), dim3(256), 0, m.st, m.dyf, N, O, O, m.grad + kOffB3);
   // dz2 = (dy W3) * gelu'(z2)  (n x 128, K = 8)
   g = Gemm{N, H, O, m.dy, O, m.wb + kOffW3, H, nullptr, m.dz2f, H, m.dz2, H, m.z2, H, 0};
   gemm<false, true, EPI_GELU_BWD>(g, m.st);
   // dW2 = dz2^T a1
-  g = Gemm{H, H, N, m.dz2, H, m.a1, H, nullptr, m.grad + kOffW2, H, nullptr, 0, nullptr, 0, 0};
-  gemm<true, true, EPI_PLAIN>(g, m.st);
+  weight_grad(Gemm{H, H, N, m.dz2, H, m.a1, H, nullptr, m.grad + kOffW2, H, nullptr, 0, nullptr, 0, 0}, m);
   hipLaunchKernelGGL(k_colsum, dim3(4), dim3(256), 0, m.st, m.dz2f, N, H, H, m.grad + kOffB2);
   // dz1 = (dz2 W2) * gelu'(z1)
   g = Gemm{N, H, H, m.dz2, H, m.wb + kOffW2, H, nullptr, m.dz1f, H, m.dz1, H, m.z1, H, 0};
   gemm<false, true, EPI_GELU_BWD>(g, m.st);
   // dW1 = dz1^T x  (128 x 1536)
-  const int cw = split_chunk(H, kMlpIn, N);
-  if (cw) {
-    const int splits = (N + cw - 1) / cw;
-    g = Gemm{H, kMlpIn, N, m.dz1, H, m.xb, kMlpIn, nullptr, m.part, kMlpIn, nullptr, 0, nullptr, 0, cw};
-    gemm<true, true, EPI_PARTIAL>(g, m.st);
-    const size_t cnt = (size_t)H * kMlpIn;
-    hipLaunchKernelGGL(k_split_sum, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, m.st, m.part, splits, cnt,
-                       m.grad + kOffW1);
-  } else {
-    g = Gemm{H, kMlpIn, N, m.dz1, H, m.xb, kMlpIn, nullptr, m.grad + kOffW1, kMlpIn, nullptr, 0, nullptr, 0, 0};
-    gemm<true, true, EPI_PLAIN>(g, m.st);
-  }
+  weight_grad(Gemm{H, kMlpIn, N, m.dz1, H, m.xb, kMlpIn, nullptr, m.grad + kOffW1, kMlpIn, nullptr, 0, nullptr, 0, 0},
+              m);
   hipLaunchKernelGGL(k_colsum, dim3(4), dim3(256), 0, m.st, m.dz1f, N, H, H, m.grad + kOffB1);
 }
 

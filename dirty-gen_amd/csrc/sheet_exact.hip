@@ -240,8 +240,7 @@ __global__ void k_force_gauss_seidel(const float* __restrict__ h, float* u, floa
 // at step t it loads column c + 4 (not yet updated: its update comes 4
 // wavefronts later), and an updated cell goes to the ring and to memory.
 // One lane per ring row (box rows plus 2 above and below, clamped to the
-// grid); one barrier per wavefront (the slot a lane loads at t + 1 is one
-// no cell of wavefront t reads).  Same arithmetic in the same order as the
+// grid).  Same arithmetic in the same order as the
 // global-memory sweep.  Needs the box's rows + 4 <= 256.
 constexpr int kRing = 16;
 constexpr int kRingRows = 256;
@@ -282,13 +281,47 @@ __global__ __launch_bounds__(256) void k_force_gauss_seidel_lds(const float* __r
     const float dudz = (at(ru, x, yp) - at(ru, x, ym)) / (2.0f * cell);
     return dvdx - dudz;
   };
+  // The column a row needs next comes into registers one wavefront early
+  // and goes to the ring at the start of the next: its memory latency hides
+  // behind this wavefront's update.  The slot it lands in holds column
+  // c - 11, which no cell of the current wavefront reads, so one barrier per
+  // wavefront orders everything.
+  float fu = 0.0f, fv = 0.0f, fp = 0.0f, fh = 0.0f;
+  int fc = -1;
+  auto fetch = [&](int c) {
+    fc = (live && c >= 0 && c < res) ? c : -1;
+    if (fc < 0) return;
+    const size_t idx = (size_t)jj * res + c;
+    fu = u[idx];
+    fv = v[idx];
+    fp = p[idx];
+    fh = h[idx];
+  };
+  // The viscosity of the row's next cell (per-cell viscosity only).
+  const bool cell_nu = !a.nu_uniform;
+  auto nu_of = [&](int c) {
+    return (cell_nu && live && c >= 0 && c < res) ? nu[(size_t)jj * res + c] : 0.0f;
+  };
+  float nu_next;
   {
     const int c0 = col_at(jj, 0);
-    for (int c = c0 - 4; c < c0 + 4; c++) load(c);
+    for (int c = c0 - 4; c <= c0 + 4; c++) load(c);
+    fetch(c0 + 5);
+    nu_next = nu_of(c0);
   }
   for (int t = 0; t <= tmax; t++) {
     const int c = live ? col_at(jj, t) : 0;
-    load(c + 4);
+    const float nu_c = nu_next;
+    nu_next = nu_of(c + 1);
+    if (t > 0) {
+      if (fc >= 0) {
+        ru[row][fc & (kRing - 1)] = fu;
+        rv[row][fc & (kRing - 1)] = fv;
+        rp[row][fc & (kRing - 1)] = fp;
+        rh[row][fc & (kRing - 1)] = fh;
+      }
+      fetch(c + 5);
+    }
     __syncthreads();
     if (live && jj >= b.j0 && jj <= b.j1 && c >= b.i0 && c <= b.i1) {
       const int i = c, j = jj;
@@ -303,7 +336,7 @@ __global__ __launch_bounds__(256) void k_force_gauss_seidel_lds(const float* __r
           const int jp = j + 1 < res ? j + 1 : res - 1, jm = j > 0 ? j - 1 : 0;
           const float px = (at(rp, ip, j) - at(rp, im, j)) / (2.0f * cell);
           const float pz = (at(rp, i, jp) - at(rp, i, jm)) / (2.0f * cell);
-          const float mob = a.nu_uniform ? a.mob0 : sheet_mobility(nu[idx]);
+          const float mob = a.nu_uniform ? a.mob0 : sheet_mobility(nu_c);
           uu = uu + -kGravity * px * mob * a.dt;
           vv = vv + -kGravity * pz * mob * a.dt;
         }
@@ -333,7 +366,6 @@ __global__ __launch_bounds__(256) void k_force_gauss_seidel_lds(const float* __r
         v[idx] = vv;
       }
     }
-    __syncthreads();
   }
 }
 
