@@ -98,6 +98,9 @@ struct silt_core {
   // prediction (surrogate), 2-3 the surrogate's flux apply.
   hipEvent_t tev[4] = {nullptr, nullptr, nullptr, nullptr};
   int timed = 0;  // 0 none, 1 nested, 2 surrogate
+  // The slurry layer's stream and the events that fence it to the main one.
+  hipStream_t st2 = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
   hipStream_t st = nullptr;
   std::vector<void*> allocs;
 };
@@ -325,12 +328,18 @@ PushList push_list(const std::vector<CorePush>& v) {
   return pl;
 }
 
+// step_pair (src/fluid_core.rs:7942-8150): the water and slurry layers are
+// independent (the reference joins them on two threads), so the slurry runs
+// on the handle's second stream with its own scratch, fenced by events.
 int step_pair(silt_core* c, float dt, bool fine) {
   if (fine && !c->fine_on) return 0;
   int res = fine ? kFineRes : kWindowRes;
   float cell = fine ? kFineCell : kCoarseCell;
-  Workspace* ws = workspace_for(res);
-  if (!ws) return SILT_ENOMEM;
+  Workspace* ws = workspace_for(res, 0);
+  Workspace* ws2 = workspace_for(res, 1);
+  if (!ws || !ws2) return SILT_ENOMEM;
+  HIPCHK(hipEventRecord(c->fork, c->st));
+  HIPCHK(hipStreamWaitEvent(c->st2, c->fork, 0));
   silt_sheet_opts o;
   silt_sheet_opts_default(&o);
   o.path = c->path;
@@ -349,17 +358,20 @@ int step_pair(silt_core* c, float dt, bool fine) {
   silt_drive dw{SILT_DRIVE_GRAVITY, 0.0f, 0.0f, 0.08f};
   int rc = step_device_planes(s, res, cell, dt, dw, o, true, true, nu_w, c->st);
   if (rc < 0) return rc;
-  // Slurry layer at its own viscosity.
+  // Slurry layer at its own viscosity, on the second stream.
+  SheetPlanes s2 = ws2->planes();
   float* nu = fine ? c->nu_f : c->nu_c;
-  core_nu(nu, fine ? c->f.grain : c->c.grain, fine ? c->f.moisture : c->c.moisture, res * res, c->st);
-  if (!fine) core_soften(c->c.us, c->c.vs, res, c->st);
-  s.h = fine ? c->f.slurry : c->c.slurry;
-  s.u = fine ? c->f.us : c->c.us;
-  s.v = fine ? c->f.vs : c->c.vs;
-  s.bed = fine ? c->f.bed_s : c->c.bed;
-  s.nu = nu;
+  core_nu(nu, fine ? c->f.grain : c->c.grain, fine ? c->f.moisture : c->c.moisture, res * res, c->st2);
+  if (!fine) core_soften(c->c.us, c->c.vs, res, c->st2);
+  s2.h = fine ? c->f.slurry : c->c.slurry;
+  s2.u = fine ? c->f.us : c->c.us;
+  s2.v = fine ? c->f.vs : c->c.vs;
+  s2.bed = fine ? c->f.bed_s : c->c.bed;
+  s2.nu = nu;
   silt_drive ds{SILT_DRIVE_GRAVITY, 0.0f, 0.0f, 0.05f};
-  rc = step_device_planes(s, res, cell, dt, ds, o, true, false, 0.0f, c->st);
+  rc = step_device_planes(s2, res, cell, dt, ds, o, true, false, 0.0f, c->st2);
+  HIPCHK(hipEventRecord(c->join, c->st2));
+  HIPCHK(hipStreamWaitEvent(c->st, c->join, 0));
   return rc < 0 ? rc : 0;
 }
 
@@ -566,7 +578,10 @@ silt_core* silt_core_create(float center_x, float center_z, int path) {
   c->origin[0] = floorf((center_x - extent * 0.5f) / kCoarseCell) * kCoarseCell;
   c->origin[1] = floorf((center_z - extent * 0.5f) / kCoarseCell) * kCoarseCell;
   c->hold.assign(WN, 0);
-  bool ok = hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) == hipSuccess;
+  bool ok = hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) == hipSuccess &&
+            hipStreamCreateWithFlags(&c->st2, hipStreamNonBlocking) == hipSuccess &&
+            hipEventCreateWithFlags(&c->fork, hipEventDisableTiming) == hipSuccess &&
+            hipEventCreateWithFlags(&c->join, hipEventDisableTiming) == hipSuccess;
   float** cp[8] = {&c->c.water, &c->c.slurry, &c->c.uw, &c->c.vw, &c->c.us, &c->c.vs, &c->c.bed, &c->c.moisture};
   for (auto p : cp) ok = ok && dalloc(c, p, WN);
   float** fp[9] = {&c->f.water, &c->f.slurry, &c->f.uw, &c->f.vw, &c->f.us, &c->f.vs, &c->f.bed, &c->f.bed_s,
@@ -598,7 +613,11 @@ void silt_core_destroy(silt_core* c) {
   if (c->net_ready) mlp_free(c->net);
   for (hipEvent_t e : c->tev)
     if (e) (void)hipEventDestroy(e);
+  if (c->st2) (void)hipStreamSynchronize(c->st2);
   for (void* p : c->allocs) (void)hipFree(p);
+  if (c->fork) (void)hipEventDestroy(c->fork);
+  if (c->join) (void)hipEventDestroy(c->join);
+  if (c->st2) (void)hipStreamDestroy(c->st2);
   if (c->st) (void)hipStreamDestroy(c->st);
   delete c;
 }

@@ -1,18 +1,19 @@
 #include "workspace.hpp"
 
 #include <map>
+#include <tuple>
 #include <memory>
 #include <utility>
 
 namespace silt {
 
-Workspace* workspace_for(int res) {
+Workspace* workspace_for(int res, int slot) {
   static std::mutex mu;
-  static std::map<std::pair<int, int>, std::unique_ptr<Workspace>> cache;
+  static std::map<std::tuple<int, int, int>, std::unique_ptr<Workspace>> cache;
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess) return nullptr;
   std::lock_guard<std::mutex> lock(mu);
-  auto key = std::make_pair(dev, res);
+  auto key = std::make_tuple(dev, res, slot);
   auto it = cache.find(key);
   if (it != cache.end()) return it->second.get();
   auto ws = std::make_unique<Workspace>();

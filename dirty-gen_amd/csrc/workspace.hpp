@@ -43,6 +43,7 @@ constexpr int kWorkspacePlanes = 13;
 
 // The calling thread's current device's workspace for grids of res x res,
 // allocated on first use and kept until the process ends.  NULL on failure.
-Workspace* workspace_for(int res);
+// Work that runs concurrently on two streams takes two slots.
+Workspace* workspace_for(int res, int slot = 0);
 
 }  // namespace silt
