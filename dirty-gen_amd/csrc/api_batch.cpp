@@ -74,6 +74,7 @@ struct silt_batch {
   int rows = 0;
   float last_ms = 0.0f;
   int last_fallbacks = 0;
+  PlanLabeller lab;
   std::vector<PlanScratch> ps;  // [n]
 };
 
@@ -180,6 +181,7 @@ void silt_batch_destroy(silt_batch* b) {
   if (!b) return;
   if (b->stream) (void)hipStreamSynchronize(b->stream);
   drop_graph(b);
+  plan_labeller_free(b->lab);
   for (PlanScratch& p : b->ps) plan_scratch_free(p);
   for (Window& w : b->w) {
     (void)hipFree(w.bed);
@@ -242,31 +244,23 @@ int silt_batch_step(silt_batch* b, float dt) {
   // step's kernels, or by the upload).
   HIPCHK(hipMemcpyAsync(b->h_stats, b->stats, n * sizeof(SheetStats), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
-  // Every window's labelling is enqueued before the first wait, and every
-  // box maximum before the second: two waits for the whole batch.
+  // Every window's boxes come from one batched labelling, and every box
+  // maximum is enqueued before the second wait: two waits for the batch.
   std::vector<StepPlan> plans(n);
   std::vector<char> host(n, 0);
   b->last_fallbacks = 0;
+  std::vector<PlanSheet> sheets(n);
+  for (size_t k = 0; k < n; k++) {
+    Window& w = b->w[k];
+    sheets[k] = PlanSheet{w.planes[w.cur][0], w.res, w.cell, b->h_stats + k};
+  }
+  int pr = plan_boxes_many(b->lab, sheets.data(), (int)n, dt, kEps, plans.data(), host.data(), st);
+  if (pr < 0) return pr;
   bool wait = false;
   for (size_t k = 0; k < n; k++) {
     Window& w = b->w[k];
-    int pr = plan_begin(w.planes[w.cur][0], w.res, w.cell, dt, kEps, b->h_stats[k], b->ps[k], &plans[k], st);
-    if (pr < 0) return pr;
-    host[k] = pr == 0;
-    wait = wait || b->ps[k].labelled;
-  }
-  if (wait) HIPCHK(hipStreamSynchronize(st));
-  for (size_t k = 0; k < n; k++) {
-    if (host[k]) continue;
-    int pr = plan_boxes(b->ps[k], b->w[k].res, &plans[k], st);
-    if (pr < 0) return pr;
-    host[k] = pr == 0;
-  }
-  wait = false;
-  for (size_t k = 0; k < n; k++) {
-    Window& w = b->w[k];
     if (host[k]) {
-      // Too many wet bodies to read back: the planes go to the host.
+      // Too many wet clusters to read back: the planes go to the host.
       b->last_fallbacks++;
       std::vector<float> hh(w.n), uh(w.n), vh(w.n), bh(w.n);
       size_t fb = w.n * sizeof(float);
@@ -532,3 +526,62 @@ int silt_batch_autotune(silt_batch* b, float dt, const int* rows, size_t n, int 
 }
 
 }  // extern "C"
+
+int silt_plan_boxes_debug(const float* const* h, const int* res, size_t n, float cell, float eps, int* counts,
+                          int* boxes, size_t cap) {
+  if (!h || !res || !counts || !boxes || n == 0) return SILT_EINVAL;
+  for (size_t k = 0; k < n; k++)
+    if (!h[k] || res[k] <= 0) return SILT_EINVAL;
+  hipStream_t st = nullptr;
+  HIPCHK(hipStreamCreate(&st));
+  std::vector<float*> d(n, nullptr);
+  SheetStats* ds = nullptr;
+  std::vector<SheetStats> hs(n);
+  PlanLabeller lab;
+  int rc = SILT_OK;
+  auto done = [&](int r) {
+    (void)hipStreamSynchronize(st);
+    for (float* q : d)
+      if (q) (void)hipFree(q);
+    if (ds) (void)hipFree(ds);
+    plan_labeller_free(lab);
+    (void)hipStreamDestroy(st);
+    return r;
+  };
+  size_t most = 0;
+  for (size_t k = 0; k < n; k++) most = std::max(most, (size_t)res[k] * res[k]);
+  float* zero = nullptr;
+  if (hipMalloc(&ds, n * sizeof(SheetStats)) != hipSuccess || hipMalloc(&zero, most * sizeof(float)) != hipSuccess)
+    return done(SILT_ENOMEM);
+  d.push_back(zero);
+  if (hipMemsetAsync(zero, 0, most * sizeof(float), st) != hipSuccess) return done(SILT_EDEVICE);
+  for (size_t k = 0; k < n; k++) {
+    const size_t cells = (size_t)res[k] * res[k];
+    if (hipMalloc(&d[k], cells * sizeof(float)) != hipSuccess) return done(SILT_ENOMEM);
+    if (hipMemcpyAsync(d[k], h[k], cells * sizeof(float), hipMemcpyHostToDevice, st) != hipSuccess)
+      return done(SILT_EDEVICE);
+    stats_reset(ds + k, 1, st);
+    stats_compute(d[k], zero, zero, res[k], eps, ds + k, st);
+  }
+  if (hipMemcpyAsync(hs.data(), ds, n * sizeof(SheetStats), hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess)
+    return done(SILT_EDEVICE);
+  std::vector<PlanSheet> sheets(n);
+  for (size_t k = 0; k < n; k++) sheets[k] = PlanSheet{d[k], res[k], cell, &hs[k]};
+  std::vector<StepPlan> plans(n);
+  std::vector<char> host(n, 0);
+  rc = plan_boxes_many(lab, sheets.data(), (int)n, 1.0f / 120.0f, eps, plans.data(), host.data(), st);
+  if (rc < 0) return done(rc);
+  for (size_t k = 0; k < n; k++) {
+    counts[k] = host[k] ? -1 : (int)plans[k].boxes.size();
+    for (size_t b = 0; b < plans[k].boxes.size() && b < cap; b++) {
+      const Box& x = plans[k].boxes[b];
+      int* o = boxes + 4 * (cap * k + b);
+      o[0] = x.i0;
+      o[1] = x.i1;
+      o[2] = x.j0;
+      o[3] = x.j1;
+    }
+  }
+  return done(SILT_OK);
+}

@@ -56,7 +56,8 @@ struct silt_pair {
   float last_kernel_ms = 0.0f;
   int last_nsub[2] = {0, 0};
   int last_fallback = 0;
-  PlanScratch ps;
+  PlanLabeller lab;
+  PlanScratch ps[2];
   // Tires on the sheet (silt_pair_step_tires): the ground the ruts cut (the
   // sheet bed is ground bed + placed), the world corner of cell (0, 0), the
   // network and its per-vehicle context and limiter scratch.
@@ -134,7 +135,8 @@ silt_pair* silt_pair_create(size_t res, float cell, int path) {
 void silt_pair_destroy(silt_pair* p) {
   if (!p) return;
   if (p->stream) (void)hipStreamSynchronize(p->stream);
-  plan_scratch_free(p->ps);
+  plan_labeller_free(p->lab);
+  for (PlanScratch& q : p->ps) plan_scratch_free(q);
   if (p->net_ready) mlp_free(p->net);
   for (void* q : {(void*)p->soil_bed, (void*)p->soil_placed, (void*)p->soil_init, (void*)p->soil_rut,
                   (void*)p->d_contacts, (void*)p->sur_ctx, (void*)p->sur_fac})
@@ -204,13 +206,18 @@ int silt_pair_step(silt_pair* p, float dt) {
   HIPCHK(hipStreamSynchronize(st));
   StepPlan plans[2];
   p->last_fallback = 0;
+  // Both layers' boxes in one labelling, both substep counts with one wait.
+  PlanSheet sheets[2];
+  char host[2];
+  for (int l = 0; l < 2; l++) sheets[l] = PlanSheet{p->planes[l][p->cur[l]][0], res, p->cell, p->h_stats + l};
+  int pr = plan_boxes_many(p->lab, sheets, 2, dt, kEps, plans, host, st);
+  if (pr < 0) return pr;
+  bool wait = false;
   for (int l = 0; l < 2; l++) {
     float** pl = p->planes[l][p->cur[l]];
-    int pr = plan_sheet_device(pl[0], pl[1], pl[2], res, p->cell, dt, kEps, p->h_stats[l], p->ps, &plans[l], st);
-    if (pr < 0) return pr;
-    if (pr == 0) {
-      // A 2 cm sheet (sleeping tiles) or more wet bodies than are read back:
-      // the planes go to the host.
+    if (host[l]) {
+      // A 2 cm sheet (sleeping tiles) or more wet clusters than are read
+      // back: the planes go to the host.
       p->last_fallback |= 1 << l;
       std::vector<float> hh(p->n), uh(p->n), vh(p->n), bh(p->n);
       size_t fb = p->n * sizeof(float);
@@ -220,8 +227,15 @@ int silt_pair_step(silt_pair* p, float dt) {
       HIPCHK(hipMemcpyAsync(bh.data(), p->bed, fb, hipMemcpyDeviceToHost, st));
       HIPCHK(hipStreamSynchronize(st));
       plans[l] = plan_step(hh.data(), uh.data(), vh.data(), bh.data(), res, p->cell, dt, 0, 0.0f, 0.0f, kEps, 8.0f, 0);
+      continue;
     }
+    int rc = plan_nsub_begin(pl[1], pl[2], res, p->cell, dt, p->h_stats[l], p->ps[l], &plans[l], st);
+    if (rc < 0) return rc;
+    wait = wait || p->ps[l].maxs2_pending;
   }
+  if (wait) HIPCHK(hipStreamSynchronize(st));
+  for (int l = 0; l < 2; l++)
+    if (!host[l]) plan_nsub_end(p->ps[l], p->cell, dt, &plans[l]);
   silt_sheet_opts o;
   silt_sheet_opts_default(&o);
   o.path = p->path;

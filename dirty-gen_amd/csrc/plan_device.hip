@@ -1,27 +1,40 @@
-// Work boxes of a sheet found on the GPU: sheet_work_boxes
+// Work boxes of sheets found on the GPU: sheet_work_boxes
 // (src/fluid_core.rs:1325-1447) for a wet set that is not one rectangle.
 //
 // The reference labels 4-connected wet bodies by row runs with a union-find
-// on the host.  Here the same idea runs on the GPU over the wet bounding box:
+// on the host, grows every body's box by 6 cells, merges boxes that come
+// within one cell of each other (boxes_touch, :1449-1456) and keeps the
+// merged list unless it has one box or more than 8 (then the wet bounding
+// box).  Only the merged boxes matter, and they are coarser than the bodies:
+// two wet cells at most 14 cells apart on both axes always end in one merged
+// box (their grown boxes touch).  So the GPU labels 4 x 4 TILES instead of
+// cells:
 //
-//   1. one wavefront per row finds the row's runs (ballot over 64 columns at a
-//      time) and points every wet cell at its run's first cell (its head);
-//   2. each pair of overlapping runs of neighbouring rows is joined once (at
-//      the first column of their overlap, where one of the two cells starts
-//      its run) by a lock-free union of heads: links always point to a
-//      smaller index and only ever decrease (atomicMin), so finds need no
-//      locks and the forest has no cycles; finds halve their path as they go.
-//      Rows join inside blocks of 32 first, the heads are compressed, then
-//      the block boundaries join: a mostly wet sheet would otherwise chain
-//      one head per row and every find would walk the chain;
-//   3. heads are compressed to their root, then every cell reads its head's
-//      root; each root takes a body number and every cell widens its body's
-//      box with atomics.
+//   * a tile is wet when it holds a wet cell; it keeps the tight box of its
+//     wet cells and its first wet cell in row-major order;
+//   * 4-connected wet tiles form clusters.  A body's cells sit in one cluster
+//     (neighbouring cells are in the same or 4-neighbouring tiles), and a
+//     cluster's cells sit in one merged box (cells of neighbouring tiles are
+//     at most 7 apart), so a cluster is a union of bodies inside one merged
+//     box;
+//   * each cluster's tight box (of its wet cells) and first cell come back;
+//     the host grows and merges them as the reference does its bodies.
 //
-// Links join heads only, so a find walks at most one link per row of a body.
-// Only the body boxes (with the body's first cell, to restore the
-// reference's row-scan order) come back to the host, which grows them by 6,
-// merges touching ones and applies the reference's one-or-more-than-8 rule.
+// Merging is order-free in what it produces: every merge the reference makes
+// from bodies also happens from clusters (boxes only grow), and nothing else
+// does, so both end in the same set of boxes.  The list order can differ
+// when 2 to 8 boxes remain; those boxes are at least 2 cells apart and the
+// sheet steps them independently.
+//
+// The tile labelling is the run-based union-find the reference uses, on the
+// GPU: one wavefront per tile row finds the runs (ballot over 64 tiles at a
+// time); overlapping runs of neighbouring rows are joined once by a
+// lock-free union of run heads (links only point to smaller indices and only
+// decrease, so no locks and no cycles; finds halve their path), first inside
+// blocks of 32 rows and then across the block boundaries, so a mostly wet
+// sheet does not chain one head per row.  Every kernel serves a batch of
+// sheets (a pair's two layers, every window of a batch): a block finds its
+// sheet from the per-sheet block starts in its arguments.
 #include <limits.h>
 #include <string.h>
 
@@ -36,9 +49,113 @@ namespace silt {
 
 namespace {
 
+constexpr int kTile = 4;        // tile side in cells (<= 7 keeps clusters inside merged boxes)
 constexpr int kBodyStride = 5;  // i0, i1, j0, j1, first cell
+constexpr int kJoinBlock = 32;  // tile rows joined in the first phase
 
-__device__ inline bool wet_at(const float* h, int c, float eps) { return h[c] > eps; }
+// Summary of a wet tile: bit 31 set, then the tight box of its wet cells in
+// tile coordinates (li0, li1, lj0, lj1, 2 bits each) and the first wet
+// column of row lj0 (2 bits).  0 for a dry tile.
+constexpr unsigned kWetBit = 1u << 31;
+
+__device__ inline unsigned tile_field(unsigned s, int k) { return (s >> (2 * k)) & 3u; }
+
+// One sheet of a labelling batch (device copy of the host's table).
+struct PlanJob {
+  const float* h;    // cell depths, res x res
+  int* parent;       // tile labels, tw x th
+  unsigned* info;    // tile summaries, tw x th
+  int res, tw, th;   // cells per side; tile grid over the wet box
+  int ti0, tj0;      // first tile (grid coordinates / kTile)
+  int bx;            // workgroups across the tile grid (64 x 4 tiles each)
+  Box cb;            // wet cell box (statistics)
+  float eps;
+  int slot;          // body slot of this sheet
+};
+
+// Block starts of one launch: job k of the launch owns blocks
+// [start[k], start[k + 1]).
+constexpr int kLaunchJobs = 64;
+struct JobStarts {
+  int first;  // job index of start[0]
+  int n;
+  int start[kLaunchJobs + 1];
+};
+
+__device__ inline int job_of(const JobStarts& js, int b) {
+  int lo = 0, hi = js.n - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (js.start[mid] <= b)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+// Body record (id) of sheet slot s: ids below kPlanBodyPrefix sit in the
+// head region that comes back in one copy, the rest in the tail region.
+__host__ __device__ inline size_t body_offset(int nslots, int s, int id) {
+  const size_t counts = (size_t)((nslots + 63) & ~63);
+  if (id < kPlanBodyPrefix) return counts + ((size_t)s * kPlanBodyPrefix + id) * kBodyStride;
+  const size_t head = (size_t)nslots * kPlanBodyPrefix * kBodyStride;
+  return counts + head + ((size_t)s * (kPlanBodyCap - kPlanBodyPrefix) + (id - kPlanBodyPrefix)) * kBodyStride;
+}
+
+// Tile coordinates of thread (threadIdx.x, threadIdx.y) of a 64 x 4 block
+// of a job, or false when outside the job's tile grid.
+__device__ inline bool tile_of(const JobStarts& js, const PlanJob* jobs, const PlanJob*& J, int& x, int& y) {
+  const int b = (int)blockIdx.x;
+  const int k = job_of(js, b);
+  J = jobs + js.first + k;
+  const int l = b - js.start[k];
+  x = (l % J->bx) * 64 + (int)threadIdx.x;
+  y = (l / J->bx) * 4 + (int)threadIdx.y;
+  return x < J->tw && y < J->th;
+}
+
+// Tile summaries (and the sheet's body count cleared).
+__global__ __launch_bounds__(256) void k_tile_summary(JobStarts js, const PlanJob* __restrict__ jobs,
+                                                      int* __restrict__ bodies) {
+  const PlanJob* J;
+  int x, y;
+  const bool in = tile_of(js, jobs, J, x, y);
+  if (blockIdx.x == (unsigned)js.start[job_of(js, (int)blockIdx.x)] && threadIdx.x == 0 && threadIdx.y == 0)
+    bodies[J->slot] = 0;
+  if (!in) return;
+  const int i0 = (J->ti0 + x) * kTile, j0 = (J->tj0 + y) * kTile;
+  unsigned rows[kTile];
+  unsigned cols = 0;
+  int lj0 = -1, lj1 = -1;
+#pragma unroll
+  for (int r = 0; r < kTile; r++) {
+    const int j = j0 + r;
+    unsigned m = 0;
+    if (j >= J->cb.j0 && j <= J->cb.j1) {
+      const float* row = J->h + (size_t)j * J->res;
+#pragma unroll
+      for (int c = 0; c < kTile; c++) {
+        const int i = i0 + c;
+        if (i >= J->cb.i0 && i <= J->cb.i1 && row[i] > J->eps) m |= 1u << c;
+      }
+    }
+    rows[r] = m;
+    cols |= m;
+    if (m) {
+      if (lj0 < 0) lj0 = r;
+      lj1 = r;
+    }
+  }
+  unsigned s = 0;
+  if (cols) {
+    const int li0 = __ffs((int)cols) - 1, li1 = 31 - __clz((int)cols);
+    const int fc = __ffs((int)rows[lj0]) - 1;
+    s = kWetBit | (unsigned)li0 | ((unsigned)li1 << 2) | ((unsigned)lj0 << 4) | ((unsigned)lj1 << 6) |
+        ((unsigned)fc << 8);
+  }
+  J->info[(size_t)y * J->tw + x] = s;
+}
 
 // Root of x with path halving.  Every store only lowers a link (atomicMin),
 // so a concurrent union that lowered it first is never undone.
@@ -70,122 +187,166 @@ __device__ inline void uf_unite(int* parent, int a, int b) {
   }
 }
 
-// Step 1: one 64-lane wavefront per row of the box.  parent[c] = the head of
-// c's run for wet cells, -1 for dry ones.  Cells outside the wet box are
-// never read (the box holds every wet cell).
-__global__ __launch_bounds__(256) void k_label_runs(const float* __restrict__ h, int* __restrict__ parent, int res,
-                                                    Box bb, float eps) {
+// Runs of wet tiles: one 64-lane wavefront per tile row.  parent[t] = the
+// head of t's run for wet tiles, -1 for dry ones.
+__global__ __launch_bounds__(256) void k_tile_runs(JobStarts js, const PlanJob* __restrict__ jobs) {
+  const int b = (int)blockIdx.x;
+  const int k = job_of(js, b);
+  const PlanJob* J = jobs + js.first + k;
   const int lane = (int)(threadIdx.x & 63);
-  const int j = bb.j0 + (int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
-  if (j > bb.j1) return;  // whole wavefronts leave together
-  const int row = j * res;
+  const int y = (b - js.start[k]) * 4 + (int)(threadIdx.x >> 6);
+  if (y >= J->th) return;  // whole wavefronts leave together
+  const int row = y * J->tw;
   int carry = -1;  // head of the run that reaches the previous segment's end
-  for (int s = bb.i0; s <= bb.i1; s += 64) {
-    const int i = s + lane;
-    const bool in = i <= bb.i1;
-    const bool w = in && wet_at(h, row + i, eps);
+  for (int s = 0; s < J->tw; s += 64) {
+    const int x = s + lane;
+    const bool in = x < J->tw;
+    const bool w = in && J->info[row + x] != 0u;
     const unsigned long long wm = __ballot(w);
-    // A run starts at a wet cell whose left neighbour in this segment is dry
-    // (bit 0: always a start here; it continues the carried run instead when
-    // the previous segment ended wet).
+    // A run starts at a wet tile whose left neighbour in this segment is dry
+    // (bit 0: continues the carried run when the previous segment ended wet).
     const unsigned long long starts = wm & ~(wm << 1);
     const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
     const unsigned long long mine = starts & upto;
-    int head = -1;  // stays -1 only on dry lanes
+    int head = -1;
     if (mine) {
-      const int b = 63 - __clzll((long long)mine);
-      head = (b == 0 && carry >= 0) ? carry : row + s + b;
+      const int bit = 63 - __clzll((long long)mine);
+      head = (bit == 0 && carry >= 0) ? carry : row + s + bit;
     }
-    if (in) parent[row + i] = w ? head : -1;
+    if (in) J->parent[row + x] = w ? head : -1;
     const int last = __shfl(head, 63);
     carry = ((wm >> 63) & 1ull) ? last : -1;
   }
 }
 
-// Step 2: join overlapping runs of rows j-1 and j, in two phases so no find
-// walks a long chain: phase 0 joins rows inside blocks of kJoinBlock rows
-// (chains stay within a block), the heads are compressed, phase 1 joins the
-// block boundaries (chains of block roots).
-constexpr int kJoinBlock = 32;
-
-__global__ void k_label_join(const float* __restrict__ h, int* parent, int res, Box bb, float eps, int phase) {
-  const int i = bb.i0 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  const int j = bb.j0 + 1 + (int)(blockIdx.y * blockDim.y + threadIdx.y);
-  if (i > bb.i1 || j > bb.j1) return;
-  if ((((j - bb.j0) % kJoinBlock) == 0) != (phase == 1)) return;
-  const int c = j * res + i;
-  if (!wet_at(h, c, eps) || !wet_at(h, c - res, eps)) return;
-  const bool left_in = i > bb.i0;
-  const bool starts_here = !left_in || !wet_at(h, c - 1, eps);
-  const bool starts_above = !left_in || !wet_at(h, c - res - 1, eps);
+// Join overlapping runs of tile rows y-1 and y, once per pair (at the first
+// column of the overlap, where one of the two tiles starts its run).  Phase
+// 0 joins rows inside blocks of kJoinBlock rows, phase 1 the block
+// boundaries after the heads were compressed.
+__global__ __launch_bounds__(256) void k_tile_join(JobStarts js, const PlanJob* __restrict__ jobs, int phase) {
+  const PlanJob* J;
+  int x, y;
+  if (!tile_of(js, jobs, J, x, y) || y == 0) return;
+  if (((y % kJoinBlock) == 0) != (phase == 1)) return;
+  const int t = y * J->tw + x, tw = J->tw;
+  const unsigned* info = J->info;
+  if (info[t] == 0u || info[t - tw] == 0u) return;
+  const bool starts_here = x == 0 || info[t - 1] == 0u;
+  const bool starts_above = x == 0 || info[t - tw - 1] == 0u;
   if (!starts_here && !starts_above) return;
-  uf_unite(parent, parent[c], parent[c - res]);
+  uf_unite(J->parent, J->parent[t], J->parent[t - tw]);
 }
 
-// Step 3a: heads point straight at their root.
-__global__ void k_label_heads(int* parent, int res, Box bb) {
-  const int i = bb.i0 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  const int j = bb.j0 + (int)(blockIdx.y * blockDim.y + threadIdx.y);
-  if (i > bb.i1 || j > bb.j1) return;
-  const int c = j * res + i;
-  if (parent[c] < 0) return;
-  if (i == bb.i0 || parent[c - 1] < 0) parent[c] = uf_find(parent, c);
+// Run heads point straight at their root.
+__global__ __launch_bounds__(256) void k_tile_heads(JobStarts js, const PlanJob* __restrict__ jobs) {
+  const PlanJob* J;
+  int x, y;
+  if (!tile_of(js, jobs, J, x, y)) return;
+  const int t = y * J->tw + x;
+  int* parent = J->parent;
+  if (parent[t] < 0) return;
+  if (x == 0 || parent[t - 1] < 0) parent[t] = uf_find(parent, t);
 }
 
-// Step 3b: every cell takes its head's root (heads are final here, and a
-// head's slot already holds that root).
-__global__ void k_label_cells(int* parent, int res, Box bb) {
-  const int i = bb.i0 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  const int j = bb.j0 + (int)(blockIdx.y * blockDim.y + threadIdx.y);
-  if (i > bb.i1 || j > bb.j1) return;
-  const int c = j * res + i;
-  const int p = parent[c];
-  if (p < 0 || p == c) return;
-  parent[c] = parent[p];
+// Every tile takes its head's root (heads are final here).
+__global__ __launch_bounds__(256) void k_tile_cells(JobStarts js, const PlanJob* __restrict__ jobs) {
+  const PlanJob* J;
+  int x, y;
+  if (!tile_of(js, jobs, J, x, y)) return;
+  const int t = y * J->tw + x;
+  int* parent = J->parent;
+  const int p = parent[t];
+  if (p < 0 || p == t) return;
+  parent[t] = parent[p];
 }
 
-// Step 3c: number the roots.  A root's own slot becomes -(id + 2).
-__global__ void k_label_roots(int* parent, int* bodies, unsigned* count, int cap, int res, Box bb) {
-  const int i = bb.i0 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  const int j = bb.j0 + (int)(blockIdx.y * blockDim.y + threadIdx.y);
-  if (i > bb.i1 || j > bb.j1) return;
-  const int c = j * res + i;
-  if (parent[c] != c) return;
-  const unsigned id = atomicAdd(count, 1u);
-  if (id < (unsigned)cap) {
-    int* b = bodies + kBodyStride * id;
+// Number the roots (a root's slot becomes -(id + 2)) and open their records.
+__global__ __launch_bounds__(256) void k_tile_roots(JobStarts js, const PlanJob* __restrict__ jobs, int* bodies,
+                                                    int nslots) {
+  const PlanJob* J;
+  int x, y;
+  if (!tile_of(js, jobs, J, x, y)) return;
+  const int t = y * J->tw + x;
+  if (J->parent[t] != t) return;
+  const unsigned id = atomicAdd(reinterpret_cast<unsigned*>(bodies) + J->slot, 1u);
+  if (id < (unsigned)kPlanBodyCap) {
+    int* b = bodies + body_offset(nslots, J->slot, (int)id);
     b[0] = INT_MAX;
     b[1] = -1;
     b[2] = INT_MAX;
     b[3] = -1;
-    b[4] = c;  // the body's first cell in row-major order (links point down)
+    b[4] = INT_MAX;
   }
-  parent[c] = -(int)id - 2;
+  J->parent[t] = -(int)id - 2;
 }
 
-__global__ void k_label_boxes(const int* parent, int* bodies, int cap, int res, Box bb) {
-  const int i = bb.i0 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  const int j = bb.j0 + (int)(blockIdx.y * blockDim.y + threadIdx.y);
-  if (i > bb.i1 || j > bb.j1) return;
-  const int c = j * res + i;
-  const int r = parent[c];
-  if (r == -1) return;
-  // Only the ends of a run can widen its body's box (every row of a body
-  // has a run, so the first cells give the rows too): a few atomics per run
-  // instead of four per cell on one hot address.
-  const bool first = i == bb.i0 || parent[c - 1] == -1;
-  const bool last = i == bb.i1 || parent[c + 1] == -1;
-  if (!first && !last) return;
-  const int code = r < -1 ? r : parent[r];
-  const int id = -code - 2;
-  if (id >= cap) return;
-  int* b = bodies + kBodyStride * id;
-  if (first) {
-    atomicMin(&b[0], i);
-    atomicMin(&b[2], j);
-    atomicMax(&b[3], j);
+// Lowest (hi = false) or highest value of v over the lanes of the wavefront
+// whose act is set, per cluster id: one atomic per distinct id in the
+// wavefront (a cluster usually covers the whole wavefront).
+__device__ inline void wave_extreme(int* bodies, int nslots, int slot, bool act, int id, int field, int v, bool hi) {
+  unsigned long long todo = __ballot(act);
+  const int lane = (int)(threadIdx.x & 63);
+  while (todo) {
+    const int leader = __ffsll((long long)todo) - 1;
+    const int lid = __shfl(id, leader);
+    const bool mine = act && id == lid;
+    int r = mine ? v : (hi ? INT_MIN : INT_MAX);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      const int q = __shfl_xor(r, o);
+      r = hi ? max(r, q) : min(r, q);
+    }
+    if (lane == leader) {
+      int* b = bodies + body_offset(nslots, slot, lid);
+      if (hi)
+        atomicMax(b + field, r);
+      else
+        atomicMin(b + field, r);
+    }
+    todo &= ~__ballot(mine);
   }
-  if (last) atomicMax(&b[1], i);
+}
+
+// Cluster boxes in cells.  Only edge tiles can hold a cluster's extreme
+// cells: the leftmost column of a cluster has dry tiles to its left (a wet
+// left neighbour would be in the cluster), and so on; the first cell lies
+// in the cluster's top tile row, whose tiles all have a dry tile above.
+__global__ __launch_bounds__(256) void k_tile_boxes(JobStarts js, const PlanJob* __restrict__ jobs, int* bodies,
+                                                    int nslots) {
+  const PlanJob* J;
+  int x, y;
+  const bool in = tile_of(js, jobs, J, x, y);
+  const int tw = J->tw;
+  const int t = y * tw + x;
+  const int r = in ? J->parent[t] : -1;
+  int id = -1;
+  if (r != -1) {
+    const int code = r < -1 ? r : J->parent[r];
+    id = -code - 2;
+  }
+  const bool ok = id >= 0 && id < kPlanBodyCap;
+  bool left = false, right = false, top = false, bottom = false;
+  int ci0 = 0, ci1 = 0, cj0 = 0, cj1 = 0, first = 0;
+  if (ok) {
+    const unsigned s = J->info[t];
+    left = x == 0 || J->parent[t - 1] == -1;
+    right = x == tw - 1 || J->parent[t + 1] == -1;
+    top = y == 0 || J->parent[t - tw] == -1;
+    bottom = y == J->th - 1 || J->parent[t + tw] == -1;
+    const int gi = (J->ti0 + x) * kTile, gj = (J->tj0 + y) * kTile;
+    ci0 = gi + (int)tile_field(s, 0);
+    ci1 = gi + (int)tile_field(s, 1);
+    cj0 = gj + (int)tile_field(s, 2);
+    cj1 = gj + (int)tile_field(s, 3);
+    first = cj0 * J->res + gi + (int)tile_field(s, 4);
+  }
+  // Every lane of a wavefront belongs to the same job (blocks never span
+  // two jobs).
+  wave_extreme(bodies, nslots, J->slot, left, id, 0, ci0, false);
+  wave_extreme(bodies, nslots, J->slot, right, id, 1, ci1, true);
+  wave_extreme(bodies, nslots, J->slot, top, id, 2, cj0, false);
+  wave_extreme(bodies, nslots, J->slot, bottom, id, 3, cj1, true);
+  wave_extreme(bodies, nslots, J->slot, top, id, 4, first, false);
 }
 
 __global__ void k_zero_u32(unsigned* p, int n) {
@@ -193,95 +354,156 @@ __global__ void k_zero_u32(unsigned* p, int n) {
   if (k < n) p[k] = 0u;
 }
 
-dim3 grid_over(int w, int hgt, dim3 blk) {
-  return dim3((unsigned)((w + (int)blk.x - 1) / (int)blk.x), (unsigned)((hgt + (int)blk.y - 1) / (int)blk.y));
-}
-
 Box halo3_of(const SheetStats& st, int res) {
   return Box{std::max(st.i0 - 3, 0), std::min(st.i1 + 3, res - 1), std::max(st.j0 - 3, 0),
              std::min(st.j1 + 3, res - 1)};
 }
 
+size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
 }  // namespace
 
-int plan_begin(const float* h, int res, float cell, float dt, float eps, const SheetStats& st, PlanScratch& ps,
-               StepPlan* plan, hipStream_t stream) {
-  *plan = StepPlan{};
-  ps.labelled = false;
-  if (!(dt > 0.0f) || st.wet == 0) return 1;
-  if (cell <= kFineCell + 1.0e-6f) return 0;  // 2 cm sheets plan on awake tiles
-  const long long area = (long long)(st.i1 - st.i0 + 1) * (long long)(st.j1 - st.j0 + 1);
-  if ((long long)st.wet == area) {
-    plan->boxes.push_back(halo3_of(st, res));
-    return 1;
+int plan_boxes_many(PlanLabeller& L, const PlanSheet* s, int n, float dt, float eps, StepPlan* plans, char* host,
+                    hipStream_t stream) {
+  std::vector<int> lab;
+  for (int k = 0; k < n; k++) {
+    plans[k] = StepPlan{};
+    host[k] = 0;
+    const SheetStats& st = *s[k].stats;
+    if (!(dt > 0.0f) || st.wet == 0) continue;
+    if (s[k].cell <= kFineCell + 1.0e-6f) {  // 2 cm sheets plan on awake tiles
+      host[k] = 1;
+      continue;
+    }
+    const long long area = (long long)(st.i1 - st.i0 + 1) * (long long)(st.j1 - st.j0 + 1);
+    if ((long long)st.wet == area) {
+      plans[k].boxes.push_back(halo3_of(st, s[k].res));
+      continue;
+    }
+    lab.push_back(k);
   }
-  const size_t n = (size_t)res * res;
-  if (ps.cap_cells < n) {
-    if (ps.parent) (void)hipFree(ps.parent);
-    ps.parent = nullptr;
-    ps.cap_cells = 0;
-    if (hipMalloc(&ps.parent, n * sizeof(int)) != hipSuccess) return SILT_ENOMEM;
-    ps.cap_cells = n;
+  if (lab.empty()) return 0;
+  const int nj = (int)lab.size();
+  // Tile grids over each wet box, packed in one buffer.
+  std::vector<PlanJob> jobs((size_t)nj);
+  std::vector<size_t> off((size_t)nj);
+  size_t tiles_bytes = 0;
+  for (int q = 0; q < nj; q++) {
+    const PlanSheet& sh = s[lab[q]];
+    const SheetStats& st = *sh.stats;
+    PlanJob& J = jobs[q];
+    J.h = sh.h;
+    J.res = sh.res;
+    J.ti0 = st.i0 / kTile;
+    J.tj0 = st.j0 / kTile;
+    J.tw = st.i1 / kTile - J.ti0 + 1;
+    J.th = st.j1 / kTile - J.tj0 + 1;
+    J.bx = (J.tw + 63) / 64;
+    J.cb = Box{st.i0, st.i1, st.j0, st.j1};
+    J.eps = eps;
+    J.slot = q;
+    off[q] = tiles_bytes;
+    tiles_bytes += align256((size_t)J.tw * J.th * sizeof(int)) * 2;
   }
-  // Layout: [count, 3 pad][bodies...]; one copy brings the count and the
-  // first kPlanBodyPrefix bodies.
-  const size_t body_bytes = 16 + (size_t)kPlanBodyCap * kBodyStride * sizeof(int);
-  const size_t head_bytes = 16 + (size_t)kPlanBodyPrefix * kBodyStride * sizeof(int);
-  if (!ps.bodies) {
-    if (hipMalloc(&ps.bodies, body_bytes) != hipSuccess ||
-        hipHostMalloc(&ps.h_bodies, body_bytes, hipHostMallocDefault) != hipSuccess)
+  if (L.cap_tiles_bytes < tiles_bytes) {
+    if (L.tiles) (void)hipFree(L.tiles);
+    L.tiles = nullptr;
+    L.cap_tiles_bytes = 0;
+    if (hipMalloc(&L.tiles, tiles_bytes) != hipSuccess) return SILT_ENOMEM;
+    L.cap_tiles_bytes = tiles_bytes;
+  }
+  for (int q = 0; q < nj; q++) {
+    char* base = static_cast<char*>(L.tiles) + off[q];
+    jobs[q].parent = reinterpret_cast<int*>(base);
+    jobs[q].info = reinterpret_cast<unsigned*>(base + align256((size_t)jobs[q].tw * jobs[q].th * sizeof(int)));
+  }
+  if (L.cap_slots < nj) {
+    const size_t body_ints = body_offset(nj, nj, kPlanBodyPrefix);  // counts, heads and tails of nj slots
+    if (L.bodies) (void)hipFree(L.bodies);
+    if (L.h_bodies) (void)hipHostFree(L.h_bodies);
+    if (L.jobs) (void)hipFree(L.jobs);
+    if (L.h_jobs) (void)hipHostFree(L.h_jobs);
+    L.bodies = L.h_bodies = nullptr;
+    L.jobs = L.h_jobs = nullptr;
+    L.cap_slots = 0;
+    if (hipMalloc(&L.bodies, body_ints * sizeof(int)) != hipSuccess ||
+        hipHostMalloc(&L.h_bodies, body_ints * sizeof(int), hipHostMallocDefault) != hipSuccess ||
+        hipMalloc(&L.jobs, (size_t)nj * sizeof(PlanJob)) != hipSuccess ||
+        hipHostMalloc(&L.h_jobs, (size_t)nj * sizeof(PlanJob), hipHostMallocDefault) != hipSuccess)
       return SILT_ENOMEM;
+    L.cap_slots = nj;
   }
-  unsigned* d_count = reinterpret_cast<unsigned*>(ps.bodies);
-  int* d_list = ps.bodies + 4;
-  const Box bb{st.i0, st.i1, st.j0, st.j1};
-  const int bw = bb.i1 - bb.i0 + 1, bh = bb.j1 - bb.j0 + 1;
-  const dim3 blk(64, 4);
-  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, stream, d_count, 1);
-  hipLaunchKernelGGL(k_label_runs, dim3((unsigned)((bh + 3) / 4)), dim3(256), 0, stream, h, ps.parent, res, bb, eps);
-  if (bh > 1) {
-    hipLaunchKernelGGL(k_label_join, grid_over(bw, bh - 1, blk), blk, 0, stream, h, ps.parent, res, bb, eps, 0);
-    if (bh > kJoinBlock) {
-      hipLaunchKernelGGL(k_label_heads, grid_over(bw, bh, blk), blk, 0, stream, ps.parent, res, bb);
-      hipLaunchKernelGGL(k_label_join, grid_over(bw, bh - 1, blk), blk, 0, stream, h, ps.parent, res, bb, eps, 1);
+  // The body layout depends on the slot count: lay it out for cap_slots.
+  const int nslots = L.cap_slots;
+  memcpy(L.h_jobs, jobs.data(), (size_t)nj * sizeof(PlanJob));
+  if (hipMemcpyAsync(L.jobs, L.h_jobs, (size_t)nj * sizeof(PlanJob), hipMemcpyHostToDevice, stream) != hipSuccess)
+    return SILT_EDEVICE;
+  const PlanJob* dj = static_cast<const PlanJob*>(L.jobs);
+  bool two_phase = false;
+  for (const PlanJob& J : jobs) two_phase = two_phase || J.th > kJoinBlock;
+  for (int c0 = 0; c0 < nj; c0 += kLaunchJobs) {
+    const int c1 = std::min(nj, c0 + kLaunchJobs);
+    JobStarts tiles{}, rows{};
+    tiles.first = rows.first = c0;
+    tiles.n = rows.n = c1 - c0;
+    for (int q = c0; q < c1; q++) {
+      const PlanJob& J = jobs[q];
+      tiles.start[q - c0 + 1] = tiles.start[q - c0] + J.bx * ((J.th + 3) / 4);
+      rows.start[q - c0 + 1] = rows.start[q - c0] + (J.th + 3) / 4;
+    }
+    const dim3 blk(64, 4);
+    const unsigned gt = (unsigned)tiles.start[c1 - c0], gr = (unsigned)rows.start[c1 - c0];
+    hipLaunchKernelGGL(k_tile_summary, dim3(gt), blk, 0, stream, tiles, dj, L.bodies);
+    hipLaunchKernelGGL(k_tile_runs, dim3(gr), dim3(256), 0, stream, rows, dj);
+    hipLaunchKernelGGL(k_tile_join, dim3(gt), blk, 0, stream, tiles, dj, 0);
+    if (two_phase) {
+      hipLaunchKernelGGL(k_tile_heads, dim3(gt), blk, 0, stream, tiles, dj);
+      hipLaunchKernelGGL(k_tile_join, dim3(gt), blk, 0, stream, tiles, dj, 1);
+    }
+    hipLaunchKernelGGL(k_tile_heads, dim3(gt), blk, 0, stream, tiles, dj);
+    hipLaunchKernelGGL(k_tile_cells, dim3(gt), blk, 0, stream, tiles, dj);
+    hipLaunchKernelGGL(k_tile_roots, dim3(gt), blk, 0, stream, tiles, dj, L.bodies, nslots);
+    hipLaunchKernelGGL(k_tile_boxes, dim3(gt), blk, 0, stream, tiles, dj, L.bodies, nslots);
+  }
+  // Counts and the first kPlanBodyPrefix records of every sheet: one copy.
+  const size_t head_ints = body_offset(nslots, nj, 0);
+  if (hipMemcpyAsync(L.h_bodies, L.bodies, head_ints * sizeof(int), hipMemcpyDeviceToHost, stream) != hipSuccess ||
+      hipStreamSynchronize(stream) != hipSuccess)
+    return SILT_EDEVICE;
+  const unsigned* count = reinterpret_cast<const unsigned*>(L.h_bodies);
+  bool tail = false;
+  for (int q = 0; q < nj; q++) {
+    if (count[q] > (unsigned)kPlanBodyCap) continue;
+    if (count[q] > (unsigned)kPlanBodyPrefix) {
+      const size_t o = body_offset(nslots, q, kPlanBodyPrefix);
+      if (hipMemcpyAsync(L.h_bodies + o, L.bodies + o,
+                         (size_t)(count[q] - kPlanBodyPrefix) * kBodyStride * sizeof(int), hipMemcpyDeviceToHost,
+                         stream) != hipSuccess)
+        return SILT_EDEVICE;
+      tail = true;
     }
   }
-  hipLaunchKernelGGL(k_label_heads, grid_over(bw, bh, blk), blk, 0, stream, ps.parent, res, bb);
-  hipLaunchKernelGGL(k_label_cells, grid_over(bw, bh, blk), blk, 0, stream, ps.parent, res, bb);
-  hipLaunchKernelGGL(k_label_roots, grid_over(bw, bh, blk), blk, 0, stream, ps.parent, d_list, d_count,
-                     kPlanBodyCap, res, bb);
-  hipLaunchKernelGGL(k_label_boxes, grid_over(bw, bh, blk), blk, 0, stream, ps.parent, d_list, kPlanBodyCap, res,
-                     bb);
-  if (hipMemcpyAsync(ps.h_bodies, ps.bodies, head_bytes, hipMemcpyDeviceToHost, stream) != hipSuccess)
-    return SILT_EDEVICE;
-  ps.labelled = true;
-  ps.halo3 = halo3_of(st, res);
-  return 1;
-}
-
-int plan_boxes(PlanScratch& ps, int res, StepPlan* plan, hipStream_t stream) {
-  if (!ps.labelled) return 1;
-  ps.labelled = false;
-  const unsigned count = reinterpret_cast<const unsigned*>(ps.h_bodies)[0];
-  if (count > (unsigned)kPlanBodyCap) return 0;  // too many bodies to read back: plan on the host
-  if (count > (unsigned)kPlanBodyPrefix &&
-      (hipMemcpyAsync(ps.h_bodies + 4, ps.bodies + 4, (size_t)count * kBodyStride * sizeof(int),
-                      hipMemcpyDeviceToHost, stream) != hipSuccess ||
-       hipStreamSynchronize(stream) != hipSuccess))
-    return SILT_EDEVICE;
-  // Body numbers come from atomics; the reference lists bodies in the
-  // row-major order of their first cell, so sort on that.
-  std::vector<std::pair<int, Box>> found((size_t)count);
-  for (unsigned k = 0; k < count; k++) {
-    const int* b = ps.h_bodies + 4 + kBodyStride * k;
-    found[k] = {b[4], Box{b[0], b[1], b[2], b[3]}};
+  if (tail && hipStreamSynchronize(stream) != hipSuccess) return SILT_EDEVICE;
+  for (int q = 0; q < nj; q++) {
+    const int k = lab[q];
+    if (count[q] > (unsigned)kPlanBodyCap) {  // too many clusters to read back: plan on the host
+      host[k] = 1;
+      continue;
+    }
+    // Cluster numbers come from atomics; list them in the row-major order of
+    // their first cell, as the reference lists its bodies.
+    std::vector<std::pair<int, Box>> found((size_t)count[q]);
+    for (unsigned id = 0; id < count[q]; id++) {
+      const int* b = L.h_bodies + body_offset(nslots, q, (int)id);
+      found[id] = {b[4], Box{b[0], b[1], b[2], b[3]}};
+    }
+    std::sort(found.begin(), found.end(), [](const auto& x, const auto& y) { return x.first < y.first; });
+    std::vector<Box> bodies;
+    bodies.reserve(found.size());
+    for (auto& f : found) bodies.push_back(f.second);
+    plans[k].boxes = boxes_from_bodies(std::move(bodies), s[k].res, halo3_of(*s[k].stats, s[k].res));
   }
-  std::sort(found.begin(), found.end(), [](const auto& x, const auto& y) { return x.first < y.first; });
-  std::vector<Box> bodies;
-  bodies.reserve(found.size());
-  for (auto& f : found) bodies.push_back(f.second);
-  plan->boxes = boxes_from_bodies(std::move(bodies), res, ps.halo3);
-  return 1;
+  return 0;
 }
 
 int plan_nsub_begin(const float* u, const float* v, int res, float cell, float dt, const SheetStats& st,
@@ -315,27 +537,19 @@ void plan_nsub_end(PlanScratch& ps, float cell, float dt, StepPlan* plan) {
   plan->nsub = nsub_from_s2(s2, dt, cell);
 }
 
-int plan_sheet_device(const float* h, const float* u, const float* v, int res, float cell, float dt, float eps,
-                      const SheetStats& st, PlanScratch& ps, StepPlan* plan, hipStream_t stream) {
-  int rc = plan_begin(h, res, cell, dt, eps, st, ps, plan, stream);
-  if (rc <= 0) return rc;
-  if (ps.labelled) {
-    if (hipStreamSynchronize(stream) != hipSuccess) return SILT_EDEVICE;
-    if ((rc = plan_boxes(ps, res, plan, stream)) <= 0) return rc;
-  }
-  if ((rc = plan_nsub_begin(u, v, res, cell, dt, st, ps, plan, stream)) < 0) return rc;
-  if (ps.maxs2_pending && hipStreamSynchronize(stream) != hipSuccess) return SILT_EDEVICE;
-  plan_nsub_end(ps, cell, dt, plan);
-  return 1;
-}
-
 void plan_scratch_free(PlanScratch& ps) {
-  if (ps.parent) (void)hipFree(ps.parent);
-  if (ps.bodies) (void)hipFree(ps.bodies);
-  if (ps.h_bodies) (void)hipHostFree(ps.h_bodies);
   if (ps.maxs2) (void)hipFree(ps.maxs2);
   if (ps.h_maxs2) (void)hipHostFree(ps.h_maxs2);
   ps = PlanScratch{};
+}
+
+void plan_labeller_free(PlanLabeller& L) {
+  if (L.tiles) (void)hipFree(L.tiles);
+  if (L.bodies) (void)hipFree(L.bodies);
+  if (L.h_bodies) (void)hipHostFree(L.h_bodies);
+  if (L.jobs) (void)hipFree(L.jobs);
+  if (L.h_jobs) (void)hipHostFree(L.h_jobs);
+  L = PlanLabeller{};
 }
 
 }  // namespace silt

@@ -60,45 +60,53 @@ StepArgs make_args(int res, float cell, float dt, const silt_drive& d, float eps
 // Substep count of a gravity sheet whose boxes' largest squared speed is s2.
 int nsub_from_s2(float s2, float dt, float cell);
 
-// Device planning of a gravity sheet with cell > 2 cm whose planes are on the
+// Device planning of gravity sheets with cell > 2 cm whose planes are on the
 // device, from statistics of those planes: one box for a solid wet rectangle,
-// else the wet bodies are found on the GPU (sheet_work_boxes).  The body list
-// holds at most kPlanBodyCap bodies; the first kPlanBodyPrefix come back with
-// the count in one copy.
+// else the wet set's clusters are labelled on the GPU (sheet_work_boxes).
+// A sheet returns at most kPlanBodyCap clusters; the first kPlanBodyPrefix
+// of every sheet come back with the counts in one copy.
 constexpr int kPlanBodyCap = 4096;
 constexpr int kPlanBodyPrefix = 64;
-// Scratch of one sheet's plan (grown on demand, freed with plan_scratch_free):
-// cell labels (res^2 ints), the body list and its pinned copy, the boxes'
-// largest squared speed.
-struct PlanScratch {
-  int* parent = nullptr;
-  size_t cap_cells = 0;
+
+// Scratch of a labelling batch (grown on demand, freed with
+// plan_labeller_free): tile labels and summaries, cluster records and their
+// pinned copy, the sheet table.
+struct PlanLabeller {
+  void* tiles = nullptr;
+  size_t cap_tiles_bytes = 0;
   int* bodies = nullptr;
   int* h_bodies = nullptr;  // pinned
+  void* jobs = nullptr;
+  void* h_jobs = nullptr;  // pinned
+  int cap_slots = 0;
+};
+// One sheet to plan: depth plane on the device and its statistics.
+struct PlanSheet {
+  const float* h;
+  int res;
+  float cell;
+  const SheetStats* stats;
+};
+// Work boxes of n sheets with one batched labelling and one wait.  plans[k]
+// gets the boxes (nsub left 0); host[k] = 1 when sheet k must plan from its
+// planes on the host (2 cm cell, or more than kPlanBodyCap clusters).
+// Returns 0 or a SILT_E* code.
+int plan_boxes_many(PlanLabeller& L, const PlanSheet* s, int n, float dt, float eps, StepPlan* plans, char* host,
+                    hipStream_t stream);
+void plan_labeller_free(PlanLabeller& L);
+
+// Substep count of a planned sheet, split so many sheets wait once:
+//   plan_nsub_begin  enqueue the boxes' speed maximum when the global one
+//                    allows more than one substep;
+//   (wait)           plan_nsub_end sets nsub.
+struct PlanScratch {
   unsigned* maxs2 = nullptr;
   unsigned* h_maxs2 = nullptr;  // pinned
-  bool labelled = false;       // plan_begin enqueued a labelling
-  bool maxs2_pending = false;  // plan_nsub_begin enqueued a box maximum
-  Box halo3{};
+  bool maxs2_pending = false;   // plan_nsub_begin enqueued a box maximum
 };
-// Split form, so many sheets plan with two waits in all:
-//   plan_begin      enqueue the labelling (or plan a solid / dry sheet outright);
-//   (wait)          plan_boxes turns the read-back bodies into boxes;
-//   plan_nsub_begin enqueue the boxes' speed maximum when the global one
-//                   allows more than one substep;
-//   (wait)          plan_nsub_end sets nsub.
-// plan_begin and plan_boxes return 1 when planned, 0 when the host must plan
-// from the planes (2 cm cell, or more than kPlanBodyCap bodies), or a SILT_E*
-// code; plan_nsub_begin returns 0 or a SILT_E* code.
-int plan_begin(const float* h, int res, float cell, float dt, float eps, const SheetStats& st, PlanScratch& ps,
-               StepPlan* plan, hipStream_t stream);
-int plan_boxes(PlanScratch& ps, int res, StepPlan* plan, hipStream_t stream);
 int plan_nsub_begin(const float* u, const float* v, int res, float cell, float dt, const SheetStats& st,
                     PlanScratch& ps, StepPlan* plan, hipStream_t stream);
 void plan_nsub_end(PlanScratch& ps, float cell, float dt, StepPlan* plan);
-// All four steps for one sheet (blocks on its small read backs).
-int plan_sheet_device(const float* h, const float* u, const float* v, int res, float cell, float dt, float eps,
-                      const SheetStats& st, PlanScratch& ps, StepPlan* plan, hipStream_t stream);
 void plan_scratch_free(PlanScratch& ps);
 
 // One substep of one or two layers (same res, drive kind and projection) in

@@ -127,6 +127,7 @@ def _declare(L):
     L.silt_batch_set_rows.argtypes = [_P, C.c_int]
     L.silt_batch_autotune.argtypes = [_P, _F, C.POINTER(C.c_int), _Z, C.c_int, C.POINTER(C.c_int),
                                       C.POINTER(C.c_float)]
+    L.silt_plan_boxes_debug.argtypes = [C.POINTER(C.c_void_p), C.POINTER(C.c_int), _Z, _F, _F, _P, _P, _Z]
     L.silt_set_chunk_rows.argtypes = [_Z, C.c_int]
     L.silt_set_chunk_rows.restype = C.c_int
     L.silt_chunk_rows.argtypes = [_Z]
@@ -272,6 +273,29 @@ class Batch:
         _check_status(lib().silt_batch_autotune(self._h, dt, arr, len(rows) if rows else 0, reps, C.byref(best),
                                                 C.byref(ms)))
         return best.value, ms.value
+
+
+def plan_boxes(planes, cell, eps=1.0e-6, cap=64):
+    """Work boxes (i0, i1, j0, j1) of square depth planes planned together on
+    the GPU, as a batch step plans them (the reference's sheet_work_boxes,
+    src/fluid_core.rs:1325-1447).  One list per plane; [] for a dry plane,
+    None when the plane would be planned on the host."""
+    arrs = [np.ascontiguousarray(a, np.float32).ravel() for a in planes]
+    res = [int(round(np.sqrt(a.size))) for a in arrs]
+    for a, r in zip(arrs, res):
+        if r * r != a.size:
+            raise ValueError("planes must be square")
+    n = len(arrs)
+    hp = (C.c_void_p * n)(*[a.ctypes.data for a in arrs])
+    rs = (C.c_int * n)(*res)
+    counts = np.zeros(n, np.int32)
+    boxes = np.zeros((n, cap, 4), np.int32)
+    _check_status(lib().silt_plan_boxes_debug(hp, rs, n, cell, eps, _ptr(counts), _ptr(boxes), cap))
+    out = []
+    for k in range(n):
+        c = int(counts[k])
+        out.append(None if c < 0 else [tuple(int(v) for v in boxes[k, b]) for b in range(min(c, cap))])
+    return out
 
 
 class SheetPair:

@@ -49,7 +49,7 @@ int silt_batch_ledger(silt_batch *b, size_t k, silt_layer_ledger_t *out);
 /* GPU time of the last step (HIP events around its launches). */
 float silt_batch_last_kernel_ms(const silt_batch *b);
 /* Windows whose last plan needed a read back of their planes (more than
- * 4096 separate wet bodies); the launch still ran on the GPU. */
+ * 4096 separate wet clusters); the launch still ran on the GPU. */
 int silt_batch_last_fallbacks(const silt_batch *b);
 /* Launch graphs captured so far (a new one each time the launch shape of a
  * step changes). */
@@ -65,6 +65,15 @@ int silt_batch_set_rows(silt_batch *b, int rows);
  * after each; keep the fastest.  best_rows / best_ms may be NULL. */
 int silt_batch_autotune(silt_batch *b, float dt, const int *rows, size_t n, int reps, int *best_rows,
                         float *best_ms);
+
+/* Test hook, no counterpart in the reference (whose sheet_work_boxes,
+ * src/fluid_core.rs:1325-1447, it answers for device planes): the work boxes
+ * of n host depth planes (h[k], res[k] x res[k], cell metres, wet above eps)
+ * planned together on the GPU as a batch step plans them.  counts[k] = the
+ * number of boxes of sheet k (boxes + 4 * cap * k holds them as i0, i1, j0,
+ * j1), 0 for a dry sheet, -1 when the sheet would be planned on the host. */
+int silt_plan_boxes_debug(const float *const *h, const int *res, size_t n, float cell, float eps, int *counts,
+                          int *boxes, size_t cap);
 
 #ifdef __cplusplus
 }
