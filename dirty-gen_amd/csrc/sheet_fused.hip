@@ -145,6 +145,9 @@ struct Ar {
   }
 };
 
+// Squared speed for the plan statistics (one expression for every store).
+__device__ __forceinline__ float speed2(float u, float v) { return u * u + v * v; }
+
 // Shift a register window one row on: w[0] drops, w[N-1] = v.
 template <int N>
 __device__ __forceinline__ void push(float (&w)[N], float v) {
@@ -310,6 +313,14 @@ __global__ __launch_bounds__(kLanes, 4) void k_sheet_stream(FusedArgs fa) {
   int acc_j0 = 0x7fffffff, acc_j1 = -1;
   float acc_s2 = 0.0f;
   const bool out_col = t >= kHaloL && t < kHaloL + kOutW && col_in;
+
+  // A tile that no work box reaches stores its input unchanged (outside the
+  // boxes no force acts, no transport face exists and no dry cut runs), so it
+  // only copies its cells and adds them to the statistics.
+  bool reached = FULL;
+  if (!FULL)
+    for (int k = 0; k < boxes.n; k++)
+      reached = reached || (boxes.i0[k] < x0 + kOutW && boxes.i1[k] >= x0 && boxes.j0[k] < y1 && boxes.j1[k] >= y0);
 
   const bool nu_uni = a.nu_uniform != 0;
   const bool gate_uni = a.gate_uniform != 0;
@@ -673,7 +684,7 @@ __global__ __launch_bounds__(kLanes, 4) void k_sheet_stream(FusedArgs fa) {
         acc_wet += wet ? 1u : 0u;
         acc_j0 = wet ? min(acc_j0, y) : acc_j0;
         acc_j1 = wet ? y : acc_j1;  // rows come in increasing order
-        acc_s2 = store ? fmaxf(acc_s2, uo * uo + vo * vo) : acc_s2;
+        acc_s2 = store ? fmaxf(acc_s2, speed2(uo, vo)) : acc_s2;
       }
       if (store) {
         size_t idx = (size_t)y * res + col;
@@ -694,24 +705,43 @@ __global__ __launch_bounds__(kLanes, 4) void k_sheet_stream(FusedArgs fa) {
   using Odd = std::integral_constant<int, 1>;
   using Edge = std::true_type;
   using Inside = std::false_type;
-  prefetch(Edge{}, r_begin);
-  // Rows r-20 .. r+5 inside the grid cover every stage row, stencil
-  // neighbour and load of the pair of rows (r, r+1): those pairs take the
-  // interior body.  They form one run, so three loops: top edge, interior,
-  // bottom edge.
-  auto interior = [&](int r) { return r - 20 >= 0 && r + 5 < res; };
-  int r = r_begin;
-  for (; r <= r_end && !interior(r); r += 2) {
-    row(Even{}, Edge{}, r);
-    row(Odd{}, Edge{}, r + 1);
-  }
-  for (; r <= r_end && interior(r); r += 2) {
-    row(Even{}, Inside{}, r);
-    row(Odd{}, Inside{}, r + 1);
-  }
-  for (; r <= r_end; r += 2) {
-    row(Even{}, Edge{}, r);
-    row(Odd{}, Edge{}, r + 1);
+  if (reached) {
+    prefetch(Edge{}, r_begin);
+    // Rows r-20 .. r+5 inside the grid cover every stage row, stencil
+    // neighbour and load of the pair of rows (r, r+1): those pairs take the
+    // interior body.  They form one run, so three loops: top edge, interior,
+    // bottom edge.
+    auto interior = [&](int r) { return r - 20 >= 0 && r + 5 < res; };
+    int r = r_begin;
+    for (; r <= r_end && !interior(r); r += 2) {
+      row(Even{}, Edge{}, r);
+      row(Odd{}, Edge{}, r + 1);
+    }
+    for (; r <= r_end && interior(r); r += 2) {
+      row(Even{}, Inside{}, r);
+      row(Odd{}, Inside{}, r + 1);
+    }
+    for (; r <= r_end; r += 2) {
+      row(Even{}, Edge{}, r);
+      row(Odd{}, Edge{}, r + 1);
+    }
+  } else if (out_col) {
+#pragma unroll 4
+    for (int y = y0; y < y1; y++) {
+      const size_t idx = (size_t)y * res + col;
+      const float hv = L.h[idx], uv = L.u[idx], vv = L.v[idx];
+      L.ho[idx] = hv;
+      L.uo[idx] = uv;
+      L.vo[idx] = vv;
+      if (want_stats) {
+        acc_vol += (double)fmaxf(hv, 0.0f);
+        const bool wet = hv > eps;
+        acc_wet += wet ? 1u : 0u;
+        acc_j0 = wet ? min(acc_j0, y) : acc_j0;
+        acc_j1 = wet ? y : acc_j1;
+        acc_s2 = fmaxf(acc_s2, speed2(uv, vv));
+      }
+    }
   }
 
   if (want_stats) {
