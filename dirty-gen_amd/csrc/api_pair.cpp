@@ -70,6 +70,12 @@ struct silt_pair {
   float* sur_ctx = nullptr;
   float* sur_fac = nullptr;
   size_t cap_groups = 0;
+  // Per-vehicle launch table (TireJob records, then the pushes) and its
+  // pinned staging copy.
+  void* d_tires = nullptr;
+  void* h_tires = nullptr;
+  size_t cap_tires = 0;
+  bool tire_batching = true;
   hipEvent_t tev[4] = {nullptr, nullptr, nullptr, nullptr};
   float last_tire_ms = 0.0f;
 };
@@ -139,8 +145,9 @@ void silt_pair_destroy(silt_pair* p) {
   for (PlanScratch& q : p->ps) plan_scratch_free(q);
   if (p->net_ready) mlp_free(p->net);
   for (void* q : {(void*)p->soil_bed, (void*)p->soil_placed, (void*)p->soil_init, (void*)p->soil_rut,
-                  (void*)p->d_contacts, (void*)p->sur_ctx, (void*)p->sur_fac})
+                  (void*)p->d_contacts, (void*)p->sur_ctx, (void*)p->sur_fac, p->d_tires})
     if (q) (void)hipFree(q);
+  if (p->h_tires) (void)hipHostFree(p->h_tires);
   for (hipEvent_t e : p->tev)
     if (e) (void)hipEventDestroy(e);
   (void)hipFree(p->bed);
@@ -446,6 +453,12 @@ int silt_pair_set_surrogate(silt_pair* p, const float* params) {
 
 float silt_pair_last_tire_ms(const silt_pair* p) { return p ? p->last_tire_ms : 0.0f; }
 
+int silt_pair_set_tire_batching(silt_pair* p, int on) {
+  if (!p) return SILT_EINVAL;
+  p->tire_batching = on != 0;
+  return SILT_OK;
+}
+
 int silt_pair_step_tires(silt_pair* p, float dt, const silt_tire_push* pushes, size_t npush, size_t per_vehicle,
                          const silt_tire_contact* contacts, size_t ncontacts) {
   if (!p || (npush && (!pushes || per_vehicle == 0 || npush % per_vehicle || per_vehicle > (size_t)kMaxPush)) ||
@@ -475,21 +488,88 @@ int silt_pair_step_tires(silt_pair* p, float dt, const silt_tire_push* pushes, s
   // Vehicles: groups of per_vehicle pushes.
   const size_t groups = npush ? npush / per_vehicle : 0;
   std::vector<PushList> lists(groups);
-  std::vector<SurRegion> regions(groups);
-  std::vector<size_t> offset(groups + 1, 0);
+  std::vector<TireJob> jobs(groups);
+  std::vector<CorePush> flat(npush);
+  size_t cells = 0;
+  int max_w = 0, max_h = 0;
   for (size_t v = 0; v < groups; v++) {
     PushList& pl = lists[v];
     pl.n = (int)per_vehicle;
     for (size_t k = 0; k < per_vehicle; k++) {
       const silt_tire_push& q = pushes[v * per_vehicle + k];
       pl.p[k] = CorePush{q.x, q.z, q.forward_x, q.forward_z, q.radius, q.width, q.speed, q.slip};
+      flat[v * per_vehicle + k] = pl.p[k];
     }
-    regions[v] = p->net_ready ? sur_region_at(pl.p, pl.n, p->origin[0], p->origin[1], p->cell, res) : SurRegion{};
-    offset[v + 1] = offset[v] + (size_t)regions[v].nx * regions[v].nz;
+    TireJob& J = jobs[v];
+    J.r = p->net_ready ? sur_region_at(pl.p, pl.n, p->origin[0], p->origin[1], p->cell, res) : SurRegion{};
+    J.row = (int)cells;
+    cells += (size_t)J.r.nx * J.r.nz;
+    sur_mean_velocity(pl.p, pl.n, &J.vtx, &J.vtz);
+    // The cells push_on (src/fluid_core.rs:8152-8239) can reach around the
+    // vehicle's pushes.
+    float x0 = 1e30f, x1 = -1e30f, z0 = 1e30f, z1 = -1e30f;
+    for (int k = 0; k < pl.n; k++) {
+      const CorePush& q = pl.p[k];
+      const float reach = q.radius + q.width + 0.5f;
+      x0 = fminf(x0, q.x - reach);
+      x1 = fmaxf(x1, q.x + reach);
+      z0 = fminf(z0, q.z - reach);
+      z1 = fmaxf(z1, q.z + reach);
+    }
+    int i0 = std::max(0, (int)floorf((x0 - p->origin[0]) / p->cell));
+    int j0 = std::max(0, (int)floorf((z0 - p->origin[1]) / p->cell));
+    int i1 = std::min(res, (int)ceilf((x1 - p->origin[0]) / p->cell) + 1);
+    int j1 = std::min(res, (int)ceilf((z1 - p->origin[1]) / p->cell) + 1);
+    J.push = PushBox{(int)(v * per_vehicle), pl.n, i0, j0, std::max(i1 - i0, 0), std::max(j1 - j0, 0)};
+    max_w = std::max(max_w, J.push.w);
+    max_h = std::max(max_h, J.push.h);
+  }
+  // Vehicles whose network regions and push boxes are pairwise disjoint run
+  // every stage in one launch; otherwise one vehicle after another, in order.
+  auto apart = [](int a0, int an, int b0, int bn) { return a0 + an <= b0 || b0 + bn <= a0; };
+  bool batched = p->tire_batching && groups > 1;
+  for (size_t a = 0; a < groups && batched; a++)
+    for (size_t b = a + 1; b < groups && batched; b++) {
+      const TireJob &A = jobs[a], &B = jobs[b];
+      const bool r_ok = A.r.nx == 0 || B.r.nx == 0 || apart(A.r.i0, A.r.nx, B.r.i0, B.r.nx) ||
+                        apart(A.r.j0, A.r.nz, B.r.j0, B.r.nz);
+      const bool p_ok = A.push.w == 0 || A.push.h == 0 || B.push.w == 0 || B.push.h == 0 ||
+                        apart(A.push.i0, A.push.w, B.push.i0, B.push.w) ||
+                        apart(A.push.j0, A.push.h, B.push.j0, B.push.h);
+      batched = r_ok && p_ok;
+    }
+  const TireJob* d_jobs = nullptr;
+  const PushBox* d_boxes = nullptr;
+  const CorePush* d_push = nullptr;
+  if (batched) {
+    // Table: the jobs, their push boxes, the pushes (256-byte aligned parts).
+    const size_t jb = (groups * sizeof(TireJob) + 255) & ~(size_t)255;
+    const size_t bb = (groups * sizeof(PushBox) + 255) & ~(size_t)255;
+    const size_t bytes = jb + bb + npush * sizeof(CorePush);
+    if (p->cap_tires < bytes) {
+      HIPCHK(hipStreamSynchronize(st));
+      if (p->d_tires) (void)hipFree(p->d_tires);
+      if (p->h_tires) (void)hipHostFree(p->h_tires);
+      p->d_tires = p->h_tires = nullptr;
+      p->cap_tires = 0;
+      if (hipMalloc(&p->d_tires, bytes) != hipSuccess ||
+          hipHostMalloc(&p->h_tires, bytes, hipHostMallocDefault) != hipSuccess)
+        return SILT_ENOMEM;
+      p->cap_tires = bytes;
+    }
+    // The previous call's copy has completed: every step ends with a wait.
+    char* h = static_cast<char*>(p->h_tires);
+    memcpy(h, jobs.data(), groups * sizeof(TireJob));
+    for (size_t v = 0; v < groups; v++) reinterpret_cast<PushBox*>(h + jb)[v] = jobs[v].push;
+    memcpy(h + jb + bb, flat.data(), npush * sizeof(CorePush));
+    HIPCHK(hipMemcpyAsync(p->d_tires, p->h_tires, bytes, hipMemcpyHostToDevice, st));
+    const char* d = static_cast<const char*>(p->d_tires);
+    d_jobs = reinterpret_cast<const TireJob*>(d);
+    d_boxes = reinterpret_cast<const PushBox*>(d + jb);
+    d_push = reinterpret_cast<const CorePush*>(d + jb + bb);
   }
   const bool timed = p->tev[0] != nullptr;
   if (timed) HIPCHK(hipEventRecord(p->tev[0], st));
-  const size_t cells = offset[groups];
   float** wl = p->planes[0][p->cur[0]];
   float** sl = p->planes[1][p->cur[1]];
   if (p->net_ready && cells) {
@@ -507,32 +587,29 @@ int silt_pair_step_tires(silt_pair* p, float dt, const silt_tire_push* pushes, s
     int rc = mlp_reserve(p->net, cells);
     if (rc) return rc;
     const SurGrid g{wl[0], sl[0], wl[1], wl[2], p->bed, res, p->cell, p->origin[0], p->origin[1]};
-    for (size_t v = 0; v < groups; v++) {
-      if (!regions[v].nx) continue;
-      float* ctx = p->sur_ctx + v * kSurCtxFloats;
-      sur_context(g, regions[v], lists[v], ctx, st);
-      sur_gather_region(ctx, regions[v], p->net.xb + offset[v] * kMlpIn, st);
+    if (batched) {
+      sur_context_many(g, d_jobs, d_push, (int)groups, p->sur_ctx, st);
+      sur_gather_many(p->sur_ctx, d_jobs, (int)groups, p->net.xb, st);
+    } else {
+      for (size_t v = 0; v < groups; v++) {
+        if (!jobs[v].r.nx) continue;
+        float* ctx = p->sur_ctx + v * kSurCtxFloats;
+        sur_context(g, jobs[v].r, lists[v], ctx, st);
+        sur_gather_region(ctx, jobs[v].r, p->net.xb + (size_t)jobs[v].row * kMlpIn, st);
+      }
     }
     mlp_forward(p->net, cells);
   }
   // The pushes shove both layers around each vehicle (push_on, src/fluid_core.rs:8152-8239).
   const GridGeo geo{res, p->cell, p->origin[0], p->origin[1]};
-  for (size_t v = 0; v < groups; v++) {
-    float x0 = 1e30f, x1 = -1e30f, z0 = 1e30f, z1 = -1e30f;
-    for (int k = 0; k < lists[v].n; k++) {
-      const CorePush& q = lists[v].p[k];
-      const float reach = q.radius + q.width + 0.5f;
-      x0 = fminf(x0, q.x - reach);
-      x1 = fmaxf(x1, q.x + reach);
-      z0 = fminf(z0, q.z - reach);
-      z1 = fmaxf(z1, q.z + reach);
+  if (batched) {
+    core_push_on_many(geo, wl[0], sl[0], wl[1], wl[2], sl[1], sl[2], d_boxes, d_push, (int)groups, max_w, max_h, st);
+  } else {
+    for (size_t v = 0; v < groups; v++) {
+      const PushBox& b = jobs[v].push;
+      if (b.w > 0 && b.h > 0)
+        core_push_on(geo, wl[0], sl[0], wl[1], wl[2], sl[1], sl[2], lists[v], b.i0, b.j0, b.w, b.h, st);
     }
-    int i0 = std::max(0, (int)floorf((x0 - p->origin[0]) / p->cell));
-    int j0 = std::max(0, (int)floorf((z0 - p->origin[1]) / p->cell));
-    int i1 = std::min(res, (int)ceilf((x1 - p->origin[0]) / p->cell) + 1);
-    int j1 = std::min(res, (int)ceilf((z1 - p->origin[1]) / p->cell) + 1);
-    if (i1 <= i0 || j1 <= j0) continue;
-    core_push_on(geo, wl[0], sl[0], wl[1], wl[2], sl[1], sl[2], lists[v], i0, j0, i1 - i0, j1 - j0, st);
   }
   if (timed) HIPCHK(hipEventRecord(p->tev[1], st));
   // Fresh statistics for the plan (the shove changed speeds, the last
@@ -547,9 +624,14 @@ int silt_pair_step_tires(silt_pair* p, float dt, const silt_tire_push* pushes, s
   if (p->net_ready && cells) {
     float* wh = p->planes[0][p->cur[0]][0];
     float* sh = p->planes[1][p->cur[1]][0];
-    for (size_t v = 0; v < groups; v++) {
-      if (!regions[v].nx) continue;
-      sur_apply(wh, sh, res, p->net.y + offset[v] * kMlpOut, regions[v], p->sur_fac + v * 2 * kSurMax * kSurMax, st);
+    if (batched) {
+      sur_apply_many(wh, sh, res, p->net.y, d_jobs, (int)groups, p->sur_fac, st);
+    } else {
+      for (size_t v = 0; v < groups; v++) {
+        if (!jobs[v].r.nx) continue;
+        sur_apply(wh, sh, res, p->net.y + (size_t)jobs[v].row * kMlpOut, jobs[v].r,
+                  p->sur_fac + v * 2 * kSurMax * kSurMax, st);
+      }
     }
   }
   if (timed) {

@@ -31,6 +31,12 @@ struct PushList {
   CorePush p[kMaxPush];
 };
 
+// A group of pushes (p0 .. p0 + np of a push array) and the cells it shoves.
+struct PushBox {
+  int p0, np;
+  int i0, j0, w, h;
+};
+
 // SprayEvent (src/fluid_core.rs:371-382).
 struct SprayEvent {
   float x, y, z, vx, vy, vz, volume;
@@ -78,6 +84,10 @@ struct NearTable {
 
 void core_push_on(const GridGeo& g, float* water, float* slurry, float* uw, float* vw, float* us, float* vs,
                   const PushList& pl, int i0, int j0, int w, int h, hipStream_t st);
+// push_on of n groups whose boxes do not overlap, in one launch (boxes and
+// pushes in device memory; max_w / max_h bound the boxes).
+void core_push_on_many(const GridGeo& g, float* water, float* slurry, float* uw, float* vw, float* us, float* vs,
+                       const PushBox* boxes, const CorePush* pushes, int n, int max_w, int max_h, hipStream_t st);
 void core_peel(const GridGeo& g, float* water, const float* uw, const float* vw, const PushList& pl, int i0, int j0,
                int w, int h, const uint8_t* hold, const GridGeo& cg, bool skip_held, SprayEvent* ev, unsigned* nev,
                unsigned cap, double* vol, hipStream_t st);

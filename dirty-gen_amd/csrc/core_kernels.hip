@@ -24,16 +24,14 @@ __device__ inline int span_hi(float origin, float cell, int res, float center, f
   return h > lo ? h : lo;
 }
 
-// push_on (src/fluid_core.rs:8169-8239): every push, in order, on one cell.
-__global__ void k_push_on(GridGeo g, float* water, float* slurry, float* uw, float* vw, float* us, float* vs,
-                          PushList pl, int i0, int j0, int w, int h) {
-  int i = i0 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  int j = j0 + (int)(blockIdx.y * blockDim.y + threadIdx.y);
-  if (i >= i0 + w || j >= j0 + h) return;
+// push_on (src/fluid_core.rs:8169-8239): pushes p[0..n), in order, on cell
+// (i, j).
+__device__ inline void push_on_cell(const GridGeo& g, float* water, float* slurry, float* uw, float* vw, float* us,
+                                    float* vs, const CorePush* pushes, int n, int i, int j) {
   size_t idx = (size_t)j * g.res + i;
   const float cell = g.cell;
-  for (int q = 0; q < pl.n; q++) {
-    const CorePush& p = pl.p[q];
+  for (int q = 0; q < n; q++) {
+    const CorePush& p = pushes[q];
     float len = fmaxf(sqrtf(p.fx * p.fx + p.fz * p.fz), 1.0e-4f);
     float fx = p.fx / len, fz = p.fz / len;
     float rx = fz, rz = -fx;
@@ -66,6 +64,24 @@ __global__ void k_push_on(GridGeo g, float* water, float* slurry, float* uw, flo
       vs[idx] = vs[idx] + rz * sg * shove * 0.28f * flank;
     }
   }
+}
+
+__global__ void k_push_on(GridGeo g, float* water, float* slurry, float* uw, float* vw, float* us, float* vs,
+                          PushList pl, int i0, int j0, int w, int h) {
+  int i = i0 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  int j = j0 + (int)(blockIdx.y * blockDim.y + threadIdx.y);
+  if (i >= i0 + w || j >= j0 + h) return;
+  push_on_cell(g, water, slurry, uw, vw, us, vs, pl.p, pl.n, i, j);
+}
+
+// Many groups of pushes on disjoint boxes, one group per blockIdx.z.
+__global__ void k_push_on_many(GridGeo g, float* water, float* slurry, float* uw, float* vw, float* us, float* vs,
+                               const PushBox* __restrict__ boxes, const CorePush* __restrict__ pushes) {
+  const PushBox b = boxes[blockIdx.z];
+  int i = b.i0 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  int j = b.j0 + (int)(blockIdx.y * blockDim.y + threadIdx.y);
+  if (i >= b.i0 + b.w || j >= b.j0 + b.h) return;
+  push_on_cell(g, water, slurry, uw, vw, us, vs, pushes + b.p0, b.np, i, j);
 }
 
 // peel_spray (src/fluid_core.rs:8241-8314) on one cell, every push in order.
@@ -535,6 +551,15 @@ void core_push_on(const GridGeo& g, float* water, float* slurry, float* uw, floa
   if (w <= 0 || h <= 0 || pl.n <= 0) return;
   dim3 b(32, 8);
   hipLaunchKernelGGL(k_push_on, grid2(w, h, b), b, 0, st, g, water, slurry, uw, vw, us, vs, pl, i0, j0, w, h);
+}
+
+void core_push_on_many(const GridGeo& g, float* water, float* slurry, float* uw, float* vw, float* us, float* vs,
+                       const PushBox* boxes, const CorePush* pushes, int n, int max_w, int max_h, hipStream_t st) {
+  if (n <= 0 || max_w <= 0 || max_h <= 0) return;
+  dim3 b(32, 8);
+  dim3 gr = grid2(max_w, max_h, b);
+  gr.z = (unsigned)n;
+  hipLaunchKernelGGL(k_push_on_many, gr, b, 0, st, g, water, slurry, uw, vw, us, vs, boxes, pushes);
 }
 
 void core_peel(const GridGeo& g, float* water, const float* uw, const float* vw, const PushList& pl, int i0, int j0,

@@ -14,9 +14,9 @@ namespace {
 constexpr float kContactLength = 0.30f;
 constexpr float kWheelLoad = 3500.0f;
 
-__global__ void k_sur_context(SurGrid c, SurRegion r, PushList pl, float vtx, float vtz, float* __restrict__ ctx) {
-  const int a = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  if (a >= kSurCtx * kSurCtx) return;
+// Context cell a of region r for pushes p[0..n) with mean velocity (vtx, vtz).
+__device__ inline void sur_context_cell(const SurGrid& c, const SurRegion& r, const CorePush* pushes, int n, float vtx,
+                                        float vtz, float* __restrict__ ctx, int a) {
   const int ca = a % kSurCtx, cb = a / kSurCtx;
   const int i = r.i0 - kSurReach + ca, j = r.j0 - kSurReach + cb;
   const size_t plane = (size_t)kSurCtx * kSurCtx;
@@ -32,8 +32,8 @@ __global__ void k_sur_context(SurGrid c, SurRegion r, PushList pl, float vtx, fl
     v[4] = wet ? (c.vw[idx] - vtz) * 0.125f : 0.0f;
     float p = 0.0f;
     const float x0 = c.ox + (float)i * c.cell, z0 = c.oz + (float)j * c.cell;
-    for (int t = 0; t < pl.n; t++) {
-      const CorePush q = pl.p[t];
+    for (int t = 0; t < n; t++) {
+      const CorePush q = pushes[t];
       const float fl = sqrtf(q.fx * q.fx + q.fz * q.fz);
       if (!(fl > 0.0f)) continue;
       const float fx = q.fx / fl, fz = q.fz / fl;
@@ -53,6 +53,22 @@ __global__ void k_sur_context(SurGrid c, SurRegion r, PushList pl, float vtx, fl
     v[6] = 1.0f;
   }
   for (int ch = 0; ch < kSurChannels; ch++) ctx[(size_t)ch * plane + a] = v[ch];
+}
+
+__global__ void k_sur_context(SurGrid c, SurRegion r, PushList pl, float vtx, float vtz, float* __restrict__ ctx) {
+  const int a = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (a >= kSurCtx * kSurCtx) return;
+  sur_context_cell(c, r, pl.p, pl.n, vtx, vtz, ctx, a);
+}
+
+// One vehicle per blockIdx.y.
+__global__ void k_sur_context_many(SurGrid c, const TireJob* __restrict__ jobs, const CorePush* __restrict__ pushes,
+                                   float* __restrict__ ctx) {
+  const int a = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  const TireJob J = jobs[blockIdx.y];
+  if (a >= kSurCtx * kSurCtx || J.r.nx == 0) return;
+  sur_context_cell(c, J.r, pushes + J.push.p0, J.push.np, J.vtx, J.vtz, ctx + (size_t)blockIdx.y * kSurCtxFloats,
+                   a);
 }
 
 // The square's symmetries as x' = M x, M = {m00, m01, m10, m11}.
@@ -96,6 +112,17 @@ __global__ void k_sur_gather_region(const float* __restrict__ ctx, SurRegion r, 
   if (t >= cells * kMlpIn) return;
   const int q = (int)(t / kMlpIn), k = (int)(t % kMlpIn);
   xb[t] = f2bf(sur_input(ctx, q % r.nx, q / r.nx, k, 0));
+}
+
+__global__ void k_sur_gather_many(const float* __restrict__ ctx, const TireJob* __restrict__ jobs,
+                                  uint16_t* __restrict__ xb) {
+  const TireJob J = jobs[blockIdx.y];
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t cells = (size_t)J.r.nx * J.r.nz;
+  if (t >= cells * kMlpIn) return;
+  const int q = (int)(t / kMlpIn), k = (int)(t % kMlpIn);
+  xb[(size_t)J.row * kMlpIn + t] =
+      f2bf(sur_input(ctx + (size_t)blockIdx.y * kSurCtxFloats, q % J.r.nx, q / J.r.nx, k, 0));
 }
 
 __device__ inline uint32_t mix32(uint32_t x) {
@@ -261,6 +288,46 @@ __global__ void k_sur_apply(float* __restrict__ water, float* __restrict__ slurr
   h[idx] = h[idx] - net;
 }
 
+__global__ void k_sur_factor_many(const float* __restrict__ water, const float* __restrict__ slurry, int res,
+                                  const float* __restrict__ y, const TireJob* __restrict__ jobs,
+                                  float* __restrict__ fac) {
+  const TireJob J = jobs[blockIdx.y];
+  const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  const int cells = J.r.nx * J.r.nz;
+  if (t >= 2 * cells) return;
+  const int layer = t / cells, q = t % cells;
+  const SurRegion r = J.r;
+  const float* yv = y + (size_t)J.row * kMlpOut;
+  const size_t idx = (size_t)(r.j0 + q / r.nx) * res + (r.i0 + q % r.nx);
+  const float h = layer ? slurry[idx] : water[idx];
+  float out = 0.0f;
+  for (int dir = 0; dir < 4; dir++) out += fmaxf(sur_face(yv, r, q, dir, layer), 0.0f);
+  fac[(size_t)blockIdx.y * 2 * kSurMax * kSurMax + t] = out > h ? 0.999999f * h / out : 1.0f;
+}
+
+__global__ void k_sur_apply_many(float* __restrict__ water, float* __restrict__ slurry, int res,
+                                 const float* __restrict__ y, const TireJob* __restrict__ jobs,
+                                 const float* __restrict__ fac_all) {
+  const TireJob J = jobs[blockIdx.y];
+  const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  const int cells = J.r.nx * J.r.nz;
+  if (t >= 2 * cells) return;
+  const int layer = t / cells, q = t % cells;
+  const SurRegion r = J.r;
+  const float* yv = y + (size_t)J.row * kMlpOut;
+  const float* fac = fac_all + (size_t)blockIdx.y * 2 * kSurMax * kSurMax;
+  const size_t idx = (size_t)(r.j0 + q / r.nx) * res + (r.i0 + q % r.nx);
+  float* h = layer ? slurry : water;
+  float net = 0.0f;
+  for (int dir = 0; dir < 4; dir++) {
+    const float f = sur_face(yv, r, q, dir, layer);
+    if (f == 0.0f) continue;
+    const int nb = sur_nb(r, q, dir);
+    net += f * (f > 0.0f ? fac[layer * cells + q] : fac[layer * cells + nb]);
+  }
+  h[idx] = h[idx] - net;
+}
+
 __global__ void k_sur_max_abs_diff(const float* __restrict__ a, const float* __restrict__ b, int n,
                                    unsigned* out) {
   float m = 0.0f;
@@ -312,20 +379,50 @@ SurRegion sur_region_at(const CorePush* p, int n, float ox, float oz, float cell
 }
 
 void sur_context(const SurGrid& c, const SurRegion& r, const PushList& pl, float* ctx, hipStream_t st) {
-  float vx = 0.0f, vz = 0.0f;
-  for (int t = 0; t < pl.n; t++) {
-    const float fl = sqrtf(pl.p[t].fx * pl.p[t].fx + pl.p[t].fz * pl.p[t].fz);
-    if (fl > 0.0f) {
-      vx += pl.p[t].speed * pl.p[t].fx / fl;
-      vz += pl.p[t].speed * pl.p[t].fz / fl;
-    }
-  }
-  if (pl.n) {
-    vx /= (float)pl.n;
-    vz /= (float)pl.n;
-  }
+  float vx, vz;
+  sur_mean_velocity(pl.p, pl.n, &vx, &vz);
   const int n = kSurCtx * kSurCtx;
   hipLaunchKernelGGL(k_sur_context, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, c, r, pl, vx, vz, ctx);
+}
+
+void sur_mean_velocity(const CorePush* p, int n, float* vx, float* vz) {
+  float ax = 0.0f, az = 0.0f;
+  for (int t = 0; t < n; t++) {
+    const float fl = sqrtf(p[t].fx * p[t].fx + p[t].fz * p[t].fz);
+    if (fl > 0.0f) {
+      ax += p[t].speed * p[t].fx / fl;
+      az += p[t].speed * p[t].fz / fl;
+    }
+  }
+  if (n) {
+    ax /= (float)n;
+    az /= (float)n;
+  }
+  *vx = ax;
+  *vz = az;
+}
+
+void sur_context_many(const SurGrid& g, const TireJob* jobs, const CorePush* pushes, int n, float* ctx,
+                      hipStream_t st) {
+  if (n <= 0) return;
+  const int cells = kSurCtx * kSurCtx;
+  hipLaunchKernelGGL(k_sur_context_many, dim3((unsigned)((cells + 255) / 256), (unsigned)n), dim3(256), 0, st, g, jobs,
+                     pushes, ctx);
+}
+
+void sur_gather_many(const float* ctx, const TireJob* jobs, int n, uint16_t* xb, hipStream_t st) {
+  if (n <= 0) return;
+  const size_t most = (size_t)kSurMax * kSurMax * kMlpIn;
+  hipLaunchKernelGGL(k_sur_gather_many, dim3((unsigned)((most + 255) / 256), (unsigned)n), dim3(256), 0, st, ctx,
+                     jobs, xb);
+}
+
+void sur_apply_many(float* water, float* slurry, int res, const float* y, const TireJob* jobs, int n, float* fac,
+                    hipStream_t st) {
+  if (n <= 0) return;
+  const dim3 grid((unsigned)((2 * kSurMax * kSurMax + 255) / 256), (unsigned)n);
+  hipLaunchKernelGGL(k_sur_factor_many, grid, dim3(256), 0, st, water, slurry, res, y, jobs, fac);
+  hipLaunchKernelGGL(k_sur_apply_many, grid, dim3(256), 0, st, water, slurry, res, y, jobs, fac);
 }
 
 void sur_gather_region(const float* ctx, const SurRegion& r, uint16_t* xb, hipStream_t st) {

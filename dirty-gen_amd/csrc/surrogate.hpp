@@ -69,6 +69,24 @@ void sur_targets(const float* hn_w, const float* hc_w, const float* hn_s, const 
 // no donor gives more than it holds.  fac: scratch of 2 * nx * nz floats.
 void sur_apply(float* water, float* slurry, int res, const float* y, const SurRegion& r, float* fac,
                hipStream_t st);
+// Many vehicles in one launch per stage: vehicle v's region, its first row
+// of network inputs / outputs, its pushes and their mean velocity, and the
+// box its pushes shove.  Regions (and push boxes) must not overlap.  The
+// contexts of vehicle v sit at ctx + v * kSurCtxFloats, its factor scratch
+// at fac + v * 2 * kSurMax^2.
+struct TireJob {
+  SurRegion r;
+  int row;
+  float vtx, vtz;
+  PushBox push;
+};
+// Mean velocity of pushes p[0..n) along their headings.
+void sur_mean_velocity(const CorePush* p, int n, float* vx, float* vz);
+void sur_context_many(const SurGrid& g, const TireJob* jobs, const CorePush* pushes, int n, float* ctx,
+                      hipStream_t st);
+void sur_gather_many(const float* ctx, const TireJob* jobs, int n, uint16_t* xb, hipStream_t st);
+void sur_apply_many(float* water, float* slurry, int res, const float* y, const TireJob* jobs, int n, float* fac,
+                    hipStream_t st);
 // Sum over the region of (a - b)^2 added into *out (FP64).
 void sur_region_sq_diff(const float* a, const float* b, int res, const SurRegion& r, double* out, hipStream_t st);
 // Largest |a - b| over n cells, as float bits, max-ed into *out.

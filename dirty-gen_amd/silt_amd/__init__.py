@@ -361,6 +361,7 @@ class SheetPair:
             L.silt_pair_step_tires.argtypes = [_P, _F, C.POINTER(TirePush), _Z, _Z, C.POINTER(TireContact), _Z]
             L.silt_pair_last_tire_ms.argtypes = [_P]
             L.silt_pair_last_tire_ms.restype = _F
+            L.silt_pair_set_tire_batching.argtypes = [_P, C.c_int]
             L._pair_tires_declared = True
         return L
 
@@ -389,6 +390,11 @@ class SheetPair:
 
     def last_tire_ms(self):
         return self._tires_declared().silt_pair_last_tire_ms(self._h)
+
+    def set_tire_batching(self, on):
+        """One launch per tire stage for all vehicles (default) or one per
+        vehicle; the results are the same."""
+        _check_status(self._tires_declared().silt_pair_set_tire_batching(self._h, 1 if on else 0))
 
     def autotune(self, dt, rows=None, reps=5):
         """Time candidate rows per workgroup of the fused kernel on this

@@ -107,6 +107,11 @@ int silt_pair_step_tires(silt_pair *p, float dt, const silt_tire_push *pushes, s
 /* GPU time of the last silt_pair_step_tires' tire work (network inputs and
  * forward, shoves, flux apply), milliseconds; 0 before a network is set. */
 float silt_pair_last_tire_ms(const silt_pair *p);
+/* Launch shape of the tire work, no counterpart in the reference: on (the
+ * default), vehicles whose network regions and push boxes do not overlap
+ * run each stage in one launch; off, one vehicle after another.  The results
+ * are the same either way. */
+int silt_pair_set_tire_batching(silt_pair *p, int on);
 
 #ifdef __cplusplus
 }

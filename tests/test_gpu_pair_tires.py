@@ -72,3 +72,28 @@ def test_ruts_follow_the_oracle_and_the_ledger_closes_with_the_network():
         assert h[0].min() >= 0.0 and h[3].min() >= 0.0
     finally:
         p.close()
+
+
+def test_one_launch_per_stage_equals_one_vehicle_after_another():
+    """Vehicles apart: each tire stage runs for every vehicle in one launch;
+    one vehicle after another must give the same planes bit for bit."""
+    bed, water, slurry = _scene(4)
+    m = S.Mlp(12)
+    params = m.params()
+    m.close()
+    pairs = [_pair(bed, water, slurry), _pair(bed, water, slurry)]
+    fleets = [D.Fleet(seed=8, n=4, extent=RES * CELL, half=5.0) for _ in pairs]
+    try:
+        for p, on in zip(pairs, (True, False)):
+            p.set_surrogate(params)
+            p.set_tire_batching(on)
+        for k in range(10):
+            for p, f in zip(pairs, fleets):
+                p.step_tires(DT, f.pushes(), 4, f.contacts())
+                f.advance(DT)
+            for x, y in zip(pairs[0].download(), pairs[1].download()):
+                assert np.array_equal(x, y), k
+        assert pairs[0].last_tire_ms() > 0.0
+    finally:
+        for p in pairs:
+            p.close()
