@@ -255,58 +255,63 @@ int silt_pair_step(silt_pair* p, float dt) {
       if (!reference_uses_jacobi(b, p->cell, 0, true)) return false;
     return true;
   };
-  std::vector<FusedLayer> joint;
+  // The layers are independent: substep k of every fused layer that has one
+  // shares a launch (a layer of one substep rides with the first substep of
+  // the other).
+  int fused[2], nfused = 0, most = 0;
+  bool staged = false;
   for (int l = 0; l < 2; l++) {
     p->last_nsub[l] = plans[l].boxes.empty() ? 0 : plans[l].nsub;
     if (plans[l].boxes.empty()) continue;
-    silt_drive d{SILT_DRIVE_GRAVITY, 0.0f, 0.0f, p->vort[l]};
-    float dts = dt / (float)plans[l].nsub;
     if (fused_ok(l)) {
-      for (int k = 0; k < plans[l].nsub; k++) {
-        FusedLayer f{};
-        f.in = layer_planes(p, l, p->cur[l]);
-        int nb = 1 - p->cur[l];
-        f.ho = p->planes[l][nb][0];
-        f.uo = p->planes[l][nb][1];
-        f.vo = p->planes[l][nb][2];
-        f.a = make_args(res, p->cell, dts, d, kEps, 0, true, layer_nu_uniform(p, l), layer_nu0(p, l));
-        f.boxes = plans[l].boxes;
-        f.stats = p->stats + l;
-        f.totals = p->totals + l;
-        if (plans[l].nsub == 1) {
-          joint.push_back(f);
-          p->cur[l] = nb;
-        } else {
-          stats_reset(p->stats + l, 1, st);
-          int rc = fused_substep_layers(&f, 1, p->bed, res, p->path == SILT_PATH_FUSED_EXACT, st);
-          if (rc != 0) return rc;
-          p->cur[l] = nb;
-        }
-      }
-    } else {
-      // Stage kernels in place on the current planes (Gauss-Seidel boxes,
-      // overlapping boxes, or the exact path).
-      Workspace* ws = workspace_for(res);
-      if (!ws) return SILT_ENOMEM;
-      SheetPlanes s = ws->planes();
-      SheetPlanes cur = layer_planes(p, l, p->cur[l]);
-      s.h = cur.h;
-      s.u = cur.u;
-      s.v = cur.v;
-      s.bed = p->bed;
-      s.nu = cur.nu ? cur.nu : s.nu;
-      o.path = SILT_PATH_EXACT;
-      int rc = run_plan(s, plans[l], res, p->cell, dt, d, o, true, layer_nu_uniform(p, l), layer_nu0(p, l), st);
-      if (rc < 0) return rc;
-      o.path = p->path;
-      stats_compute(cur.h, cur.u, cur.v, res, kEps, p->stats + l, st);
+      fused[nfused++] = l;
+      most = std::max(most, plans[l].nsub);
+      continue;
     }
+    // Stage kernels in place on the current planes (Gauss-Seidel boxes,
+    // overlapping boxes, or the exact path).
+    silt_drive d{SILT_DRIVE_GRAVITY, 0.0f, 0.0f, p->vort[l]};
+    Workspace* ws = workspace_for(res);
+    if (!ws) return SILT_ENOMEM;
+    SheetPlanes s = ws->planes();
+    SheetPlanes cur = layer_planes(p, l, p->cur[l]);
+    s.h = cur.h;
+    s.u = cur.u;
+    s.v = cur.v;
+    s.bed = p->bed;
+    s.nu = cur.nu ? cur.nu : s.nu;
+    o.path = SILT_PATH_EXACT;
+    int rc = run_plan(s, plans[l], res, p->cell, dt, d, o, true, layer_nu_uniform(p, l), layer_nu0(p, l), st);
+    if (rc < 0) return rc;
+    o.path = p->path;
+    stats_compute(cur.h, cur.u, cur.v, res, kEps, p->stats + l, st);
+    staged = true;
   }
-  if (!joint.empty()) {
-    for (const FusedLayer& f : joint) stats_reset(f.stats, 1, st);
-    // Time the joint launch alone when it is the step's only kernel work.
-    if (joint.size() == 2) HIPCHK(hipEventRecord(p->ev0, st));
-    int rc = fused_substep_layers(joint.data(), (int)joint.size(), p->bed, res, p->path == SILT_PATH_FUSED_EXACT, st);
+  // Time the fused launches alone when they are the step's only kernel work.
+  if (!staged && nfused) HIPCHK(hipEventRecord(p->ev0, st));
+  for (int k = 0; k < most; k++) {
+    FusedLayer launch[2];
+    int m = 0;
+    for (int q = 0; q < nfused; q++) {
+      const int l = fused[q];
+      if (plans[l].nsub <= k) continue;
+      silt_drive d{SILT_DRIVE_GRAVITY, 0.0f, 0.0f, p->vort[l]};
+      FusedLayer& f = launch[m++];
+      f = FusedLayer{};
+      f.in = layer_planes(p, l, p->cur[l]);
+      const int nb = 1 - p->cur[l];
+      f.ho = p->planes[l][nb][0];
+      f.uo = p->planes[l][nb][1];
+      f.vo = p->planes[l][nb][2];
+      f.a = make_args(res, p->cell, dt / (float)plans[l].nsub, d, kEps, 0, true, layer_nu_uniform(p, l),
+                      layer_nu0(p, l));
+      f.boxes = plans[l].boxes;
+      f.stats = p->stats + l;
+      f.totals = p->totals + l;
+      stats_reset(f.stats, 1, st);
+      p->cur[l] = nb;
+    }
+    int rc = fused_substep_layers(launch, m, p->bed, res, p->path == SILT_PATH_FUSED_EXACT, st);
     if (rc != 0) return rc;
   }
   HIPCHK(hipEventRecord(p->ev1, st));
@@ -376,11 +381,12 @@ int silt_pair_autotune(silt_pair* p, float dt, const int* rows, size_t n, int re
                   hipMemcpyAsync(totals0, p->totals, sizeof totals0, hipMemcpyDeviceToHost, st) != hipSuccess ||
                   hipStreamSynchronize(st) != hipSuccess))
     rc = SILT_EDEVICE;
-  const int pinned0 = fused_pinned_chunk_rows(p->res);
+  // Heights are tuned for the joint launch of both layers.
+  const int pinned0 = fused_pinned_chunk_rows(p->res, 2);
   float best = INFINITY;
   int best_c = pinned0;
   for (size_t c = 0; c < cand.size() && rc == 0; c++) {
-    fused_set_chunk_rows_for(p->res, cand[c]);
+    fused_set_chunk_rows_for(p->res, cand[c], 2);
     rc = silt_pair_step(p, dt);  // warm-up of this height
     double sum = 0.0;
     for (int k = 0; k < reps && rc >= 0; k++) {
@@ -397,7 +403,7 @@ int silt_pair_autotune(silt_pair* p, float dt, const int* rows, size_t n, int re
       best_c = cand[c];
     }
   }
-  fused_set_chunk_rows_for(p->res, rc == 0 ? best_c : pinned0);
+  fused_set_chunk_rows_for(p->res, rc == 0 ? best_c : pinned0, 2);
   (void)hipFree(save);
   if (rc < 0) return rc;
   if (best_rows) *best_rows = best_c;
@@ -480,10 +486,10 @@ int silt_pair_step_tires(silt_pair* p, float dt, const silt_tire_push* pushes, s
     // Contacts come in the pushes' vehicle groups; vehicles' footprints do
     // not overlap, so each vehicle stamps in its own workgroup.
     const int per = (npush && ncontacts == npush) ? (int)per_vehicle : 0;
+    // The stamp refreshes the sheet bed and the rut over its footprints.
     int rc = rut_stamp(p->soil_bed, p->soil_placed, p->soil_init, res, p->cell, p->origin[0], p->origin[1], contacts,
-                       ncontacts, dt, p->d_contacts, st, per);
+                       ncontacts, dt, p->d_contacts, st, per, p->bed, p->soil_rut);
     if (rc) return rc;
-    core_ground(p->soil_bed, p->soil_placed, p->soil_init, p->bed, p->soil_rut, (int)p->n, st);
   }
   // Vehicles: groups of per_vehicle pushes.
   const size_t groups = npush ? npush / per_vehicle : 0;

@@ -8,7 +8,7 @@
 // exponential rates are scalar per contact and are computed on the host; one
 // workgroup then stamps the contacts in order, cells of a footprint in
 // parallel, and lane 0 sums the cut volume and lays the spoil on the berm
-// band in the reference's raster order.
+// band in the reference's raster order (over lists compacted in LDS).
 #include <hip/hip_runtime.h>
 #include <math.h>
 
@@ -66,14 +66,49 @@ float contact_length(float radius, float sinkage) {
 namespace {
 
 constexpr int kMaxFoot = 64 * 64;
+constexpr int kStampLanes = 256;
+
+// Exclusive prefix sum of v over the workgroup's 256 lanes (wavefront
+// shuffles, then the four wavefront totals); *total gets the sum.
+__device__ inline int block_exclusive_scan(int v, int* wave_tot, int* total) {
+  const int lane = (int)(threadIdx.x & 63), w = (int)(threadIdx.x >> 6);
+  int x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(x, o);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) wave_tot[w] = x;
+  __syncthreads();
+  int base = 0, all = 0;
+#pragma unroll
+  for (int k = 0; k < kStampLanes / 64; k++) {
+    base += k < w ? wave_tot[k] : 0;
+    all += wave_tot[k];
+  }
+  __syncthreads();
+  *total = all;
+  return base + x - v;
+}
 
 // One workgroup applies a group of `per` consecutive contacts in order
 // (stamp_wheel per tire, src/soil.rs:2526-2879); groups whose footprints do
-// not overlap run in parallel, one workgroup each.
-__global__ __launch_bounds__(256) void k_stamp(float* bed, float* placed, const float* initial, int res, float cell,
-                                               float ox, float oz, const RutContact* cs, int nc, int per) {
+// not overlap run in parallel, one workgroup each.  The footprint's cells
+// run in parallel; the two sequential FP32 sums of the reference (the cut
+// volume, and the spoil rounds of deposit_spoil) run on lane 0 over lists
+// compacted in raster order in LDS, so they add in the reference's order.
+// With sheet_bed set, the footprint's sheet bed (bed + placed) and rut
+// (max(initial - bed, 0)) are refreshed after each contact.
+__global__ __launch_bounds__(kStampLanes) void k_stamp(float* bed, float* placed, const float* initial, int res,
+                                                       float cell, float ox, float oz, const RutContact* cs, int nc,
+                                                       int per, float* sheet_bed, float* rut) {
   __shared__ float drop_s[kMaxFoot];
   __shared__ unsigned char berm_s[kMaxFoot];
+  __shared__ float drops_c[kMaxFoot];   // positive drops, raster order
+  __shared__ short berm_k[kMaxFoot];    // berm cells, raster order
+  __shared__ float berm_p[kMaxFoot];    // their placed depth
+  __shared__ int wave_tot[kStampLanes / 64];
+  __shared__ int n_berm_s;
   const int t = (int)threadIdx.x;
   const int q0 = (int)blockIdx.x * per, q1 = min(nc, q0 + per);
   for (int q = q0; q < q1; q++) {
@@ -130,50 +165,81 @@ __global__ __launch_bounds__(256) void k_stamp(float* bed, float* placed, const 
       }
     }
     __syncthreads();
-    if (t == 0 && nfoot > 0) {
+    // Compact the positive drops and the berm cells in raster order: lane t
+    // takes a contiguous run of the footprint.
+    const int seg = (nfoot + kStampLanes - 1) / kStampLanes;
+    const int k0 = min(t * seg, nfoot), k1 = min(k0 + seg, nfoot);
+    int cd = 0, cb = 0;
+    for (int k = k0; k < k1; k++) {
+      cd += drop_s[k] > 0.0f ? 1 : 0;
+      cb += berm_s[k];
+    }
+    int nd = 0, nb = 0;
+    int pd = block_exclusive_scan(cd, wave_tot, &nd);
+    int pb = block_exclusive_scan(cb, wave_tot, &nb);
+    for (int k = k0; k < k1; k++) {
+      if (drop_s[k] > 0.0f) drops_c[pd++] = drop_s[k];
+      if (berm_s[k]) {
+        berm_k[pb] = (short)k;
+        berm_p[pb] = placed[(size_t)(c.jlo + k / W) * res + (c.ilo + k % W)];
+        pb++;
+      }
+    }
+    __syncthreads();
+    if (t == 0) {
+      bool spoil = false;
       // Cut volume in raster order, then deposit_spoil (src/soil.rs:2881-2946)
       // over the berm cells in raster order, at most six rounds to the cap.
       const float area = cell * cell;
       float vol = 0.0f;
-      int nb = 0;
-      for (int k = 0; k < nfoot; k++) {
-        if (drop_s[k] > 0.0f) vol += drop_s[k] * area;
-        nb += berm_s[k];
-      }
-      if (vol > 1.0e-8f && nb > 0) {
+      for (int m = 0; m < nd; m++) vol += drops_c[m] * area;
+      if (nfoot > 0 && vol > 1.0e-8f && nb > 0) {
+        spoil = true;
         float left = vol;
         const float cap = 0.08f;
-        // "open" is the berm list minus the cells that reached the cap.
-        for (int k = 0; k < nfoot; k++) drop_s[k] = berm_s[k] ? 1.0f : 0.0f;
+        // "open" is the berm list minus the cells that reached the cap
+        // (drop_s is free again: it holds the open flags).
+        for (int m = 0; m < nb; m++) drop_s[m] = 1.0f;
         int no = nb;
         for (int round = 0; round < 6; round++) {
           if (left <= 1.0e-8f || no == 0) break;
           float share = left / ((float)no * area);
           int nn = 0;
-          for (int k = 0; k < nfoot; k++) {
-            if (drop_s[k] == 0.0f) continue;
-            size_t idx = (size_t)(c.jlo + k / W) * res + (c.ilo + k % W);
-            float room = fmaxf(cap - placed[idx], 0.0f);
+          for (int m = 0; m < nb; m++) {
+            if (drop_s[m] == 0.0f) continue;
+            float room = fmaxf(cap - berm_p[m], 0.0f);
             float add = fminf(share, room);
             if (add > 0.0f) {
-              placed[idx] = placed[idx] + add;
+              berm_p[m] = berm_p[m] + add;
               left -= add * area;
             }
-            if (placed[idx] < cap - 1.0e-4f)
+            if (berm_p[m] < cap - 1.0e-4f)
               nn++;
             else
-              drop_s[k] = 0.0f;
+              drop_s[m] = 0.0f;
           }
           no = nn;
         }
         if (left > 1.0e-8f) {
           float add = left / ((float)nb * area);
-          for (int k = 0; k < nfoot; k++)
-            if (berm_s[k]) {
-              size_t idx = (size_t)(c.jlo + k / W) * res + (c.ilo + k % W);
-              placed[idx] = placed[idx] + add;
-            }
+          for (int m = 0; m < nb; m++) berm_p[m] = berm_p[m] + add;
         }
+      }
+      n_berm_s = spoil ? nb : 0;
+    }
+    __syncthreads();
+    const int nw = n_berm_s;
+    for (int m = t; m < nw; m += (int)blockDim.x) {
+      const int k = berm_k[m];
+      placed[(size_t)(c.jlo + k / W) * res + (c.ilo + k % W)] = berm_p[m];
+    }
+    if (sheet_bed) {
+      __syncthreads();
+      for (int k = t; k < nfoot; k += (int)blockDim.x) {
+        const size_t idx = (size_t)(c.jlo + k / W) * res + (c.ilo + k % W);
+        const float b = bed[idx];
+        sheet_bed[idx] = b + placed[idx];
+        rut[idx] = fmaxf(initial[idx] - b, 0.0f);
       }
     }
     __syncthreads();
@@ -228,7 +294,7 @@ int prepare_contacts(const silt_tire_contact* in, size_t n, int res, float cell,
 
 int rut_stamp(float* bed, float* placed, const float* initial, int res, float cell, float ox, float oz,
               const silt_tire_contact* contacts, size_t n, float dt, RutContact* d_scratch, hipStream_t st,
-              int per_group) {
+              int per_group, float* sheet_bed, float* rut) {
   std::vector<RutContact> cs;
   int rc = prepare_contacts(contacts, n, res, cell, ox, oz, dt, &cs);
   if (rc) return rc;
@@ -237,8 +303,8 @@ int rut_stamp(float* bed, float* placed, const float* initial, int res, float ce
     return SILT_EDEVICE;
   const int nc = (int)cs.size();
   const int per = per_group > 0 && per_group < nc ? per_group : nc;
-  hipLaunchKernelGGL(k_stamp, dim3((unsigned)((nc + per - 1) / per)), dim3(256), 0, st, bed, placed, initial, res,
-                     cell, ox, oz, d_scratch, nc, per);
+  hipLaunchKernelGGL(k_stamp, dim3((unsigned)((nc + per - 1) / per)), dim3(kStampLanes), 0, st, bed, placed, initial,
+                     res, cell, ox, oz, d_scratch, nc, per, sheet_bed, rut);
   return hipGetLastError() == hipSuccess ? 0 : SILT_EDEVICE;
 }
 

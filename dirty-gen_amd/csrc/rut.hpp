@@ -28,9 +28,11 @@ int prepare_contacts(const silt_tire_contact* in, size_t n, int res, float cell,
 // Stamps the contacts in order on device planes; d_scratch holds >= n
 // contacts.  per_group > 0: consecutive groups of that many contacts whose
 // footprints do not overlap each other are stamped in parallel (in order
-// within a group); 0: all in one sequence.
+// within a group); 0: all in one sequence.  sheet_bed / rut set: the
+// footprints' sheet bed (bed + placed) and rut (max(initial - bed, 0)) are
+// refreshed too, so the caller needs no full-plane pass.
 int rut_stamp(float* bed, float* placed, const float* initial, int res, float cell, float ox, float oz,
               const silt_tire_contact* contacts, size_t n, float dt, RutContact* d_scratch, hipStream_t st,
-              int per_group = 0);
+              int per_group = 0, float* sheet_bed = nullptr, float* rut = nullptr);
 
 }  // namespace silt

@@ -52,6 +52,7 @@
 #include <string.h>
 #include <mutex>
 #include <type_traits>
+#include <utility>
 #include <vector>
 
 namespace silt {
@@ -823,9 +824,10 @@ bool boxes_fit(const std::vector<Box>& boxes) {
 }
 
 int g_chunk_rows = 0;  // 0: pick automatically
-// Rows per workgroup pinned for one side length (by the autotuner).
+// Rows per workgroup pinned for one side length and layer count (by the
+// autotuner, or for every layer count by silt_set_chunk_rows).
 std::mutex g_chunk_mu;
-std::map<int, int> g_chunk_by_res;
+std::map<std::pair<int, int>, int> g_chunk_by_res;
 
 }  // namespace
 
@@ -836,7 +838,7 @@ int fused_chunk_rows(int res, int strips, int layers, int resident) {
   if (g_chunk_rows > 0) return g_chunk_rows;
   {
     std::lock_guard<std::mutex> lock(g_chunk_mu);
-    auto it = g_chunk_by_res.find(res);
+    auto it = g_chunk_by_res.find({res, layers});
     if (it != g_chunk_by_res.end()) return it->second;
   }
   int per_column = resident / (strips * layers);
@@ -847,18 +849,25 @@ int fused_chunk_rows(int res, int strips, int layers, int resident) {
 
 void fused_set_chunk_rows(int rows) { g_chunk_rows = rows; }
 
-void fused_set_chunk_rows_for(int res, int rows) {
+void fused_set_chunk_rows_for(int res, int rows, int layers) {
   std::lock_guard<std::mutex> lock(g_chunk_mu);
-  if (rows > 0)
-    g_chunk_by_res[res] = rows;
-  else
-    g_chunk_by_res.erase(res);
+  for (int l = 1; l <= 2; l++) {
+    if (layers != 0 && layers != l) continue;
+    if (rows > 0)
+      g_chunk_by_res[{res, l}] = rows;
+    else
+      g_chunk_by_res.erase({res, l});
+  }
 }
 
-int fused_pinned_chunk_rows(int res) {
+int fused_pinned_chunk_rows(int res, int layers) {
   std::lock_guard<std::mutex> lock(g_chunk_mu);
-  auto it = g_chunk_by_res.find(res);
-  return it == g_chunk_by_res.end() ? 0 : it->second;
+  for (int l = 2; l >= 1; l--) {
+    if (layers != 0 && layers != l) continue;
+    auto it = g_chunk_by_res.find({res, l});
+    if (it != g_chunk_by_res.end()) return it->second;
+  }
+  return 0;
 }
 
 template <bool EXACT, bool PROJECT, bool WIND, bool FULL>

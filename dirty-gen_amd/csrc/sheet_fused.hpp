@@ -152,9 +152,10 @@ int step_device_planes(const SheetPlanes& s, int res, float cell, float dt, cons
 // launch resident, unless fused_set_chunk_rows pinned one (0 unpins).
 int fused_chunk_rows(int res, int strips, int layers, int resident);
 void fused_set_chunk_rows(int rows);
-// Pin the rows per workgroup for sheets of side res (0 unpins); the pinned
-// value for res, or 0.
-void fused_set_chunk_rows_for(int res, int rows);
-int fused_pinned_chunk_rows(int res);
+// Pin the rows per workgroup for launches of `layers` sheets of side res
+// (0: one- and two-sheet launches alike; rows 0 unpins); the pinned value,
+// or 0 (layers 0: the two-sheet pin if any, else the one-sheet pin).
+void fused_set_chunk_rows_for(int res, int rows, int layers = 0);
+int fused_pinned_chunk_rows(int res, int layers = 0);
 
 }  // namespace silt

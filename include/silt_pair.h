@@ -72,7 +72,9 @@ int silt_pair_last_fallback(const silt_pair *p);
  * this handle's side: every candidate (n of them in rows; NULL or n = 0 for
  * a built-in set around the default, 0 meaning the default itself) is timed
  * over `reps` steps of dt from the handle's current state, which is restored
- * after each candidate; the fastest is pinned.  best_rows / best_ms (mean
+ * after each candidate; the fastest is pinned for launches that carry both
+ * layers (a launch of one layer, a layer's extra substeps, keeps the
+ * occupancy default).  best_rows / best_ms (mean
  * kernel time of a step) may be NULL.  Allocates one temporary copy of the
  * six planes.  No counterpart in the reference: a tuning step of the GPU
  * path, run once before a long simulation. */
