@@ -947,6 +947,36 @@ int silt_core_publish(silt_core* c, const float* soil_bed, const float* soil_pla
   return SILT_OK;
 }
 
+int silt_core_publish_surface(silt_core* c, const float* soil_wetness, const float* soil_placed, float* moisture1,
+                              float* wet_edge1, float* foam_crest2) {
+  if (!c) return SILT_EINVAL;
+  // Device scratch: the two optional soil inputs, then the three outputs.
+  float* d = nullptr;
+  if (hipMalloc(&d, (size_t)WN * 6 * sizeof(float)) != hipSuccess) return SILT_ENOMEM;
+  float *d_wet = nullptr, *d_placed = nullptr;
+  int rc = SILT_OK;
+  if (soil_wetness) {
+    d_wet = d;
+    if (hipMemcpyAsync(d_wet, soil_wetness, WN * 4, hipMemcpyHostToDevice, c->st) != hipSuccess) rc = SILT_EDEVICE;
+  }
+  if (soil_placed) {
+    d_placed = d + WN;
+    if (hipMemcpyAsync(d_placed, soil_placed, WN * 4, hipMemcpyHostToDevice, c->st) != hipSuccess) rc = SILT_EDEVICE;
+  } else if (c->ground) {
+    d_placed = c->soil_placed;  // the core's own ground, as silt_core_publish publishes it
+  }
+  if (rc == SILT_OK) {
+    core_publish_surface(c->c, d_wet, d_placed, kWindowRes, d + 2 * WN, d + 3 * WN, d + 4 * WN, c->st);
+    if ((moisture1 && hipMemcpyAsync(moisture1, d + 2 * WN, WN * 4, hipMemcpyDeviceToHost, c->st) != hipSuccess) ||
+        (wet_edge1 && hipMemcpyAsync(wet_edge1, d + 3 * WN, WN * 4, hipMemcpyDeviceToHost, c->st) != hipSuccess) ||
+        (foam_crest2 && hipMemcpyAsync(foam_crest2, d + 4 * WN, 2 * WN * 4, hipMemcpyDeviceToHost, c->st) != hipSuccess))
+      rc = SILT_EDEVICE;
+  }
+  if (hipStreamSynchronize(c->st) != hipSuccess) rc = SILT_EDEVICE;
+  (void)hipFree(d);
+  return rc;
+}
+
 int silt_core_fine_water_debug(silt_core* c, const silt_tire_push* pushes, size_t n, float* water, float* uw, float* vw,
                                float* origin2) {
   if (!c || (!pushes && n) || !water || !origin2 || n > (size_t)kMaxPush) return SILT_EINVAL;

@@ -107,5 +107,9 @@ void core_ground(const float* soil_bed, const float* soil_placed, const float* s
                  int n, hipStream_t st);
 void core_publish(const CorePlanes& c, const float* soil_bed, const float* soil_placed, const float* soil_rut,
                   float* ground_rut4, float* film2, float* flow4, hipStream_t st);
+// moisture (max of the sheet's and the soil's wetness), wet_edge and
+// foam_crest (0, placed + film excess) maps; wetness / placed may be null.
+void core_publish_surface(const CorePlanes& c, const float* wetness, const float* placed, int res, float* moisture1,
+                          float* wet_edge1, float* foam2, hipStream_t st);
 
 }  // namespace silt

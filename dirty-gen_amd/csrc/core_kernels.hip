@@ -530,6 +530,59 @@ __global__ void k_publish(CorePlanes c, const float* soil_bed, const float* soil
   }
 }
 
+// The render contract's moisture, wet_edge and foam_crest maps
+// (src/render_contract.rs:457-500, :526-569): one lane per texel, reading
+// its four neighbours' films and wet flags directly.
+__device__ inline float fin0(float x) { return isfinite(x) ? x : 0.0f; }
+
+__device__ inline float surface_moisture(const CorePlanes& c, const float* wetness, int k) {
+  float m = fin0(c.moisture[k]);
+  return wetness ? fmaxf(m, wetness[k]) : m;
+}
+
+__device__ inline bool surface_wet(const CorePlanes& c, const float* wetness, int k) {
+  // is_wet (src/render_contract.rs:261-263) on coarse_cell's finite films.
+  return surface_moisture(c, wetness, k) >= 0.45f || fin0(c.water[k]) + fin0(c.slurry[k]) >= 0.004f;
+}
+
+__global__ void k_publish_surface(CorePlanes c, const float* wetness, const float* placed, int res, float* moisture1,
+                                  float* wet_edge1, float* foam2) {
+  const int k = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (k >= res * res) return;
+  const int i = k % res, j = k / res;
+  if (moisture1) moisture1[k] = fin0(surface_moisture(c, wetness, k));
+  const int di[4] = {-1, 1, 0, 0}, dj[4] = {0, 0, -1, 1};
+  if (wet_edge1) {
+    const bool w = surface_wet(c, wetness, k);
+    float edge = 0.0f;
+    for (int q = 0; q < 4; q++) {
+      const int ni = i + di[q], nj = j + dj[q];
+      if (ni < 0 || nj < 0 || ni >= res || nj >= res) continue;
+      if (w != surface_wet(c, wetness, nj * res + ni)) {
+        edge = 1.0f;
+        break;
+      }
+    }
+    wet_edge1[k] = edge;
+  }
+  if (foam2) {
+    const float h = fin0(c.water[k]) + fin0(c.slurry[k]);
+    float sum = 0.0f, count = 0.0f;
+    for (int q = 0; q < 4; q++) {
+      const int ni = i + di[q], nj = j + dj[q];
+      if (ni < 0 || nj < 0 || ni >= res || nj >= res) continue;
+      const int n = nj * res + ni;
+      sum += fin0(c.water[n]) + fin0(c.slurry[n]);
+      count += 1.0f;
+    }
+    const float mean = count > 0.0f ? sum / count : h;
+    const float excess = fin0(fmaxf(h - mean, 0.0f));
+    const float pl = placed ? fin0(fmaxf(placed[k], 0.0f)) : 0.0f;
+    foam2[2 * k] = 0.0f;  // no foam tracer
+    foam2[2 * k + 1] = fin0(pl + excess);
+  }
+}
+
 // pull_substrate (src/world.rs:5511-5537): the sheet's bed is the pressed
 // ground, bed + placed; the rut map is the cut below the initial ground.
 __global__ void k_ground(const float* soil_bed, const float* soil_placed, const float* soil_init, float* sheet_bed,
@@ -632,6 +685,13 @@ void core_publish(const CorePlanes& c, const float* soil_bed, const float* soil_
   int n = kWindowRes * kWindowRes;
   hipLaunchKernelGGL(k_publish, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, c, soil_bed, soil_placed, soil_rut,
                      ground_rut4, film2, flow4, n);
+}
+
+void core_publish_surface(const CorePlanes& c, const float* wetness, const float* placed, int res, float* moisture1,
+                          float* wet_edge1, float* foam2, hipStream_t st) {
+  const int n = res * res;
+  hipLaunchKernelGGL(k_publish_surface, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, c, wetness, placed, res,
+                     moisture1, wet_edge1, foam2);
 }
 
 }  // namespace silt

@@ -608,6 +608,19 @@ class FluidCore:
                                               _ptr(gr), _ptr(film), _ptr(flow)))
         return gr, film, flow
 
+    def publish_surface(self, soil_wetness=None, soil_placed=None):
+        """moisture, wet_edge and foam_crest maps (include/silt_core.h)."""
+        L = lib()
+        if not getattr(L, "_publish_surface_declared", False):
+            L.silt_core_publish_surface.argtypes = [_P, _P, _P, _P, _P, _P]
+            L._publish_surface_declared = True
+        n = WINDOW_RES * WINDOW_RES
+        moist, edge, foam = np.empty(n, np.float32), np.empty(n, np.float32), np.empty(2 * n, np.float32)
+        soils = [None if a is None else np.ascontiguousarray(a, np.float32) for a in (soil_wetness, soil_placed)]
+        _check_status(L.silt_core_publish_surface(self._c, *(None if a is None else _ptr(a) for a in soils),
+                                                  _ptr(moist), _ptr(edge), _ptr(foam)))
+        return moist, edge, foam
+
     def near_tiles(self):
         return int(_check_status(lib().silt_core_near_tiles(self._c)))
 

@@ -116,6 +116,17 @@ int silt_core_set_planes(silt_core *c, const float *water, const float *slurry, 
  * set, the sheet bed with no placed soil and no rut.  Host outputs. */
 int silt_core_publish(silt_core *c, const float *soil_bed, const float *soil_placed, const float *soil_rut,
                       float *ground_rut4, float *film2, float *flow4);
+/* The render contract's moisture (r32f), wet_edge (r32f) and foam_crest
+ * (rg32f) maps on the 200 x 200 window (publish_with_snow,
+ * src/render_contract.rs:457-500 and :526-569): moisture = max(sheet
+ * moisture, soil wetness); wet_edge = 1 where a 4-neighbour inside the
+ * window differs in is_wet (moisture >= 0.45 or film >= 0.004 m,
+ * :261-263); foam_crest = (foam, placed + max(film - mean neighbour film,
+ * 0)).  The foam tracer is not part of this library: foam is 0.  Host
+ * arrays; soil_wetness / soil_placed may be NULL (no soil texel; placed then
+ * comes from silt_core_set_ground's ground, or is 0); any output may be NULL. */
+int silt_core_publish_surface(silt_core *c, const float *soil_wetness, const float *soil_placed, float *moisture1,
+                              float *wet_edge1, float *foam_crest2);
 
 /* The ground under the sheet on the window grid (SoilField bed and placed
  * soil sampled at the cell centres).  The initial ground is this bed; the

@@ -1049,6 +1049,59 @@ void F(oracle_core_publish)(Core *c, real *film2, real *flow4) {
     }
 }
 
+/* The render contract's moisture, wet_edge and foam_crest maps
+ * (src/render_contract.rs:457-500 and :526-569) over the window: moisture =
+ * max(coarse_cell moisture, soil wetness) (wetness NULL: no soil texel);
+ * wet = moisture >= 0.45 or water + slurry >= 0.004 (is_wet, :261-263);
+ * wet_edge = 1 where an in-window 4-neighbour's wet differs; foam_crest =
+ * (foam, placed + excess) with excess = max(film - mean of the in-window
+ * 4-neighbours' films, 0) and placed = max(ground_rut placed channel, 0)
+ * (placed NULL: 0).  No foam tracer here: foam is 0. */
+static real finite0r(real x) { return isfinite((double)x) ? x : K(0.0); }
+
+void F(oracle_core_publish_surface)(Core *c, const real *wetness, const real *placed, real *moisture1,
+                                    real *wet_edge1, real *foam2) {
+    uint8_t *wet = (uint8_t *)malloc(WN);
+    real *film = (real *)malloc(sizeof(real) * WN);
+    for (int k = 0; k < WN; k++) {
+        real water = finite0r(c->water[k]), slurry = finite0r(c->slurry[k]);
+        real m = finite0r(c->moisture[k]);
+        if (wetness) m = R_FMAX(m, wetness[k]);
+        moisture1[k] = finite0r(m);
+        wet[k] = m >= K(0.45) || water + slurry >= K(0.004);
+        film[k] = water + slurry;
+    }
+    static const int di[4] = {-1, 1, 0, 0}, dj[4] = {0, 0, -1, 1};
+    for (int j = 0; j < WRES; j++)
+        for (int i = 0; i < WRES; i++) {
+            int idx = j * WRES + i;
+            real edge = K(0.0);
+            for (int q = 0; q < 4; q++) {
+                int ni = i + di[q], nj = j + dj[q];
+                if (ni < 0 || nj < 0 || ni >= WRES || nj >= WRES) continue;
+                if (wet[idx] != wet[nj * WRES + ni]) {
+                    edge = K(1.0);
+                    break;
+                }
+            }
+            wet_edge1[idx] = edge;
+            real h = film[idx], sum = K(0.0), count = K(0.0);
+            for (int q = 0; q < 4; q++) {
+                int ni = i + di[q], nj = j + dj[q];
+                if (ni < 0 || nj < 0 || ni >= WRES || nj >= WRES) continue;
+                sum += film[nj * WRES + ni];
+                count += K(1.0);
+            }
+            real mean = count > K(0.0) ? sum / count : h;
+            real excess = finite0r(R_FMAX(h - mean, K(0.0)));
+            real pl = placed ? finite0r(R_FMAX(placed[idx], K(0.0))) : K(0.0);
+            foam2[2 * idx] = K(0.0);
+            foam2[2 * idx + 1] = finite0r(pl + excess);
+        }
+    free(film);
+    free(wet);
+}
+
 /* drain_over_bed (src/fluid_core.rs:5123-5214) with step_water_drain /
  * step_slurry_drain (:5159-5214) and restore_wiped (:4622-4637). */
 static void restore_wiped(real *h, real before) {

@@ -202,6 +202,7 @@ def _declare_core(L):
                       ("oracle_core_read_substrate", [_P, _P, _P, _P]), ("oracle_core_get", [_P, _P]),
                       ("oracle_core_set", [_P, _P]), ("oracle_core_origin", [_P, _P]),
                       ("oracle_core_ledger", [_P, _P]), ("oracle_core_publish", [_P, _P, _P]),
+                      ("oracle_core_publish_surface", [_P, _P, _P, _P, _P, _P]),
                       ("oracle_core_refine", [_P, R]), ("oracle_core_restrict", [_P]),
                       ("oracle_core_fine_get", [_P, _P, _P]), ("oracle_recentre", [_P]),
                       ("oracle_core_fine_water_debug", [_P, _P, C.c_int, _P, _P, _P, _P])):
@@ -328,6 +329,18 @@ class Core:
         flow = np.zeros(4 * n, self.dtype)
         self._f("oracle_core_publish")(self._c, _ptr(film), _ptr(flow))
         return film, flow
+
+    def publish_surface(self, wetness=None, placed=None):
+        """moisture, wet_edge and foam_crest maps of the render contract
+        (src/render_contract.rs:457-500, :526-569); wetness / placed are the
+        soil texels under the window (None: no soil)."""
+        n = WINDOW_RES * WINDOW_RES
+        w = None if wetness is None else np.ascontiguousarray(wetness, self.dtype)
+        p = None if placed is None else np.ascontiguousarray(placed, self.dtype)
+        moist, edge, foam = np.zeros(n, self.dtype), np.zeros(n, self.dtype), np.zeros(2 * n, self.dtype)
+        self._f("oracle_core_publish_surface")(self._c, None if w is None else _ptr(w), None if p is None else _ptr(p),
+                                               _ptr(moist), _ptr(edge), _ptr(foam))
+        return moist, edge, foam
 
     def near_tiles(self):
         return int(self._f("oracle_core_near_tiles")(self._c))

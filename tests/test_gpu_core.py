@@ -156,3 +156,35 @@ def test_core_coarse_jacobi_fast_path_tracks_reference_sweep():
     pg, po = g.planes(), o.planes()
     l1 = float(np.abs(pg["water"] - po["water"]).sum()) / float(np.abs(po["water"]).sum())
     assert l1 < 0.05, l1
+
+
+def test_core_publish_surface_matches_oracle():
+    """moisture / wet_edge / foam_crest (src/render_contract.rs:457-500,
+    :526-569) from the same planes on the GPU and in the oracle, bit for bit:
+    a pool with a slurry patch, NaN and negative cells, random sheet
+    moisture, with and without soil wetness and placed soil."""
+    g, o = _pair()
+    rng = np.random.default_rng(21)
+    n = 200 * 200
+    for c in (g, o):
+        c.pour_water(-3.0, 3.0, -3.0, 3.0, 0.05)
+        c.pour_slurry(-1.0, 1.0, -1.0, 1.0, 0.02, "clay", 0.6)
+    for k in range(3):
+        for t in _tires(k, speed=2.0):
+            g.push_tire(**t)
+        g.step(1.0 / 30.0)
+    p = g.planes()
+    p["moisture"] = rng.uniform(0.0, 0.6, n).astype(np.float32)
+    p["water"][[5, 900, 20500]] = [np.nan, -0.01, 0.003]
+    p["slurry"][[6, 901]] = [np.inf, 0.0039]
+    g.set_planes(p)
+    o.set_planes(p)
+    wet = rng.uniform(0.0, 0.6, n).astype(np.float32)
+    wet[7] = np.nan
+    placed = rng.uniform(-0.01, 0.05, n).astype(np.float32)
+    for args in ((None, None), (wet, None), (None, placed), (wet, placed)):
+        gm, ge, gf = g.publish_surface(*args)
+        om, oe, of = o.publish_surface(*args)
+        for a, b, name in zip((gm, ge, gf), (om, oe, of), ("moisture", "wet_edge", "foam_crest")):
+            assert np.array_equal(a, b), (name, args[0] is not None, args[1] is not None)
+        assert 0 < ge.sum() < n and np.isfinite(gf).all() and not gf[0::2].any()

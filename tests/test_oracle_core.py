@@ -130,3 +130,35 @@ def test_publish_film_and_flow_layout():
     assert np.allclose(flow[0::4], u, rtol=1e-6, atol=1e-9)
     assert np.allclose(flow[2::4], np.hypot(flow[0::4], flow[1::4]), rtol=1e-6, atol=1e-9)
     assert not flow[0::4][~wet].any() and np.isfinite(flow).all()
+
+
+def test_publish_surface_edges_and_crest():
+    """wet_edge / foam_crest / moisture (src/render_contract.rs:457-500,
+    :526-569) on a square pool: the edge ring is the pool's inner and outer
+    perimeter, the crest excess of a corner cell is its depth minus the mean
+    of its two wet and two dry neighbours, placed soil adds to the crest,
+    and the soil's wetness can make dry cells wet (no edge inside a wet
+    patch)."""
+    core = O.Core((0.0, 0.0))
+    core.pour_water(-3.0, 3.0, -3.0, 3.0, 0.05)  # 30 x 30 cells
+    p = core.planes()
+    wet = p["water"].reshape(200, 200) > 0
+    js, is_ = np.nonzero(wet)
+    side = js.max() - js.min() + 1
+    assert side == 30 and wet.sum() == 900
+    moist, edge, foam = core.publish_surface()
+    assert edge.sum() == (4 * side - 4) + 4 * side  # inner ring + outer ring
+    crest = foam[1::2].reshape(200, 200)
+    corner = (js.min(), is_.min())
+    assert abs(crest[corner] - 0.05 * 0.5) < 1e-7  # two of four neighbours wet
+    assert crest[js.min() + 5, is_.min() + 5] == 0.0  # flat inside
+    assert not foam[0::2].any() and not moist.any()
+    placed = np.zeros(200 * 200, np.float32)
+    placed[0] = 0.04
+    _, _, foam2 = core.publish_surface(placed=placed)
+    assert foam2[1] == np.float32(0.04)
+    wetness = np.zeros(200 * 200, np.float32)
+    wetness.reshape(200, 200)[js.min() - 5:js.max() + 6, is_.min() - 5:is_.max() + 6] = 0.9
+    m2, edge2, _ = core.publish_surface(wetness=wetness)
+    assert m2.max() == np.float32(0.9)
+    assert edge2.sum() == (4 * (side + 10) - 4) + 4 * (side + 10)  # the edge moved out to the wet soil
