@@ -450,6 +450,177 @@ __global__ void k_near_apply(FinePlanes f, NearTable t, const NearColumn* cols, 
     }
 }
 
+// near_capture with one 128-lane workgroup per held column: lane k < 100
+// owns fine cell (k / 10, k % 10) of the column.  The sums the reference
+// makes in order (row sums then their sum, the FP64 mean, the
+// momentum-weighted velocity) run on lane 0 over LDS copies in the same
+// order; the peaks are maxima (order-free once NaN is dropped, as the
+// one-lane scan drops it).
+__global__ __launch_bounds__(128) void k_near_capture_wg(FinePlanes f, NearTable t, const NearColumn* cols) {
+  __shared__ float w_s[100], s_s[100], u_s[100], v_s[100];
+  __shared__ float bc_f[4];
+  __shared__ float pk[2][4];
+  __shared__ int bc_i[2];
+  const NearColumn nc = cols[blockIdx.x];
+  const int k = (int)threadIdx.x;
+  const bool cellk = k < 100;
+  const size_t fi = cellk ? (size_t)(nc.j0 + k / 10) * kFineRes + nc.i0 + k % 10 : 0;
+  float w = 0.0f, sl = 0.0f;
+  if (cellk) {
+    w = f.water[fi];
+    sl = f.slurry[fi];
+    w_s[k] = w;
+    s_s[k] = sl;
+  }
+  __syncthreads();
+  if (k == 0) {
+    float sw = 0.0f, ss = 0.0f;
+    for (int b = 0; b < 10; b++) {
+      float rw = -0.0f, rs = -0.0f;
+      for (int a = 0; a < 10; a++) {
+        rw = rw + w_s[b * 10 + a];
+        rs = rs + s_s[b * 10 + a];
+      }
+      sw = sw + rw;
+      ss = ss + rs;
+    }
+    bc_f[0] = sw;
+    bc_f[1] = ss;
+    const bool empty = sw <= 0.0f && ss <= 0.0f;
+    bc_i[0] = empty;
+    if (empty) tile_remove(t, nc.kx, nc.kz);
+  }
+  __syncthreads();
+  if (bc_i[0]) return;
+  const float sw = bc_f[0], ss = bc_f[1];
+  const float mw = sw / 100.0f, ms = ss / 100.0f;
+  const float rw = w - mw, rs = sl - ms;
+  // abs_peak: the one-lane scan skips NaN (a > hi is false), so drop it here
+  // and take plain maxima.
+  float aw = cellk ? fabsf(rw) : 0.0f, as = cellk ? fabsf(rs) : 0.0f;
+  aw = aw > 0.0f ? aw : 0.0f;
+  as = as > 0.0f ? as : 0.0f;
+  for (int o = 32; o > 0; o >>= 1) {
+    aw = fmaxf(aw, __shfl_xor(aw, o));
+    as = fmaxf(as, __shfl_xor(as, o));
+  }
+  if ((k & 63) == 0) {
+    pk[0][k >> 6] = aw;
+    pk[1][k >> 6] = as;
+  }
+  if (cellk) {
+    w_s[k] = rw;
+    s_s[k] = rs;
+  }
+  __syncthreads();
+  const float peak = fmaxf(fmaxf(pk[0][0], pk[0][1]), fmaxf(pk[1][0], pk[1][1]));
+  if (peak <= 1.0e-5f && !nc.inner) {
+    if (k == 0) tile_remove(t, nc.kx, nc.kz);
+    return;
+  }
+  if (k == 0) {
+    // recentre (near.rs) in the reference's order, FP64.
+    double m = 0.0, n = 0.0;
+    for (int q = 0; q < 100; q++) {
+      m += (double)w_s[q];
+      n += (double)s_s[q];
+    }
+    bc_f[2] = (float)(m / 100.0);
+    bc_f[3] = (float)(n / 100.0);
+    bc_i[1] = tile_insert(t, nc.kx, nc.kz);
+  }
+  __syncthreads();
+  const int slot = bc_i[1];
+  if (slot < 0) return;
+  float* d = t.data + (size_t)slot * 400;
+  if (cellk) {
+    d[k] = rw - bc_f[2];
+    d[100 + k] = rs - bc_f[3];
+  }
+  if (nc.inner && sw > 1.0e-10f) {
+    float uu = 0.0f, vv = 0.0f;
+    if (cellk) {
+      uu = f.uw[fi];
+      vv = f.vw[fi];
+      u_s[k] = w * uu;
+      v_s[k] = w * vv;
+    }
+    __syncthreads();
+    if (k == 0) {
+      float mu = 0.0f, mv = 0.0f;
+      for (int q = 0; q < 100; q++) {
+        mu = mu + u_s[q];
+        mv = mv + v_s[q];
+      }
+      bc_f[0] = mu / sw;
+      bc_f[1] = mv / sw;
+    }
+    __syncthreads();
+    if (cellk) {
+      const bool wet = w > 1.0e-6f;
+      d[200 + k] = wet ? uu - bc_f[0] : 0.0f;
+      d[300 + k] = wet ? vv - bc_f[1] : 0.0f;
+    }
+  } else if (cellk) {
+    d[200 + k] = 0.0f;
+    d[300 + k] = 0.0f;
+  }
+  if (k == 0) {
+    t.peak[slot] = peak;
+    t.idle[slot] = 0.0f;
+    t.fresh[slot] = 1;
+  }
+}
+
+// near_apply with one 128-lane workgroup per held column (lane k < 100 owns
+// fine cell k); floor_scale's minima are order-free (fminf drops NaN).
+__global__ __launch_bounds__(128) void k_near_apply_wg(FinePlanes f, NearTable t, const NearColumn* cols) {
+  __shared__ float lo[2][2];
+  __shared__ int slot_s;
+  const NearColumn nc = cols[blockIdx.x];
+  const int k = (int)threadIdx.x;
+  const bool cellk = k < 100;
+  if (k == 0) slot_s = tile_find(t, nc.kx, nc.kz);
+  __syncthreads();
+  const int s = slot_s;
+  if (s < 0) return;
+  const float* d = t.data + (size_t)s * 400;
+  float dw = cellk ? d[k] : 0.0f, ds = cellk ? d[100 + k] : 0.0f;
+  float mw = fminf(0.0f, dw), ms = fminf(0.0f, ds);
+  for (int o = 32; o > 0; o >>= 1) {
+    mw = fminf(mw, __shfl_xor(mw, o));
+    ms = fminf(ms, __shfl_xor(ms, o));
+  }
+  if ((k & 63) == 0) {
+    lo[0][k >> 6] = mw;
+    lo[1][k >> 6] = ms;
+  }
+  __syncthreads();
+  const size_t f0 = (size_t)nc.j0 * kFineRes + nc.i0;
+  const float bw = f.water[f0], bs = f.slurry[f0];
+  const float left = tile_left(t.idle[s]);
+  auto scale = [&](float base, float low) -> float {
+    if (low >= 0.0f || base + low >= 0.0f) return 1.0f;
+    if (base <= 0.0f) return 0.0f;
+    return clampf(base / -low, 0.0f, 1.0f);
+  };
+  const float aw = left * scale(bw / fmaxf(left, 1.0e-6f), fminf(lo[0][0], lo[0][1]));
+  const float as = left * scale(bs / fmaxf(left, 1.0e-6f), fminf(lo[1][0], lo[1][1]));
+  // Every lane has read the column's first cell above; the writes below may
+  // overwrite it.
+  __syncthreads();
+  if (!cellk) return;
+  const size_t fi = (size_t)(nc.j0 + k / 10) * kFineRes + nc.i0 + k % 10;
+  const float w = fmaxf(bw + aw * dw, 0.0f);
+  const float sv = fmaxf(bs + as * ds, 0.0f);
+  f.water[fi] = w;
+  f.slurry[fi] = sv;
+  if (nc.inner && w > 1.0e-6f) {
+    f.uw[fi] = f.uw[fi] + left * d[200 + k];
+    f.vw[fi] = f.vw[fi] + left * d[300 + k];
+  }
+}
+
 // near_fade (near.rs:300-313), one table slot per thread.
 __global__ void k_near_fade(NearTable t, float dt) {
   unsigned s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -653,12 +824,12 @@ void core_restrict(const CorePlanes& c, const FinePlanes& f, const uint8_t* hold
 
 void core_near_capture(const FinePlanes& f, const NearTable& t, const NearColumn* cols, int ncols, hipStream_t st) {
   if (ncols <= 0) return;
-  hipLaunchKernelGGL(k_near_capture, dim3((unsigned)((ncols + 63) / 64)), dim3(64), 0, st, f, t, cols, ncols);
+  hipLaunchKernelGGL(k_near_capture_wg, dim3((unsigned)ncols), dim3(128), 0, st, f, t, cols);
 }
 
 void core_near_apply(const FinePlanes& f, const NearTable& t, const NearColumn* cols, int ncols, hipStream_t st) {
   if (ncols <= 0) return;
-  hipLaunchKernelGGL(k_near_apply, dim3((unsigned)((ncols + 63) / 64)), dim3(64), 0, st, f, t, cols, ncols);
+  hipLaunchKernelGGL(k_near_apply_wg, dim3((unsigned)ncols), dim3(128), 0, st, f, t, cols);
 }
 
 void core_near_fade(const NearTable& t, float dt, hipStream_t st) {

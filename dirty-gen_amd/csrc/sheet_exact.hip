@@ -258,6 +258,15 @@ __global__ __launch_bounds__(256) void k_force_gauss_seidel_lds(const float* __r
   const int W = b.i1 - b.i0 + 1, H = b.j1 - b.j0 + 1;
   const int tmax = (W - 1) + 2 * (H - 1);
   const float cell = a.cell;
+  // x / (2 cell), correctly rounded without a division: q = x * y with
+  // y = RN(1 / (2 cell)) is within one ulp of the quotient, the remainder
+  // x - 2 cell q is exact in one FMA, and q + remainder * y rounds to the
+  // IEEE quotient (Markstein's theorem) while nothing underflows or
+  // overflows; zeros keep their sign from x * y.  A cell any of whose
+  // quotients lies outside [2^-100, 2^101) or is not finite is redone with
+  // the division itself.
+  const float d2c = 2.0f * cell;
+  const float y2c = 1.0f / d2c;
   auto col_at = [&](int r, int t) { return b.i0 + t - 2 * (r - b.j0); };
   auto load = [&](int c) {
     if (!live || c < 0 || c >= res) return;
@@ -274,11 +283,11 @@ __global__ __launch_bounds__(256) void k_force_gauss_seidel_lds(const float* __r
     y = y < 0 ? 0 : (y > res - 1 ? res - 1 : y);
     return r[y - r_lo][x & (kRing - 1)];
   };
-  auto vort = [&](int x, int y) -> float {
+  auto vort = [&](int x, int y, auto div2c) -> float {
     const int xp = x + 1 < res ? x + 1 : res - 1, xm = x > 0 ? x - 1 : 0;
     const int yp = y + 1 < res ? y + 1 : res - 1, ym = y > 0 ? y - 1 : 0;
-    const float dvdx = (at(rv, xp, y) - at(rv, xm, y)) / (2.0f * cell);
-    const float dudz = (at(ru, x, yp) - at(ru, x, ym)) / (2.0f * cell);
+    const float dvdx = div2c(at(rv, xp, y) - at(rv, xm, y));
+    const float dudz = div2c(at(ru, x, yp) - at(ru, x, ym));
     return dvdx - dudz;
   };
   // The column a row needs next comes into registers one wavefront early
@@ -327,41 +336,59 @@ __global__ __launch_bounds__(256) void k_force_gauss_seidel_lds(const float* __r
       const int i = c, j = jj;
       const size_t idx = (size_t)j * res + i;
       if (at(rh, i, j) > a.eps) {
-        float uu = at(ru, i, j), vv = at(rv, i, j);
-        if (a.wind) {
-          uu = uu + (a.wx - uu) * a.blend;
-          vv = vv + (a.wz - vv) * a.blend;
-        } else {
-          const int ip = i + 1 < res ? i + 1 : res - 1, im = i > 0 ? i - 1 : 0;
-          const int jp = j + 1 < res ? j + 1 : res - 1, jm = j > 0 ? j - 1 : 0;
-          const float px = (at(rp, ip, j) - at(rp, im, j)) / (2.0f * cell);
-          const float pz = (at(rp, i, jp) - at(rp, i, jm)) / (2.0f * cell);
-          const float mob = a.nu_uniform ? a.mob0 : sheet_mobility(nu_c);
-          uu = uu + -kGravity * px * mob * a.dt;
-          vv = vv + -kGravity * pz * mob * a.dt;
-        }
-        ru[row][i & (kRing - 1)] = uu;
-        rv[row][i & (kRing - 1)] = vv;
-        if (a.vort > 0.0f) {
-          // confine_at on the ring: this cell's own kick is already in it.
-          const float w = vort(i, j);
-          if (isfinite(w) && !(fabsf(w) < 1.0e-4f)) {
-            const float wxp = fabsf(vort(i + 1 < res ? i + 1 : res - 1, j));
-            const float wxm = fabsf(vort(i > 0 ? i - 1 : 0, j));
-            const float wzp = fabsf(vort(i, j + 1 < res ? j + 1 : res - 1));
-            const float wzm = fabsf(vort(i, j > 0 ? j - 1 : 0));
-            const float nx = (wxp - wxm) / (2.0f * cell);
-            const float nz = (wzp - wzm) / (2.0f * cell);
-            const float du = a.vort * a.dt * (-nz * w);
-            const float dv = a.vort * a.dt * (nx * w);
-            if (isfinite(du) && isfinite(dv) && du * du + dv * dv <= 0.0625f) {
-              uu = uu + du;
-              vv = vv + dv;
-              ru[row][i & (kRing - 1)] = uu;
-              rv[row][i & (kRing - 1)] = vv;
+        const float u0 = at(ru, i, j), v0 = at(rv, i, j);
+        // The cell's kick and confinement with the quotient div2c; the ring
+        // holds the cell's own values as it goes (no other cell of this
+        // wavefront reads them).
+        auto update = [&](auto div2c, float& uu, float& vv) {
+          uu = u0;
+          vv = v0;
+          if (a.wind) {
+            uu = uu + (a.wx - uu) * a.blend;
+            vv = vv + (a.wz - vv) * a.blend;
+          } else {
+            const int ip = i + 1 < res ? i + 1 : res - 1, im = i > 0 ? i - 1 : 0;
+            const int jp = j + 1 < res ? j + 1 : res - 1, jm = j > 0 ? j - 1 : 0;
+            const float px = div2c(at(rp, ip, j) - at(rp, im, j));
+            const float pz = div2c(at(rp, i, jp) - at(rp, i, jm));
+            const float mob = a.nu_uniform ? a.mob0 : sheet_mobility(nu_c);
+            uu = uu + -kGravity * px * mob * a.dt;
+            vv = vv + -kGravity * pz * mob * a.dt;
+          }
+          ru[row][i & (kRing - 1)] = uu;
+          rv[row][i & (kRing - 1)] = vv;
+          if (a.vort > 0.0f) {
+            // confine_at on the ring: this cell's own kick is already in it.
+            const float w = vort(i, j, div2c);
+            if (isfinite(w) && !(fabsf(w) < 1.0e-4f)) {
+              const float wxp = fabsf(vort(i + 1 < res ? i + 1 : res - 1, j, div2c));
+              const float wxm = fabsf(vort(i > 0 ? i - 1 : 0, j, div2c));
+              const float wzp = fabsf(vort(i, j + 1 < res ? j + 1 : res - 1, div2c));
+              const float wzm = fabsf(vort(i, j > 0 ? j - 1 : 0, div2c));
+              const float nx = div2c(wxp - wxm);
+              const float nz = div2c(wzp - wzm);
+              const float du = a.vort * a.dt * (-nz * w);
+              const float dv = a.vort * a.dt * (nx * w);
+              if (isfinite(du) && isfinite(dv) && du * du + dv * dv <= 0.0625f) {
+                uu = uu + du;
+                vv = vv + dv;
+                ru[row][i & (kRing - 1)] = uu;
+                rv[row][i & (kRing - 1)] = vv;
+              }
             }
           }
-        }
+        };
+        bool ok = true;
+        auto fast = [&](float x) -> float {
+          const float q = x * y2c;
+          const float q2 = __builtin_fmaf(__builtin_fmaf(-d2c, q, x), y2c, q);
+          const unsigned e = (__float_as_uint(q) >> 23) & 0xffu;
+          ok = ok & ((x == 0.0f) | (e - 27u <= 200u));
+          return x == 0.0f ? q : q2;
+        };
+        float uu, vv;
+        update(fast, uu, vv);
+        if (!ok) update([&](float x) -> float { return x / d2c; }, uu, vv);
         u[idx] = uu;
         v[idx] = vv;
       }
