@@ -101,6 +101,12 @@ struct silt_core {
   // The slurry layer's stream and the events that fence it to the main one.
   hipStream_t st2 = nullptr;
   hipEvent_t fork = nullptr, join = nullptr;
+  // Device planning of the coarse pair: statistics of both layers, the
+  // batched wet-set labelling, the substep maxima.
+  SheetStats* d_stats = nullptr;
+  SheetStats* h_stats = nullptr;  // pinned
+  PlanLabeller lab;
+  PlanScratch ps[2];
   hipStream_t st = nullptr;
   std::vector<void*> allocs;
 };
@@ -331,6 +337,67 @@ PushList push_list(const std::vector<CorePush>& v) {
 // step_pair (src/fluid_core.rs:7942-8150): the water and slurry layers are
 // independent (the reference joins them on two threads), so the slurry runs
 // on the handle's second stream with its own scratch, fenced by events.
+// The coarse pair (step_pair on the 200^2 window, src/fluid_core.rs:7942-8150)
+// planned on the GPU: soften both layers, then the statistics of both,
+// their boxes from one batched labelling and their substep counts (the
+// same plan as the host scan: gravity, pace 0, cap 8), then each layer's
+// stages on its own stream.
+int step_coarse_pair(silt_core* c, float dt, Workspace* ws, Workspace* ws2, const silt_sheet_opts& o, float nu_w) {
+  const int res = kWindowRes;
+  const float cell = kCoarseCell;
+  core_soften(c->c.uw, c->c.vw, res, c->st);
+  float* nu = c->nu_c;
+  core_nu(nu, c->c.grain, c->c.moisture, res * res, c->st2);
+  core_soften(c->c.us, c->c.vs, res, c->st2);
+  stats_compute(c->c.water, c->c.uw, c->c.vw, res, o.eps, c->d_stats, c->st);
+  stats_compute(c->c.slurry, c->c.us, c->c.vs, res, o.eps, c->d_stats + 1, c->st2);
+  HIPCHK(hipMemcpyAsync(c->h_stats, c->d_stats, sizeof(SheetStats), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipMemcpyAsync(c->h_stats + 1, c->d_stats + 1, sizeof(SheetStats), hipMemcpyDeviceToHost, c->st2));
+  HIPCHK(hipStreamSynchronize(c->st2));
+  HIPCHK(hipStreamSynchronize(c->st));
+  PlanSheet sheets[2] = {{c->c.water, res, cell, c->h_stats}, {c->c.slurry, res, cell, c->h_stats + 1}};
+  StepPlan plans[2];
+  char host[2];
+  int rc = plan_boxes_many(c->lab, sheets, 2, dt, o.eps, plans, host, c->st);
+  if (rc < 0) return rc;
+  const float* uv[2][2] = {{c->c.uw, c->c.vw}, {c->c.us, c->c.vs}};
+  bool wait = false;
+  for (int l = 0; l < 2; l++) {
+    if (host[l]) continue;
+    if ((rc = plan_nsub_begin(uv[l][0], uv[l][1], res, cell, dt, c->h_stats[l], c->ps[l], &plans[l], c->st)) < 0)
+      return rc;
+    wait = wait || c->ps[l].maxs2_pending;
+  }
+  if (wait) HIPCHK(hipStreamSynchronize(c->st));
+  for (int l = 0; l < 2; l++)
+    if (!host[l]) plan_nsub_end(c->ps[l], cell, dt, &plans[l]);
+  // Water on the main stream, slurry on the second (both wait for the
+  // planning above, which read both layers on the main stream).
+  HIPCHK(hipEventRecord(c->fork, c->st));
+  HIPCHK(hipStreamWaitEvent(c->st2, c->fork, 0));
+  SheetPlanes s = ws->planes();
+  s.h = c->c.water;
+  s.u = c->c.uw;
+  s.v = c->c.vw;
+  s.bed = c->c.bed;
+  silt_drive dw{SILT_DRIVE_GRAVITY, 0.0f, 0.0f, 0.08f};
+  rc = host[0] ? step_device_planes(s, res, cell, dt, dw, o, true, true, nu_w, c->st)
+               : (plans[0].boxes.empty() ? 0 : run_plan(s, plans[0], res, cell, dt, dw, o, true, true, nu_w, c->st));
+  if (rc < 0) return rc;
+  SheetPlanes s2 = ws2->planes();
+  s2.h = c->c.slurry;
+  s2.u = c->c.us;
+  s2.v = c->c.vs;
+  s2.bed = c->c.bed;
+  s2.nu = nu;
+  silt_drive ds{SILT_DRIVE_GRAVITY, 0.0f, 0.0f, 0.05f};
+  rc = host[1] ? step_device_planes(s2, res, cell, dt, ds, o, true, false, 0.0f, c->st2)
+               : (plans[1].boxes.empty() ? 0 : run_plan(s2, plans[1], res, cell, dt, ds, o, true, false, 0.0f, c->st2));
+  HIPCHK(hipEventRecord(c->join, c->st2));
+  HIPCHK(hipStreamWaitEvent(c->st, c->join, 0));
+  return rc < 0 ? rc : 0;
+}
+
 int step_pair(silt_core* c, float dt, bool fine) {
   if (fine && !c->fine_on) return 0;
   int res = fine ? kFineRes : kWindowRes;
@@ -349,8 +416,8 @@ int step_pair(silt_core* c, float dt, bool fine) {
   o.confine = fine ? SILT_CONFINE_REFERENCE : c->coarse_confine;
   SheetPlanes s = ws->planes();
   const float nu_w = dynamic_viscosity(3, 1.0f);
+  if (!fine) return step_coarse_pair(c, dt, ws, ws2, o, nu_w);
   // Water layer.
-  if (!fine) core_soften(c->c.uw, c->c.vw, res, c->st);
   s.h = fine ? c->f.water : c->c.water;
   s.u = fine ? c->f.uw : c->c.uw;
   s.v = fine ? c->f.vw : c->c.vw;
@@ -591,7 +658,9 @@ silt_core* silt_core_create(float center_x, float center_z, int path) {
        dalloc(c, &c->nu_c, WN) && dalloc(c, &c->nu_f, FN) && near_alloc(c, &c->near[0]) && near_alloc(c, &c->near[1]) &&
        dalloc(c, &c->d_cols, 1024) && dalloc(c, &c->d_ev, kSprayCap) && dalloc(c, &c->d_nev, 1) &&
        dalloc(c, &c->d_acc, 4) && dalloc(c, &c->soil_bed, WN) && dalloc(c, &c->soil_placed, WN) &&
-       dalloc(c, &c->soil_init, WN) && dalloc(c, &c->soil_rut, WN) && dalloc(c, &c->d_contacts, kMaxPush);
+       dalloc(c, &c->soil_init, WN) && dalloc(c, &c->soil_rut, WN) && dalloc(c, &c->d_contacts, kMaxPush) &&
+       dalloc(c, &c->d_stats, 2) &&
+       hipHostMalloc((void**)&c->h_stats, 2 * sizeof(SheetStats), hipHostMallocDefault) == hipSuccess;
   if (ok) {
     // Grain::Other everywhere (FluidCore::new, src/fluid_core.rs:4979-4980).
     ok = hipMemset(c->c.grain, 4, WN) == hipSuccess && hipMemset(c->f.grain, 4, FN) == hipSuccess;
@@ -615,6 +684,9 @@ void silt_core_destroy(silt_core* c) {
     if (e) (void)hipEventDestroy(e);
   if (c->st2) (void)hipStreamSynchronize(c->st2);
   for (void* p : c->allocs) (void)hipFree(p);
+  if (c->h_stats) (void)hipHostFree(c->h_stats);
+  plan_labeller_free(c->lab);
+  for (PlanScratch& q : c->ps) plan_scratch_free(q);
   if (c->fork) (void)hipEventDestroy(c->fork);
   if (c->join) (void)hipEventDestroy(c->join);
   if (c->st2) (void)hipStreamDestroy(c->st2);

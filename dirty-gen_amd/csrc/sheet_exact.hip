@@ -386,9 +386,53 @@ __global__ __launch_bounds__(256) void k_force_gauss_seidel_lds(const float* __r
           ok = ok & ((x == 0.0f) | (e - 27u <= 200u));
           return x == 0.0f ? q : q2;
         };
-        float uu, vv;
-        update(fast, uu, vv);
-        if (!ok) update([&](float x) -> float { return x / d2c; }, uu, vv);
+        float uu = u0, vv = v0;
+        // Two cells or more from every grid side, no read clamps: every value
+        // the cell needs is read from the ring first (none of them changes
+        // during this wavefront), the cell's own new values stay in
+        // registers, and the confinement's tests become selects.
+        const bool interior = !a.wind && i >= 2 && i <= res - 3 && j >= 2 && j <= res - 3;
+        if (interior) {
+          const int y0 = j - r_lo;
+          auto U = [&](int dx, int dy) { return ru[y0 + dy][(i + dx) & (kRing - 1)]; };
+          auto Vv = [&](int dx, int dy) { return rv[y0 + dy][(i + dx) & (kRing - 1)]; };
+          auto P = [&](int dx, int dy) { return rp[y0 + dy][(i + dx) & (kRing - 1)]; };
+          const float p_e = P(1, 0), p_w = P(-1, 0), p_n = P(0, 1), p_s = P(0, -1);
+          const float v_e = Vv(1, 0), v_w = Vv(-1, 0), v_ee = Vv(2, 0), v_ww = Vv(-2, 0);
+          const float v_ne = Vv(1, 1), v_nw = Vv(-1, 1), v_se = Vv(1, -1), v_sw = Vv(-1, -1);
+          const float u_n = U(0, 1), u_s = U(0, -1), u_nn = U(0, 2), u_ss = U(0, -2);
+          const float u_ne = U(1, 1), u_se = U(1, -1), u_nw = U(-1, 1), u_sw = U(-1, -1);
+          const float px = fast(p_e - p_w);
+          const float pz = fast(p_n - p_s);
+          const float mob = a.nu_uniform ? a.mob0 : sheet_mobility(nu_c);
+          uu = uu + -kGravity * px * mob * a.dt;
+          vv = vv + -kGravity * pz * mob * a.dt;
+          if (a.vort > 0.0f) {
+            const float w = fast(v_e - v_w) - fast(u_n - u_s);
+            const float wxp = fabsf(fast(v_ee - vv) - fast(u_ne - u_se));
+            const float wxm = fabsf(fast(vv - v_ww) - fast(u_nw - u_sw));
+            const float wzp = fabsf(fast(v_ne - v_nw) - fast(u_nn - uu));
+            const float wzm = fabsf(fast(v_se - v_sw) - fast(uu - u_ss));
+            const float nx = fast(wxp - wxm);
+            const float nz = fast(wzp - wzm);
+            const float du = a.vort * a.dt * (-nz * w);
+            const float dv = a.vort * a.dt * (nx * w);
+            const bool take = isfinite(w) && !(fabsf(w) < 1.0e-4f) && isfinite(du) && isfinite(dv) &&
+                              du * du + dv * dv <= 0.0625f;
+            uu = take ? uu + du : uu;
+            vv = take ? vv + dv : vv;
+          }
+        }
+        if (interior && ok) {
+          ru[row][i & (kRing - 1)] = uu;
+          rv[row][i & (kRing - 1)] = vv;
+        } else {
+          // Cells near a grid side (clamped reads may land on the cell
+          // itself), the wind blend, or a quotient outside the fast range.
+          ok = true;
+          update(fast, uu, vv);
+          if (!ok) update([&](float x) -> float { return x / d2c; }, uu, vv);
+        }
         u[idx] = uu;
         v[idx] = vv;
       }
