@@ -351,9 +351,74 @@ def run_large(args):
                      "kernel": "k_sheet_stream (both layers, one launch)", "kernel_ms": kms,
                      "alg_bytes_per_cell": ALG_BYTES_PER_CELL},
         "tire_ms_mean": float(np.mean(tire_ms)),
+        "speculation": dict(zip(("held", "undone"), pair.speculation_counts())),
         "ledger_gap_rel": {"max": float(np.max(gaps)), "last": float(gaps[-1])},
         "training": info,
     }
+    print(json.dumps(out), flush=True)
+
+
+def run_core(args):
+    """The reference's own unit of work: one FluidCore step on the 200^2
+    window with the nested 2 cm patch under four tires (ruts, pushes, step),
+    the FP64 ledger read every step, at the dt given (the reference publishes
+    its per-step sheet time at dt 1/30)."""
+    import silt_amd as S
+    from silt_amd import drive as D
+    from silt_amd import scenes
+
+    dt = 1.0 / args.core_hz
+    core = D.nested_scene_core(S, scenes, seed=930, path=args.path if args.path != "fused_exact" else "exact")
+    dv = D.Drive(seed=931)
+    first = core.ledger()["total_m3"]
+    for _ in range(args.warmup):
+        D.drive_step(core, dv, dt)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        D.drive_step(core, dv, dt)
+        led = core.ledger()
+    el = time.perf_counter() - t0
+    last = led["total_m3"]
+    core.close()
+    out = {
+        "metric": "FluidCore steps/s (200^2 window, nested 2 cm patch, 4 tires)",
+        "value": args.steps / el,
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": el * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (FP64 ledger)",
+        "data": "synthetic (seeded window scene, seeded steering path)",
+        "config": {"workload": f"FluidCore::step on the 200^2 window at 0.2 m with the 160^2 patch at 2 cm under "
+                               f"four tires at 8 m/s, Bekker ruts, ledger every step, dt 1/{args.core_hz}",
+                   "path": args.path, "parallelism": "single GPU, water and slurry on two streams"},
+        "reference_published_ms_per_step": {"water": 0.927, "slurry": 0.539,
+                                            "hardware": "Intel Xeon, 4 cores, dt 1/30 (docs/PERF_BUDGET.md:68-69)"},
+        "ledger_drift_rel": (last - first) / first,
+    }
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle as O
+        sc = scenes.window_scene(seed=930)
+        o = O.Core((0.0, 0.0))
+        o.map_water_bed(sc["bed"], sc["water"])
+        o.lay_slurry(sc["slurry"], "clay", 0.6)
+        dv = D.Drive(seed=931)
+        k, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < 10.0 and k < 400:
+            for q in dv.pushes():
+                o.push_tire(**q)
+            o.step(dt)
+            dv.advance(dt)
+            k += 1
+        el = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": k / el, "unit": "steps/s", "cores": 1, "kind": "port",
+                               "sample": f"{k} nested FluidCore steps of the same scene (four tires, no ruts), "
+                                         f"FP32 oracle, 1 thread, {el:.1f} s"}
     print(json.dumps(out), flush=True)
 
 
@@ -366,10 +431,11 @@ def main():
     ap.add_argument("--path", default="fast", choices=["fast", "fused_exact", "exact"])
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", default="pair", choices=["pair", "batch", "surrogate", "large"],
+    ap.add_argument("--core-hz", type=int, default=120, help="core: steps per second of simulated time (dt = 1/hz)")
+    ap.add_argument("--workload", default="pair", choices=["pair", "batch", "surrogate", "large", "core"],
                     help="pair: the 4096^2 two-layer sheet (default); batch: 64 uneven windows; surrogate: the "
                          "tire sub-grid network against the nested 2 cm patch; large: 4096^2 with 64 tire "
-                         "contacts, ruts and the network")
+                         "contacts, ruts and the network; core: FluidCore steps on the 200^2 window")
     ap.add_argument("--sur-drives", type=int, default=16, help="surrogate/large: recorded drives of 500 steps")
     ap.add_argument("--sur-rounds", type=int, default=20, help="surrogate/large: training rounds of 1000 batches")
     ap.add_argument("--sur-tol", type=float, default=1.0e-3, help="surrogate: judge tolerance, metres")
@@ -391,6 +457,9 @@ def main():
         return
     if args.workload == "large":
         run_large(args)
+        return
+    if args.workload == "core":
+        run_core(args)
         return
 
     import silt_amd as S
@@ -414,16 +483,21 @@ def main():
     print(f"[bench] warmup {args.warmup} steps", file=sys.stderr, flush=True)
     for _ in range(args.warmup):
         pair.step(dt)
-    kernel_ms = []
     fallbacks = 0
     print(f"[bench] timing {args.steps} steps", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     for _ in range(args.steps):
         pair.step(dt)
-        kernel_ms.append(pair.last_kernel_ms())
         fallbacks += pair.last_fallback() != 0
+    led = pair.ledger()  # waits for the last step
     el = time.perf_counter() - t0
-    led = pair.ledger()
+    spec_hits, spec_misses = pair.speculation_counts()
+    # Kernel time from the launches' own events, over steps that each wait
+    # for their launch (the timed loop above never waits between steps).
+    kernel_ms = []
+    for _ in range(min(args.steps, 50)):
+        pair.step(dt)
+        kernel_ms.append(pair.last_kernel_ms())
     ms = el * 1e3 / args.steps
     kms = float(np.mean(kernel_ms))
     achieved = ALG_BYTES_PER_CELL * n / (kms * 1e-3) / 1e9
@@ -453,6 +527,7 @@ def main():
                    "slurry_gap_rel": led.slurry.gap_m3 / max(led.slurry.initial_m3, 1e-30),
                    "total_m3": led.total_m3},
         "host_plan_steps": fallbacks,
+        "speculation": {"held": spec_hits, "undone": spec_misses},
     }
     if not args.no_cpu:
         print(f"[bench] CPU baseline, {args.cpu_steps} oracle steps", file=sys.stderr, flush=True)

@@ -54,6 +54,15 @@ struct silt_pair {
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   float last_kernel_ms = 0.0f;
+  bool kernel_ms_pending = false;  // ev0 / ev1 recorded, time not read yet
+  // Speculative launch of the last step's plan (silt_pair_step).
+  bool speculate = true;
+  bool spec_valid = false;
+  float spec_dt = 0.0f;
+  StepPlan spec_plans[2];
+  SheetTotals* totals_save = nullptr;
+  hipEvent_t ev_stats = nullptr;
+  long long spec_hits = 0, spec_misses = 0;
   int last_nsub[2] = {0, 0};
   int last_fallback = 0;
   PlanLabeller lab;
@@ -98,120 +107,12 @@ SheetPlanes layer_planes(silt_pair* p, int layer, int buf) {
 float layer_nu0(const silt_pair* p, int layer) { return layer == 0 ? p->nu_water : p->nu_slurry0; }
 bool layer_nu_uniform(const silt_pair* p, int layer) { return layer == 0 || p->nu_slurry_uniform; }
 
-}  // namespace
-
-extern "C" {
-
-silt_pair* silt_pair_create(size_t res, float cell, int path) {
-  if (res < 1 || res > 65536 || !(cell > 0.0f)) return nullptr;
-  if (path != SILT_PATH_EXACT && path != SILT_PATH_FAST && path != SILT_PATH_FUSED_EXACT) return nullptr;
-  silt_pair* p = new silt_pair();
-  p->res = (int)res;
-  p->cell = cell;
-  p->path = path;
-  p->n = res * res;
-  size_t fb = p->n * sizeof(float);
-  bool ok = hipGetDevice(&p->device) == hipSuccess && hipMalloc(&p->bed, fb) == hipSuccess &&
-            hipMalloc(&p->nu_slurry, fb) == hipSuccess;
-  for (int l = 0; l < 2 && ok; l++)
-    for (int b = 0; b < 2 && ok; b++)
-      for (int k = 0; k < 3 && ok; k++) ok = hipMalloc(&p->planes[l][b][k], fb) == hipSuccess;
-  ok = ok && hipMalloc(&p->stats, 2 * sizeof(SheetStats)) == hipSuccess &&
-       hipMalloc(&p->totals, 2 * sizeof(SheetTotals)) == hipSuccess &&
-       hipMalloc(&p->boxmax, 2 * sizeof(unsigned)) == hipSuccess &&
-       hipHostMalloc(&p->h_stats, 2 * sizeof(SheetStats), hipHostMallocDefault) == hipSuccess &&
-       hipHostMalloc(&p->h_boxmax, 2 * sizeof(unsigned), hipHostMallocDefault) == hipSuccess &&
-       hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) == hipSuccess &&
-       hipEventCreate(&p->ev0) == hipSuccess && hipEventCreate(&p->ev1) == hipSuccess;
-  if (ok) ok = hipMemset(p->totals, 0, 2 * sizeof(SheetTotals)) == hipSuccess;
-  for (int l = 0; l < 2 && ok; l++)
-    for (int b = 0; b < 2 && ok; b++)
-      for (int k = 0; k < 3 && ok; k++) ok = hipMemset(p->planes[l][b][k], 0, fb) == hipSuccess;
-  if (!ok) {
-    silt_pair_destroy(p);
-    return nullptr;
-  }
-  stats_reset(p->stats, 2, p->stream);
-  (void)hipStreamSynchronize(p->stream);
-  p->origin[0] = -0.5f * (float)res * cell;
-  p->origin[1] = -0.5f * (float)res * cell;
-  return p;
-}
-
-void silt_pair_destroy(silt_pair* p) {
-  if (!p) return;
-  if (p->stream) (void)hipStreamSynchronize(p->stream);
-  plan_labeller_free(p->lab);
-  for (PlanScratch& q : p->ps) plan_scratch_free(q);
-  if (p->net_ready) mlp_free(p->net);
-  for (void* q : {(void*)p->soil_bed, (void*)p->soil_placed, (void*)p->soil_init, (void*)p->soil_rut,
-                  (void*)p->d_contacts, (void*)p->sur_ctx, (void*)p->sur_fac, p->d_tires})
-    if (q) (void)hipFree(q);
-  if (p->h_tires) (void)hipHostFree(p->h_tires);
-  for (hipEvent_t e : p->tev)
-    if (e) (void)hipEventDestroy(e);
-  (void)hipFree(p->bed);
-  (void)hipFree(p->nu_slurry);
-  for (int l = 0; l < 2; l++)
-    for (int b = 0; b < 2; b++)
-      for (int k = 0; k < 3; k++) (void)hipFree(p->planes[l][b][k]);
-  (void)hipFree(p->stats);
-  (void)hipFree(p->totals);
-  (void)hipFree(p->boxmax);
-  if (p->h_stats) (void)hipHostFree(p->h_stats);
-  if (p->h_boxmax) (void)hipHostFree(p->h_boxmax);
-  if (p->ev0) (void)hipEventDestroy(p->ev0);
-  if (p->ev1) (void)hipEventDestroy(p->ev1);
-  if (p->stream) (void)hipStreamDestroy(p->stream);
-  delete p;
-}
-
-int silt_pair_upload(silt_pair* p, const float* bed, const float* water, const float* uw, const float* vw,
-                     const float* slurry, const float* us, const float* vs, const float* nu_slurry,
-                     float nu_slurry_uniform) {
-  if (!p || !bed || !water || !uw || !vw || !slurry || !us || !vs) return SILT_EINVAL;
-  bool uniform = isfinite(nu_slurry_uniform);
-  if (!uniform && !nu_slurry) return SILT_EINVAL;
-  size_t fb = p->n * sizeof(float);
-  hipStream_t st = p->stream;
-  HIPCHK(hipMemcpyAsync(p->bed, bed, fb, hipMemcpyHostToDevice, st));
-  const float* src[2][3] = {{water, uw, vw}, {slurry, us, vs}};
-  for (int l = 0; l < 2; l++) {
-    p->cur[l] = 0;
-    for (int k = 0; k < 3; k++) HIPCHK(hipMemcpyAsync(p->planes[l][0][k], src[l][k], fb, hipMemcpyHostToDevice, st));
-  }
-  p->nu_slurry_uniform = uniform;
-  p->nu_slurry0 = uniform ? nu_slurry_uniform : 0.0f;
-  if (!uniform) HIPCHK(hipMemcpyAsync(p->nu_slurry, nu_slurry, fb, hipMemcpyHostToDevice, st));
-  HIPCHK(hipMemsetAsync(p->totals, 0, 2 * sizeof(SheetTotals), st));
-  for (int l = 0; l < 2; l++)
-    stats_compute(p->planes[l][0][0], p->planes[l][0][1], p->planes[l][0][2], p->res, kEps, p->stats + l, st);
-  HIPCHK(hipMemcpyAsync(p->h_stats, p->stats, 2 * sizeof(SheetStats), hipMemcpyDeviceToHost, st));
-  HIPCHK(hipStreamSynchronize(st));
-  for (int l = 0; l < 2; l++) p->vol0[l] = p->h_stats[l].vol;
-  return SILT_OK;
-}
-
-int silt_pair_download(silt_pair* p, float* water, float* uw, float* vw, float* slurry, float* us, float* vs) {
-  if (!p || !water || !uw || !vw || !slurry || !us || !vs) return SILT_EINVAL;
-  size_t fb = p->n * sizeof(float);
-  float* dst[2][3] = {{water, uw, vw}, {slurry, us, vs}};
-  for (int l = 0; l < 2; l++)
-    for (int k = 0; k < 3; k++)
-      HIPCHK(hipMemcpyAsync(dst[l][k], p->planes[l][p->cur[l]][k], fb, hipMemcpyDeviceToHost, p->stream));
-  HIPCHK(hipStreamSynchronize(p->stream));
-  return SILT_OK;
-}
-
-int silt_pair_step(silt_pair* p, float dt) {
-  if (!p) return SILT_EINVAL;
-  if (!(dt > 0.0f)) return 0;
+// The plan of both layers from the statistics in p->h_stats (of the planes
+// p->cur points at): boxes from one batched labelling, substep counts with
+// at most one more wait.
+int pair_plan(silt_pair* p, float dt, StepPlan* plans) {
   hipStream_t st = p->stream;
   const int res = p->res;
-  // Statistics of the current planes, left by the previous step's kernel.
-  HIPCHK(hipMemcpyAsync(p->h_stats, p->stats, 2 * sizeof(SheetStats), hipMemcpyDeviceToHost, st));
-  HIPCHK(hipStreamSynchronize(st));
-  StepPlan plans[2];
   p->last_fallback = 0;
   // Both layers' boxes in one labelling, both substep counts with one wait.
   PlanSheet sheets[2];
@@ -243,6 +144,15 @@ int silt_pair_step(silt_pair* p, float dt) {
   if (wait) HIPCHK(hipStreamSynchronize(st));
   for (int l = 0; l < 2; l++)
     if (!host[l]) plan_nsub_end(p->ps[l], p->cell, dt, &plans[l]);
+  return 0;
+}
+
+// Enqueue the step of both layers for the given plans: fused launches
+// (ping-pong into the other planes) and staged layers (in place).  Returns
+// the largest substep count or a SILT_E* code.
+int pair_launch(silt_pair* p, float dt, const StepPlan* plans) {
+  hipStream_t st = p->stream;
+  const int res = p->res;
   silt_sheet_opts o;
   silt_sheet_opts_default(&o);
   o.path = p->path;
@@ -315,9 +225,199 @@ int silt_pair_step(silt_pair* p, float dt) {
     if (rc != 0) return rc;
   }
   HIPCHK(hipEventRecord(p->ev1, st));
-  HIPCHK(hipEventSynchronize(p->ev1));
-  HIPCHK(hipEventElapsedTime(&p->last_kernel_ms, p->ev0, p->ev1));
+  p->kernel_ms_pending = true;
+  // Staged layers use the process-wide workspace of this side length, which
+  // the next call of another handle may use on its own stream: finish them
+  // here.  Fused launches touch only this handle's buffers and may run on.
+  if (staged) HIPCHK(hipStreamSynchronize(st));
   return std::max(p->last_nsub[0], p->last_nsub[1]);
+}
+
+// Whether a plan may be launched before it is confirmed: every layer fused
+// with one substep, so the step only writes the other planes and the
+// current ones stay as they were.
+bool pair_speculable(const silt_pair* p, const StepPlan* plans) {
+  if (p->path == SILT_PATH_EXACT) return false;
+  for (int l = 0; l < 2; l++) {
+    if (plans[l].boxes.empty()) continue;
+    if (plans[l].nsub != 1 || plans[l].pace3) return false;
+    for (const Box& b : plans[l].boxes)
+      if (!reference_uses_jacobi(b, p->cell, 0, true)) return false;
+  }
+  return true;
+}
+
+bool same_plans(const StepPlan* a, const StepPlan* b) {
+  for (int l = 0; l < 2; l++) {
+    if (a[l].nsub != b[l].nsub || a[l].pace3 != b[l].pace3 || a[l].boxes.size() != b[l].boxes.size()) return false;
+    for (size_t k = 0; k < a[l].boxes.size(); k++) {
+      const Box &x = a[l].boxes[k], &y = b[l].boxes[k];
+      if (x.i0 != y.i0 || x.i1 != y.i1 || x.j0 != y.j0 || x.j1 != y.j1) return false;
+    }
+  }
+  return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+silt_pair* silt_pair_create(size_t res, float cell, int path) {
+  if (res < 1 || res > 65536 || !(cell > 0.0f)) return nullptr;
+  if (path != SILT_PATH_EXACT && path != SILT_PATH_FAST && path != SILT_PATH_FUSED_EXACT) return nullptr;
+  silt_pair* p = new silt_pair();
+  p->res = (int)res;
+  p->cell = cell;
+  p->path = path;
+  p->n = res * res;
+  size_t fb = p->n * sizeof(float);
+  bool ok = hipGetDevice(&p->device) == hipSuccess && hipMalloc(&p->bed, fb) == hipSuccess &&
+            hipMalloc(&p->nu_slurry, fb) == hipSuccess;
+  for (int l = 0; l < 2 && ok; l++)
+    for (int b = 0; b < 2 && ok; b++)
+      for (int k = 0; k < 3 && ok; k++) ok = hipMalloc(&p->planes[l][b][k], fb) == hipSuccess;
+  ok = ok && hipMalloc(&p->stats, 2 * sizeof(SheetStats)) == hipSuccess &&
+       hipMalloc(&p->totals, 2 * sizeof(SheetTotals)) == hipSuccess &&
+       hipMalloc(&p->totals_save, 2 * sizeof(SheetTotals)) == hipSuccess &&
+       hipEventCreateWithFlags(&p->ev_stats, hipEventDisableTiming) == hipSuccess &&
+       hipMalloc(&p->boxmax, 2 * sizeof(unsigned)) == hipSuccess &&
+       hipHostMalloc(&p->h_stats, 2 * sizeof(SheetStats), hipHostMallocDefault) == hipSuccess &&
+       hipHostMalloc(&p->h_boxmax, 2 * sizeof(unsigned), hipHostMallocDefault) == hipSuccess &&
+       hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) == hipSuccess &&
+       hipEventCreate(&p->ev0) == hipSuccess && hipEventCreate(&p->ev1) == hipSuccess;
+  if (ok) ok = hipMemset(p->totals, 0, 2 * sizeof(SheetTotals)) == hipSuccess;
+  for (int l = 0; l < 2 && ok; l++)
+    for (int b = 0; b < 2 && ok; b++)
+      for (int k = 0; k < 3 && ok; k++) ok = hipMemset(p->planes[l][b][k], 0, fb) == hipSuccess;
+  if (!ok) {
+    silt_pair_destroy(p);
+    return nullptr;
+  }
+  stats_reset(p->stats, 2, p->stream);
+  (void)hipStreamSynchronize(p->stream);
+  p->origin[0] = -0.5f * (float)res * cell;
+  p->origin[1] = -0.5f * (float)res * cell;
+  return p;
+}
+
+void silt_pair_destroy(silt_pair* p) {
+  if (!p) return;
+  if (p->stream) (void)hipStreamSynchronize(p->stream);
+  plan_labeller_free(p->lab);
+  for (PlanScratch& q : p->ps) plan_scratch_free(q);
+  if (p->net_ready) mlp_free(p->net);
+  for (void* q : {(void*)p->soil_bed, (void*)p->soil_placed, (void*)p->soil_init, (void*)p->soil_rut,
+                  (void*)p->d_contacts, (void*)p->sur_ctx, (void*)p->sur_fac, p->d_tires})
+    if (q) (void)hipFree(q);
+  if (p->h_tires) (void)hipHostFree(p->h_tires);
+  for (hipEvent_t e : p->tev)
+    if (e) (void)hipEventDestroy(e);
+  (void)hipFree(p->bed);
+  (void)hipFree(p->nu_slurry);
+  for (int l = 0; l < 2; l++)
+    for (int b = 0; b < 2; b++)
+      for (int k = 0; k < 3; k++) (void)hipFree(p->planes[l][b][k]);
+  (void)hipFree(p->stats);
+  (void)hipFree(p->totals);
+  if (p->totals_save) (void)hipFree(p->totals_save);
+  if (p->ev_stats) (void)hipEventDestroy(p->ev_stats);
+  (void)hipFree(p->boxmax);
+  if (p->h_stats) (void)hipHostFree(p->h_stats);
+  if (p->h_boxmax) (void)hipHostFree(p->h_boxmax);
+  if (p->ev0) (void)hipEventDestroy(p->ev0);
+  if (p->ev1) (void)hipEventDestroy(p->ev1);
+  if (p->stream) (void)hipStreamDestroy(p->stream);
+  delete p;
+}
+
+int silt_pair_upload(silt_pair* p, const float* bed, const float* water, const float* uw, const float* vw,
+                     const float* slurry, const float* us, const float* vs, const float* nu_slurry,
+                     float nu_slurry_uniform) {
+  if (!p || !bed || !water || !uw || !vw || !slurry || !us || !vs) return SILT_EINVAL;
+  bool uniform = isfinite(nu_slurry_uniform);
+  if (!uniform && !nu_slurry) return SILT_EINVAL;
+  size_t fb = p->n * sizeof(float);
+  hipStream_t st = p->stream;
+  HIPCHK(hipMemcpyAsync(p->bed, bed, fb, hipMemcpyHostToDevice, st));
+  const float* src[2][3] = {{water, uw, vw}, {slurry, us, vs}};
+  for (int l = 0; l < 2; l++) {
+    p->cur[l] = 0;
+    for (int k = 0; k < 3; k++) HIPCHK(hipMemcpyAsync(p->planes[l][0][k], src[l][k], fb, hipMemcpyHostToDevice, st));
+  }
+  p->nu_slurry_uniform = uniform;
+  p->nu_slurry0 = uniform ? nu_slurry_uniform : 0.0f;
+  if (!uniform) HIPCHK(hipMemcpyAsync(p->nu_slurry, nu_slurry, fb, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemsetAsync(p->totals, 0, 2 * sizeof(SheetTotals), st));
+  for (int l = 0; l < 2; l++)
+    stats_compute(p->planes[l][0][0], p->planes[l][0][1], p->planes[l][0][2], p->res, kEps, p->stats + l, st);
+  HIPCHK(hipMemcpyAsync(p->h_stats, p->stats, 2 * sizeof(SheetStats), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  for (int l = 0; l < 2; l++) p->vol0[l] = p->h_stats[l].vol;
+  return SILT_OK;
+}
+
+int silt_pair_download(silt_pair* p, float* water, float* uw, float* vw, float* slurry, float* us, float* vs) {
+  if (!p || !water || !uw || !vw || !slurry || !us || !vs) return SILT_EINVAL;
+  size_t fb = p->n * sizeof(float);
+  float* dst[2][3] = {{water, uw, vw}, {slurry, us, vs}};
+  for (int l = 0; l < 2; l++)
+    for (int k = 0; k < 3; k++)
+      HIPCHK(hipMemcpyAsync(dst[l][k], p->planes[l][p->cur[l]][k], fb, hipMemcpyDeviceToHost, p->stream));
+  HIPCHK(hipStreamSynchronize(p->stream));
+  return SILT_OK;
+}
+
+int silt_pair_step(silt_pair* p, float dt) {
+  if (!p) return SILT_EINVAL;
+  if (!(dt > 0.0f)) return 0;
+  hipStream_t st = p->stream;
+  // Speculation: the last step's plan is launched at once, while the
+  // statistics of the current planes come back and the true plan is made;
+  // a plan that differs undoes the launch (the current planes are untouched,
+  // the ledger lines are restored) and launches the true plan.  The GPU
+  // then does not wait for the host's planning between steps.
+  const bool spec = p->speculate && p->spec_valid && p->spec_dt == dt;
+  int cur0[2] = {p->cur[0], p->cur[1]};
+  if (spec) {
+    HIPCHK(hipMemcpyAsync(p->h_stats, p->stats, 2 * sizeof(SheetStats), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipEventRecord(p->ev_stats, st));
+    HIPCHK(hipMemcpyAsync(p->totals_save, p->totals, 2 * sizeof(SheetTotals), hipMemcpyDeviceToDevice, st));
+    int rc = pair_launch(p, dt, p->spec_plans);
+    if (rc < 0) return rc;
+    HIPCHK(hipEventSynchronize(p->ev_stats));
+  } else {
+    HIPCHK(hipMemcpyAsync(p->h_stats, p->stats, 2 * sizeof(SheetStats), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+  }
+  // Plan from the statistics of the planes the step started from.
+  int cur1[2] = {p->cur[0], p->cur[1]};
+  p->cur[0] = cur0[0];
+  p->cur[1] = cur0[1];
+  StepPlan plans[2];
+  int rc = pair_plan(p, dt, plans);
+  p->cur[0] = cur1[0];
+  p->cur[1] = cur1[1];
+  if (rc < 0) return rc;
+  if (spec) {
+    if (same_plans(plans, p->spec_plans)) {
+      p->spec_hits++;
+      return std::max(p->last_nsub[0], p->last_nsub[1]);
+    }
+    // Undo: back to the planes the step started from, ledger lines as before.
+    p->cur[0] = cur0[0];
+    p->cur[1] = cur0[1];
+    HIPCHK(hipMemcpyAsync(p->totals, p->totals_save, 2 * sizeof(SheetTotals), hipMemcpyDeviceToDevice, st));
+    p->spec_misses++;
+  }
+  rc = pair_launch(p, dt, plans);
+  if (rc < 0) return rc;
+  p->spec_valid = pair_speculable(p, plans);
+  if (p->spec_valid) {
+    p->spec_plans[0] = plans[0];
+    p->spec_plans[1] = plans[1];
+    p->spec_dt = dt;
+  }
+  return rc;
 }
 
 int silt_pair_ledger(silt_pair* p, silt_pair_ledger_t* out) {
@@ -339,7 +439,32 @@ int silt_pair_ledger(silt_pair* p, silt_pair_ledger_t* out) {
   return SILT_OK;
 }
 
-float silt_pair_last_kernel_ms(const silt_pair* p) { return p ? p->last_kernel_ms : 0.0f; }
+float silt_pair_last_kernel_ms(const silt_pair* p) {
+  if (!p) return 0.0f;
+  silt_pair* q = const_cast<silt_pair*>(p);
+  if (q->kernel_ms_pending) {
+    // The step returns while its launch may still run: wait for it here.
+    if (hipEventSynchronize(q->ev1) != hipSuccess ||
+        hipEventElapsedTime(&q->last_kernel_ms, q->ev0, q->ev1) != hipSuccess)
+      q->last_kernel_ms = 0.0f;
+    q->kernel_ms_pending = false;
+  }
+  return q->last_kernel_ms;
+}
+
+int silt_pair_set_speculation(silt_pair* p, int on) {
+  if (!p) return SILT_EINVAL;
+  p->speculate = on != 0;
+  p->spec_valid = false;
+  return SILT_OK;
+}
+
+int silt_pair_speculation_counts(const silt_pair* p, long long* hits, long long* misses) {
+  if (!p) return SILT_EINVAL;
+  if (hits) *hits = p->spec_hits;
+  if (misses) *misses = p->spec_misses;
+  return SILT_OK;
+}
 
 int silt_pair_autotune(silt_pair* p, float dt, const int* rows, size_t n, int reps, int* best_rows, float* best_ms) {
   if (!p || !(dt > 0.0f) || reps < 1 || reps > 1000 || (n && !rows)) return SILT_EINVAL;
@@ -391,7 +516,7 @@ int silt_pair_autotune(silt_pair* p, float dt, const int* rows, size_t n, int re
     double sum = 0.0;
     for (int k = 0; k < reps && rc >= 0; k++) {
       rc = silt_pair_step(p, dt);
-      sum += p->last_kernel_ms;
+      sum += silt_pair_last_kernel_ms(p);
     }
     int rr = restore();
     if (rc >= 0) rc = rr;

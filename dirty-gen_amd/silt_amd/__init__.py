@@ -111,6 +111,8 @@ def _declare(L):
     L.silt_pair_autotune.argtypes = [_P, _F, C.POINTER(C.c_int), _Z, C.c_int, C.POINTER(C.c_int),
                                      C.POINTER(C.c_float)]
     L.silt_pair_autotune.restype = C.c_int
+    L.silt_pair_set_speculation.argtypes = [_P, C.c_int]
+    L.silt_pair_speculation_counts.argtypes = [_P, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
     L.silt_batch_create.argtypes = [_Z, C.POINTER(C.c_size_t), C.POINTER(C.c_float), C.c_int]
     L.silt_batch_create.restype = _P
     L.silt_batch_destroy.argtypes = [_P]
@@ -351,6 +353,16 @@ class SheetPair:
 
     def last_fallback(self):
         return lib().silt_pair_last_fallback(self._h)
+
+    def set_speculation(self, on):
+        """Launch the last step's plan before this step's plan is confirmed
+        (default on); a different plan is undone and redone."""
+        _check_status(lib().silt_pair_set_speculation(self._h, 1 if on else 0))
+
+    def speculation_counts(self):
+        h, m = C.c_longlong(0), C.c_longlong(0)
+        _check_status(lib().silt_pair_speculation_counts(self._h, C.byref(h), C.byref(m)))
+        return h.value, m.value
 
     def _tires_declared(self):
         L = lib()

@@ -55,14 +55,25 @@ int silt_pair_upload(silt_pair *p, const float *bed, const float *water, const f
 int silt_pair_download(silt_pair *p, float *water, float *uw, float *vw, float *slurry, float *us, float *vs);
 
 /* One step of both layers.  Returns the larger substep count (0: both dry or
- * dt <= 0), or a negative SILT_E* code. */
+ * dt <= 0), or a negative SILT_E* code.  The step's launches may still run
+ * when it returns; download, ledger and silt_pair_last_kernel_ms wait for
+ * them.  With speculation on (the default), the last step's plan is
+ * launched before this step's plan is confirmed from the statistics; a
+ * different plan undoes it (the starting planes are untouched, the ledger
+ * lines restored) and launches the right one, so results do not depend on
+ * it. */
 int silt_pair_step(silt_pair *p, float dt);
 
 int silt_pair_ledger(silt_pair *p, silt_pair_ledger_t *out);
 
 /* GPU time of the last step's kernels, milliseconds (HIP events on the
- * handle's stream). */
+ * handle's stream; waits for the step's launches). */
 float silt_pair_last_kernel_ms(const silt_pair *p);
+/* Speculative launch of the last plan (default on); no counterpart in the
+ * reference: a host-latency hiding switch of the GPU path.  Counts of
+ * speculations that held and that were undone. */
+int silt_pair_set_speculation(silt_pair *p, int on);
+int silt_pair_speculation_counts(const silt_pair *p, long long *hits, long long *misses);
 /* Bit l set when layer l's last plan read its planes back to the host (a
  * 2 cm sheet, or more than 4096 separate wet bodies; other multi-body wet
  * sets are labelled on the GPU). */
