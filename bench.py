@@ -429,7 +429,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--res", type=int, default=4096)
     ap.add_argument("--path", default="fast", choices=["fast", "fused_exact", "exact"])
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=7, help="pair: oracle steps of the CPU baseline (about 10 s)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--core-hz", type=int, default=120, help="core: steps per second of simulated time (dt = 1/hz)")
     ap.add_argument("--workload", default="pair", choices=["pair", "batch", "surrogate", "large", "core"],

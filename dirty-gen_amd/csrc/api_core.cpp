@@ -105,6 +105,8 @@ struct silt_core {
   // batched wet-set labelling, the substep maxima.
   SheetStats* d_stats = nullptr;
   SheetStats* h_stats = nullptr;  // pinned
+  // Scratch of the publish calls: 10 output planes and 3 soil inputs.
+  float* d_pub = nullptr;
   PlanLabeller lab;
   PlanScratch ps[2];
   hipStream_t st = nullptr;
@@ -659,7 +661,7 @@ silt_core* silt_core_create(float center_x, float center_z, int path) {
        dalloc(c, &c->d_cols, 1024) && dalloc(c, &c->d_ev, kSprayCap) && dalloc(c, &c->d_nev, 1) &&
        dalloc(c, &c->d_acc, 4) && dalloc(c, &c->soil_bed, WN) && dalloc(c, &c->soil_placed, WN) &&
        dalloc(c, &c->soil_init, WN) && dalloc(c, &c->soil_rut, WN) && dalloc(c, &c->d_contacts, kMaxPush) &&
-       dalloc(c, &c->d_stats, 2) &&
+       dalloc(c, &c->d_stats, 2) && dalloc(c, &c->d_pub, 13 * WN) &&
        hipHostMalloc((void**)&c->h_stats, 2 * sizeof(SheetStats), hipHostMallocDefault) == hipSuccess;
   if (ok) {
     // Grain::Other everywhere (FluidCore::new, src/fluid_core.rs:4979-4980).
@@ -984,69 +986,52 @@ int silt_core_get_ground(silt_core* c, float* bed, float* placed, float* rut) {
 int silt_core_publish(silt_core* c, const float* soil_bed, const float* soil_placed, const float* soil_rut,
                       float* ground_rut4, float* film2, float* flow4) {
   if (!c) return SILT_EINVAL;
-  if (c->ground && !soil_bed && !soil_placed && !soil_rut) {
-    // The core's own ground: publish it without a round trip through the host.
-    float* d_out = nullptr;
-    if (hipMalloc(&d_out, (size_t)WN * 10 * sizeof(float)) != hipSuccess) return SILT_ENOMEM;
-    core_publish(c->c, c->soil_bed, c->soil_placed, c->soil_rut, d_out, d_out + 4 * WN, d_out + 6 * WN, c->st);
-    if (ground_rut4) HIPCHK(hipMemcpyAsync(ground_rut4, d_out, 4 * WN * 4, hipMemcpyDeviceToHost, c->st));
-    if (film2) HIPCHK(hipMemcpyAsync(film2, d_out + 4 * WN, 2 * WN * 4, hipMemcpyDeviceToHost, c->st));
-    if (flow4) HIPCHK(hipMemcpyAsync(flow4, d_out + 6 * WN, 4 * WN * 4, hipMemcpyDeviceToHost, c->st));
-    HIPCHK(hipStreamSynchronize(c->st));
-    (void)hipFree(d_out);
-    return SILT_OK;
-  }
-  float *d_out = nullptr, *d_soil = nullptr;
-  if (hipMalloc(&d_out, (size_t)WN * 10 * sizeof(float)) != hipSuccess) return SILT_ENOMEM;
-  if (hipMalloc(&d_soil, (size_t)WN * 3 * sizeof(float)) != hipSuccess) {
-    (void)hipFree(d_out);
-    return SILT_ENOMEM;
-  }
-  const float* soils[3] = {soil_bed, soil_placed, soil_rut};
+  float* d_out = c->d_pub;
+  float* d_soil = c->d_pub + 10 * WN;
   const float* dsoil[3] = {nullptr, nullptr, nullptr};
-  for (int k = 0; k < 3; k++)
-    if (soils[k]) {
-      HIPCHK(hipMemcpyAsync(d_soil + (size_t)k * WN, soils[k], WN * 4, hipMemcpyHostToDevice, c->st));
-      dsoil[k] = d_soil + (size_t)k * WN;
-    }
+  if (c->ground && !soil_bed && !soil_placed && !soil_rut) {
+    // The core's own ground: published without a round trip through the host.
+    dsoil[0] = c->soil_bed;
+    dsoil[1] = c->soil_placed;
+    dsoil[2] = c->soil_rut;
+  } else {
+    const float* soils[3] = {soil_bed, soil_placed, soil_rut};
+    for (int k = 0; k < 3; k++)
+      if (soils[k]) {
+        HIPCHK(hipMemcpyAsync(d_soil + (size_t)k * WN, soils[k], WN * 4, hipMemcpyHostToDevice, c->st));
+        dsoil[k] = d_soil + (size_t)k * WN;
+      }
+  }
   core_publish(c->c, dsoil[0], dsoil[1], dsoil[2], d_out, d_out + 4 * WN, d_out + 6 * WN, c->st);
   if (ground_rut4) HIPCHK(hipMemcpyAsync(ground_rut4, d_out, 4 * WN * 4, hipMemcpyDeviceToHost, c->st));
   if (film2) HIPCHK(hipMemcpyAsync(film2, d_out + 4 * WN, 2 * WN * 4, hipMemcpyDeviceToHost, c->st));
   if (flow4) HIPCHK(hipMemcpyAsync(flow4, d_out + 6 * WN, 4 * WN * 4, hipMemcpyDeviceToHost, c->st));
   HIPCHK(hipStreamSynchronize(c->st));
-  (void)hipFree(d_out);
-  (void)hipFree(d_soil);
   return SILT_OK;
 }
 
 int silt_core_publish_surface(silt_core* c, const float* soil_wetness, const float* soil_placed, float* moisture1,
                               float* wet_edge1, float* foam_crest2) {
   if (!c) return SILT_EINVAL;
-  // Device scratch: the two optional soil inputs, then the three outputs.
-  float* d = nullptr;
-  if (hipMalloc(&d, (size_t)WN * 6 * sizeof(float)) != hipSuccess) return SILT_ENOMEM;
-  float *d_wet = nullptr, *d_placed = nullptr;
-  int rc = SILT_OK;
+  // Scratch: the two optional soil inputs, then the three outputs.
+  float* d = c->d_pub;
+  const float *d_wet = nullptr, *d_placed = nullptr;
   if (soil_wetness) {
+    HIPCHK(hipMemcpyAsync(d, soil_wetness, WN * 4, hipMemcpyHostToDevice, c->st));
     d_wet = d;
-    if (hipMemcpyAsync(d_wet, soil_wetness, WN * 4, hipMemcpyHostToDevice, c->st) != hipSuccess) rc = SILT_EDEVICE;
   }
   if (soil_placed) {
+    HIPCHK(hipMemcpyAsync(d + WN, soil_placed, WN * 4, hipMemcpyHostToDevice, c->st));
     d_placed = d + WN;
-    if (hipMemcpyAsync(d_placed, soil_placed, WN * 4, hipMemcpyHostToDevice, c->st) != hipSuccess) rc = SILT_EDEVICE;
   } else if (c->ground) {
     d_placed = c->soil_placed;  // the core's own ground, as silt_core_publish publishes it
   }
-  if (rc == SILT_OK) {
-    core_publish_surface(c->c, d_wet, d_placed, kWindowRes, d + 2 * WN, d + 3 * WN, d + 4 * WN, c->st);
-    if ((moisture1 && hipMemcpyAsync(moisture1, d + 2 * WN, WN * 4, hipMemcpyDeviceToHost, c->st) != hipSuccess) ||
-        (wet_edge1 && hipMemcpyAsync(wet_edge1, d + 3 * WN, WN * 4, hipMemcpyDeviceToHost, c->st) != hipSuccess) ||
-        (foam_crest2 && hipMemcpyAsync(foam_crest2, d + 4 * WN, 2 * WN * 4, hipMemcpyDeviceToHost, c->st) != hipSuccess))
-      rc = SILT_EDEVICE;
-  }
-  if (hipStreamSynchronize(c->st) != hipSuccess) rc = SILT_EDEVICE;
-  (void)hipFree(d);
-  return rc;
+  core_publish_surface(c->c, d_wet, d_placed, kWindowRes, d + 2 * WN, d + 3 * WN, d + 4 * WN, c->st);
+  if (moisture1) HIPCHK(hipMemcpyAsync(moisture1, d + 2 * WN, WN * 4, hipMemcpyDeviceToHost, c->st));
+  if (wet_edge1) HIPCHK(hipMemcpyAsync(wet_edge1, d + 3 * WN, WN * 4, hipMemcpyDeviceToHost, c->st));
+  if (foam_crest2) HIPCHK(hipMemcpyAsync(foam_crest2, d + 4 * WN, 2 * WN * 4, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  return SILT_OK;
 }
 
 int silt_core_fine_water_debug(silt_core* c, const silt_tire_push* pushes, size_t n, float* water, float* uw, float* vw,
