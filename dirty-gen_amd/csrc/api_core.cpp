@@ -109,6 +109,7 @@ struct silt_core {
   float* d_pub = nullptr;
   PlanLabeller lab;
   PlanScratch ps[2];
+  FineScratch fs[2];  // the 2 cm sheets' tile tests, per layer
   hipStream_t st = nullptr;
   std::vector<void*> allocs;
 };
@@ -425,7 +426,17 @@ int step_pair(silt_core* c, float dt, bool fine) {
   s.v = fine ? c->f.vw : c->c.vw;
   s.bed = fine ? c->f.bed : c->c.bed;
   silt_drive dw{SILT_DRIVE_GRAVITY, 0.0f, 0.0f, 0.08f};
-  int rc = step_device_planes(s, res, cell, dt, dw, o, true, true, nu_w, c->st);
+  // The 2 cm sheet plans from its tiles' tests on the GPU (paces 0 and 1);
+  // the whole-wet paces read the planes back.
+  auto fine_step = [&](const SheetPlanes& sp, const silt_drive& d, bool nu_uni, float nu0, FineScratch& fsc,
+                       hipStream_t q) -> int {
+    StepPlan plan;
+    int pr = plan_fine_device(sp.h, sp.u, sp.v, sp.bed, res, cell, dt, o.eps, o.sub_cap, o.pace, fsc, &plan, q);
+    if (pr < 0) return pr;
+    if (pr == 0) return step_device_planes(sp, res, cell, dt, d, o, true, nu_uni, nu0, q);
+    return plan.boxes.empty() ? 0 : run_plan(sp, plan, res, cell, dt, d, o, true, nu_uni, nu0, q);
+  };
+  int rc = fine_step(s, dw, true, nu_w, c->fs[0], c->st);
   if (rc < 0) return rc;
   // Slurry layer at its own viscosity, on the second stream.
   SheetPlanes s2 = ws2->planes();
@@ -438,7 +449,7 @@ int step_pair(silt_core* c, float dt, bool fine) {
   s2.bed = fine ? c->f.bed_s : c->c.bed;
   s2.nu = nu;
   silt_drive ds{SILT_DRIVE_GRAVITY, 0.0f, 0.0f, 0.05f};
-  rc = step_device_planes(s2, res, cell, dt, ds, o, true, false, 0.0f, c->st2);
+  rc = fine_step(s2, ds, false, 0.0f, c->fs[1], c->st2);
   HIPCHK(hipEventRecord(c->join, c->st2));
   HIPCHK(hipStreamWaitEvent(c->st, c->join, 0));
   return rc < 0 ? rc : 0;
@@ -689,6 +700,7 @@ void silt_core_destroy(silt_core* c) {
   if (c->h_stats) (void)hipHostFree(c->h_stats);
   plan_labeller_free(c->lab);
   for (PlanScratch& q : c->ps) plan_scratch_free(q);
+  for (FineScratch& q : c->fs) fine_scratch_free(q);
   if (c->fork) (void)hipEventDestroy(c->fork);
   if (c->join) (void)hipEventDestroy(c->join);
   if (c->st2) (void)hipStreamDestroy(c->st2);

@@ -147,7 +147,7 @@ std::vector<Box> awake_work_boxes(const float* h, const float* u, const float* v
                                   float cell, float eps) {
   const int T = 8;
   int tiles = (res + T - 1) / T;
-  std::vector<uint8_t> hot((size_t)tiles * tiles, 0), grown((size_t)tiles * tiles, 0), seen((size_t)tiles * tiles, 0);
+  std::vector<uint8_t> hot((size_t)tiles * tiles, 0);
   for (int tj = 0; tj < tiles; tj++)
     for (int ti = 0; ti < tiles; ti++) {
       int ia = ti * T, ja = tj * T, ib = std::min(ia + T, res), jb = std::min(ja + T, res);
@@ -172,6 +172,13 @@ std::vector<Box> awake_work_boxes(const float* h, const float* u, const float* v
         }
       if (awake) hot[(size_t)tj * tiles + ti] = 1;
     }
+  return awake_boxes_from_hot(hot, res);
+}
+
+std::vector<Box> awake_boxes_from_hot(const std::vector<uint8_t>& hot, int res) {
+  const int T = 8;
+  int tiles = (res + T - 1) / T;
+  std::vector<uint8_t> grown((size_t)tiles * tiles, 0), seen((size_t)tiles * tiles, 0);
   for (int tj = 0; tj < tiles; tj++)
     for (int ti = 0; ti < tiles; ti++) {
       if (!hot[(size_t)tj * tiles + ti]) continue;

@@ -28,6 +28,9 @@ std::vector<Box> boxes_from_bodies(std::vector<Box> bodies, int res, const Box& 
 // awake_work_boxes (src/fluid_core.rs:1537-1657).
 std::vector<Box> awake_work_boxes(const float* h, const float* u, const float* v, const float* bed, int res,
                                   float cell, float eps);
+// The boxes of awake_work_boxes from its 8 x 8 tiles' awake flags
+// (row-major, ceil(res / 8) per side): grown by one tile, 4-connected.
+std::vector<Box> awake_boxes_from_hot(const std::vector<uint8_t>& hot, int res);
 
 // Whole plan from host copies of the planes.  Returns an empty plan when the
 // reference's early returns fire (dry sheet, dt <= 0).

@@ -39,6 +39,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <cmath>
 #include <utility>
 #include <vector>
 
@@ -349,6 +350,52 @@ __global__ __launch_bounds__(256) void k_tile_boxes(JobStarts js, const PlanJob*
   wave_extreme(bodies, nslots, J->slot, top, id, 4, first, false);
 }
 
+// awake_work_boxes' tile test (src/fluid_core.rs:1537-1600) for a 2 cm
+// sheet: one wavefront per 8 x 8 tile, one lane per cell.  The speed test
+// is |(u, v)| > 0.05 with the magnitude formed as the host's hypotf forms it
+// (single-precision result of the double-precision root of the double sum
+// of squares); the tile also keeps its largest u^2 + v^2 (NaN never wins,
+// as in the host's scan) and whether it holds a wet cell.
+__global__ __launch_bounds__(64) void k_awake_tiles(const float* __restrict__ h, const float* __restrict__ u,
+                                                    const float* __restrict__ v, const float* __restrict__ bed,
+                                                    int res, float cell, float eps, unsigned char* hot,
+                                                    unsigned* s2_bits, unsigned char* wet_any) {
+  const int tiles = (res + 7) / 8;
+  const int t = (int)blockIdx.x;
+  const int ti = t % tiles, tj = t / tiles;
+  const int lane = (int)threadIdx.x;
+  const int i = ti * 8 + (lane & 7), j = tj * 8 + (lane >> 3);
+  bool awake = false, wet = false;
+  float s2 = 0.0f;
+  if (i < res && j < res) {
+    const size_t idx = (size_t)j * res + i;
+    const float uu = u[idx], vv = v[idx];
+    const float q = uu * uu + vv * vv;
+    s2 = q > 0.0f ? q : 0.0f;
+    const float hv = h[idx];
+    wet = hv > eps;
+    if (wet) {
+      const float speed = (float)sqrt((double)uu * (double)uu + (double)vv * (double)vv);
+      const float eta = bed[idx] + hv;
+      const size_t ip = i + 1 < res ? idx + 1 : idx;
+      const size_t jp = j + 1 < res ? idx + res : idx;
+      const float dex = fabsf(bed[ip] + h[ip] - eta);
+      const float dez = fabsf(bed[jp] + h[jp] - eta);
+      awake = speed > 0.05f || fmaxf(dex, dez) > 0.02f * cell;
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    const float w = __shfl_xor(s2, o);
+    s2 = w > s2 ? w : s2;
+  }
+  const bool any_awake = __ballot(awake) != 0ull, any_wet = __ballot(wet) != 0ull;
+  if (lane == 0) {
+    hot[t] = any_awake ? 1 : 0;
+    wet_any[t] = any_wet ? 1 : 0;
+    s2_bits[t] = __float_as_uint(s2);
+  }
+}
+
 __global__ void k_zero_u32(unsigned* p, int n) {
   int k = (int)(blockIdx.x * blockDim.x + threadIdx.x);
   if (k < n) p[k] = 0u;
@@ -535,6 +582,68 @@ void plan_nsub_end(PlanScratch& ps, float cell, float dt, StepPlan* plan) {
   float s2;
   memcpy(&s2, ps.h_maxs2, 4);
   plan->nsub = nsub_from_s2(s2, dt, cell);
+}
+
+int plan_fine_device(const float* h, const float* u, const float* v, const float* bed, int res, float cell, float dt,
+                     float eps, float sub_cap, int pace, FineScratch& fs, StepPlan* plan, hipStream_t stream) {
+  *plan = StepPlan{};
+  if (!(dt > 0.0f) || res <= 0) return 1;
+  if (pace == 2 || pace == 3) return 0;  // the whole-wet plans scan the planes on the host
+  const int tiles = (res + 7) / 8, nt = tiles * tiles;
+  if (fs.cap_tiles < nt) {
+    if (fs.d) (void)hipFree(fs.d);
+    if (fs.h) (void)hipHostFree(fs.h);
+    fs.d = fs.h = nullptr;
+    fs.cap_tiles = 0;
+    if (hipMalloc(&fs.d, (size_t)nt * 6) != hipSuccess || hipHostMalloc(&fs.h, (size_t)nt * 6, 0) != hipSuccess)
+      return SILT_ENOMEM;
+    fs.cap_tiles = nt;
+  }
+  // Layout: s2 bits [nt] u32, hot [nt] u8, wet [nt] u8.
+  unsigned* d_s2 = reinterpret_cast<unsigned*>(fs.d);
+  unsigned char* d_hot = reinterpret_cast<unsigned char*>(fs.d) + (size_t)nt * 4;
+  unsigned char* d_wet = d_hot + nt;
+  hipLaunchKernelGGL(k_awake_tiles, dim3((unsigned)nt), dim3(64), 0, stream, h, u, v, bed, res, cell, eps, d_hot, d_s2,
+                     d_wet);
+  if (hipMemcpyAsync(fs.h, fs.d, (size_t)nt * 6, hipMemcpyDeviceToHost, stream) != hipSuccess ||
+      hipStreamSynchronize(stream) != hipSuccess)
+    return SILT_EDEVICE;
+  const unsigned* s2 = reinterpret_cast<const unsigned*>(fs.h);
+  const unsigned char* hot = reinterpret_cast<const unsigned char*>(fs.h) + (size_t)nt * 4;
+  const unsigned char* wet = hot + nt;
+  // plan_step's wet-sum test: a sum of depths above eps exceeds eps exactly
+  // when one cell is above eps.
+  bool any = false;
+  for (int k = 0; k < nt && !any; k++) any = wet[k] != 0;
+  if (!any) return 1;
+  plan->fine_open = pace == 0;
+  plan->boxes = awake_boxes_from_hot(std::vector<uint8_t>(hot, hot + nt), res);
+  if (plan->boxes.empty()) return 1;
+  if (plan->fine_open) {
+    plan->nsub = 1;
+    return 1;
+  }
+  // The boxes are whole tiles (clipped to the grid): their speed maximum is
+  // the maximum of their tiles'.
+  float max_s2 = 0.0f;
+  for (const Box& b : plan->boxes)
+    for (int tj = b.j0 / 8; tj <= b.j1 / 8; tj++)
+      for (int ti = b.i0 / 8; ti <= b.i1 / 8; ti++) {
+        float q;
+        memcpy(&q, &s2[tj * tiles + ti], 4);
+        if (q > max_s2) max_s2 = q;
+      }
+  const float max_s = fmaxf(0.15f, sqrtf(max_s2));
+  const float cap = std::isfinite(sub_cap) ? clampf(sub_cap, 1.0f, 8.0f) : 8.0f;
+  const float q = ceilf((dt * max_s) / (0.42f * fmaxf(cell, 1.0e-3f)));
+  plan->nsub = (int)clampf(q, 1.0f, cap);
+  return 1;
+}
+
+void fine_scratch_free(FineScratch& fs) {
+  if (fs.d) (void)hipFree(fs.d);
+  if (fs.h) (void)hipHostFree(fs.h);
+  fs = FineScratch{};
 }
 
 void plan_scratch_free(PlanScratch& ps) {

@@ -109,6 +109,20 @@ int plan_nsub_begin(const float* u, const float* v, int res, float cell, float d
 void plan_nsub_end(PlanScratch& ps, float cell, float dt, StepPlan* plan);
 void plan_scratch_free(PlanScratch& ps);
 
+// Plan of a 2 cm sheet on the GPU for paces 0 and 1 (awake_work_boxes'
+// tiles tested on the device, their flags and speed maxima read back, the
+// boxes and substep count made on the host as plan_step makes them).
+// Returns 1 when planned, 0 when the host must plan (paces 2 and 3), or a
+// SILT_E* code.
+struct FineScratch {
+  void* d = nullptr;
+  void* h = nullptr;  // pinned
+  int cap_tiles = 0;
+};
+int plan_fine_device(const float* h, const float* u, const float* v, const float* bed, int res, float cell, float dt,
+                     float eps, float sub_cap, int pace, FineScratch& fs, StepPlan* plan, hipStream_t stream);
+void fine_scratch_free(FineScratch& fs);
+
 // One substep of one or two layers (same res, drive kind and projection) in
 // a single launch.  exact selects the reference-order arithmetic.
 int fused_substep_layers(const FusedLayer* layers, int nlayers, const float* bed, int res, bool exact,
