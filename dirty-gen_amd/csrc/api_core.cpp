@@ -524,7 +524,7 @@ int step_coarse(silt_core* c, const std::vector<CorePush>& pushes, const PushLis
       if (timed) c->timed = 2;
       sur_context(core_grid(c), r, pl, c->sur_ctx, c->st);
       sur_gather_region(c->sur_ctx, r, c->net.xb, c->st);
-      mlp_forward(c->net, (size_t)r.nx * r.nz);
+      mlp_infer(c->net, (size_t)r.nx * r.nz);
       if (timed) HIPCHK(hipEventRecord(c->tev[1], c->st));
     }
   }
@@ -1689,7 +1689,7 @@ int silt_trainer_eval(silt_trainer* t, size_t first, size_t count, double* mse_n
     const SurRegion& r = t->h_regions[s];
     const size_t cells = (size_t)r.nx * r.nz;
     sur_gather_region(t->ctx + s * kSurCtxFloats, r, t->net.xb, t->st);
-    mlp_forward(t->net, cells);
+    mlp_infer(t->net, cells);
     HIPCHK(hipMemcpyAsync(y.data(), t->net.y, cells * kMlpOut * sizeof(float), hipMemcpyDeviceToHost, t->st));
     HIPCHK(hipMemcpyAsync(tg.data(), t->tgt + s * kSurTgtFloats, cells * kMlpOut * sizeof(float),
                           hipMemcpyDeviceToHost, t->st));

@@ -91,6 +91,17 @@ int silt_mlp_forward(silt_mlp* h, const float* x, size_t n, float* y) {
   return SILT_OK;
 }
 
+int silt_mlp_infer(silt_mlp* h, const float* x, size_t n, float* y) {
+  if (!h || !x || !y || n == 0) return SILT_EINVAL;
+  int rc = stage(h, x, nullptr, n);
+  if (rc) return rc;
+  silt::mlp_infer(h->m, n);
+  if (hipMemcpyAsync(y, h->m.y, n * SILT_MLP_OUT * sizeof(float), hipMemcpyDeviceToHost, h->m.st) != hipSuccess ||
+      hipStreamSynchronize(h->m.st) != hipSuccess)
+    return SILT_EDEVICE;
+  return SILT_OK;
+}
+
 int silt_mlp_gradients(silt_mlp* h, const float* x, const float* t, size_t n, double* sse, float* grad) {
   if (!h || !x || !t || n == 0) return SILT_EINVAL;
   int rc = stage(h, x, t, n);

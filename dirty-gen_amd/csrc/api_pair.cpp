@@ -47,9 +47,7 @@ struct silt_pair {
   float vort[2] = {0.08f, 0.05f};
   SheetStats* stats = nullptr;     // [2] device
   SheetTotals* totals = nullptr;   // [2] device
-  unsigned* boxmax = nullptr;      // [2] device
   SheetStats* h_stats = nullptr;   // pinned host copy
-  unsigned* h_boxmax = nullptr;
   double vol0[2] = {0, 0};
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -280,9 +278,7 @@ silt_pair* silt_pair_create(size_t res, float cell, int path) {
        hipMalloc(&p->totals, 2 * sizeof(SheetTotals)) == hipSuccess &&
        hipMalloc(&p->totals_save, 2 * sizeof(SheetTotals)) == hipSuccess &&
        hipEventCreateWithFlags(&p->ev_stats, hipEventDisableTiming) == hipSuccess &&
-       hipMalloc(&p->boxmax, 2 * sizeof(unsigned)) == hipSuccess &&
        hipHostMalloc(&p->h_stats, 2 * sizeof(SheetStats), hipHostMallocDefault) == hipSuccess &&
-       hipHostMalloc(&p->h_boxmax, 2 * sizeof(unsigned), hipHostMallocDefault) == hipSuccess &&
        hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) == hipSuccess &&
        hipEventCreate(&p->ev0) == hipSuccess && hipEventCreate(&p->ev1) == hipSuccess;
   if (ok) ok = hipMemset(p->totals, 0, 2 * sizeof(SheetTotals)) == hipSuccess;
@@ -321,9 +317,7 @@ void silt_pair_destroy(silt_pair* p) {
   (void)hipFree(p->totals);
   if (p->totals_save) (void)hipFree(p->totals_save);
   if (p->ev_stats) (void)hipEventDestroy(p->ev_stats);
-  (void)hipFree(p->boxmax);
   if (p->h_stats) (void)hipHostFree(p->h_stats);
-  if (p->h_boxmax) (void)hipHostFree(p->h_boxmax);
   if (p->ev0) (void)hipEventDestroy(p->ev0);
   if (p->ev1) (void)hipEventDestroy(p->ev1);
   if (p->stream) (void)hipStreamDestroy(p->stream);
@@ -729,7 +723,7 @@ int silt_pair_step_tires(silt_pair* p, float dt, const silt_tire_push* pushes, s
         sur_gather_region(ctx, jobs[v].r, p->net.xb + (size_t)jobs[v].row * kMlpIn, st);
       }
     }
-    mlp_forward(p->net, cells);
+    mlp_infer(p->net, cells);
   }
   // The pushes shove both layers around each vehicle (push_on, src/fluid_core.rs:8152-8239).
   const GridGeo geo{res, p->cell, p->origin[0], p->origin[1]};

@@ -338,6 +338,79 @@ void mlp_load_x(Mlp& m, const float* x, size_t n) {
   hipLaunchKernelGGL(k_to_bf16, dim3((unsigned)((c + 255) / 256)), dim3(256), 0, m.st, x, m.xb, c);
 }
 
+// Inference of the whole network for up to 16 rows per 256-lane workgroup
+// in one launch: layer 1 straight from the inputs and the weights in global
+// memory (each lane reads 8 consecutive k of its row, split over two MFMA
+// steps: the product sums K in a permuted order), the hidden activations
+// in LDS, layer 3 on the VALU.  Wave w owns hidden columns 32 w .. 32 w + 31.
+__global__ __launch_bounds__(256) void k_mlp_infer(const uint16_t* __restrict__ x, const uint16_t* __restrict__ wb,
+                                                   const float* __restrict__ master, float* __restrict__ y, int n) {
+  __shared__ __attribute__((aligned(16))) uint16_t hs[16][kMlpHidden + 8];
+  const int tid = (int)threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int r0 = (int)blockIdx.x * 16;
+  const int row = r0 + (lane & 15);
+  const bool row_in = row < n;
+  const int q = lane >> 4;
+  // Hidden layer from a bf16 operand A (row-major, lda) and weights W
+  // ([H][K]): lane holds C[4 q + i][lane & 15] of each 16 x 16 tile.
+  auto hidden = [&](auto loadA, const uint16_t* W, int K, v4f (&acc)[2]) {
+    acc[0] = acc[1] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int k0 = 0; k0 < K; k0 += 32) {
+      const int k = k0 + 8 * q;
+      const uint4 av = loadA(k);
+      v4s a0, a1;
+      memcpy(&a0, &av, 8);
+      memcpy(&a1, reinterpret_cast<const char*>(&av) + 8, 8);
+#pragma unroll
+      for (int t = 0; t < 2; t++) {
+        const int col = wave * 32 + t * 16 + (lane & 15);
+        const uint4 bv = *reinterpret_cast<const uint4*>(W + (size_t)col * K + k);
+        v4s b0, b1;
+        memcpy(&b0, &bv, 8);
+        memcpy(&b1, reinterpret_cast<const char*>(&bv) + 8, 8);
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a0, b0, acc[t], 0, 0, 0);
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a1, b1, acc[t], 0, 0, 0);
+      }
+    }
+  };
+  auto store_gelu = [&](const v4f (&acc)[2], const float* bias) {
+#pragma unroll
+    for (int t = 0; t < 2; t++)
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const int col = wave * 32 + t * 16 + (lane & 15);
+        hs[4 * q + i][col] = f2bf(gelu(acc[t][i] + bias[col]));
+      }
+  };
+  v4f acc[2];
+  hidden(
+      [&](int k) -> uint4 {
+        return row_in ? *reinterpret_cast<const uint4*>(x + (size_t)row * kMlpIn + k) : uint4{0u, 0u, 0u, 0u};
+      },
+      wb + kOffW1, kMlpIn, acc);
+  store_gelu(acc, master + kOffB1);
+  __syncthreads();
+  hidden([&](int k) -> uint4 { return *reinterpret_cast<const uint4*>(&hs[lane & 15][k]); }, wb + kOffW2, kMlpHidden,
+         acc);
+  __syncthreads();  // every wave has read layer 1's activations
+  store_gelu(acc, master + kOffB2);
+  __syncthreads();
+  // Output layer: lane t < 128 computes output (t / 8, t % 8).
+  if (tid < 16 * kMlpOut) {
+    const int r = tid / kMlpOut, o = tid % kMlpOut;
+    float s = 0.0f;
+    const uint16_t* w3 = wb + kOffW3 + (size_t)o * kMlpHidden;
+    for (int k = 0; k < kMlpHidden; k++) s += bf2f(hs[r][k]) * bf2f(w3[k]);
+    if (r0 + r < n) y[(size_t)(r0 + r) * kMlpOut + o] = s + master[kOffB3 + o];
+  }
+}
+
+void mlp_infer(Mlp& m, size_t n) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_mlp_infer, dim3((unsigned)((n + 15) / 16)), dim3(256), 0, m.st, m.xb, m.wb, m.master, m.y,
+                     (int)n);
+}
+
 void mlp_forward(Mlp& m, size_t n) {
   const int N = (int)n, H = kMlpHidden;
   Gemm g{};

@@ -63,6 +63,10 @@ void mlp_sync_bf16(Mlp& m);
 void mlp_load_x(Mlp& m, const float* x, size_t n);
 // Forward of m.xb[0..n) into m.y (and the activations kept for backward).
 void mlp_forward(Mlp& m, size_t n);
+// Inference only: the network's outputs for m.xb[0..n) into m.y in one
+// launch (K summed in another order than mlp_forward's; no activations
+// kept for backward).
+void mlp_infer(Mlp& m, size_t n);
 // Backward of the mean squared error against m.tg over n samples after
 // mlp_forward: gradients into m.grad, the sum of squared errors into m.loss.
 void mlp_backward(Mlp& m, size_t n);

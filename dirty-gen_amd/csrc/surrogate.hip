@@ -255,69 +255,26 @@ __device__ inline int sur_nb(const SurRegion& r, int q, int dir) {
   return dir == 0 ? q + 1 : dir == 1 ? q - 1 : dir == 2 ? q + r.nx : q - r.nx;
 }
 
-// Donor factor: a cell whose predicted outflow exceeds its depth gives all
-// of its outflows scaled down to what it holds.
-__global__ void k_sur_factor(const float* __restrict__ water, const float* __restrict__ slurry, int res,
-                             const float* __restrict__ y, SurRegion r, float* __restrict__ fac) {
-  const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+// Factor and apply in one workgroup per region (2 x 20 x 20 lanes at most):
+// the donor factors of both layers go to LDS, one barrier, then every cell
+// applies its faces: a donor whose predicted outflow exceeds its depth gives
+// all of its outflows scaled down to what it holds.
+__device__ inline void sur_factor_apply_wg(float* __restrict__ water, float* __restrict__ slurry, int res,
+                                           const float* __restrict__ yv, const SurRegion& r, float* fac) {
+  const int t = (int)threadIdx.x;
   const int cells = r.nx * r.nz;
-  if (t >= 2 * cells) return;
-  const int layer = t / cells, q = t % cells;
-  const size_t idx = (size_t)(r.j0 + q / r.nx) * res + (r.i0 + q % r.nx);
-  const float h = layer ? slurry[idx] : water[idx];
-  float out = 0.0f;
-  for (int dir = 0; dir < 4; dir++) out += fmaxf(sur_face(y, r, q, dir, layer), 0.0f);
-  fac[t] = out > h ? 0.999999f * h / out : 1.0f;
-}
-
-__global__ void k_sur_apply(float* __restrict__ water, float* __restrict__ slurry, int res,
-                            const float* __restrict__ y, SurRegion r, const float* __restrict__ fac) {
-  const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  const int cells = r.nx * r.nz;
-  if (t >= 2 * cells) return;
-  const int layer = t / cells, q = t % cells;
+  const bool mine = t < 2 * cells;
+  const int layer = mine ? t / cells : 0, q = mine ? t % cells : 0;
   const size_t idx = (size_t)(r.j0 + q / r.nx) * res + (r.i0 + q % r.nx);
   float* h = layer ? slurry : water;
-  float net = 0.0f;
-  for (int dir = 0; dir < 4; dir++) {
-    const float f = sur_face(y, r, q, dir, layer);
-    if (f == 0.0f) continue;
-    const int nb = sur_nb(r, q, dir);
-    net += f * (f > 0.0f ? fac[layer * cells + q] : fac[layer * cells + nb]);
+  if (mine) {
+    const float hv = h[idx];
+    float out = 0.0f;
+    for (int dir = 0; dir < 4; dir++) out += fmaxf(sur_face(yv, r, q, dir, layer), 0.0f);
+    fac[t] = out > hv ? 0.999999f * hv / out : 1.0f;
   }
-  h[idx] = h[idx] - net;
-}
-
-__global__ void k_sur_factor_many(const float* __restrict__ water, const float* __restrict__ slurry, int res,
-                                  const float* __restrict__ y, const TireJob* __restrict__ jobs,
-                                  float* __restrict__ fac) {
-  const TireJob J = jobs[blockIdx.y];
-  const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  const int cells = J.r.nx * J.r.nz;
-  if (t >= 2 * cells) return;
-  const int layer = t / cells, q = t % cells;
-  const SurRegion r = J.r;
-  const float* yv = y + (size_t)J.row * kMlpOut;
-  const size_t idx = (size_t)(r.j0 + q / r.nx) * res + (r.i0 + q % r.nx);
-  const float h = layer ? slurry[idx] : water[idx];
-  float out = 0.0f;
-  for (int dir = 0; dir < 4; dir++) out += fmaxf(sur_face(yv, r, q, dir, layer), 0.0f);
-  fac[(size_t)blockIdx.y * 2 * kSurMax * kSurMax + t] = out > h ? 0.999999f * h / out : 1.0f;
-}
-
-__global__ void k_sur_apply_many(float* __restrict__ water, float* __restrict__ slurry, int res,
-                                 const float* __restrict__ y, const TireJob* __restrict__ jobs,
-                                 const float* __restrict__ fac_all) {
-  const TireJob J = jobs[blockIdx.y];
-  const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  const int cells = J.r.nx * J.r.nz;
-  if (t >= 2 * cells) return;
-  const int layer = t / cells, q = t % cells;
-  const SurRegion r = J.r;
-  const float* yv = y + (size_t)J.row * kMlpOut;
-  const float* fac = fac_all + (size_t)blockIdx.y * 2 * kSurMax * kSurMax;
-  const size_t idx = (size_t)(r.j0 + q / r.nx) * res + (r.i0 + q % r.nx);
-  float* h = layer ? slurry : water;
+  __syncthreads();
+  if (!mine) return;
   float net = 0.0f;
   for (int dir = 0; dir < 4; dir++) {
     const float f = sur_face(yv, r, q, dir, layer);
@@ -326,6 +283,20 @@ __global__ void k_sur_apply_many(float* __restrict__ water, float* __restrict__ 
     net += f * (f > 0.0f ? fac[layer * cells + q] : fac[layer * cells + nb]);
   }
   h[idx] = h[idx] - net;
+}
+
+__global__ __launch_bounds__(1024) void k_sur_factor_apply(float* __restrict__ water, float* __restrict__ slurry,
+                                                           int res, const float* __restrict__ y, SurRegion r) {
+  __shared__ float fac[2 * kSurMax * kSurMax];
+  sur_factor_apply_wg(water, slurry, res, y, r, fac);
+}
+
+__global__ __launch_bounds__(1024) void k_sur_factor_apply_many(float* __restrict__ water, float* __restrict__ slurry,
+                                                                int res, const float* __restrict__ y,
+                                                                const TireJob* __restrict__ jobs) {
+  __shared__ float fac[2 * kSurMax * kSurMax];
+  const TireJob J = jobs[blockIdx.x];
+  sur_factor_apply_wg(water, slurry, res, y + (size_t)J.row * kMlpOut, J.r, fac);
 }
 
 __global__ void k_sur_max_abs_diff(const float* __restrict__ a, const float* __restrict__ b, int n,
@@ -419,10 +390,9 @@ void sur_gather_many(const float* ctx, const TireJob* jobs, int n, uint16_t* xb,
 
 void sur_apply_many(float* water, float* slurry, int res, const float* y, const TireJob* jobs, int n, float* fac,
                     hipStream_t st) {
+  (void)fac;  // the factors live in LDS
   if (n <= 0) return;
-  const dim3 grid((unsigned)((2 * kSurMax * kSurMax + 255) / 256), (unsigned)n);
-  hipLaunchKernelGGL(k_sur_factor_many, grid, dim3(256), 0, st, water, slurry, res, y, jobs, fac);
-  hipLaunchKernelGGL(k_sur_apply_many, grid, dim3(256), 0, st, water, slurry, res, y, jobs, fac);
+  hipLaunchKernelGGL(k_sur_factor_apply_many, dim3((unsigned)n), dim3(1024), 0, st, water, slurry, res, y, jobs);
 }
 
 void sur_gather_region(const float* ctx, const SurRegion& r, uint16_t* xb, hipStream_t st) {
@@ -444,11 +414,9 @@ void sur_targets(const float* hn_w, const float* hc_w, const float* hn_s, const 
 
 void sur_apply(float* water, float* slurry, int res, const float* y, const SurRegion& r, float* fac,
                hipStream_t st) {
-  const int n = 2 * r.nx * r.nz;
-  hipLaunchKernelGGL(k_sur_factor, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, water, slurry, res, y, r,
-                     fac);
-  hipLaunchKernelGGL(k_sur_apply, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, water, slurry, res, y, r,
-                     fac);
+  (void)fac;  // the factors live in LDS
+  if (r.nx <= 0 || r.nz <= 0) return;
+  hipLaunchKernelGGL(k_sur_factor_apply, dim3(1), dim3(1024), 0, st, water, slurry, res, y, r);
 }
 
 void sur_max_abs_diff(const float* a, const float* b, int n, unsigned* out, hipStream_t st) {

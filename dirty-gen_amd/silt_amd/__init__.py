@@ -141,6 +141,7 @@ def _declare(L):
     L.silt_mlp_get_params.argtypes = [_P, _P]
     L.silt_mlp_set_params.argtypes = [_P, _P]
     L.silt_mlp_forward.argtypes = [_P, _P, _Z, _P]
+    L.silt_mlp_infer.argtypes = [_P, _P, _Z, _P]
     L.silt_mlp_gradients.argtypes = [_P, _P, _P, _Z, C.POINTER(C.c_double), _P]
     L.silt_mlp_train_batch.argtypes = [_P, _P, _P, _Z, _F, C.POINTER(C.c_double)]
 
@@ -823,6 +824,12 @@ class Mlp:
         x = self._x(x)
         y = np.empty((x.shape[0], MLP_OUT), np.float32)
         _check_status(lib().silt_mlp_forward(self._h, _ptr(x), x.shape[0], _ptr(y)))
+        return y
+
+    def infer(self, x):
+        x = self._x(x)
+        y = np.empty((x.shape[0], MLP_OUT), np.float32)
+        _check_status(lib().silt_mlp_infer(self._h, _ptr(x), x.shape[0], _ptr(y)))
         return y
 
     def gradients(self, x, t):

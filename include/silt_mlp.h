@@ -46,6 +46,10 @@ int silt_mlp_set_params(silt_mlp *m, const float *params);
 
 /* y[n][8] = net(x[n][1536]) (host arrays). */
 int silt_mlp_forward(silt_mlp *m, const float *x, size_t n, float *y);
+/* The same outputs from the one-launch inference kernel the tire steps use
+ * (K summed in another order, so equal to silt_mlp_forward up to FP32
+ * rounding).  No counterpart in the reference. */
+int silt_mlp_infer(silt_mlp *m, const float *x, size_t n, float *y);
 /* Gradient of the loss against targets t[n][8] into grad (SILT_MLP_PARAMS
  * values); *sse = sum of squared errors.  Weights unchanged. */
 int silt_mlp_gradients(silt_mlp *m, const float *x, const float *t, size_t n, double *sse, float *grad);

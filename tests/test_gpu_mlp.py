@@ -78,3 +78,20 @@ def test_training_fits_a_smooth_target():
         assert last < first / 20.0, (first, last)
     finally:
         m.close()
+
+
+def test_one_launch_inference_matches_the_training_forward():
+    """The tire steps' one-launch inference (all three layers, K summed in a
+    permuted order) against the training forward and the oracle."""
+    rng = np.random.default_rng(31)
+    m = S.Mlp(17)
+    try:
+        for n in (1, 15, 16, 17, 400):
+            x = rng.standard_normal((n, S.MLP_IN)).astype(np.float32)
+            yf, yi = m.forward(x), m.infer(x)
+            scale = max(float(np.abs(yf).max()), 1e-3)
+            assert np.abs(yi - yf).max() <= 2e-2 * scale, n
+        yo = O.mlp_forward(m.params(), x, bf16=True, precision=32)
+        assert np.abs(yi - yo).max() <= 3e-2 * max(float(np.abs(yo).max()), 1e-3)
+    finally:
+        m.close()
