@@ -338,68 +338,100 @@ void mlp_load_x(Mlp& m, const float* x, size_t n) {
   hipLaunchKernelGGL(k_to_bf16, dim3((unsigned)((c + 255) / 256)), dim3(256), 0, m.st, x, m.xb, c);
 }
 
-// Inference of the whole network for up to 16 rows per 256-lane workgroup
-// in one launch: layer 1 straight from the inputs and the weights in global
-// memory (each lane reads 8 consecutive k of its row, split over two MFMA
-// steps: the product sums K in a permuted order), the hidden activations
-// in LDS, layer 3 on the VALU.  Wave w owns hidden columns 32 w .. 32 w + 31.
-__global__ __launch_bounds__(256) void k_mlp_infer(const uint16_t* __restrict__ x, const uint16_t* __restrict__ wb,
-                                                   const float* __restrict__ master, float* __restrict__ y, int n) {
+// Inference of the whole network for up to 16 rows per workgroup of 16
+// wavefronts, in one launch.  Layer 1 (K = 1536) is split over the waves:
+// wave w owns hidden columns 32 (w mod 4) .. +31 over the K quarter w / 4,
+// reading the inputs and the weights straight from global memory (each lane
+// reads 8 consecutive k of its row, split over two MFMA steps, so the
+// product sums K in a permuted order); the quarters' partial sums meet in
+// LDS and add in order.  The hidden activations stay in LDS; layer 2 runs
+// on the first four waves, layer 3 on the VALU.
+__global__ __launch_bounds__(1024) void k_mlp_infer(const uint16_t* __restrict__ x, const uint16_t* __restrict__ wb,
+                                                    const float* __restrict__ master, float* __restrict__ y, int n) {
   __shared__ __attribute__((aligned(16))) uint16_t hs[16][kMlpHidden + 8];
+  __shared__ float part[4][16][kMlpHidden];
+  __shared__ uint16_t w3s[kMlpOut * kMlpHidden];
   const int tid = (int)threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  // The output layer's weights (2 KB) into LDS while layer 1 runs.
+  for (int e = tid; e < kMlpOut * kMlpHidden; e += 1024) w3s[e] = wb[kOffW3 + e];
+  const int cg = wave & 3, kq = wave >> 2;
   const int r0 = (int)blockIdx.x * 16;
   const int row = r0 + (lane & 15);
   const bool row_in = row < n;
   const int q = lane >> 4;
-  // Hidden layer from a bf16 operand A (row-major, lda) and weights W
-  // ([H][K]): lane holds C[4 q + i][lane & 15] of each 16 x 16 tile.
-  auto hidden = [&](auto loadA, const uint16_t* W, int K, v4f (&acc)[2]) {
+  // Product of 16 rows of a bf16 operand A with weights W ([H][K], K
+  // contiguous) over k in [kb, ke), for hidden columns 32 cg .. +31: lane
+  // holds C[4 q + i][lane & 15] of each 16 x 16 tile.  K in groups of 128,
+  // the loads of a group issued before its products.
+  auto hidden = [&](auto loadA, const uint16_t* W, int K, int kb, int ke, v4f (&acc)[2]) {
     acc[0] = acc[1] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
-    for (int k0 = 0; k0 < K; k0 += 32) {
-      const int k = k0 + 8 * q;
-      const uint4 av = loadA(k);
-      v4s a0, a1;
-      memcpy(&a0, &av, 8);
-      memcpy(&a1, reinterpret_cast<const char*>(&av) + 8, 8);
+    const uint16_t* w0 = W + (size_t)(cg * 32 + (lane & 15)) * K;
+    const uint16_t* w1 = w0 + (size_t)16 * K;
+    for (int k0 = kb; k0 < ke; k0 += 128) {
+      uint4 av[4], bv[4][2];
 #pragma unroll
-      for (int t = 0; t < 2; t++) {
-        const int col = wave * 32 + t * 16 + (lane & 15);
-        const uint4 bv = *reinterpret_cast<const uint4*>(W + (size_t)col * K + k);
-        v4s b0, b1;
-        memcpy(&b0, &bv, 8);
-        memcpy(&b1, reinterpret_cast<const char*>(&bv) + 8, 8);
-        acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a0, b0, acc[t], 0, 0, 0);
-        acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a1, b1, acc[t], 0, 0, 0);
+      for (int u = 0; u < 4; u++) {
+        const int k = k0 + 32 * u + 8 * q;
+        av[u] = loadA(k);
+        bv[u][0] = *reinterpret_cast<const uint4*>(w0 + k);
+        bv[u][1] = *reinterpret_cast<const uint4*>(w1 + k);
+      }
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        v4s a0, a1;
+        memcpy(&a0, &av[u], 8);
+        memcpy(&a1, reinterpret_cast<const char*>(&av[u]) + 8, 8);
+#pragma unroll
+        for (int t = 0; t < 2; t++) {
+          v4s b0, b1;
+          memcpy(&b0, &bv[u][t], 8);
+          memcpy(&b1, reinterpret_cast<const char*>(&bv[u][t]) + 8, 8);
+          acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a0, b0, acc[t], 0, 0, 0);
+          acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a1, b1, acc[t], 0, 0, 0);
+        }
       }
     }
   };
-  auto store_gelu = [&](const v4f (&acc)[2], const float* bias) {
-#pragma unroll
-    for (int t = 0; t < 2; t++)
-#pragma unroll
-      for (int i = 0; i < 4; i++) {
-        const int col = wave * 32 + t * 16 + (lane & 15);
-        hs[4 * q + i][col] = f2bf(gelu(acc[t][i] + bias[col]));
-      }
-  };
   v4f acc[2];
+  constexpr int kQuarter = kMlpIn / 4;  // 384, a multiple of 128
   hidden(
       [&](int k) -> uint4 {
         return row_in ? *reinterpret_cast<const uint4*>(x + (size_t)row * kMlpIn + k) : uint4{0u, 0u, 0u, 0u};
       },
-      wb + kOffW1, kMlpIn, acc);
-  store_gelu(acc, master + kOffB1);
+      wb + kOffW1, kMlpIn, kq * kQuarter, (kq + 1) * kQuarter, acc);
+#pragma unroll
+  for (int t = 0; t < 2; t++)
+#pragma unroll
+    for (int i = 0; i < 4; i++) part[kq][4 * q + i][cg * 32 + t * 16 + (lane & 15)] = acc[t][i];
   __syncthreads();
-  hidden([&](int k) -> uint4 { return *reinterpret_cast<const uint4*>(&hs[lane & 15][k]); }, wb + kOffW2, kMlpHidden,
-         acc);
+  // Quarters in order, bias, GELU: one hidden value per lane pair.
+  for (int e = tid; e < 16 * kMlpHidden; e += 1024) {
+    const int r = e / kMlpHidden, c = e % kMlpHidden;
+    const float z = ((part[0][r][c] + part[1][r][c]) + part[2][r][c]) + part[3][r][c];
+    hs[r][c] = f2bf(gelu(z + master[kOffB1 + c]));
+  }
+  __syncthreads();
+  if (kq == 0) {
+    hidden([&](int k) -> uint4 { return *reinterpret_cast<const uint4*>(&hs[lane & 15][k]); }, wb + kOffW2, kMlpHidden,
+           0, kMlpHidden, acc);
+  }
   __syncthreads();  // every wave has read layer 1's activations
-  store_gelu(acc, master + kOffB2);
+  if (kq == 0) {
+#pragma unroll
+    for (int t = 0; t < 2; t++)
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const int col = cg * 32 + t * 16 + (lane & 15);
+        hs[4 * q + i][col] = f2bf(gelu(acc[t][i] + master[kOffB2 + col]));
+      }
+  }
   __syncthreads();
   // Output layer: lane t < 128 computes output (t / 8, t % 8).
   if (tid < 16 * kMlpOut) {
     const int r = tid / kMlpOut, o = tid % kMlpOut;
     float s = 0.0f;
-    const uint16_t* w3 = wb + kOffW3 + (size_t)o * kMlpHidden;
+    const uint16_t* w3 = w3s + o * kMlpHidden;
+#pragma unroll 16
     for (int k = 0; k < kMlpHidden; k++) s += bf2f(hs[r][k]) * bf2f(w3[k]);
     if (r0 + r < n) y[(size_t)(r0 + r) * kMlpOut + o] = s + master[kOffB3 + o];
   }
@@ -407,7 +439,7 @@ __global__ __launch_bounds__(256) void k_mlp_infer(const uint16_t* __restrict__ 
 
 void mlp_infer(Mlp& m, size_t n) {
   if (n == 0) return;
-  hipLaunchKernelGGL(k_mlp_infer, dim3((unsigned)((n + 15) / 16)), dim3(256), 0, m.st, m.xb, m.wb, m.master, m.y,
+  hipLaunchKernelGGL(k_mlp_infer, dim3((unsigned)((n + 15) / 16)), dim3(1024), 0, m.st, m.xb, m.wb, m.master, m.y,
                      (int)n);
 }
 

@@ -15,13 +15,44 @@ constexpr float kContactLength = 0.30f;
 constexpr float kWheelLoad = 3500.0f;
 
 // Context cell a of region r for pushes p[0..n) with mean velocity (vtx, vtz).
+// Context cell a of region r for pushes p[0..n) with mean velocity (vtx,
+// vtz), computed by the 8 lanes sub = 0..7 of an aligned group: each lane
+// tests one row of the 8 x 8 coverage samples of every push, the integer
+// hit counts are summed over the group, and lane 0 forms the pressure in
+// push order and stores the cell.
 __device__ inline void sur_context_cell(const SurGrid& c, const SurRegion& r, const CorePush* pushes, int n, float vtx,
-                                        float vtz, float* __restrict__ ctx, int a) {
+                                        float vtz, float* __restrict__ ctx, int a, int sub) {
+  const bool cell_in = a < kSurCtx * kSurCtx;
   const int ca = a % kSurCtx, cb = a / kSurCtx;
   const int i = r.i0 - kSurReach + ca, j = r.j0 - kSurReach + cb;
   const size_t plane = (size_t)kSurCtx * kSurCtx;
+  const bool on_grid = cell_in && i >= 0 && j >= 0 && i < c.res && j < c.res;
+  float p = 0.0f;
+  const float x0 = c.ox + (float)i * c.cell, z0 = c.oz + (float)j * c.cell;
+  for (int t = 0; t < n; t++) {
+    const CorePush q = pushes[t];
+    const float fl = sqrtf(q.fx * q.fx + q.fz * q.fz);
+    if (!(fl > 0.0f)) continue;
+    const float fx = q.fx / fl, fz = q.fz / fl;
+    // Coverage of the cell by the contact rectangle on 8 x 8 sample points
+    // (2.5 cm apart at 0.2 m cells, about the 2 cm patch's spacing).
+    int hits = 0;
+    if (on_grid)
+      for (int s = sub * 8; s < sub * 8 + 8; s++) {
+        const float xs = x0 + ((float)(s & 7) + 0.5f) * (c.cell * 0.125f);
+        const float zs = z0 + ((float)(s >> 3) + 0.5f) * (c.cell * 0.125f);
+        const float dx = xs - q.x, dz = zs - q.z;
+        const float along = dx * fx + dz * fz, across = dz * fx - dx * fz;
+        hits += (fabsf(along) <= 0.5f * kContactLength && fabsf(across) <= 0.5f * q.width) ? 1 : 0;
+      }
+    hits += __shfl_xor(hits, 1, 8);
+    hits += __shfl_xor(hits, 2, 8);
+    hits += __shfl_xor(hits, 4, 8);
+    if (hits) p += (kWheelLoad / (kContactLength * fmaxf(q.width, 0.05f))) * 1.0e-5f * ((float)hits / 64.0f);
+  }
+  if (!cell_in || sub != 0) return;
   float v[kSurChannels] = {0, 0, 0, 0, 0, 0, 0};
-  if (i >= 0 && j >= 0 && i < c.res && j < c.res) {
+  if (on_grid) {
     const size_t idx = (size_t)j * c.res + i;
     const float hw = c.water[idx];
     const bool wet = hw > 1.0e-6f;
@@ -30,45 +61,28 @@ __device__ inline void sur_context_cell(const SurGrid& c, const SurRegion& r, co
     v[2] = c.slurry[idx] * 10.0f;
     v[3] = wet ? (c.uw[idx] - vtx) * 0.125f : 0.0f;
     v[4] = wet ? (c.vw[idx] - vtz) * 0.125f : 0.0f;
-    float p = 0.0f;
-    const float x0 = c.ox + (float)i * c.cell, z0 = c.oz + (float)j * c.cell;
-    for (int t = 0; t < n; t++) {
-      const CorePush q = pushes[t];
-      const float fl = sqrtf(q.fx * q.fx + q.fz * q.fz);
-      if (!(fl > 0.0f)) continue;
-      const float fx = q.fx / fl, fz = q.fz / fl;
-      // Coverage of the cell by the contact rectangle on 8 x 8 sample points
-      // (2.5 cm apart at 0.2 m cells, about the 2 cm patch's spacing).
-      int hits = 0;
-      for (int s = 0; s < 64; s++) {
-        const float xs = x0 + ((float)(s & 7) + 0.5f) * (c.cell * 0.125f);
-        const float zs = z0 + ((float)(s >> 3) + 0.5f) * (c.cell * 0.125f);
-        const float dx = xs - q.x, dz = zs - q.z;
-        const float along = dx * fx + dz * fz, across = dz * fx - dx * fz;
-        hits += (fabsf(along) <= 0.5f * kContactLength && fabsf(across) <= 0.5f * q.width) ? 1 : 0;
-      }
-      if (hits) p += (kWheelLoad / (kContactLength * fmaxf(q.width, 0.05f))) * 1.0e-5f * ((float)hits / 64.0f);
-    }
     v[5] = p;
     v[6] = 1.0f;
   }
   for (int ch = 0; ch < kSurChannels; ch++) ctx[(size_t)ch * plane + a] = v[ch];
 }
 
-__global__ void k_sur_context(SurGrid c, SurRegion r, PushList pl, float vtx, float vtz, float* __restrict__ ctx) {
-  const int a = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  if (a >= kSurCtx * kSurCtx) return;
-  sur_context_cell(c, r, pl.p, pl.n, vtx, vtz, ctx, a);
+// Eight lanes per context cell.
+__global__ __launch_bounds__(256) void k_sur_context(SurGrid c, SurRegion r, PushList pl, float vtx, float vtz,
+                                                     float* __restrict__ ctx) {
+  const int g = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  sur_context_cell(c, r, pl.p, pl.n, vtx, vtz, ctx, g >> 3, g & 7);
 }
 
 // One vehicle per blockIdx.y.
-__global__ void k_sur_context_many(SurGrid c, const TireJob* __restrict__ jobs, const CorePush* __restrict__ pushes,
-                                   float* __restrict__ ctx) {
-  const int a = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+__global__ __launch_bounds__(256) void k_sur_context_many(SurGrid c, const TireJob* __restrict__ jobs,
+                                                          const CorePush* __restrict__ pushes,
+                                                          float* __restrict__ ctx) {
+  const int g = (int)(blockIdx.x * blockDim.x + threadIdx.x);
   const TireJob J = jobs[blockIdx.y];
-  if (a >= kSurCtx * kSurCtx || J.r.nx == 0) return;
+  if (J.r.nx == 0) return;
   sur_context_cell(c, J.r, pushes + J.push.p0, J.push.np, J.vtx, J.vtz, ctx + (size_t)blockIdx.y * kSurCtxFloats,
-                   a);
+                   g >> 3, g & 7);
 }
 
 // The square's symmetries as x' = M x, M = {m00, m01, m10, m11}.
@@ -352,7 +366,7 @@ SurRegion sur_region_at(const CorePush* p, int n, float ox, float oz, float cell
 void sur_context(const SurGrid& c, const SurRegion& r, const PushList& pl, float* ctx, hipStream_t st) {
   float vx, vz;
   sur_mean_velocity(pl.p, pl.n, &vx, &vz);
-  const int n = kSurCtx * kSurCtx;
+  const int n = kSurCtx * kSurCtx * 8;
   hipLaunchKernelGGL(k_sur_context, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, c, r, pl, vx, vz, ctx);
 }
 
@@ -376,8 +390,8 @@ void sur_mean_velocity(const CorePush* p, int n, float* vx, float* vz) {
 void sur_context_many(const SurGrid& g, const TireJob* jobs, const CorePush* pushes, int n, float* ctx,
                       hipStream_t st) {
   if (n <= 0) return;
-  const int cells = kSurCtx * kSurCtx;
-  hipLaunchKernelGGL(k_sur_context_many, dim3((unsigned)((cells + 255) / 256), (unsigned)n), dim3(256), 0, st, g, jobs,
+  const int lanes = kSurCtx * kSurCtx * 8;
+  hipLaunchKernelGGL(k_sur_context_many, dim3((unsigned)((lanes + 255) / 256), (unsigned)n), dim3(256), 0, st, g, jobs,
                      pushes, ctx);
 }
 
