@@ -105,6 +105,8 @@ struct silt_core {
   // batched wet-set labelling, the substep maxima.
   SheetStats* d_stats = nullptr;
   SheetStats* h_stats = nullptr;  // pinned
+  // The ledger's statistics of the four planes (coarse and fine layers).
+  SheetStats* d_lstats = nullptr;
   // Scratch of the publish calls: 10 output planes and 3 soil inputs.
   float* d_pub = nullptr;
   PlanLabeller lab;
@@ -672,7 +674,7 @@ silt_core* silt_core_create(float center_x, float center_z, int path) {
        dalloc(c, &c->d_cols, 1024) && dalloc(c, &c->d_ev, kSprayCap) && dalloc(c, &c->d_nev, 1) &&
        dalloc(c, &c->d_acc, 4) && dalloc(c, &c->soil_bed, WN) && dalloc(c, &c->soil_placed, WN) &&
        dalloc(c, &c->soil_init, WN) && dalloc(c, &c->soil_rut, WN) && dalloc(c, &c->d_contacts, kMaxPush) &&
-       dalloc(c, &c->d_stats, 2) && dalloc(c, &c->d_pub, 13 * WN) &&
+       dalloc(c, &c->d_stats, 2) && dalloc(c, &c->d_lstats, 4) && dalloc(c, &c->d_pub, 13 * WN) &&
        hipHostMalloc((void**)&c->h_stats, 2 * sizeof(SheetStats), hipHostMallocDefault) == hipSuccess;
   if (ok) {
     // Grain::Other everywhere (FluidCore::new, src/fluid_core.rs:4979-4980).
@@ -925,10 +927,7 @@ int silt_core_step(silt_core* c, float dt, silt_spray_event* events, size_t cap,
 
 int silt_core_ledger(silt_core* c, silt_fluid_ledger* out) {
   if (!c || !out) return SILT_EINVAL;
-  Workspace* ws = workspace_for(kWindowRes);
-  if (!ws) return SILT_ENOMEM;
-  SheetStats* s = nullptr;
-  if (hipMalloc(&s, 4 * sizeof(SheetStats)) != hipSuccess) return SILT_ENOMEM;
+  SheetStats* s = c->d_lstats;
   stats_compute(c->c.water, c->c.uw, c->c.vw, kWindowRes, 1.0e-6f, s, c->st);
   stats_compute(c->c.slurry, c->c.us, c->c.vs, kWindowRes, 1.0e-6f, s + 1, c->st);
   if (c->fine_on) {
@@ -940,7 +939,6 @@ int silt_core_ledger(silt_core* c, silt_fluid_ledger* out) {
   HIPCHK(hipMemcpyAsync(h, s, sizeof h, hipMemcpyDeviceToHost, c->st));
   HIPCHK(hipMemcpyAsync(acc, c->d_acc, sizeof acc, hipMemcpyDeviceToHost, c->st));
   HIPCHK(hipStreamSynchronize(c->st));
-  (void)hipFree(s);
   double ca = (double)kCoarseCell * kCoarseCell, fa = (double)kFineCell * kFineCell;
   out->water_m3 = h[0].vol * ca + (c->fine_on ? h[2].vol * fa : 0.0);
   out->slurry_m3 = h[1].vol * ca + (c->fine_on ? h[3].vol * fa : 0.0);

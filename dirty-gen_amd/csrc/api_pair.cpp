@@ -59,7 +59,11 @@ struct silt_pair {
   float spec_dt = 0.0f;
   StepPlan spec_plans[2];
   SheetTotals* totals_save = nullptr;
+  SheetStats* stats_save = nullptr;  // [2] device: stats before a speculative launch
   hipEvent_t ev_stats = nullptr;
+  // The statistics no longer describe the planes (the network's fluxes
+  // moved depth after the step wrote them): recomputed before they are read.
+  bool stats_stale = false;
   long long spec_hits = 0, spec_misses = 0;
   int last_nsub[2] = {0, 0};
   int last_fallback = 0;
@@ -159,6 +163,7 @@ int pair_launch(silt_pair* p, float dt, const StepPlan* plans) {
   // together in one launch; anything else runs per layer.
   auto fused_ok = [&](int l) {
     if (p->path == SILT_PATH_EXACT || plans[l].boxes.empty() || plans[l].pace3) return false;
+    if (!fused_boxes_fit(plans[l].boxes)) return false;
     for (const Box& b : plans[l].boxes)
       if (!reference_uses_jacobi(b, p->cell, 0, true)) return false;
     return true;
@@ -199,7 +204,7 @@ int pair_launch(silt_pair* p, float dt, const StepPlan* plans) {
   if (!staged && nfused) HIPCHK(hipEventRecord(p->ev0, st));
   for (int k = 0; k < most; k++) {
     FusedLayer launch[2];
-    int m = 0;
+    int flip[2], m = 0;
     for (int q = 0; q < nfused; q++) {
       const int l = fused[q];
       if (plans[l].nsub <= k) continue;
@@ -217,10 +222,12 @@ int pair_launch(silt_pair* p, float dt, const StepPlan* plans) {
       f.stats = p->stats + l;
       f.totals = p->totals + l;
       stats_reset(f.stats, 1, st);
-      p->cur[l] = nb;
+      flip[m - 1] = l;
     }
     int rc = fused_substep_layers(launch, m, p->bed, res, p->path == SILT_PATH_FUSED_EXACT, st);
     if (rc != 0) return rc;
+    // The output planes become current only once their launch is enqueued.
+    for (int q = 0; q < m; q++) p->cur[flip[q]] = 1 - p->cur[flip[q]];
   }
   HIPCHK(hipEventRecord(p->ev1, st));
   p->kernel_ms_pending = true;
@@ -238,7 +245,7 @@ bool pair_speculable(const silt_pair* p, const StepPlan* plans) {
   if (p->path == SILT_PATH_EXACT) return false;
   for (int l = 0; l < 2; l++) {
     if (plans[l].boxes.empty()) continue;
-    if (plans[l].nsub != 1 || plans[l].pace3) return false;
+    if (plans[l].nsub != 1 || plans[l].pace3 || !fused_boxes_fit(plans[l].boxes)) return false;
     for (const Box& b : plans[l].boxes)
       if (!reference_uses_jacobi(b, p->cell, 0, true)) return false;
   }
@@ -277,6 +284,7 @@ silt_pair* silt_pair_create(size_t res, float cell, int path) {
   ok = ok && hipMalloc(&p->stats, 2 * sizeof(SheetStats)) == hipSuccess &&
        hipMalloc(&p->totals, 2 * sizeof(SheetTotals)) == hipSuccess &&
        hipMalloc(&p->totals_save, 2 * sizeof(SheetTotals)) == hipSuccess &&
+       hipMalloc(&p->stats_save, 2 * sizeof(SheetStats)) == hipSuccess &&
        hipEventCreateWithFlags(&p->ev_stats, hipEventDisableTiming) == hipSuccess &&
        hipHostMalloc(&p->h_stats, 2 * sizeof(SheetStats), hipHostMallocDefault) == hipSuccess &&
        hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) == hipSuccess &&
@@ -316,6 +324,7 @@ void silt_pair_destroy(silt_pair* p) {
   (void)hipFree(p->stats);
   (void)hipFree(p->totals);
   if (p->totals_save) (void)hipFree(p->totals_save);
+  if (p->stats_save) (void)hipFree(p->stats_save);
   if (p->ev_stats) (void)hipEventDestroy(p->ev_stats);
   if (p->h_stats) (void)hipHostFree(p->h_stats);
   if (p->ev0) (void)hipEventDestroy(p->ev0);
@@ -347,6 +356,8 @@ int silt_pair_upload(silt_pair* p, const float* bed, const float* water, const f
   HIPCHK(hipMemcpyAsync(p->h_stats, p->stats, 2 * sizeof(SheetStats), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
   for (int l = 0; l < 2; l++) p->vol0[l] = p->h_stats[l].vol;
+  p->stats_stale = false;
+  p->spec_valid = false;
   return SILT_OK;
 }
 
@@ -361,10 +372,26 @@ int silt_pair_download(silt_pair* p, float* water, float* uw, float* vw, float* 
   return SILT_OK;
 }
 
+namespace {
+
+// Statistics of the current planes again, when something other than a step
+// kernel changed them.
+void refresh_stats(silt_pair* p) {
+  if (!p->stats_stale) return;
+  for (int l = 0; l < 2; l++) {
+    float** pl = p->planes[l][p->cur[l]];
+    stats_compute(pl[0], pl[1], pl[2], p->res, kEps, p->stats + l, p->stream);
+  }
+  p->stats_stale = false;
+}
+
+}  // namespace
+
 int silt_pair_step(silt_pair* p, float dt) {
   if (!p) return SILT_EINVAL;
   if (!(dt > 0.0f)) return 0;
   hipStream_t st = p->stream;
+  refresh_stats(p);
   // Speculation: the last step's plan is launched at once, while the
   // statistics of the current planes come back and the true plan is made;
   // a plan that differs undoes the launch (the current planes are untouched,
@@ -376,6 +403,7 @@ int silt_pair_step(silt_pair* p, float dt) {
     HIPCHK(hipMemcpyAsync(p->h_stats, p->stats, 2 * sizeof(SheetStats), hipMemcpyDeviceToHost, st));
     HIPCHK(hipEventRecord(p->ev_stats, st));
     HIPCHK(hipMemcpyAsync(p->totals_save, p->totals, 2 * sizeof(SheetTotals), hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(p->stats_save, p->stats, 2 * sizeof(SheetStats), hipMemcpyDeviceToDevice, st));
     int rc = pair_launch(p, dt, p->spec_plans);
     if (rc < 0) return rc;
     HIPCHK(hipEventSynchronize(p->ev_stats));
@@ -397,10 +425,12 @@ int silt_pair_step(silt_pair* p, float dt) {
       p->spec_hits++;
       return std::max(p->last_nsub[0], p->last_nsub[1]);
     }
-    // Undo: back to the planes the step started from, ledger lines as before.
+    // Undo: back to the planes the step started from, ledger lines and
+    // statistics as before (a layer the true plan leaves dry keeps them).
     p->cur[0] = cur0[0];
     p->cur[1] = cur0[1];
     HIPCHK(hipMemcpyAsync(p->totals, p->totals_save, 2 * sizeof(SheetTotals), hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(p->stats, p->stats_save, 2 * sizeof(SheetStats), hipMemcpyDeviceToDevice, st));
     p->spec_misses++;
   }
   rc = pair_launch(p, dt, plans);
@@ -416,6 +446,7 @@ int silt_pair_step(silt_pair* p, float dt) {
 
 int silt_pair_ledger(silt_pair* p, silt_pair_ledger_t* out) {
   if (!p || !out) return SILT_EINVAL;
+  refresh_stats(p);
   SheetTotals t[2];
   HIPCHK(hipMemcpyAsync(p->h_stats, p->stats, 2 * sizeof(SheetStats), hipMemcpyDeviceToHost, p->stream));
   HIPCHK(hipMemcpyAsync(t, p->totals, sizeof t, hipMemcpyDeviceToHost, p->stream));
@@ -476,11 +507,18 @@ int silt_pair_autotune(silt_pair* p, float dt, const int* rows, size_t n, int re
   if (hipMalloc(&save, 6 * fb) != hipSuccess) return SILT_ENOMEM;
   hipStream_t st = p->stream;
   const int cur0[2] = {p->cur[0], p->cur[1]};
+  refresh_stats(p);
   SheetStats stats0[2];
   SheetTotals totals0[2];
+  // The tuning steps are not the caller's: their speculation counts and
+  // plans are dropped with their planes.
+  const long long hits0 = p->spec_hits, misses0 = p->spec_misses;
   auto restore = [&]() -> int {
     p->cur[0] = cur0[0];
     p->cur[1] = cur0[1];
+    p->spec_valid = false;
+    p->spec_hits = hits0;
+    p->spec_misses = misses0;
     for (int l = 0; l < 2; l++)
       for (int k = 0; k < 3; k++)
         HIPCHK(hipMemcpyAsync(p->planes[l][cur0[l]][k], save + (size_t)(3 * l + k) * p->n, fb,
@@ -742,6 +780,7 @@ int silt_pair_step_tires(silt_pair* p, float dt, const silt_tire_push* pushes, s
   if (groups) {
     stats_compute(wl[0], wl[1], wl[2], res, kEps, p->stats, st);
     stats_compute(sl[0], sl[1], sl[2], res, kEps, p->stats + 1, st);
+    p->stats_stale = false;
   }
   int nsub = silt_pair_step(p, dt);
   if (nsub < 0) return nsub;
@@ -750,14 +789,15 @@ int silt_pair_step_tires(silt_pair* p, float dt, const silt_tire_push* pushes, s
     float* wh = p->planes[0][p->cur[0]][0];
     float* sh = p->planes[1][p->cur[1]][0];
     if (batched) {
-      sur_apply_many(wh, sh, res, p->net.y, d_jobs, (int)groups, p->sur_fac, st);
+      sur_apply_many(wh, sh, res, p->net.y, d_jobs, (int)groups, p->sur_fac, st, p->totals);
     } else {
       for (size_t v = 0; v < groups; v++) {
         if (!jobs[v].r.nx) continue;
         sur_apply(wh, sh, res, p->net.y + (size_t)jobs[v].row * kMlpOut, jobs[v].r,
-                  p->sur_fac + v * 2 * kSurMax * kSurMax, st);
+                  p->sur_fac + v * 2 * kSurMax * kSurMax, st, p->totals);
       }
     }
+    p->stats_stale = true;
   }
   if (timed) {
     HIPCHK(hipEventRecord(p->tev[3], st));

@@ -961,6 +961,8 @@ static int dispatch_any(const FusedArgs* fa, int tiles, bool exact, bool full, b
 
 size_t fused_desc_size() { return sizeof(LayerArgs); }
 
+bool fused_boxes_fit(const std::vector<Box>& boxes) { return boxes_fit(boxes); }
+
 int fused_build(const FusedLayer* wins, int n, bool exact, int rows, FusedLaunch* out) {
   if (n < 1 || !out) return SILT_EINVAL;
   bool project = wins[0].a.project != 0, wind = wins[0].a.wind != 0;

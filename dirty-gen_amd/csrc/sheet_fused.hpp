@@ -143,6 +143,10 @@ struct FusedLaunch {
   std::vector<int> tile_window;
 };
 size_t fused_desc_size();
+// Whether the fused kernel covers a box list: 1 to 8 boxes that neither
+// overlap nor touch (so two cells in boxes are in one box exactly when a
+// face between them exists).
+bool fused_boxes_fit(const std::vector<Box>& boxes);
 int fused_build(const FusedLayer* wins, int n, bool exact, int rows, FusedLaunch* out);
 int fused_enqueue(const FusedLaunch& l, const void* dev_table, const int* dev_tile_window, hipStream_t st);
 

@@ -4,6 +4,7 @@
 #include <math.h>
 
 #include "mlp.hpp"
+#include "sheet_fused.hpp"
 
 namespace silt {
 
@@ -273,8 +274,12 @@ __device__ inline int sur_nb(const SurRegion& r, int q, int dir) {
 // the donor factors of both layers go to LDS, one barrier, then every cell
 // applies its faces: a donor whose predicted outflow exceeds its depth gives
 // all of its outflows scaled down to what it holds.
+// With totals, each cell's residue ((new - old) + what its faces carried
+// out, in FP64; the faces' FP32 fluxes cancel in pairs) is summed per wave
+// and layer and added to that layer's residue line.
 __device__ inline void sur_factor_apply_wg(float* __restrict__ water, float* __restrict__ slurry, int res,
-                                           const float* __restrict__ yv, const SurRegion& r, float* fac) {
+                                           const float* __restrict__ yv, const SurRegion& r, float* fac,
+                                           SheetTotals* totals) {
   const int t = (int)threadIdx.x;
   const int cells = r.nx * r.nz;
   const bool mine = t < 2 * cells;
@@ -288,29 +293,50 @@ __device__ inline void sur_factor_apply_wg(float* __restrict__ water, float* __r
     fac[t] = out > hv ? 0.999999f * hv / out : 1.0f;
   }
   __syncthreads();
-  if (!mine) return;
-  float net = 0.0f;
-  for (int dir = 0; dir < 4; dir++) {
-    const float f = sur_face(yv, r, q, dir, layer);
-    if (f == 0.0f) continue;
-    const int nb = sur_nb(r, q, dir);
-    net += f * (f > 0.0f ? fac[layer * cells + q] : fac[layer * cells + nb]);
+  double res_w = 0.0, res_s = 0.0;
+  if (mine) {
+    float net = 0.0f;
+    double out = 0.0;
+    for (int dir = 0; dir < 4; dir++) {
+      const float f = sur_face(yv, r, q, dir, layer);
+      if (f == 0.0f) continue;
+      const int nb = sur_nb(r, q, dir);
+      const float moved = f * (f > 0.0f ? fac[layer * cells + q] : fac[layer * cells + nb]);
+      net += moved;
+      out += (double)moved;
+    }
+    const float hv = h[idx];
+    const float hn = hv - net;
+    h[idx] = hn;
+    const double rr = ((double)hn - (double)hv) + out;
+    res_w = layer == 0 ? rr : 0.0;
+    res_s = layer == 1 ? rr : 0.0;
   }
-  h[idx] = h[idx] - net;
+  if (!totals) return;
+  for (int off = 32; off > 0; off >>= 1) {
+    res_w += __shfl_xor(res_w, off);
+    res_s += __shfl_xor(res_s, off);
+  }
+  if ((t & 63) == 0) {
+    if (res_w != 0.0) atomicAdd(&totals[0].residue, res_w);
+    if (res_s != 0.0) atomicAdd(&totals[1].residue, res_s);
+  }
 }
 
 __global__ __launch_bounds__(1024) void k_sur_factor_apply(float* __restrict__ water, float* __restrict__ slurry,
-                                                           int res, const float* __restrict__ y, SurRegion r) {
+                                                           int res, const float* __restrict__ y, SurRegion r,
+                                                           SheetTotals* totals) {
   __shared__ float fac[2 * kSurMax * kSurMax];
-  sur_factor_apply_wg(water, slurry, res, y, r, fac);
+  sur_factor_apply_wg(water, slurry, res, y, r, fac, totals);
 }
 
 __global__ __launch_bounds__(1024) void k_sur_factor_apply_many(float* __restrict__ water, float* __restrict__ slurry,
                                                                 int res, const float* __restrict__ y,
-                                                                const TireJob* __restrict__ jobs) {
+                                                                const TireJob* __restrict__ jobs,
+                                                                SheetTotals* totals) {
   __shared__ float fac[2 * kSurMax * kSurMax];
   const TireJob J = jobs[blockIdx.x];
-  sur_factor_apply_wg(water, slurry, res, y + (size_t)J.row * kMlpOut, J.r, fac);
+  sur_factor_apply_wg(water, slurry, res, y + (size_t)J.row * kMlpOut, J.r, fac, totals);
 }
 
 __global__ void k_sur_max_abs_diff(const float* __restrict__ a, const float* __restrict__ b, int n,
@@ -403,10 +429,11 @@ void sur_gather_many(const float* ctx, const TireJob* jobs, int n, uint16_t* xb,
 }
 
 void sur_apply_many(float* water, float* slurry, int res, const float* y, const TireJob* jobs, int n, float* fac,
-                    hipStream_t st) {
+                    hipStream_t st, SheetTotals* totals) {
   (void)fac;  // the factors live in LDS
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_sur_factor_apply_many, dim3((unsigned)n), dim3(1024), 0, st, water, slurry, res, y, jobs);
+  hipLaunchKernelGGL(k_sur_factor_apply_many, dim3((unsigned)n), dim3(1024), 0, st, water, slurry, res, y, jobs,
+                     totals);
 }
 
 void sur_gather_region(const float* ctx, const SurRegion& r, uint16_t* xb, hipStream_t st) {
@@ -427,10 +454,10 @@ void sur_targets(const float* hn_w, const float* hc_w, const float* hn_s, const 
 }
 
 void sur_apply(float* water, float* slurry, int res, const float* y, const SurRegion& r, float* fac,
-               hipStream_t st) {
+               hipStream_t st, SheetTotals* totals) {
   (void)fac;  // the factors live in LDS
   if (r.nx <= 0 || r.nz <= 0) return;
-  hipLaunchKernelGGL(k_sur_factor_apply, dim3(1), dim3(1024), 0, st, water, slurry, res, y, r);
+  hipLaunchKernelGGL(k_sur_factor_apply, dim3(1), dim3(1024), 0, st, water, slurry, res, y, r, totals);
 }
 
 void sur_max_abs_diff(const float* a, const float* b, int n, unsigned* out, hipStream_t st) {
