@@ -250,7 +250,12 @@ __global__ __launch_bounds__(kStampLanes) void k_stamp(float* bed, float* placed
 
 int prepare_contacts(const silt_tire_contact* in, size_t n, int res, float cell, float ox, float oz, float dt,
                      std::vector<RutContact>* out) {
-  out->clear();
+  out->assign(n, RutContact{});
+  return n ? prepare_contacts_into(in, n, res, cell, ox, oz, dt, out->data()) : 0;
+}
+
+int prepare_contacts_into(const silt_tire_contact* in, size_t n, int res, float cell, float ox, float oz, float dt,
+                          RutContact* out) {
   const float e_fast = 1.0f - expf(-dt * 48.0f), e_slow = 1.0f - expf(-dt * 2.2f);
   for (size_t q = 0; q < n; q++) {
     const silt_tire_contact& s = in[q];
@@ -287,7 +292,7 @@ int prepare_contacts(const silt_tire_contact* in, size_t n, int res, float cell,
     c.want = c.loaded ? fminf(c.forced, 0.40f) : 0.0f;
     c.e_fast = e_fast;
     c.e_slow = e_slow;
-    out->push_back(c);
+    out[q] = c;
   }
   return 0;
 }
@@ -299,9 +304,16 @@ int rut_stamp(float* bed, float* placed, const float* initial, int res, float ce
   int rc = prepare_contacts(contacts, n, res, cell, ox, oz, dt, &cs);
   if (rc) return rc;
   if (cs.empty()) return 0;
-  if (hipMemcpyAsync(d_scratch, cs.data(), cs.size() * sizeof(RutContact), hipMemcpyHostToDevice, st) != hipSuccess)
+  return rut_stamp_prepared(bed, placed, initial, res, cell, ox, oz, cs.data(), (int)cs.size(), d_scratch, st,
+                            per_group, sheet_bed, rut);
+}
+
+int rut_stamp_prepared(float* bed, float* placed, const float* initial, int res, float cell, float ox, float oz,
+                       const RutContact* host, int nc, RutContact* d_scratch, hipStream_t st, int per_group,
+                       float* sheet_bed, float* rut) {
+  if (nc <= 0) return 0;
+  if (hipMemcpyAsync(d_scratch, host, (size_t)nc * sizeof(RutContact), hipMemcpyHostToDevice, st) != hipSuccess)
     return SILT_EDEVICE;
-  const int nc = (int)cs.size();
   const int per = per_group > 0 && per_group < nc ? per_group : nc;
   hipLaunchKernelGGL(k_stamp, dim3((unsigned)((nc + per - 1) / per)), dim3(kStampLanes), 0, st, bed, placed, initial,
                      res, cell, ox, oz, d_scratch, nc, per, sheet_bed, rut);

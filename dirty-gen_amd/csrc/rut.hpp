@@ -25,6 +25,14 @@ float contact_length(float radius, float sinkage);
 
 int prepare_contacts(const silt_tire_contact* in, size_t n, int res, float cell, float ox, float oz, float dt,
                      std::vector<RutContact>* out);
+// The same into out[0..n) (no allocation).
+int prepare_contacts_into(const silt_tire_contact* in, size_t n, int res, float cell, float ox, float oz, float dt,
+                          RutContact* out);
+// Copies nc prepared contacts (host) to d_scratch and enqueues the stamp
+// kernel over them (the launch of rut_stamp).
+int rut_stamp_prepared(float* bed, float* placed, const float* initial, int res, float cell, float ox, float oz,
+                       const RutContact* host, int nc, RutContact* d_scratch, hipStream_t st, int per_group = 0,
+                       float* sheet_bed = nullptr, float* rut = nullptr);
 // Stamps the contacts in order on device planes; d_scratch holds >= n
 // contacts.  per_group > 0: consecutive groups of that many contacts whose
 // footprints do not overlap each other are stamped in parallel (in order

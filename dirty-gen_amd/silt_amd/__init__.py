@@ -23,6 +23,8 @@ SILT_EINVAL = -1
 SILT_EDEVICE = -2
 SILT_ENOMEM = -3
 SILT_EUNSUPPORTED = -4
+SILT_ETOOBIG = -5
+NOALLOC_MAX_CONTACTS = 1024  # SILT_NOALLOC_MAX_CONTACTS, include/silt_rut.h
 DRIVE_GRAVITY = 0
 DRIVE_WIND = 1
 CONFINE = {"reference": 0, "jacobi": 1}
@@ -754,6 +756,8 @@ def _declare_rut(L):
     L.silt_rut_depth.restype = _F
     L.silt_rut_stamp_device.argtypes = [_P, _P, _P, _Z, _F, _F, _F, C.POINTER(TireContact), _Z, _F, _P]
     L.silt_rut_stamp_device.restype = C.c_int
+    L.silt_rut_stamp_device_noalloc.argtypes = [_P, _P, _P, _Z, _F, _F, _F, C.POINTER(TireContact), _Z, _F, _P]
+    L.silt_rut_stamp_device_noalloc.restype = C.c_int
 
 
 def _rut_lib():
@@ -778,6 +782,15 @@ def rut_stamp_device(bed, placed, initial, res, cell, origin, contacts, dt, stre
     arr = (TireContact * len(contacts))(*[TireContact(**c) for c in contacts])
     return _check_status(_rut_lib().silt_rut_stamp_device(bed, placed, initial, res, cell, origin[0], origin[1], arr,
                                                           len(contacts), dt, stream or None))
+
+
+def rut_stamp_device_noalloc(bed, placed, initial, res, cell, origin, contacts, dt, stream=0):
+    """rut_stamp_device without any memory allocation (at most
+    NOALLOC_MAX_CONTACTS contacts; the library's one array per GPU is shared
+    by every call)."""
+    arr = (TireContact * max(len(contacts), 1))(*[TireContact(**c) for c in contacts])
+    return _check_status(_rut_lib().silt_rut_stamp_device_noalloc(bed, placed, initial, res, cell, origin[0],
+                                                                  origin[1], arr, len(contacts), dt, stream or None))
 
 
 MLP_IN, MLP_HIDDEN, MLP_OUT, MLP_PARAMS = 1536, 128, 8, 214280

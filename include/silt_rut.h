@@ -44,6 +44,21 @@ float silt_rut_depth(float normal, float push, float radius, float width, float 
 int silt_rut_stamp_device(float *bed, float *placed, const float *initial, size_t res, float cell, float origin_x,
                           float origin_z, const silt_tire_contact *contacts, size_t n, float dt, void *hip_stream);
 
+/* silt_rut_stamp_device without any memory allocation.  No counterpart in
+ * the reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * prepared contacts go to a fixed device array of SILT_NOALLOC_MAX_CONTACTS
+ * records (88 KiB per GPU, held until the process ends) instead of a buffer
+ * the library grows; more contacts return SILT_ETOOBIG.  Every call on one
+ * GPU shares that array: calls that overlap on one GPU race (the call waits
+ * on hip_stream for its own stamp before it returns, as
+ * silt_rut_stamp_device does).  Same arguments and results as
+ * silt_rut_stamp_device. */
+#define SILT_NOALLOC_MAX_CONTACTS 1024
+int silt_rut_stamp_device_noalloc(float *bed, float *placed, const float *initial, size_t res, float cell,
+                                  float origin_x, float origin_z, const silt_tire_contact *contacts, size_t n,
+                                  float dt, void *hip_stream);
+
 #ifdef __cplusplus
 }
 #endif

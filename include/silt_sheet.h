@@ -33,6 +33,8 @@ extern "C" {
 #define SILT_EDEVICE (-2)
 #define SILT_ENOMEM (-3)
 #define SILT_EUNSUPPORTED (-4)
+/* A size beyond the fixed maximum of an entry point that allocates nothing. */
+#define SILT_ETOOBIG (-5)
 
 /* SheetDrive, src/fluid_core.rs:204-211. */
 #define SILT_DRIVE_GRAVITY 0
