@@ -22,13 +22,23 @@
 // ring (slot = row parity), and is read one iteration after it is written.
 // So one barrier per iteration orders every exchange: it separates this
 // row's writes from the next row's reads, and this row's reads of a slot from
-// the next row's writes to it.  The row loop is unrolled by two, so the
-// parity and with it every LDS offset is a compile-time constant.  The x
+// the next row's writes to it; every such read of a row is issued at the
+// row's start.  The row loop is unrolled by two (six in the interior), so
+// the parity and with it every LDS offset is a compile-time constant.  The x
 // transport is itself pipelined over three rows: each lane computes the two
 // fluxes of its own face (col, col+1) and takes its neighbour's from LDS.
 // A value a later stage reads only in its own column (the vertical
 // neighbours of every z difference, the lags between stages) stays in
-// registers: each lane keeps a short window of the rows it produced.
+// registers: each lane keeps a short window of the rows it produced.  No
+// window is deeper than six rows, so in the six-row interior loop every
+// window rotates by register renaming, not by moves.  Global loads run two
+// rows ahead of their use.
+//
+// A workgroup whose strip lies inside the grid and whose window has one
+// viscosity and open gates everywhere (the common case) runs a lean copy of
+// the row body without the ghost-column stores and per-cell law branches.
+// Three workgroups per CU (168 registers per lane) keep the windows out of
+// scratch memory.
 //
 // The reference clamps every stencil at the grid edge.  Columns: the lane of
 // the first or last grid column also writes its value into two ghost columns.
@@ -160,7 +170,7 @@ __device__ __forceinline__ void push(float (&w)[N], float v) {
 }  // namespace
 
 template <bool EXACT, bool PROJECT, bool WIND, bool FULL>
-__global__ __launch_bounds__(kLanes, 4) void k_sheet_stream(FusedArgs fa) {
+__global__ __launch_bounds__(kLanes, 3) void k_sheet_stream(FusedArgs fa) {
   using A = Ar<EXACT>;
   __shared__ float lds[kRows * kRow];
 
@@ -232,10 +242,22 @@ __global__ __launch_bounds__(kLanes, 4) void k_sheet_stream(FusedArgs fa) {
     }
   };
   auto at = [&](int row, int d) -> float { return my[row * kRow + d]; };
+  // Global addresses: the row's base (uniform, scalar registers) plus this
+  // lane's byte offset (fixed for the launch), so an access needs no vector
+  // address arithmetic.
+  const unsigned off_ld = (unsigned)colc * 4u;
+  const unsigned off_st = (unsigned)(col < 0 ? 0 : col) * 4u;
+  auto row_ld = [&](const float* p, int y) -> float {
+    const char* rp = reinterpret_cast<const char*>(p + (size_t)y * (size_t)res);
+    return *reinterpret_cast<const float*>(rp + off_ld);
+  };
+  auto row_st = [&](float* p, int y, float val) {
+    char* rp = reinterpret_cast<char*>(p + (size_t)y * (size_t)res);
+    *reinterpret_cast<float*>(rp + off_st) = val;
+  };
   auto load = [&](const float* p, int y) -> float {
     int yc = y < 0 ? 0 : (y >= res ? res - 1 : y);
-    const float* rp = p + (size_t)yc * res;
-    return rp[colc];
+    return row_ld(p, yc);
   };
 
   const float inv2c = L.inv2c;
@@ -296,6 +318,9 @@ __global__ __launch_bounds__(kLanes, 4) void k_sheet_stream(FusedArgs fa) {
   // (col, col+1) (row r-14).
   float xa1 = 0.0f, xa2 = 0.0f, xh1 = 0.0f, xh2 = 0.0f, xhdf = 0.0f, xflm1 = 0.0f, xb = 0.0f;
   bool xpre = false;
+  // The depth before transport of the x pass's row (r-14) and of the z pass's
+  // row (r-15), for the residue.
+  float x3_h = 0.0f, x4_h = 0.0f;
   // z fluxes of the edge (y, y+1) of the previous S12 row y, which is the
   // edge (y-1, y) of the next: up-going and down-going.
   float c_fu_0 = 0.0f, c_fd_0 = 0.0f;
@@ -304,15 +329,20 @@ __global__ __launch_bounds__(kLanes, 4) void k_sheet_stream(FusedArgs fa) {
   // dry-cut volume and transport residue (what FP32 rounding and the max(0)
   // clamp added beyond the fluxes), plus the wet bounding box, wet count and
   // largest squared speed the next step's plan needs.  A cell's residue is
-  // formed exactly in FP64; the residues and cut depths of one column (each
-  // below 1e-6 m) are summed in FP32, whose error on a column's few hundred
-  // terms is some 1e-13 m, and the column sums reduce in FP64.
+  // its depth after both transport passes less its depth before them, in
+  // FP64: every face's FP32 flux leaves one cell and enters another, so over
+  // the grid the exchanged fluxes cancel and the residues sum to exactly the
+  // volume the FP32 updates created or removed.  Cut depths (each below
+  // 1e-6 m) are summed per column in FP32 and reduced in FP64.
   const bool want_stats = L.stats != nullptr;
-  double acc_vol = 0.0;
-  float acc_cut = 0.0f, acc_res = 0.0f;
+  double acc_vol = 0.0, acc_res = 0.0;
+  float acc_cut = 0.0f;
+  float acc_s2 = 0.0f;
+  // Wet cells: counted per wavefront from ballots, their rows and lanes
+  // kept as scalars (rows come in increasing order).
   unsigned acc_wet = 0;
   int acc_j0 = 0x7fffffff, acc_j1 = -1;
-  float acc_s2 = 0.0f;
+  uint64_t acc_lanes = 0;
   const bool out_col = t >= kHaloL && t < kHaloL + kOutW && col_in;
 
   // A tile that no work box reaches stores its input unchanged (outside the
@@ -327,35 +357,48 @@ __global__ __launch_bounds__(kLanes, 4) void k_sheet_stream(FusedArgs fa) {
   const bool gate_uni = a.gate_uniform != 0;
   const bool vort = a.vort > 0.0f;
 
-  // Global loads run one row ahead of their use: the loads of row r + 1 are
-  // issued at the start of row r and consumed when row r + 1 starts.
+  // Global loads run two rows ahead of their use, in two register sets by
+  // row parity: the loads of row r + 2 are issued at the start of row r and
+  // consumed when row r + 2 starts (a row takes about a microsecond, less
+  // than a loaded HBM access).
   //   pf_h, pf_b   h and bed of S1's row         (r)
   //   pf_h4, pf_u4, pf_v4   h, u, v of S4's row  (r-4)
   //   pf_h11       h of S10's row                (r-11)
   // S6 reads h of row r-6, which S4 loaded two rows earlier (h6).
-  float pf_h, pf_b, pf_h4, pf_u4, pf_v4, pf_h11, h6[2] = {0.0f, 0.0f};
-  auto prefetch = [&](auto edge, int r) {
+  float pf_h[2], pf_b[2], pf_h4[2], pf_u4[2], pf_v4[2], pf_h11[2], h6[2] = {0.0f, 0.0f};
+  auto prefetch = [&](auto edge, auto slot, auto lean, int r) __attribute__((always_inline)) {
     constexpr bool EDGE = decltype(edge)::value;
+    constexpr int Q = decltype(slot)::value;
+    constexpr bool LEAN = decltype(lean)::value;
     auto ld = [&](const float* p, int y) -> float {
-      if constexpr (EDGE)
-        return load(p, y);
-      else
-        return p[(size_t)y * res + colc];
+      if constexpr (EDGE) y = y < 0 ? 0 : (y >= res ? res - 1 : y);
+      return row_ld(p, y);
     };
-    pf_h = ld(L.h, r);
-    pf_b = ld(L.bed, r);
-    pf_h4 = ld(L.h, r - 4);
-    pf_u4 = ld(L.u, r - 4);
-    pf_v4 = ld(L.v, r - 4);
-    pf_h11 = ld(L.h, r - 11);
+    pf_h[Q] = ld(L.h, r);
+    pf_b[Q] = ld(L.bed, r);
+    pf_h4[Q] = ld(L.h, r - 4);
+    pf_u4[Q] = ld(L.u, r - 4);
+    pf_v4[Q] = ld(L.v, r - 4);
+    pf_h11[Q] = ld(L.h, r - 11);
   };
 
   // One row of the pipeline.  EDGE = false is the body for rows whose
   // stencils and loads all lie inside the grid: no clamps, no edge selects,
   // and with FULL no row masks.
-  auto row = [&](auto par, auto edge, int r) {
+  auto row = [&](auto par, auto edge, auto lean, int r) __attribute__((always_inline)) {
     constexpr int P = decltype(par)::value;
     constexpr bool EDGE = decltype(edge)::value;
+    // LEAN: a strip inside the grid (no ghost columns) with one viscosity and
+    // open gates everywhere, decided once per workgroup.
+    constexpr bool LEAN = decltype(lean)::value;
+    const bool nu_u = LEAN || nu_uni;
+    const bool gate_u = LEAN || gate_uni;
+    auto putv = [&](int rw, float val) {
+      if constexpr (LEAN)
+        my[rw * kRow] = val;
+      else
+        put(rw, val);
+    };
     if constexpr (!FULL) {
 #pragma unroll
       for (int k = 15; k > 1; k--) RM[k] = RM[k - 1];
@@ -372,7 +415,7 @@ __global__ __launch_bounds__(kLanes, 4) void k_sheet_stream(FusedArgs fa) {
       if constexpr (EDGE)
         return load(p, y);
       else
-        return p[(size_t)y * res + colc];
+        return row_ld(p, y);
     };
     auto abv = [&](int y, float c, float p) -> float {
       if constexpr (EDGE)
@@ -393,20 +436,38 @@ __global__ __launch_bounds__(kLanes, 4) void k_sheet_stream(FusedArgs fa) {
         return acc5(w[0], w[1], w[2], w[3], w[4], k);
     };
 
-    const float h4 = pf_h4, u4 = pf_u4, v4 = pf_v4, h11 = pf_h11;
+    const float h4 = pf_h4[P], u4 = pf_u4[P], v4 = pf_v4[P], h11 = pf_h11[P];
+    // Every value this row reads across lanes was written by the previous
+    // row, before the barrier that ended it, into the ring slot this row
+    // does not write: all of them are read here, at once, so their latency
+    // overlaps (a value whose stage is off this launch is read and unused).
+    constexpr int RE = R_ETA + kSlot<P, 1>, RP = R_P + kSlot<P, 4>, RV = R_VG + kSlot<P, 5>,
+                  RW = R_W + kSlot<P, 6>, RU = R_UC + kSlot<P, 7>, RB = R_B + kSlot<P, 8>, RQ = R_Q + kSlot<P, 11>,
+                  RUF = R_UF + kSlot<P, 12>, RHX = R_HX + kSlot<P, 12>, RGT = R_GT + kSlot<P, 12>,
+                  RFA = R_FA + kSlot<P, 13>, RFB = R_FB + kSlot<P, 14>;
+    const float l_e0 = at(RE, -2), l_e1 = at(RE, -1), l_e2 = at(RE, 0), l_e3 = at(RE, 1), l_e4 = at(RE, 2);
+    const float l_pp = at(RP, 1), l_pm = at(RP, -1);
+    const float l_vp = at(RV, 1), l_vm = at(RV, -1);
+    const float l_wp = at(RW, 1), l_wm = at(RW, -1);
+    const float l_up = at(RU, 1), l_um = at(RU, -1);
+    const float l_lp = my[(R_LW + ((r - 7) & 7)) * kRow];
+    const float l_b0 = at(RB, -2), l_b1 = at(RB, -1), l_b2 = at(RB, 0), l_b3 = at(RB, 1), l_b4 = at(RB, 2);
+    const float l_qp = at(RQ, 1), l_qm = at(RQ, -1);
+    const float l_uf0 = at(RUF, 0), l_uf1 = at(RUF, 1), l_hx0 = at(RHX, 0), l_hx1 = at(RHX, 1);
+    const float l_g0 = gate_u ? 1.0f : at(RGT, 0), l_g1 = gate_u ? 1.0f : at(RGT, 1);
+    const float l_fa = at(RFA, -1), l_fb = at(RFB, -1);
     // ---- S1: eta at row r ------------------------------------------------
     {
-      const float ev = pf_b + pf_h;
-      prefetch(edge, r + 1);
-      put(R_ETA + kSlot<P, 0>, ev);
+      const float ev = pf_b[P] + pf_h[P];
+      prefetch(edge, par, lean, r + 2);
+      putv(R_ETA + kSlot<P, 0>, ev);
       push(E, ev);
     }
     // ---- S2: x filters of eta, lap(eta) at row r-1 -------------------------
     {
-      constexpr int RE = R_ETA + kSlot<P, 1>;
       float g = 0.0f, s = 0.0f;
       if (!WIND) {
-        float e0 = at(RE, -2), e1 = at(RE, -1), e2 = at(RE, 0), e3 = at(RE, 1), e4 = at(RE, 2);
+        float e0 = l_e0, e1 = l_e1, e2 = l_e2, e3 = l_e3, e4 = l_e4;
         bool inside = (colmask & rmk(1)) != 0;
         // accum5 (interior) and accum5_clamped (edges) differ only in a
         // leading 0.0 +, which leaves the value unchanged.
@@ -417,7 +478,7 @@ __global__ __launch_bounds__(kLanes, 4) void k_sheet_stream(FusedArgs fa) {
       push(SX, s);
       if (PROJECT) {
         // Rows r and r-2 of this column: E[3], E[1] (S1 loaded clamped rows).
-        float sum = at(RE, 1) + at(RE, -1) + E[3] + E[1] - 4.0f * E[2];
+        float sum = l_e3 + l_e1 + E[3] + E[1] - 4.0f * E[2];
         my[(R_LW + ((r - 1) & 7)) * kRow] = A::div(sum, cell * cell, L.invc2);
       }
     }
@@ -427,7 +488,7 @@ __global__ __launch_bounds__(kLanes, 4) void k_sheet_stream(FusedArgs fa) {
       float gz = zac(GX, y, kG), sz = zac(SX, y, kS);
       float filtered = A::fma_(0.38f, sz, 0.62f * gz);
       float pv = (colmask & rmk(3)) != 0 ? A::fma_(0.55f, E[0], 0.45f * filtered) : 0.0f;
-      put(R_P + kSlot<P, 3>, pv);
+      putv(R_P + kSlot<P, 3>, pv);
       push(PW, pv);
     }
     // ---- S4: gravity / wind blend at row r-4 -> snapshot ------------------
@@ -444,10 +505,9 @@ __global__ __launch_bounds__(kLanes, 4) void k_sheet_stream(FusedArgs fa) {
         un = A::fma_(a.wx - uu, a.blend, uu);
         vn = A::fma_(a.wz - vv, a.blend, vv);
       } else {
-        constexpr int RP = R_P + kSlot<P, 4>;
-        float px = A::div(at(RP, 1) - at(RP, -1), two, inv2c);
+        float px = A::div(l_pp - l_pm, two, inv2c);
         float pz = A::div(abv(y, PW[1], PW[2]) - blw(y, PW[0], PW[1]), two, inv2c);
-        float mob = nu_uni ? a.mob0 : sheet_mobility(ld(L.nu, y));
+        float mob = nu_u ? a.mob0 : sheet_mobility(ld(L.nu, y));
         if (EXACT) {
           un = uu + -kGravity * px * mob * a.dt;
           vn = vv + -kGravity * pz * mob * a.dt;
@@ -459,18 +519,17 @@ __global__ __launch_bounds__(kLanes, 4) void k_sheet_stream(FusedArgs fa) {
       }
       uu = act ? un : uu;
       vv = act ? vn : vv;
-      put(R_VG + kSlot<P, 4>, vv);
+      putv(R_VG + kSlot<P, 4>, vv);
       push(UG, uu);
       push(VG, vv);
     }
     // ---- S5: vorticity of the snapshot at row r-5 -------------------------
     if (vort) {
       const int y = r - 5;
-      constexpr int RV = R_VG + kSlot<P, 5>;
-      float dvdx = A::div(at(RV, 1) - at(RV, -1), two, inv2c);
+      float dvdx = A::div(l_vp - l_vm, two, inv2c);
       float dudz = A::div(abv(y, UG[1], UG[2]) - blw(y, UG[0], UG[1]), two, inv2c);
       float w = dvdx - dudz;
-      put(R_W + kSlot<P, 5>, w);
+      putv(R_W + kSlot<P, 5>, w);
       push(WW, w);
     }
     // ---- S6: confinement at row r-6 ---------------------------------------
@@ -481,8 +540,7 @@ __global__ __launch_bounds__(kLanes, 4) void k_sheet_stream(FusedArgs fa) {
       if (vort) {
         const float hv = h6[P];
         const float w = WW[1];
-        constexpr int RW = R_W + kSlot<P, 6>;
-        float wxp = fabsf(at(RW, 1)), wxm = fabsf(at(RW, -1));
+        float wxp = fabsf(l_wp), wxm = fabsf(l_wm);
         float wzp = fabsf(abv(y, WW[1], WW[2])), wzm = fabsf(blw(y, WW[0], WW[1]));
         float nx = A::div(wxp - wxm, two, inv2c);
         float nz = A::div(wzp - wzm, two, inv2c);
@@ -499,7 +557,7 @@ __global__ __launch_bounds__(kLanes, 4) void k_sheet_stream(FusedArgs fa) {
         uu = ok ? uu + du : uu;
         vv = ok ? vv + dv : vv;
       }
-      put(R_UC + kSlot<P, 6>, uu);
+      putv(R_UC + kSlot<P, 6>, uu);
       push(UC, uu);
       push(VC, vv);
       h6[P] = h4;  // S6's row two rows on
@@ -507,32 +565,30 @@ __global__ __launch_bounds__(kLanes, 4) void k_sheet_stream(FusedArgs fa) {
     // ---- S7: divergence source at row r-7 ---------------------------------
     if (PROJECT) {
       const int y = r - 7;
-      constexpr int RU = R_UC + kSlot<P, 7>;
-      float ux = A::div(at(RU, 1) - at(RU, -1), two, inv2c);
+      float ux = A::div(l_up - l_um, two, inv2c);
       // VC holds rows r-11..r-6: y-1, y, y+1 at 3, 4, 5.
       float vz = A::div(abv(y, VC[4], VC[5]) - blw(y, VC[3], VC[4]), two, inv2c);
       float div = ux + vz;
-      float lp = my[(R_LW + ((r - 7) & 7)) * kRow];
-      float ov = nu_uni ? a.ov0 : overload_of(ld(L.nu, y));
+      float lp = l_lp;
+      float ov = nu_u ? a.ov0 : overload_of(ld(L.nu, y));
       float bv;
       if (EXACT)
         bv = div * ov - 0.015f * ov * lp;
       else
         bv = ov * __builtin_fmaf(-0.015f, lp, div);
       bv = (colmask & rmk(7)) != 0 ? bv : 0.0f;
-      put(R_B + kSlot<P, 7>, bv);
+      putv(R_B + kSlot<P, 7>, bv);
     }
     // ---- S8: x filters of b at row r-8; S9: q at row r-10 ------------------
     if (PROJECT) {
-      constexpr int RB = R_B + kSlot<P, 8>;
       bool inside = (colmask & rmk(8)) != 0;
-      float b0 = at(RB, -2), b1 = at(RB, -1), b2 = at(RB, 0), b3 = at(RB, 1), b4 = at(RB, 2);
+      float b0 = l_b0, b1 = l_b1, b2 = l_b2, b3 = l_b3, b4 = l_b4;
       push(GB, inside ? acc5(b0, b1, b2, b3, b4, kG) : 0.0f);
       push(SB, inside ? acc5(b0, b1, b2, b3, b4, kS) : 0.0f);
       const int yq = r - 10;
       float gz = zac(GB, yq, kG), sz = zac(SB, yq, kS);
       float qv = (colmask & rmk(10)) != 0 ? A::fma_(0.38f, sz, 0.62f * gz) : 0.0f;
-      put(R_Q + kSlot<P, 10>, qv);
+      putv(R_Q + kSlot<P, 10>, qv);
       push(QW, qv);
     }
     // ---- S10: correction, damping, cap at row r-11 -------------------------
@@ -544,8 +600,7 @@ __global__ __launch_bounds__(kLanes, 4) void k_sheet_stream(FusedArgs fa) {
       const bool act = (colmask & rmk(11)) != 0 && hv > eps;
       float un = uu, vn = vv;
       if (PROJECT) {
-        constexpr int RQ = R_Q + kSlot<P, 11>;
-        float qx = A::div(at(RQ, 1) - at(RQ, -1), two, inv2c);
+        float qx = A::div(l_qp - l_qm, two, inv2c);
         float qz = A::div(abv(y, QW[1], QW[2]) - blw(y, QW[0], QW[1]), two, inv2c);
         if (EXACT) {
           un = uu + 0.012f * a.dt * qx;
@@ -555,19 +610,19 @@ __global__ __launch_bounds__(kLanes, 4) void k_sheet_stream(FusedArgs fa) {
           vn = __builtin_fmaf(L.kdt, qz, vv);
         }
       }
-      float decay = nu_uni ? a.decay0 : expf(-sheet_damp(ld(L.nu, y)) * a.dt);
+      float decay = nu_u ? a.decay0 : expf(-sheet_damp(ld(L.nu, y)) * a.dt);
       un = finite_speed(un * decay, a.lim);
       vn = finite_speed(vn * decay, a.lim);
       uu = act ? un : uu;
       vv = act ? vn : vv;
-      put(R_UF + kSlot<P, 11>, uu);
+      putv(R_UF + kSlot<P, 11>, uu);
       push(UF, uu);
       push(VF, vv);  // the z transport reads only this lane's column
-      put(R_HX + kSlot<P, 11>, hv);
-      if (!gate_uni) {
+      putv(R_HX + kSlot<P, 11>, hv);
+      if (!gate_u) {
         int yc = EDGE ? (y < 0 ? 0 : (y >= res ? res - 1 : y)) : y;
         float gv = (float)L.gate[(size_t)yc * res + colc];
-        put(R_GT + kSlot<P, 11>, gv);
+        putv(R_GT + kSlot<P, 11>, gv);
         push(GW, gv);
       }
     }
@@ -577,14 +632,13 @@ __global__ __launch_bounds__(kLanes, 4) void k_sheet_stream(FusedArgs fa) {
       // X1, row r-12: the left-going flux of face (col, col+1) (donor col+1)
       // goes to LDS; of its right-going flux (donor col) all but the room,
       // which needs the neighbour's left-going flux.
-      constexpr int RUF = R_UF + kSlot<P, 12>, RHX = R_HX + kSlot<P, 12>, RGT = R_GT + kSlot<P, 12>;
       const bool e_0 = (colmask & colmask_r & rmk(12)) != 0;
-      const float speed = 0.5f * (at(RUF, 0) + at(RUF, 1));
-      const float hc = at(RHX, 0), hr = at(RHX, 1);
+      const float speed = 0.5f * (l_uf0 + l_uf1);
+      const float hc = l_hx0, hr = l_hx1;
       const bool live = !(fabsf(speed) < 1.0e-5f);
       const float frac = fminf(EXACT ? fabsf(speed) * a.dt / cell : fabsf(speed) * dtc, 0.45f);
-      const bool shut_l = !gate_uni && at(RGT, 0) == 0.0f && hc <= 1.0e-6f;
-      const bool shut_r = !gate_uni && at(RGT, 1) == 0.0f && hr <= 1.0e-6f;
+      const bool shut_l = !gate_u && l_g0 == 0.0f && hc <= 1.0e-6f;
+      const bool shut_r = !gate_u && l_g1 == 0.0f && hr <= 1.0e-6f;
       const float dl = fminf(hr * frac, hr);
       const bool ok_l = e_0 && live && !(hr <= 1.0e-8f) && !shut_l && !(dl <= 1.0e-8f) && !(speed > 0.0f);
       const float fa = ok_l ? dl : 0.0f;
@@ -594,7 +648,7 @@ __global__ __launch_bounds__(kLanes, 4) void k_sheet_stream(FusedArgs fa) {
       // X2, row r-13: the right-going flux of face (col, col+1), whose donor
       // has already lost the left-going flux of face (col-1, col).
       const bool e_m1b = (colmask_l1 & colmask & rmk(13)) != 0;
-      const float fl_m1 = e_m1b ? at(R_FA + kSlot<P, 13>, -1) : 0.0f;
+      const float fl_m1 = e_m1b ? l_fa : 0.0f;
       const float room = xh1 - (0.0f + fl_m1);
       const float dr = fminf(xhdf, room);
       const float fb = (xpre && !(room <= 1.0e-8f) && !(dr <= 1.0e-8f)) ? dr : 0.0f;
@@ -602,15 +656,16 @@ __global__ __launch_bounds__(kLanes, 4) void k_sheet_stream(FusedArgs fa) {
       // X3, row r-14: the depth after the x pass.
       const int yx = r - 14;
       const bool e_m1c = (colmask_l1 & colmask & rmk(14)) != 0;
-      const float fr_m1 = e_m1c ? at(R_FB + kSlot<P, 14>, -1) : 0.0f;
-      const float rem = (0.0f + xflm1) + xb;
-      const float add = (0.0f + fr_m1) + xa2;
+      const float fr_m1 = e_m1c ? l_fb : 0.0f;
+      // The reference sums (0.0 + a) + b; a flux is +0 or positive, never -0,
+      // so a + b is the same value.
+      const float rem = xflm1 + xb;
+      const float add = fr_m1 + xa2;
       const float hv = xh2;
       const float moved = fmaxf(hv - rem + add, 0.0f);
       const float h1 = (rem > 0.0f || add > 0.0f) ? moved : hv;
-      if (want_stats)
-        acc_res += out_col && yx >= y0 && yx < y1 ? (float)(((double)h1 - (double)hv) - ((double)add - (double)rem))
-                                                  : 0.0f;
+      (void)yx;
+      x3_h = hv;
       push(H1, h1);
       xa2 = xa1;
       xa1 = fa;
@@ -628,7 +683,7 @@ __global__ __launch_bounds__(kLanes, 4) void k_sheet_stream(FusedArgs fa) {
       // the masks of rows y, y+1.
       const bool e_0 = (colmask & rmk(15) & rmk(14)) != 0;  // edge (y, y+1)
       const float dtc = L.dtc;
-      auto gate_ok = [&](float g) -> bool { return gate_uni ? true : g != 0.0f; };
+      auto gate_ok = [&](float g) -> bool { return gate_u ? true : g != 0.0f; };
       // Up-going flux of edge (k, k+1) (donor k+1 = window index k+1).
       auto fu = [&](int k, bool recv_gate) -> float {
         float speed = 0.5f * (VF[k] + VF[k + 1]);
@@ -660,9 +715,11 @@ __global__ __launch_bounds__(kLanes, 4) void k_sheet_stream(FusedArgs fa) {
       const float fd_0 = e_0 ? fd_c : 0.0f;
       c_fu_0 = fu_0;
       c_fd_0 = fd_0;
-      float rem = (0.0f + fu_m1) + fd_0;
-      float add = (0.0f + fd_m1) + fu_0;
+      float rem = fu_m1 + fd_0;
+      float add = fd_m1 + fu_0;
       float hv = H1[2];
+      const float h_orig = x4_h;
+      x4_h = x3_h;
       float moved = fmaxf(hv - rem + add, 0.0f);
       float h2 = (rem > 0.0f || add > 0.0f) ? moved : hv;
       float uo = UF[0];
@@ -673,25 +730,27 @@ __global__ __launch_bounds__(kLanes, 4) void k_sheet_stream(FusedArgs fa) {
       if (want_stats) {
         // Adding +0.0 for the lanes and rows that are not stored keeps the
         // sums unchanged without a branch.
-        acc_res += store ? (float)(((double)h2 - (double)hv) - ((double)add - (double)rem)) : 0.0f;
+        if (store) acc_res += (double)h2 - (double)h_orig;
         acc_cut += store && cut ? h2 : 0.0f;
       }
       h2 = cut ? 0.0f : h2;
       uo = cut ? 0.0f : uo;
       vo = cut ? 0.0f : vo;
       if (want_stats) {
-        acc_vol += store ? (double)fmaxf(h2, 0.0f) : 0.0;
-        const bool wet = store && h2 > eps;
-        acc_wet += wet ? 1u : 0u;
-        acc_j0 = wet ? min(acc_j0, y) : acc_j0;
-        acc_j1 = wet ? y : acc_j1;  // rows come in increasing order
+        if (store) acc_vol += (double)fmaxf(h2, 0.0f);
+        const uint64_t wm = __builtin_amdgcn_ballot_w64(store && h2 > eps);
+        acc_wet += (unsigned)__builtin_popcountll(wm);
+        acc_lanes |= wm;
+        if (wm) {
+          acc_j0 = min(acc_j0, y);
+          acc_j1 = y;
+        }
         acc_s2 = store ? fmaxf(acc_s2, speed2(uo, vo)) : acc_s2;
       }
       if (store) {
-        size_t idx = (size_t)y * res + col;
-        L.ho[idx] = h2;
-        L.uo[idx] = uo;
-        L.vo[idx] = vo;
+        row_st(L.ho, y, h2);
+        row_st(L.uo, y, uo);
+        row_st(L.vo, y, vo);
       }
     }
     __syncthreads();
@@ -707,58 +766,79 @@ __global__ __launch_bounds__(kLanes, 4) void k_sheet_stream(FusedArgs fa) {
   using Edge = std::true_type;
   using Inside = std::false_type;
   if (reached) {
-    prefetch(Edge{}, r_begin);
-    // Rows r-20 .. r+5 inside the grid cover every stage row, stencil
-    // neighbour and load of the pair of rows (r, r+1): those pairs take the
-    // interior body.  They form one run, so three loops: top edge, interior,
-    // bottom edge.
-    auto interior = [&](int r) { return r - 20 >= 0 && r + 5 < res; };
-    int r = r_begin;
-    for (; r <= r_end && !interior(r); r += 2) {
-      row(Even{}, Edge{}, r);
-      row(Odd{}, Edge{}, r + 1);
-    }
-    for (; r <= r_end && interior(r); r += 2) {
-      row(Even{}, Inside{}, r);
-      row(Odd{}, Inside{}, r + 1);
-    }
-    for (; r <= r_end; r += 2) {
-      row(Even{}, Edge{}, r);
-      row(Odd{}, Edge{}, r + 1);
-    }
-  } else if (out_col) {
-#pragma unroll 4
+    auto run = [&](auto lean) __attribute__((always_inline)) {
+      prefetch(Edge{}, Even{}, lean, r_begin);
+      prefetch(Edge{}, Odd{}, lean, r_begin + 1);
+      // Rows r-20 .. r+5 inside the grid cover every stage row, stencil
+      // neighbour and load of the pair of rows (r, r+1): those pairs take the
+      // interior body.  They form one run, so three loops: top edge, interior,
+      // bottom edge.
+      auto interior = [&](int r) { return r - 20 >= 0 && r + 5 < res; };
+      int r = r_begin;
+      for (; r <= r_end && !interior(r); r += 2) {
+        row(Even{}, Edge{}, lean, r);
+        row(Odd{}, Edge{}, lean, r + 1);
+      }
+      // Six rows per iteration in the interior: every register window is
+      // at most six rows deep, so each rotates by renaming instead of moves.
+      for (; r + 4 <= r_end && interior(r) && interior(r + 4); r += 6) {
+        row(Even{}, Inside{}, lean, r);
+        row(Odd{}, Inside{}, lean, r + 1);
+        row(Even{}, Inside{}, lean, r + 2);
+        row(Odd{}, Inside{}, lean, r + 3);
+        row(Even{}, Inside{}, lean, r + 4);
+        row(Odd{}, Inside{}, lean, r + 5);
+      }
+      for (; r <= r_end && interior(r); r += 2) {
+        row(Even{}, Inside{}, lean, r);
+        row(Odd{}, Inside{}, lean, r + 1);
+      }
+      for (; r <= r_end; r += 2) {
+        row(Even{}, Edge{}, lean, r);
+        row(Odd{}, Edge{}, lean, r + 1);
+      }
+    };
+    if (!edge_strip && nu_uni && gate_uni)
+      run(std::true_type{});
+    else
+      run(std::false_type{});
+  } else {
+    // Every lane walks the rows (the wet ballots count whole waves); only the
+    // output columns load and store.
     for (int y = y0; y < y1; y++) {
-      const size_t idx = (size_t)y * res + col;
-      const float hv = L.h[idx], uv = L.u[idx], vv = L.v[idx];
-      L.ho[idx] = hv;
-      L.uo[idx] = uv;
-      L.vo[idx] = vv;
+      float hv = 0.0f, uv = 0.0f, vv = 0.0f;
+      if (out_col) {
+        hv = row_ld(L.h, y);
+        uv = row_ld(L.u, y);
+        vv = row_ld(L.v, y);
+        row_st(L.ho, y, hv);
+        row_st(L.uo, y, uv);
+        row_st(L.vo, y, vv);
+      }
       if (want_stats) {
-        acc_vol += (double)fmaxf(hv, 0.0f);
-        const bool wet = hv > eps;
-        acc_wet += wet ? 1u : 0u;
-        acc_j0 = wet ? min(acc_j0, y) : acc_j0;
-        acc_j1 = wet ? y : acc_j1;
-        acc_s2 = fmaxf(acc_s2, speed2(uv, vv));
+        if (out_col) acc_vol += (double)fmaxf(hv, 0.0f);
+        const uint64_t wm = __builtin_amdgcn_ballot_w64(out_col && hv > eps);
+        acc_wet += (unsigned)__builtin_popcountll(wm);
+        acc_lanes |= wm;
+        if (wm) {
+          acc_j0 = min(acc_j0, y);
+          acc_j1 = y;
+        }
+        acc_s2 = out_col ? fmaxf(acc_s2, speed2(uv, vv)) : acc_s2;
       }
     }
   }
 
   if (want_stats) {
-    // The wet columns of this lane span one column; the bounding box reduces
-    // over lanes with shuffles, then one atomic per wave.
-    int acc_i0 = acc_wet ? col : 0x7fffffff, acc_i1 = acc_wet ? col : -1;
+    // Wet lanes of this wave: the columns of the lowest and highest set bit.
+    const int lane0 = x0 - kHaloL + (t & ~63);
+    int acc_i0 = acc_lanes ? lane0 + __builtin_ctzll(acc_lanes) : 0x7fffffff;
+    int acc_i1 = acc_lanes ? lane0 + 63 - __builtin_clzll(acc_lanes) : -1;
     double sum_cut = acc_cut, sum_res = acc_res;
     for (int off = 32; off > 0; off >>= 1) {
       acc_vol += __shfl_xor(acc_vol, off);
       sum_cut += __shfl_xor(sum_cut, off);
       sum_res += __shfl_xor(sum_res, off);
-      acc_wet += __shfl_xor(acc_wet, off);
-      acc_i0 = min(acc_i0, __shfl_xor(acc_i0, off));
-      acc_i1 = max(acc_i1, __shfl_xor(acc_i1, off));
-      acc_j0 = min(acc_j0, __shfl_xor(acc_j0, off));
-      acc_j1 = max(acc_j1, __shfl_xor(acc_j1, off));
       acc_s2 = fmaxf(acc_s2, __shfl_xor(acc_s2, off));
     }
     if ((t & 63) == 0) {
