@@ -869,6 +869,22 @@ void F(oracle_core_fine_water_debug)(Core *c, const real *p8, int np, real *fw, 
     restrict_fine(c);
 }
 
+/*
+ * Test entry points: push_on, peel_spray and soften_border on their own.
+ * p8: x, z, forward x, forward z, radius, width, speed, slip.
+ * oracle_core_peel uses the pushes given with oracle_core_push_tire and
+ * returns the volume peeled; events receives the event count.
+ */
+void F(oracle_core_push_on)(Core *c, const real *p8, int fine) {
+    Push p = {p8[0], p8[1], p8[2], p8[3], p8[4], p8[5], p8[6], p8[7]};
+    push_on(c, &p, fine);
+}
+real F(oracle_core_peel)(Core *c, int fine, int *events) {
+    *events = 0;
+    return peel_spray(c, fine, 0, events);
+}
+void F(oracle_soften_border)(real *u, real *v, int res) { soften_border(u, v, res); }
+
 void F(oracle_core_push_tire)(Core *c, const real *p) {
     if (c->npush >= MAX_PUSH) return;
     Push q = {p[0], p[1], p[2], p[3], p[4], p[5], p[6], p[7]};

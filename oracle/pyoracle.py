@@ -72,6 +72,15 @@ def _declare(L):
         f = getattr(L, "oracle_column_sum" + sfx)
         f.argtypes = [_P, _Z, R]
         f.restype = R
+        f = getattr(L, "oracle_surface_pressure" + sfx)
+        f.argtypes = [_P, _P, _Z, _Z, _Z, _Z, _Z, _P]
+        f.restype = None
+        f = getattr(L, "oracle_force_open" + sfx)
+        f.argtypes = [_P, _P, _P, _P, _P, _Z, R, R, R, R, _Z, _Z, _Z, _Z, C.c_int, C.c_int]
+        f.restype = None
+        f = getattr(L, "oracle_soften_border" + sfx)
+        f.argtypes = [_P, _P, C.c_int]
+        f.restype = None
         f = getattr(L, "oracle_mlp_forward" + sfx)
         f.argtypes = [_P, _P, _Z, C.c_int, _P, _P, _P, _P, _P]
         f.restype = None
@@ -130,6 +139,42 @@ def step_sheet(h, u, v, bed, nu, gate, res, cell, dt, drive=("gravity", 0.0), *,
         k = min(nb.value, 8)
         return nsub, [tuple(int(x) for x in boxes[4 * i:4 * i + 4]) for i in range(k)]
     return nsub
+
+
+def surface_pressure(h, bed, res, box, precision=32):
+    """The substep's pressure plane d = 0.45 (0.62 G5(eta) + 0.38 S5(eta)) +
+    0.55 eta on box (i0, i1, j0, j1) (src/fluid_core.rs:1903-1935); zero
+    outside the box."""
+    dtype = _dt(precision)
+    n = res * res
+    _check(h, dtype, n, "h")
+    _check(bed, dtype, n, "bed")
+    d = np.zeros(n, dtype)
+    getattr(lib(), "oracle_surface_pressure" + _sfx(precision))(_ptr(h), _ptr(bed), res, *box, _ptr(d))
+    return d
+
+
+def force_open(h, u, v, nu, p, res, cell, dt, vorticity, box, eps=1.0e-6, inner_par=True, confine="reference",
+               precision=32):
+    """Gravity kick from pressure plane p and the confinement sweep on the wet
+    cells of box (force_gravity_open and the Jacobi sweeps,
+    src/fluid_core.rs:2204-2595), in place on u, v."""
+    dtype = _dt(precision)
+    n = res * res
+    for a, nm in ((h, "h"), (u, "u"), (v, "v"), (nu, "nu"), (p, "p")):
+        _check(a, dtype, n, nm)
+    getattr(lib(), "oracle_force_open" + _sfx(precision))(_ptr(h), _ptr(u), _ptr(v), _ptr(nu), _ptr(p), res, cell,
+                                                          dt, vorticity, eps, *box, int(inner_par),
+                                                          CONFINE[confine])
+
+
+def soften_border(u, v, res, precision=32):
+    """soften_border (src/fluid_core.rs:9594-9620) with no open reach, in
+    place."""
+    dtype = _dt(precision)
+    _check(u, dtype, res * res, "u")
+    _check(v, dtype, res * res, "v")
+    getattr(lib(), "oracle_soften_border" + _sfx(precision))(_ptr(u), _ptr(v), res)
 
 
 def transport(h, vel, gate, res, cell, dt, i0, i1, j0, j1, along_x, bed=None, frac_cap=0.45,
@@ -222,6 +267,10 @@ def _declare_core(L):
         g("oracle_core_fine_on").restype = C.c_int
         g("oracle_floor_scale").argtypes = [R, _P]
         g("oracle_floor_scale").restype = R
+        g("oracle_core_push_on").argtypes = [_P, _P, C.c_int]
+        g("oracle_core_push_on").restype = None
+        g("oracle_core_peel").argtypes = [_P, C.c_int, C.POINTER(C.c_int)]
+        g("oracle_core_peel").restype = R
 
 
 class Core:
@@ -297,6 +346,19 @@ class Core:
     def push_tire(self, x, z, forward_x, forward_z, radius, width, speed, slip):
         p = np.array([x, z, forward_x, forward_z, radius, width, speed, slip], self.dtype)
         self._f("oracle_core_push_tire")(self._c, _ptr(p))
+
+    def push_on(self, x, z, forward_x, forward_z, radius, width, speed, slip=0.0, fine=False):
+        """push_on (src/fluid_core.rs:8169-8239) of one push on the coarse
+        (or fine) planes, nothing else."""
+        p = np.array([x, z, forward_x, forward_z, radius, width, speed, slip], self.dtype)
+        self._f("oracle_core_push_on")(self._c, _ptr(p), int(fine))
+
+    def peel(self, fine=False):
+        """peel_spray (src/fluid_core.rs:8241-8314) of the pushes given with
+        push_tire: (volume peeled, event count)."""
+        ev = C.c_int(0)
+        vol = self._f("oracle_core_peel")(self._c, int(fine), C.byref(ev))
+        return float(vol), ev.value
 
     def step(self, dt):
         vol = np.zeros(1, self.dtype)

@@ -858,13 +858,15 @@ static void transport_rows(real *h, const real *vel, const uint8_t *gate, size_t
                                   frac_cap, sc->removed, sc->added);
 }
 
-static void substep(real *h, real *u, real *v, const real *bed, const real *nu, const uint8_t *gate,
-                    size_t res, real cell, real dt, Drive drive, real eps, Box b, int banks,
-                    int do_flux, int inner_par, int confine_mode, Scratch *sc) {
+/*
+ * The substep's surface and pressure planes on box b (:1903-1935): a = eta =
+ * bed + h on the box and a 2-cell rim, b = G5(eta), c = S5(eta),
+ * d = 0.45 (0.62 b + 0.38 c) + 0.55 eta.  Each box starts from cleared filter
+ * planes on its rim (see the scratch note at the top of this file).
+ */
+static void surface_pressure(const real *h, const real *bed, size_t res, Box b, int wind_open, Scratch *sc) {
     size_t hi0 = b.i0 >= 2 ? b.i0 - 2 : 0, hi1 = (b.i1 + 2 < res - 1) ? b.i1 + 2 : res - 1;
     size_t hj0 = b.j0 >= 2 ? b.j0 - 2 : 0, hj1 = (b.j1 + 2 < res - 1) ? b.j1 + 2 : res - 1;
-    /* Each box starts from cleared filter planes on its 2-cell rim (see the
-     * scratch note at the top of this file). */
     for (size_t j = hj0; j <= hj1; j++)
         for (size_t i = hi0; i <= hi1; i++) {
             size_t k = j * res + i;
@@ -872,7 +874,6 @@ static void substep(real *h, real *u, real *v, const real *bed, const real *nu, 
         }
     for (size_t j = hj0; j <= hj1; j++)
         for (size_t i = hi0; i <= hi1; i++) sc->a[j * res + i] = bed[j * res + i] + h[j * res + i];
-    int wind_open = drive.wind;
     if (!wind_open) {
         blur_box(sc->a, sc->b, sc->e, res, b, GAUSS_F);
         blur_box(sc->a, sc->c, sc->e, res, b, STEEP_F);
@@ -883,6 +884,12 @@ static void substep(real *h, real *u, real *v, const real *bed, const real *nu, 
                 sc->d[idx] = K(0.45) * filtered + K(0.55) * sc->a[idx];
             }
     }
+}
+
+static void substep(real *h, real *u, real *v, const real *bed, const real *nu, const uint8_t *gate,
+                    size_t res, real cell, real dt, Drive drive, real eps, Box b, int banks,
+                    int do_flux, int inner_par, int confine_mode, Scratch *sc) {
+    surface_pressure(h, bed, res, b, drive.wind, sc);
     force_open(h, u, v, nu, sc->d, res, cell, dt, drive, eps, b, inner_par, confine_mode, sc);
     int project = !drive.wind && cell > FINE_CELL_M + K(1.0e-6);
     if (project) {
@@ -1033,6 +1040,43 @@ int F(oracle_step_sheet_ex)(real *h, real *u, real *v, const real *bed, const re
     }
     free(pool);
     return (int)nsub;
+}
+
+/*
+ * Test entry points: one part of the substep on its own, through the same
+ * functions the step calls.  oracle_surface_pressure writes the pressure
+ * plane d of box (i0..i1, j0..j1) into d_out (box cells only);
+ * oracle_force_open runs the gravity kick from the given pressure plane and
+ * the confinement sweep on the wet cells of the box (confine_mode as in
+ * oracle_step_sheet_ex, inner_par: row splits allowed).
+ */
+static real *scratch_pool(Scratch *sc, size_t n) {
+    real *pool = calloc(9 * n, sizeof(real));
+    sc->a = pool; sc->b = pool + n; sc->c = pool + 2 * n; sc->d = pool + 3 * n; sc->e = pool + 4 * n;
+    sc->removed = pool + 5 * n; sc->added = pool + 6 * n; sc->su = pool + 7 * n; sc->sv = pool + 8 * n;
+    return pool;
+}
+
+void F(oracle_surface_pressure)(const real *h, const real *bed, size_t res, size_t i0, size_t i1, size_t j0,
+                                size_t j1, real *d_out) {
+    Scratch sc;
+    real *pool = scratch_pool(&sc, res * res);
+    Box b = {i0, i1, j0, j1};
+    surface_pressure(h, bed, res, b, 0, &sc);
+    for (size_t j = j0; j <= j1; j++)
+        for (size_t i = i0; i <= i1; i++) d_out[j * res + i] = sc.d[j * res + i];
+    free(pool);
+}
+
+void F(oracle_force_open)(const real *h, real *u, real *v, const real *nu, const real *p, size_t res, real cell,
+                          real dt, real vorticity, real eps, size_t i0, size_t i1, size_t j0, size_t j1,
+                          int inner_par, int confine_mode) {
+    Scratch sc;
+    real *pool = scratch_pool(&sc, res * res);
+    Box b = {i0, i1, j0, j1};
+    Drive drive = {0, K(0.0), K(0.0), vorticity};
+    force_open(h, u, v, nu, p, res, cell, dt, drive, eps, b, inner_par, confine_mode, &sc);
+    free(pool);
 }
 
 /* step_sheet, src/fluid_core.rs:770-785: eps 1e-6, no banks, cap 8, pace 0. */
