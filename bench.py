@@ -11,9 +11,14 @@ Prints one JSON line (the driver's contract).  ``value`` is cell-steps/s
 (one cell-step = both layers of one cell advanced one step).
 """
 import argparse
+import csv
+import glob
 import json
 import os
+import shutil
+import subprocess
 import sys
+import tempfile
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -27,32 +32,130 @@ HBM_PEAK_GBS = 8000.0
 # water h, u, v (12), slurry h, u, v (12); write both layers' h, u, v (24).
 # The slurry viscosity is uniform in this workload, so no viscosity plane.
 ALG_BYTES_PER_CELL = 52
-# HBM bytes per launch of the fused kernel measured with rocprofv3 --pmc
-# FETCH_SIZE and WRITE_SIZE on this workload (default rows per workgroup),
-# each scaled by the counter's reading for a 1 GiB copy (FETCH x 2, WRITE x 1);
-# profiles/k_sheet_stream_4096_two_layer.md holds the runs and commands.
-PMC_TRAFFIC_BYTES = {("fast", 4096): 1_325_663_385}
+# Bytes of the copy whose counter readings scale the kernel's (HBM traffic).
+CALIB_BYTES = 256 << 20
 
 
-def cpu_baseline(sc, dt, steps):
-    """The FP32 oracle (one core, the reference's arithmetic restated in C) on
-    ``steps`` full steps of the same workload from the same initial state."""
+def host_cpu():
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"model": model, "logical_cpus": os.cpu_count()}
+
+
+def cpu_baseline(sc, dt, steps, steps_single):
+    """The FP32 oracle (the reference's arithmetic restated in C) on full
+    steps of the same workload from the same initial state, timed on this
+    host's cores in the reference's threading shape: water and slurry as two
+    tasks (step_pair's rayon::join, src/fluid_core.rs:8127-8131), each pass of
+    a layer in row stripes of at most 4 threads (:2154 and the other
+    clamp(1, 4) sites), AVX2 width (liboracle_par.so, the serial oracle's
+    values bit for bit).  The same oracle on one thread is reported beside it."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as O
 
     res, cell = sc["res"], sc["cell"]
     n = res * res
     gate = np.ones(n, np.uint8)
-    layers = [(sc["water"].copy(), np.full(n, sc["nu_water"], np.float32), 0.08),
-              (sc["slurry"].copy(), np.full(n, sc["nu_slurry"], np.float32), 0.05)]
-    state = [(h, np.zeros(n, np.float32), np.zeros(n, np.float32), nu, vort) for h, nu, vort in layers]
+    nu_w = np.full(n, sc["nu_water"], np.float32)
+    nu_s = np.full(n, sc["nu_slurry"], np.float32)
+
+    def fresh():
+        return ([sc["water"].copy(), np.zeros(n, np.float32), np.zeros(n, np.float32)],
+                [sc["slurry"].copy(), np.zeros(n, np.float32), np.zeros(n, np.float32)])
+
+    w, s = fresh()
+    O.step_pair_par(w, s, sc["bed"], nu_w, nu_s, gate, res, cell, dt)  # first touch of the scratch
     t0 = time.perf_counter()
     for _ in range(steps):
-        for h, u, v, nu, vort in state:
-            O.step_sheet(h, u, v, sc["bed"], nu, gate, res, cell, dt, ("gravity", vort))
-    el = time.perf_counter() - t0
-    return {"value": n * steps / el, "unit": "cell-steps/s", "cores": 1, "kind": "port",
-            "sample": f"{steps} full steps of the {res}^2 two-layer workload, FP32 oracle, 1 thread, {el:.1f} s"}
+        O.step_pair_par(w, s, sc["bed"], nu_w, nu_s, gate, res, cell, dt)
+    el_par = time.perf_counter() - t0
+    w, s = fresh()
+    t0 = time.perf_counter()
+    for _ in range(steps_single):
+        O.step_sheet(*w, sc["bed"], nu_w, gate, res, cell, dt, ("gravity", 0.08))
+        O.step_sheet(*s, sc["bed"], nu_s, gate, res, cell, dt, ("gravity", 0.05))
+    el_one = time.perf_counter() - t0
+    return {"value": n * steps / el_par, "unit": "cell-steps/s", "cores": 8, "kind": "port",
+            "sample": f"{steps} full steps of the {res}^2 two-layer workload in {el_par:.1f} s: FP32 oracle at "
+                      f"AVX2 width, water || slurry, each pass in <= 4 row stripes (8 threads)",
+            "single_thread": {"value": n * steps_single / el_one, "cores": 1,
+                              "sample": f"{steps_single} steps in {el_one:.1f} s"},
+            "host": host_cpu()}
+
+
+def traffic_probe(args):
+    """Run under rocprofv3 --pmc by pmc_traffic(): a few launches of the
+    workload, then a device copy of CALIB_BYTES whose reading scales the
+    counter."""
+    import ctypes
+
+    import silt_amd as S
+    from silt_amd import scenes
+
+    sc = scenes.large_scene(res=args.res)
+    n = sc["res"] ** 2
+    z = np.zeros(n, np.float32)
+    pair = S.SheetPair(sc["res"], sc["cell"], path=args.path)
+    pair.upload(sc["bed"], sc["water"], z, z, sc["slurry"], z, z, sc["nu_slurry"])
+    for _ in range(4):
+        pair.step(1.0 / 120.0)
+    pair.last_kernel_ms()
+    hip = ctypes.CDLL("libamdhip64.so.7")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(CALIB_BYTES)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(CALIB_BYTES)) == 0
+    assert hip.hipMemset(a, 0, ctypes.c_size_t(CALIB_BYTES)) == 0
+    assert hip.hipMemcpy(b, a, ctypes.c_size_t(CALIB_BYTES), 3) == 0  # device to device
+    assert hip.hipDeviceSynchronize() == 0
+    pair.close()
+
+
+def pmc_traffic(args):
+    """HBM bytes per launch of the fused kernel in this run: two child runs of
+    this script under rocprofv3, FETCH_SIZE then WRITE_SIZE (KiB per
+    dispatch), each scaled by what the counter reads for a device copy of
+    known size.  None (with the reason) when the profiler is not available."""
+    if shutil.which("rocprofv3") is None:
+        return None, "rocprofv3 not found"
+    out = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        d = tempfile.mkdtemp(prefix="pmc_", dir="/tmp")
+        cmd = ["timeout", "-s", "KILL", "180", "rocprofv3", "--output-format", "csv", "--pmc", counter, "-d", d,
+               "-o", "pmc", "--", sys.executable, os.path.abspath(__file__), "--traffic-probe", "--res",
+               str(args.res), "--path", args.path]
+        print(f"[bench] traffic: {counter} pass", file=sys.stderr, flush=True)
+        r = subprocess.run(cmd, capture_output=True, text=True, cwd="/tmp", env=dict(os.environ, TMPDIR="/tmp"))
+        files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
+        if r.returncode != 0 or not files:
+            shutil.rmtree(d, ignore_errors=True)
+            return None, f"rocprofv3 {counter} pass failed (rc {r.returncode})"
+        kern, copy = [], []
+        with open(files[0]) as f:
+            for row in csv.DictReader(f):
+                if row.get("Counter_Name") != counter:
+                    continue
+                name, val = row.get("Kernel_Name", ""), float(row["Counter_Value"])
+                if "k_sheet_stream" in name:
+                    kern.append(val)
+                elif "copyBuffer" in name:
+                    copy.append(val)
+        shutil.rmtree(d, ignore_errors=True)
+        if not kern or not copy or max(copy) <= 0:
+            return None, f"{counter}: kernel or copy dispatch missing"
+        factor = CALIB_BYTES / (max(copy) * 1024.0)
+        out[counter] = (float(np.mean(kern)) * 1024.0 * factor, factor)
+    traffic = out["FETCH_SIZE"][0] + out["WRITE_SIZE"][0]
+    return traffic, {"fetch_bytes": out["FETCH_SIZE"][0], "write_bytes": out["WRITE_SIZE"][0],
+                     "fetch_factor": out["FETCH_SIZE"][1], "write_factor": out["WRITE_SIZE"][1],
+                     "source": f"rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE in this run, each scaled by the reading "
+                               f"for a {CALIB_BYTES >> 20} MiB device copy"}
 
 
 def batch_cpu_baseline(ws, dt, windows):
@@ -429,8 +532,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--res", type=int, default=4096)
     ap.add_argument("--path", default="fast", choices=["fast", "fused_exact", "exact"])
-    ap.add_argument("--cpu-steps", type=int, default=7, help="pair: oracle steps of the CPU baseline (about 10 s)")
+    ap.add_argument("--cpu-steps", type=int, default=24,
+                    help="pair: oracle steps of the CPU baseline in the reference's threading shape (about 10 s)")
+    ap.add_argument("--cpu-steps-single", type=int, default=3, help="pair: oracle steps on one thread")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-pmc", action="store_true", help="pair: skip the two rocprofv3 passes that measure traffic")
+    ap.add_argument("--traffic-probe", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--core-hz", type=int, default=120, help="core: steps per second of simulated time (dt = 1/hz)")
     ap.add_argument("--workload", default="pair", choices=["pair", "batch", "surrogate", "large", "core"],
                     help="pair: the 4096^2 two-layer sheet (default); batch: 64 uneven windows; surrogate: the "
@@ -445,6 +552,9 @@ def main():
                     help="time candidate rows per workgroup first and keep the fastest (default: the "
                          "occupancy-derived height, so a kernel trace of the run holds one launch shape)")
     args = ap.parse_args()
+    if args.traffic_probe:
+        traffic_probe(args)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported on this build "
               "(the sheet's multi-GPU halo exchange is a design, not code)", file=sys.stderr)
@@ -519,19 +629,25 @@ def main():
                    "chunk_rows": chunk or "default",
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None if args.autotune else PMC_TRAFFIC_BYTES.get((args.path, res)),
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "k_sheet_stream (both layers, one launch)", "kernel_ms": kms,
                      "alg_bytes_per_cell": ALG_BYTES_PER_CELL},
         "ledger": {"water_gap_rel": led.water.gap_m3 / max(led.water.initial_m3, 1e-30),
                    "slurry_gap_rel": led.slurry.gap_m3 / max(led.slurry.initial_m3, 1e-30),
+                   "water_cut_m3": led.water.cut_m3, "water_residue_m3": led.water.residue_m3,
+                   "slurry_cut_m3": led.slurry.cut_m3, "slurry_residue_m3": led.slurry.residue_m3,
                    "total_m3": led.total_m3},
         "host_plan_steps": fallbacks,
         "speculation": {"held": spec_hits, "undone": spec_misses},
     }
+    if not args.no_pmc and not args.autotune:
+        traffic, note = pmc_traffic(args)
+        out["roofline"]["traffic"] = traffic
+        out["roofline"]["traffic_detail"] = note
     if not args.no_cpu:
-        print(f"[bench] CPU baseline, {args.cpu_steps} oracle steps", file=sys.stderr, flush=True)
-        out["cpu_baseline"] = cpu_baseline(sc, dt, args.cpu_steps)
+        print(f"[bench] CPU baseline, {args.cpu_steps} + {args.cpu_steps_single} oracle steps", file=sys.stderr,
+              flush=True)
+        out["cpu_baseline"] = cpu_baseline(sc, dt, args.cpu_steps, args.cpu_steps_single)
         out["gpu_over_cpu"] = out["value"] / out["cpu_baseline"]["value"]
     print(json.dumps(out), flush=True)
 

@@ -19,11 +19,13 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
+PAR_LIB_PATH = os.path.join(HERE, "liboracle_par.so")
 
 CONFINE = {"reference": 0, "jacobi": 1, "gauss_seidel": 2}
 GRAIN = {"clay": 0, "silt": 1, "sand": 2, "water": 3, "other": 4}
 
 _lib = None
+_lib_par = None
 
 
 def build() -> None:
@@ -43,6 +45,36 @@ def lib():
 
 _P = C.c_void_p
 _Z = C.c_size_t
+
+
+def lib_par():
+    """The CPU baseline build (oracle/liboracle_par.so): the FP32 step at AVX2
+    width in row stripes of at most 4 threads, water || slurry."""
+    global _lib_par
+    if _lib_par is None:
+        if not os.path.exists(PAR_LIB_PATH):
+            build()
+        _lib_par = C.CDLL(PAR_LIB_PATH)
+        f = _lib_par.oracle_step_pair_par_f32
+        R = C.c_float
+        f.argtypes = [_P, _P, _P, _P, R, _P, _P, _P, _P, R, _P, _P, _Z, R, R]
+        f.restype = C.c_int
+    return _lib_par
+
+
+def step_pair_par(water, slurry, bed, nu_w, nu_s, gate, res, cell, dt, vort_w=0.08, vort_s=0.05):
+    """One two-layer step on the CPU baseline build: water = (h, u, v) and
+    slurry = (h, u, v) FP32 planes in place, each a step_sheet with its own
+    viscosity plane and vorticity, the two layers on two threads."""
+    n = res * res
+    for a, nm in ((bed, "bed"), (nu_w, "nu_w"), (nu_s, "nu_s")):
+        _check(a, np.float32, n, nm)
+    for a in tuple(water) + tuple(slurry):
+        _check(a, np.float32, n, "plane")
+    _check(gate, np.uint8, n, "gate")
+    return lib_par().oracle_step_pair_par_f32(*(_ptr(a) for a in water), _ptr(nu_w), vort_w,
+                                              *(_ptr(a) for a in slurry), _ptr(nu_s), vort_s, _ptr(bed),
+                                              _ptr(gate), res, cell, dt)
 
 
 def _declare(L):

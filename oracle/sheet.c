@@ -41,6 +41,22 @@ typedef REAL real;
 /* Reference constants are f32; the FP64 build widens the f32 value. */
 #define K(x) ((real)(float)(x))
 
+/*
+ * ORACLE_THREADS: the CPU baseline build (Makefile target liboracle_par.so,
+ * -O3 -mavx2 -fopenmp, contraction still off).  Every pass that is
+ * independent row by row runs in row stripes of at most ORACLE_STRIPES
+ * threads, as the reference clamps its rayon stripes (:2154, :2287, :3236,
+ * :3652, :4008, clamp(1, 4)), on boxes of at least 48 rows; the results are
+ * the serial build's bit for bit.
+ */
+#ifdef ORACLE_THREADS
+#include <omp.h>
+#define ORACLE_STRIPES 4
+#define ROWS_PAR(b) _Pragma("omp parallel for schedule(static) num_threads(ORACLE_STRIPES) if((b).j1 - (b).j0 >= 47)")
+#else
+#define ROWS_PAR(b)
+#endif
+
 #if defined(REAL_IS_DOUBLE)
 #define R_EXP exp
 #define R_POW pow
@@ -567,6 +583,7 @@ static inline real accum5(real a0, real a1, real a2, real a3, real a4, const flo
  */
 static void blur_box(const real *src, real *dst, real *tmp, size_t res, Box b, const float *k) {
     if (b.i0 > b.i1 || b.j0 > b.j1 || res == 0) return;
+    ROWS_PAR(b)
     for (size_t j = b.j0; j <= b.j1; j++) {
         const real *row = src + j * res;
         for (size_t i = b.i0; i <= b.i1; i++) {
@@ -580,6 +597,7 @@ static void blur_box(const real *src, real *dst, real *tmp, size_t res, Box b, c
             tmp[j * res + i] = a;
         }
     }
+    ROWS_PAR(b)
     for (size_t j = b.j0; j <= b.j1; j++) {
         for (size_t i = b.i0; i <= b.i1; i++) {
             real a;
@@ -618,6 +636,7 @@ static inline void flux_open(const real *h, const uint8_t *gate, size_t a, size_
 }
 
 static void apply_moves(real *h, const real *removed, const real *added, size_t res, Box b) {
+    ROWS_PAR(b)
     for (size_t j = b.j0; j <= b.j1; j++)
         for (size_t i = b.i0; i <= b.i1; i++) {
             size_t idx = j * res + i;
@@ -685,14 +704,91 @@ static inline real open_edge_dh(real hd, real hr, real remd, uint8_t gate_r, int
  * left-going fluxes first, then right-going, then each cell sums its two
  * edges in the serial order.  Equal to the serial loop bit for bit.
  */
+#ifdef ORACLE_THREADS
+/*
+ * The z pass of transport_open_rows in row stripes.  The serial loop walks
+ * the faces (j, j+1) in order: the up flux of a face (donor j+1) needs no
+ * room, the down flux (donor j) has room h_j less the up flux j already gave
+ * through face (j-1, j), and each row's removed and added sums collect the
+ * two faces' fluxes in that order.  Every face's up flux first, then every
+ * down flux, then each row's sums in the serial order: the same values.
+ */
+static void transport_z_rows_par(real *h, const real *vel, const uint8_t *gate, size_t res, real cell, real dt,
+                                 Box b, const real *bed, real frac_cap, real *removed, real *added) {
+    size_t m = b.i1 + 1 - b.i0, nf = b.j1 - b.j0;
+    int ub = bed != NULL;
+    static _Thread_local real *up_down = NULL;
+    static _Thread_local size_t cap = 0;
+    if (cap < 2 * nf * m) {
+        free(up_down);
+        up_down = malloc(2 * nf * m * sizeof(real));
+        cap = 2 * nf * m;
+    }
+    real *up = up_down, *down = up_down + nf * m;
+    ROWS_PAR(b)
+    for (size_t f = 0; f < nf; f++) {
+        size_t lo = (b.j0 + f) * res + b.i0;
+        for (size_t i = 0; i < m; i++) {
+            size_t a = lo + i, c = lo + res + i;
+            real speed = K(0.5) * (vel[a] + vel[c]);
+            real dh = open_edge_dh(h[c], h[a], K(0.0), gate[a], ub, ub ? bed[c] : 0, ub ? bed[a] : 0, speed, dt, cell,
+                                   frac_cap);
+            up[f * m + i] = speed > K(0.0) ? K(0.0) : dh;
+        }
+    }
+    ROWS_PAR(b)
+    for (size_t f = 0; f < nf; f++) {
+        size_t lo = (b.j0 + f) * res + b.i0;
+        for (size_t i = 0; i < m; i++) {
+            size_t a = lo + i, c = lo + res + i;
+            real speed = K(0.5) * (vel[a] + vel[c]);
+            real remd = f > 0 ? K(0.0) + up[(f - 1) * m + i] : K(0.0);
+            real dh = open_edge_dh(h[a], h[c], remd, gate[c], ub, ub ? bed[a] : 0, ub ? bed[c] : 0, speed, dt, cell,
+                                   frac_cap);
+            down[f * m + i] = speed > K(0.0) ? dh : K(0.0);
+        }
+    }
+    ROWS_PAR(b)
+    for (size_t j = b.j0; j <= b.j1; j++) {
+        size_t f = j - b.j0, lo = j * res + b.i0;
+        for (size_t i = 0; i < m; i++) {
+            real r = f > 0 ? K(0.0) + up[(f - 1) * m + i] : K(0.0);
+            real a = f > 0 ? K(0.0) + down[(f - 1) * m + i] : K(0.0);
+            if (j < b.j1) {
+                r = r + down[f * m + i];
+                a = a + up[f * m + i];
+            }
+            removed[lo + i] = r;
+            added[lo + i] = a;
+        }
+    }
+    apply_moves(h, removed, added, res, b);
+}
+#endif
+
 void F(oracle_transport_open_rows)(real *h, const real *vel, const uint8_t *gate, size_t res,
                                    real cell, real dt, size_t i0, size_t i1, size_t j0, size_t j1,
                                    int along_x, const real *bed, real frac_cap, real *removed,
                                    real *added) {
     size_t m = i1 + 1 - i0;
     int ub = bed != NULL;
+#ifdef ORACLE_THREADS
+    Box bt = {i0, i1, j0, j1};
+    (void)bt;
+    if (!along_x) {
+        transport_z_rows_par(h, vel, gate, res, cell, dt, bt, bed, frac_cap, removed, added);
+        return;
+    }
+#endif
     if (along_x) {
+#ifdef ORACLE_THREADS
+#pragma omp parallel num_threads(ORACLE_STRIPES) if (j1 - j0 >= 47)
+        {
+#endif
         real *fl = calloc(m, sizeof *fl), *fr = calloc(m, sizeof *fr);
+#ifdef ORACLE_THREADS
+#pragma omp for schedule(static)
+#endif
         for (size_t j = j0; j <= j1; j++) {
             size_t lo = j * res + i0;
             real *hr = h + lo;
@@ -729,6 +825,9 @@ void F(oracle_transport_open_rows)(real *h, const real *vel, const uint8_t *gate
         }
         free(fl);
         free(fr);
+#ifdef ORACLE_THREADS
+        }
+#endif
         return;
     }
     for (size_t j = j0; j <= j1; j++)
@@ -822,6 +921,7 @@ static void force_open(const real *h, real *u, real *v, const real *nu, const re
             }
         return;
     }
+    ROWS_PAR(b)
     for (size_t j = b.j0; j <= b.j1; j++)
         for (size_t i = b.i0; i <= b.i1; i++) {
             size_t idx = j * res + i;
@@ -845,6 +945,7 @@ static void force_open(const real *h, real *u, real *v, const real *nu, const re
     if (drive.vorticity <= K(0.0)) return;
     memcpy(sc->su, u, res * res * sizeof(real));
     memcpy(sc->sv, v, res * res * sizeof(real));
+    ROWS_PAR(b)
     for (size_t j = b.j0; j <= b.j1; j++)
         for (size_t i = b.i0; i <= b.i1; i++) {
             if (h[j * res + i] <= eps) continue;
@@ -867,16 +968,17 @@ static void transport_rows(real *h, const real *vel, const uint8_t *gate, size_t
 static void surface_pressure(const real *h, const real *bed, size_t res, Box b, int wind_open, Scratch *sc) {
     size_t hi0 = b.i0 >= 2 ? b.i0 - 2 : 0, hi1 = (b.i1 + 2 < res - 1) ? b.i1 + 2 : res - 1;
     size_t hj0 = b.j0 >= 2 ? b.j0 - 2 : 0, hj1 = (b.j1 + 2 < res - 1) ? b.j1 + 2 : res - 1;
+    ROWS_PAR(b)
     for (size_t j = hj0; j <= hj1; j++)
         for (size_t i = hi0; i <= hi1; i++) {
             size_t k = j * res + i;
             sc->b[k] = sc->c[k] = sc->d[k] = sc->e[k] = K(0.0);
+            sc->a[k] = bed[k] + h[k];
         }
-    for (size_t j = hj0; j <= hj1; j++)
-        for (size_t i = hi0; i <= hi1; i++) sc->a[j * res + i] = bed[j * res + i] + h[j * res + i];
     if (!wind_open) {
         blur_box(sc->a, sc->b, sc->e, res, b, GAUSS_F);
         blur_box(sc->a, sc->c, sc->e, res, b, STEEP_F);
+        ROWS_PAR(b)
         for (size_t j = b.j0; j <= b.j1; j++)
             for (size_t i = b.i0; i <= b.i1; i++) {
                 size_t idx = j * res + i;
@@ -893,6 +995,7 @@ static void substep(real *h, real *u, real *v, const real *bed, const real *nu, 
     force_open(h, u, v, nu, sc->d, res, cell, dt, drive, eps, b, inner_par, confine_mode, sc);
     int project = !drive.wind && cell > FINE_CELL_M + K(1.0e-6);
     if (project) {
+        ROWS_PAR(b)
         for (size_t j = b.j0; j <= b.j1; j++)
             for (size_t i = b.i0; i <= b.i1; i++) {
                 size_t idx = j * res + i;
@@ -907,17 +1010,20 @@ static void substep(real *h, real *u, real *v, const real *bed, const real *nu, 
             }
         blur_box(sc->b, sc->c, sc->e, res, b, GAUSS_F);
         blur_box(sc->b, sc->a, sc->e, res, b, STEEP_F);
+        ROWS_PAR(b)
         for (size_t j = b.j0; j <= b.j1; j++)
             for (size_t i = b.i0; i <= b.i1; i++) {
                 size_t idx = j * res + i;
                 sc->d[idx] = K(0.62) * sc->c[idx] + K(0.38) * sc->a[idx];
             }
     }
-    /* Damping, memoised per viscosity bit pattern: :2014-2042 */
-    int memo_set = 0;
-    real memo_nu = K(0.0), memo_decay = K(1.0);
+    /* Damping, memoised per viscosity bit pattern: :2014-2042 (the memo is a
+     * cache of a pure function, so one per row gives the same values). */
     real lim = drive.wind ? R_FMAX(R_HYPOT(drive.wx, drive.wz), FILM_SPEED_CAP) : FILM_SPEED_CAP;
-    for (size_t j = b.j0; j <= b.j1; j++)
+    ROWS_PAR(b)
+    for (size_t j = b.j0; j <= b.j1; j++) {
+        int memo_set = 0;
+        real memo_nu = K(0.0), memo_decay = K(1.0);
         for (size_t i = b.i0; i <= b.i1; i++) {
             size_t idx = j * res + i;
             if (h[idx] <= eps) continue;
@@ -936,10 +1042,12 @@ static void substep(real *h, real *u, real *v, const real *bed, const real *nu, 
             u[idx] = finite_speed(u[idx] * memo_decay, lim);
             v[idx] = finite_speed(v[idx] * memo_decay, lim);
         }
+    }
     if (!do_flux) return;
     const real *open_bed = banks ? bed : NULL;
     transport_rows(h, u, gate, res, cell, dt, b, 1, open_bed, K(0.45), sc);
     transport_rows(h, v, gate, res, cell, dt, b, 0, open_bed, K(0.45), sc);
+    ROWS_PAR(b)
     for (size_t j = b.j0; j <= b.j1; j++)
         for (size_t i = b.i0; i <= b.i1; i++) {
             size_t idx = j * res + i;
@@ -1019,8 +1127,17 @@ int F(oracle_step_sheet_ex)(real *h, real *u, real *v, const real *bed, const re
         nsub = (uint32_t)clampr(q, K(1.0), cap);
     }
     real dts = dt / (real)nsub;
+    /* Nine scratch planes, kept per thread between calls as the reference
+     * keeps its thread-local SCRATCH (:581-589): every plane a substep reads
+     * is cleared or written first on the cells it reads. */
+    static _Thread_local real *pool = NULL;
+    static _Thread_local size_t pool_n = 0;
+    if (pool_n < n) {
+        free(pool);
+        pool = calloc(9 * n, sizeof(real));
+        pool_n = n;
+    }
     Scratch sc;
-    real *pool = calloc(9 * n, sizeof(real));
     sc.a = pool; sc.b = pool + n; sc.c = pool + 2 * n; sc.d = pool + 3 * n; sc.e = pool + 4 * n;
     sc.removed = pool + 5 * n; sc.added = pool + 6 * n; sc.su = pool + 7 * n; sc.sv = pool + 8 * n;
     if (pace == 3 && whole_wet && nsub > 1) {
@@ -1038,9 +1155,34 @@ int F(oracle_step_sheet_ex)(real *h, real *u, real *v, const real *bed, const re
                         confine_mode, &sc);
             }
     }
-    free(pool);
     return (int)nsub;
 }
+
+#ifdef ORACLE_THREADS
+int F(oracle_step_sheet)(real *h, real *u, real *v, const real *bed, const real *nu, const uint8_t *gate,
+                         size_t res, real cell, real dt, int drive_wind, real wind_x, real wind_z, real vorticity);
+
+/*
+ * The CPU baseline of one two-layer step: water and slurry as two tasks (the
+ * reference's rayon::join in step_pair, :8127-8131), each a step_sheet
+ * (:770-785) whose passes run in row stripes.  Returns the larger substep
+ * count.
+ */
+int F(oracle_step_pair_par)(real *hw, real *uw, real *vw, const real *nu_w, real vort_w, real *hs, real *us,
+                            real *vs, const real *nu_s, real vort_s, const real *bed, const uint8_t *gate, size_t res,
+                            real cell, real dt) {
+    int n0 = 0, n1 = 0;
+    omp_set_max_active_levels(2);
+#pragma omp parallel sections num_threads(2)
+    {
+#pragma omp section
+        n0 = F(oracle_step_sheet)(hw, uw, vw, bed, nu_w, gate, res, cell, dt, 0, K(0.0), K(0.0), vort_w);
+#pragma omp section
+        n1 = F(oracle_step_sheet)(hs, us, vs, bed, nu_s, gate, res, cell, dt, 0, K(0.0), K(0.0), vort_s);
+    }
+    return n0 > n1 ? n0 : n1;
+}
+#endif
 
 /*
  * Test entry points: one part of the substep on its own, through the same
