@@ -1044,6 +1044,56 @@ int silt_core_publish_surface(silt_core* c, const float* soil_wetness, const flo
   return SILT_OK;
 }
 
+namespace {
+
+// Orders work on the caller's stream with the handle's: the handle's stream
+// waits for what the caller enqueued so far (device inputs), and (after) the
+// caller's stream waits for the handle's (device outputs).
+int fence_from(silt_core* c, hipStream_t user) {
+  HIPCHK(hipEventRecord(c->join, user));
+  HIPCHK(hipStreamWaitEvent(c->st, c->join, 0));
+  return 0;
+}
+int fence_to(silt_core* c, hipStream_t user) {
+  HIPCHK(hipEventRecord(c->fork, c->st));
+  HIPCHK(hipStreamWaitEvent(user, c->fork, 0));
+  return 0;
+}
+
+}  // namespace
+
+int silt_core_publish_device(silt_core* c, const float* soil_bed, const float* soil_placed, const float* soil_rut,
+                             float* ground_rut4, float* film2, float* flow4, void* hip_stream) {
+  if (!c) return SILT_EINVAL;
+  hipStream_t user = (hipStream_t)hip_stream;
+  if (fence_from(c, user)) return SILT_EDEVICE;
+  const float* dsoil[3] = {soil_bed, soil_placed, soil_rut};
+  if (c->ground && !soil_bed && !soil_placed && !soil_rut) {
+    dsoil[0] = c->soil_bed;
+    dsoil[1] = c->soil_placed;
+    dsoil[2] = c->soil_rut;
+  }
+  // Outputs the caller does not want land in the handle's scratch.
+  float* d_out = c->d_pub;
+  core_publish(c->c, dsoil[0], dsoil[1], dsoil[2], ground_rut4 ? ground_rut4 : d_out,
+               film2 ? film2 : d_out + 4 * WN, flow4 ? flow4 : d_out + 6 * WN, c->st);
+  if (hipGetLastError() != hipSuccess) return SILT_EDEVICE;
+  return fence_to(c, user) ? SILT_EDEVICE : SILT_OK;
+}
+
+int silt_core_publish_surface_device(silt_core* c, const float* soil_wetness, const float* soil_placed,
+                                     float* moisture1, float* wet_edge1, float* foam_crest2, void* hip_stream) {
+  if (!c) return SILT_EINVAL;
+  hipStream_t user = (hipStream_t)hip_stream;
+  if (fence_from(c, user)) return SILT_EDEVICE;
+  const float* d_placed = soil_placed ? soil_placed : (c->ground ? c->soil_placed : nullptr);
+  float* d = c->d_pub;
+  core_publish_surface(c->c, soil_wetness, d_placed, kWindowRes, moisture1 ? moisture1 : d + 2 * WN,
+                       wet_edge1 ? wet_edge1 : d + 3 * WN, foam_crest2 ? foam_crest2 : d + 4 * WN, c->st);
+  if (hipGetLastError() != hipSuccess) return SILT_EDEVICE;
+  return fence_to(c, user) ? SILT_EDEVICE : SILT_OK;
+}
+
 int silt_core_fine_water_debug(silt_core* c, const silt_tire_push* pushes, size_t n, float* water, float* uw, float* vw,
                                float* origin2) {
   if (!c || (!pushes && n) || !water || !origin2 || n > (size_t)kMaxPush) return SILT_EINVAL;

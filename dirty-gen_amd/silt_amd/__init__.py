@@ -636,6 +636,27 @@ class FluidCore:
                                                   _ptr(moist), _ptr(edge), _ptr(foam)))
         return moist, edge, foam
 
+    def publish_device(self, ground_rut4, film2, flow4, soil_bed=0, soil_placed=0, soil_rut=0, stream=0):
+        """silt_core_publish into device buffers given as integer addresses
+        (0 for none); asynchronous on ``stream``."""
+        L = lib()
+        if not getattr(L, "_publish_device_declared", False):
+            L.silt_core_publish_device.argtypes = [_P] * 8
+            L.silt_core_publish_surface_device.argtypes = [_P] * 7
+            L._publish_device_declared = True
+        a = [x or None for x in (soil_bed, soil_placed, soil_rut, ground_rut4, film2, flow4, stream)]
+        _check_status(L.silt_core_publish_device(self._c, *a))
+
+    def publish_surface_device(self, moisture1, wet_edge1, foam_crest2, soil_wetness=0, soil_placed=0, stream=0):
+        """silt_core_publish_surface into device buffers (integer addresses)."""
+        L = lib()
+        if not getattr(L, "_publish_device_declared", False):
+            L.silt_core_publish_device.argtypes = [_P] * 8
+            L.silt_core_publish_surface_device.argtypes = [_P] * 7
+            L._publish_device_declared = True
+        a = [x or None for x in (soil_wetness, soil_placed, moisture1, wet_edge1, foam_crest2, stream)]
+        _check_status(L.silt_core_publish_surface_device(self._c, *a))
+
     def near_tiles(self):
         return int(_check_status(lib().silt_core_near_tiles(self._c)))
 

@@ -127,6 +127,18 @@ int silt_core_publish(silt_core *c, const float *soil_bed, const float *soil_pla
  * comes from silt_core_set_ground's ground, or is 0); any output may be NULL. */
 int silt_core_publish_surface(silt_core *c, const float *soil_wetness, const float *soil_placed, float *moisture1,
                               float *wet_edge1, float *foam_crest2);
+/* silt_core_publish and silt_core_publish_surface into device memory: every
+ * pointer is a device pointer (the renderer's own buffers, which
+ * ContractGpu::upload fills from host copies in the reference,
+ * src/render_contract.rs:1146-1166), nothing goes through the host.
+ * Asynchronous: the maps are written after the handle's pending work and
+ * after what hip_stream holds when the call is made, and work enqueued on
+ * hip_stream afterwards sees them.  Inputs and outputs may be NULL as in the
+ * host calls. */
+int silt_core_publish_device(silt_core *c, const float *soil_bed, const float *soil_placed, const float *soil_rut,
+                             float *ground_rut4, float *film2, float *flow4, void *hip_stream);
+int silt_core_publish_surface_device(silt_core *c, const float *soil_wetness, const float *soil_placed,
+                                     float *moisture1, float *wet_edge1, float *foam_crest2, void *hip_stream);
 
 /* The ground under the sheet on the window grid (SoilField bed and placed
  * soil sampled at the cell centres).  The initial ground is this bed; the

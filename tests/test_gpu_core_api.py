@@ -149,3 +149,41 @@ def test_lay_slurry_on_water_and_window_bounds():
         assert (x0, z0) == (float(o[0]), float(o[1])) and ext == np.float32(40.0)
     finally:
         g.close()
+
+
+def test_device_publish_equals_the_host_publish():
+    """silt_core_publish_device / _surface_device write the same maps as the
+    host calls, straight into device buffers (the renderer's textures in
+    ContractGpu::upload, src/render_contract.rs:1146-1166), with soil inputs
+    taken from device buffers or from the core's own ground."""
+    from hipmem import DeviceArray, synchronize
+    from silt_amd import drive as D
+
+    g = D.nested_scene_core(S, scenes, seed=2)
+    drv = D.Drive(seed=4)
+    try:
+        for _ in range(6):
+            D.drive_step(g, drv, 1.0 / 30.0)
+        n = 200 * 200
+        ground, film, flow = g.publish()
+        outs = [DeviceArray(np.zeros(k * n, np.float32)) for k in (4, 2, 4)]
+        g.publish_device(outs[0].ptr, outs[1].ptr, outs[2].ptr)
+        synchronize()
+        for a, b, name in zip(outs, (ground, film, flow), ("ground_rut", "film", "flow")):
+            assert np.array_equal(a.numpy(), b), name
+        # Soil planes given by the caller, on the device.
+        rng = np.random.default_rng(1)
+        soil = [rng.random(n).astype(np.float32) for _ in range(3)]
+        want = g.publish(*soil)
+        dsoil = [DeviceArray(a) for a in soil]
+        g.publish_device(outs[0].ptr, outs[1].ptr, 0, *(d.ptr for d in dsoil))
+        synchronize()
+        assert np.array_equal(outs[0].numpy(), want[0]) and np.array_equal(outs[1].numpy(), want[1])
+        moist, edge, foam = g.publish_surface(soil_wetness=soil[0])
+        souts = [DeviceArray(np.zeros(k * n, np.float32)) for k in (1, 1, 2)]
+        g.publish_surface_device(souts[0].ptr, souts[1].ptr, souts[2].ptr, soil_wetness=dsoil[0].ptr)
+        synchronize()
+        for a, b, name in zip(souts, (moist, edge, foam), ("moisture", "wet_edge", "foam_crest")):
+            assert np.array_equal(a.numpy(), b), name
+    finally:
+        g.close()
