@@ -476,12 +476,17 @@ def run_core(args):
     first = core.ledger()["total_m3"]
     for _ in range(args.warmup):
         D.drive_step(core, dv, dt)
+    prev = core.ledger_lines()
+    worst = 0.0
     t0 = time.perf_counter()
     for _ in range(args.steps):
         D.drive_step(core, dv, dt)
-        led = core.ledger()
+        lines = core.ledger_lines()
+        for l in (0, 1):
+            worst = max(worst, abs(lines["gap_m3"][l] - prev["gap_m3"][l]) / lines["total_m3"][l])
+        prev = lines
     el = time.perf_counter() - t0
-    last = led["total_m3"]
+    last = core.ledger()["total_m3"]
     core.close()
     out = {
         "metric": "FluidCore steps/s (200^2 window, nested 2 cm patch, 4 tires)",
@@ -502,6 +507,11 @@ def run_core(args):
         "reference_published_ms_per_step": {"water": 0.927, "slurry": 0.539,
                                             "hardware": "Intel Xeon, 4 cores, dt 1/30 (docs/PERF_BUDGET.md:68-69)"},
         "ledger_drift_rel": (last - first) / first,
+        "ledger": {"gap_rel_worst_step": worst,
+                   "gap_rel": [prev["gap_m3"][l] / prev["total_m3"][l] for l in (0, 1)],
+                   **{k: list(prev[k]) for k in ("cut_m3", "residue_m3", "handoff_m3")},
+                   "lines": "per layer (water, slurry) since the scene was laid; gap = total - base - "
+                            "(residue - cut + handoff)"},
     }
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))

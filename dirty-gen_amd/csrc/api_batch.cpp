@@ -397,7 +397,7 @@ int silt_batch_step(silt_batch* b, float dt) {
     silt_sheet_opts_default(&o);
     o.path = SILT_PATH_EXACT;
     silt_drive d{SILT_DRIVE_GRAVITY, 0.0f, 0.0f, w.vort};
-    int rc = run_plan(sp, plans[k], w.res, w.cell, dt, d, o, true, true, w.nu, st);
+    int rc = run_plan(sp, plans[k], w.res, w.cell, dt, d, o, true, true, w.nu, st, b->totals + k);
     if (rc < 0) return rc;
     stats_compute(sp.h, sp.u, sp.v, w.res, kEps, b->stats + k, st);
   }

@@ -69,6 +69,12 @@ struct silt_core {
   unsigned* d_nev = nullptr;
   double* d_acc = nullptr;  // [0] spray volume this step, [1..2] frozen water / slurry
   double spray_m3 = 0.0;
+  // FP64 ledger lines (CoreBook) and, behind them, the copy the judged
+  // surrogate step restores.  base: each layer's total at the last caller
+  // edit less the lines booked by then; rebase: an edit since.
+  double* d_book = nullptr;
+  double base[2] = {0.0, 0.0};
+  bool rebase = true;
   // Ground under the sheet (SoilField bed / placed / initial height on the
   // window grid), set by silt_core_set_ground and cut by the rut stamp.
   bool ground = false;
@@ -117,6 +123,20 @@ struct silt_core {
 };
 
 namespace {
+
+// Layout of silt_core::d_book: what each sheet step's transports and dry cuts
+// booked (depth summed over cells of the 20 cm window and of the 2 cm patch,
+// water and slurry), then the handoffs' FP64 volumes in m^3.
+struct CoreBook {
+  SheetTotals coarse[2];
+  SheetTotals fine[2];
+  double handoff[2];
+};
+constexpr size_t kBookDoubles = sizeof(CoreBook) / sizeof(double);
+
+SheetTotals* coarse_book(silt_core* c, int l) { return reinterpret_cast<CoreBook*>(c->d_book)->coarse + l; }
+SheetTotals* fine_book(silt_core* c, int l) { return reinterpret_cast<CoreBook*>(c->d_book)->fine + l; }
+double* handoff_book(silt_core* c) { return reinterpret_cast<CoreBook*>(c->d_book)->handoff; }
 
 template <typename T>
 bool dalloc(silt_core* c, T** p, size_t n) {
@@ -285,7 +305,7 @@ int restrict_fine(silt_core* c) {
   }
   FineAxes ax = fine_axes(c);
   CoarseSpans sp = coarse_spans(ax);
-  core_restrict(c->c, c->f, c->d_hold, sp, ax, c->d_acc + 1, c->st);
+  core_restrict(c->c, c->f, c->d_hold, sp, ax, c->d_acc + 1, handoff_book(c), c->st);
   core_fill(c->f.water, 0.0f, FN, c->st);
   core_fill(c->f.slurry, 0.0f, FN, c->st);
   c->fine_on = false;
@@ -325,7 +345,7 @@ int refine(silt_core* c, const std::vector<CorePush>& p, float dt) {
   if (c->near_on) {
     auto cols = near_columns(c);
     if (upload_cols(c, cols)) return SILT_EDEVICE;
-    core_near_apply(c->f, c->near[c->near_cur], c->d_cols, (int)cols.size(), c->st);
+    core_near_apply(c->f, c->near[c->near_cur], c->d_cols, (int)cols.size(), handoff_book(c), c->st);
   }
   near_fade(c, dt);
   c->fine_on = true;
@@ -386,8 +406,10 @@ int step_coarse_pair(silt_core* c, float dt, Workspace* ws, Workspace* ws2, cons
   s.v = c->c.vw;
   s.bed = c->c.bed;
   silt_drive dw{SILT_DRIVE_GRAVITY, 0.0f, 0.0f, 0.08f};
-  rc = host[0] ? step_device_planes(s, res, cell, dt, dw, o, true, true, nu_w, c->st)
-               : (plans[0].boxes.empty() ? 0 : run_plan(s, plans[0], res, cell, dt, dw, o, true, true, nu_w, c->st));
+  rc = host[0] ? step_device_planes(s, res, cell, dt, dw, o, true, true, nu_w, c->st, coarse_book(c, 0))
+               : (plans[0].boxes.empty()
+                      ? 0
+                      : run_plan(s, plans[0], res, cell, dt, dw, o, true, true, nu_w, c->st, coarse_book(c, 0)));
   if (rc < 0) return rc;
   SheetPlanes s2 = ws2->planes();
   s2.h = c->c.slurry;
@@ -396,8 +418,10 @@ int step_coarse_pair(silt_core* c, float dt, Workspace* ws, Workspace* ws2, cons
   s2.bed = c->c.bed;
   s2.nu = nu;
   silt_drive ds{SILT_DRIVE_GRAVITY, 0.0f, 0.0f, 0.05f};
-  rc = host[1] ? step_device_planes(s2, res, cell, dt, ds, o, true, false, 0.0f, c->st2)
-               : (plans[1].boxes.empty() ? 0 : run_plan(s2, plans[1], res, cell, dt, ds, o, true, false, 0.0f, c->st2));
+  rc = host[1] ? step_device_planes(s2, res, cell, dt, ds, o, true, false, 0.0f, c->st2, coarse_book(c, 1))
+               : (plans[1].boxes.empty()
+                      ? 0
+                      : run_plan(s2, plans[1], res, cell, dt, ds, o, true, false, 0.0f, c->st2, coarse_book(c, 1)));
   HIPCHK(hipEventRecord(c->join, c->st2));
   HIPCHK(hipStreamWaitEvent(c->st, c->join, 0));
   return rc < 0 ? rc : 0;
@@ -431,14 +455,14 @@ int step_pair(silt_core* c, float dt, bool fine) {
   // The 2 cm sheet plans from its tiles' tests on the GPU (paces 0 and 1);
   // the whole-wet paces read the planes back.
   auto fine_step = [&](const SheetPlanes& sp, const silt_drive& d, bool nu_uni, float nu0, FineScratch& fsc,
-                       hipStream_t q) -> int {
+                       hipStream_t q, SheetTotals* book) -> int {
     StepPlan plan;
     int pr = plan_fine_device(sp.h, sp.u, sp.v, sp.bed, res, cell, dt, o.eps, o.sub_cap, o.pace, fsc, &plan, q);
     if (pr < 0) return pr;
-    if (pr == 0) return step_device_planes(sp, res, cell, dt, d, o, true, nu_uni, nu0, q);
-    return plan.boxes.empty() ? 0 : run_plan(sp, plan, res, cell, dt, d, o, true, nu_uni, nu0, q);
+    if (pr == 0) return step_device_planes(sp, res, cell, dt, d, o, true, nu_uni, nu0, q, book);
+    return plan.boxes.empty() ? 0 : run_plan(sp, plan, res, cell, dt, d, o, true, nu_uni, nu0, q, book);
   };
-  int rc = fine_step(s, dw, true, nu_w, c->fs[0], c->st);
+  int rc = fine_step(s, dw, true, nu_w, c->fs[0], c->st, fine_book(c, 0));
   if (rc < 0) return rc;
   // Slurry layer at its own viscosity, on the second stream.
   SheetPlanes s2 = ws2->planes();
@@ -451,7 +475,7 @@ int step_pair(silt_core* c, float dt, bool fine) {
   s2.bed = fine ? c->f.bed_s : c->c.bed;
   s2.nu = nu;
   silt_drive ds{SILT_DRIVE_GRAVITY, 0.0f, 0.0f, 0.05f};
-  rc = fine_step(s2, ds, false, 0.0f, c->fs[1], c->st2);
+  rc = fine_step(s2, ds, false, 0.0f, c->fs[1], c->st2, fine_book(c, 1));
   HIPCHK(hipEventRecord(c->join, c->st2));
   HIPCHK(hipStreamWaitEvent(c->st, c->join, 0));
   return rc < 0 ? rc : 0;
@@ -462,7 +486,8 @@ void peel(silt_core* c, const PushList& pl, bool fine, bool skip_held) {
             fine ? c->fine_origin[1] : c->origin[1]};
   GridGeo cg{kWindowRes, kCoarseCell, c->origin[0], c->origin[1]};
   core_peel(g, fine ? c->f.water : c->c.water, fine ? c->f.uw : c->c.uw, fine ? c->f.vw : c->c.vw, pl, 0, 0, g.res,
-            g.res, c->d_hold, cg, skip_held, c->d_ev, c->d_nev, kSprayCap, c->d_acc, c->st);
+            g.res, c->d_hold, cg, skip_held, c->d_ev, c->d_nev, kSprayCap, c->d_acc, fine ? kFineArea : kCoarseArea,
+            c->st);
 }
 
 void shove(silt_core* c, const std::vector<CorePush>& v, bool fine) {
@@ -535,7 +560,7 @@ int step_coarse(silt_core* c, const std::vector<CorePush>& pushes, const PushLis
   peel(c, pl, false, false);
   if (r.nx) {
     if (timed) HIPCHK(hipEventRecord(c->tev[2], c->st));
-    sur_apply(c->c.water, c->c.slurry, kWindowRes, c->net.y, r, c->sur_fac, c->st);
+    sur_apply(c->c.water, c->c.slurry, kWindowRes, c->net.y, r, c->sur_fac, c->st, coarse_book(c, 0));
     if (timed) HIPCHK(hipEventRecord(c->tev[3], c->st));
   }
   return 0;
@@ -593,6 +618,8 @@ int step_surrogate(silt_core* c, const std::vector<CorePush>& pushes, const Push
   if (rc) return rc;
   for (int k = 0; k < 6; k++)
     HIPCHK(hipMemcpyAsync(c->snap[k], p6[k], WN * sizeof(float), hipMemcpyDeviceToDevice, c->st));
+  // The judged step's ledger lines are undone with its planes.
+  HIPCHK(hipMemcpyAsync(c->d_book + kBookDoubles, c->d_book, sizeof(CoreBook), hipMemcpyDeviceToDevice, c->st));
   if ((rc = step_coarse(c, pushes, pl, dt, false, true, true))) return rc;
   HIPCHK(hipEventSynchronize(c->tev[3]));
   const float surrogate_ms = tire_ms(c);
@@ -601,6 +628,7 @@ int step_surrogate(silt_core* c, const std::vector<CorePush>& pushes, const Push
   HIPCHK(hipMemcpyAsync(c->judged_h[1], c->c.slurry, WN * sizeof(float), hipMemcpyDeviceToDevice, c->st));
   for (int k = 0; k < 6; k++)
     HIPCHK(hipMemcpyAsync(p6[k], c->snap[k], WN * sizeof(float), hipMemcpyDeviceToDevice, c->st));
+  HIPCHK(hipMemcpyAsync(c->d_book, c->d_book + kBookDoubles, sizeof(CoreBook), hipMemcpyDeviceToDevice, c->st));
   // The surrogate's spray is undone with its planes (the coarse spray of a
   // step starts from zero; frozen volumes only move in a restrict).
   HIPCHK(hipMemsetAsync(c->d_nev, 0, sizeof(unsigned), c->st));
@@ -629,6 +657,63 @@ int step_surrogate(silt_core* c, const std::vector<CorePush>& pushes, const Push
   }
   c->last_mode = SILT_TIRE_NESTED;
   c->last_tire_ms = surrogate_ms;
+  return 0;
+}
+
+// The ledger's volumes (FluidCore::ledger, src/fluid_core.rs:6821-6883) in
+// FP64 over the window and, while it is on, the 2 cm patch; with book, the
+// booked lines too, read in the same wait.
+int read_ledger(silt_core* c, silt_fluid_ledger* out, CoreBook* book) {
+  SheetStats* s = c->d_lstats;
+  stats_compute(c->c.water, c->c.uw, c->c.vw, kWindowRes, 1.0e-6f, s, c->st);
+  stats_compute(c->c.slurry, c->c.us, c->c.vs, kWindowRes, 1.0e-6f, s + 1, c->st);
+  if (c->fine_on) {
+    stats_compute(c->f.water, c->f.uw, c->f.vw, kFineRes, 1.0e-6f, s + 2, c->st);
+    stats_compute(c->f.slurry, c->f.us, c->f.vs, kFineRes, 1.0e-6f, s + 3, c->st);
+  }
+  SheetStats h[4]{};
+  double acc[4];
+  HIPCHK(hipMemcpyAsync(h, s, sizeof h, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipMemcpyAsync(acc, c->d_acc, sizeof acc, hipMemcpyDeviceToHost, c->st));
+  if (book) HIPCHK(hipMemcpyAsync(book, c->d_book, sizeof(CoreBook), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  out->water_m3 = h[0].vol * kCoarseArea + (c->fine_on ? h[2].vol * kFineArea : 0.0);
+  out->slurry_m3 = h[1].vol * kCoarseArea + (c->fine_on ? h[3].vol * kFineArea : 0.0);
+  double pw = 0.0, ps = 0.0;
+  for (const auto& kv : c->parked) {
+    pw += (double)kv.second.water;
+    ps += (double)kv.second.slurry;
+  }
+  out->frozen_water_m3 = acc[1] + pw * kCoarseArea;
+  out->frozen_slurry_m3 = acc[2] + ps * kCoarseArea;
+  out->spray_m3 = c->spray_m3;
+  out->total_m3 = out->water_m3 + out->slurry_m3 + out->frozen_water_m3 + out->frozen_slurry_m3 + out->spray_m3;
+  return SILT_OK;
+}
+
+// Each layer's total (water with its frozen part and the spray; slurry with
+// its frozen part) and what the booked lines say it gained, in m^3.
+void layer_sums(const silt_fluid_ledger& l, const CoreBook& b, double tot[2], double lines[2]) {
+  tot[0] = l.water_m3 + l.frozen_water_m3 + l.spray_m3;
+  tot[1] = l.slurry_m3 + l.frozen_slurry_m3;
+  for (int k = 0; k < 2; k++)
+    lines[k] = (b.coarse[k].residue - b.coarse[k].cut) * kCoarseArea +
+               (b.fine[k].residue - b.fine[k].cut) * kFineArea + b.handoff[k];
+}
+
+// After a caller edit, the base becomes the current totals less the lines
+// booked so far, so the gap starts again from zero.
+int ensure_base(silt_core* c) {
+  if (!c->rebase) return 0;
+  silt_fluid_ledger l{};
+  CoreBook b{};
+  int rc = read_ledger(c, &l, &b);
+  if (rc) return rc;
+  double tot[2], lines[2];
+  layer_sums(l, b, tot, lines);
+  c->base[0] = tot[0] - lines[0];
+  c->base[1] = tot[1] - lines[1];
+  c->rebase = false;
   return 0;
 }
 
@@ -675,6 +760,7 @@ silt_core* silt_core_create(float center_x, float center_z, int path) {
        dalloc(c, &c->d_acc, 4) && dalloc(c, &c->soil_bed, WN) && dalloc(c, &c->soil_placed, WN) &&
        dalloc(c, &c->soil_init, WN) && dalloc(c, &c->soil_rut, WN) && dalloc(c, &c->d_contacts, kMaxPush) &&
        dalloc(c, &c->d_stats, 2) && dalloc(c, &c->d_lstats, 4) && dalloc(c, &c->d_pub, 13 * WN) &&
+       dalloc(c, &c->d_book, 2 * kBookDoubles) &&
        hipHostMalloc((void**)&c->h_stats, 2 * sizeof(SheetStats), hipHostMallocDefault) == hipSuccess;
   if (ok) {
     // Grain::Other everywhere (FluidCore::new, src/fluid_core.rs:4979-4980).
@@ -759,6 +845,7 @@ int silt_core_get_planes(silt_core* c, float* water, float* slurry, float* uw, f
 int silt_core_set_planes(silt_core* c, const float* water, const float* slurry, const float* uw, const float* vw,
                          const float* us, const float* vs, const float* bed, const float* moisture) {
   if (!c) return SILT_EINVAL;
+  c->rebase = true;
   if (restrict_fine(c)) return SILT_EDEVICE;
   const float* src[8] = {water, slurry, uw, vw, us, vs, bed, moisture};
   float* dst[8] = {c->c.water, c->c.slurry, c->c.uw, c->c.vw, c->c.us, c->c.vs, c->c.bed, c->c.moisture};
@@ -772,6 +859,7 @@ int silt_core_set_planes(silt_core* c, const float* water, const float* slurry, 
 // not on the per-step path): read, change as the reference does, write back.
 int silt_core_map_water_bed(silt_core* c, const float* bed, const float* water) {
   if (!c || !bed || !water) return SILT_EINVAL;
+  c->rebase = true;
   if (restrict_fine(c)) return SILT_EDEVICE;
   std::vector<float> b(WN), w(WN), m(WN);
   std::vector<uint8_t> g(WN);
@@ -798,6 +886,7 @@ int silt_core_map_water_bed(silt_core* c, const float* bed, const float* water) 
 int silt_core_pour(silt_core* c, float x0, float x1, float z0, float z1, float depth, int water, int grain,
                    float moisture) {
   if (!c || grain < 0 || grain > 4) return SILT_EINVAL;
+  c->rebase = true;
   float d = depth > 0.0f ? depth : 0.0f;
   std::vector<float> h(WN), m(WN);
   std::vector<uint8_t> g(WN);
@@ -827,6 +916,7 @@ int silt_core_pour(silt_core* c, float x0, float x1, float z0, float z1, float d
 
 int silt_core_lay_slurry(silt_core* c, const float* depth, int grain, float moisture) {
   if (!c || !depth || grain < 0 || grain > 4) return SILT_EINVAL;
+  c->rebase = true;
   if (restrict_fine(c)) return SILT_EDEVICE;
   std::vector<float> s(WN), m(WN);
   std::vector<uint8_t> g(WN);
@@ -851,6 +941,7 @@ int silt_core_lay_slurry(silt_core* c, const float* depth, int grain, float mois
 
 int silt_core_read_substrate(silt_core* c, const float* bed, const float* moisture, const uint8_t* grain) {
   if (!c || !bed || !moisture || !grain) return SILT_EINVAL;
+  c->rebase = true;
   if (restrict_fine(c)) return SILT_EDEVICE;
   std::vector<float> s(WN), m(WN);
   std::vector<uint8_t> g(WN);
@@ -892,6 +983,7 @@ int silt_core_push_tire(silt_core* c, const silt_tire_push* p) {
 
 int silt_core_step(silt_core* c, float dt, silt_spray_event* events, size_t cap, size_t* nevents) {
   if (!c) return SILT_EINVAL;
+  if (int rb = ensure_base(c)) return rb;
   std::vector<CorePush> pushes;
   pushes.swap(c->pushes);
   PushList pl = push_list(pushes);
@@ -927,30 +1019,26 @@ int silt_core_step(silt_core* c, float dt, silt_spray_event* events, size_t cap,
 
 int silt_core_ledger(silt_core* c, silt_fluid_ledger* out) {
   if (!c || !out) return SILT_EINVAL;
-  SheetStats* s = c->d_lstats;
-  stats_compute(c->c.water, c->c.uw, c->c.vw, kWindowRes, 1.0e-6f, s, c->st);
-  stats_compute(c->c.slurry, c->c.us, c->c.vs, kWindowRes, 1.0e-6f, s + 1, c->st);
-  if (c->fine_on) {
-    stats_compute(c->f.water, c->f.uw, c->f.vw, kFineRes, 1.0e-6f, s + 2, c->st);
-    stats_compute(c->f.slurry, c->f.us, c->f.vs, kFineRes, 1.0e-6f, s + 3, c->st);
+  return read_ledger(c, out, nullptr);
+}
+
+int silt_core_ledger_lines(silt_core* c, silt_core_lines* out) {
+  if (!c || !out) return SILT_EINVAL;
+  int rc = ensure_base(c);
+  if (rc) return rc;
+  silt_fluid_ledger l{};
+  CoreBook b{};
+  if ((rc = read_ledger(c, &l, &b))) return rc;
+  double tot[2], lines[2];
+  layer_sums(l, b, tot, lines);
+  for (int k = 0; k < 2; k++) {
+    out->base_m3[k] = c->base[k];
+    out->cut_m3[k] = b.coarse[k].cut * kCoarseArea + b.fine[k].cut * kFineArea;
+    out->residue_m3[k] = b.coarse[k].residue * kCoarseArea + b.fine[k].residue * kFineArea;
+    out->handoff_m3[k] = b.handoff[k];
+    out->total_m3[k] = tot[k];
+    out->gap_m3[k] = tot[k] - c->base[k] - lines[k];
   }
-  SheetStats h[4]{};
-  double acc[4];
-  HIPCHK(hipMemcpyAsync(h, s, sizeof h, hipMemcpyDeviceToHost, c->st));
-  HIPCHK(hipMemcpyAsync(acc, c->d_acc, sizeof acc, hipMemcpyDeviceToHost, c->st));
-  HIPCHK(hipStreamSynchronize(c->st));
-  double ca = (double)kCoarseCell * kCoarseCell, fa = (double)kFineCell * kFineCell;
-  out->water_m3 = h[0].vol * ca + (c->fine_on ? h[2].vol * fa : 0.0);
-  out->slurry_m3 = h[1].vol * ca + (c->fine_on ? h[3].vol * fa : 0.0);
-  double pw = 0.0, ps = 0.0;
-  for (const auto& kv : c->parked) {
-    pw += (double)kv.second.water;
-    ps += (double)kv.second.slurry;
-  }
-  out->frozen_water_m3 = acc[1] + pw * ca;
-  out->frozen_slurry_m3 = acc[2] + ps * ca;
-  out->spray_m3 = c->spray_m3;
-  out->total_m3 = out->water_m3 + out->slurry_m3 + out->frozen_water_m3 + out->frozen_slurry_m3 + out->spray_m3;
   return SILT_OK;
 }
 
@@ -1307,6 +1395,7 @@ extern "C" {
 // wipes, until the sheet has moved and then stayed calm for 24 steps.
 float silt_core_drain_over_bed(silt_core* c, float dt, unsigned steps) {
   if (!c) return NAN;
+  c->rebase = true;
   auto volume = [&]() -> float {
     std::vector<float> w(WN);
     if (hipMemcpyAsync(w.data(), c->c.water, WN * 4, hipMemcpyDeviceToHost, c->st) != hipSuccess ||
@@ -1496,6 +1585,7 @@ extern "C" {
 // their array index (the caller reads the substrate again).
 int silt_core_recenter(silt_core* c, float x, float z) {
   if (!c || !isfinite(x) || !isfinite(z)) return SILT_EINVAL;
+  c->rebase = true;
   if (restrict_fine(c)) return SILT_EDEVICE;
   c->center[0] = x;
   c->center[1] = z;
@@ -1568,6 +1658,7 @@ int silt_core_recenter(silt_core* c, float x, float z) {
 // reference's f32 sum).
 float silt_core_lay_slurry_on_water(silt_core* c, float depth, int grain) {
   if (!c || !(isfinite(depth) && depth > 0.0f) || grain < 0 || grain > 4) return 0.0f;
+  c->rebase = true;
   if (restrict_fine(c)) return NAN;
   std::vector<float> w(WN), sl(WN), m(WN);
   std::vector<uint8_t> g(WN);
@@ -1681,6 +1772,7 @@ int silt_trainer_record(silt_trainer* t, silt_core* c, float dt, silt_spray_even
   for (int k = 0; k < 8; k++)
     HIPCHK(hipMemcpyAsync(dst[k], src[k], WN * sizeof(float), hipMemcpyDeviceToDevice, c->st));
   HIPCHK(hipMemcpyAsync(w->c.grain, c->c.grain, WN, hipMemcpyDeviceToDevice, c->st));
+  w->rebase = true;
   const PushList pl = push_list(c->pushes);
   sur_context(core_grid(c), r, pl, t->ctx + t->nsteps * kSurCtxFloats, c->st);
   HIPCHK(hipStreamSynchronize(c->st));

@@ -194,7 +194,8 @@ int pair_launch(silt_pair* p, float dt, const StepPlan* plans) {
     s.bed = p->bed;
     s.nu = cur.nu ? cur.nu : s.nu;
     o.path = SILT_PATH_EXACT;
-    int rc = run_plan(s, plans[l], res, p->cell, dt, d, o, true, layer_nu_uniform(p, l), layer_nu0(p, l), st);
+    int rc = run_plan(s, plans[l], res, p->cell, dt, d, o, true, layer_nu_uniform(p, l), layer_nu0(p, l), st,
+                      p->totals + l);
     if (rc < 0) return rc;
     o.path = p->path;
     stats_compute(cur.h, cur.u, cur.v, res, kEps, p->stats + l, st);

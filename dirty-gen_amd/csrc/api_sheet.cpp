@@ -48,7 +48,8 @@ StepArgs make_args(int res, float cell, float dt, const silt_drive& d, float eps
 // Runs a planned step on device planes.  Exact path: stage kernels per box,
 // boxes in order, as the reference loops them.  Fast path: the fused kernel.
 int run_plan(const SheetPlanes& s, const StepPlan& plan, int res, float cell, float dt, const silt_drive& d,
-             const silt_sheet_opts& o, bool gate_uniform, bool nu_uniform, float nu0, hipStream_t st) {
+             const silt_sheet_opts& o, bool gate_uniform, bool nu_uniform, float nu0, hipStream_t st,
+             SheetTotals* book) {
   if (plan.boxes.empty() || plan.nsub <= 0) return 0;
   float dts = dt / (float)plan.nsub;
   bool wind = d.kind == SILT_DRIVE_WIND;
@@ -63,7 +64,7 @@ int run_plan(const SheetPlanes& s, const StepPlan& plan, int res, float cell, fl
     bool any_gs = false;
     for (const Box& b : plan.boxes) any_gs = any_gs || gs_for(b);
     if (!any_gs) {
-      int rc = fused_step(s, plan, res, cell, dt, d, o, gate_uniform, nu_uniform, nu0, st);
+      int rc = fused_step(s, plan, res, cell, dt, d, o, gate_uniform, nu_uniform, nu0, st, book);
       if (rc != SILT_EUNSUPPORTED) return rc;
     }
   }
@@ -71,6 +72,7 @@ int run_plan(const SheetPlanes& s, const StepPlan& plan, int res, float cell, fl
     StepArgs full = make_args(res, cell, dt, d, o.eps, o.banks, gate_uniform, nu_uniform, nu0);
     StepArgs slice = make_args(res, cell, dts, d, o.eps, o.banks, gate_uniform, nu_uniform, nu0);
     slice.banks = 0;
+    full.book = slice.book = book;
     for (const Box& b : plan.boxes) {
       exact_substep(s, b, full, false, gs_for(b), st);
       for (int k = 0; k < plan.nsub; k++) exact_transport(s, b, slice, dts, st);
@@ -79,6 +81,7 @@ int run_plan(const SheetPlanes& s, const StepPlan& plan, int res, float cell, fl
     return plan.nsub;
   }
   StepArgs a = make_args(res, cell, dts, d, o.eps, o.banks, gate_uniform, nu_uniform, nu0);
+  a.book = book;
   for (int k = 0; k < plan.nsub; k++)
     for (const Box& b : plan.boxes) exact_substep(s, b, a, true, gs_for(b), st);
   return plan.nsub;
@@ -88,7 +91,8 @@ int run_plan(const SheetPlanes& s, const StepPlan& plan, int res, float cell, fl
 // depth, velocity and bed back to the host (blocking), then the kernels are
 // enqueued on st.  The workspace scratch planes in s must be valid.
 int step_device_planes(const SheetPlanes& s, int res, float cell, float dt, const silt_drive& d,
-                       const silt_sheet_opts& o, bool gate_uniform, bool nu_uniform, float nu0, hipStream_t st) {
+                       const silt_sheet_opts& o, bool gate_uniform, bool nu_uniform, float nu0, hipStream_t st,
+                       SheetTotals* book) {
   size_t n = (size_t)res * res;
   std::vector<float> hh(n), uh(n), vh(n), bh(n);
   size_t fb = n * sizeof(float);
@@ -100,7 +104,7 @@ int step_device_planes(const SheetPlanes& s, int res, float cell, float dt, cons
   StepPlan plan = plan_step(hh.data(), uh.data(), vh.data(), bh.data(), res, cell, dt, d.kind == SILT_DRIVE_WIND, d.x,
                             d.z, o.eps, o.sub_cap, o.pace);
   if (plan.boxes.empty()) return 0;
-  return run_plan(s, plan, res, cell, dt, d, o, gate_uniform, nu_uniform, nu0, st);
+  return run_plan(s, plan, res, cell, dt, d, o, gate_uniform, nu_uniform, nu0, st, book);
 }
 
 bool valid_opts(const silt_sheet_opts& o) {

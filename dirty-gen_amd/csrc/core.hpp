@@ -12,6 +12,11 @@ constexpr int kWindowRes = 200;     // WINDOW_RES, src/fluid_core.rs:72
 constexpr int kFineRes = 160;       // FINE_RES, :76
 constexpr float kCoarseCell = 0.20f;  // COARSE_CELL_M, :70
 constexpr int kMaxPush = 64;
+// Cell areas of the FP64 ledger.  A coarse cell is exactly 10 x 10 patch
+// cells, so the patch's area is the coarse one over 100 (the FP32 cell sides
+// 0.2 and 0.02 would make a refine or restrict change the volume by 7e-8).
+constexpr double kCoarseArea = (double)kCoarseCell * (double)kCoarseCell;
+constexpr double kFineArea = kCoarseArea / 100.0;
 constexpr unsigned long long kEmptyKey = 0xFFFFFFFFFFFFFFFFull;
 constexpr unsigned long long kTombKey = 0xFFFFFFFFFFFFFFFEull;
 
@@ -90,16 +95,22 @@ void core_push_on_many(const GridGeo& g, float* water, float* slurry, float* uw,
                        const PushBox* boxes, const CorePush* pushes, int n, int max_w, int max_h, hipStream_t st);
 void core_peel(const GridGeo& g, float* water, const float* uw, const float* vw, const PushList& pl, int i0, int j0,
                int w, int h, const uint8_t* hold, const GridGeo& cg, bool skip_held, SprayEvent* ev, unsigned* nev,
-               unsigned cap, double* vol, hipStream_t st);
+               unsigned cap, double* vol, double area, hipStream_t st);
 void core_soften(float* u, float* v, int res, hipStream_t st);
 void core_nu(float* nu, const uint8_t* grain, const float* moisture, int n, hipStream_t st);
 void core_fill(float* p, float val, int n, hipStream_t st);
 void core_refine_copy(const CorePlanes& c, const FinePlanes& f, const uint8_t* hold, const FineAxes& ax, int ci0,
                       int cj0, int cw, int ch, hipStream_t st);
+// frozen[0..1]: FP64 water / slurry volume parked off the window.
+// handoff[0..1]: FP64 volume the coarse cells gained beyond the patch volume
+// handed to them (the FP32 sums' rounding).
 void core_restrict(const CorePlanes& c, const FinePlanes& f, const uint8_t* hold, const CoarseSpans& sp,
-                   const FineAxes& ax, double* frozen, hipStream_t st);
+                   const FineAxes& ax, double* frozen, double* handoff, hipStream_t st);
 void core_near_capture(const FinePlanes& f, const NearTable& t, const NearColumn* cols, int ncols, hipStream_t st);
-void core_near_apply(const FinePlanes& f, const NearTable& t, const NearColumn* cols, int ncols, hipStream_t st);
+// handoff[0..1]: FP64 volume the lay-down added to the water / slurry (the
+// stored detail's FP32 mean and the max(0) clamp).
+void core_near_apply(const FinePlanes& f, const NearTable& t, const NearColumn* cols, int ncols, double* handoff,
+                     hipStream_t st);
 void core_near_fade(const NearTable& t, float dt, hipStream_t st);
 void core_near_clear(const NearTable& t, hipStream_t st);
 void core_near_rehash(const NearTable& from, const NearTable& to, hipStream_t st);

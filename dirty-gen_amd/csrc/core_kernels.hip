@@ -89,7 +89,7 @@ __global__ void k_push_on_many(GridGeo g, float* water, float* slurry, float* uw
 // FP64 (the reference sums it in f32 in event order).
 __global__ void k_peel(GridGeo g, float* water, const float* uw, const float* vw, PushList pl, int i0, int j0, int w,
                        int h, const uint8_t* hold, GridGeo cg, int skip_held, SprayEvent* ev, unsigned* nev,
-                       unsigned cap, double* vol) {
+                       unsigned cap, double* vol, double ledger_area) {
   int i = i0 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
   int j = j0 + (int)(blockIdx.y * blockDim.y + threadIdx.y);
   if (i >= i0 + w || j >= j0 + h) return;
@@ -117,8 +117,10 @@ __global__ void k_peel(GridGeo g, float* water, const float* uw, const float* vw
     if (wv < 0.01f) continue;
     float take = fminf(wv * 0.04f, 0.004f);
     if (take * area < 1.0e-6f) continue;
-    water[idx] = wv - take;
-    atomicAdd(vol, (double)(take * area));
+    const float left = wv - take;
+    water[idx] = left;
+    // The ledger books the depth the cell lost, exactly.
+    atomicAdd(vol, ((double)wv - (double)left) * ledger_area);
     unsigned k = atomicAdd(nev, 1u);
     if (k < cap) {
       SprayEvent e;
@@ -196,7 +198,9 @@ __global__ void k_zero_held(CorePlanes c, const uint8_t* hold, int ci0, int cj0,
 
 // Fine -> coarse sum of restrict_fine (src/fluid_core.rs:8462-8549) for one
 // coarse cell: its fine cells in the fine raster order, then the hand-back.
-__global__ void k_restrict(CorePlanes c, FinePlanes f, const uint8_t* hold, CoarseSpans sp) {
+// With the ledger: each coarse cell's FP64 gain less the FP64 volume of its
+// fine cells goes to handoff[0..1].
+__global__ void k_restrict(CorePlanes c, FinePlanes f, const uint8_t* hold, CoarseSpans sp, double* handoff) {
   int a = (int)(blockIdx.x * blockDim.x + threadIdx.x), b = (int)(blockIdx.y * blockDim.y + threadIdx.y);
   if (a >= sp.ncol || b >= sp.nrow) return;
   int ci = sp.ci0 + a, cj = sp.cj0 + b;
@@ -205,11 +209,14 @@ __global__ void k_restrict(CorePlanes c, FinePlanes f, const uint8_t* hold, Coar
   if (ni == 0 || nj == 0) return;
   const float fine_area = kFineCell * kFineCell, coarse_area = kCoarseCell * kCoarseCell;
   float w = 0.0f, s = 0.0f, auw = 0.0f, avw = 0.0f, aus = 0.0f, avs = 0.0f;
+  double fine_w = 0.0, fine_s = 0.0;
   for (int j = j0; j < j0 + nj; j++)
     for (int i = i0; i < i0 + ni; i++) {
       size_t fidx = (size_t)j * kFineRes + i;
       float fw = f.water[fidx], fs = f.slurry[fidx];
       if (fw == 0.0f && fs == 0.0f) continue;
+      fine_w += (double)fmaxf(fw, 0.0f);
+      fine_s += (double)fmaxf(fs, 0.0f);
       w = w + fw * fine_area;
       s = s + fs * fine_area;
       auw = auw + f.uw[fidx] * fw * fine_area;
@@ -220,9 +227,21 @@ __global__ void k_restrict(CorePlanes c, FinePlanes f, const uint8_t* hold, Coar
   size_t idx = (size_t)cj * kWindowRes + ci;
   bool held = hold[idx] == 1;
   float wd = w / coarse_area, sd = s / coarse_area;
+  const float was_w = c.water[idx], was_s = c.slurry[idx];
+  // Books the cell's gain beyond its fine cells' volume once it is written.
+  auto book = [&](float now_w, float now_s) {
+    if (!handoff) return;
+    const double dw = ((double)fmaxf(now_w, 0.0f) - (double)fmaxf(was_w, 0.0f)) * kCoarseArea - fine_w * kFineArea;
+    const double ds = ((double)fmaxf(now_s, 0.0f) - (double)fmaxf(was_s, 0.0f)) * kCoarseArea - fine_s * kFineArea;
+    if (dw != 0.0) atomicAdd(&handoff[0], dw);
+    if (ds != 0.0) atomicAdd(&handoff[1], ds);
+  };
   if (!held && ni * nj >= 100) {
-    if (wd <= 0.0f && sd <= 0.0f) return;
-    float old_w = c.water[idx], old_s = c.slurry[idx];
+    if (wd <= 0.0f && sd <= 0.0f) {
+      book(was_w, was_s);
+      return;
+    }
+    float old_w = was_w, old_s = was_s;
     float new_w = old_w + wd, new_s = old_s + sd;
     if (new_w > 1.0e-6f) {
       c.uw[idx] = (old_w * c.uw[idx] * coarse_area + auw) / (new_w * coarse_area);
@@ -234,10 +253,12 @@ __global__ void k_restrict(CorePlanes c, FinePlanes f, const uint8_t* hold, Coar
     }
     c.water[idx] = new_w;
     c.slurry[idx] = new_s;
+    book(new_w, new_s);
     return;
   }
   c.water[idx] = wd;
   c.slurry[idx] = sd;
+  book(wd, sd);
   if (wd > 1.0e-6f) {
     c.uw[idx] = auw / (wd * coarse_area);
     c.vw[idx] = avw / (wd * coarse_area);
@@ -255,10 +276,9 @@ __global__ void k_park(FinePlanes f, FineAxes ax, double* frozen) {
   if (i >= kFineRes || j >= kFineRes) return;
   if (ax.col[i] >= 0 && ax.row[j] >= 0) return;
   size_t fidx = (size_t)j * kFineRes + i;
-  const float fa = kFineCell * kFineCell;
   float w = f.water[fidx], s = f.slurry[fidx];
-  if (w != 0.0f) atomicAdd(&frozen[0], (double)(w * fa));
-  if (s != 0.0f) atomicAdd(&frozen[1], (double)(s * fa));
+  if (w != 0.0f) atomicAdd(&frozen[0], (double)fmaxf(w, 0.0f) * kFineArea);
+  if (s != 0.0f) atomicAdd(&frozen[1], (double)fmaxf(s, 0.0f) * kFineArea);
 }
 
 __global__ void k_fill(float* p, float val, int n) {
@@ -574,7 +594,8 @@ __global__ __launch_bounds__(128) void k_near_capture_wg(FinePlanes f, NearTable
 
 // near_apply with one 128-lane workgroup per held column (lane k < 100 owns
 // fine cell k); floor_scale's minima are order-free (fminf drops NaN).
-__global__ __launch_bounds__(128) void k_near_apply_wg(FinePlanes f, NearTable t, const NearColumn* cols) {
+__global__ __launch_bounds__(128) void k_near_apply_wg(FinePlanes f, NearTable t, const NearColumn* cols,
+                                                      double* handoff) {
   __shared__ float lo[2][2];
   __shared__ int slot_s;
   const NearColumn nc = cols[blockIdx.x];
@@ -609,15 +630,29 @@ __global__ __launch_bounds__(128) void k_near_apply_wg(FinePlanes f, NearTable t
   // Every lane has read the column's first cell above; the writes below may
   // overwrite it.
   __syncthreads();
-  if (!cellk) return;
-  const size_t fi = (size_t)(nc.j0 + k / 10) * kFineRes + nc.i0 + k % 10;
-  const float w = fmaxf(bw + aw * dw, 0.0f);
-  const float sv = fmaxf(bs + as * ds, 0.0f);
-  f.water[fi] = w;
-  f.slurry[fi] = sv;
-  if (nc.inner && w > 1.0e-6f) {
-    f.uw[fi] = f.uw[fi] + left * d[200 + k];
-    f.vw[fi] = f.vw[fi] + left * d[300 + k];
+  double gain_w = 0.0, gain_s = 0.0;
+  if (cellk) {
+    const size_t fi = (size_t)(nc.j0 + k / 10) * kFineRes + nc.i0 + k % 10;
+    const float was_w = f.water[fi], was_s = f.slurry[fi];
+    const float w = fmaxf(bw + aw * dw, 0.0f);
+    const float sv = fmaxf(bs + as * ds, 0.0f);
+    f.water[fi] = w;
+    f.slurry[fi] = sv;
+    if (nc.inner && w > 1.0e-6f) {
+      f.uw[fi] = f.uw[fi] + left * d[200 + k];
+      f.vw[fi] = f.vw[fi] + left * d[300 + k];
+    }
+    gain_w = ((double)w - (double)fmaxf(was_w, 0.0f)) * kFineArea;
+    gain_s = ((double)sv - (double)fmaxf(was_s, 0.0f)) * kFineArea;
+  }
+  if (!handoff) return;
+  for (int o = 32; o > 0; o >>= 1) {
+    gain_w += __shfl_xor(gain_w, o);
+    gain_s += __shfl_xor(gain_s, o);
+  }
+  if ((k & 63) == 0) {
+    if (gain_w != 0.0) atomicAdd(&handoff[0], gain_w);
+    if (gain_s != 0.0) atomicAdd(&handoff[1], gain_s);
   }
 }
 
@@ -788,11 +823,11 @@ void core_push_on_many(const GridGeo& g, float* water, float* slurry, float* uw,
 
 void core_peel(const GridGeo& g, float* water, const float* uw, const float* vw, const PushList& pl, int i0, int j0,
                int w, int h, const uint8_t* hold, const GridGeo& cg, bool skip_held, SprayEvent* ev, unsigned* nev,
-               unsigned cap, double* vol, hipStream_t st) {
+               unsigned cap, double* vol, double area, hipStream_t st) {
   if (w <= 0 || h <= 0 || pl.n <= 0) return;
   dim3 b(32, 8);
   hipLaunchKernelGGL(k_peel, grid2(w, h, b), b, 0, st, g, water, uw, vw, pl, i0, j0, w, h, hold, cg, (int)skip_held, ev,
-                     nev, cap, vol);
+                     nev, cap, vol, area);
 }
 
 void core_soften(float* u, float* v, int res, hipStream_t st) {
@@ -816,9 +851,10 @@ void core_refine_copy(const CorePlanes& c, const FinePlanes& f, const uint8_t* h
 }
 
 void core_restrict(const CorePlanes& c, const FinePlanes& f, const uint8_t* hold, const CoarseSpans& sp,
-                   const FineAxes& ax, double* frozen, hipStream_t st) {
+                   const FineAxes& ax, double* frozen, double* handoff, hipStream_t st) {
   dim3 b(16, 16);
-  if (sp.ncol > 0 && sp.nrow > 0) hipLaunchKernelGGL(k_restrict, grid2(sp.ncol, sp.nrow, b), b, 0, st, c, f, hold, sp);
+  if (sp.ncol > 0 && sp.nrow > 0)
+    hipLaunchKernelGGL(k_restrict, grid2(sp.ncol, sp.nrow, b), b, 0, st, c, f, hold, sp, handoff);
   hipLaunchKernelGGL(k_park, grid2(kFineRes, kFineRes, b), b, 0, st, f, ax, frozen);
 }
 
@@ -827,9 +863,10 @@ void core_near_capture(const FinePlanes& f, const NearTable& t, const NearColumn
   hipLaunchKernelGGL(k_near_capture_wg, dim3((unsigned)ncols), dim3(128), 0, st, f, t, cols);
 }
 
-void core_near_apply(const FinePlanes& f, const NearTable& t, const NearColumn* cols, int ncols, hipStream_t st) {
+void core_near_apply(const FinePlanes& f, const NearTable& t, const NearColumn* cols, int ncols, double* handoff,
+                     hipStream_t st) {
   if (ncols <= 0) return;
-  hipLaunchKernelGGL(k_near_apply_wg, dim3((unsigned)ncols), dim3(128), 0, st, f, t, cols);
+  hipLaunchKernelGGL(k_near_apply_wg, dim3((unsigned)ncols), dim3(128), 0, st, f, t, cols, handoff);
 }
 
 void core_near_fade(const NearTable& t, float dt, hipStream_t st) {

@@ -16,6 +16,14 @@ struct Box {
   int i0, i1, j0, j1;
 };
 
+// Running FP64 ledger lines of one layer, metres of depth summed over cells:
+// depth the dry cut deleted, and the transport residue (FP32 rounding and the
+// max(0) clamp beyond the exchanged fluxes).
+struct SheetTotals {
+  double cut;
+  double residue;
+};
+
 // Everything one substep of one layer needs besides the planes.
 struct StepArgs {
   int res;
@@ -33,6 +41,8 @@ struct StepArgs {
   // Viscosity-derived scalars when every cell has the same viscosity.
   int nu_uniform;
   float nu0, mob0, ov0, decay0;
+  // Ledger lines the transport and the dry cut add to, or null.
+  SheetTotals* book;
 };
 
 // Device planes of one layer plus the scratch one substep uses.

@@ -534,31 +534,57 @@ __device__ inline float transport_cell(const float* hin, const float* vel, const
   return (rem > 0.0f || add > 0.0f) ? moved : hv;
 }
 
+// Adds v summed over the 64 lanes of each wave to *p, one atomic per wave
+// (the blocks here are 64 x 4: a wave is one row of the block).  Every lane
+// of the block must reach it.
+__device__ inline void book_wave(double* p, double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  if (threadIdx.x == 0 && v != 0.0) atomicAdd(p, v);
+}
+
+// One transport pass over the box.  The z pass (along_x 0) writes h over the
+// depth the x pass started from, so with a ledger it books each cell's change
+// across both passes: the exchanged face fluxes cancel over the grid and the
+// sum is what FP32 rounding and the max(0) clamp created or removed.
 __global__ void k_transport(const float* __restrict__ hin, float* __restrict__ hout, const float* __restrict__ vel,
                             const uint8_t* __restrict__ gate, const float* __restrict__ bed, Box b, int along_x,
                             StepArgs a, float dt) {
   Region r{b.i0, b.i1, b.j0, b.j1};
   int i, j;
-  if (!region_cell(r, &i, &j)) return;
-  size_t idx = (size_t)j * a.res + i;
-  if (along_x)
-    hout[idx] = transport_cell(hin, vel, gate, bed, a.banks, idx, 1, i, b.i0, b.i1, dt, a.cell, a.gate_uniform);
-  else
-    hout[idx] = transport_cell(hin, vel, gate, bed, a.banks, idx, (size_t)a.res, j, b.j0, b.j1, dt, a.cell,
-                               a.gate_uniform);
+  const bool in = region_cell(r, &i, &j);
+  double moved = 0.0;
+  if (in) {
+    size_t idx = (size_t)j * a.res + i;
+    if (along_x) {
+      hout[idx] = transport_cell(hin, vel, gate, bed, a.banks, idx, 1, i, b.i0, b.i1, dt, a.cell, a.gate_uniform);
+    } else {
+      const float was = hout[idx];
+      const float now = transport_cell(hin, vel, gate, bed, a.banks, idx, (size_t)a.res, j, b.j0, b.j1, dt, a.cell,
+                                       a.gate_uniform);
+      hout[idx] = now;
+      moved = (double)fmaxf(now, 0.0f) - (double)fmaxf(was, 0.0f);
+    }
+  }
+  if (!along_x && a.book) book_wave(&a.book->residue, moved);
 }
 
-// Dry cut (:2072-2081): depth under eps, and its velocity, go to zero.
+// Dry cut (:2072-2081): depth under eps, and its velocity, go to zero.  With
+// a ledger the deleted depth is booked as cut.
 __global__ void k_dry_cut(float* __restrict__ h, float* __restrict__ u, float* __restrict__ v, Box b, StepArgs a) {
   Region r{b.i0, b.i1, b.j0, b.j1};
   int i, j;
-  if (!region_cell(r, &i, &j)) return;
-  size_t idx = (size_t)j * a.res + i;
-  if (a.eps > 0.0f && h[idx] < a.eps) {
-    h[idx] = 0.0f;
-    u[idx] = 0.0f;
-    v[idx] = 0.0f;
+  double cut = 0.0;
+  if (region_cell(r, &i, &j)) {
+    size_t idx = (size_t)j * a.res + i;
+    const float hv = h[idx];
+    if (a.eps > 0.0f && hv < a.eps) {
+      cut = (double)fmaxf(hv, 0.0f);
+      h[idx] = 0.0f;
+      u[idx] = 0.0f;
+      v[idx] = 0.0f;
+    }
   }
+  if (a.book) book_wave(&a.book->cut, cut);
 }
 
 inline dim3 grid_for(const Region& r, dim3 blk) {

@@ -334,7 +334,7 @@ __global__ __launch_bounds__(kLanes, 3) void k_sheet_stream(FusedArgs fa) {
   // the grid the exchanged fluxes cancel and the residues sum to exactly the
   // volume the FP32 updates created or removed.  Cut depths (each below
   // 1e-6 m) are summed per column in FP32 and reduced in FP64.
-  const bool want_stats = L.stats != nullptr;
+  const bool want_stats = L.stats != nullptr || L.totals != nullptr;
   double acc_vol = 0.0, acc_res = 0.0;
   float acc_cut = 0.0f;
   float acc_s2 = 0.0f;
@@ -842,15 +842,17 @@ __global__ __launch_bounds__(kLanes, 3) void k_sheet_stream(FusedArgs fa) {
       acc_s2 = fmaxf(acc_s2, __shfl_xor(acc_s2, off));
     }
     if ((t & 63) == 0) {
-      atomicAdd(&L.stats->vol, acc_vol);
-      atomicAdd(&L.stats->wet, (unsigned long long)acc_wet);
-      if (acc_wet) {
-        atomicMin(&L.stats->i0, acc_i0);
-        atomicMax(&L.stats->i1, acc_i1);
-        atomicMin(&L.stats->j0, acc_j0);
-        atomicMax(&L.stats->j1, acc_j1);
+      if (L.stats) {
+        atomicAdd(&L.stats->vol, acc_vol);
+        atomicAdd(&L.stats->wet, (unsigned long long)acc_wet);
+        if (acc_wet) {
+          atomicMin(&L.stats->i0, acc_i0);
+          atomicMax(&L.stats->i1, acc_i1);
+          atomicMin(&L.stats->j0, acc_j0);
+          atomicMax(&L.stats->j1, acc_j1);
+        }
+        atomicMax(&L.stats->max_s2_bits, __float_as_uint(acc_s2));
       }
-      atomicMax(&L.stats->max_s2_bits, __float_as_uint(acc_s2));
       if (L.totals) {
         atomicAdd(&L.totals->cut, sum_cut);
         atomicAdd(&L.totals->residue, sum_res);
@@ -1140,7 +1142,8 @@ int fused_substep_layers(const FusedLayer* layers, int nlayers, const float* bed
 // not cover (pace 3 slices, Gauss-Seidel sweeps, overlapping boxes) return
 // SILT_EUNSUPPORTED so the caller can take the exact path.
 int fused_step(const SheetPlanes& s, const StepPlan& plan, int res, float cell, float dt, const silt_drive& d,
-               const silt_sheet_opts& o, bool gate_uniform, bool nu_uniform, float nu0, hipStream_t st) {
+               const silt_sheet_opts& o, bool gate_uniform, bool nu_uniform, float nu0, hipStream_t st,
+               SheetTotals* book) {
   if (plan.pace3 || !boxes_fit(plan.boxes)) return SILT_EUNSUPPORTED;
   float dts = dt / (float)plan.nsub;
   StepArgs a = make_args(res, cell, dts, d, o.eps, o.banks, gate_uniform, nu_uniform, nu0);
@@ -1164,6 +1167,7 @@ int fused_step(const SheetPlanes& s, const StepPlan& plan, int res, float cell, 
     f.vo = from_user ? bv : s.v;
     f.a = a;
     f.boxes = plan.boxes;
+    f.totals = book;
     int rc = fused_substep_layers(&f, 1, s.bed, res, o.path == SILT_PATH_FUSED_EXACT, st);
     if (rc != 0) return rc;
   }

@@ -23,14 +23,6 @@ struct SheetStats {
   int pad;
 };
 
-// Running FP64 ledger lines of one layer, metres of depth summed over cells:
-// depth the dry cut deleted, and the transport residue (FP32 rounding and the
-// max(0) clamp beyond the exchanged fluxes).
-struct SheetTotals {
-  double cut;
-  double residue;
-};
-
 // One layer of one fused substep: read planes in `in`, write ho/uo/vo.
 struct FusedLayer {
   SheetPlanes in;  // in.bed is the window's bed
@@ -154,17 +146,20 @@ int fused_enqueue(const FusedLaunch& l, const void* dev_table, const int* dev_ti
 // SILT_EUNSUPPORTED for plans the fused kernel does not cover (pace-3 slices,
 // touching or overlapping boxes, dry-bank transport), or another SILT_E* code.
 int fused_step(const SheetPlanes& s, const StepPlan& plan, int res, float cell, float dt, const silt_drive& d,
-               const silt_sheet_opts& o, bool gate_uniform, bool nu_uniform, float nu0, hipStream_t st);
+               const silt_sheet_opts& o, bool gate_uniform, bool nu_uniform, float nu0, hipStream_t st,
+               SheetTotals* book = nullptr);
 
 // Runs a planned step of one layer on the path opts.path names (fused paths
 // fall back to the stage kernels for plans they do not cover).
 int run_plan(const SheetPlanes& s, const StepPlan& plan, int res, float cell, float dt, const silt_drive& d,
-             const silt_sheet_opts& o, bool gate_uniform, bool nu_uniform, float nu0, hipStream_t st);
+             const silt_sheet_opts& o, bool gate_uniform, bool nu_uniform, float nu0, hipStream_t st,
+             SheetTotals* book = nullptr);
 
 // A step of one layer on device planes, planned on the host from a read
 // back of depth, velocity and bed (blocking); kernels enqueued on st.
 int step_device_planes(const SheetPlanes& s, int res, float cell, float dt, const silt_drive& d,
-                       const silt_sheet_opts& o, bool gate_uniform, bool nu_uniform, float nu0, hipStream_t st);
+                       const silt_sheet_opts& o, bool gate_uniform, bool nu_uniform, float nu0, hipStream_t st,
+                       SheetTotals* book = nullptr);
 
 // Output rows per workgroup: the smallest chunk that keeps every tile of the
 // launch resident, unless fused_set_chunk_rows pinned one (0 unpins).

@@ -474,6 +474,7 @@ def _declare_core(L):
     L.silt_core_coarse_cell.argtypes = [_P, _Z, _Z, C.POINTER(CoarseCell)]
     L.silt_core_sample.argtypes = [_P, _F, _F, C.POINTER(GroundSample)]
     L.silt_core_ledger_full.argtypes = [_P, C.POINTER(FluidLedgerFull)]
+    L.silt_core_ledger_lines.argtypes = [_P, C.POINTER(CoreLines)]
     L.silt_core_recenter.argtypes = [_P, _F, _F]
     L.silt_core_window_bounds.argtypes = [_P, _P]
     L.silt_core_lay_slurry_on_water.argtypes = [_P, _F, C.c_int]
@@ -518,6 +519,11 @@ class FluidLedgerFull(C.Structure):
         "water_m3", "slurry_m3", "dust_air_m3", "dust_settled_m3", "frozen_water_m3", "frozen_slurry_m3",
         "frozen_dust_m3", "spray_m3", "fog_air_m3", "suspended_m3", "settled_suspended_m3", "frozen_suspended_m3",
         "entrained_m3", "reach_m3", "total_m3")]
+
+
+class CoreLines(C.Structure):
+    _fields_ = [(n, C.c_double * 2) for n in (
+        "base_m3", "cut_m3", "residue_m3", "handoff_m3", "total_m3", "gap_m3")]
 
 
 class SurrogateStats(C.Structure):
@@ -691,6 +697,14 @@ class FluidCore:
         led = FluidLedgerFull()
         _check_status(lib().silt_core_ledger_full(self._c, C.byref(led)))
         return {k: getattr(led, k) for k, _ in FluidLedgerFull._fields_}
+
+    def ledger_lines(self):
+        """The FP64 lines that close the ledger (include/silt_core.h):
+        {line: (water, slurry)} for base, cut, residue, handoff, total and
+        gap, in m^3."""
+        out = CoreLines()
+        _check_status(lib().silt_core_ledger_lines(self._c, C.byref(out)))
+        return {k: (getattr(out, k)[0], getattr(out, k)[1]) for k, _ in CoreLines._fields_}
 
     def window_bounds(self):
         out = np.zeros(3, np.float32)

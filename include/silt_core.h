@@ -102,6 +102,22 @@ int silt_core_step(silt_core *c, float dt, silt_spray_event *events, size_t cap,
 /* ledger() (src/fluid_core.rs:6821-6883) for the sheets, parked fine volume and spray. */
 int silt_core_ledger(silt_core *c, silt_fluid_ledger *out);
 
+/* The FP64 lines that close the ledger, per layer ([0] water, [1] slurry).
+ * total: the layer's volume with its frozen part (and, for water, the
+ * spray).  cut: depth the sheet steps' dry cuts deleted.  residue: what the
+ * FP32 transports and the surrogate's fluxes created or removed beyond the
+ * exchanged fluxes (rounding and the max(0) clamp).  handoff: what the
+ * window <-> 2 cm patch handoffs changed (the FP32 restrict sums, the near
+ * field's lay-down).  base: the total at the last caller edit (set_planes,
+ * pours, map_water_bed, read_substrate, lay_slurry*, drain_over_bed,
+ * recenter) less the lines booked by then.  gap = total - base -
+ * (residue - cut + handoff): volume no line accounts for.  The reference's
+ * ledger() has no such lines (FluidLedger, src/fluid_core.rs:483-509). */
+typedef struct silt_core_lines {
+    double base_m3[2], cut_m3[2], residue_m3[2], handoff_m3[2], total_m3[2], gap_m3[2];
+} silt_core_lines;
+int silt_core_ledger_lines(silt_core *c, silt_core_lines *out);
+
 /* Coarse planes in and out, 200 x 200 each (any pointer may be NULL to skip). */
 int silt_core_get_planes(silt_core *c, float *water, float *slurry, float *uw, float *vw, float *us, float *vs,
                          float *bed, float *moisture);

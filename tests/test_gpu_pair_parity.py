@@ -185,6 +185,10 @@ def test_two_centimetre_pair_equals_the_fp32_oracle():
                 bad = np.flatnonzero(a != b)
                 assert bad.size == 0, f"step {k} {name}: {bad.size} cells differ, first {bad[:4]}"
         assert unfit > 0, "no step planned boxes the fused kernel does not cover"
+        # The stage kernels book their dry cut and transport residue too.
+        led = p.ledger()
+        for L in (led.water, led.slurry):
+            assert abs(L.gap_m3) <= 1e-9 * L.initial_m3, (L.gap_m3, L.initial_m3)
     finally:
         p.close()
 
