@@ -75,6 +75,16 @@ struct silt_core {
   double* d_book = nullptr;
   double base[2] = {0.0, 0.0};
   bool rebase = true;
+  // Hydro solids (set_hydro_solids), their mask descriptors, the wall masks
+  // of the window and the patch, and the scour's bed drop, eroded volume
+  // (FP64, m^3) and deepest drop (float bits).
+  std::vector<silt_hydro_solid> solids;
+  SolidDesc* d_solids = nullptr;
+  uint8_t* d_mask = nullptr;
+  uint8_t* d_fmask = nullptr;
+  float* d_bed_drop = nullptr;
+  double* d_eroded = nullptr;
+  unsigned* d_hole = nullptr;
   // Ground under the sheet (SoilField bed / placed / initial height on the
   // window grid), set by silt_core_set_ground and cut by the rut stamp.
   bool ground = false;
@@ -131,12 +141,14 @@ struct CoreBook {
   SheetTotals coarse[2];
   SheetTotals fine[2];
   double handoff[2];
+  double source[2];  // volume that entered the layers from the bed (scour)
 };
 constexpr size_t kBookDoubles = sizeof(CoreBook) / sizeof(double);
 
 SheetTotals* coarse_book(silt_core* c, int l) { return reinterpret_cast<CoreBook*>(c->d_book)->coarse + l; }
 SheetTotals* fine_book(silt_core* c, int l) { return reinterpret_cast<CoreBook*>(c->d_book)->fine + l; }
 double* handoff_book(silt_core* c) { return reinterpret_cast<CoreBook*>(c->d_book)->handoff; }
+double* source_book(silt_core* c) { return reinterpret_cast<CoreBook*>(c->d_book)->source; }
 
 template <typename T>
 bool dalloc(silt_core* c, T** p, size_t n) {
@@ -427,8 +439,112 @@ int step_coarse_pair(silt_core* c, float dt, Workspace* ws, Workspace* ws2, cons
   return rc < 0 ? rc : 0;
 }
 
+// The wall mask of one grid from its planes now (solid_mask,
+// src/fluid_core.rs:7288-7362): descriptors on the host (the cell ranges and
+// the yaw's sine and cosine, as the reference computes them), the per-cell
+// test on the GPU.
+int solid_mask_device(silt_core* c, bool fine, uint8_t* mask, hipStream_t st) {
+  const int res = fine ? kFineRes : kWindowRes;
+  const float cell = fine ? kFineCell : kCoarseCell;
+  const float* org = fine ? c->fine_origin : c->origin;
+  std::vector<SolidDesc> d;
+  for (int pass = 0; pass < 2; pass++)
+    for (const silt_hydro_solid& s : c->solids) {
+      if ((s.contact != 0) != (pass == 1)) continue;
+      if (s.half_length <= 1.0e-4f || s.half_width <= 1.0e-4f) continue;
+      const float reach = hypotf(fmaxf(s.half_length, 0.0f), fmaxf(s.half_width, 0.0f)) + cell;
+      if (!isfinite(reach) || !isfinite(s.x) || !isfinite(s.z)) continue;
+      const float lo_i = floorf((s.x - reach - org[0]) / cell - 0.5f);
+      const float hi_i = ceilf((s.x + reach - org[0]) / cell - 0.5f);
+      const float lo_j = floorf((s.z - reach - org[1]) / cell - 0.5f);
+      const float hi_j = ceilf((s.z + reach - org[1]) / cell - 0.5f);
+      const float top = (float)(res - 1);
+      if (hi_i < 0.0f || hi_j < 0.0f || lo_i > top || lo_j > top) continue;
+      SolidDesc q{};
+      q.x = s.x;
+      q.z = s.z;
+      q.fwd_x = sinf(s.yaw);
+      q.fwd_z = cosf(s.yaw);
+      q.half_length = s.half_length;
+      q.half_width = s.half_width;
+      q.bottom_y = s.bottom_y;
+      q.top_y = s.top_y;
+      q.i0 = (int)fmaxf(lo_i, 0.0f);
+      q.i1 = (int)fminf(hi_i, top);
+      q.j0 = (int)fmaxf(lo_j, 0.0f);
+      q.j1 = (int)fminf(hi_j, top);
+      q.value = pass == 1 ? 1 : 2;
+      d.push_back(q);
+    }
+  if (!d.empty()) HIPCHK(hipMemcpyAsync(c->d_solids, d.data(), d.size() * sizeof(SolidDesc), hipMemcpyHostToDevice, st));
+  const float* water = fine ? c->f.water : c->c.water;
+  const float* bed = fine ? c->f.bed : c->c.bed;
+  core_solid_mask(water, bed, res, cell, org[0], org[1], c->d_solids, (int)d.size(), mask, st);
+  // The descriptors are read by the launch above before the next call
+  // rewrites them: the copy is ordered on the same stream.
+  return hipGetLastError() == hipSuccess ? 0 : SILT_EDEVICE;
+}
+
+// step_pair with walls (src/fluid_core.rs:7954-8125): the grid's mask from
+// the planes before either layer steps, then each layer softened (window
+// only), displaced out of the walls and stepped as a walled sheet, water on
+// the main stream and slurry on the second.  Both plan on the host.
+int step_pair_walled(silt_core* c, float dt, bool fine) {
+  const int res = fine ? kFineRes : kWindowRes;
+  const float cell = fine ? kFineCell : kCoarseCell;
+  Workspace* ws = workspace_for(res, 0);
+  Workspace* ws2 = workspace_for(res, 1);
+  if (!ws || !ws2) return SILT_ENOMEM;
+  uint8_t* mask = fine ? c->d_fmask : c->d_mask;
+  if (int rc = solid_mask_device(c, fine, mask, c->st)) return rc;
+  HIPCHK(hipEventRecord(c->fork, c->st));
+  HIPCHK(hipStreamWaitEvent(c->st2, c->fork, 0));
+  silt_sheet_opts o;
+  silt_sheet_opts_default(&o);
+  o.path = c->path;
+  o.row_split = 0;
+  o.sub_cap = fine ? c->fine_sub_cap : 8.0f;
+  const float nu_w = dynamic_viscosity(3, 1.0f);
+  SheetPlanes s = ws->planes();
+  s.h = fine ? c->f.water : c->c.water;
+  s.u = fine ? c->f.uw : c->c.uw;
+  s.v = fine ? c->f.vw : c->c.vw;
+  s.bed = fine ? c->f.bed : c->c.bed;
+  s.solid = mask;
+  if (!fine) core_soften(s.u, s.v, res, c->st);
+  displace_occupied(s.h, s.u, s.v, mask, res, s.tg, s.ts, fine ? fine_book(c, 0) : coarse_book(c, 0), c->st);
+  silt_drive dw{SILT_DRIVE_GRAVITY, 0.0f, 0.0f, 0.08f};
+  int rc = step_device_planes(s, res, cell, dt, dw, o, true, true, nu_w, c->st, fine ? fine_book(c, 0) : coarse_book(c, 0));
+  if (rc < 0) return rc;
+  SheetPlanes s2 = ws2->planes();
+  float* nu = fine ? c->nu_f : c->nu_c;
+  core_nu(nu, fine ? c->f.grain : c->c.grain, fine ? c->f.moisture : c->c.moisture, res * res, c->st2);
+  s2.h = fine ? c->f.slurry : c->c.slurry;
+  s2.u = fine ? c->f.us : c->c.us;
+  s2.v = fine ? c->f.vs : c->c.vs;
+  s2.bed = fine ? c->f.bed_s : c->c.bed;
+  s2.nu = nu;
+  s2.solid = mask;
+  if (!fine) core_soften(s2.u, s2.v, res, c->st2);
+  displace_occupied(s2.h, s2.u, s2.v, mask, res, s2.tg, s2.ts, fine ? fine_book(c, 1) : coarse_book(c, 1), c->st2);
+  silt_drive ds{SILT_DRIVE_GRAVITY, 0.0f, 0.0f, 0.05f};
+  rc = step_device_planes(s2, res, cell, dt, ds, o, true, false, 0.0f, c->st2, fine ? fine_book(c, 1) : coarse_book(c, 1));
+  HIPCHK(hipEventRecord(c->join, c->st2));
+  HIPCHK(hipStreamWaitEvent(c->st, c->join, 0));
+  return rc < 0 ? rc : 0;
+}
+
+// scour_contacts (src/fluid_core.rs:7364-7421) after the window's step.
+int scour(silt_core* c, float dt) {
+  if (!(dt > 0.0f) || c->solids.empty()) return 0;
+  if (int rc = solid_mask_device(c, false, c->d_mask, c->st)) return rc;
+  core_scour(c->c, c->d_mask, c->d_bed_drop, dt, c->d_eroded, c->d_hole, source_book(c) + 1, c->st);
+  return hipGetLastError() == hipSuccess ? 0 : SILT_EDEVICE;
+}
+
 int step_pair(silt_core* c, float dt, bool fine) {
   if (fine && !c->fine_on) return 0;
+  if (!c->solids.empty()) return step_pair_walled(c, dt, fine);
   int res = fine ? kFineRes : kWindowRes;
   float cell = fine ? kFineCell : kCoarseCell;
   Workspace* ws = workspace_for(res, 0);
@@ -522,7 +638,7 @@ int step_nested(silt_core* c, const std::vector<CorePush>& pushes, const PushLis
   shove(c, coarse_p, false);
   if ((rc = step_pair(c, dt, false))) return rc;
   peel(c, pl, false, true);
-  return 0;
+  return scour(c, dt);
 }
 
 SurGrid core_grid(const silt_core* c) {
@@ -610,7 +726,8 @@ int step_surrogate(silt_core* c, const std::vector<CorePush>& pushes, const Push
   c->sur_steps++;
   if (!judge) {
     c->last_mode = SILT_TIRE_SURROGATE;
-    return step_coarse(c, pushes, pl, dt, true, true, true);
+    int rc = step_coarse(c, pushes, pl, dt, true, true, true);
+    return rc ? rc : scour(c, dt);
   }
   float* p6[6];
   core_planes6(c, p6);
@@ -660,6 +777,108 @@ int step_surrogate(silt_core* c, const std::vector<CorePush>& pushes, const Push
   return 0;
 }
 
+// cylinder_segment_volume (src/fluid_core.rs:4606-4615).
+float cylinder_segment_volume(float depth, float radius, float width) {
+  const float d = clampf(depth, 0.0f, radius * 2.0f);
+  if (d <= 1.0e-4f || width <= 0.0f || radius <= 1.0e-3f) return 0.0f;
+  const float cc = clampf((radius - d) / radius, -1.0f, 1.0f);
+  const float area = radius * radius * acosf(cc) - (radius - d) * sqrtf(fmaxf(2.0f * radius * d - d * d, 0.0f));
+  return fmaxf(area, 0.0f) * width;
+}
+
+bool solid_contains_host(const silt_hydro_solid& s, float x, float z) {
+  const float dx = x - s.x, dz = z - s.z;
+  const float fwd_x = sinf(s.yaw), fwd_z = cosf(s.yaw);
+  const float right_x = fwd_z, right_z = -fwd_x;
+  const float along = dx * fwd_x + dz * fwd_z;
+  const float lat = dx * right_x + dz * right_z;
+  return fabsf(along) <= fmaxf(s.half_length, 0.0f) && fabsf(lat) <= fmaxf(s.half_width, 0.0f);
+}
+
+// hydro_load (src/fluid_core.rs:7164-7249) from host copies of the window's
+// water, velocity and bed and its wall mask: the hydrostatic and dynamic
+// push on the wall faces, and each solid's displaced water from the mean
+// surface of the wet ring around it (ring_surface, :7251-7286).
+silt_hydro_load hydro_load_host(const silt_core* c, const std::vector<uint8_t>& mask, const std::vector<float>& water,
+                                const std::vector<float>& uw, const std::vector<float>& vw,
+                                const std::vector<float>& bedp) {
+  silt_hydro_load L{};
+  const float cell = kCoarseCell;
+  const int res = kWindowRes;
+  for (uint8_t m : mask) {
+    if (m == 1) L.wheel_blocks++;
+    else if (m == 2) L.belly_blocks++;
+  }
+  static const int nb[4][2] = {{-1, 0}, {1, 0}, {0, -1}, {0, 1}};
+  float fx = 0.0f, fz = 0.0f;
+  for (int j = 0; j < res; j++)
+    for (int i = 0; i < res; i++) {
+      const size_t idx = (size_t)j * res + i;
+      if (mask[idx] != 0) continue;
+      const float h = water[idx];
+      if (h < 1.0e-4f || !isfinite(h)) continue;
+      for (int q = 0; q < 4; q++) {
+        const int ni = i + nb[q][0], nj = j + nb[q][1];
+        if (ni < 0 || nj < 0 || ni >= res || nj >= res) continue;
+        if (mask[(size_t)nj * res + ni] == 0) continue;
+        const float nx = (float)nb[q][0], nz = (float)nb[q][1];
+        const float hydro = 1000.0f * kGravity * h * h * 0.5f * cell;
+        fx += nx * hydro;
+        fz += nz * hydro;
+        const float vn = uw[idx] * nx + vw[idx] * nz;
+        if (vn > 0.0f) {
+          const float dyn_f = 0.5f * 1000.0f * vn * vn * h * cell;
+          fx += nx * dyn_f;
+          fz += nz * dyn_f;
+        }
+      }
+    }
+  float wheel_b = 0.0f, hull_b = 0.0f, displaced = 0.0f;
+  for (const silt_hydro_solid& s : c->solids) {
+    const float reach = fmaxf(s.half_length, s.half_width) + 0.55f;
+    const int i0 = (int)floorf((s.x - reach - c->origin[0]) / cell);
+    const int i1 = (int)floorf((s.x + reach - c->origin[0]) / cell);
+    const int j0 = (int)floorf((s.z - reach - c->origin[1]) / cell);
+    const int j1 = (int)floorf((s.z + reach - c->origin[1]) / cell);
+    float bed = 0.0f, wat = 0.0f, n = 0.0f;
+    for (int j = j0; j <= j1; j++)
+      for (int i = i0; i <= i1; i++) {
+        if (i < 0 || j < 0 || i >= res || j >= res) continue;
+        float x, z;
+        coarse_xz(c, i, j, &x, &z);
+        if (solid_contains_host(s, x, z)) continue;
+        if (hypotf(x - s.x, z - s.z) > reach) continue;
+        const size_t idx = (size_t)j * res + i;
+        if (water[idx] < 0.01f) continue;
+        bed += bedp[idx];
+        wat += water[idx];
+        n += 1.0f;
+      }
+    if (n < 1.0f) continue;
+    bed = bed / n;
+    wat = wat / n;
+    if (s.radius > 0.0f) {
+      const float vol = cylinder_segment_volume(wat, s.radius, s.width);
+      displaced += vol;
+      wheel_b += 1000.0f * kGravity * vol;
+    } else {
+      const float eta = bed + wat;
+      const float height = fmaxf(s.top_y - s.bottom_y, 0.0f);
+      const float sub = clampf(eta - s.bottom_y, 0.0f, height);
+      const float vol = (s.half_length * 2.0f) * (s.half_width * 2.0f) * sub * 0.36f;
+      displaced += vol;
+      hull_b += 1000.0f * kGravity * vol;
+    }
+  }
+  L.force_x = fx;
+  L.force_z = fz;
+  L.buoyancy_n = wheel_b + hull_b;
+  L.hull_buoyancy_n = hull_b;
+  L.wheel_buoyancy_n = wheel_b;
+  L.displaced_m3 = displaced;
+  return L;
+}
+
 // The ledger's volumes (FluidCore::ledger, src/fluid_core.rs:6821-6883) in
 // FP64 over the window and, while it is on, the 2 cm patch; with book, the
 // booked lines too, read in the same wait.
@@ -698,7 +917,7 @@ void layer_sums(const silt_fluid_ledger& l, const CoreBook& b, double tot[2], do
   tot[1] = l.slurry_m3 + l.frozen_slurry_m3;
   for (int k = 0; k < 2; k++)
     lines[k] = (b.coarse[k].residue - b.coarse[k].cut) * kCoarseArea +
-               (b.fine[k].residue - b.fine[k].cut) * kFineArea + b.handoff[k];
+               (b.fine[k].residue - b.fine[k].cut) * kFineArea + b.handoff[k] + b.source[k];
 }
 
 // After a caller edit, the base becomes the current totals less the lines
@@ -720,9 +939,11 @@ int ensure_base(silt_core* c) {
 int step_by_model(silt_core* c, const std::vector<CorePush>& pushes, const PushList& pl, float dt) {
   c->timed = 0;
   switch (c->tire_model) {
-    case SILT_TIRE_COARSE:
+    case SILT_TIRE_COARSE: {
       c->last_mode = SILT_TIRE_COARSE;
-      return step_coarse(c, pushes, pl, dt, true, false, false);
+      int rc = step_coarse(c, pushes, pl, dt, true, false, false);
+      return rc ? rc : scour(c, dt);
+    }
     case SILT_TIRE_SURROGATE:
       return step_surrogate(c, pushes, pl, dt);
     default:
@@ -760,7 +981,9 @@ silt_core* silt_core_create(float center_x, float center_z, int path) {
        dalloc(c, &c->d_acc, 4) && dalloc(c, &c->soil_bed, WN) && dalloc(c, &c->soil_placed, WN) &&
        dalloc(c, &c->soil_init, WN) && dalloc(c, &c->soil_rut, WN) && dalloc(c, &c->d_contacts, kMaxPush) &&
        dalloc(c, &c->d_stats, 2) && dalloc(c, &c->d_lstats, 4) && dalloc(c, &c->d_pub, 13 * WN) &&
-       dalloc(c, &c->d_book, 2 * kBookDoubles) &&
+       dalloc(c, &c->d_book, 2 * kBookDoubles) && dalloc(c, &c->d_solids, kMaxSolids) &&
+       dalloc(c, &c->d_mask, WN) && dalloc(c, &c->d_fmask, FN) && dalloc(c, &c->d_bed_drop, WN) &&
+       dalloc(c, &c->d_eroded, 1) && dalloc(c, &c->d_hole, 1) &&
        hipHostMalloc((void**)&c->h_stats, 2 * sizeof(SheetStats), hipHostMallocDefault) == hipSuccess;
   if (ok) {
     // Grain::Other everywhere (FluidCore::new, src/fluid_core.rs:4979-4980).
@@ -1036,6 +1259,7 @@ int silt_core_ledger_lines(silt_core* c, silt_core_lines* out) {
     out->cut_m3[k] = b.coarse[k].cut * kCoarseArea + b.fine[k].cut * kFineArea;
     out->residue_m3[k] = b.coarse[k].residue * kCoarseArea + b.fine[k].residue * kFineArea;
     out->handoff_m3[k] = b.handoff[k];
+    out->source_m3[k] = b.source[k];
     out->total_m3[k] = tot[k];
     out->gap_m3[k] = tot[k] - c->base[k] - lines[k];
   }
@@ -1337,6 +1561,46 @@ int silt_core_sample(silt_core* c, float x, float z, silt_ground_sample* out) {
   out->settled_dust_m = 0.0f;
   out->moisture = v[4];
   out->bed_m = v[5];
+  return SILT_OK;
+}
+
+int silt_core_set_hydro_solids(silt_core* c, const silt_hydro_solid* solids, size_t n) {
+  if (!c || (!solids && n)) return SILT_EINVAL;
+  if (n > (size_t)kMaxSolids) return SILT_EINVAL;
+  c->solids.assign(solids, solids + n);
+  return SILT_OK;
+}
+
+int silt_core_solid_mask(silt_core* c, int fine, uint8_t* out) {
+  if (!c || !out) return SILT_EINVAL;
+  if (fine && !c->fine_on) return SILT_EINVAL;
+  uint8_t* mask = fine ? c->d_fmask : c->d_mask;
+  if (solid_mask_device(c, fine != 0, mask, c->st)) return SILT_EDEVICE;
+  HIPCHK(hipMemcpyAsync(out, mask, fine ? FN : WN, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  return SILT_OK;
+}
+
+int silt_core_hydro_load(silt_core* c, silt_hydro_load* out) {
+  if (!c || !out) return SILT_EINVAL;
+  std::vector<uint8_t> mask(WN, 0);
+  std::vector<float> w(WN), uw(WN), vw(WN), bed(WN);
+  if (!c->solids.empty()) {
+    if (solid_mask_device(c, false, c->d_mask, c->st)) return SILT_EDEVICE;
+    HIPCHK(hipMemcpyAsync(mask.data(), c->d_mask, WN, hipMemcpyDeviceToHost, c->st));
+  }
+  HIPCHK(hipMemcpyAsync(w.data(), c->c.water, WN * 4, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipMemcpyAsync(uw.data(), c->c.uw, WN * 4, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipMemcpyAsync(vw.data(), c->c.vw, WN * 4, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipMemcpyAsync(bed.data(), c->c.bed, WN * 4, hipMemcpyDeviceToHost, c->st));
+  double eroded = 0.0;
+  unsigned hole = 0;
+  HIPCHK(hipMemcpyAsync(&eroded, c->d_eroded, sizeof eroded, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipMemcpyAsync(&hole, c->d_hole, sizeof hole, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  *out = hydro_load_host(c, mask, w, uw, vw, bed);
+  out->eroded_m3 = (float)eroded;
+  memcpy(&out->hole_m, &hole, 4);
   return SILT_OK;
 }
 

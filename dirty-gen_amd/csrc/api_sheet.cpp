@@ -53,6 +53,18 @@ int run_plan(const SheetPlanes& s, const StepPlan& plan, int res, float cell, fl
   if (plan.boxes.empty() || plan.nsub <= 0) return 0;
   float dts = dt / (float)plan.nsub;
   bool wind = d.kind == SILT_DRIVE_WIND;
+  if (s.solid) {
+    // Walls: the stage kernels, whatever the path (the streaming kernel has
+    // no walled form).  The parallel-kick form where the reference takes it.
+    StepArgs a = make_args(res, cell, dts, d, o.eps, 0, gate_uniform, nu_uniform, nu0);
+    a.book = book;
+    for (int k = 0; k < plan.nsub; k++)
+      for (const Box& b : plan.boxes) {
+        bool par = !wind && o.row_split != 0 && b.j1 - b.j0 >= 47 && b.i1 - b.i0 >= 31;
+        exact_substep_walled(s, b, a, par, st);
+      }
+    return plan.nsub;
+  }
   auto gs_for = [&](const Box& b) {
     if (o.confine == SILT_CONFINE_JACOBI || d.vorticity <= 0.0f) return false;
     return !reference_uses_jacobi(b, cell, wind, o.row_split != 0);
@@ -102,7 +114,7 @@ int step_device_planes(const SheetPlanes& s, int res, float cell, float dt, cons
   HIPCHK(hipMemcpyAsync(bh.data(), s.bed, fb, hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
   StepPlan plan = plan_step(hh.data(), uh.data(), vh.data(), bh.data(), res, cell, dt, d.kind == SILT_DRIVE_WIND, d.x,
-                            d.z, o.eps, o.sub_cap, o.pace);
+                            d.z, o.eps, o.sub_cap, o.pace, s.solid != nullptr);
   if (plan.boxes.empty()) return 0;
   return run_plan(s, plan, res, cell, dt, d, o, gate_uniform, nu_uniform, nu0, st, book);
 }
@@ -181,6 +193,105 @@ int silt_step_sheet_ex(float* h, float* u, float* v, const float* bed, const flo
   if (hipStreamSynchronize(st) != hipSuccess) return SILT_EDEVICE;
   if (hipGetLastError() != hipSuccess) return SILT_EDEVICE;
   return nsub;
+}
+
+int silt_step_sheet_walled(float* h, float* u, float* v, const float* bed, const float* nu, const uint8_t* gate,
+                           const uint8_t* solid, size_t len, size_t res, float cell, float dt, silt_drive drive,
+                           const silt_sheet_opts* opts) {
+  if (!h || !u || !v || !bed || !nu || !gate || !solid) return SILT_EINVAL;
+  if (drive.kind != SILT_DRIVE_GRAVITY && drive.kind != SILT_DRIVE_WIND) return SILT_EINVAL;
+  silt_sheet_opts o;
+  if (opts)
+    o = *opts;
+  else
+    silt_sheet_opts_default(&o);
+  if (!valid_opts(o)) return SILT_EINVAL;
+  if (!(dt > 0.0f) || res == 0 || len < res * res) return 0;
+  size_t n = res * res;
+  StepPlan plan = plan_step(h, u, v, bed, (int)res, cell, dt, drive.kind == SILT_DRIVE_WIND, drive.x, drive.z, o.eps,
+                            o.sub_cap, o.pace, true);
+  if (plan.boxes.empty()) return 0;
+  bool gate_uniform = true;
+  for (size_t k = 0; k < n && gate_uniform; k++) gate_uniform = gate[k] == 1;
+  bool nu_uniform = true;
+  for (size_t k = 1; k < n && nu_uniform; k++) nu_uniform = memcmp(&nu[k], &nu[0], sizeof(float)) == 0;
+  Workspace* ws = workspace_for((int)res);
+  if (!ws) return SILT_ENOMEM;
+  std::lock_guard<std::mutex> lock(ws->mu);
+  SheetPlanes s = ws->planes();
+  s.solid = ws->solid_stage();
+  hipStream_t st = ws->stream;
+  size_t fb = n * sizeof(float);
+  HIPCHK(hipMemcpyAsync(s.h, h, fb, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(s.u, u, fb, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(s.v, v, fb, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync((void*)s.bed, bed, fb, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync((void*)s.nu, nu, fb, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync((void*)s.gate, gate, n, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync((void*)s.solid, solid, n, hipMemcpyHostToDevice, st));
+  int nsub = run_plan(s, plan, (int)res, cell, dt, drive, o, gate_uniform, nu_uniform, nu[0], st);
+  HIPCHK(hipMemcpyAsync(h, s.h, fb, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(u, s.u, fb, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(v, s.v, fb, hipMemcpyDeviceToHost, st));
+  if (hipStreamSynchronize(st) != hipSuccess) return SILT_EDEVICE;
+  if (hipGetLastError() != hipSuccess) return SILT_EDEVICE;
+  return nsub;
+}
+
+int silt_step_sheet_walled_device(float* h, float* u, float* v, const float* bed, const float* nu, float nu_uniform,
+                                  const uint8_t* gate, const uint8_t* solid, size_t res, float cell, float dt,
+                                  silt_drive drive, const silt_sheet_opts* opts, void* hip_stream) {
+  bool uniform = isfinite(nu_uniform);
+  if (!h || !u || !v || !bed || !solid || (!nu && !uniform)) return SILT_EINVAL;
+  if (drive.kind != SILT_DRIVE_GRAVITY && drive.kind != SILT_DRIVE_WIND) return SILT_EINVAL;
+  silt_sheet_opts o;
+  if (opts)
+    o = *opts;
+  else
+    silt_sheet_opts_default(&o);
+  if (!valid_opts(o)) return SILT_EINVAL;
+  if (!(dt > 0.0f) || res == 0) return 0;
+  Workspace* ws = workspace_for((int)res);
+  if (!ws) return SILT_ENOMEM;
+  std::lock_guard<std::mutex> lock(ws->mu);
+  SheetPlanes s = ws->planes();
+  s.h = h;
+  s.u = u;
+  s.v = v;
+  s.bed = bed;
+  s.nu = uniform ? s.nu : nu;
+  s.gate = gate ? gate : s.gate;
+  s.solid = solid;
+  int nsub = step_device_planes(s, (int)res, cell, dt, drive, o, gate == nullptr, uniform, uniform ? nu_uniform : 0.0f,
+                                (hipStream_t)hip_stream);
+  if (nsub < 0) return nsub;
+  if (hipStreamSynchronize((hipStream_t)hip_stream) != hipSuccess) return SILT_EDEVICE;
+  if (hipGetLastError() != hipSuccess) return SILT_EDEVICE;
+  return nsub;
+}
+
+int silt_displace_occupied(float* h, float* u, float* v, const uint8_t* solid, size_t len, size_t res) {
+  if (!h || !u || !v || !solid) return SILT_EINVAL;
+  if (res == 0 || len < res * res) return 0;
+  size_t n = res * res;
+  Workspace* ws = workspace_for((int)res);
+  if (!ws) return SILT_ENOMEM;
+  std::lock_guard<std::mutex> lock(ws->mu);
+  SheetPlanes s = ws->planes();
+  hipStream_t st = ws->stream;
+  size_t fb = n * sizeof(float);
+  uint8_t* dsolid = ws->solid_stage();
+  HIPCHK(hipMemcpyAsync(s.h, h, fb, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(s.u, u, fb, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(s.v, v, fb, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(dsolid, solid, n, hipMemcpyHostToDevice, st));
+  displace_occupied(s.h, s.u, s.v, dsolid, (int)res, s.tg, s.ts, nullptr, st);
+  HIPCHK(hipMemcpyAsync(h, s.h, fb, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(u, s.u, fb, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(v, s.v, fb, hipMemcpyDeviceToHost, st));
+  if (hipStreamSynchronize(st) != hipSuccess) return SILT_EDEVICE;
+  if (hipGetLastError() != hipSuccess) return SILT_EDEVICE;
+  return SILT_OK;
 }
 
 int silt_step_sheet(float* h, float* u, float* v, const float* bed, const float* nu, const uint8_t* gate, size_t len,

@@ -120,6 +120,28 @@ void core_publish(const CorePlanes& c, const float* soil_bed, const float* soil_
                   float* ground_rut4, float* film2, float* flow4, hipStream_t st);
 // moisture (max of the sheet's and the soil's wetness), wet_edge and
 // foam_crest (0, placed + film excess) maps; wetness / placed may be null.
+// One hydro solid as the wall mask tests it (solid_mask, src/fluid_core.rs:
+// 7288-7362): the rectangle's centre, axes (sin / cos of the yaw, from the
+// host), half sizes, the heights between which it is a wall, the cell range
+// its bounding circle covers, and the mask value (2 wall, 1 contact).
+struct SolidDesc {
+  float x, z, fwd_x, fwd_z, half_length, half_width, bottom_y, top_y;
+  int i0, i1, j0, j1;
+  int value;
+  int pad[3];
+};
+constexpr int kMaxSolids = 64;
+
+// The wall mask of one grid from its water and bed: walls first, then
+// contacts (descs in that order).
+void core_solid_mask(const float* water, const float* bed, int res, float cell, float ox, float oz,
+                     const SolidDesc* descs, int n, uint8_t* mask, hipStream_t st);
+// scour_contacts (src/fluid_core.rs:7364-7421) on the window: bed into
+// slurry beside contacts.  acc[0]: FP64 eroded volume (m^3), acc[1]: the
+// deepest drop (float bits); source: the slurry volume the ledger books.
+void core_scour(const CorePlanes& c, const uint8_t* mask, float* bed_drop, float dt, double* eroded,
+                unsigned* hole_bits, double* source, hipStream_t st);
+
 void core_publish_surface(const CorePlanes& c, const float* wetness, const float* placed, int res, float* moisture1,
                           float* wet_edge1, float* foam2, hipStream_t st);
 

@@ -895,6 +895,95 @@ void core_publish(const CorePlanes& c, const float* soil_bed, const float* soil_
                      ground_rut4, film2, flow4, n);
 }
 
+// ---- walls: the mask and the scour ------------------------------------------
+
+// solid_overlaps (src/fluid_core.rs:4584-4604) at a cell centre.
+__device__ inline bool solid_overlaps_at(const SolidDesc& s, float x, float z, float eta, float bed) {
+  const float dx = x - s.x, dz = z - s.z;
+  const float right_x = s.fwd_z, right_z = -s.fwd_x;
+  const float along = dx * s.fwd_x + dz * s.fwd_z;
+  const float lat = dx * right_x + dz * right_z;
+  if (!(fabsf(along) <= fmaxf(s.half_length, 0.0f) && fabsf(lat) <= fmaxf(s.half_width, 0.0f))) return false;
+  return eta > s.bottom_y + 1.0e-4f && bed < s.top_y;
+}
+
+__global__ void k_solid_mask(const float* __restrict__ water, const float* __restrict__ bed, int res, float cell,
+                             float ox, float oz, const SolidDesc* __restrict__ descs, int n,
+                             uint8_t* __restrict__ mask) {
+  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x), j = (int)(blockIdx.y * blockDim.y + threadIdx.y);
+  if (i >= res || j >= res) return;
+  const size_t idx = (size_t)j * res + i;
+  const float x = ox + ((float)i + 0.5f) * cell;
+  const float z = oz + ((float)j + 0.5f) * cell;
+  // The free surface beside the cell: the highest of it and its 4-neighbours.
+  float eta = bed[idx] + water[idx];
+  if (i > 0) eta = fmaxf(eta, bed[idx - 1] + water[idx - 1]);
+  if (i + 1 < res) eta = fmaxf(eta, bed[idx + 1] + water[idx + 1]);
+  if (j > 0) eta = fmaxf(eta, bed[idx - res] + water[idx - res]);
+  if (j + 1 < res) eta = fmaxf(eta, bed[idx + res] + water[idx + res]);
+  uint8_t m = 0;
+  for (int k = 0; k < n; k++) {
+    const SolidDesc s = descs[k];
+    if (i < s.i0 || i > s.i1 || j < s.j0 || j > s.j1) continue;
+    if (solid_overlaps_at(s, x, z, eta, bed[idx])) m = (uint8_t)s.value;
+  }
+  mask[idx] = m;
+}
+
+// scour_contacts: a wet open cell touching a contact (value 1) whose
+// shear 1000 * 0.03 * |u|^2 exceeds 1.5 Pa lifts bed into the slurry, at
+// most 0.08 m in all.  Each cell reads and writes only itself (and the mask).
+__global__ void k_scour(CorePlanes c, const uint8_t* __restrict__ mask, float* __restrict__ bed_drop, float dt,
+                        double* eroded, unsigned* hole_bits, double* source) {
+  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x), j = (int)(blockIdx.y * blockDim.y + threadIdx.y);
+  const int res = kWindowRes;
+  double gain = 0.0;
+  if (i >= 1 && j >= 1 && i < res - 1 && j < res - 1) {
+    const size_t idx = (size_t)j * res + i;
+    if (mask[idx] == 0 && !(c.water[idx] < 0.02f) &&
+        (mask[idx - 1] == 1 || mask[idx + 1] == 1 || mask[idx - res] == 1 || mask[idx + res] == 1)) {
+      const float speed = hypotf(c.uw[idx], c.vw[idx]);
+      if (isfinite(speed)) {
+        const float tau = 1000.0f * 0.03f * speed * speed;
+        const float tau_c = 1.5f;
+        if (tau > tau_c) {
+          const float rate = fminf(0.0035f * (tau / tau_c - 1.0f), 0.012f);
+          const float room = fmaxf(0.08f - bed_drop[idx], 0.0f);
+          const float dh = fminf(rate * dt, room);
+          if (dh > 0.0f) {
+            const float was = c.slurry[idx];
+            c.bed[idx] = c.bed[idx] - dh;
+            c.slurry[idx] = was + dh;
+            c.us[idx] = c.uw[idx];
+            c.vs[idx] = c.vw[idx];
+            c.grain[idx] = 1;  // Grain::Silt
+            c.moisture[idx] = fmaxf(c.moisture[idx], 0.9f);
+            const float drop = bed_drop[idx] + dh;
+            bed_drop[idx] = drop;
+            atomicAdd(eroded, (double)(dh * (kCoarseCell * kCoarseCell)));
+            atomicMax(hole_bits, __float_as_uint(drop));
+            gain = ((double)(was + dh) - (double)fmaxf(was, 0.0f)) * kCoarseArea;
+          }
+        }
+      }
+    }
+  }
+  if (gain != 0.0) atomicAdd(source, gain);
+}
+
+void core_solid_mask(const float* water, const float* bed, int res, float cell, float ox, float oz,
+                     const SolidDesc* descs, int n, uint8_t* mask, hipStream_t st) {
+  dim3 b(32, 8);
+  hipLaunchKernelGGL(k_solid_mask, grid2(res, res, b), b, 0, st, water, bed, res, cell, ox, oz, descs, n, mask);
+}
+
+void core_scour(const CorePlanes& c, const uint8_t* mask, float* bed_drop, float dt, double* eroded,
+                unsigned* hole_bits, double* source, hipStream_t st) {
+  dim3 b(32, 8);
+  hipLaunchKernelGGL(k_scour, grid2(kWindowRes, kWindowRes, b), b, 0, st, c, mask, bed_drop, dt, eroded, hole_bits,
+                     source);
+}
+
 void core_publish_surface(const CorePlanes& c, const float* wetness, const float* placed, int res, float* moisture1,
                           float* wet_edge1, float* foam2, hipStream_t st) {
   const int n = res * res;

@@ -226,7 +226,7 @@ bool reference_uses_jacobi(const Box& b, float cell, int wind, bool row_split) {
 }
 
 StepPlan plan_step(const float* h, const float* u, const float* v, const float* bed, int res, float cell, float dt,
-                   int wind, float wx, float wz, float eps, float sub_cap, int pace) {
+                   int wind, float wx, float wz, float eps, float sub_cap, int pace, bool walled) {
   StepPlan plan;
   if (!(dt > 0.0f) || res <= 0) return plan;
   // wet_sum (src/fluid_core.rs:1305-1319), eight lanes then the remainder.
@@ -238,7 +238,9 @@ StepPlan plan_step(const float* h, const float* u, const float* v, const float* 
   for (int k = 0; k < 8; k++) sum += lanes[k];
   for (size_t c = full; c < n; c++) sum += h[c] > eps ? h[c] : 0.0f;
   if (sum <= eps) return plan;
-  bool open_fine = !wind && cell <= kFineCell + 1.0e-6f;
+  // A wall keeps the wet-set boxes and the substep loop on every grid
+  // (step_sheet_scratch :1694-1696) and Courant 0.42 for wind too (:1735-1738).
+  bool open_fine = !walled && !wind && cell <= kFineCell + 1.0e-6f;
   plan.fine_open = open_fine && pace == 0;
   bool whole_wet = open_fine && (pace == 2 || pace == 3);
   bool awake = open_fine && !whole_wet && !plan.fine_open;
@@ -260,7 +262,7 @@ StepPlan plan_step(const float* h, const float* u, const float* v, const float* 
           if (s2 > max_s2) max_s2 = s2;
         }
     max_s = fmaxf(max_s, sqrtf(max_s2));
-    float courant = wind ? 0.85f : 0.42f;
+    float courant = wind && !walled ? 0.85f : 0.42f;
     float q = ceilf((dt * max_s) / (courant * fmaxf(cell, 1.0e-3f)));
     plan.nsub = (int)clampf(q, 1.0f, cap);
   }

@@ -35,7 +35,7 @@ std::vector<Box> awake_boxes_from_hot(const std::vector<uint8_t>& hot, int res);
 // Whole plan from host copies of the planes.  Returns an empty plan when the
 // reference's early returns fire (dry sheet, dt <= 0).
 StepPlan plan_step(const float* h, const float* u, const float* v, const float* bed, int res, float cell, float dt,
-                   int wind, float wx, float wz, float eps, float sub_cap, int pace);
+                   int wind, float wx, float wz, float eps, float sub_cap, int pace, bool walled = false);
 
 // True when the reference sweeps confinement from a snapshot (Jacobi) on
 // this box: the 2 cm gravity sheet, or a box of >= 48 x 32 cells when row

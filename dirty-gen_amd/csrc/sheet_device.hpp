@@ -62,6 +62,8 @@ struct SheetPlanes {
   float* sv;
   float* bsrc;
   float* h1;
+  // Wall mask (nonzero = wall) of the walled sheet, or null.
+  const uint8_t* solid = nullptr;
 };
 
 // Rust f32::clamp: NaN passes through.
@@ -120,5 +122,11 @@ void exact_substep(const SheetPlanes& s, const Box& b, const StepArgs& a, bool d
                    hipStream_t st);
 void exact_transport(const SheetPlanes& s, const Box& b, const StepArgs& a, float dt, hipStream_t st);
 void exact_dry_cut(const SheetPlanes& s, const Box& b, const StepArgs& a, hipStream_t st);
+// One substep of one box beside the walls s.solid marks (par: the parallel
+// kicks, then the confinement sweep).
+void exact_substep_walled(const SheetPlanes& s, const Box& b, const StepArgs& a, bool par, hipStream_t st);
+// displace_occupied over a whole grid (share, mode: scratch planes).
+void displace_occupied(float* h, float* u, float* v, const uint8_t* solid, int res, float* share, float* mode,
+                       SheetTotals* book, hipStream_t st);
 
 }  // namespace silt

@@ -1,5 +1,5 @@
 // Exact sheet substep for gfx950: one kernel per stage of the reference
-// substep (src/fluid_core.rs:1879-2082, open sheet), each stage over one work
+// substep (src/fluid_core.rs:1879-2082, open and walled), each stage over one work
 // box plus its 2-cell rim.  Arithmetic follows the reference operation by
 // operation (no contraction: the file is built with -ffp-contract=off; IEEE
 // division), so on the same inputs it reproduces the FP32 reference bit for bit
@@ -587,6 +587,392 @@ __global__ void k_dry_cut(float* __restrict__ h, float* __restrict__ u, float* _
   if (a.book) book_wave(&a.book->cut, cut);
 }
 
+// ---- walled sheet (hydro solids) -------------------------------------------
+// The stages the substep adds beside walls (src/fluid_core.rs:1879-2082 with
+// solid = Some): the surface lift and the no-slip ghost before the forces,
+// the kick's mobility bump beside a wall, the redirect, the no-penetration
+// filter and the ghost again after the damping, and the transport that no
+// face between a wall and the sheet crosses.  Every stage keeps the
+// reference's operation order; the collect-then-write stages run in place
+// because each reads only cells it does not write.
+
+__device__ inline bool grid_cell(int i, int j, int res) { return i >= 0 && j >= 0 && i < res && j < res; }
+
+constexpr int kNb4[4][2] = {{-1, 0}, {1, 0}, {0, -1}, {0, 1}};
+
+// lift_solid_surface (:4225-4276): a solid box cell's eta becomes the mean
+// eta of the wet open cells within two cells.  It reads open cells only and
+// writes solid cells only.
+__global__ void k_lift_solid(float* __restrict__ eta, const float* __restrict__ h, const uint8_t* __restrict__ solid,
+                             Region r, int res, float eps) {
+  int i, j;
+  if (!region_cell(r, &i, &j)) return;
+  const size_t idx = (size_t)j * res + i;
+  if (!solid[idx]) return;
+  float acc = 0.0f, n = 0.0f;
+  for (int dj = -2; dj <= 2; dj++)
+    for (int di = -2; di <= 2; di++) {
+      if (di == 0 && dj == 0) continue;
+      const int ni = i + di, nj = j + dj;
+      if (!grid_cell(ni, nj, res)) continue;
+      const size_t nidx = (size_t)nj * res + ni;
+      if (solid[nidx] || h[nidx] <= eps) continue;
+      acc = acc + eta[nidx];
+      n = n + 1.0f;
+    }
+  if (n > 0.0f) eta[idx] = acc / n;
+}
+
+// write_noslip_ghost (:4278-4326): a solid box cell's velocity becomes minus
+// the mean of its wet open 4-neighbours' (0 without one).
+__global__ void k_noslip_ghost(float* __restrict__ u, float* __restrict__ v, const float* __restrict__ h,
+                               const uint8_t* __restrict__ solid, Region r, int res, float eps) {
+  int i, j;
+  if (!region_cell(r, &i, &j)) return;
+  const size_t idx = (size_t)j * res + i;
+  if (!solid[idx]) return;
+  float au = 0.0f, av = 0.0f, n = 0.0f;
+  for (int k = 0; k < 4; k++) {
+    const int ni = i + kNb4[k][0], nj = j + kNb4[k][1];
+    if (!grid_cell(ni, nj, res)) continue;
+    const size_t nidx = (size_t)nj * res + ni;
+    if (solid[nidx] || h[nidx] <= eps) continue;
+    au = au + u[nidx];
+    av = av + v[nidx];
+    n = n + 1.0f;
+  }
+  u[idx] = n > 0.0f ? -au / n : 0.0f;
+  v[idx] = n > 0.0f ? -av / n : 0.0f;
+}
+
+// solid_near with radius 1 (:4565-4582).
+__device__ inline bool solid_near1(const uint8_t* solid, int res, int i, int j) {
+  for (int dj = -1; dj <= 1; dj++)
+    for (int di = -1; di <= 1; di++) {
+      if (di == 0 && dj == 0) continue;
+      const int ni = i + di, nj = j + dj;
+      if (grid_cell(ni, nj, res) && solid[(size_t)nj * res + ni]) return true;
+    }
+  return false;
+}
+
+// The gravity kick beside walls (:2118-2124): mobility at least 1 on a deep
+// cell next to a wall.
+__device__ inline void kick_walled(const float* h, float* u, float* v, const float* nu, const float* p,
+                                   const uint8_t* solid, int res, int i, int j, const StepArgs& a) {
+  const size_t idx = (size_t)j * res + i;
+  float px, pz;
+  grad_at(p, res, i, j, a.cell, &px, &pz);
+  float mob = a.nu_uniform ? a.mob0 : sheet_mobility(nu[idx]);
+  if (h[idx] > 0.08f && solid_near1(solid, res, i, j)) mob = fmaxf(mob, 1.0f);
+  u[idx] = u[idx] + -kGravity * px * mob * a.dt;
+  v[idx] = v[idx] + -kGravity * pz * mob * a.dt;
+}
+
+// force_gravity_solid_par's kicks (:2156-2185), every wet box cell at once.
+__global__ void k_kick_walled(const float* __restrict__ h, float* __restrict__ u, float* __restrict__ v,
+                              const float* __restrict__ nu, const float* __restrict__ p,
+                              const uint8_t* __restrict__ solid, Region r, StepArgs a) {
+  int i, j;
+  if (!region_cell(r, &i, &j)) return;
+  if (h[(size_t)j * a.res + i] <= a.eps) return;
+  kick_walled(h, u, v, nu, p, solid, a.res, i, j, a);
+}
+
+// The forward sweep beside walls, one workgroup, wavefronts t = di + 2 dj
+// (the schedule of k_force_gauss_seidel): kick (with the wall bump) and
+// confinement cell by cell (force_gravity_solid :2101-2128), or, with kick
+// 0, the confinement pass that follows the parallel kicks (:2187-2199).
+__global__ void k_sweep_walled(const float* __restrict__ h, float* u, float* v, const float* __restrict__ nu,
+                               const float* __restrict__ p, const uint8_t* __restrict__ solid, Box b, StepArgs a,
+                               int kick) {
+  const int res = a.res;
+  const int W = b.i1 - b.i0 + 1, H = b.j1 - b.j0 + 1;
+  const int tmax = (W - 1) + 2 * (H - 1);
+  for (int t = 0; t <= tmax; t++) {
+    const int dj_lo = t - (W - 1) > 0 ? (t - (W - 1) + 1) / 2 : 0;
+    const int dj_hi = t / 2 < H - 1 ? t / 2 : H - 1;
+    for (int dj = dj_lo + (int)threadIdx.x; dj <= dj_hi; dj += (int)blockDim.x) {
+      const int i = b.i0 + (t - 2 * dj), j = b.j0 + dj;
+      if (h[(size_t)j * res + i] <= a.eps) continue;
+      if (kick) kick_walled(h, u, v, nu, p, solid, res, i, j, a);
+      if (a.vort > 0.0f) confine_at(u, v, u, v, res, i, j, a.cell, a.dt, a.vort);
+    }
+    __syncthreads();
+  }
+}
+
+// redirect_into_walls (:4446-4563), first half: each wet open box cell with
+// h >= 0.02 walks its four walls in order and books, per wall, the share
+// each open flank gets (share[k], k the wall's neighbour index) and which
+// flanks take it (bits 2k and 2k+1 of flanks, as a small float), and its own
+// removed depth.
+__global__ void k_redirect_give(const float* __restrict__ h, const float* __restrict__ u,
+                                const float* __restrict__ v, const uint8_t* __restrict__ solid,
+                                const float* __restrict__ bed, Region r, StepArgs a, float* __restrict__ removed,
+                                float* __restrict__ s0, float* __restrict__ s1, float* __restrict__ s2,
+                                float* __restrict__ s3, float* __restrict__ flanks) {
+  int i, j;
+  if (!region_cell(r, &i, &j)) return;
+  const int res = a.res;
+  const size_t idx = (size_t)j * res + i;
+  float share[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+  int bits = 0;
+  float rem = 0.0f;
+  const float hv = h[idx];
+  if (!solid[idx] && !(hv < 0.02f)) {
+    for (int k = 0; k < 4; k++) {
+      const int di = kNb4[k][0], dj = kNb4[k][1];
+      const int ni = i + di, nj = j + dj;
+      if (!grid_cell(ni, nj, res) || !solid[(size_t)nj * res + ni]) continue;
+      float vn = u[idx] * (float)di + v[idx] * (float)dj;
+      const int ui = i - di, uj = j - dj;
+      if (grid_cell(ui, uj, res)) {
+        const size_t uidx = (size_t)uj * res + ui;
+        if (!solid[uidx]) {
+          const float feed = u[uidx] * (float)di + v[uidx] * (float)dj;
+          if (feed > vn) vn = feed;
+        }
+      }
+      if (vn < 0.05f) continue;
+      int nd = 0, kb = 0;
+      float flank = 0.0f;
+      for (int q = 0; q < 2; q++) {
+        const int pi = di != 0 ? 0 : (q == 0 ? -1 : 1), pj = di != 0 ? (q == 0 ? -1 : 1) : 0;
+        const int qi = i + pi, qj = j + pj;
+        if (!grid_cell(qi, qj, res)) continue;
+        const size_t qq = (size_t)qj * res + qi;
+        if (solid[qq]) continue;
+        if (h[qq] <= 1.0e-6f && bed[qq] > bed[idx] + hv + 1.0e-3f) continue;
+        kb |= 1 << q;
+        flank = flank + h[qq];
+        nd++;
+      }
+      if (nd == 0) continue;
+      const float excess = hv - flank / (float)nd;
+      if (excess < 2.0e-3f) continue;
+      const float room = hv - rem;
+      if (room < 1.0e-6f) continue;
+      const float frac = fminf(vn * a.dt / a.cell, 0.45f);
+      const float dh = fminf(fminf(excess, hv * frac), room);
+      if (dh < 1.0e-8f) continue;
+      rem = rem + dh;
+      share[k] = dh / (float)nd;
+      bits |= kb << (2 * k);
+    }
+  }
+  removed[idx] = rem;
+  s0[idx] = share[0];
+  s1[idx] = share[1];
+  s2[idx] = share[2];
+  s3[idx] = share[3];
+  flanks[idx] = (float)bits;
+}
+
+// Second half, over the box and a 1-cell rim: each cell adds what its
+// neighbours sent it in the reference's order (donors in raster order, each
+// donor's walls in neighbour order), then h = max(h - removed + added, 0).
+// With a ledger each cell's change is booked as residue.
+__global__ void k_redirect_take(float* __restrict__ h, Region rim, Box b, StepArgs a,
+                                const float* __restrict__ removed, const float* __restrict__ s0,
+                                const float* __restrict__ s1, const float* __restrict__ s2,
+                                const float* __restrict__ s3, const float* __restrict__ flanks) {
+  int i, j;
+  const bool in = region_cell(rim, &i, &j);
+  double moved = 0.0;
+  if (in) {
+    const int res = a.res;
+    const size_t idx = (size_t)j * res + i;
+    const float* sh[4] = {s0, s1, s2, s3};
+    float add = 0.0f;
+    // Donor at (i + ox, j + oz): the walls k0, k1 whose flank bit q is this cell.
+    auto from = [&](int ox, int oz, int k0, int k1, int q) {
+      const int di = i + ox, dj = j + oz;
+      if (!in_box(b, di, dj)) return;
+      const size_t d = (size_t)dj * res + di;
+      const int bits = (int)flanks[d];
+      if (bits & (1 << (2 * k0 + q))) add = add + sh[k0][d];
+      if (bits & (1 << (2 * k1 + q))) add = add + sh[k1][d];
+    };
+    from(0, -1, 0, 1, 1);  // the donor above: this cell is the +z flank of its x walls
+    from(-1, 0, 2, 3, 1);  // the donor to the left: +x flank of its z walls
+    from(1, 0, 2, 3, 0);   // the donor to the right: -x flank of its z walls
+    from(0, 1, 0, 1, 0);   // the donor below: -z flank of its x walls
+    const float rem = in_box(b, i, j) ? removed[idx] : 0.0f;
+    if (rem > 0.0f || add > 0.0f) {
+      const float was = h[idx];
+      const float now = fmaxf(was - rem + add, 0.0f);
+      h[idx] = now;
+      moved = (double)now - (double)fmaxf(was, 0.0f);
+    }
+  }
+  if (a.book) book_wave(&a.book->residue, moved);
+}
+
+// block_into_solid (:4328-4380): the into-wall part of a wet open cell's
+// velocity goes, wall by wall.
+__global__ void k_block_into_solid(float* __restrict__ u, float* __restrict__ v, const float* __restrict__ h,
+                                   const uint8_t* __restrict__ solid, Region r, int res, float eps) {
+  int i, j;
+  if (!region_cell(r, &i, &j)) return;
+  const size_t idx = (size_t)j * res + i;
+  if (solid[idx] || h[idx] <= eps) return;
+  float ux = u[idx], vx = v[idx];
+  bool hit = false;
+  for (int k = 0; k < 4; k++) {
+    const int ni = i + kNb4[k][0], nj = j + kNb4[k][1];
+    if (!grid_cell(ni, nj, res) || !solid[(size_t)nj * res + ni]) continue;
+    const float nx = (float)kNb4[k][0], nz = (float)kNb4[k][1];
+    const float vn = ux * nx + vx * nz;
+    if (vn > 0.0f) {
+      ux = ux - vn * nx;
+      vx = vx - vn * nz;
+      hit = true;
+    }
+  }
+  if (hit) {
+    u[idx] = ux;
+    v[idx] = vx;
+  }
+}
+
+// move_edge_bufs (:4178-4223) as a flux of one edge: no face next to a wall
+// carries water; the dry-bank test always applies.
+__device__ inline float wall_edge_dh(const uint8_t* solid, size_t d, size_t rcv, float hd, float hr, float remd,
+                                     uint8_t gate_r, float bed_d, float bed_r, float speed, float dt, float cell) {
+  if (solid[d] || solid[rcv]) return 0.0f;
+  return edge_dh(hd, hr, remd, gate_r, true, bed_d, bed_r, speed, dt, cell);
+}
+
+// transport with solids (:3894-3977) for one cell: the serial edge loop's
+// books of its two box-interior edges, as transport_cell keeps them.
+__device__ inline float transport_cell_walled(const float* hin, const float* vel, const uint8_t* gate,
+                                              const uint8_t* solid, const float* bed, size_t idx, size_t stride,
+                                              int pos, int lo, int hi, float dt, float cell, bool gate_uniform) {
+  auto gate_of = [&](size_t k) -> uint8_t { return gate_uniform ? (uint8_t)1 : gate[k]; };
+  auto fl = [&](size_t k) -> float {
+    const float speed = 0.5f * (vel[k] + vel[k + stride]);
+    const float dh = wall_edge_dh(solid, k + stride, k, hin[k + stride], hin[k], 0.0f, gate_of(k), bed[k + stride],
+                                  bed[k], speed, dt, cell);
+    return speed > 0.0f ? 0.0f : dh;
+  };
+  auto fr = [&](size_t k, bool has_prev) -> float {
+    const float speed = 0.5f * (vel[k] + vel[k + stride]);
+    const float prev = has_prev ? fl(k - stride) : 0.0f;
+    const float dh = wall_edge_dh(solid, k, k + stride, hin[k], hin[k + stride], 0.0f + prev, gate_of(k + stride),
+                                  bed[k], bed[k + stride], speed, dt, cell);
+    return speed > 0.0f ? dh : 0.0f;
+  };
+  const bool has_left = pos > lo, has_right = pos < hi;
+  const float fl_prev = has_left ? fl(idx - stride) : 0.0f;
+  const float fr_prev = has_left ? fr(idx - stride, pos - 1 > lo) : 0.0f;
+  const float fl_here = has_right ? fl(idx) : 0.0f;
+  const float fr_here = has_right ? fr(idx, has_left) : 0.0f;
+  const float rem = (0.0f + fl_prev) + fr_here;
+  const float add = (0.0f + fr_prev) + fl_here;
+  const float hv = hin[idx];
+  const float moved = fmaxf(hv - rem + add, 0.0f);
+  return (rem > 0.0f || add > 0.0f) ? moved : hv;
+}
+
+__global__ void k_transport_walled(const float* __restrict__ hin, float* __restrict__ hout,
+                                   const float* __restrict__ vel, const uint8_t* __restrict__ gate,
+                                   const uint8_t* __restrict__ solid, const float* __restrict__ bed, Box b,
+                                   int along_x, StepArgs a, float dt) {
+  Region r{b.i0, b.i1, b.j0, b.j1};
+  int i, j;
+  const bool in = region_cell(r, &i, &j);
+  double moved = 0.0;
+  if (in) {
+    const size_t idx = (size_t)j * a.res + i;
+    if (along_x) {
+      hout[idx] = transport_cell_walled(hin, vel, gate, solid, bed, idx, 1, i, b.i0, b.i1, dt, a.cell,
+                                        a.gate_uniform);
+    } else {
+      const float was = hout[idx];
+      const float now = transport_cell_walled(hin, vel, gate, solid, bed, idx, (size_t)a.res, j, b.j0, b.j1, dt,
+                                              a.cell, a.gate_uniform);
+      hout[idx] = now;
+      moved = (double)fmaxf(now, 0.0f) - (double)fmaxf(was, 0.0f);
+    }
+  }
+  if (!along_x && a.book) book_wave(&a.book->residue, moved);
+}
+
+// displace_occupied (:4382-4444), first half: a solid cell with water
+// empties (its depth split over its open 4-neighbours, or, with none, over
+// the open cells within two cells; mode 1 or 2) unless nothing nearby is
+// open; shallower than 1e-8 it just dries.
+__global__ void k_displace_give(float* __restrict__ h, float* __restrict__ u, float* __restrict__ v,
+                                const uint8_t* __restrict__ solid, int res, float* __restrict__ share,
+                                float* __restrict__ mode, SheetTotals* book) {
+  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x), j = (int)(blockIdx.y * blockDim.y + threadIdx.y);
+  const bool in = i < res && j < res;
+  double moved = 0.0;
+  if (in) {
+    const size_t idx = (size_t)j * res + i;
+    float m = 0.0f, s = 0.0f;
+    if (solid[idx]) {
+      const float vol = h[idx];
+      if (vol <= 1.0e-8f) {
+        moved = -(double)fmaxf(vol, 0.0f);
+        h[idx] = u[idx] = v[idx] = 0.0f;
+      } else {
+        int n4 = 0;
+        for (int k = 0; k < 4; k++) {
+          const int ni = i + kNb4[k][0], nj = j + kNb4[k][1];
+          if (grid_cell(ni, nj, res) && !solid[(size_t)nj * res + ni]) n4++;
+        }
+        int n25 = 0;
+        if (n4 == 0)
+          for (int dj = -2; dj <= 2; dj++)
+            for (int di = -2; di <= 2; di++) {
+              const int ni = i + di, nj = j + dj;
+              if (grid_cell(ni, nj, res) && !solid[(size_t)nj * res + ni]) n25++;
+            }
+        if (n4 > 0 || n25 > 0) {
+          m = n4 > 0 ? 1.0f : 2.0f;
+          s = vol / (float)(n4 > 0 ? n4 : n25);
+          moved = -(double)vol;
+          h[idx] = u[idx] = v[idx] = 0.0f;
+        }
+      }
+    }
+    share[idx] = s;
+    mode[idx] = m;
+  }
+  if (book) book_wave(&book->residue, moved);
+}
+
+// Second half: an open cell adds the shares sent to it, sources in raster
+// order, then h += give once.
+__global__ void k_displace_take(float* __restrict__ h, const uint8_t* __restrict__ solid, int res,
+                                const float* __restrict__ share, const float* __restrict__ mode, SheetTotals* book) {
+  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x), j = (int)(blockIdx.y * blockDim.y + threadIdx.y);
+  const bool in = i < res && j < res;
+  double moved = 0.0;
+  if (in && !solid[(size_t)j * res + i]) {
+    float give = 0.0f;
+    for (int dj = -2; dj <= 2; dj++)
+      for (int di = -2; di <= 2; di++) {
+        const int si = i + di, sj = j + dj;
+        if (!grid_cell(si, sj, res)) continue;
+        const size_t s = (size_t)sj * res + si;
+        const float m = mode[s];
+        const bool four = (di == 0) != (dj == 0) && (di == 1 || di == -1 || dj == 1 || dj == -1);
+        if ((m == 1.0f && four) || m == 2.0f) give = give + share[s];
+      }
+    if (give > 0.0f) {
+      const size_t idx = (size_t)j * res + i;
+      const float was = h[idx];
+      const float now = was + give;
+      h[idx] = now;
+      moved = (double)now - (double)was;
+    }
+  }
+  if (book) book_wave(&book->residue, moved);
+}
+
 inline dim3 grid_for(const Region& r, dim3 blk) {
   return dim3((unsigned)((r.i1 - r.i0 + 1 + (int)blk.x - 1) / (int)blk.x),
               (unsigned)((r.j1 - r.j0 + 1 + (int)blk.y - 1) / (int)blk.y));
@@ -640,6 +1026,65 @@ void exact_transport(const SheetPlanes& s, const Box& b, const StepArgs& a, floa
   // Cells of the box now hold the x-moved depth in h1; copy the box back
   // through the z pass, which writes h.
   hipLaunchKernelGGL(k_transport, g_box, blk, 0, st, s.h1, s.h, s.v, s.gate, s.bed, b, 0, a, dt);
+}
+
+// One substep of one box beside walls (substep with solid = Some,
+// src/fluid_core.rs:1879-2082).  par: the gravity kicks all first, then the
+// confinement sweep (force_gravity_solid_par, the reference's form for a
+// box of at least 48 x 32 cells when row splits are allowed); otherwise kick
+// and confinement cell by cell.  Scratch after the damping: bsrc holds the
+// redirect's removed depth, tg / ts / su / sv its four walls' shares, h1 the
+// flank bits (then the transport's x pass).
+void exact_substep_walled(const SheetPlanes& s, const Box& b, const StepArgs& a0, bool par, hipStream_t st) {
+  StepArgs a = a0;
+  a.project = 1;
+  const int res = a.res;
+  Region rim{b.i0 >= 2 ? b.i0 - 2 : 0, b.i1 + 2 < res - 1 ? b.i1 + 2 : res - 1, b.j0 >= 2 ? b.j0 - 2 : 0,
+             b.j1 + 2 < res - 1 ? b.j1 + 2 : res - 1};
+  Region rim1{b.i0 >= 1 ? b.i0 - 1 : 0, b.i1 + 1 < res - 1 ? b.i1 + 1 : res - 1, b.j0 >= 1 ? b.j0 - 1 : 0,
+              b.j1 + 1 < res - 1 ? b.j1 + 1 : res - 1};
+  Region box{b.i0, b.i1, b.j0, b.j1};
+  dim3 blk(64, 4);
+  dim3 g_rim = grid_for(rim, blk), g_rim1 = grid_for(rim1, blk), g_box = grid_for(box, blk);
+  hipLaunchKernelGGL(k_eta, g_rim, blk, 0, st, s.h, s.bed, s.eta, rim, res);
+  hipLaunchKernelGGL(k_lift_solid, g_box, blk, 0, st, s.eta, s.h, s.solid, box, res, a.eps);
+  hipLaunchKernelGGL(k_xblur2, g_rim, blk, 0, st, s.eta, s.tg, s.ts, rim, b, res);
+  hipLaunchKernelGGL(k_zblur_pressure, g_rim, blk, 0, st, s.tg, s.ts, s.eta, s.p, rim, b, res);
+  hipLaunchKernelGGL(k_noslip_ghost, g_box, blk, 0, st, s.u, s.v, s.h, s.solid, box, res, a.eps);
+  if (a.wind) {
+    if (b.j1 - b.j0 + 1 + 4 <= kRingRows)
+      hipLaunchKernelGGL(k_force_gauss_seidel_lds, dim3(1), dim3(256), 0, st, s.h, s.u, s.v, s.nu, s.p, b, a);
+    else
+      hipLaunchKernelGGL(k_force_gauss_seidel, dim3(1), dim3(256), 0, st, s.h, s.u, s.v, s.nu, s.p, b, a);
+  } else if (par) {
+    hipLaunchKernelGGL(k_kick_walled, g_box, blk, 0, st, s.h, s.u, s.v, s.nu, s.p, s.solid, box, a);
+    if (a.vort > 0.0f)
+      hipLaunchKernelGGL(k_sweep_walled, dim3(1), dim3(256), 0, st, s.h, s.u, s.v, s.nu, s.p, s.solid, b, a, 0);
+  } else {
+    hipLaunchKernelGGL(k_sweep_walled, dim3(1), dim3(256), 0, st, s.h, s.u, s.v, s.nu, s.p, s.solid, b, a, 1);
+  }
+  hipLaunchKernelGGL(k_project_source, g_rim, blk, 0, st, s.u, s.v, s.eta, s.nu, s.bsrc, rim, b, a);
+  hipLaunchKernelGGL(k_xblur2, g_rim, blk, 0, st, s.bsrc, s.tg, s.ts, rim, b, res);
+  hipLaunchKernelGGL(k_zblur_q, g_rim, blk, 0, st, s.tg, s.ts, s.p, rim, b, res);
+  hipLaunchKernelGGL(k_damp, g_box, blk, 0, st, s.h, s.u, s.v, s.nu, s.p, box, a);
+  hipLaunchKernelGGL(k_redirect_give, g_box, blk, 0, st, s.h, s.u, s.v, s.solid, s.bed, box, a, s.bsrc, s.tg, s.ts,
+                     s.su, s.sv, s.h1);
+  hipLaunchKernelGGL(k_redirect_take, g_rim1, blk, 0, st, s.h, rim1, b, a, s.bsrc, s.tg, s.ts, s.su, s.sv, s.h1);
+  hipLaunchKernelGGL(k_block_into_solid, g_box, blk, 0, st, s.u, s.v, s.h, s.solid, box, res, a.eps);
+  hipLaunchKernelGGL(k_noslip_ghost, g_box, blk, 0, st, s.u, s.v, s.h, s.solid, box, res, a.eps);
+  hipLaunchKernelGGL(k_transport_walled, g_box, blk, 0, st, s.h, s.h1, s.u, s.gate, s.solid, s.bed, b, 1, a, a.dt);
+  hipLaunchKernelGGL(k_transport_walled, g_box, blk, 0, st, s.h1, s.h, s.v, s.gate, s.solid, s.bed, b, 0, a, a.dt);
+  hipLaunchKernelGGL(k_dry_cut, g_box, blk, 0, st, s.h, s.u, s.v, b, a);
+}
+
+// displace_occupied (:4382-4444) over the whole grid; share and mode are
+// scratch planes of res * res floats.
+void displace_occupied(float* h, float* u, float* v, const uint8_t* solid, int res, float* share, float* mode,
+                       SheetTotals* book, hipStream_t st) {
+  dim3 blk(64, 4);
+  dim3 g((unsigned)((res + 63) / 64), (unsigned)((res + 3) / 4));
+  hipLaunchKernelGGL(k_displace_give, g, blk, 0, st, h, u, v, solid, res, share, mode, book);
+  hipLaunchKernelGGL(k_displace_take, g, blk, 0, st, h, solid, res, share, mode, book);
 }
 
 void exact_dry_cut(const SheetPlanes& s, const Box& b, const StepArgs& a, hipStream_t st) {

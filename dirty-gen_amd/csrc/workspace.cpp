@@ -21,7 +21,7 @@ Workspace* workspace_for(int res, int slot) {
   ws->res = res;
   ws->n = (size_t)res * res;
   if (hipMalloc(&ws->fbuf, (size_t)kWorkspacePlanes * ws->n * sizeof(float)) != hipSuccess) return nullptr;
-  if (hipMalloc(&ws->gate, ws->n) != hipSuccess) return nullptr;
+  if (hipMalloc(&ws->gate, 2 * ws->n) != hipSuccess) return nullptr;
   if (hipMemset(ws->fbuf, 0, (size_t)kWorkspacePlanes * ws->n * sizeof(float)) != hipSuccess) return nullptr;
   if (hipStreamCreateWithFlags(&ws->stream, hipStreamNonBlocking) != hipSuccess) return nullptr;
   Workspace* out = ws.get();

@@ -15,9 +15,11 @@ struct Workspace {
   int res = 0;
   size_t n = 0;
   float* fbuf = nullptr;    // 5 staging planes + 8 scratch planes
-  uint8_t* gate = nullptr;  // staging gate plane
+  uint8_t* gate = nullptr;  // staging gate plane, then the staging wall mask
   hipStream_t stream = nullptr;
   std::mutex mu;
+
+  uint8_t* solid_stage() const { return gate + n; }
 
   SheetPlanes planes() const {
     SheetPlanes s{};

@@ -94,6 +94,13 @@ def _declare(L):
     L.silt_step_sheet_ex.restype = C.c_int
     L.silt_step_sheet_device.argtypes = [_P, _P, _P, _P, _P, _F, _P, _Z, _F, _F, Drive, C.POINTER(SheetOpts), _P]
     L.silt_step_sheet_device.restype = C.c_int
+    L.silt_step_sheet_walled.argtypes = [_P, _P, _P, _P, _P, _P, _P, _Z, _Z, _F, _F, Drive, C.POINTER(SheetOpts)]
+    L.silt_step_sheet_walled.restype = C.c_int
+    L.silt_step_sheet_walled_device.argtypes = [_P, _P, _P, _P, _P, _F, _P, _P, _Z, _F, _F, Drive,
+                                                C.POINTER(SheetOpts), _P]
+    L.silt_step_sheet_walled_device.restype = C.c_int
+    L.silt_displace_occupied.argtypes = [_P, _P, _P, _P, _Z, _Z]
+    L.silt_displace_occupied.restype = C.c_int
     L.silt_pair_create.argtypes = [_Z, _F, C.c_int]
     L.silt_pair_create.restype = _P
     L.silt_pair_destroy.argtypes = [_P]
@@ -201,6 +208,43 @@ def step_sheet_device(h, u, v, bed, nu, gate, res, cell, dt, drive, nu_uniform=m
     rc = lib().silt_step_sheet_device(h, u, v, bed, nu or None, nu_uniform, gate or None, res, cell, dt,
                                       _drive(drive), C.byref(o), stream or None)
     return _check_status(rc)
+
+
+def step_sheet_walled(h, u, v, bed, nu, gate, solid, res, cell, dt, drive, **opts):
+    """The walled sheet (include/silt_sheet.h): ``step_sheet_eps_in`` with a
+    wall mask (uint8, nonzero = wall), on numpy float32 planes in place.
+    Returns the number of substeps."""
+    for a, nm in ((h, "h"), (u, "u"), (v, "v"), (bed, "bed"), (nu, "nu")):
+        if a.dtype != np.float32 or not a.flags.c_contiguous:
+            raise TypeError(f"{nm} must be C-contiguous float32")
+    for a, nm in ((gate, "gate"), (solid, "solid")):
+        if a.dtype != np.uint8 or not a.flags.c_contiguous:
+            raise TypeError(f"{nm} must be C-contiguous uint8")
+    n = min(a.size for a in (h, u, v, bed, nu, gate, solid))
+    o = sheet_opts(**opts)
+    rc = lib().silt_step_sheet_walled(_ptr(h), _ptr(u), _ptr(v), _ptr(bed), _ptr(nu), _ptr(gate), _ptr(solid), n,
+                                      res, cell, dt, _drive(drive), C.byref(o))
+    return _check_status(rc)
+
+
+def step_sheet_walled_device(h, u, v, bed, nu, gate, solid, res, cell, dt, drive, nu_uniform=math.nan, stream=0,
+                             **opts):
+    """The walled step on device buffers (integer device addresses)."""
+    o = sheet_opts(**opts)
+    rc = lib().silt_step_sheet_walled_device(h, u, v, bed, nu or None, nu_uniform, gate or None, solid, res, cell,
+                                             dt, _drive(drive), C.byref(o), stream or None)
+    return _check_status(rc)
+
+
+def displace_occupied(h, u, v, solid, res):
+    """displace_occupied (src/fluid_core.rs:4382-4444) on numpy planes in place."""
+    for a, nm in ((h, "h"), (u, "u"), (v, "v")):
+        if a.dtype != np.float32 or not a.flags.c_contiguous:
+            raise TypeError(f"{nm} must be C-contiguous float32")
+    if solid.dtype != np.uint8 or not solid.flags.c_contiguous:
+        raise TypeError("solid must be C-contiguous uint8")
+    n = min(a.size for a in (h, u, v, solid))
+    return _check_status(lib().silt_displace_occupied(_ptr(h), _ptr(u), _ptr(v), _ptr(solid), n, res))
 
 
 def set_chunk_rows(res, rows):
@@ -475,6 +519,9 @@ def _declare_core(L):
     L.silt_core_sample.argtypes = [_P, _F, _F, C.POINTER(GroundSample)]
     L.silt_core_ledger_full.argtypes = [_P, C.POINTER(FluidLedgerFull)]
     L.silt_core_ledger_lines.argtypes = [_P, C.POINTER(CoreLines)]
+    L.silt_core_set_hydro_solids.argtypes = [_P, C.POINTER(HydroSolid), _Z]
+    L.silt_core_hydro_load.argtypes = [_P, C.POINTER(HydroLoad)]
+    L.silt_core_solid_mask.argtypes = [_P, C.c_int, _P]
     L.silt_core_recenter.argtypes = [_P, _F, _F]
     L.silt_core_window_bounds.argtypes = [_P, _P]
     L.silt_core_lay_slurry_on_water.argtypes = [_P, _F, C.c_int]
@@ -523,7 +570,18 @@ class FluidLedgerFull(C.Structure):
 
 class CoreLines(C.Structure):
     _fields_ = [(n, C.c_double * 2) for n in (
-        "base_m3", "cut_m3", "residue_m3", "handoff_m3", "total_m3", "gap_m3")]
+        "base_m3", "cut_m3", "residue_m3", "handoff_m3", "source_m3", "total_m3", "gap_m3")]
+
+
+class HydroSolid(C.Structure):
+    _fields_ = [("x", _F), ("z", _F), ("yaw", _F), ("half_length", _F), ("half_width", _F), ("bottom_y", _F),
+                ("top_y", _F), ("contact", C.c_int32), ("radius", _F), ("width", _F)]
+
+
+class HydroLoad(C.Structure):
+    _fields_ = [(n, _F) for n in ("force_x", "force_z", "buoyancy_n", "hull_buoyancy_n", "wheel_buoyancy_n",
+                                  "displaced_m3", "eroded_m3", "hole_m")] + \
+               [("wheel_blocks", C.c_uint32), ("belly_blocks", C.c_uint32)]
 
 
 class SurrogateStats(C.Structure):
@@ -697,6 +755,23 @@ class FluidCore:
         led = FluidLedgerFull()
         _check_status(lib().silt_core_ledger_full(self._c, C.byref(led)))
         return {k: getattr(led, k) for k, _ in FluidLedgerFull._fields_}
+
+    def set_hydro_solids(self, solids):
+        """set_hydro_solids (include/silt_core.h): dicts of HydroSolid's
+        fields (missing ones 0)."""
+        arr = (HydroSolid * max(len(solids), 1))(*[HydroSolid(**{k: s.get(k, 0) for k, _ in HydroSolid._fields_})
+                                                   for s in solids])
+        _check_status(lib().silt_core_set_hydro_solids(self._c, arr, len(solids)))
+
+    def hydro_load(self):
+        out = HydroLoad()
+        _check_status(lib().silt_core_hydro_load(self._c, C.byref(out)))
+        return {k: getattr(out, k) for k, _ in HydroLoad._fields_}
+
+    def solid_mask(self, fine=False):
+        out = np.zeros((160 if fine else WINDOW_RES) ** 2, np.uint8)
+        _check_status(lib().silt_core_solid_mask(self._c, int(fine), _ptr(out)))
+        return out
 
     def ledger_lines(self):
         """The FP64 lines that close the ledger (include/silt_core.h):

@@ -108,15 +108,50 @@ int silt_core_ledger(silt_core *c, silt_fluid_ledger *out);
  * FP32 transports and the surrogate's fluxes created or removed beyond the
  * exchanged fluxes (rounding and the max(0) clamp).  handoff: what the
  * window <-> 2 cm patch handoffs changed (the FP32 restrict sums, the near
- * field's lay-down).  base: the total at the last caller edit (set_planes,
- * pours, map_water_bed, read_substrate, lay_slurry*, drain_over_bed,
- * recenter) less the lines booked by then.  gap = total - base -
- * (residue - cut + handoff): volume no line accounts for.  The reference's
- * ledger() has no such lines (FluidLedger, src/fluid_core.rs:483-509). */
+ * field's lay-down).  source: what entered from the bed (the scour beside
+ * tire contacts; slurry only).  base: the total at the last caller edit
+ * (set_planes, pours, map_water_bed, read_substrate, lay_slurry*,
+ * drain_over_bed, recenter) less the lines booked by then.  gap = total -
+ * base - (residue - cut + handoff + source): volume no line accounts for.
+ * The reference's ledger() has no such lines (FluidLedger,
+ * src/fluid_core.rs:483-509). */
 typedef struct silt_core_lines {
-    double base_m3[2], cut_m3[2], residue_m3[2], handoff_m3[2], total_m3[2], gap_m3[2];
+    double base_m3[2], cut_m3[2], residue_m3[2], handoff_m3[2], source_m3[2], total_m3[2], gap_m3[2];
 } silt_core_lines;
 int silt_core_ledger_lines(silt_core *c, silt_core_lines *out);
+
+/* HydroSolid (src/fluid_core.rs:219-238): a box (the belly) or a tire
+ * contact that the sheet treats as a wall wherever the free surface stands
+ * above bottom_y.  yaw 0 points the long axis along +z.  contact: nonzero
+ * for a tire contact (scour reads the shear beside it); radius > 0 makes the
+ * buoyancy a submerged cylinder segment of that radius and width. */
+typedef struct silt_hydro_solid {
+    float x, z, yaw, half_length, half_width, bottom_y, top_y;
+    int32_t contact;
+    float radius, width;
+} silt_hydro_solid;
+
+/* HydroLoad (src/fluid_core.rs:241-253). */
+typedef struct silt_hydro_load {
+    float force_x, force_z, buoyancy_n, hull_buoyancy_n, wheel_buoyancy_n, displaced_m3, eroded_m3, hole_m;
+    uint32_t wheel_blocks, belly_blocks;
+} silt_hydro_load;
+
+/* set_hydro_solids (src/fluid_core.rs:5257-5260): the walls for the next
+ * steps (at most 64; n = 0 is open water).  With walls every layer step of
+ * FluidCore::step is the walled sheet (step_pair :7954-8125: the grid's wall
+ * mask, the body displacement, silt_step_sheet_walled's substeps), and the
+ * window's step ends with scour_contacts (:6076-6078, :7364-7421). */
+int silt_core_set_hydro_solids(silt_core *c, const silt_hydro_solid *solids, size_t n);
+
+/* hydro_load (src/fluid_core.rs:7164-7249): pressure force on the walls,
+ * buoyancy and the scour account, read from the window. */
+int silt_core_hydro_load(silt_core *c, silt_hydro_load *out);
+
+/* The wall mask solid_mask (src/fluid_core.rs:7288-7362) lays from the
+ * current planes: 200 x 200 bytes for the window (fine 0) or 160 x 160 for
+ * the 2 cm patch while it is on (fine 1); 2 wall, 1 contact, 0 open. */
+int silt_core_solid_mask(silt_core *c, int fine, uint8_t *out);
 
 /* Coarse planes in and out, 200 x 200 each (any pointer may be NULL to skip). */
 int silt_core_get_planes(silt_core *c, float *water, float *slurry, float *uw, float *vw, float *us, float *vs,

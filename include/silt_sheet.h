@@ -95,6 +95,34 @@ int silt_step_sheet_device(float *h, float *u, float *v, const float *bed, const
                            const uint8_t *gate, size_t res, float cell, float dt, silt_drive drive,
                            const silt_sheet_opts *opts, void *hip_stream);
 
+/*
+ * The walled sheet: step_sheet_eps_in (src/fluid_core.rs:856-889) with a wall
+ * mask, solid = Some (the creek and ford scenes; FluidCore::step_pair passes
+ * the mask of its hydro solids, :7954-7966).  solid: res * res bytes, nonzero
+ * = wall.  Walls keep the wet-set work boxes and the substep loop on every
+ * grid (no pace policy: opts.pace is ignored), Courant 0.42 for wind too, and
+ * every substep adds the surface lift, the no-slip ghost, the redirect into
+ * the flanks and the no-penetration filter (:1879-2082, :4225-4563).  The
+ * stage kernels run it on every path (opts.path is ignored); opts.row_split
+ * picks the reference's parallel-kick form on boxes of at least 48 x 32.
+ * Returns the substep count.
+ */
+int silt_step_sheet_walled(float *h, float *u, float *v, const float *bed, const float *nu, const uint8_t *gate,
+                           const uint8_t *solid, size_t len, size_t res, float cell, float dt, silt_drive drive,
+                           const silt_sheet_opts *opts);
+
+/* The walled step on device pointers (as silt_step_sheet_device; solid is a
+ * device plane). */
+int silt_step_sheet_walled_device(float *h, float *u, float *v, const float *bed, const float *nu, float nu_uniform,
+                                  const uint8_t *gate, const uint8_t *solid, size_t res, float cell, float dt,
+                                  silt_drive drive, const silt_sheet_opts *opts, void *hip_stream);
+
+/* displace_occupied (src/fluid_core.rs:4382-4444) on host planes: water in
+ * cells a body occupies is split over the open 4-neighbours (or the open
+ * cells within two cells).  FluidCore::step_pair runs it on each layer before
+ * the walled step. */
+int silt_displace_occupied(float *h, float *u, float *v, const uint8_t *solid, size_t len, size_t res);
+
 /* Library identity: "silt_amd <version> gfx950". */
 const char *silt_version(void);
 

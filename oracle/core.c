@@ -46,6 +46,9 @@ typedef REAL real;
 #define R_FMAX fmax
 #define R_HYPOT hypot
 #define R_COPYSIGN copysign
+#define R_SIN sin
+#define R_COS cos
+#define R_ACOS acos
 #else
 #define R_EXP expf
 #define R_CEIL ceilf
@@ -56,7 +59,18 @@ typedef REAL real;
 #define R_FMAX fmaxf
 #define R_HYPOT hypotf
 #define R_COPYSIGN copysignf
+#define R_SIN sinf
+#define R_COS cosf
+#define R_ACOS acosf
 #endif
+#define GRAV K(9.81)
+#define R_ISFINITE(x) isfinite(x)
+/* Rust f32::clamp: NaN passes through. */
+static inline real clampr(real x, real lo, real hi) {
+    if (x < lo) x = lo;
+    if (x > hi) x = hi;
+    return x;
+}
 
 #define WRES 200
 #define FRES 160
@@ -75,6 +89,18 @@ int F(oracle_step_sheet_ex)(real *h, real *u, real *v, const real *bed, const re
                             int confine_mode, size_t *boxes_out, size_t *nboxes_out);
 real F(oracle_dynamic_viscosity)(int grain, real moisture);
 real F(oracle_column_sum)(const real *h, size_t n, real cell);
+int F(oracle_step_sheet_walled)(real *h, real *u, real *v, const real *bed, const real *nu, const uint8_t *gate,
+                                const uint8_t *solid, size_t res, real cell, real dt, int drive_wind, real wind_x,
+                                real wind_z, real vorticity, real eps, real sub_cap, int inner_par);
+void F(oracle_displace_occupied)(real *h, real *u, real *v, const uint8_t *solid, size_t res, real *give);
+
+/* HydroSolid (src/fluid_core.rs:219-238). */
+typedef struct {
+    real x, z, yaw, half_length, half_width, bottom_y, top_y;
+    int contact;
+    real radius, width;
+} HSolid;
+#define MAX_SOLIDS 64
 
 typedef struct {
     real x, z, fx, fz, radius, width, speed, slip;
@@ -124,6 +150,13 @@ typedef struct {
     /* cells parked off the window by recenter: open addressing by world key */
     struct ParkedSlot *parked;
     size_t pcap, pcount, pused;
+    /* walls (set_hydro_solids) and the scour account */
+    HSolid solids[MAX_SOLIDS];
+    int nsolids;
+    real *bed_drop;
+    real eroded_m3, scour_hole_m;
+    uint8_t *mask, *fmask;
+    real *give;
 } Core;
 
 static real *zalloc(size_t n) { return (real *)calloc(n, sizeof(real)); }
@@ -154,6 +187,10 @@ Core *F(oracle_core_new)(real cx, real cz) {
     c->nu_s = zalloc(WN);
     c->gate = (uint8_t *)malloc(WN);
     memset(c->gate, 1, WN);
+    c->bed_drop = zalloc(WN);
+    c->mask = (uint8_t *)calloc(WN, 1);
+    c->fmask = (uint8_t *)calloc(FN, 1);
+    c->give = zalloc(WN);
     real nw = F(oracle_dynamic_viscosity)(3, K(1.0));
     for (size_t k = 0; k < WN; k++) c->nu_w[k] = nw;
     return c;
@@ -165,6 +202,7 @@ void F(oracle_core_free)(Core *c) {
                       c->fvw, c->fus, c->fvs, c->fbed, c->fbed_s, c->fmoist, c->nu_w, c->nu_s};
     for (size_t k = 0; k < sizeof planes / sizeof planes[0]; k++) free(planes[k]);
     free(c->grain); free(c->fgrain); free(c->hold); free(c->tiles); free(c->gate); free(c->parked);
+    free(c->bed_drop); free(c->mask); free(c->fmask); free(c->give);
     free(c);
 }
 
@@ -813,9 +851,178 @@ static void fill_nu_s(Core *c, const real *slurry, const real *moisture, const u
             c->nu_s[j * res + i] = F(oracle_dynamic_viscosity)(grain[j * res + i], moisture[j * res + i]);
 }
 
-/* step_pair (src/fluid_core.rs:7942-8150), no solids, lifts or reach. */
+
+/* ---- walls: src/fluid_core.rs:4584-4616, 7164-7421, 7942-8110 ------------ */
+
+/* solid_contains, :4584-4594 */
+static int solid_contains(const HSolid *s, real x, real z) {
+    real dx = x - s->x, dz = z - s->z;
+    real fwd_x = R_SIN(s->yaw), fwd_z = R_COS(s->yaw);
+    real right_x = fwd_z, right_z = -fwd_x;
+    real along = dx * fwd_x + dz * fwd_z;
+    real lat = dx * right_x + dz * right_z;
+    return R_FABS(along) <= R_FMAX(s->half_length, K(0.0)) && R_FABS(lat) <= R_FMAX(s->half_width, K(0.0));
+}
+
+/* solid_overlaps, :4596-4604 */
+static int solid_overlaps(const HSolid *s, real x, real z, real eta, real bed) {
+    if (s->half_length <= K(1.0e-4) || s->half_width <= K(1.0e-4)) return 0;
+    if (!solid_contains(s, x, z)) return 0;
+    return eta > s->bottom_y + K(1.0e-4) && bed < s->top_y;
+}
+
+/* solid_mask, :7288-7362: walls 2 first, then contacts 1 (a contact wins a
+ * shared cell); a cell is a wall while the free surface beside it stands
+ * above the solid's underside. */
+static void solid_mask(const Core *c, int fine, uint8_t *mask) {
+    int res = fine ? FRES : WRES;
+    real cell = fine ? FINE : COARSE;
+    const real *origin = fine ? c->fine_origin : c->origin;
+    const real *water = fine ? c->fw : c->water;
+    const real *bed = fine ? c->fbed : c->bed;
+    memset(mask, 0, (size_t)res * res);
+    for (int pass = 0; pass < 2; pass++) {
+        int contact = pass == 1;
+        uint8_t value = contact ? 1 : 2;
+        for (int k = 0; k < c->nsolids; k++) {
+            const HSolid *s = &c->solids[k];
+            if ((s->contact != 0) != contact) continue;
+            if (s->half_length <= K(1.0e-4) || s->half_width <= K(1.0e-4)) continue;
+            real reach = R_HYPOT(R_FMAX(s->half_length, K(0.0)), R_FMAX(s->half_width, K(0.0))) + cell;
+            if (!R_ISFINITE(reach) || !R_ISFINITE(s->x) || !R_ISFINITE(s->z)) continue;
+            real lo_i = R_FLOOR((s->x - reach - origin[0]) / cell - K(0.5));
+            real hi_i = R_CEIL((s->x + reach - origin[0]) / cell - K(0.5));
+            real lo_j = R_FLOOR((s->z - reach - origin[1]) / cell - K(0.5));
+            real hi_j = R_CEIL((s->z + reach - origin[1]) / cell - K(0.5));
+            real top = (real)(res - 1);
+            if (hi_i < K(0.0) || hi_j < K(0.0) || lo_i > top || lo_j > top) continue;
+            int i0 = (int)R_FMAX(lo_i, K(0.0)), i1 = (int)R_FMIN(hi_i, top);
+            int j0 = (int)R_FMAX(lo_j, K(0.0)), j1 = (int)R_FMIN(hi_j, top);
+            for (int j = j0; j <= j1; j++)
+                for (int i = i0; i <= i1; i++) {
+                    size_t idx = (size_t)j * res + i;
+                    if (mask[idx] == value) continue;
+                    real x = origin[0] + ((real)i + K(0.5)) * cell;
+                    real z = origin[1] + ((real)j + K(0.5)) * cell;
+                    real eta = bed[idx] + water[idx];
+                    static const int nb[4][2] = {{-1, 0}, {1, 0}, {0, -1}, {0, 1}};
+                    for (int q = 0; q < 4; q++) {
+                        int ni = i + nb[q][0], nj = j + nb[q][1];
+                        if (ni < 0 || nj < 0 || ni >= res || nj >= res) continue;
+                        size_t nidx = (size_t)nj * res + ni;
+                        eta = R_FMAX(eta, bed[nidx] + water[nidx]);
+                    }
+                    if (solid_overlaps(s, x, z, eta, bed[idx])) mask[idx] = value;
+                }
+        }
+    }
+}
+
+/* One layer of step_pair with walls (:8056-8125): soften (coarse), the body
+ * displacement, then the walled step. */
+static void step_layer_walled(Core *c, int fine, int slurry, const uint8_t *mask, real dt) {
+    int res = fine ? FRES : WRES;
+    real cell = fine ? FINE : COARSE;
+    real cap = fine ? c->fine_sub_cap : K(8.0);
+    real *h = slurry ? (fine ? c->fs : c->slurry) : (fine ? c->fw : c->water);
+    real *u = slurry ? (fine ? c->fus : c->us) : (fine ? c->fuw : c->uw);
+    real *v = slurry ? (fine ? c->fvs : c->vs) : (fine ? c->fvw : c->vw);
+    real *bed = slurry ? (fine ? c->fbed_s : c->bed) : (fine ? c->fbed : c->bed);
+    if (!fine) soften_border(u, v, res);
+    F(oracle_displace_occupied)(h, u, v, mask, (size_t)res, c->give);
+    F(oracle_step_sheet_walled)(h, u, v, bed, slurry ? c->nu_s : c->nu_w, c->gate, mask, (size_t)res, cell, dt, 0,
+                                K(0.0), K(0.0), slurry ? K(0.05) : K(0.08), K(1.0e-6), cap, 0);
+}
+
+/* scour_contacts, :7364-7421 */
+static void scour_contacts(Core *c, real dt) {
+    if (dt <= K(0.0) || c->nsolids == 0) return;
+    solid_mask(c, 0, c->mask);
+    const uint8_t *mask = c->mask;
+    real area = COARSE * COARSE;
+    real *cut = c->give;
+    memset(cut, 0, WN * sizeof(real));
+    for (int j = 1; j < WRES - 1; j++)
+        for (int i = 1; i < WRES - 1; i++) {
+            size_t idx = (size_t)j * WRES + i;
+            if (mask[idx] != 0 || c->water[idx] < K(0.02)) continue;
+            if (!(mask[idx - 1] == 1 || mask[idx + 1] == 1 || mask[idx - WRES] == 1 || mask[idx + WRES] == 1))
+                continue;
+            real speed = R_HYPOT(c->uw[idx], c->vw[idx]);
+            if (!R_ISFINITE(speed)) continue;
+            real tau = K(1000.0) * K(0.03) * speed * speed;
+            real tau_c = K(1.5);
+            if (tau <= tau_c) continue;
+            real rate = R_FMIN(K(0.0035) * (tau / tau_c - K(1.0)), K(0.012));
+            real room = R_FMAX(K(0.08) - c->bed_drop[idx], K(0.0));
+            cut[idx] = R_FMIN(rate * dt, room);
+        }
+    for (size_t idx = 0; idx < WN; idx++) {
+        real dh = cut[idx];
+        if (dh <= K(0.0)) continue;
+        c->bed[idx] -= dh;
+        c->slurry[idx] += dh;
+        c->us[idx] = c->uw[idx];
+        c->vs[idx] = c->vw[idx];
+        c->grain[idx] = 1; /* Grain::Silt */
+        c->moisture[idx] = R_FMAX(c->moisture[idx], K(0.9));
+        c->bed_drop[idx] += dh;
+        c->eroded_m3 += dh * area;
+        c->scour_hole_m = R_FMAX(c->scour_hole_m, c->bed_drop[idx]);
+    }
+}
+
+/* cylinder_segment_volume, :4606-4615 */
+static real cylinder_segment_volume(real depth, real radius, real width) {
+    real d = clampr(depth, K(0.0), radius * K(2.0));
+    if (d <= K(1.0e-4) || width <= K(0.0) || radius <= K(1.0e-3)) return K(0.0);
+    real cc = clampr((radius - d) / radius, K(-1.0), K(1.0));
+    real a = radius * radius * R_ACOS(cc) - (radius - d) * R_SQRT(R_FMAX(K(2.0) * radius * d - d * d, K(0.0)));
+    return R_FMAX(a, K(0.0)) * width;
+}
+
+/* ring_surface, :7251-7286: mean bed and water of the wet cells around a
+ * solid (outside it, within its reach). */
+static int ring_surface(const Core *c, const HSolid *s, real *bed_out, real *water_out) {
+    real reach = R_FMAX(s->half_length, s->half_width) + K(0.55);
+    int i0 = (int)R_FLOOR((s->x - reach - c->origin[0]) / COARSE);
+    int i1 = (int)R_FLOOR((s->x + reach - c->origin[0]) / COARSE);
+    int j0 = (int)R_FLOOR((s->z - reach - c->origin[1]) / COARSE);
+    int j1 = (int)R_FLOOR((s->z + reach - c->origin[1]) / COARSE);
+    real bed = K(0.0), water = K(0.0), n = K(0.0);
+    for (int j = j0; j <= j1; j++)
+        for (int i = i0; i <= i1; i++) {
+            if (i < 0 || j < 0 || i >= WRES || j >= WRES) continue;
+            real x, z;
+            coarse_xz(c, i, j, &x, &z);
+            if (solid_contains(s, x, z)) continue;
+            if (R_HYPOT(x - s->x, z - s->z) > reach) continue;
+            size_t idx = (size_t)j * WRES + i;
+            if (c->water[idx] < K(0.01)) continue;
+            bed += c->bed[idx];
+            water += c->water[idx];
+            n += K(1.0);
+        }
+    if (n < K(1.0)) return 0;
+    *bed_out = bed / n;
+    *water_out = water / n;
+    return 1;
+}
+
+/* step_pair (src/fluid_core.rs:7942-8150), no lifts or reach.  With walls
+ * the mask of this grid is laid once from the planes before either layer
+ * steps (:7954-7966). */
 static void step_pair(Core *c, real dt, int fine) {
     if (fine && !c->fine_on) return;
+    if (c->nsolids > 0) {
+        uint8_t *mask = fine ? c->fmask : c->mask;
+        solid_mask(c, fine, mask);
+        step_layer_walled(c, fine, 0, mask, dt);
+        int res = fine ? FRES : WRES;
+        fill_nu_s(c, fine ? c->fs : c->slurry, fine ? c->fmoist : c->moisture, fine ? c->fgrain : c->grain, res);
+        step_layer_walled(c, fine, 1, mask, dt);
+        return;
+    }
     int res = fine ? FRES : WRES;
     real cell = fine ? FINE : COARSE;
     real cap = fine ? c->fine_sub_cap : K(8.0);
@@ -909,10 +1116,93 @@ int F(oracle_core_step)(Core *c, real dt, real *spray) {
         if (!fine_held_at(c, pushes[k].x, pushes[k].z)) push_on(c, &pushes[k], 0);
     step_pair(c, dt, 0);
     vol += peel_spray(c, 0, 1, &events);
+    if (c->nsolids > 0) scour_contacts(c, dt);
     c->npush = 0;
     if (spray) *spray = vol;
     return events;
 }
+
+
+/* set_hydro_solids (:5257-5260): s10 holds n solids of x, z, yaw,
+ * half_length, half_width, bottom_y, top_y, contact, radius, width. */
+void F(oracle_core_set_hydro_solids)(Core *c, const real *s10, int n) {
+    if (n > MAX_SOLIDS) n = MAX_SOLIDS;
+    if (n < 0) n = 0;
+    for (int k = 0; k < n; k++) {
+        const real *q = s10 + 10 * k;
+        HSolid h = {q[0], q[1], q[2], q[3], q[4], q[5], q[6], q[7] != K(0.0), q[8], q[9]};
+        c->solids[k] = h;
+    }
+    c->nsolids = n;
+}
+
+/* The wall mask of the window (fine 0) or the patch (fine 1). */
+void F(oracle_core_solid_mask)(Core *c, int fine, uint8_t *out) { solid_mask(c, fine, out); }
+
+/* hydro_load (:7164-7249).  out10: force_x, force_z, buoyancy_n,
+ * hull_buoyancy_n, wheel_buoyancy_n, displaced_m3, eroded_m3, hole_m,
+ * wheel_blocks, belly_blocks. */
+void F(oracle_core_hydro_load)(Core *c, real *out10) {
+    uint8_t *mask = c->mask;
+    if (c->nsolids == 0) memset(mask, 0, WN);
+    else solid_mask(c, 0, mask);
+    real fx = K(0.0), fz = K(0.0);
+    unsigned wheel_blocks = 0, belly_blocks = 0;
+    for (size_t k = 0; k < WN; k++) {
+        if (mask[k] == 1) wheel_blocks++;
+        else if (mask[k] == 2) belly_blocks++;
+    }
+    static const int nb[4][2] = {{-1, 0}, {1, 0}, {0, -1}, {0, 1}};
+    for (int j = 0; j < WRES; j++)
+        for (int i = 0; i < WRES; i++) {
+            size_t idx = (size_t)j * WRES + i;
+            if (mask[idx] != 0) continue;
+            real h = c->water[idx];
+            if (h < K(1.0e-4) || !R_ISFINITE(h)) continue;
+            for (int q = 0; q < 4; q++) {
+                int ni = i + nb[q][0], nj = j + nb[q][1];
+                if (ni < 0 || nj < 0 || ni >= WRES || nj >= WRES) continue;
+                if (mask[(size_t)nj * WRES + ni] == 0) continue;
+                real nx = (real)nb[q][0], nz = (real)nb[q][1];
+                real hydro = K(1000.0) * GRAV * h * h * K(0.5) * COARSE;
+                fx += nx * hydro;
+                fz += nz * hydro;
+                real vn = c->uw[idx] * nx + c->vw[idx] * nz;
+                if (vn > K(0.0)) {
+                    real dyn_f = K(0.5) * K(1000.0) * vn * vn * h * COARSE;
+                    fx += nx * dyn_f;
+                    fz += nz * dyn_f;
+                }
+            }
+        }
+    real wheel_b = K(0.0), hull_b = K(0.0), displaced = K(0.0);
+    for (int k = 0; k < c->nsolids; k++) {
+        const HSolid *s = &c->solids[k];
+        real bed, water;
+        if (!ring_surface(c, s, &bed, &water)) continue;
+        if (s->radius > K(0.0)) {
+            real vol = cylinder_segment_volume(water, s->radius, s->width);
+            displaced += vol;
+            wheel_b += K(1000.0) * GRAV * vol;
+        } else {
+            real eta = bed + water;
+            real height = R_FMAX(s->top_y - s->bottom_y, K(0.0));
+            real sub = clampr(eta - s->bottom_y, K(0.0), height);
+            real vol = (s->half_length * K(2.0)) * (s->half_width * K(2.0)) * sub * K(0.36);
+            displaced += vol;
+            hull_b += K(1000.0) * GRAV * vol;
+        }
+    }
+    out10[0] = fx; out10[1] = fz; out10[2] = wheel_b + hull_b; out10[3] = hull_b; out10[4] = wheel_b;
+    out10[5] = displaced; out10[6] = c->eroded_m3; out10[7] = c->scour_hole_m;
+    out10[8] = (real)wheel_blocks; out10[9] = (real)belly_blocks;
+}
+
+/* Test entry point: scour_contacts on its own. */
+void F(oracle_core_scour)(Core *c, real dt) { scour_contacts(c, dt); }
+
+/* The scour's bed drop plane (200 x 200). */
+void F(oracle_core_bed_drop)(Core *c, real *out) { memcpy(out, c->bed_drop, WN * sizeof(real)); }
 
 /* map_water_bed with arrays sampled at cell centres (src/fluid_core.rs:5329-5355). */
 void F(oracle_core_map_water_bed)(Core *c, const real *bed, const real *water) {

@@ -116,6 +116,27 @@ def _declare(L):
         f = getattr(L, "oracle_mlp_forward" + sfx)
         f.argtypes = [_P, _P, _Z, C.c_int, _P, _P, _P, _P, _P]
         f.restype = None
+        f = getattr(L, "oracle_step_sheet_walled" + sfx)
+        f.argtypes = [_P, _P, _P, _P, _P, _P, _P, _Z, R, R, C.c_int, R, R, R, R, R, C.c_int]
+        f.restype = C.c_int
+        f = getattr(L, "oracle_lift_solid_surface" + sfx)
+        f.argtypes = [_P, _P, _P, _Z, _Z, _Z, _Z, _Z, R, _P]
+        f.restype = None
+        f = getattr(L, "oracle_noslip_ghost" + sfx)
+        f.argtypes = [_P, _P, _P, _P, _Z, _Z, _Z, _Z, _Z, R, _P, _P]
+        f.restype = None
+        f = getattr(L, "oracle_block_into_solid" + sfx)
+        f.argtypes = [_P, _P, _P, _P, _Z, _Z, _Z, _Z, _Z, R]
+        f.restype = None
+        f = getattr(L, "oracle_displace_occupied" + sfx)
+        f.argtypes = [_P, _P, _P, _P, _Z, _P]
+        f.restype = None
+        f = getattr(L, "oracle_redirect_into_walls" + sfx)
+        f.argtypes = [_P, _P, _P, _P, _P, _Z, R, R, _Z, _Z, _Z, _Z, _P, _P]
+        f.restype = None
+        f = getattr(L, "oracle_transport_walled" + sfx)
+        f.argtypes = [_P, _P, _P, _P, _P, _P, _Z, R, R, _Z, _Z, _Z, _Z, C.c_int, _P, _P]
+        f.restype = None
         f = getattr(L, "oracle_mlp_gradients" + sfx)
         f.argtypes = [_P, _P, _P, _Z, C.c_int, _P]
         f.restype = C.c_double
@@ -171,6 +192,75 @@ def step_sheet(h, u, v, bed, nu, gate, res, cell, dt, drive=("gravity", 0.0), *,
         k = min(nb.value, 8)
         return nsub, [tuple(int(x) for x in boxes[4 * i:4 * i + 4]) for i in range(k)]
     return nsub
+
+
+def step_sheet_walled(h, u, v, bed, nu, gate, solid, res, cell, dt, drive=("gravity", 0.0), *, eps=1.0e-6,
+                      sub_cap=8.0, inner_par=True, precision=32):
+    """The walled sheet: ``step_sheet_eps_in`` with a wall mask
+    (src/fluid_core.rs:856-889, substep :1879-2082 with solid = Some).
+    ``solid`` is a uint8 plane, nonzero = wall.  Returns the substep count."""
+    dtype = _dt(precision)
+    n = res * res
+    for a, nm in ((h, "h"), (u, "u"), (v, "v"), (bed, "bed"), (nu, "nu")):
+        _check(a, dtype, n, nm)
+    _check(gate, np.uint8, n, "gate")
+    _check(solid, np.uint8, n, "solid")
+    if drive[0] == "gravity":
+        kind, wx, wz, vort = 0, 0.0, 0.0, float(drive[1])
+    elif drive[0] == "wind":
+        kind, wx, wz, vort = 1, float(drive[1]), float(drive[2]), float(drive[3])
+    else:
+        raise ValueError(drive)
+    f = getattr(lib(), "oracle_step_sheet_walled" + _sfx(precision))
+    return f(_ptr(h), _ptr(u), _ptr(v), _ptr(bed), _ptr(nu), _ptr(gate), _ptr(solid), res, cell, dt, kind, wx, wz,
+             vort, eps, sub_cap, int(inner_par))
+
+
+def _box_all(res, box):
+    return (0, res - 1, 0, res - 1) if box is None else box
+
+
+def lift_solid_surface(eta, h, solid, res, box=None, eps=1.0e-6, precision=32):
+    """lift_solid_surface (src/fluid_core.rs:4225-4276) in place on eta."""
+    tmp = np.zeros(res * res, _dt(precision))
+    getattr(lib(), "oracle_lift_solid_surface" + _sfx(precision))(_ptr(eta), _ptr(h), _ptr(solid), res,
+                                                                    *_box_all(res, box), eps, _ptr(tmp))
+
+
+def noslip_ghost(u, v, h, solid, res, box=None, eps=1.0e-6, precision=32):
+    """write_noslip_ghost (src/fluid_core.rs:4278-4326) in place."""
+    gu, gv = np.zeros(res * res, _dt(precision)), np.zeros(res * res, _dt(precision))
+    getattr(lib(), "oracle_noslip_ghost" + _sfx(precision))(_ptr(u), _ptr(v), _ptr(h), _ptr(solid), res,
+                                                              *_box_all(res, box), eps, _ptr(gu), _ptr(gv))
+
+
+def block_into_solid(u, v, h, solid, res, box=None, eps=1.0e-6, precision=32):
+    """block_into_solid (src/fluid_core.rs:4328-4380) in place."""
+    getattr(lib(), "oracle_block_into_solid" + _sfx(precision))(_ptr(u), _ptr(v), _ptr(h), _ptr(solid), res,
+                                                                  *_box_all(res, box), eps)
+
+
+def displace_occupied(h, u, v, solid, res, precision=32):
+    """displace_occupied (src/fluid_core.rs:4382-4444) in place."""
+    give = np.zeros(res * res, _dt(precision))
+    getattr(lib(), "oracle_displace_occupied" + _sfx(precision))(_ptr(h), _ptr(u), _ptr(v), _ptr(solid), res,
+                                                                   _ptr(give))
+
+
+def redirect_into_walls(h, u, v, solid, bed, res, cell, dt, box=None, precision=32):
+    """redirect_into_walls (src/fluid_core.rs:4446-4563) in place on h."""
+    rem, add = np.zeros(res * res, _dt(precision)), np.zeros(res * res, _dt(precision))
+    getattr(lib(), "oracle_redirect_into_walls" + _sfx(precision))(_ptr(h), _ptr(u), _ptr(v), _ptr(solid),
+                                                                     _ptr(bed), res, cell, dt, *_box_all(res, box),
+                                                                     _ptr(rem), _ptr(add))
+
+
+def transport_walled(h, u, v, gate, solid, bed, res, cell, dt, along_x, box=None, precision=32):
+    """transport with solids (src/fluid_core.rs:3894-3977) in place on h."""
+    rem, add = np.zeros(res * res, _dt(precision)), np.zeros(res * res, _dt(precision))
+    getattr(lib(), "oracle_transport_walled" + _sfx(precision))(_ptr(h), _ptr(u), _ptr(v), _ptr(gate), _ptr(solid),
+                                                                  _ptr(bed), res, cell, dt, *_box_all(res, box),
+                                                                  int(along_x), _ptr(rem), _ptr(add))
 
 
 def surface_pressure(h, bed, res, box, precision=32):
@@ -303,6 +393,16 @@ def _declare_core(L):
         g("oracle_core_push_on").restype = None
         g("oracle_core_peel").argtypes = [_P, C.c_int, C.POINTER(C.c_int)]
         g("oracle_core_peel").restype = R
+        for n, at in (("oracle_core_set_hydro_solids", [_P, _P, C.c_int]),
+                      ("oracle_core_solid_mask", [_P, C.c_int, _P]), ("oracle_core_hydro_load", [_P, _P]),
+                      ("oracle_core_bed_drop", [_P, _P]), ("oracle_core_scour", [_P, R])):
+            g(n).argtypes = at
+            g(n).restype = None
+
+
+SOLID_FIELDS = ("x", "z", "yaw", "half_length", "half_width", "bottom_y", "top_y", "contact", "radius", "width")
+HYDRO_LOAD_FIELDS = ("force_x", "force_z", "buoyancy_n", "hull_buoyancy_n", "wheel_buoyancy_n", "displaced_m3",
+                     "eroded_m3", "hole_m", "wheel_blocks", "belly_blocks")
 
 
 class Core:
@@ -364,6 +464,31 @@ class Core:
     def map_water_bed(self, bed, water):
         b, w = self._arr(bed), self._arr(water)
         self._f("oracle_core_map_water_bed")(self._c, _ptr(b), _ptr(w))
+
+    def set_hydro_solids(self, solids):
+        """set_hydro_solids (src/fluid_core.rs:5257-5260): dicts of
+        HydroSolid's fields."""
+        a = np.array([[float(s.get(k, 0.0)) for k in SOLID_FIELDS] for s in solids] or [[0.0] * 10], self.dtype)
+        self._f("oracle_core_set_hydro_solids")(self._c, _ptr(a), len(solids))
+
+    def solid_mask(self, fine=False):
+        n = (160 if fine else WINDOW_RES) ** 2
+        out = np.zeros(n, np.uint8)
+        self._f("oracle_core_solid_mask")(self._c, int(fine), _ptr(out))
+        return out
+
+    def hydro_load(self):
+        out = np.zeros(10, self.dtype)
+        self._f("oracle_core_hydro_load")(self._c, _ptr(out))
+        return dict(zip(HYDRO_LOAD_FIELDS, (float(x) for x in out)))
+
+    def scour(self, dt):
+        self._f("oracle_core_scour")(self._c, dt)
+
+    def bed_drop(self):
+        out = np.zeros(WINDOW_RES * WINDOW_RES, self.dtype)
+        self._f("oracle_core_bed_drop")(self._c, _ptr(out))
+        return out
 
     def lay_slurry(self, depth, grain, moisture):
         d = self._arr(depth)

@@ -13,8 +13,9 @@
  * the FP32 rounding error of any implementation.
  *
  * Reference: src/fluid_core.rs of x3-team/Dirty-gen ("silt").  Each function
- * cites the lines it follows.  Only the open sheet (no hydro solids) is
- * restated: every caller on the sheet-step path passes solid = None.
+ * cites the lines it follows.  The open sheet (solid = None, every layer of
+ * the plain FluidCore step) and the walled sheet (hydro solids: the creek
+ * and ford scenes) are restated.
  *
  * The filter planes are cleared on each box's 2-cell rim when the box's
  * substep starts.  The reference keeps them in a thread-local that outlives
@@ -1074,6 +1075,384 @@ static void pace3_slices(real *h, real *u, real *v, const uint8_t *gate, size_t 
             }
 }
 
+/* ---- walled sheet (hydro solids): src/fluid_core.rs:1879-2082 with
+ * solid = Some, 2123-2200, 3894-4563 -------------------------------------- */
+
+/* A 4-neighbour offset table in the reference's order: (-1,0) (1,0) (0,-1) (0,1). */
+static const int NB4[4][2] = {{-1, 0}, {1, 0}, {0, -1}, {0, 1}};
+
+static inline int in_grid(long i, long j, size_t res) {
+    return i >= 0 && j >= 0 && i < (long)res && j < (long)res;
+}
+
+/* solid_near, :4565-4582 */
+static int solid_near(const uint8_t *mask, size_t res, size_t i, size_t j, int radius) {
+    for (int dj = -radius; dj <= radius; dj++)
+        for (int di = -radius; di <= radius; di++) {
+            if (di == 0 && dj == 0) continue;
+            long ni = (long)i + di, nj = (long)j + dj;
+            if (!in_grid(ni, nj, res)) continue;
+            if (mask[(size_t)nj * res + (size_t)ni]) return 1;
+        }
+    return 0;
+}
+
+/* lift_solid_surface, :4225-4276: each solid box cell takes the mean eta of
+ * the wet open cells within two cells (tmp holds the means first). */
+void F(oracle_lift_solid_surface)(real *eta, const real *h, const uint8_t *solid, size_t res, size_t i0,
+                                  size_t i1, size_t j0, size_t j1, real eps, real *tmp) {
+    for (size_t j = j0; j <= j1; j++)
+        for (size_t i = i0; i <= i1; i++) {
+            size_t idx = j * res + i;
+            if (!solid[idx]) continue;
+            real acc = K(0.0), n = K(0.0);
+            for (int dj = -2; dj <= 2; dj++)
+                for (int di = -2; di <= 2; di++) {
+                    if (di == 0 && dj == 0) continue;
+                    long ni = (long)i + di, nj = (long)j + dj;
+                    if (!in_grid(ni, nj, res)) continue;
+                    size_t nidx = (size_t)nj * res + (size_t)ni;
+                    if (solid[nidx] || h[nidx] <= eps) continue;
+                    acc += eta[nidx];
+                    n += K(1.0);
+                }
+            tmp[idx] = n > K(0.0) ? acc / n : eta[idx];
+        }
+    for (size_t j = j0; j <= j1; j++)
+        for (size_t i = i0; i <= i1; i++)
+            if (solid[j * res + i]) eta[j * res + i] = tmp[j * res + i];
+}
+
+/* write_noslip_ghost, :4278-4326: a solid box cell takes minus the mean
+ * velocity of its wet open 4-neighbours (0 without one), from the values
+ * before any ghost is written. */
+void F(oracle_noslip_ghost)(real *u, real *v, const real *h, const uint8_t *solid, size_t res, size_t i0,
+                            size_t i1, size_t j0, size_t j1, real eps, real *gu, real *gv) {
+    for (size_t j = j0; j <= j1; j++)
+        for (size_t i = i0; i <= i1; i++) {
+            size_t idx = j * res + i;
+            if (!solid[idx]) continue;
+            real au = K(0.0), av = K(0.0), n = K(0.0);
+            for (int k = 0; k < 4; k++) {
+                long ni = (long)i + NB4[k][0], nj = (long)j + NB4[k][1];
+                if (!in_grid(ni, nj, res)) continue;
+                size_t nidx = (size_t)nj * res + (size_t)ni;
+                if (solid[nidx] || h[nidx] <= eps) continue;
+                au += u[nidx];
+                av += v[nidx];
+                n += K(1.0);
+            }
+            gu[idx] = n > K(0.0) ? -au / n : K(0.0);
+            gv[idx] = n > K(0.0) ? -av / n : K(0.0);
+        }
+    for (size_t j = j0; j <= j1; j++)
+        for (size_t i = i0; i <= i1; i++) {
+            size_t idx = j * res + i;
+            if (!solid[idx]) continue;
+            u[idx] = gu[idx];
+            v[idx] = gv[idx];
+        }
+}
+
+/* block_into_solid, :4328-4380: the into-wall part of a wet open cell's
+ * velocity is dropped, wall by wall in neighbour order. */
+void F(oracle_block_into_solid)(real *u, real *v, const real *h, const uint8_t *solid, size_t res, size_t i0,
+                                size_t i1, size_t j0, size_t j1, real eps) {
+    /* Each cell reads only its own velocity: updating in place is the
+     * reference's collect-then-write. */
+    for (size_t j = j0; j <= j1; j++)
+        for (size_t i = i0; i <= i1; i++) {
+            size_t idx = j * res + i;
+            if (solid[idx] || h[idx] <= eps) continue;
+            real ux = u[idx], vx = v[idx];
+            for (int k = 0; k < 4; k++) {
+                long ni = (long)i + NB4[k][0], nj = (long)j + NB4[k][1];
+                if (!in_grid(ni, nj, res)) continue;
+                if (!solid[(size_t)nj * res + (size_t)ni]) continue;
+                real nx = (real)NB4[k][0], nz = (real)NB4[k][1];
+                real vn = ux * nx + vx * nz;
+                if (vn > K(0.0)) {
+                    ux -= vn * nx;
+                    vx -= vn * nz;
+                }
+            }
+            u[idx] = ux;
+            v[idx] = vx;
+        }
+}
+
+/* displace_occupied, :4382-4444: water in cells the body occupies is split
+ * over the open 4-neighbours (over the open cells within two cells when
+ * none is open); a column with nowhere to go stays. */
+void F(oracle_displace_occupied)(real *h, real *u, real *v, const uint8_t *solid, size_t res, real *give) {
+    size_t n = res * res;
+    for (size_t k = 0; k < n; k++) give[k] = K(0.0);
+    for (size_t j = 0; j < res; j++)
+        for (size_t i = 0; i < res; i++) {
+            size_t idx = j * res + i;
+            if (!solid[idx]) continue;
+            real vol = h[idx];
+            if (vol <= K(1.0e-8)) {
+                h[idx] = u[idx] = v[idx] = K(0.0);
+                continue;
+            }
+            size_t neigh[24];
+            int nn = 0;
+            for (int k = 0; k < 4; k++) {
+                long ni = (long)i + NB4[k][0], nj = (long)j + NB4[k][1];
+                if (!in_grid(ni, nj, res)) continue;
+                size_t nidx = (size_t)nj * res + (size_t)ni;
+                if (!solid[nidx]) neigh[nn++] = nidx;
+            }
+            if (nn == 0)
+                for (int dj = -2; dj <= 2; dj++)
+                    for (int di = -2; di <= 2; di++) {
+                        long ni = (long)i + di, nj = (long)j + dj;
+                        if (!in_grid(ni, nj, res)) continue;
+                        size_t nidx = (size_t)nj * res + (size_t)ni;
+                        if (!solid[nidx]) neigh[nn++] = nidx;
+                    }
+            if (nn == 0) continue;
+            h[idx] = u[idx] = v[idx] = K(0.0);
+            real share = vol / (real)nn;
+            for (int q = 0; q < nn; q++) give[neigh[q]] += share;
+        }
+    for (size_t k = 0; k < n; k++)
+        if (give[k] > K(0.0)) h[k] += give[k];
+}
+
+/* redirect_into_walls, :4446-4563: discharge aimed at a wall moves to the
+ * open flanks beside it (books over the box and a 1-cell rim). */
+void F(oracle_redirect_into_walls)(real *h, const real *u, const real *v, const uint8_t *solid, const real *bed,
+                                   size_t res, real cell, real dt, size_t i0, size_t i1, size_t j0, size_t j1,
+                                   real *removed, real *added) {
+    size_t ri0 = i0 ? i0 - 1 : 0, ri1 = i1 + 1 < res - 1 ? i1 + 1 : res - 1;
+    size_t rj0 = j0 ? j0 - 1 : 0, rj1 = j1 + 1 < res - 1 ? j1 + 1 : res - 1;
+    for (size_t j = rj0; j <= rj1; j++)
+        for (size_t i = ri0; i <= ri1; i++) removed[j * res + i] = added[j * res + i] = K(0.0);
+    for (size_t j = j0; j <= j1; j++)
+        for (size_t i = i0; i <= i1; i++) {
+            size_t idx = j * res + i;
+            if (solid[idx] || h[idx] < K(0.02)) continue;
+            for (int k = 0; k < 4; k++) {
+                int di = NB4[k][0], dj = NB4[k][1];
+                long ni = (long)i + di, nj = (long)j + dj;
+                if (!in_grid(ni, nj, res)) continue;
+                if (!solid[(size_t)nj * res + (size_t)ni]) continue;
+                real vn = u[idx] * (real)di + v[idx] * (real)dj;
+                long ui = (long)i - di, uj = (long)j - dj;
+                if (in_grid(ui, uj, res)) {
+                    size_t uidx = (size_t)uj * res + (size_t)ui;
+                    if (!solid[uidx]) {
+                        real feed = u[uidx] * (real)di + v[uidx] * (real)dj;
+                        if (feed > vn) vn = feed;
+                    }
+                }
+                if (vn < K(0.05)) continue;
+                int perps[2][2];
+                if (di != 0) {
+                    perps[0][0] = 0; perps[0][1] = -1; perps[1][0] = 0; perps[1][1] = 1;
+                } else {
+                    perps[0][0] = -1; perps[0][1] = 0; perps[1][0] = 1; perps[1][1] = 0;
+                }
+                size_t dest[2];
+                int nd = 0;
+                real flank = K(0.0);
+                for (int p = 0; p < 2; p++) {
+                    long qi = (long)i + perps[p][0], qj = (long)j + perps[p][1];
+                    if (!in_grid(qi, qj, res)) continue;
+                    size_t q = (size_t)qj * res + (size_t)qi;
+                    if (solid[q]) continue;
+                    if (h[q] <= K(1.0e-6) && bed[q] > bed[idx] + h[idx] + K(1.0e-3)) continue;
+                    dest[nd++] = q;
+                    flank += h[q];
+                }
+                if (nd == 0) continue;
+                real excess = h[idx] - flank / (real)nd;
+                if (excess < K(2.0e-3)) continue;
+                real room = h[idx] - removed[idx];
+                if (room < K(1.0e-6)) continue;
+                real frac = R_FMIN(vn * dt / cell, K(0.45));
+                real dh = R_FMIN(R_FMIN(excess, h[idx] * frac), room);
+                if (dh < K(1.0e-8)) continue;
+                removed[idx] += dh;
+                real share = dh / (real)nd;
+                for (int q = 0; q < nd; q++) added[dest[q]] += share;
+            }
+        }
+    for (size_t j = rj0; j <= rj1; j++)
+        for (size_t i = ri0; i <= ri1; i++) {
+            size_t idx = j * res + i;
+            if (removed[idx] > K(0.0) || added[idx] > K(0.0))
+                h[idx] = R_FMAX(h[idx] - removed[idx] + added[idx], K(0.0));
+        }
+}
+
+/* move_edge_bufs, :4178-4223 (solid and bed always given on this path). */
+static inline void move_edge_walled(const real *h, const uint8_t *gate, size_t a, size_t b, real vel, real cell,
+                                    real dt, const uint8_t *solid, const real *bed, real *removed, real *added) {
+    if (solid[a] || solid[b]) return;
+    if (R_FABS(vel) < K(1.0e-5)) return;
+    size_t donor = vel > K(0.0) ? a : b, recv = vel > K(0.0) ? b : a;
+    if (h[recv] <= K(1.0e-6) && bed[recv] > bed[donor] + h[donor] + K(1.0e-3)) return;
+    real room = h[donor] - removed[donor];
+    if (room <= K(1.0e-8)) return;
+    if (gate[recv] == 0 && h[recv] <= K(1.0e-6)) return;
+    real frac = R_FMIN(R_FABS(vel) * dt / cell, K(0.45));
+    real dh = R_FMIN(h[donor] * frac, room);
+    if (dh <= K(1.0e-8)) return;
+    removed[donor] += dh;
+    added[recv] += dh;
+}
+
+/* transport with solids, :3894-3977 (transport_par :3979-4148 walks the
+ * same edges in the same order per row or column). */
+void F(oracle_transport_walled)(real *h, const real *u, const real *v, const uint8_t *gate, const uint8_t *solid,
+                                const real *bed, size_t res, real cell, real dt, size_t i0, size_t i1, size_t j0,
+                                size_t j1, int along_x, real *removed, real *added) {
+    for (size_t j = j0; j <= j1; j++)
+        for (size_t i = i0; i <= i1; i++) removed[j * res + i] = added[j * res + i] = K(0.0);
+    if (along_x) {
+        for (size_t j = j0; j <= j1; j++)
+            for (size_t i = i0; i < i1; i++) {
+                size_t a = j * res + i;
+                move_edge_walled(h, gate, a, a + 1, K(0.5) * (u[a] + u[a + 1]), cell, dt, solid, bed, removed,
+                                 added);
+            }
+    } else {
+        for (size_t j = j0; j < j1; j++)
+            for (size_t i = i0; i <= i1; i++) {
+                size_t a = j * res + i;
+                move_edge_walled(h, gate, a, a + res, K(0.5) * (v[a] + v[a + res]), cell, dt, solid, bed,
+                                 removed, added);
+            }
+    }
+    Box b = {i0, i1, j0, j1};
+    apply_moves(h, removed, added, res, b);
+}
+
+/*
+ * Gravity beside walls, force_gravity_solid :2101-2128 (one pass: kick and
+ * confinement cell by cell in raster order) and force_gravity_solid_par
+ * :2131-2200 (every kick first, then the confinement sweep): the large-box
+ * form with row splits allowed, as the reference picks it.  Wind on a walled
+ * sheet: blend and confinement cell by cell (:1950-1964).
+ */
+static void force_walled(const real *h, real *u, real *v, const real *nu, const real *p, const uint8_t *mask,
+                         size_t res, real cell, real dt, Drive drive, real eps, Box b, int inner_par) {
+    if (drive.wind) {
+        real blend = K(1.0) - R_EXP(-dt * K(1.6));
+        for (size_t j = b.j0; j <= b.j1; j++)
+            for (size_t i = b.i0; i <= b.i1; i++) {
+                size_t idx = j * res + i;
+                if (h[idx] <= eps) continue;
+                u[idx] += (drive.wx - u[idx]) * blend;
+                v[idx] += (drive.wz - v[idx]) * blend;
+                if (drive.vorticity > K(0.0)) confine(u, v, u, v, res, i, j, cell, dt, drive.vorticity);
+            }
+        return;
+    }
+    int par = (b.j1 - b.j0 >= 47) && (b.i1 - b.i0 >= 31) && inner_par;
+    for (size_t j = b.j0; j <= b.j1; j++)
+        for (size_t i = b.i0; i <= b.i1; i++) {
+            size_t idx = j * res + i;
+            if (h[idx] <= eps) continue;
+            real px, pz;
+            grad(p, res, i, j, cell, &px, &pz);
+            real mob = F(oracle_sheet_mobility)(nu[idx]);
+            if (h[idx] > K(0.08) && solid_near(mask, res, i, j, 1)) mob = R_FMAX(mob, K(1.0));
+            u[idx] += -GRAVITY * px * mob * dt;
+            v[idx] += -GRAVITY * pz * mob * dt;
+            if (!par && drive.vorticity > K(0.0)) confine(u, v, u, v, res, i, j, cell, dt, drive.vorticity);
+        }
+    if (!par || drive.vorticity <= K(0.0)) return;
+    for (size_t j = b.j0; j <= b.j1; j++)
+        for (size_t i = b.i0; i <= b.i1; i++) {
+            if (h[j * res + i] <= eps) continue;
+            confine(u, v, u, v, res, i, j, cell, dt, drive.vorticity);
+        }
+}
+
+/* The substep with walls (:1879-2082, solid = Some).  Scratch e doubles as
+ * the lift's and the ghosts' temporary, su / sv as the ghost velocities. */
+static void substep_walled(real *h, real *u, real *v, const real *bed, const real *nu, const uint8_t *gate,
+                           const uint8_t *mask, size_t res, real cell, real dt, Drive drive, real eps, Box b,
+                           int inner_par, Scratch *sc) {
+    size_t hi0 = b.i0 >= 2 ? b.i0 - 2 : 0, hi1 = (b.i1 + 2 < res - 1) ? b.i1 + 2 : res - 1;
+    size_t hj0 = b.j0 >= 2 ? b.j0 - 2 : 0, hj1 = (b.j1 + 2 < res - 1) ? b.j1 + 2 : res - 1;
+    for (size_t j = hj0; j <= hj1; j++)
+        for (size_t i = hi0; i <= hi1; i++) {
+            size_t k = j * res + i;
+            sc->b[k] = sc->c[k] = sc->d[k] = sc->e[k] = K(0.0);
+            sc->a[k] = bed[k] + h[k];
+        }
+    F(oracle_lift_solid_surface)(sc->a, h, mask, res, b.i0, b.i1, b.j0, b.j1, eps, sc->e);
+    blur_box(sc->a, sc->b, sc->e, res, b, GAUSS_F);
+    blur_box(sc->a, sc->c, sc->e, res, b, STEEP_F);
+    for (size_t j = b.j0; j <= b.j1; j++)
+        for (size_t i = b.i0; i <= b.i1; i++) {
+            size_t idx = j * res + i;
+            real filtered = K(0.62) * sc->b[idx] + K(0.38) * sc->c[idx];
+            sc->d[idx] = K(0.45) * filtered + K(0.55) * sc->a[idx];
+        }
+    F(oracle_noslip_ghost)(u, v, h, mask, res, b.i0, b.i1, b.j0, b.j1, eps, sc->su, sc->sv);
+    force_walled(h, u, v, nu, sc->d, mask, res, cell, dt, drive, eps, b, inner_par);
+    /* The projection always runs beside walls (:1978-2002). */
+    for (size_t j = b.j0; j <= b.j1; j++)
+        for (size_t i = b.i0; i <= b.i1; i++) {
+            size_t idx = j * res + i;
+            size_t ip = i + 1 < res ? i + 1 : res - 1, im = i ? i - 1 : 0;
+            size_t jp = j + 1 < res ? j + 1 : res - 1, jm = j ? j - 1 : 0;
+            real ux = (u[j * res + ip] - u[j * res + im]) / (K(2.0) * cell);
+            real vz = (v[jp * res + i] - v[jm * res + i]) / (K(2.0) * cell);
+            real div = ux + vz;
+            real lap_h = lap(sc->a, res, i, j, cell);
+            real overload = K(1.0) + clampr(nu[idx] / K(231.0), K(0.0), K(6.0));
+            sc->b[idx] = div * overload - K(0.015) * overload * lap_h;
+        }
+    blur_box(sc->b, sc->c, sc->e, res, b, GAUSS_F);
+    blur_box(sc->b, sc->a, sc->e, res, b, STEEP_F);
+    for (size_t j = b.j0; j <= b.j1; j++)
+        for (size_t i = b.i0; i <= b.i1; i++) {
+            size_t idx = j * res + i;
+            sc->d[idx] = K(0.62) * sc->c[idx] + K(0.38) * sc->a[idx];
+        }
+    real lim = drive.wind ? R_FMAX(R_HYPOT(drive.wx, drive.wz), FILM_SPEED_CAP) : FILM_SPEED_CAP;
+    for (size_t j = b.j0; j <= b.j1; j++) {
+        int memo_set = 0;
+        real memo_nu = K(0.0), memo_decay = K(1.0);
+        for (size_t i = b.i0; i <= b.i1; i++) {
+            size_t idx = j * res + i;
+            if (h[idx] <= eps) continue;
+            real qx, qz;
+            grad(sc->d, res, i, j, cell, &qx, &qz);
+            u[idx] += K(0.012) * dt * qx;
+            v[idx] += K(0.012) * dt * qz;
+            real nu_here = nu[idx];
+            if (!memo_set || memcmp(&nu_here, &memo_nu, sizeof(real)) != 0) {
+                memo_set = 1;
+                memo_nu = nu_here;
+                memo_decay = R_EXP(-F(oracle_sheet_damp)(nu_here) * dt);
+            }
+            u[idx] = finite_speed(u[idx] * memo_decay, lim);
+            v[idx] = finite_speed(v[idx] * memo_decay, lim);
+        }
+    }
+    F(oracle_redirect_into_walls)(h, u, v, mask, bed, res, cell, dt, b.i0, b.i1, b.j0, b.j1, sc->removed,
+                                  sc->added);
+    F(oracle_block_into_solid)(u, v, h, mask, res, b.i0, b.i1, b.j0, b.j1, eps);
+    F(oracle_noslip_ghost)(u, v, h, mask, res, b.i0, b.i1, b.j0, b.j1, eps, sc->su, sc->sv);
+    F(oracle_transport_walled)(h, u, v, gate, mask, bed, res, cell, dt, b.i0, b.i1, b.j0, b.j1, 1, sc->removed,
+                               sc->added);
+    F(oracle_transport_walled)(h, u, v, gate, mask, bed, res, cell, dt, b.i0, b.i1, b.j0, b.j1, 0, sc->removed,
+                               sc->added);
+    for (size_t j = b.j0; j <= b.j1; j++)
+        for (size_t i = b.i0; i <= b.i1; i++) {
+            size_t idx = j * res + i;
+            if (eps > K(0.0) && h[idx] < eps) h[idx] = u[idx] = v[idx] = K(0.0);
+        }
+}
+
 /*
  * The step: step_sheet_eps (:856-889) and step_sheet_scratch (:1659-1831).
  * Returns the number of substeps taken (0 when the call is a no-op).
@@ -1155,6 +1534,54 @@ int F(oracle_step_sheet_ex)(real *h, real *u, real *v, const real *bed, const re
                         confine_mode, &sc);
             }
     }
+    return (int)nsub;
+}
+
+
+/*
+ * The walled step: step_sheet_eps_in (:856-889) with solid = Some and the
+ * plan of step_sheet_scratch (:1659-1831) for it: wet-set work boxes, every
+ * substep at Courant 0.42 (wind too), no pace policy.  solid: res * res
+ * bytes, nonzero = wall.  Returns the substep count.
+ */
+int F(oracle_step_sheet_walled)(real *h, real *u, real *v, const real *bed, const real *nu, const uint8_t *gate,
+                                const uint8_t *solid, size_t res, real cell, real dt, int drive_wind, real wind_x,
+                                real wind_z, real vorticity, real eps, real sub_cap, int inner_par) {
+    if (!(dt > K(0.0)) || res == 0) return 0;
+    size_t n = res * res;
+    real sum = wet_sum(h, n, eps);
+    if (sum <= eps) return 0;
+    Drive drive = {drive_wind, wind_x, wind_z, vorticity};
+    size_t raw[4 * 4096];
+    size_t nb = F(oracle_sheet_work_boxes)(h, res, eps, 1, raw, 4096);
+    if (nb > 4096) nb = 4096;
+    if (nb == 0) return 0;
+    real cap = R_ISFINITE(sub_cap) ? clampr(sub_cap, K(1.0), K(8.0)) : K(8.0);
+    real max_s = K(0.15);
+    if (drive_wind) max_s = R_FMAX(max_s, R_HYPOT(wind_x, wind_z));
+    real max_s2 = K(0.0);
+    for (size_t k = 0; k < nb; k++)
+        for (size_t j = raw[4 * k + 2]; j <= raw[4 * k + 3]; j++)
+            for (size_t i = raw[4 * k + 0]; i <= raw[4 * k + 1]; i++) {
+                real uu = u[j * res + i], vv = v[j * res + i];
+                real s2 = uu * uu + vv * vv;
+                if (s2 > max_s2) max_s2 = s2;
+            }
+    max_s = R_FMAX(max_s, R_SQRT(max_s2));
+    real q = R_CEIL((dt * max_s) / (K(0.42) * R_FMAX(cell, K(1.0e-3))));
+    uint32_t nsub = (uint32_t)clampr(q, K(1.0), cap);
+    real dts = dt / (real)nsub;
+    real *pool = calloc(9 * n, sizeof(real));
+    if (!pool) return -1;
+    Scratch sc;
+    sc.a = pool; sc.b = pool + n; sc.c = pool + 2 * n; sc.d = pool + 3 * n; sc.e = pool + 4 * n;
+    sc.removed = pool + 5 * n; sc.added = pool + 6 * n; sc.su = pool + 7 * n; sc.sv = pool + 8 * n;
+    for (uint32_t s = 0; s < nsub; s++)
+        for (size_t k = 0; k < nb; k++) {
+            Box b = {raw[4 * k], raw[4 * k + 1], raw[4 * k + 2], raw[4 * k + 3]};
+            substep_walled(h, u, v, bed, nu, gate, solid, res, cell, dts, drive, eps, b, inner_par, &sc);
+        }
+    free(pool);
     return (int)nsub;
 }
 
