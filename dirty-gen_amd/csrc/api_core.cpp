@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "core.hpp"
+#include "tracers.hpp"
 #include "mlp.hpp"
 #include "plan.hpp"
 #include "rut.hpp"
@@ -44,7 +45,7 @@ struct silt_core {
   float moisture_ambient = 0.0f;  // read_substrate's most common moisture
   // Coarse cells parked off the window by recenter, by world cell key.
   struct Parked {
-    float water, slurry, uw, vw, us, vs, moisture;
+    float water, slurry, uw, vw, us, vs, moisture, foam, turbid;
   };
   std::unordered_map<uint64_t, Parked> parked;
   float fine_origin[2] = {0, 0};
@@ -85,6 +86,11 @@ struct silt_core {
   float* d_bed_drop = nullptr;
   double* d_eroded = nullptr;
   unsigned* d_hole = nullptr;
+  // Foam and stirred-silt tracers of the window, their pass control and the
+  // last step's pushes (note_surface_foam reads them).
+  TracerPlanes tr{};
+  TracerState* d_tstate = nullptr;
+  std::vector<CorePush> last_pushes;
   // Ground under the sheet (SoilField bed / placed / initial height on the
   // window grid), set by silt_core_set_ground and cut by the rut stamp.
   bool ground = false;
@@ -879,6 +885,40 @@ silt_hydro_load hydro_load_host(const silt_core* c, const std::vector<uint8_t>& 
   return L;
 }
 
+// A tire wakes the tracers at 0.25 m/s or slipping 5 % (step_tracers
+// :6215-6219), and its 8 m reach joins the visit box (tracer_visit_box
+// :6313-6330).
+bool tracer_wake(const std::vector<CorePush>& p) {
+  for (const CorePush& q : p)
+    if ((isfinite(q.speed) && q.speed >= 0.25f) || (isfinite(q.slip) && fabsf(q.slip) >= 0.05f)) return true;
+  return false;
+}
+
+SpanBox tracer_push_box(const silt_core* c, const std::vector<CorePush>& p) {
+  SpanBox b{kWindowRes, -1, kWindowRes, -1};
+  const int reach = (int)ceilf(8.0f / kCoarseCell);
+  for (const CorePush& q : p) {
+    const int ci = coarse_axis(c->origin[0], q.x), cj = coarse_axis(c->origin[1], q.z);
+    if (ci < 0 || cj < 0) continue;
+    b.i0 = std::min(b.i0, std::max(ci - reach, 0));
+    b.i1 = std::max(b.i1, std::min(ci + reach, kWindowRes - 1));
+    b.j0 = std::min(b.j0, std::max(cj - reach, 0));
+    b.j1 = std::max(b.j1, std::min(cj + reach, kWindowRes - 1));
+  }
+  return b;
+}
+
+// step_tracers after the window's step (FluidCore::step :6079).
+int tracer_pass(silt_core* c, const std::vector<CorePush>& pushes, float dt) {
+  if (!(dt > 0.0f) || !isfinite(dt)) return 0;
+  TracerPushes tp{};
+  tp.n = (int)std::min(pushes.size(), (size_t)kMaxPush);
+  for (int k = 0; k < tp.n; k++) tp.p[k] = pushes[k];
+  tracer_step(c->tr, c->c, c->d_tstate, tp, tracer_wake(pushes) ? 1 : 0, tracer_push_box(c, pushes), dt,
+              expf(-dt / 1.8f), expf(-dt / 4.0f), c->origin[0], c->origin[1], c->st);
+  return hipGetLastError() == hipSuccess ? 0 : SILT_EDEVICE;
+}
+
 // The ledger's volumes (FluidCore::ledger, src/fluid_core.rs:6821-6883) in
 // FP64 over the window and, while it is on, the 2 cm patch; with book, the
 // booked lines too, read in the same wait.
@@ -983,7 +1023,10 @@ silt_core* silt_core_create(float center_x, float center_z, int path) {
        dalloc(c, &c->d_stats, 2) && dalloc(c, &c->d_lstats, 4) && dalloc(c, &c->d_pub, 13 * WN) &&
        dalloc(c, &c->d_book, 2 * kBookDoubles) && dalloc(c, &c->d_solids, kMaxSolids) &&
        dalloc(c, &c->d_mask, WN) && dalloc(c, &c->d_fmask, FN) && dalloc(c, &c->d_bed_drop, WN) &&
-       dalloc(c, &c->d_eroded, 1) && dalloc(c, &c->d_hole, 1) &&
+       dalloc(c, &c->d_eroded, 1) && dalloc(c, &c->d_hole, 1) && dalloc(c, &c->tr.foam, WN) &&
+       dalloc(c, &c->tr.turbid, WN) && dalloc(c, &c->tr.foam_live, WN) && dalloc(c, &c->tr.wet, WN) &&
+       dalloc(c, &c->tr.shore, WN) && dalloc(c, &c->tr.scratch, 2 * WN) && dalloc(c, &c->tr.acc, 2) &&
+       dalloc(c, &c->d_tstate, 1) &&
        hipHostMalloc((void**)&c->h_stats, 2 * sizeof(SheetStats), hipHostMallocDefault) == hipSuccess;
   if (ok) {
     // Grain::Other everywhere (FluidCore::new, src/fluid_core.rs:4979-4980).
@@ -1215,6 +1258,8 @@ int silt_core_step(silt_core* c, float dt, silt_spray_event* events, size_t cap,
   HIPCHK(hipMemsetAsync(c->d_acc, 0, sizeof(double), st));
   int rc = step_by_model(c, pushes, pl, dt);
   if (rc) return rc;
+  if ((rc = tracer_pass(c, pushes, dt))) return rc;
+  c->last_pushes = pushes;
   unsigned nev = 0;
   double vol = 0.0;
   HIPCHK(hipMemcpyAsync(&nev, c->d_nev, sizeof nev, hipMemcpyDeviceToHost, st));
@@ -1348,7 +1393,7 @@ int silt_core_publish_surface(silt_core* c, const float* soil_wetness, const flo
   } else if (c->ground) {
     d_placed = c->soil_placed;  // the core's own ground, as silt_core_publish publishes it
   }
-  core_publish_surface(c->c, d_wet, d_placed, kWindowRes, d + 2 * WN, d + 3 * WN, d + 4 * WN, c->st);
+  core_publish_surface(c->c, d_wet, d_placed, c->tr.foam, kWindowRes, d + 2 * WN, d + 3 * WN, d + 4 * WN, c->st);
   if (moisture1) HIPCHK(hipMemcpyAsync(moisture1, d + 2 * WN, WN * 4, hipMemcpyDeviceToHost, c->st));
   if (wet_edge1) HIPCHK(hipMemcpyAsync(wet_edge1, d + 3 * WN, WN * 4, hipMemcpyDeviceToHost, c->st));
   if (foam_crest2) HIPCHK(hipMemcpyAsync(foam_crest2, d + 4 * WN, 2 * WN * 4, hipMemcpyDeviceToHost, c->st));
@@ -1400,7 +1445,7 @@ int silt_core_publish_surface_device(silt_core* c, const float* soil_wetness, co
   if (fence_from(c, user)) return SILT_EDEVICE;
   const float* d_placed = soil_placed ? soil_placed : (c->ground ? c->soil_placed : nullptr);
   float* d = c->d_pub;
-  core_publish_surface(c->c, soil_wetness, d_placed, kWindowRes, moisture1 ? moisture1 : d + 2 * WN,
+  core_publish_surface(c->c, soil_wetness, d_placed, c->tr.foam, kWindowRes, moisture1 ? moisture1 : d + 2 * WN,
                        wet_edge1 ? wet_edge1 : d + 3 * WN, foam_crest2 ? foam_crest2 : d + 4 * WN, c->st);
   if (hipGetLastError() != hipSuccess) return SILT_EDEVICE;
   return fence_to(c, user) ? SILT_EDEVICE : SILT_OK;
@@ -1604,6 +1649,42 @@ int silt_core_hydro_load(silt_core* c, silt_hydro_load* out) {
   return SILT_OK;
 }
 
+int silt_core_note_surface_foam(silt_core* c, float dt) {
+  if (!c) return SILT_EINVAL;
+  if (!(dt > 0.0f) || !isfinite(dt)) return SILT_OK;
+  tracer_note_foam(c->tr, c->c, c->d_tstate, tracer_wake(c->last_pushes) ? 1 : 0, tracer_push_box(c, c->last_pushes),
+                   dt, c->st);
+  HIPCHK(hipStreamSynchronize(c->st));
+  return hipGetLastError() == hipSuccess ? SILT_OK : SILT_EDEVICE;
+}
+
+int silt_core_get_tracers(silt_core* c, float* foam, float* turbid, double* accounts2) {
+  if (!c) return SILT_EINVAL;
+  if (foam) HIPCHK(hipMemcpyAsync(foam, c->tr.foam, WN * 4, hipMemcpyDeviceToHost, c->st));
+  if (turbid) HIPCHK(hipMemcpyAsync(turbid, c->tr.turbid, WN * 4, hipMemcpyDeviceToHost, c->st));
+  if (accounts2) HIPCHK(hipMemcpyAsync(accounts2, c->tr.acc, 2 * sizeof(double), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  return SILT_OK;
+}
+
+int silt_core_set_tracers(silt_core* c, const float* foam, const float* turbid) {
+  if (!c) return SILT_EINVAL;
+  std::vector<float> f(WN);
+  if (foam) {
+    HIPCHK(hipMemcpyAsync(c->tr.foam, foam, WN * 4, hipMemcpyHostToDevice, c->st));
+    memcpy(f.data(), foam, WN * 4);
+  } else {
+    HIPCHK(hipMemcpyAsync(f.data(), c->tr.foam, WN * 4, hipMemcpyDeviceToHost, c->st));
+  }
+  if (turbid) HIPCHK(hipMemcpyAsync(c->tr.turbid, turbid, WN * 4, hipMemcpyHostToDevice, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  std::vector<uint8_t> live(WN);
+  for (int k = 0; k < WN; k++) live[k] = isfinite(f[k]) && f[k] > 1.0e-4f;
+  HIPCHK(hipMemcpyAsync(c->tr.foam_live, live.data(), WN, hipMemcpyHostToDevice, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  return SILT_OK;
+}
+
 int silt_core_ledger_full(silt_core* c, silt_fluid_ledger_full* out) {
   if (!out) return SILT_EINVAL;
   silt_fluid_ledger l{};
@@ -1785,8 +1866,11 @@ uint64_t park_key(float x, float z) {
 }
 
 // park_slot (src/fluid_core.rs:4797-4805) for the planes this library holds.
+// park_slot (src/fluid_core.rs:4797-4805) without the dust and suspended
+// columns this library does not carry.
 bool park_worth(const silt_core::Parked& p, float ambient) {
-  return p.water > 1.0e-8f || p.slurry > 1.0e-8f || fabsf(p.moisture - ambient) > 6.0e-3f;
+  return p.water > 1.0e-8f || p.slurry > 1.0e-8f || fabsf(p.moisture - ambient) > 6.0e-3f || p.foam > 1.0e-4f ||
+         p.turbid > 1.0e-8f;
 }
 
 // each_leaving (src/fluid_core.rs:9936-9986): cells that leave the window
@@ -1857,16 +1941,18 @@ int silt_core_recenter(silt_core* c, float x, float z) {
   const float nw[2] = {floorf((x - extent * 0.5f) / kCoarseCell) * kCoarseCell,
                        floorf((z - extent * 0.5f) / kCoarseCell) * kCoarseCell};
   if (fabsf(nw[0] - c->origin[0]) < 1.0e-4f && fabsf(nw[1] - c->origin[1]) < 1.0e-4f) return SILT_OK;
-  float* dev[7] = {c->c.water, c->c.slurry, c->c.uw, c->c.vw, c->c.us, c->c.vs, c->c.moisture};
-  std::vector<float> h[7], o[7];
-  for (int k = 0; k < 7; k++) {
+  float* dev[9] = {c->c.water, c->c.slurry, c->c.uw, c->c.vw, c->c.us, c->c.vs, c->c.moisture, c->tr.foam,
+                   c->tr.turbid};
+  std::vector<float> h[9], o[9];
+  for (int k = 0; k < 9; k++) {
     h[k].resize(WN);
     HIPCHK(hipMemcpyAsync(h[k].data(), dev[k], WN * 4, hipMemcpyDeviceToHost, c->st));
   }
   HIPCHK(hipStreamSynchronize(c->st));
   const float amb = c->moisture_ambient;
   auto slot_at = [&](int idx) {
-    return silt_core::Parked{h[0][idx], h[1][idx], h[2][idx], h[3][idx], h[4][idx], h[5][idx], h[6][idx]};
+    return silt_core::Parked{h[0][idx], h[1][idx], h[2][idx], h[3][idx], h[4][idx],
+                             h[5][idx], h[6][idx], h[7][idx], h[8][idx]};
   };
   auto park = [&](const float* org, int i, int j) {
     const int idx = j * kWindowRes + i;
@@ -1883,11 +1969,11 @@ int silt_core_recenter(silt_core* c, float x, float z) {
     auto it = c->parked.find(key);
     if (it == c->parked.end()) return;
     const silt_core::Parked& p = it->second;
-    const float v[7] = {p.water, p.slurry, p.uw, p.vw, p.us, p.vs, p.moisture};
-    for (int k = 0; k < 7; k++) o[k][idx] = v[k];
+    const float v[9] = {p.water, p.slurry, p.uw, p.vw, p.us, p.vs, p.moisture, p.foam, p.turbid};
+    for (int k = 0; k < 9; k++) o[k][idx] = v[k];
     c->parked.erase(it);
   };
-  for (int k = 0; k < 7; k++) o[k].assign(WN, k == 6 ? amb : 0.0f);
+  for (int k = 0; k < 9; k++) o[k].assign(WN, k == 6 ? amb : 0.0f);
   int di = 0, dj = 0;
   if (uniform_cell_shift(c->origin, nw, &di, &dj)) {
     const float old[2] = {c->origin[0], c->origin[1]};
@@ -1897,8 +1983,10 @@ int silt_core_recenter(silt_core* c, float x, float z) {
         const int si = i + di, sj = j + dj;
         if (si < 0 || sj < 0 || si >= kWindowRes || sj >= kWindowRes) continue;
         const int src = sj * kWindowRes + si, dst = j * kWindowRes + i;
-        if (!(h[0][src] > 1.0e-8f || h[1][src] > 1.0e-8f || h[6][src] > 1.0e-4f)) continue;
-        for (int k = 0; k < 7; k++) o[k][dst] = h[k][src];
+        if (!(h[0][src] > 1.0e-8f || h[1][src] > 1.0e-8f || h[6][src] > 1.0e-4f || h[7][src] > 1.0e-5f ||
+              h[8][src] > 1.0e-8f))
+          continue;
+        for (int k = 0; k < 9; k++) o[k][dst] = h[k][src];
       }
     c->origin[0] = nw[0];
     c->origin[1] = nw[1];
@@ -1912,7 +2000,9 @@ int silt_core_recenter(silt_core* c, float x, float z) {
     for (int j = 0; j < kWindowRes; j++)
       for (int i = 0; i < kWindowRes; i++) restore(c->origin, i, j);
   }
-  for (int k = 0; k < 7; k++) HIPCHK(hipMemcpyAsync(dev[k], o[k].data(), WN * 4, hipMemcpyHostToDevice, c->st));
+  for (int k = 0; k < 9; k++) HIPCHK(hipMemcpyAsync(dev[k], o[k].data(), WN * 4, hipMemcpyHostToDevice, c->st));
+  // reset_tracer_indices (:9228-9233): no foam is live after a move.
+  core_fill_bytes(c->tr.foam_live, 0, WN, c->st);
   HIPCHK(hipStreamSynchronize(c->st));
   return SILT_OK;
 }

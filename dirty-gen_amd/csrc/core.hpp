@@ -99,6 +99,7 @@ void core_peel(const GridGeo& g, float* water, const float* uw, const float* vw,
 void core_soften(float* u, float* v, int res, hipStream_t st);
 void core_nu(float* nu, const uint8_t* grain, const float* moisture, int n, hipStream_t st);
 void core_fill(float* p, float val, int n, hipStream_t st);
+void core_fill_bytes(uint8_t* p, uint8_t val, int n, hipStream_t st);
 void core_refine_copy(const CorePlanes& c, const FinePlanes& f, const uint8_t* hold, const FineAxes& ax, int ci0,
                       int cj0, int cw, int ch, hipStream_t st);
 // frozen[0..1]: FP64 water / slurry volume parked off the window.
@@ -118,8 +119,6 @@ void core_ground(const float* soil_bed, const float* soil_placed, const float* s
                  int n, hipStream_t st);
 void core_publish(const CorePlanes& c, const float* soil_bed, const float* soil_placed, const float* soil_rut,
                   float* ground_rut4, float* film2, float* flow4, hipStream_t st);
-// moisture (max of the sheet's and the soil's wetness), wet_edge and
-// foam_crest (0, placed + film excess) maps; wetness / placed may be null.
 // One hydro solid as the wall mask tests it (solid_mask, src/fluid_core.rs:
 // 7288-7362): the rectangle's centre, axes (sin / cos of the yaw, from the
 // host), half sizes, the heights between which it is a wall, the cell range
@@ -142,7 +141,10 @@ void core_solid_mask(const float* water, const float* bed, int res, float cell, 
 void core_scour(const CorePlanes& c, const uint8_t* mask, float* bed_drop, float dt, double* eroded,
                 unsigned* hole_bits, double* source, hipStream_t st);
 
-void core_publish_surface(const CorePlanes& c, const float* wetness, const float* placed, int res, float* moisture1,
-                          float* wet_edge1, float* foam2, hipStream_t st);
+// moisture (max of the sheet's and the soil's wetness), wet_edge and
+// foam_crest (foam tracer, placed + film excess) maps; wetness / placed /
+// foam may be null.
+void core_publish_surface(const CorePlanes& c, const float* wetness, const float* placed, const float* foam, int res,
+                          float* moisture1, float* wet_edge1, float* foam2, hipStream_t st);
 
 }  // namespace silt

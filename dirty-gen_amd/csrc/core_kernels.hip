@@ -751,8 +751,8 @@ __device__ inline bool surface_wet(const CorePlanes& c, const float* wetness, in
   return surface_moisture(c, wetness, k) >= 0.45f || fin0(c.water[k]) + fin0(c.slurry[k]) >= 0.004f;
 }
 
-__global__ void k_publish_surface(CorePlanes c, const float* wetness, const float* placed, int res, float* moisture1,
-                                  float* wet_edge1, float* foam2) {
+__global__ void k_publish_surface(CorePlanes c, const float* wetness, const float* placed, const float* foam,
+                                  int res, float* moisture1, float* wet_edge1, float* foam2) {
   const int k = (int)(blockIdx.x * blockDim.x + threadIdx.x);
   if (k >= res * res) return;
   const int i = k % res, j = k / res;
@@ -784,7 +784,9 @@ __global__ void k_publish_surface(CorePlanes c, const float* wetness, const floa
     const float mean = count > 0.0f ? sum / count : h;
     const float excess = fin0(fmaxf(h - mean, 0.0f));
     const float pl = placed ? fin0(fmaxf(placed[k], 0.0f)) : 0.0f;
-    foam2[2 * k] = 0.0f;  // no foam tracer
+    // The foam tracer where the film is at least 4 mm (render_contract.rs:563-566).
+    const float f = foam ? fin0(foam[k]) : 0.0f;
+    foam2[2 * k] = h >= 0.004f ? fminf(fmaxf(f, 0.0f), 1.0f) : 0.0f;
     foam2[2 * k + 1] = fin0(pl + excess);
   }
 }
@@ -837,6 +839,15 @@ void core_soften(float* u, float* v, int res, hipStream_t st) {
 
 void core_nu(float* nu, const uint8_t* grain, const float* moisture, int n, hipStream_t st) {
   hipLaunchKernelGGL(k_nu, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, nu, grain, moisture, n);
+}
+
+__global__ void k_fill_bytes(uint8_t* p, uint8_t val, int n) {
+  int k = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (k < n) p[k] = val;
+}
+
+void core_fill_bytes(uint8_t* p, uint8_t val, int n, hipStream_t st) {
+  hipLaunchKernelGGL(k_fill_bytes, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, val, n);
 }
 
 void core_fill(float* p, float val, int n, hipStream_t st) {
@@ -984,11 +995,11 @@ void core_scour(const CorePlanes& c, const uint8_t* mask, float* bed_drop, float
                      source);
 }
 
-void core_publish_surface(const CorePlanes& c, const float* wetness, const float* placed, int res, float* moisture1,
-                          float* wet_edge1, float* foam2, hipStream_t st) {
+void core_publish_surface(const CorePlanes& c, const float* wetness, const float* placed, const float* foam, int res,
+                          float* moisture1, float* wet_edge1, float* foam2, hipStream_t st) {
   const int n = res * res;
-  hipLaunchKernelGGL(k_publish_surface, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, c, wetness, placed, res,
-                     moisture1, wet_edge1, foam2);
+  hipLaunchKernelGGL(k_publish_surface, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, c, wetness, placed, foam,
+                     res, moisture1, wet_edge1, foam2);
 }
 
 }  // namespace silt

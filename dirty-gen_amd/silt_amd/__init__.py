@@ -522,6 +522,9 @@ def _declare_core(L):
     L.silt_core_set_hydro_solids.argtypes = [_P, C.POINTER(HydroSolid), _Z]
     L.silt_core_hydro_load.argtypes = [_P, C.POINTER(HydroLoad)]
     L.silt_core_solid_mask.argtypes = [_P, C.c_int, _P]
+    L.silt_core_note_surface_foam.argtypes = [_P, _F]
+    L.silt_core_get_tracers.argtypes = [_P, _P, _P, _P]
+    L.silt_core_set_tracers.argtypes = [_P, _P, _P]
     L.silt_core_recenter.argtypes = [_P, _F, _F]
     L.silt_core_window_bounds.argtypes = [_P, _P]
     L.silt_core_lay_slurry_on_water.argtypes = [_P, _F, C.c_int]
@@ -767,6 +770,22 @@ class FluidCore:
         out = HydroLoad()
         _check_status(lib().silt_core_hydro_load(self._c, C.byref(out)))
         return {k: getattr(out, k) for k, _ in HydroLoad._fields_}
+
+    def note_surface_foam(self, dt):
+        _check_status(lib().silt_core_note_surface_foam(self._c, dt))
+
+    def tracers(self):
+        """(foam, turbid, (stirred_m3, settled_m3)) of the window."""
+        n = WINDOW_RES * WINDOW_RES
+        foam, turbid, acc = np.zeros(n, np.float32), np.zeros(n, np.float32), np.zeros(2, np.float64)
+        _check_status(lib().silt_core_get_tracers(self._c, _ptr(foam), _ptr(turbid), _ptr(acc)))
+        return foam, turbid, (float(acc[0]), float(acc[1]))
+
+    def set_tracers(self, foam=None, turbid=None):
+        f = None if foam is None else np.ascontiguousarray(foam, np.float32)
+        t = None if turbid is None else np.ascontiguousarray(turbid, np.float32)
+        _check_status(lib().silt_core_set_tracers(self._c, None if f is None else _ptr(f),
+                                                  None if t is None else _ptr(t)))
 
     def solid_mask(self, fine=False):
         out = np.zeros((160 if fine else WINDOW_RES) ** 2, np.uint8)

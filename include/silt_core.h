@@ -46,7 +46,8 @@ typedef struct silt_fluid_ledger {
 
 /* FluidLedger (src/fluid_core.rs:483-509) with all 15 fields, FP64.  The
  * dust, fog, suspended-solid and river-reach lines belong to subsystems this
- * library does not build (the air grid, tracers, walled reaches) and are 0. */
+ * library does not build (the air grid, the suspended-solid column, river
+ * reaches) and are 0. */
 typedef struct silt_fluid_ledger_full {
     double water_m3, slurry_m3, dust_air_m3, dust_settled_m3, frozen_water_m3, frozen_slurry_m3, frozen_dust_m3,
         spray_m3, fog_air_m3, suspended_m3, settled_suspended_m3, frozen_suspended_m3, entrained_m3, reach_m3,
@@ -147,6 +148,22 @@ int silt_core_set_hydro_solids(silt_core *c, const silt_hydro_solid *solids, siz
 /* hydro_load (src/fluid_core.rs:7164-7249): pressure force on the walls,
  * buoyancy and the scour account, read from the window. */
 int silt_core_hydro_load(silt_core *c, silt_hydro_load *out);
+
+/* The foam and stirred-silt tracers (src/fluid_core.rs:6210-6750): every
+ * silt_core_step ends with step_tracers (:6079), which carries foam
+ * (semi-Lagrangian, 1.8 s life) and stirred silt (donor-cell, settling over
+ * 4 s) on the window and spawns foam where tires cut the surface, at shores
+ * and where the film runs supercritical, and silt where a tire slips and at
+ * shores.  foam_crest's foam channel publishes the foam.
+ * note_surface_foam (:6526-6538): the crest foam of a seated sheet, for the
+ * caller that seats the rut water after the step (ThroughDrive). */
+int silt_core_note_surface_foam(silt_core *c, float dt);
+/* foam fraction and stirred silt (m), 200 x 200 each (either may be NULL);
+ * accounts2 = FP64 silt volume stirred in and settled out (m^3), or NULL. */
+int silt_core_get_tracers(silt_core *c, float *foam, float *turbid, double *accounts2);
+/* Lay foam and silt planes (either may be NULL); cells with foam over 1e-4
+ * become live foam. */
+int silt_core_set_tracers(silt_core *c, const float *foam, const float *turbid);
 
 /* The wall mask solid_mask (src/fluid_core.rs:7288-7362) lays from the
  * current planes: 200 x 200 bytes for the window (fine 0) or 160 x 160 for

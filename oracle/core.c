@@ -118,7 +118,7 @@ typedef struct {
 struct ParkedSlot {
     int32_t kx, kz;
     int state; /* 0 empty, 1 live, 2 deleted */
-    real water, slurry, uw, vw, us, vs, moisture;
+    real water, slurry, uw, vw, us, vs, moisture, foam, turbid;
 };
 
 typedef struct {
@@ -157,6 +157,14 @@ typedef struct {
     real eroded_m3, scour_hole_m;
     uint8_t *mask, *fmask;
     real *give;
+    /* tracers: foam fraction and stirred silt (m) on the window, the live
+     * foam and this step's wet and shore cells, the silt accounts, the
+     * pushes of the last step */
+    real *foam, *turbid, *tr_s;
+    uint8_t *foam_live, *wet_m, *shore_m;
+    real stirred_m3, turbid_settled_m3;
+    Push last_pushes[MAX_PUSH];
+    int nlast;
 } Core;
 
 static real *zalloc(size_t n) { return (real *)calloc(n, sizeof(real)); }
@@ -191,6 +199,12 @@ Core *F(oracle_core_new)(real cx, real cz) {
     c->mask = (uint8_t *)calloc(WN, 1);
     c->fmask = (uint8_t *)calloc(FN, 1);
     c->give = zalloc(WN);
+    c->foam = zalloc(WN);
+    c->turbid = zalloc(WN);
+    c->tr_s = zalloc(2 * WN);
+    c->foam_live = (uint8_t *)calloc(WN, 1);
+    c->wet_m = (uint8_t *)calloc(WN, 1);
+    c->shore_m = (uint8_t *)calloc(WN, 1);
     real nw = F(oracle_dynamic_viscosity)(3, K(1.0));
     for (size_t k = 0; k < WN; k++) c->nu_w[k] = nw;
     return c;
@@ -203,6 +217,7 @@ void F(oracle_core_free)(Core *c) {
     for (size_t k = 0; k < sizeof planes / sizeof planes[0]; k++) free(planes[k]);
     free(c->grain); free(c->fgrain); free(c->hold); free(c->tiles); free(c->gate); free(c->parked);
     free(c->bed_drop); free(c->mask); free(c->fmask); free(c->give);
+    free(c->foam); free(c->turbid); free(c->tr_s); free(c->foam_live); free(c->wet_m); free(c->shore_m);
     free(c);
 }
 
@@ -1009,6 +1024,499 @@ static int ring_surface(const Core *c, const HSolid *s, real *bed_out, real *wat
     return 1;
 }
 
+/* ---- tracers: foam and stirred silt on the window, src/fluid_core.rs:
+ * 6210-6750, 9676-9880 --------------------------------------------------- */
+
+#define FOAM_LIFE_S K(1.8)
+#define FOAM_EQ K(0.85)
+#define FOAM_TAU_S K(0.35)
+#define SILT_SETTLE_S K(4.0)
+#define SILT_COLUMN_CAP_M K(0.028)
+
+/* smoothstep (:9688-9691) */
+static real smooth_step(real e0, real e1, real x) {
+    real t = clampr((x - e0) / (e1 - e0), K(0.0), K(1.0));
+    return t * t * (K(3.0) - K(2.0) * t);
+}
+
+/* breaking_foam (:9676-9686): Froude foam of a film. */
+static real breaking_foam(real speed, real thickness) {
+    if (!R_ISFINITE(speed) || !R_ISFINITE(thickness) || thickness < K(0.004) || speed <= K(0.05)) return K(0.0);
+    real froude = speed / R_SQRT(GRAV * thickness);
+    return smooth_step(K(0.85), K(1.45), froude);
+}
+
+/* sample_foam (:9693-9714): bilinear in index space. */
+static real sample_foam(const real *src, int res, real x, real z) {
+    if (!R_ISFINITE(x) || !R_ISFINITE(z) || x < K(0.0) || z < K(0.0)) return K(0.0);
+    size_t i0 = (size_t)R_FLOOR(x), j0 = (size_t)R_FLOOR(z);
+    if (i0 >= (size_t)res || j0 >= (size_t)res) return K(0.0);
+    size_t i1 = i0 + 1 < (size_t)res ? i0 + 1 : (size_t)res - 1;
+    size_t j1 = j0 + 1 < (size_t)res ? j0 + 1 : (size_t)res - 1;
+    real tx = clampr(x - (real)i0, K(0.0), K(1.0)), tz = clampr(z - (real)j0, K(0.0), K(1.0));
+    real a = src[j0 * res + i0], b = src[j0 * res + i1], cc = src[j1 * res + i0], d = src[j1 * res + i1];
+    real v = a + (b - a) * tx + (cc - a) * tz + (a - b - cc + d) * tx * tz;
+    return R_ISFINITE(v) ? clampr(v, K(0.0), K(1.0)) : K(0.0);
+}
+
+/* turbid_span (:9766-9794): cells holding silt, grown by one. */
+static int turbid_span(const real *t, int res, int *i0, int *i1, int *j0, int *j1) {
+    int a = res, b = 0, cmin = res, d = 0, any = 0;
+    for (int j = 0; j < res; j++) {
+        int first = -1, last = -1;
+        for (int i = 0; i < res; i++)
+            if (t[j * res + i] > K(1.0e-8)) {
+                if (first < 0) first = i;
+                last = i;
+            }
+        if (first < 0) continue;
+        any = 1;
+        if (first < a) a = first;
+        if (last > b) b = last;
+        if (j < cmin) cmin = j;
+        if (j > d) d = j;
+    }
+    if (!any) return 0;
+    *i0 = a > 0 ? a - 1 : 0;
+    *i1 = b + 1 < res - 1 ? b + 1 : res - 1;
+    *j0 = cmin > 0 ? cmin - 1 : 0;
+    *j1 = d + 1 < res - 1 ? d + 1 : res - 1;
+    return 1;
+}
+
+/* shore_wet_cell (:9752-9764) */
+static int shore_wet_cell(const real *water, int res, int i, int j) {
+    if (i == 0 || j == 0 || i + 1 >= res || j + 1 >= res) return 0;
+    static const int nb[4][2] = {{-1, 0}, {1, 0}, {0, -1}, {0, 1}};
+    for (int q = 0; q < 4; q++) {
+        real nh = water[(j + nb[q][1]) * res + i + nb[q][0]];
+        if (R_ISFINITE(nh) && nh < K(0.004)) return 1;
+    }
+    return 0;
+}
+
+/* shore_cell (:9826-9874): foam production and silt lift rate at a bank. */
+static void shore_cell(const real *water, const real *uw, const real *vw, int res, int idx, real *foam_out,
+                       real *rate_out) {
+    *foam_out = K(0.0);
+    *rate_out = K(0.0);
+    int i = idx % res, j = idx / res;
+    if (i == 0 || j == 0 || i + 1 >= res || j + 1 >= res) return;
+    real h = water[idx];
+    if (!R_ISFINITE(h) || h < K(0.004)) return;
+    real nx = K(0.0), nz = K(0.0);
+    int edge = 0;
+    static const int nb[4][2] = {{-1, 0}, {1, 0}, {0, -1}, {0, 1}};
+    for (int q = 0; q < 4; q++) {
+        real nh = water[(j + nb[q][1]) * res + i + nb[q][0]];
+        if (R_ISFINITE(nh) && nh < K(0.004)) {
+            edge = 1;
+            nx += (real)nb[q][0];
+            nz += (real)nb[q][1];
+        }
+    }
+    if (!edge) return;
+    real u = uw[idx], v = vw[idx];
+    if (!R_ISFINITE(u) || !R_ISFINITE(v)) return;
+    real film = R_HYPOT(u, v);
+    real nlen = R_FMAX(R_HYPOT(nx, nz), K(1.0e-4));
+    real into = R_FMAX((u * nx + v * nz) / nlen, K(0.0));
+    real strain = R_FABS(uw[idx + 1] - uw[idx - 1]) + R_FABS(vw[idx + res] - vw[idx - res]);
+    real shear_v = R_FMAX(film, strain);
+    if (shear_v < K(0.2) && into < K(0.2)) return;
+    real prod = R_FMAX(breaking_foam(R_FMAX(shear_v, into), h), smooth_step(K(0.25), K(1.1), into));
+    *foam_out = prod > K(0.02) ? prod : K(0.0);
+    if (h > K(0.012) && h < K(0.055) && (shear_v > K(0.45) || into > K(0.4))) {
+        real tau = K(1000.0) * K(0.025) * shear_v * shear_v;
+        if (tau > K(3.0) || into > K(0.45)) *rate_out = R_FMIN(K(0.006) * (into + shear_v * K(0.35)), K(0.01));
+    }
+}
+
+static int push_wakes(const Push *p, int np) {
+    for (int k = 0; k < np; k++)
+        if ((R_ISFINITE(p[k].speed) && p[k].speed >= K(0.25)) || (R_ISFINITE(p[k].slip) && R_FABS(p[k].slip) >= K(0.05)))
+            return 1;
+    return 0;
+}
+
+/* coarse_ij (:9490-9495) */
+static int coarse_ij(const Core *c, real x, real z, int *i, int *j) {
+    *i = coarse_axis(c->origin[0], x);
+    *j = coarse_axis(c->origin[1], z);
+    return *i >= 0 && *j >= 0;
+}
+
+/* collect_tracer_wet (:6246-6290) and tracer_visit_box (:6292-6337): the
+ * wet (h >= 4 mm) and shore cells of the window when a tire wakes it, else
+ * of the box around the live foam, the silt and the tires' 8 m reach. */
+static int collect_tracer_wet(Core *c, const Push *p, int np) {
+    memset(c->wet_m, 0, WN);
+    memset(c->shore_m, 0, WN);
+    int i0, i1, j0, j1;
+    if (push_wakes(p, np)) {
+        i0 = 0; i1 = WRES - 1; j0 = 0; j1 = WRES - 1;
+    } else {
+        int a = WRES, b = 0, cc = WRES, d = 0, any = 0;
+        for (int k = 0; k < WN; k++)
+            if (c->foam_live[k]) {
+                int i = k % WRES, j = k / WRES;
+                if (i < a) a = i;
+                if (i > b) b = i;
+                if (j < cc) cc = j;
+                if (j > d) d = j;
+                any = 1;
+            }
+        int ti0, ti1, tj0, tj1;
+        if (turbid_span(c->turbid, WRES, &ti0, &ti1, &tj0, &tj1)) {
+            if (ti0 < a) a = ti0;
+            if (ti1 > b) b = ti1;
+            if (tj0 < cc) cc = tj0;
+            if (tj1 > d) d = tj1;
+            any = 1;
+        }
+        int reach = (int)R_CEIL(K(8.0) / COARSE);
+        for (int k = 0; k < np; k++) {
+            int ci, cj;
+            if (!coarse_ij(c, p[k].x, p[k].z, &ci, &cj)) continue;
+            int lo_i = ci - reach > 0 ? ci - reach : 0, hi_i = ci + reach < WRES - 1 ? ci + reach : WRES - 1;
+            int lo_j = cj - reach > 0 ? cj - reach : 0, hi_j = cj + reach < WRES - 1 ? cj + reach : WRES - 1;
+            if (lo_i < a) a = lo_i;
+            if (hi_i > b) b = hi_i;
+            if (lo_j < cc) cc = lo_j;
+            if (hi_j > d) d = hi_j;
+            any = 1;
+        }
+        if (!any) return 0;
+        i0 = a >= 2 ? a - 2 : 0;
+        j0 = cc >= 2 ? cc - 2 : 0;
+        i1 = b + 2 < WRES - 1 ? b + 2 : WRES - 1;
+        j1 = d + 2 < WRES - 1 ? d + 2 : WRES - 1;
+    }
+    int nwet = 0;
+    for (int j = j0; j <= j1; j++)
+        for (int i = i0; i <= i1; i++) {
+            int idx = j * WRES + i;
+            real h = c->water[idx] + c->slurry[idx];
+            if (!R_ISFINITE(h) || h < K(0.004)) continue;
+            c->wet_m[idx] = 1;
+            nwet++;
+            if (shore_wet_cell(c->water, WRES, i, j)) c->shore_m[idx] = 1;
+        }
+    return nwet;
+}
+
+/* advect_foam (:6352-6398): semi-Lagrangian on the wet cells; a live cell
+ * that is no longer wet is written 0. */
+static void advect_foam(Core *c, real dt) {
+    real *smp = c->tr_s;
+    for (int idx = 0; idx < WN; idx++) {
+        if (!c->wet_m[idx]) continue;
+        real u = c->uw[idx], v = c->vw[idx];
+        real s = K(0.0);
+        if (R_ISFINITE(u) && R_ISFINITE(v)) {
+            real x = (real)(idx % WRES) - u * dt / COARSE;
+            real z = (real)(idx / WRES) - v * dt / COARSE;
+            s = sample_foam(c->foam, WRES, x, z);
+        }
+        smp[idx] = s;
+    }
+    real decay = R_EXP(-dt / FOAM_LIFE_S);
+    for (int idx = 0; idx < WN; idx++) {
+        if (c->foam_live[idx] || c->wet_m[idx]) c->foam[idx] = K(0.0);
+        if (!c->wet_m[idx]) continue;
+        real v = smp[idx] * decay;
+        if (v > K(1.0e-4) && R_ISFINITE(v)) c->foam[idx] = R_FMIN(v, FOAM_EQ);
+    }
+}
+
+/* add_foam (:6400-6419): approach FOAM_EQ under sustained production. */
+static void add_foam(Core *c, int idx, real prod, real dt) {
+    if (prod <= K(0.0) || !R_ISFINITE(prod) || !(dt > K(0.0))) return;
+    real cur = R_ISFINITE(c->foam[idx]) ? clampr(c->foam[idx], K(0.0), FOAM_EQ) : K(0.0);
+    real room = FOAM_EQ - cur;
+    if (room <= K(1.0e-5)) return;
+    real grown = cur + room * prod * R_FMIN(dt / FOAM_TAU_S, K(1.0));
+    c->foam[idx] = R_ISFINITE(grown) ? R_FMIN(grown, FOAM_EQ) : K(0.0);
+}
+
+/* flux_edge (:9796-9824) */
+static void flux_edge(const real *h, real *rem, real *add, int a, int b, real vel, real dt) {
+    if (!R_ISFINITE(vel) || R_FABS(vel) < K(1.0e-5)) return;
+    int donor = vel > K(0.0) ? a : b, recv = vel > K(0.0) ? b : a;
+    real column = R_ISFINITE(h[donor]) ? h[donor] : K(0.0);
+    real room = column - rem[donor];
+    if (room <= K(1.0e-8)) return;
+    real frac = R_FMIN(R_FABS(vel) * dt / COARSE, K(0.45));
+    real dh = R_FMIN(column * frac, room);
+    if (dh <= K(1.0e-8) || !R_ISFINITE(dh)) return;
+    rem[donor] += dh;
+    add[recv] += dh;
+}
+
+/* advect_turbid (:6421-6471) with flux_thickness (:9716-9750). */
+static void advect_turbid(Core *c, real dt) {
+    int i0, i1, j0, j1;
+    if (!turbid_span(c->turbid, WRES, &i0, &i1, &j0, &j1)) return;
+    real *rem = c->tr_s, *add = c->tr_s + WN;
+    for (int j = j0; j <= j1; j++)
+        for (int i = i0; i <= i1; i++) rem[j * WRES + i] = add[j * WRES + i] = K(0.0);
+    for (int j = j0; j <= j1; j++)
+        for (int i = i0; i < (i1 < WRES - 1 ? i1 : WRES - 1); i++) {
+            int a = j * WRES + i;
+            flux_edge(c->turbid, rem, add, a, a + 1, K(0.5) * (c->uw[a] + c->uw[a + 1]), dt);
+        }
+    for (int j = j0; j < (j1 < WRES - 1 ? j1 : WRES - 1); j++)
+        for (int i = i0; i <= i1; i++) {
+            int a = j * WRES + i;
+            flux_edge(c->turbid, rem, add, a, a + WRES, K(0.5) * (c->vw[a] + c->vw[a + WRES]), dt);
+        }
+    for (int j = j0; j <= j1; j++)
+        for (int i = i0; i <= i1; i++) {
+            int idx = j * WRES + i;
+            if (rem[idx] == K(0.0) && add[idx] == K(0.0)) {
+                if (!R_ISFINITE(c->turbid[idx])) c->turbid[idx] = K(0.0);
+                continue;
+            }
+            real next = c->turbid[idx] - rem[idx] + add[idx];
+            c->turbid[idx] = R_ISFINITE(next) ? R_FMAX(next, K(0.0)) : K(0.0);
+        }
+}
+
+/* settle_turbid (:6473-6509) */
+static void settle_turbid(Core *c, real dt) {
+    real area = COARSE * COARSE;
+    real keep = R_EXP(-dt / SILT_SETTLE_S);
+    real settled = K(0.0);
+    int i0, i1, j0, j1;
+    if (!turbid_span(c->turbid, WRES, &i0, &i1, &j0, &j1)) return;
+    for (int j = j0; j <= j1; j++)
+        for (int i = i0; i <= i1; i++) {
+            int idx = j * WRES + i;
+            real h = c->water[idx] + c->slurry[idx];
+            real column = R_ISFINITE(c->turbid[idx]) ? R_FMAX(c->turbid[idx], K(0.0)) : K(0.0);
+            if (column <= K(0.0)) {
+                c->turbid[idx] = K(0.0);
+                continue;
+            }
+            if (!R_ISFINITE(h) || h < K(1.0e-5)) {
+                settled += column * area;
+                c->turbid[idx] = K(0.0);
+                continue;
+            }
+            real stay = column * keep;
+            settled += (column - stay) * area;
+            c->turbid[idx] = stay;
+        }
+    if (R_ISFINITE(settled)) c->turbid_settled_m3 += settled;
+}
+
+/* add_silt (:6704-6715) and lift_silt (:6691-6702) */
+static void add_silt(Core *c, int idx, real dh, real area) {
+    if (dh <= K(0.0) || !R_ISFINITE(dh)) return;
+    real room = R_FMAX(SILT_COLUMN_CAP_M - R_FMAX(c->turbid[idx], K(0.0)), K(0.0));
+    real add = R_FMIN(dh, room);
+    if (add <= K(0.0)) return;
+    c->turbid[idx] += add;
+    c->stirred_m3 += add * area;
+}
+
+static void lift_silt(Core *c, int idx, real shear_v, real dt, real area, real rate_cap, real tau_c) {
+    if (!R_ISFINITE(shear_v) || shear_v <= K(0.0)) return;
+    real tau = K(1000.0) * K(0.035) * shear_v * shear_v;
+    if (tau <= tau_c) return;
+    real rate = R_FMIN(K(0.008) * (tau / tau_c - K(1.0)), rate_cap);
+    real dh = R_FMIN(rate * dt, K(0.0025));
+    add_silt(c, idx, dh, area);
+}
+
+/* spawn_tire_tracers (:6602-6669): foam on the ring where the free surface
+ * cuts the tire, silt lifted by the slipping contact patch. */
+static void spawn_tire_tracers(Core *c, const Push *p, real dt, real area) {
+    real speed = R_ISFINITE(p->speed) ? R_FMAX(p->speed, K(0.0)) : K(0.0);
+    real slip = R_ISFINITE(p->slip) ? R_FMIN(R_FABS(p->slip), K(1.5)) : K(0.0);
+    if (speed < K(0.25) && slip < K(0.05)) return;
+    real radius = R_FMAX(p->radius, K(0.05));
+    real len = R_FMAX(R_SQRT(p->fx * p->fx + p->fz * p->fz), K(1.0e-4));
+    real fx = p->fx / len, fz = p->fz / len;
+    real rx = fz, rz = -fx;
+    real reach = radius * K(1.15) + COARSE;
+    real reach2 = reach * reach;
+    real slip_v = speed * slip;
+    real cell = COARSE;
+    for (int idx = 0; idx < WN; idx++) {
+        if (!c->wet_m[idx]) continue;
+        real h = c->water[idx];
+        if (!R_ISFINITE(h) || h < K(0.004)) continue;
+        int i = idx % WRES, j = idx / WRES;
+        real x = c->origin[0] + ((real)i + K(0.5)) * cell;
+        real z = c->origin[1] + ((real)j + K(0.5)) * cell;
+        real dx = x - p->x, dz = z - p->z;
+        if (dx * dx + dz * dz > reach2) continue;
+        real along = dx * fx + dz * fz;
+        real lat = dx * rx + dz * rz;
+        real dist = R_HYPOT(dx, dz);
+        real film = R_HYPOT(c->uw[idx], c->vw[idx]);
+        film = R_ISFINITE(film) ? film : K(0.0);
+        real immerse = R_FMIN(h, radius * K(0.9));
+        real r_line = R_SQRT(R_FMAX(K(2.0) * radius * immerse - immerse * immerse, K(0.0)));
+        real band = K(1.0) - clampr(R_FABS(dist - r_line) / (cell * K(1.35)), K(0.0), K(1.0));
+        if (band > K(0.0)) {
+            real motion = R_FMAX(R_FMAX(slip_v, speed * K(0.45)), film);
+            real prod = breaking_foam(motion, h) * band;
+            add_foam(c, idx, prod, dt);
+        }
+        int on_patch = R_FABS(lat) <= p->width * K(0.55) + cell * K(0.35) && along >= -radius * K(0.55) &&
+                       along <= radius * K(0.35);
+        if (on_patch && h > K(0.02)) {
+            real shear_v = R_FMAX(slip_v, film * K(0.35));
+            lift_silt(c, idx, shear_v, dt, area, K(0.015), K(6.0));
+        }
+    }
+}
+
+/* spawn_foam_and_silt (:6511-6524): tires, shores, then Froude foam
+ * (spawn_shore_tracers :6671-6689, spawn_breaking_speed :6542-6561). */
+static void spawn_foam_and_silt(Core *c, real dt, const Push *p, int np) {
+    real area = COARSE * COARSE;
+    for (int k = 0; k < np; k++) spawn_tire_tracers(c, &p[k], dt, area);
+    for (int idx = 0; idx < WN; idx++) {
+        if (!c->shore_m[idx]) continue;
+        real prod, rate;
+        shore_cell(c->water, c->uw, c->vw, WRES, idx, &prod, &rate);
+        add_foam(c, idx, prod, dt);
+        if (rate > K(0.0)) add_silt(c, idx, R_FMIN(rate * dt, K(0.0012)), area);
+    }
+    for (int idx = 0; idx < WN; idx++) {
+        if (!c->wet_m[idx]) continue;
+        real h = c->water[idx] + c->slurry[idx];
+        if (!R_ISFINITE(h) || h < K(0.004)) continue;
+        real u = c->uw[idx], v = c->vw[idx];
+        if (!R_ISFINITE(u) || !R_ISFINITE(v) || R_HYPOT(u, v) <= K(0.05)) continue;
+        add_foam(c, idx, breaking_foam(R_HYPOT(u, v), h), dt);
+    }
+}
+
+/* spawn_breaking_crest (:6563-6600): foam where the seated sheet stands
+ * above the mean of its neighbours. */
+static void spawn_breaking_crest(Core *c, real dt) {
+    static const int nb[4][2] = {{-1, 0}, {1, 0}, {0, -1}, {0, 1}};
+    for (int idx = 0; idx < WN; idx++) {
+        if (!c->wet_m[idx]) continue;
+        int i = idx % WRES, j = idx / WRES;
+        real h = c->water[idx] + c->slurry[idx];
+        if (!R_ISFINITE(h) || h < K(0.004)) continue;
+        real sum = K(0.0), count = K(0.0);
+        for (int q = 0; q < 4; q++) {
+            int ni = i + nb[q][0], nj = j + nb[q][1];
+            if (ni < 0 || nj < 0 || ni >= WRES || nj >= WRES) continue;
+            real nh = c->water[nj * WRES + ni] + c->slurry[nj * WRES + ni];
+            if (R_ISFINITE(nh)) {
+                sum += R_FMAX(nh, K(0.0));
+                count += K(1.0);
+            }
+        }
+        if (count <= K(0.0)) continue;
+        real excess = h - sum / count;
+        if (excess <= K(0.0)) continue;
+        add_foam(c, idx, smooth_step(K(0.004), K(0.018), excess), dt);
+    }
+}
+
+/* clear_dry_tracers (:6717-6749) */
+static void clear_dry_tracers(Core *c) {
+    real area = COARSE * COARSE;
+    for (int idx = 0; idx < WN; idx++) {
+        if (!c->wet_m[idx]) continue;
+        real h = c->water[idx] + c->slurry[idx];
+        if (!R_ISFINITE(h) || h < K(0.004) || !R_ISFINITE(c->foam[idx])) c->foam[idx] = K(0.0);
+    }
+    int i0, i1, j0, j1;
+    if (!turbid_span(c->turbid, WRES, &i0, &i1, &j0, &j1)) return;
+    for (int j = j0; j <= j1; j++)
+        for (int i = i0; i <= i1; i++) {
+            int idx = j * WRES + i;
+            real h = c->water[idx] + c->slurry[idx];
+            if (!R_ISFINITE(h) || h < K(1.0e-5)) {
+                if (c->turbid[idx] > K(0.0)) {
+                    c->turbid_settled_m3 += c->turbid[idx] * area;
+                    c->turbid[idx] = K(0.0);
+                }
+            } else if (!R_ISFINITE(c->turbid[idx])) {
+                c->turbid[idx] = K(0.0);
+            }
+        }
+}
+
+/* sync_foam_live (:6339-6350) */
+static void sync_foam_live(Core *c) {
+    for (int idx = 0; idx < WN; idx++)
+        c->foam_live[idx] = c->wet_m[idx] && R_ISFINITE(c->foam[idx]) && c->foam[idx] > K(1.0e-4);
+}
+
+/* step_tracers (:6210-6239) */
+static void step_tracers(Core *c, real dt, const Push *p, int np) {
+    if (dt <= K(0.0) || !R_ISFINITE(dt)) return;
+    int wake = push_wakes(p, np);
+    int live_silt = 0, live_foam = 0;
+    for (int k = 0; k < WN; k++) {
+        if (R_ISFINITE(c->turbid[k]) && c->turbid[k] > K(1.0e-8)) live_silt = 1;
+        if (c->foam_live[k]) live_foam = 1;
+    }
+    if (!wake && !live_silt && !live_foam) return;
+    int nwet = collect_tracer_wet(c, p, np);
+    if (nwet == 0 && !live_silt) return;
+    if (live_foam) advect_foam(c, dt);
+    advect_turbid(c, dt);
+    settle_turbid(c, dt);
+    spawn_foam_and_silt(c, dt, p, np);
+    clear_dry_tracers(c);
+    sync_foam_live(c);
+}
+
+/* note_surface_foam (:6526-6538): crest foam after the seated sheet, from
+ * the last step's pushes. */
+void F(oracle_core_note_surface_foam)(Core *c, real dt) {
+    if (dt <= K(0.0) || !R_ISFINITE(dt)) return;
+    int nwet = collect_tracer_wet(c, c->last_pushes, c->nlast);
+    if (nwet == 0) return;
+    spawn_breaking_crest(c, dt);
+    clear_dry_tracers(c);
+    sync_foam_live(c);
+}
+
+/* Test entry point: step_tracers alone with the pushes given since the
+ * last step (which it consumes, as a step does). */
+void F(oracle_core_tracer_step)(Core *c, real dt) {
+    step_tracers(c, dt, c->pushes, c->npush);
+    memcpy(c->last_pushes, c->pushes, sizeof(Push) * (size_t)c->npush);
+    c->nlast = c->npush;
+    c->npush = 0;
+}
+
+/* Test entry points for the tracer laws on their own. */
+real F(oracle_breaking_foam)(real speed, real thickness) { return breaking_foam(speed, thickness); }
+real F(oracle_sample_foam)(const real *src, int res, real x, real z) { return sample_foam(src, res, x, z); }
+void F(oracle_shore_cell)(const real *water, const real *uw, const real *vw, int res, int idx, real *out2) {
+    shore_cell(water, uw, vw, res, idx, &out2[0], &out2[1]);
+}
+/* Lay foam and turbid planes (either may be NULL); the live foam becomes
+ * the cells holding more than 1e-4 of foam. */
+void F(oracle_core_set_tracers)(Core *c, const real *foam, const real *turbid) {
+    if (foam) memcpy(c->foam, foam, WN * sizeof(real));
+    if (turbid) memcpy(c->turbid, turbid, WN * sizeof(real));
+    for (int k = 0; k < WN; k++) c->foam_live[k] = R_ISFINITE(c->foam[k]) && c->foam[k] > K(1.0e-4);
+}
+
+/* foam, turbid (200 x 200 each) and the silt accounts (stirred, settled). */
+void F(oracle_core_tracers)(Core *c, real *foam, real *turbid, real *acc2) {
+    if (foam) memcpy(foam, c->foam, WN * sizeof(real));
+    if (turbid) memcpy(turbid, c->turbid, WN * sizeof(real));
+    if (acc2) {
+        acc2[0] = c->stirred_m3;
+        acc2[1] = c->turbid_settled_m3;
+    }
+}
+
 /* step_pair (src/fluid_core.rs:7942-8150), no lifts or reach.  With walls
  * the mask of this grid is laid once from the planes before either layer
  * steps (:7954-7966). */
@@ -1117,6 +1625,9 @@ int F(oracle_core_step)(Core *c, real dt, real *spray) {
     step_pair(c, dt, 0);
     vol += peel_spray(c, 0, 1, &events);
     if (c->nsolids > 0) scour_contacts(c, dt);
+    step_tracers(c, dt, pushes, np);
+    memcpy(c->last_pushes, pushes, sizeof(Push) * (size_t)np);
+    c->nlast = np;
     c->npush = 0;
     if (spray) *spray = vol;
     return events;
@@ -1401,7 +1912,10 @@ void F(oracle_core_publish_surface)(Core *c, const real *wetness, const real *pl
             real mean = count > K(0.0) ? sum / count : h;
             real excess = finite0r(R_FMAX(h - mean, K(0.0)));
             real pl = placed ? finite0r(R_FMAX(placed[idx], K(0.0))) : K(0.0);
-            foam2[2 * idx] = K(0.0);
+            /* foam_crest (render_contract.rs:563-568): the foam tracer
+             * where the film is at least 4 mm. */
+            real fm = h >= K(0.004) ? clampr(finite0r(c->foam[idx]), K(0.0), K(1.0)) : K(0.0);
+            foam2[2 * idx] = fm;
             foam2[2 * idx + 1] = finite0r(pl + excess);
         }
     free(film);
@@ -1538,10 +2052,13 @@ static void park_cell(Core *c, const real *org, int i, int j, real *const *h) {
     int idx = j * WRES + i;
     int32_t kx, kz;
     world_key_c(org[0] + ((real)i + K(0.5)) * COARSE, org[1] + ((real)j + K(0.5)) * COARSE, &kx, &kz);
-    struct ParkedSlot v = {0, 0, 1, h[0][idx], h[1][idx], h[2][idx], h[3][idx], h[4][idx], h[5][idx], h[6][idx]};
+    struct ParkedSlot v = {0, 0, 1, h[0][idx], h[1][idx], h[2][idx], h[3][idx], h[4][idx], h[5][idx], h[6][idx],
+                           h[7][idx], h[8][idx]};
     real dm = v.moisture - c->moisture_ambient;
     if (dm < 0) dm = -dm;
-    if (v.water > K(1.0e-8) || v.slurry > K(1.0e-8) || dm > K(6.0e-3)) {
+    /* park_slot (:4797-4805) */
+    if (v.water > K(1.0e-8) || v.slurry > K(1.0e-8) || dm > K(6.0e-3) || v.foam > K(1.0e-4) ||
+        v.turbid > K(1.0e-8)) {
         park_put(c, kx, kz, &v);
     } else {
         struct ParkedSlot *p = park_find(c, kx, kz);
@@ -1555,8 +2072,8 @@ static void restore_cell(Core *c, int i, int j, real *const *o) {
     world_key_c(c->origin[0] + ((real)i + K(0.5)) * COARSE, c->origin[1] + ((real)j + K(0.5)) * COARSE, &kx, &kz);
     struct ParkedSlot *p = park_find(c, kx, kz);
     if (!p) return;
-    const real v[7] = {p->water, p->slurry, p->uw, p->vw, p->us, p->vs, p->moisture};
-    for (int k = 0; k < 7; k++) o[k][idx] = v[k];
+    const real v[9] = {p->water, p->slurry, p->uw, p->vw, p->us, p->vs, p->moisture, p->foam, p->turbid};
+    for (int k = 0; k < 9; k++) o[k][idx] = v[k];
     park_remove(c, p);
 }
 
@@ -1613,9 +2130,9 @@ void F(oracle_core_recenter)(Core *c, real x, real z) {
     real nw[2] = {R_FLOOR((x - extent * K(0.5)) / COARSE) * COARSE, R_FLOOR((z - extent * K(0.5)) / COARSE) * COARSE};
     real d0 = nw[0] - c->origin[0], d1 = nw[1] - c->origin[1];
     if ((d0 < 0 ? -d0 : d0) < K(1.0e-4) && (d1 < 0 ? -d1 : d1) < K(1.0e-4)) return;
-    real *h[7] = {c->water, c->slurry, c->uw, c->vw, c->us, c->vs, c->moisture};
-    real *o[7];
-    for (int k = 0; k < 7; k++) {
+    real *h[9] = {c->water, c->slurry, c->uw, c->vw, c->us, c->vs, c->moisture, c->foam, c->turbid};
+    real *o[9];
+    for (int k = 0; k < 9; k++) {
         o[k] = zalloc(WN);
         if (k == 6)
             for (int q = 0; q < WN; q++) o[k][q] = c->moisture_ambient;
@@ -1629,8 +2146,10 @@ void F(oracle_core_recenter)(Core *c, real x, real z) {
                 int si = i + di, sj = j + dj;
                 if (si < 0 || sj < 0 || si >= WRES || sj >= WRES) continue;
                 int src = sj * WRES + si, dst = j * WRES + i;
-                if (!(h[0][src] > K(1.0e-8) || h[1][src] > K(1.0e-8) || h[6][src] > K(1.0e-4))) continue;
-                for (int k = 0; k < 7; k++) o[k][dst] = h[k][src];
+                if (!(h[0][src] > K(1.0e-8) || h[1][src] > K(1.0e-8) || h[6][src] > K(1.0e-4) ||
+                      h[7][src] > K(1.0e-5) || h[8][src] > K(1.0e-8)))
+                    continue;
+                for (int k = 0; k < 9; k++) o[k][dst] = h[k][src];
             }
         c->origin[0] = nw[0];
         c->origin[1] = nw[1];
@@ -1643,10 +2162,12 @@ void F(oracle_core_recenter)(Core *c, real x, real z) {
         for (int j = 0; j < WRES; j++)
             for (int i = 0; i < WRES; i++) restore_cell(c, i, j, o);
     }
-    for (int k = 0; k < 7; k++) {
+    for (int k = 0; k < 9; k++) {
         memcpy(h[k], o[k], WN * sizeof(real));
         free(o[k]);
     }
+    /* reset_tracer_indices (:9228-9233) */
+    memset(c->foam_live, 0, WN);
 }
 
 long F(oracle_core_parked_cells)(Core *c) { return (long)c->pcount; }

@@ -393,11 +393,45 @@ def _declare_core(L):
         g("oracle_core_push_on").restype = None
         g("oracle_core_peel").argtypes = [_P, C.c_int, C.POINTER(C.c_int)]
         g("oracle_core_peel").restype = R
+        g("oracle_breaking_foam").argtypes = [R, R]
+        g("oracle_breaking_foam").restype = R
+        g("oracle_sample_foam").argtypes = [_P, C.c_int, R, R]
+        g("oracle_sample_foam").restype = R
         for n, at in (("oracle_core_set_hydro_solids", [_P, _P, C.c_int]),
                       ("oracle_core_solid_mask", [_P, C.c_int, _P]), ("oracle_core_hydro_load", [_P, _P]),
-                      ("oracle_core_bed_drop", [_P, _P]), ("oracle_core_scour", [_P, R])):
+                      ("oracle_core_bed_drop", [_P, _P]), ("oracle_core_scour", [_P, R]),
+                      ("oracle_core_note_surface_foam", [_P, R]), ("oracle_core_tracers", [_P, _P, _P, _P]),
+                      ("oracle_core_set_tracers", [_P, _P, _P]), ("oracle_core_tracer_step", [_P, R]),
+                      ("oracle_shore_cell", [_P, _P, _P, C.c_int, C.c_int, _P])):
             g(n).argtypes = at
             g(n).restype = None
+
+
+def breaking_foam(speed, thickness, precision=32):
+    """breaking_foam (src/fluid_core.rs:9676-9686)."""
+    _declare_core_once()
+    return getattr(lib(), "oracle_breaking_foam" + _sfx(precision))(speed, thickness)
+
+
+def sample_foam(src, res, x, z, precision=32):
+    """sample_foam (src/fluid_core.rs:9693-9714) at index-space (x, z)."""
+    _declare_core_once()
+    return getattr(lib(), "oracle_sample_foam" + _sfx(precision))(_ptr(src), res, x, z)
+
+
+def shore_cell(water, uw, vw, res, idx, precision=32):
+    """shore_cell (src/fluid_core.rs:9826-9874): (foam production, silt rate)."""
+    _declare_core_once()
+    out = np.zeros(2, _dt(precision))
+    getattr(lib(), "oracle_shore_cell" + _sfx(precision))(_ptr(water), _ptr(uw), _ptr(vw), res, idx, _ptr(out))
+    return float(out[0]), float(out[1])
+
+
+def _declare_core_once():
+    L = lib()
+    if not getattr(L, "_core_declared", False):
+        _declare_core(L)
+        L._core_declared = True
 
 
 SOLID_FIELDS = ("x", "z", "yaw", "half_length", "half_width", "bottom_y", "top_y", "contact", "radius", "width")
@@ -481,6 +515,26 @@ class Core:
         out = np.zeros(10, self.dtype)
         self._f("oracle_core_hydro_load")(self._c, _ptr(out))
         return dict(zip(HYDRO_LOAD_FIELDS, (float(x) for x in out)))
+
+    def tracer_step(self, dt):
+        """step_tracers alone (src/fluid_core.rs:6210-6239) with the pushes
+        given since the last step."""
+        self._f("oracle_core_tracer_step")(self._c, dt)
+
+    def note_surface_foam(self, dt):
+        self._f("oracle_core_note_surface_foam")(self._c, dt)
+
+    def tracers(self):
+        """(foam, turbid, stirred_m3, settled_m3) of the window."""
+        foam, turbid = np.zeros(WINDOW_RES ** 2, self.dtype), np.zeros(WINDOW_RES ** 2, self.dtype)
+        acc = np.zeros(2, self.dtype)
+        self._f("oracle_core_tracers")(self._c, _ptr(foam), _ptr(turbid), _ptr(acc))
+        return foam, turbid, float(acc[0]), float(acc[1])
+
+    def set_tracers(self, foam=None, turbid=None):
+        f = None if foam is None else self._arr(foam)
+        t = None if turbid is None else self._arr(turbid)
+        self._f("oracle_core_set_tracers")(self._c, _ptr(f), _ptr(t))
 
     def scour(self, dt):
         self._f("oracle_core_scour")(self._c, dt)
