@@ -15,6 +15,7 @@
 #include "tracers.hpp"
 #include "mlp.hpp"
 #include "plan.hpp"
+#include "pond.hpp"
 #include "rut.hpp"
 #include "sheet_device.hpp"
 #include "sheet_fused.hpp"
@@ -90,6 +91,9 @@ struct silt_core {
   // last step's pushes (note_surface_foam reads them).
   TracerPlanes tr{};
   TracerState* d_tstate = nullptr;
+  // Scratch and result of pond_ruts / seat_rut_water.
+  PondScratch pond{};
+  PondResult* d_pond = nullptr;
   std::vector<CorePush> last_pushes;
   // Ground under the sheet (SoilField bed / placed / initial height on the
   // window grid), set by silt_core_set_ground and cut by the rut stamp.
@@ -1026,7 +1030,12 @@ silt_core* silt_core_create(float center_x, float center_z, int path) {
        dalloc(c, &c->d_eroded, 1) && dalloc(c, &c->d_hole, 1) && dalloc(c, &c->tr.foam, WN) &&
        dalloc(c, &c->tr.turbid, WN) && dalloc(c, &c->tr.foam_live, WN) && dalloc(c, &c->tr.wet, WN) &&
        dalloc(c, &c->tr.shore, WN) && dalloc(c, &c->tr.scratch, 2 * WN) && dalloc(c, &c->tr.acc, 2) &&
-       dalloc(c, &c->d_tstate, 1) &&
+       dalloc(c, &c->d_tstate, 1) && dalloc(c, &c->pond.run, 2 * WN) &&
+       dalloc(c, &c->pond.pre, 2 * (kWindowRes + 1) * (kWindowRes + 1)) && dalloc(c, &c->pond.f0, WN) &&
+       dalloc(c, &c->pond.f1, WN) && dalloc(c, &c->pond.f2, WN) && dalloc(c, &c->pond.f3, WN) &&
+       dalloc(c, &c->pond.f4, WN) && dalloc(c, &c->pond.old_w, WN) && dalloc(c, &c->pond.old_s, WN) &&
+       dalloc(c, &c->pond.m0, WN) && dalloc(c, &c->pond.m1, WN) && dalloc(c, &c->pond.ring, WN) &&
+       dalloc(c, &c->d_pond, 1) &&
        hipHostMalloc((void**)&c->h_stats, 2 * sizeof(SheetStats), hipHostMallocDefault) == hipSuccess;
   if (ok) {
     // Grain::Other everywhere (FluidCore::new, src/fluid_core.rs:4979-4980).
@@ -1656,6 +1665,75 @@ int silt_core_note_surface_foam(silt_core* c, float dt) {
                    dt, c->st);
   HIPCHK(hipStreamSynchronize(c->st));
   return hipGetLastError() == hipSuccess ? SILT_OK : SILT_EDEVICE;
+}
+
+// pond_ruts (src/fluid_core.rs:5509-5611); the pass books the FP64 change of
+// the water it makes as a handoff line.
+int silt_core_pond_ruts(silt_core* c, float* moved) {
+  if (!c) return SILT_EINVAL;
+  if (int rb = ensure_base(c)) return rb;
+  if (restrict_fine(c)) return SILT_EDEVICE;
+  core_pond_ruts(c->c, c->pond, c->d_pond, handoff_book(c), c->st);
+  if (hipGetLastError() != hipSuccess) return SILT_EDEVICE;
+  if (moved) {
+    PondResult r{};
+    HIPCHK(hipMemcpyAsync(&r, c->d_pond, sizeof r, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    *moved = r.moved;
+  }
+  return SILT_OK;
+}
+
+namespace {
+
+int seat_launch(silt_core* c, int gate, float* eta, int* has_floor) {
+  core_seat_rut_water(c->c, c->pond, gate, c->d_pond, handoff_book(c), c->st);
+  if (hipGetLastError() != hipSuccess) return SILT_EDEVICE;
+  if (eta || has_floor) {
+    PondResult r{};
+    HIPCHK(hipMemcpyAsync(&r, c->d_pond, sizeof r, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    if (eta) *eta = r.has_floor == 1 ? r.eta : NAN;
+    if (has_floor) *has_floor = r.has_floor;
+  }
+  return SILT_OK;
+}
+
+}  // namespace
+
+// seat_rut_water (src/fluid_core.rs:5622-5736) with the classify closure as
+// host arrays.
+int silt_core_seat_rut_water(silt_core* c, const float* pressed, const uint8_t* is_rut, float* eta, int* has_floor) {
+  if (!c || !pressed || !is_rut) return SILT_EINVAL;
+  if (int rb = ensure_base(c)) return rb;
+  if (restrict_fine(c)) return SILT_EDEVICE;
+  HIPCHK(hipMemcpyAsync(c->pond.f0, pressed, WN * 4, hipMemcpyHostToDevice, c->st));
+  HIPCHK(hipMemcpyAsync(c->pond.m1, is_rut, WN, hipMemcpyHostToDevice, c->st));
+  return seat_launch(c, 0, eta, has_floor);
+}
+
+// seat_rut_sheet (src/world.rs:5539-5561) on the core's ground: a window
+// whose coarse sheet is dry (and whose patch is off) with no cut is skipped.
+int silt_core_seat_rut_sheet(silt_core* c, int* seated) {
+  if (!c) return SILT_EINVAL;
+  if (!c->ground) return SILT_EINVAL;
+  if (int rb = ensure_base(c)) return rb;
+  const int gate = c->fine_on ? 0 : 1;
+  if (restrict_fine(c)) return SILT_EDEVICE;
+  core_seat_classify(c->soil_bed, c->soil_placed, c->soil_init, c->pond, c->st);
+  int has = 0;
+  int rc = seat_launch(c, gate, nullptr, seated ? &has : nullptr);
+  if (rc) return rc;
+  if (seated) *seated = has == 1 ? 1 : 0;
+  return SILT_OK;
+}
+
+int silt_core_get_grain(silt_core* c, uint8_t* grain) {
+  if (!c || !grain) return SILT_EINVAL;
+  if (restrict_fine(c)) return SILT_EDEVICE;
+  HIPCHK(hipMemcpyAsync(grain, c->c.grain, WN, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  return SILT_OK;
 }
 
 int silt_core_get_tracers(silt_core* c, float* foam, float* turbid, double* accounts2) {

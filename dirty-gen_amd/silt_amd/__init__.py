@@ -523,6 +523,10 @@ def _declare_core(L):
     L.silt_core_hydro_load.argtypes = [_P, C.POINTER(HydroLoad)]
     L.silt_core_solid_mask.argtypes = [_P, C.c_int, _P]
     L.silt_core_note_surface_foam.argtypes = [_P, _F]
+    L.silt_core_pond_ruts.argtypes = [_P, _P]
+    L.silt_core_get_grain.argtypes = [_P, _P]
+    L.silt_core_seat_rut_water.argtypes = [_P, _P, _P, _P, _P]
+    L.silt_core_seat_rut_sheet.argtypes = [_P, _P]
     L.silt_core_get_tracers.argtypes = [_P, _P, _P, _P]
     L.silt_core_set_tracers.argtypes = [_P, _P, _P]
     L.silt_core_recenter.argtypes = [_P, _F, _F]
@@ -773,6 +777,34 @@ class FluidCore:
 
     def note_surface_foam(self, dt):
         _check_status(lib().silt_core_note_surface_foam(self._c, dt))
+
+    def pond_ruts(self):
+        """pond_ruts (src/fluid_core.rs:5509-5611): cubic metres moved into
+        the tire cuts."""
+        moved = np.zeros(1, np.float32)
+        _check_status(lib().silt_core_pond_ruts(self._c, _ptr(moved)))
+        return float(moved[0])
+
+    def seat_rut_water(self, pressed, is_rut):
+        """seat_rut_water (src/fluid_core.rs:5622-5736) with the classify
+        closure sampled per coarse cell: the pond surface, or None."""
+        p = np.ascontiguousarray(pressed, np.float32)
+        r = np.ascontiguousarray(is_rut, np.uint8)
+        eta, has = np.zeros(1, np.float32), np.zeros(1, np.int32)
+        _check_status(lib().silt_core_seat_rut_water(self._c, _ptr(p), _ptr(r), _ptr(eta), _ptr(has)))
+        return float(eta[0]) if has[0] == 1 else None
+
+    def grain(self):
+        out = np.zeros(WINDOW_RES * WINDOW_RES, np.uint8)
+        _check_status(lib().silt_core_get_grain(self._c, _ptr(out)))
+        return out
+
+    def seat_rut_sheet(self):
+        """seat_rut_sheet (src/world.rs:5539-5561) on the core's ground:
+        True when the sheet was seated on a pond floor."""
+        seated = np.zeros(1, np.int32)
+        _check_status(lib().silt_core_seat_rut_sheet(self._c, _ptr(seated)))
+        return bool(seated[0])
 
     def tracers(self):
         """(foam, turbid, (stirred_m3, settled_m3)) of the window."""

@@ -84,13 +84,17 @@ def nested_scene_core(S, scenes, seed, path="exact"):
     return core
 
 
-def drive_step(core, drive, dt, step_fn=None):
-    """Ruts, tires, one step (step_fn(core, dt) or core.step), then the
-    vehicle moves."""
+def drive_step(core, drive, dt, step_fn=None, ponds=False):
+    """Ruts, tires, one step (step_fn(core, dt) or core.step), with ponds
+    the pond and seat passes ThroughDrive runs after the step
+    (src/world.rs:4686-4694), then the vehicle moves."""
     core.stamp_ruts(drive.contacts(), dt)
     for p in drive.pushes():
         core.push_tire(**p)
     out = step_fn(core, dt) if step_fn else core.step(dt)
+    if ponds:
+        core.pond_ruts()
+        core.seat_rut_sheet()
     drive.advance(dt)
     return out
 

@@ -158,6 +158,35 @@ int silt_core_hydro_load(silt_core *c, silt_hydro_load *out);
  * note_surface_foam (:6526-6538): the crest foam of a seated sheet, for the
  * caller that seats the rut water after the step (ThroughDrive). */
 int silt_core_note_surface_foam(silt_core *c, float dt);
+/* pond_ruts (src/fluid_core.rs:5509-5611): water in a tire cut (bed more
+ * than 12 mm below the pour plane) rises to the mean free surface of the
+ * still pond cells within 10 cells, and those cells give the same cubic
+ * metres in proportion to their depth above 12 mm.  moved (may be NULL: then
+ * the call does not wait for the GPU) = the cubic metres moved.  The FP64
+ * change of the window's water, the f32 rounding of the move, is booked as
+ * the water's handoff line (silt_core_ledger_lines). */
+int silt_core_pond_ruts(silt_core *c, float *moved);
+/* seat_rut_water (src/fluid_core.rs:5622-5736) with shove_rut_slurry
+ * (:5740-5792).  The classify closure becomes two host arrays sampled at the
+ * coarse cell centres (200 x 200): the pressed ground (bed + placed, m;
+ * non-finite reads as 0) and the cut flags.  The pond surface is the median
+ * free surface of the floor cells; cut and bank columns above it are capped,
+ * the water they drop goes back onto the floor in proportion to depth, and
+ * slurry scraped off a cut goes to the nearest ring of uncut cells.
+ * has_floor = 1 and *eta = the pond surface when the window has at least 8
+ * floor cells, else 0 and NaN (the reference's None).  eta / has_floor may be
+ * NULL (no wait).  The FP64 changes of water and slurry are booked as
+ * handoff lines. */
+int silt_core_seat_rut_water(silt_core *c, const float *pressed, const uint8_t *is_rut, float *eta, int *has_floor);
+/* seat_rut_sheet (src/world.rs:5539-5561): seat_rut_water classified from
+ * the core's ground (silt_core_set_ground, cut by silt_core_stamp_ruts):
+ * pressed = bed + placed, a cut where the ground lies more than 12 mm below
+ * its initial height.  A window whose coarse sheet is dry and which holds no
+ * cut is skipped.  seated (may be NULL: no wait) = 1 when a floor was found.
+ * SILT_EINVAL without a ground. */
+int silt_core_seat_rut_sheet(silt_core *c, int *seated);
+/* The window's grain plane out (Grain codes, src/soil.rs; 200 x 200 bytes). */
+int silt_core_get_grain(silt_core *c, uint8_t *grain);
 /* foam fraction and stirred silt (m), 200 x 200 each (either may be NULL);
  * accounts2 = FP64 silt volume stirred in and settled out (m^3), or NULL. */
 int silt_core_get_tracers(silt_core *c, float *foam, float *turbid, double *accounts2);

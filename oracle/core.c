@@ -2183,3 +2183,255 @@ void F(oracle_core_parked_volume)(Core *c, double *out2) {
     out2[0] = w * (double)COARSE * (double)COARSE;
     out2[1] = sl * (double)COARSE * (double)COARSE;
 }
+
+/* ------------------------------------------------------------------------
+ * Ponds in tire cuts: pond_ruts (src/fluid_core.rs:5509-5611),
+ * seat_rut_water (:5622-5736) and shove_rut_slurry (:5740-5792), with the
+ * summed-area helpers prefix_plane / plane_window (:1502-1525).
+ * ------------------------------------------------------------------------ */
+
+/* prefix_plane (:1502-1516): inclusive 2D prefix into a (n+1)^2 plane whose
+ * first row and column stay zero; each row is a running sum along the row,
+ * added to the prefix row above. */
+static void prefix_plane(const real *src, int n, real *dst) {
+    int stride = n + 1;
+    for (int j = 0; j < n; j++) {
+        real run = K(0.0);
+        for (int i = 0; i < n; i++) {
+            run += src[j * n + i];
+            dst[(j + 1) * stride + i + 1] = run + dst[j * stride + i + 1];
+        }
+    }
+}
+
+/* plane_window (:1518-1525): sum of the inclusive rectangle. */
+static real plane_window(const real *p, int n, int i0, int i1, int j0, int j1) {
+    int stride = n + 1;
+    real a = p[j0 * stride + i0], b = p[j0 * stride + (i1 + 1)];
+    real cc = p[(j1 + 1) * stride + i0], d = p[(j1 + 1) * stride + (i1 + 1)];
+    return d - b - cc + a;
+}
+
+static void window_span(int i, int j, int rad, int n, int *i0, int *i1, int *j0, int *j1) {
+    *j0 = j - rad < 0 ? 0 : j - rad;
+    *j1 = j + rad > n - 1 ? n - 1 : j + rad;
+    *i0 = i - rad < 0 ? 0 : i - rad;
+    *i1 = i + rad > n - 1 ? n - 1 : i + rad;
+}
+
+/* pond_ruts (:5509-5611): water in a tire cut rises to the mean free surface
+ * of the still pond cells within 10 cells; the still cells give the same
+ * cubic metres in proportion to their depth above 12 mm. */
+real F(oracle_core_pond_ruts)(Core *c) {
+    if (c->fine_on) restrict_fine(c);
+    const int n = WRES;
+    const real area = COARSE * COARSE;
+    int any_rut = 0;
+    for (int idx = 0; idx < WN; idx++)
+        if (R_ISFINITE(c->bed[idx]) && c->bed[idx] < K(-0.012)) {
+            any_rut = 1;
+            break;
+        }
+    if (!any_rut) return K(0.0);
+    uint8_t *still = (uint8_t *)calloc(WN, 1);
+    real *eta_of = zalloc(WN), *mark = zalloc(WN);
+    real *count_p = zalloc((size_t)(n + 1) * (n + 1)), *eta_p = zalloc((size_t)(n + 1) * (n + 1));
+    real *extra_of = zalloc(WN), *room_of = zalloc(WN);
+    for (int idx = 0; idx < WN; idx++) {
+        real bed = c->bed[idx], water = c->water[idx];
+        if (R_ISFINITE(bed) && R_ISFINITE(water) && bed > K(-0.008) && bed < K(0.03) && water > K(0.018)) {
+            still[idx] = 1;
+            eta_of[idx] = bed + water;
+            mark[idx] = K(1.0);
+        }
+    }
+    prefix_plane(mark, n, count_p);
+    prefix_plane(eta_of, n, eta_p);
+    real total_need = K(0.0);
+    for (int j = 0; j < n; j++)
+        for (int i = 0; i < n; i++) {
+            int idx = j * n + i;
+            real bed = c->bed[idx], water = c->water[idx];
+            if (!R_ISFINITE(bed) || !R_ISFINITE(water) || bed >= K(-0.012)) continue;
+            int i0, i1, j0, j1;
+            window_span(i, j, 10, n, &i0, &i1, &j0, &j1);
+            real count = plane_window(count_p, n, i0, i1, j0, j1);
+            real sum = plane_window(eta_p, n, i0, i1, j0, j1);
+            if (count < K(1.0)) continue;
+            real extra = R_FMAX((sum / count) - bed - water, K(0.0));
+            if (extra > K(1.0e-4)) {
+                extra_of[idx] = extra;
+                total_need += extra * area;
+            }
+        }
+    real moved = K(0.0);
+    if (total_need > K(1.0e-8)) {
+        real total_room = K(0.0);
+        for (int idx = 0; idx < WN; idx++) {
+            if (!still[idx]) continue;
+            real room = R_FMAX(c->water[idx] - K(0.012), K(0.0));
+            if (room > K(1.0e-4)) {
+                room_of[idx] = room;
+                total_room += room * area;
+            }
+        }
+        if (total_room > K(1.0e-8)) {
+            real scale = R_FMIN(total_room / total_need, K(1.0));
+            moved = total_need * scale;
+            for (int idx = 0; idx < WN; idx++)
+                if (extra_of[idx] > K(0.0)) c->water[idx] += extra_of[idx] * scale;
+            for (int idx = 0; idx < WN; idx++)
+                if (room_of[idx] > K(0.0)) {
+                    real give = moved * (room_of[idx] * area) / total_room;
+                    c->water[idx] = R_FMAX(c->water[idx] - give / area, K(0.0));
+                }
+        }
+    }
+    free(still);
+    free(eta_of);
+    free(mark);
+    free(count_p);
+    free(eta_p);
+    free(extra_of);
+    free(room_of);
+    return moved;
+}
+
+static int cmp_real(const void *a, const void *b) {
+    real x = *(const real *)a, y = *(const real *)b;
+    return x < y ? -1 : (x > y ? 1 : 0);
+}
+
+#define SHOVE_RING 4
+
+/* shove_rut_slurry (:5740-5792): slurry scraped off a cut goes, in equal
+ * shares, to the non-cut cells of the nearest Chebyshev ring (1 to 4) that
+ * has any; a cell that had no slurry takes the cut's grain.  With no such
+ * ring the slurry stays on the cut. */
+static void shove_rut_slurry(Core *c, const uint8_t *is_rut, const int *sidx, const real *smetres, int ns) {
+    const int n = WRES;
+    for (int e = 0; e < ns; e++) {
+        int idx = sidx[e];
+        real metres = smetres[e];
+        int i = idx % n, j = idx / n;
+        int took = 0;
+        for (int d = 1; d <= SHOVE_RING; d++) {
+            unsigned count = 0;
+            for (int pass = 0; pass < 2; pass++) {
+                real share = pass == 0 ? K(0.0) : metres / (real)count;
+                int ja = j - d < 0 ? 0 : j - d, jb = j + d > n - 1 ? n - 1 : j + d;
+                int ia = i - d < 0 ? 0 : i - d, ib = i + d > n - 1 ? n - 1 : i + d;
+                for (int jj = ja; jj <= jb; jj++)
+                    for (int ii = ia; ii <= ib; ii++) {
+                        int di = ii - i < 0 ? i - ii : ii - i, dj = jj - j < 0 ? j - jj : jj - j;
+                        if ((di > dj ? di : dj) != d) continue;
+                        int k = jj * n + ii;
+                        if (is_rut[k]) continue;
+                        if (pass == 0) {
+                            count++;
+                        } else {
+                            if (c->slurry[k] <= K(1.0e-6)) c->grain[k] = c->grain[idx];
+                            c->slurry[k] += share;
+                        }
+                    }
+                if (count == 0) break;
+            }
+            if (count > 0) {
+                took = 1;
+                break;
+            }
+        }
+        if (!took) c->slurry[idx] += metres;
+    }
+}
+
+/* seat_rut_water (:5622-5736).  pressed / is_rut: the classify closure
+ * sampled per coarse cell (ground = bed + placed, cut flag).  Returns 1 and
+ * the pond surface in *eta_out when at least 8 floor cells exist, else 0. */
+int F(oracle_core_seat_rut_water)(Core *c, const real *pressed_in, const uint8_t *is_rut, real *eta_out) {
+    restrict_fine(c);
+    const int n = WRES;
+    const real area = COARSE * COARSE;
+    real *pressed = zalloc(WN), *etas = zalloc(WN), *floor_src = zalloc(WN);
+    real *floor_p = zalloc((size_t)(n + 1) * (n + 1));
+    uint8_t *floor_m = (uint8_t *)calloc(WN, 1);
+    int *sidx = (int *)malloc(sizeof(int) * WN);
+    real *smetres = zalloc(WN);
+    int netas = 0, ns = 0, result = 0;
+    for (int idx = 0; idx < WN; idx++) {
+        real ground = R_ISFINITE(pressed_in[idx]) ? pressed_in[idx] : K(0.0);
+        int rut = is_rut[idx] != 0;
+        real water = c->water[idx];
+        pressed[idx] = ground;
+        int on_floor = !rut && ground >= K(-0.02) && ground < K(0.04) &&
+                       (water > K(0.008) || c->moisture[idx] > K(0.55));
+        floor_m[idx] = (uint8_t)on_floor;
+        if (on_floor) etas[netas++] = ground + water;
+    }
+    if (netas >= 8) {
+        /* select_nth_unstable_by at len / 2: the element a full sort puts there. */
+        qsort(etas, (size_t)netas, sizeof(real), cmp_real);
+        real eta = etas[netas / 2];
+        real excess = K(0.0);
+        int cut_wet = 0;
+        for (int idx = 0; idx < WN; idx++)
+            if (is_rut[idx] && c->water[idx] >= K(0.012)) cut_wet = 1;
+        if (cut_wet) {
+            for (int idx = 0; idx < WN; idx++) floor_src[idx] = floor_m[idx] ? K(1.0) : K(0.0);
+            prefix_plane(floor_src, n, floor_p);
+            for (int idx = 0; idx < WN; idx++) {
+                if (!is_rut[idx] || c->water[idx] < K(0.012)) continue;
+                int i = idx % n, j = idx / n, i0, i1, j0, j1;
+                window_span(i, j, 8, n, &i0, &i1, &j0, &j1);
+                if (plane_window(floor_p, n, i0, i1, j0, j1) <= K(0.0)) continue;
+                real limit = R_FMAX(eta - pressed[idx], K(0.0));
+                if (c->water[idx] > limit + K(0.004)) {
+                    real drop = c->water[idx] - limit;
+                    c->water[idx] = limit;
+                    excess += drop * area;
+                }
+                real slurry_room = R_FMAX(eta - pressed[idx] - c->water[idx], K(0.0));
+                if (c->slurry[idx] > slurry_room + K(0.004)) {
+                    sidx[ns] = idx;
+                    smetres[ns] = c->slurry[idx] - slurry_room;
+                    ns++;
+                    c->slurry[idx] = slurry_room;
+                }
+            }
+        }
+        for (int idx = 0; idx < WN; idx++) {
+            if (is_rut[idx] || pressed[idx] < K(0.02)) continue;
+            real limit = R_FMAX(eta - pressed[idx], K(0.0));
+            if (c->water[idx] > limit + K(0.004)) {
+                real drop = c->water[idx] - limit;
+                c->water[idx] = limit;
+                excess += drop * area;
+            }
+            real slurry_room = R_FMAX(eta - pressed[idx] - c->water[idx], K(0.0));
+            if (c->slurry[idx] > slurry_room + K(0.004)) c->slurry[idx] = slurry_room;
+        }
+        shove_rut_slurry(c, is_rut, sidx, smetres, ns);
+        if (excess > K(1.0e-8)) {
+            real room = K(0.0);
+            for (int idx = 0; idx < WN; idx++)
+                if (floor_m[idx]) room += R_FMAX(c->water[idx], K(0.0));
+            if (room > K(1.0e-6))
+                for (int idx = 0; idx < WN; idx++)
+                    if (floor_m[idx] && c->water[idx] > K(0.0))
+                        c->water[idx] += excess / area * (c->water[idx] / room);
+        }
+        *eta_out = eta;
+        result = 1;
+    }
+    free(pressed);
+    free(etas);
+    free(floor_src);
+    free(floor_p);
+    free(floor_m);
+    free(sidx);
+    free(smetres);
+    return result;
+}
+
+/* The grain plane out (200 x 200 bytes). */
+void F(oracle_core_grain)(Core *c, uint8_t *out) { memcpy(out, c->grain, WN); }

@@ -397,6 +397,12 @@ def _declare_core(L):
         g("oracle_breaking_foam").restype = R
         g("oracle_sample_foam").argtypes = [_P, C.c_int, R, R]
         g("oracle_sample_foam").restype = R
+        g("oracle_core_pond_ruts").argtypes = [_P]
+        g("oracle_core_pond_ruts").restype = R
+        g("oracle_core_seat_rut_water").argtypes = [_P, _P, _P, _P]
+        g("oracle_core_seat_rut_water").restype = C.c_int
+        g("oracle_core_grain").argtypes = [_P, _P]
+        g("oracle_core_grain").restype = None
         for n, at in (("oracle_core_set_hydro_solids", [_P, _P, C.c_int]),
                       ("oracle_core_solid_mask", [_P, C.c_int, _P]), ("oracle_core_hydro_load", [_P, _P]),
                       ("oracle_core_bed_drop", [_P, _P]), ("oracle_core_scour", [_P, R]),
@@ -535,6 +541,25 @@ class Core:
         f = None if foam is None else self._arr(foam)
         t = None if turbid is None else self._arr(turbid)
         self._f("oracle_core_set_tracers")(self._c, _ptr(f), _ptr(t))
+
+    def pond_ruts(self):
+        """pond_ruts (src/fluid_core.rs:5509-5611): cubic metres moved into
+        the tire cuts."""
+        return float(self._f("oracle_core_pond_ruts")(self._c))
+
+    def seat_rut_water(self, pressed, is_rut):
+        """seat_rut_water (src/fluid_core.rs:5622-5736) with the classify
+        closure sampled per coarse cell: the pond surface, or None."""
+        p = self._arr(pressed)
+        r = np.ascontiguousarray(is_rut, np.uint8)
+        eta = np.zeros(1, self.dtype)
+        ok = self._f("oracle_core_seat_rut_water")(self._c, _ptr(p), _ptr(r), _ptr(eta))
+        return float(eta[0]) if ok else None
+
+    def grain(self):
+        out = np.zeros(WINDOW_RES * WINDOW_RES, np.uint8)
+        self._f("oracle_core_grain")(self._c, _ptr(out))
+        return out
 
     def scour(self, dt):
         self._f("oracle_core_scour")(self._c, dt)
