@@ -46,7 +46,7 @@ struct silt_core {
   float moisture_ambient = 0.0f;  // read_substrate's most common moisture
   // Coarse cells parked off the window by recenter, by world cell key.
   struct Parked {
-    float water, slurry, uw, vw, us, vs, moisture, foam, turbid;
+    float water, slurry, uw, vw, us, vs, moisture, foam, turbid, susp;
   };
   std::unordered_map<uint64_t, Parked> parked;
   float fine_origin[2] = {0, 0};
@@ -91,6 +91,8 @@ struct silt_core {
   // last step's pushes (note_surface_foam reads them).
   TracerPlanes tr{};
   TracerState* d_tstate = nullptr;
+  // The wheel-lifted suspended solids and their scratch.
+  SuspPlanes sus{};
   // Scratch and result of pond_ruts / seat_rut_water.
   PondScratch pond{};
   PondResult* d_pond = nullptr;
@@ -1035,7 +1037,9 @@ silt_core* silt_core_create(float center_x, float center_z, int path) {
        dalloc(c, &c->pond.f1, WN) && dalloc(c, &c->pond.f2, WN) && dalloc(c, &c->pond.f3, WN) &&
        dalloc(c, &c->pond.f4, WN) && dalloc(c, &c->pond.old_w, WN) && dalloc(c, &c->pond.old_s, WN) &&
        dalloc(c, &c->pond.m0, WN) && dalloc(c, &c->pond.m1, WN) && dalloc(c, &c->pond.ring, WN) &&
-       dalloc(c, &c->d_pond, 1) &&
+       dalloc(c, &c->d_pond, 1) && dalloc(c, &c->sus.susp, WN) && dalloc(c, &c->sus.give, WN) &&
+       dalloc(c, &c->sus.conc, WN) && dalloc(c, &c->sus.accept, WN) && dalloc(c, &c->sus.taken, WN) &&
+       dalloc(c, &c->sus.dir, WN) && dalloc(c, &c->sus.acc, 2) &&
        hipHostMalloc((void**)&c->h_stats, 2 * sizeof(SheetStats), hipHostMallocDefault) == hipSuccess;
   if (ok) {
     // Grain::Other everywhere (FluidCore::new, src/fluid_core.rs:4979-4980).
@@ -1265,9 +1269,20 @@ int silt_core_step(silt_core* c, float dt, silt_spray_event* events, size_t cap,
   hipStream_t st = c->st;
   HIPCHK(hipMemsetAsync(c->d_nev, 0, sizeof(unsigned), st));
   HIPCHK(hipMemsetAsync(c->d_acc, 0, sizeof(double), st));
+  const bool live_dt = dt > 0.0f && isfinite(dt);
+  // Lift from the column the tire rolls into, before the sheet shoves it
+  // (entrain_wake, FluidCore::step :6050-6053).
+  if (live_dt && !pushes.empty()) {
+    TracerPushes tp{};
+    tp.n = (int)std::min(pushes.size(), (size_t)kMaxPush);
+    for (int k = 0; k < tp.n; k++) tp.p[k] = pushes[k];
+    susp_entrain(c->sus, c->c.water, tp, dt, c->origin[0], c->origin[1], st);
+  }
   int rc = step_by_model(c, pushes, pl, dt);
   if (rc) return rc;
   if ((rc = tracer_pass(c, pushes, dt))) return rc;
+  // step_suspended (:6086-6089, :8750-8756)
+  if (live_dt) susp_step(c->sus, c->c.water, c->c.vw, dt, st);
   c->last_pushes = pushes;
   unsigned nev = 0;
   double vol = 0.0;
@@ -1378,7 +1393,8 @@ int silt_core_publish(silt_core* c, const float* soil_bed, const float* soil_pla
         dsoil[k] = d_soil + (size_t)k * WN;
       }
   }
-  core_publish(c->c, dsoil[0], dsoil[1], dsoil[2], d_out, d_out + 4 * WN, d_out + 6 * WN, c->st);
+  core_publish(c->c, dsoil[0], dsoil[1], dsoil[2], c->sus.susp, c->tr.turbid, d_out, d_out + 4 * WN, d_out + 6 * WN,
+               c->st);
   if (ground_rut4) HIPCHK(hipMemcpyAsync(ground_rut4, d_out, 4 * WN * 4, hipMemcpyDeviceToHost, c->st));
   if (film2) HIPCHK(hipMemcpyAsync(film2, d_out + 4 * WN, 2 * WN * 4, hipMemcpyDeviceToHost, c->st));
   if (flow4) HIPCHK(hipMemcpyAsync(flow4, d_out + 6 * WN, 4 * WN * 4, hipMemcpyDeviceToHost, c->st));
@@ -1441,7 +1457,7 @@ int silt_core_publish_device(silt_core* c, const float* soil_bed, const float* s
   }
   // Outputs the caller does not want land in the handle's scratch.
   float* d_out = c->d_pub;
-  core_publish(c->c, dsoil[0], dsoil[1], dsoil[2], ground_rut4 ? ground_rut4 : d_out,
+  core_publish(c->c, dsoil[0], dsoil[1], dsoil[2], c->sus.susp, c->tr.turbid, ground_rut4 ? ground_rut4 : d_out,
                film2 ? film2 : d_out + 4 * WN, flow4 ? flow4 : d_out + 6 * WN, c->st);
   if (hipGetLastError() != hipSuccess) return SILT_EDEVICE;
   return fence_to(c, user) ? SILT_EDEVICE : SILT_OK;
@@ -1579,8 +1595,10 @@ int silt_core_coarse_cell(silt_core* c, size_t i, size_t j, silt_coarse_cell* ou
   float v[7];
   float* planes[7] = {c->c.water, c->c.slurry, c->c.uw, c->c.us, c->c.vw, c->c.vs, c->c.bed};
   for (int k = 0; k < 7; k++) HIPCHK(hipMemcpyAsync(&v[k], planes[k] + idx, 4, hipMemcpyDeviceToHost, c->st));
-  float m = 0.0f;
+  float m = 0.0f, load[2] = {0.0f, 0.0f};
   HIPCHK(hipMemcpyAsync(&m, c->c.moisture + idx, 4, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipMemcpyAsync(&load[0], c->sus.susp + idx, 4, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipMemcpyAsync(&load[1], c->tr.turbid + idx, 4, hipMemcpyDeviceToHost, c->st));
   HIPCHK(hipStreamSynchronize(c->st));
   auto fin = [](float x) { return isfinite(x) ? x : 0.0f; };
   const float wm = fin(v[0]), sm = fin(v[1]), film = wm + sm;
@@ -1593,7 +1611,13 @@ int silt_core_coarse_cell(silt_core* c, size_t i, size_t j, silt_coarse_cell* ou
   out->u = film < 1.0e-5f ? 0.0f : fin((v[2] * wm + v[3] * sm) / film);
   out->v = film < 1.0e-5f ? 0.0f : fin((v[4] * wm + v[5] * sm) / film);
   out->settled_dust_m = 0.0f;
-  out->turbidity = 0.0f;
+  // wheel-lifted plus stirred solids over the water (:7025-7032)
+  float turb = 0.0f;
+  if (!(wm < 1.0e-5f)) {
+    turb = (fmaxf(fin(load[0]), 0.0f) + fmaxf(fin(load[1]), 0.0f)) / wm;
+    turb = turb < 0.0f ? 0.0f : (turb > 1.0f ? 1.0f : turb);
+  }
+  out->turbidity = turb;
   return 1;
 }
 
@@ -1736,6 +1760,21 @@ int silt_core_get_grain(silt_core* c, uint8_t* grain) {
   return SILT_OK;
 }
 
+int silt_core_get_suspended(silt_core* c, float* susp, double* accounts2) {
+  if (!c) return SILT_EINVAL;
+  if (susp) HIPCHK(hipMemcpyAsync(susp, c->sus.susp, WN * 4, hipMemcpyDeviceToHost, c->st));
+  if (accounts2) HIPCHK(hipMemcpyAsync(accounts2, c->sus.acc, 2 * sizeof(double), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  return SILT_OK;
+}
+
+int silt_core_set_suspended(silt_core* c, const float* susp) {
+  if (!c || !susp) return SILT_EINVAL;
+  HIPCHK(hipMemcpyAsync(c->sus.susp, susp, WN * 4, hipMemcpyHostToDevice, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  return SILT_OK;
+}
+
 int silt_core_get_tracers(silt_core* c, float* foam, float* turbid, double* accounts2) {
   if (!c) return SILT_EINVAL;
   if (foam) HIPCHK(hipMemcpyAsync(foam, c->tr.foam, WN * 4, hipMemcpyDeviceToHost, c->st));
@@ -1774,7 +1813,22 @@ int silt_core_ledger_full(silt_core* c, silt_fluid_ledger_full* out) {
   out->frozen_water_m3 = l.frozen_water_m3;
   out->frozen_slurry_m3 = l.frozen_slurry_m3;
   out->spray_m3 = l.spray_m3;
-  out->total_m3 = l.total_m3;
+  // The suspended-solid lines (:6830, :6840, :6876-6879): the column in the
+  // water, what settled back, what is parked off the window, and the
+  // entrained source; the total counts where the solids are.
+  double acc[2];
+  if ((rc = silt_core_get_suspended(c, nullptr, acc))) return rc;
+  std::vector<float> sp(WN);
+  HIPCHK(hipMemcpyAsync(sp.data(), c->sus.susp, WN * 4, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  double col = 0.0, parked = 0.0;
+  for (float v : sp) col += (double)fmaxf(v, 0.0f);
+  for (const auto& kv : c->parked) parked += (double)kv.second.susp;
+  out->suspended_m3 = col * kCoarseArea;
+  out->settled_suspended_m3 = acc[1];
+  out->frozen_suspended_m3 = parked * kCoarseArea;
+  out->entrained_m3 = acc[0];
+  out->total_m3 = l.total_m3 + out->suspended_m3 + out->settled_suspended_m3 + out->frozen_suspended_m3;
   return SILT_OK;
 }
 
@@ -1947,8 +2001,8 @@ uint64_t park_key(float x, float z) {
 // park_slot (src/fluid_core.rs:4797-4805) without the dust and suspended
 // columns this library does not carry.
 bool park_worth(const silt_core::Parked& p, float ambient) {
-  return p.water > 1.0e-8f || p.slurry > 1.0e-8f || fabsf(p.moisture - ambient) > 6.0e-3f || p.foam > 1.0e-4f ||
-         p.turbid > 1.0e-8f;
+  return p.water > 1.0e-8f || p.slurry > 1.0e-8f || p.susp > 1.0e-8f || fabsf(p.moisture - ambient) > 6.0e-3f ||
+         p.foam > 1.0e-4f || p.turbid > 1.0e-8f;
 }
 
 // each_leaving (src/fluid_core.rs:9936-9986): cells that leave the window
@@ -2019,10 +2073,10 @@ int silt_core_recenter(silt_core* c, float x, float z) {
   const float nw[2] = {floorf((x - extent * 0.5f) / kCoarseCell) * kCoarseCell,
                        floorf((z - extent * 0.5f) / kCoarseCell) * kCoarseCell};
   if (fabsf(nw[0] - c->origin[0]) < 1.0e-4f && fabsf(nw[1] - c->origin[1]) < 1.0e-4f) return SILT_OK;
-  float* dev[9] = {c->c.water, c->c.slurry, c->c.uw, c->c.vw, c->c.us, c->c.vs, c->c.moisture, c->tr.foam,
-                   c->tr.turbid};
-  std::vector<float> h[9], o[9];
-  for (int k = 0; k < 9; k++) {
+  float* dev[10] = {c->c.water, c->c.slurry, c->c.uw, c->c.vw, c->c.us, c->c.vs, c->c.moisture, c->tr.foam,
+                    c->tr.turbid, c->sus.susp};
+  std::vector<float> h[10], o[10];
+  for (int k = 0; k < 10; k++) {
     h[k].resize(WN);
     HIPCHK(hipMemcpyAsync(h[k].data(), dev[k], WN * 4, hipMemcpyDeviceToHost, c->st));
   }
@@ -2030,7 +2084,7 @@ int silt_core_recenter(silt_core* c, float x, float z) {
   const float amb = c->moisture_ambient;
   auto slot_at = [&](int idx) {
     return silt_core::Parked{h[0][idx], h[1][idx], h[2][idx], h[3][idx], h[4][idx],
-                             h[5][idx], h[6][idx], h[7][idx], h[8][idx]};
+                             h[5][idx], h[6][idx], h[7][idx], h[8][idx], h[9][idx]};
   };
   auto park = [&](const float* org, int i, int j) {
     const int idx = j * kWindowRes + i;
@@ -2047,11 +2101,11 @@ int silt_core_recenter(silt_core* c, float x, float z) {
     auto it = c->parked.find(key);
     if (it == c->parked.end()) return;
     const silt_core::Parked& p = it->second;
-    const float v[9] = {p.water, p.slurry, p.uw, p.vw, p.us, p.vs, p.moisture, p.foam, p.turbid};
-    for (int k = 0; k < 9; k++) o[k][idx] = v[k];
+    const float v[10] = {p.water, p.slurry, p.uw, p.vw, p.us, p.vs, p.moisture, p.foam, p.turbid, p.susp};
+    for (int k = 0; k < 10; k++) o[k][idx] = v[k];
     c->parked.erase(it);
   };
-  for (int k = 0; k < 9; k++) o[k].assign(WN, k == 6 ? amb : 0.0f);
+  for (int k = 0; k < 10; k++) o[k].assign(WN, k == 6 ? amb : 0.0f);
   int di = 0, dj = 0;
   if (uniform_cell_shift(c->origin, nw, &di, &dj)) {
     const float old[2] = {c->origin[0], c->origin[1]};
@@ -2062,9 +2116,9 @@ int silt_core_recenter(silt_core* c, float x, float z) {
         if (si < 0 || sj < 0 || si >= kWindowRes || sj >= kWindowRes) continue;
         const int src = sj * kWindowRes + si, dst = j * kWindowRes + i;
         if (!(h[0][src] > 1.0e-8f || h[1][src] > 1.0e-8f || h[6][src] > 1.0e-4f || h[7][src] > 1.0e-5f ||
-              h[8][src] > 1.0e-8f))
+              h[8][src] > 1.0e-8f || h[9][src] > 1.0e-8f))
           continue;
-        for (int k = 0; k < 9; k++) o[k][dst] = h[k][src];
+        for (int k = 0; k < 10; k++) o[k][dst] = h[k][src];
       }
     c->origin[0] = nw[0];
     c->origin[1] = nw[1];
@@ -2078,7 +2132,7 @@ int silt_core_recenter(silt_core* c, float x, float z) {
     for (int j = 0; j < kWindowRes; j++)
       for (int i = 0; i < kWindowRes; i++) restore(c->origin, i, j);
   }
-  for (int k = 0; k < 9; k++) HIPCHK(hipMemcpyAsync(dev[k], o[k].data(), WN * 4, hipMemcpyHostToDevice, c->st));
+  for (int k = 0; k < 10; k++) HIPCHK(hipMemcpyAsync(dev[k], o[k].data(), WN * 4, hipMemcpyHostToDevice, c->st));
   // reset_tracer_indices (:9228-9233): no foam is live after a move.
   core_fill_bytes(c->tr.foam_live, 0, WN, c->st);
   HIPCHK(hipStreamSynchronize(c->st));

@@ -117,8 +117,11 @@ void core_near_clear(const NearTable& t, hipStream_t st);
 void core_near_rehash(const NearTable& from, const NearTable& to, hipStream_t st);
 void core_ground(const float* soil_bed, const float* soil_placed, const float* soil_init, float* sheet_bed, float* rut,
                  int n, hipStream_t st);
+// susp / turbid: the suspended and stirred solids (m) behind flow's
+// turbidity channel, either may be null.
 void core_publish(const CorePlanes& c, const float* soil_bed, const float* soil_placed, const float* soil_rut,
-                  float* ground_rut4, float* film2, float* flow4, hipStream_t st);
+                  const float* susp, const float* turbid, float* ground_rut4, float* film2, float* flow4,
+                  hipStream_t st);
 // One hydro solid as the wall mask tests it (solid_mask, src/fluid_core.rs:
 // 7288-7362): the rectangle's centre, axes (sin / cos of the yaw, from the
 // host), half sizes, the heights between which it is a wall, the cell range

@@ -701,7 +701,8 @@ __global__ void k_near_clear(NearTable t) {
 // turbidity) and, when soil planes are given, ground_rut
 // (bed + placed, rut, bed, placed).
 __global__ void k_publish(CorePlanes c, const float* soil_bed, const float* soil_placed, const float* soil_rut,
-                          float* ground_rut4, float* film2, float* flow4, int n) {
+                          const float* susp, const float* turbid, float* ground_rut4, float* film2, float* flow4,
+                          int n) {
   int k = (int)(blockIdx.x * blockDim.x + threadIdx.x);
   if (k >= n) return;
   auto fin = [](float x) { return isfinite(x) ? x : 0.0f; };
@@ -723,7 +724,18 @@ __global__ void k_publish(CorePlanes c, const float* soil_bed, const float* soil
     flow4[4 * k] = u;
     flow4[4 * k + 1] = v;
     flow4[4 * k + 2] = speed;
-    flow4[4 * k + 3] = 0.0f;
+    // coarse_cell's turbidity (src/fluid_core.rs:7025-7032): wheel-lifted
+    // plus stirred solids over the water, clamped; the contract's channel
+    // (src/render_contract.rs:476-481) clamps it again.
+    float turb = 0.0f;
+    if (!(water < 1.0e-5f)) {
+      const float load = fmaxf(fin(susp ? susp[k] : 0.0f), 0.0f) + fmaxf(fin(turbid ? turbid[k] : 0.0f), 0.0f);
+      float tb = load / water;
+      tb = tb < 0.0f ? 0.0f : (tb > 1.0f ? 1.0f : tb);
+      tb = fin(tb);
+      turb = tb < 0.0f ? 0.0f : (tb > 1.0f ? 1.0f : tb);
+    }
+    flow4[4 * k + 3] = turb;
   }
   if (ground_rut4) {
     float bed = soil_bed ? soil_bed[k] : c.bed[k];
@@ -900,10 +912,11 @@ void core_ground(const float* soil_bed, const float* soil_placed, const float* s
 }
 
 void core_publish(const CorePlanes& c, const float* soil_bed, const float* soil_placed, const float* soil_rut,
-                  float* ground_rut4, float* film2, float* flow4, hipStream_t st) {
+                  const float* susp, const float* turbid, float* ground_rut4, float* film2, float* flow4,
+                  hipStream_t st) {
   int n = kWindowRes * kWindowRes;
   hipLaunchKernelGGL(k_publish, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, c, soil_bed, soil_placed, soil_rut,
-                     ground_rut4, film2, flow4, n);
+                     susp, turbid, ground_rut4, film2, flow4, n);
 }
 
 // ---- walls: the mask and the scour ------------------------------------------

@@ -42,6 +42,29 @@ struct TracerPushes {
   CorePush p[kMaxPush];
 };
 
+// The wheel-lifted suspended solids (src/fluid_core.rs:8750-9003): metres of
+// solids in the water column; scratch per cell: a donor's give, its
+// concentration, what its receiver accepted, what a receiver took, and the
+// donor's direction; the FP64 accounts [0] entrained from the bed,
+// [1] settled back (m^3).
+struct SuspPlanes {
+  float* susp;
+  float* give;
+  float* conc;
+  float* accept;
+  float* taken;
+  int8_t* dir;
+  double* acc;
+};
+
+// entrain_wake (:8758-8808) of the pushes, in push order.
+void susp_entrain(const SuspPlanes& sp, const float* water, const TracerPushes& tp, float dt, float ox, float oz,
+                  hipStream_t st);
+// step_suspended / advect_suspended (:8750-9003) on a sheet that is not
+// surface driven (no waves, no river reach: the plume rides v only, no eddy
+// mixing).
+void susp_step(const SuspPlanes& sp, const float* water, const float* vw, float dt, hipStream_t st);
+
 // step_tracers (src/fluid_core.rs:6210-6239) with the step's pushes; wake:
 // a push at 0.25 m/s or slipping 5 %; push_box: the union of the pushes' 8 m
 // reach in window cells (empty if none is on the window); foam_decay =

@@ -525,6 +525,8 @@ def _declare_core(L):
     L.silt_core_note_surface_foam.argtypes = [_P, _F]
     L.silt_core_pond_ruts.argtypes = [_P, _P]
     L.silt_core_get_grain.argtypes = [_P, _P]
+    L.silt_core_get_suspended.argtypes = [_P, _P, _P]
+    L.silt_core_set_suspended.argtypes = [_P, _P]
     L.silt_core_seat_rut_water.argtypes = [_P, _P, _P, _P, _P]
     L.silt_core_seat_rut_sheet.argtypes = [_P, _P]
     L.silt_core_get_tracers.argtypes = [_P, _P, _P, _P]
@@ -793,6 +795,17 @@ class FluidCore:
         eta, has = np.zeros(1, np.float32), np.zeros(1, np.int32)
         _check_status(lib().silt_core_seat_rut_water(self._c, _ptr(p), _ptr(r), _ptr(eta), _ptr(has)))
         return float(eta[0]) if has[0] == 1 else None
+
+    def suspended(self):
+        """The wheel-lifted suspended solids (m, 200 x 200) and the FP64
+        accounts (entrained_m3, settled_m3)."""
+        s, acc = np.zeros(WINDOW_RES * WINDOW_RES, np.float32), np.zeros(2, np.float64)
+        _check_status(lib().silt_core_get_suspended(self._c, _ptr(s), _ptr(acc)))
+        return s, (float(acc[0]), float(acc[1]))
+
+    def set_suspended(self, susp):
+        s = np.ascontiguousarray(susp, np.float32)
+        _check_status(lib().silt_core_set_suspended(self._c, _ptr(s)))
 
     def grain(self):
         out = np.zeros(WINDOW_RES * WINDOW_RES, np.uint8)

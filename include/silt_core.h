@@ -45,17 +45,19 @@ typedef struct silt_fluid_ledger {
 } silt_fluid_ledger;
 
 /* FluidLedger (src/fluid_core.rs:483-509) with all 15 fields, FP64.  The
- * dust, fog, suspended-solid and river-reach lines belong to subsystems this
- * library does not build (the air grid, the suspended-solid column, river
- * reaches) and are 0. */
+ * suspended-solid lines are the wheel-lifted column (silt_core_get_suspended),
+ * what settled back, what is parked off the window and the entrained source;
+ * total_m3 adds the first three to the sheet's total, as the reference does.
+ * The dust, fog and river-reach lines belong to subsystems this library does
+ * not build (the air grid, river reaches) and are 0. */
 typedef struct silt_fluid_ledger_full {
     double water_m3, slurry_m3, dust_air_m3, dust_settled_m3, frozen_water_m3, frozen_slurry_m3, frozen_dust_m3,
         spray_m3, fog_air_m3, suspended_m3, settled_suspended_m3, frozen_suspended_m3, entrained_m3, reach_m3,
         total_m3;
 } silt_fluid_ledger_full;
 
-/* CoarseCell (src/fluid_core.rs:450-463); settled dust and turbidity are 0
- * (no air grid, no tracers here). */
+/* CoarseCell (src/fluid_core.rs:450-463); turbidity = (suspended + stirred
+ * solids) / water, clamped to [0, 1]; settled dust is 0 (no air grid). */
 typedef struct silt_coarse_cell {
     float x, z, bed_m, water_m, slurry_m, moisture, u, v, settled_dust_m, turbidity;
 } silt_coarse_cell;
@@ -185,6 +187,16 @@ int silt_core_seat_rut_water(silt_core *c, const float *pressed, const uint8_t *
  * cut is skipped.  seated (may be NULL: no wait) = 1 when a floor was found.
  * SILT_EINVAL without a ground. */
 int silt_core_seat_rut_sheet(silt_core *c, int *seated);
+/* The wheel-lifted suspended solids (src/fluid_core.rs:8750-9003): every
+ * silt_core_step with pushes first lifts solids into the water column behind
+ * each tire moving at 0.3 m/s or more (entrain_wake :8758-8808), and every
+ * step lets them fall out at 4 mm/s and carries them with the water's v
+ * (step_suspended :8750-8756, advect_suspended :8810-9003; the window is
+ * never surface driven here, so no u and no eddy mixing).  They feed the
+ * contract's turbidity with the stirred silt.  susp: 200 x 200 (m) or NULL;
+ * accounts2 = FP64 [entrained from the bed, settled back] (m^3) or NULL. */
+int silt_core_get_suspended(silt_core *c, float *susp, double *accounts2);
+int silt_core_set_suspended(silt_core *c, const float *susp);
 /* The window's grain plane out (Grain codes, src/soil.rs; 200 x 200 bytes). */
 int silt_core_get_grain(silt_core *c, uint8_t *grain);
 /* foam fraction and stirred silt (m), 200 x 200 each (either may be NULL);
@@ -207,8 +219,9 @@ int silt_core_set_planes(silt_core *c, const float *water, const float *slurry, 
 
 /* publish maps 1/3/4 of the render contract (src/render_contract.rs:424-496,
  * docs/RENDER_CONTRACT.md:35-38) on the 200 x 200 window: film rg32f
- * (water, slurry), flow rgba32f (u, v, speed, turbidity), ground_rut rgba32f
- * (bed + placed, rut, bed, placed).  soil_* (host, 200 x 200) may be NULL:
+ * (water, slurry), flow rgba32f (u, v, speed, turbidity: coarse_cell's, 0
+ * on a dry cell), ground_rut rgba32f (bed + placed, rut, bed, placed).
+ * soil_* (host, 200 x 200) may be NULL:
  * then the core's ground (silt_core_set_ground) is published, or, with none
  * set, the sheet bed with no placed soil and no rut.  Host outputs. */
 int silt_core_publish(silt_core *c, const float *soil_bed, const float *soil_placed, const float *soil_rut,
@@ -219,7 +232,7 @@ int silt_core_publish(silt_core *c, const float *soil_bed, const float *soil_pla
  * moisture, soil wetness); wet_edge = 1 where a 4-neighbour inside the
  * window differs in is_wet (moisture >= 0.45 or film >= 0.004 m,
  * :261-263); foam_crest = (foam, placed + max(film - mean neighbour film,
- * 0)).  The foam tracer is not part of this library: foam is 0.  Host
+ * 0)), foam from the foam tracer (silt_core_get_tracers).  Host
  * arrays; soil_wetness / soil_placed may be NULL (no soil texel; placed then
  * comes from silt_core_set_ground's ground, or is 0); any output may be NULL. */
 int silt_core_publish_surface(silt_core *c, const float *soil_wetness, const float *soil_placed, float *moisture1,

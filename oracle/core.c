@@ -7,17 +7,19 @@
  * FP64, like sheet.c, whose step_sheet_ex it calls for every sheet.
  *
  * Reference: src/fluid_core.rs and src/fluid_core/near.rs of x3-team/Dirty-gen.
- * Restated: FluidCore::step (:6034-6128) restricted to the open-sheet path --
- * refine (:8358-8460), near field (near.rs:150-328), tire shoves
- * (push_on :8169-8239, apply_pushes :8152-8167), step_pair (:7942-8150),
- * spray peel (:8241-8314), restrict_fine (:8462-8555), soften_border
- * (:9594-9620); the setters map_water_bed (:5329-5355), paint/pour
- * (:5820-5855, :5424-5440), lay_slurry_mask (:5857-5880),
- * read_substrate_grid (:5928-5962); ledger() (:6821-6883) for the two sheets,
- * parked volume and spray; coarse_cell (:6998-7034).
- * Not restated (no sheet-path configuration enables them): wet bodies, hydro
- * solids, waves, open water, river reach, stream cycle, tracers, suspended
- * solids, dust and fog, window moves.
+ * Restated: FluidCore::step (:6034-6128) on the sheet path -- refine
+ * (:8358-8460), near field (near.rs:150-328), tire shoves (push_on
+ * :8169-8239, apply_pushes :8152-8167), step_pair (:7942-8150) with hydro
+ * solids, spray peel (:8241-8314), restrict_fine (:8462-8555), soften_border
+ * (:9594-9620), scour_contacts (:7364-7421), the foam and stirred-silt
+ * tracers (:6210-6750), the wheel-lifted suspended solids (:8750-9003); the
+ * setters map_water_bed (:5329-5355), paint/pour (:5820-5855, :5424-5440),
+ * lay_slurry_mask (:5857-5880), read_substrate_grid (:5928-5962); ledger()
+ * (:6821-6883) for the two sheets, parked volume and spray; coarse_cell
+ * (:6998-7034); recenter with parking (:5409-5422, :9004-9226); drain_over_bed
+ * (:5123-5214); pond_ruts and seat_rut_water (:5509-5792).
+ * Not restated (no sheet-path configuration enables them): wet bodies, waves,
+ * open water, river reach, stream cycle, dust and fog.
  */
 #include <math.h>
 #include <stdint.h>
@@ -118,7 +120,7 @@ typedef struct {
 struct ParkedSlot {
     int32_t kx, kz;
     int state; /* 0 empty, 1 live, 2 deleted */
-    real water, slurry, uw, vw, us, vs, moisture, foam, turbid;
+    real water, slurry, uw, vw, us, vs, moisture, foam, turbid, susp;
 };
 
 typedef struct {
@@ -163,6 +165,10 @@ typedef struct {
     real *foam, *turbid, *tr_s;
     uint8_t *foam_live, *wet_m, *shore_m;
     real stirred_m3, turbid_settled_m3;
+    /* suspended solids lifted by the wheels (metres in the water column) and
+     * their accounts: lifted from the bed, landed back */
+    real *susp;
+    real entrained_m3, settled_susp_m3;
     Push last_pushes[MAX_PUSH];
     int nlast;
 } Core;
@@ -201,6 +207,7 @@ Core *F(oracle_core_new)(real cx, real cz) {
     c->give = zalloc(WN);
     c->foam = zalloc(WN);
     c->turbid = zalloc(WN);
+    c->susp = zalloc(WN);
     c->tr_s = zalloc(2 * WN);
     c->foam_live = (uint8_t *)calloc(WN, 1);
     c->wet_m = (uint8_t *)calloc(WN, 1);
@@ -217,6 +224,7 @@ void F(oracle_core_free)(Core *c) {
     for (size_t k = 0; k < sizeof planes / sizeof planes[0]; k++) free(planes[k]);
     free(c->grain); free(c->fgrain); free(c->hold); free(c->tiles); free(c->gate); free(c->parked);
     free(c->bed_drop); free(c->mask); free(c->fmask); free(c->give);
+    free(c->susp);
     free(c->foam); free(c->turbid); free(c->tr_s); free(c->foam_live); free(c->wet_m); free(c->shore_m);
     free(c);
 }
@@ -1517,6 +1525,202 @@ void F(oracle_core_tracers)(Core *c, real *foam, real *turbid, real *acc2) {
     }
 }
 
+/* ---- suspended solids (src/fluid_core.rs:8750-9003) -------------------------
+ * entrain_wake lifts solids into the water column behind each moving tire;
+ * advect_suspended lets them fall out at SUSP_FALL_M_S and carries them with
+ * the film.  surface_driven() (:6241-6243) is false on the sheet path (no
+ * waves, no river reach), so the plume rides the along-track velocity v only
+ * and there is no eddy mixing. */
+#define SUSP_FALL K(0.004)
+
+static void entrain_wake(Core *c, const Push *pushes, int np, real dt) {
+    const int res = WRES;
+    const real cell = COARSE, area = cell * cell;
+    for (int k = 0; k < np; k++) {
+        const Push *p = &pushes[k];
+        if (p->speed < K(0.3)) continue;
+        real fl = R_FMAX(R_SQRT(p->fx * p->fx + p->fz * p->fz), K(1.0e-4));
+        real fx = p->fx / fl, fz = p->fz / fl;
+        real rx = fz, rz = -fx;
+        real reach = p->radius * K(3.4) + cell;
+        int i0 = (int)R_FLOOR(((p->x - reach) - c->origin[0]) / cell);
+        int i1 = (int)R_FLOOR(((p->x + reach) - c->origin[0]) / cell);
+        int j0 = (int)R_FLOOR(((p->z - reach) - c->origin[1]) / cell);
+        int j1 = (int)R_FLOOR(((p->z + reach) - c->origin[1]) / cell);
+        real aft = p->radius * K(3.2);
+        real half = p->width * K(0.55) + K(0.08);
+        if (i0 < 0) i0 = 0;
+        if (j0 < 0) j0 = 0;
+        if (i1 > res - 1) i1 = res - 1;
+        if (j1 > res - 1) j1 = res - 1;
+        for (int j = j0; j <= j1; j++)
+            for (int i = i0; i <= i1; i++) {
+                real x = c->origin[0] + ((real)i + K(0.5)) * cell;
+                real z = c->origin[1] + ((real)j + K(0.5)) * cell;
+                real dx = x - p->x, dz = z - p->z;
+                real along = dx * fx + dz * fz;
+                if (along > K(-0.05) || along < -aft) continue;
+                real lat = R_FABS(dx * rx + dz * rz);
+                if (lat > half) continue;
+                int idx = j * WRES + i;
+                real water = c->water[idx];
+                if (!R_ISFINITE(water) || water < K(0.004)) continue;
+                real along_t = clampr((-along) / aft, K(0.0), K(1.0));
+                real lat_t = clampr(lat / half, K(0.0), K(1.0));
+                real core = (K(1.0) - along_t) * (K(1.0) - lat_t);
+                real room = R_FMAX(water - c->susp[idx], K(0.0));
+                real add = R_FMIN(K(4.0) * dt * core * water, room);
+                if (add <= K(1.0e-8)) continue;
+                c->susp[idx] += add;
+                c->entrained_m3 += add * area;
+            }
+    }
+}
+
+/* film_uv (:3490-3499) of the water film alone (slurry and its velocity 0). */
+static void film_uv_water(real water, real uw, real vw, real *u, real *v) {
+    real h = water + K(0.0);
+    *u = K(0.0);
+    *v = K(0.0);
+    if (!R_ISFINITE(h) || h < K(1.0e-5)) return;
+    real a = (uw * water + K(0.0) * K(0.0)) / h, b = (vw * water + K(0.0) * K(0.0)) / h;
+    *u = R_ISFINITE(a) ? a : K(0.0);
+    *v = R_ISFINITE(b) ? b : K(0.0);
+}
+
+static void advect_suspended(Core *c, real dt) {
+    const int res = WRES;
+    const real cell = COARSE, area = cell * cell;
+    int i0 = res, i1 = 0, j0 = res, j1 = 0;
+    double landed_m3 = 0.0;
+    for (int j = 0; j < res; j++)
+        for (int i = 0; i < res; i++) {
+            int idx = j * res + i;
+            real s = c->susp[idx];
+            if (!R_ISFINITE(s) || s <= K(1.0e-12)) {
+                c->susp[idx] = K(0.0);
+                continue;
+            }
+            real w = c->water[idx];
+            if (!R_ISFINITE(w) || w < K(1.0e-5)) {
+                c->settled_susp_m3 += s * area;
+                c->susp[idx] = K(0.0);
+                continue;
+            }
+            if (s > w) {
+                c->settled_susp_m3 += s * area;
+                c->susp[idx] = K(0.0);
+                continue;
+            }
+            real keep = R_EXP(-dt * SUSP_FALL / w);
+            real landed = s * (K(1.0) - keep);
+            if (landed > K(0.0)) {
+                landed_m3 += (double)landed * (double)area;
+                s -= landed;
+                c->susp[idx] = s;
+            }
+            if (s > K(1.0e-8)) {
+                if (i < i0) i0 = i;
+                if (i > i1) i1 = i;
+                if (j < j0) j0 = j;
+                if (j > j1) j1 = j;
+            }
+        }
+    c->settled_susp_m3 += (real)landed_m3;
+    if (i0 > i1) return;
+    i0 = i0 > 0 ? i0 - 1 : 0;
+    j0 = j0 > 0 ? j0 - 1 : 0;
+    i1 = i1 + 1 < res - 1 ? i1 + 1 : res - 1;
+    j1 = j1 + 1 < res - 1 ? j1 + 1 : res - 1;
+    real *next = zalloc(WN), *taken = zalloc(WN);
+    memcpy(next, c->susp, WN * sizeof(real));
+    for (int j = j0; j <= j1; j++)
+        for (int i = i0; i <= i1; i++) {
+            int idx = j * res + i;
+            real depth = c->susp[idx], water = c->water[idx];
+            if (depth <= K(1.0e-12) || !R_ISFINITE(depth) || !R_ISFINITE(water) || water < K(1.0e-5)) continue;
+            real u, v;
+            film_uv_water(water, c->uw[idx], c->vw[idx], &u, &v);
+            u = K(0.0); /* not surface driven */
+            if (!R_ISFINITE(v)) v = K(0.0);
+            real conc = depth / water;
+            real fu = R_FABS(u) * dt / cell;
+            real fv = R_FABS(v) * dt / cell;
+            real sum = fu + fv;
+            if (sum > K(0.45)) {
+                fu *= K(0.45) / sum;
+                fv *= K(0.45) / sum;
+            }
+            int up_i = u > K(0.0) ? (i + 1 < res ? idx + 1 : -1) : (i > 0 ? idx - 1 : -1);
+            int up_j = v > K(0.0) ? (j + 1 < res ? idx + res : -1) : (j > 0 ? idx - res : -1);
+            const real fr[2] = {fu, fv};
+            const int rc[2] = {up_i, up_j};
+            for (int q = 0; q < 2; q++) {
+                real give = depth * fr[q];
+                if (give <= K(1.0e-12)) continue;
+                int n = rc[q];
+                if (n < 0) {
+                    c->settled_susp_m3 += give * area;
+                    next[idx] -= give;
+                    continue;
+                }
+                real wr = c->water[n];
+                if (!R_ISFINITE(wr) || wr < K(1.0e-5)) continue;
+                real have = c->susp[n] + taken[n];
+                real room = R_FMAX(conc * wr - have, K(0.0));
+                real accept = R_FMIN(R_FMIN(give, room), R_FMAX(wr - have, K(0.0)));
+                if (accept <= K(1.0e-12)) continue;
+                next[idx] -= accept;
+                taken[n] += accept;
+            }
+        }
+    for (int j = j0; j <= j1; j++)
+        for (int i = i0; i <= i1; i++) next[j * res + i] += taken[j * res + i];
+    for (int j = j0; j <= j1; j++)
+        for (int i = i0; i <= i1; i++) {
+            int idx = j * res + i;
+            real s = next[idx];
+            if (!R_ISFINITE(s) || s <= K(0.0)) {
+                next[idx] = K(0.0);
+                continue;
+            }
+            real w = c->water[idx];
+            if (!R_ISFINITE(w) || w < K(1.0e-5)) {
+                c->settled_susp_m3 += s * area;
+                s = K(0.0);
+            } else if (s > w) {
+                c->settled_susp_m3 += (s - w) * area;
+                s = w;
+            }
+            next[idx] = s;
+        }
+    memcpy(c->susp, next, WN * sizeof(real));
+    free(next);
+    free(taken);
+}
+
+/* Test entry points: entrain_wake of the pushes given, and step_suspended. */
+void F(oracle_core_entrain)(Core *c, real dt) { entrain_wake(c, c->pushes, c->npush, dt); c->npush = 0; }
+void F(oracle_core_suspended_step)(Core *c, real dt) {
+    if (dt > K(0.0) && R_ISFINITE(dt)) advect_suspended(c, dt);
+}
+/* The suspended plane (200 x 200) in or out, and the accounts: [0] column
+ * (column_sum of the plane), [1] settled, [2] parked off the window,
+ * [3] entrained. */
+void F(oracle_core_set_susp)(Core *c, const real *s) { memcpy(c->susp, s, WN * sizeof(real)); }
+void F(oracle_core_susp)(Core *c, real *s, real *acc4) {
+    if (s) memcpy(s, c->susp, WN * sizeof(real));
+    if (acc4) {
+        acc4[0] = F(oracle_column_sum)(c->susp, WN, COARSE);
+        acc4[1] = c->settled_susp_m3;
+        real fz = K(0.0);
+        for (size_t k = 0; k < c->pcap; k++)
+            if (c->parked[k].state == 1) fz += c->parked[k].susp * COARSE * COARSE;
+        acc4[2] = fz;
+        acc4[3] = c->entrained_m3;
+    }
+}
+
 /* step_pair (src/fluid_core.rs:7942-8150), no lifts or reach.  With walls
  * the mask of this grid is laid once from the planes before either layer
  * steps (:7954-7966). */
@@ -1614,6 +1818,8 @@ int F(oracle_core_step)(Core *c, real dt, real *spray) {
     memcpy(pushes, c->pushes, sizeof(Push) * (size_t)np);
     int events = 0;
     real vol = K(0.0);
+    /* lift from the column the tire rolls into, before the sheet shoves it (:6050-6053) */
+    if (dt > K(0.0) && R_ISFINITE(dt) && np > 0) entrain_wake(c, pushes, np, dt);
     refine(c, pushes, np, dt);
     for (int k = 0; k < np; k++)
         if (cell_is_refined(c, pushes[k].x, pushes[k].z)) push_on(c, &pushes[k], 1);
@@ -1626,6 +1832,8 @@ int F(oracle_core_step)(Core *c, real dt, real *spray) {
     vol += peel_spray(c, 0, 1, &events);
     if (c->nsolids > 0) scour_contacts(c, dt);
     step_tracers(c, dt, pushes, np);
+    /* step_suspended (:8750-8756) */
+    if (dt > K(0.0) && R_ISFINITE(dt)) advect_suspended(c, dt);
     memcpy(c->last_pushes, pushes, sizeof(Push) * (size_t)np);
     c->nlast = np;
     c->npush = 0;
@@ -1841,7 +2049,9 @@ void F(oracle_core_ledger)(Core *c, real *out) {
 
 /* The render-contract film and flow maps from coarse_cell
  * (src/fluid_core.rs:6998-7034; src/render_contract.rs:468-496):
- * film2 = (water, slurry), flow4 = (u, v, speed, turbidity 0). */
+ * film2 = (water, slurry), flow4 = (u, v, speed, turbidity). */
+static real finite0r(real x) { return isfinite((double)x) ? x : K(0.0); }
+
 void F(oracle_core_publish)(Core *c, real *film2, real *flow4) {
     for (int k = 0; k < WN; k++) {
         real water = isfinite(c->water[k]) ? c->water[k] : K(0.0);
@@ -1862,7 +2072,15 @@ void F(oracle_core_publish)(Core *c, real *film2, real *flow4) {
         flow4[4 * k] = dry ? K(0.0) : u;
         flow4[4 * k + 1] = dry ? K(0.0) : v;
         flow4[4 * k + 2] = speed;
-        flow4[4 * k + 3] = K(0.0);
+        /* coarse_cell's turbidity (:7025-7032): wheel-lifted solids plus
+         * stirred silt over the water; the contract's channel (:476-481) */
+        real turb = K(0.0);
+        if (!(water < K(1.0e-5))) {
+            real load = R_FMAX(finite0r(c->susp[k]), K(0.0)) + R_FMAX(finite0r(c->turbid[k]), K(0.0));
+            turb = finite0r(clampr(load / water, K(0.0), K(1.0)));
+            turb = clampr(turb, K(0.0), K(1.0));
+        }
+        flow4[4 * k + 3] = turb;
     }
 }
 
@@ -1874,7 +2092,6 @@ void F(oracle_core_publish)(Core *c, real *film2, real *flow4) {
  * (foam, placed + excess) with excess = max(film - mean of the in-window
  * 4-neighbours' films, 0) and placed = max(ground_rut placed channel, 0)
  * (placed NULL: 0).  No foam tracer here: foam is 0. */
-static real finite0r(real x) { return isfinite((double)x) ? x : K(0.0); }
 
 void F(oracle_core_publish_surface)(Core *c, const real *wetness, const real *placed, real *moisture1,
                                     real *wet_edge1, real *foam2) {
@@ -2053,12 +2270,12 @@ static void park_cell(Core *c, const real *org, int i, int j, real *const *h) {
     int32_t kx, kz;
     world_key_c(org[0] + ((real)i + K(0.5)) * COARSE, org[1] + ((real)j + K(0.5)) * COARSE, &kx, &kz);
     struct ParkedSlot v = {0, 0, 1, h[0][idx], h[1][idx], h[2][idx], h[3][idx], h[4][idx], h[5][idx], h[6][idx],
-                           h[7][idx], h[8][idx]};
+                           h[7][idx], h[8][idx], h[9][idx]};
     real dm = v.moisture - c->moisture_ambient;
     if (dm < 0) dm = -dm;
     /* park_slot (:4797-4805) */
-    if (v.water > K(1.0e-8) || v.slurry > K(1.0e-8) || dm > K(6.0e-3) || v.foam > K(1.0e-4) ||
-        v.turbid > K(1.0e-8)) {
+    if (v.water > K(1.0e-8) || v.slurry > K(1.0e-8) || v.susp > K(1.0e-8) || dm > K(6.0e-3) ||
+        v.foam > K(1.0e-4) || v.turbid > K(1.0e-8)) {
         park_put(c, kx, kz, &v);
     } else {
         struct ParkedSlot *p = park_find(c, kx, kz);
@@ -2072,8 +2289,8 @@ static void restore_cell(Core *c, int i, int j, real *const *o) {
     world_key_c(c->origin[0] + ((real)i + K(0.5)) * COARSE, c->origin[1] + ((real)j + K(0.5)) * COARSE, &kx, &kz);
     struct ParkedSlot *p = park_find(c, kx, kz);
     if (!p) return;
-    const real v[9] = {p->water, p->slurry, p->uw, p->vw, p->us, p->vs, p->moisture, p->foam, p->turbid};
-    for (int k = 0; k < 9; k++) o[k][idx] = v[k];
+    const real v[10] = {p->water, p->slurry, p->uw, p->vw, p->us, p->vs, p->moisture, p->foam, p->turbid, p->susp};
+    for (int k = 0; k < 10; k++) o[k][idx] = v[k];
     park_remove(c, p);
 }
 
@@ -2130,9 +2347,9 @@ void F(oracle_core_recenter)(Core *c, real x, real z) {
     real nw[2] = {R_FLOOR((x - extent * K(0.5)) / COARSE) * COARSE, R_FLOOR((z - extent * K(0.5)) / COARSE) * COARSE};
     real d0 = nw[0] - c->origin[0], d1 = nw[1] - c->origin[1];
     if ((d0 < 0 ? -d0 : d0) < K(1.0e-4) && (d1 < 0 ? -d1 : d1) < K(1.0e-4)) return;
-    real *h[9] = {c->water, c->slurry, c->uw, c->vw, c->us, c->vs, c->moisture, c->foam, c->turbid};
-    real *o[9];
-    for (int k = 0; k < 9; k++) {
+    real *h[10] = {c->water, c->slurry, c->uw, c->vw, c->us, c->vs, c->moisture, c->foam, c->turbid, c->susp};
+    real *o[10];
+    for (int k = 0; k < 10; k++) {
         o[k] = zalloc(WN);
         if (k == 6)
             for (int q = 0; q < WN; q++) o[k][q] = c->moisture_ambient;
@@ -2147,9 +2364,9 @@ void F(oracle_core_recenter)(Core *c, real x, real z) {
                 if (si < 0 || sj < 0 || si >= WRES || sj >= WRES) continue;
                 int src = sj * WRES + si, dst = j * WRES + i;
                 if (!(h[0][src] > K(1.0e-8) || h[1][src] > K(1.0e-8) || h[6][src] > K(1.0e-4) ||
-                      h[7][src] > K(1.0e-5) || h[8][src] > K(1.0e-8)))
+                      h[7][src] > K(1.0e-5) || h[8][src] > K(1.0e-8) || h[9][src] > K(1.0e-8)))
                     continue;
-                for (int k = 0; k < 9; k++) o[k][dst] = h[k][src];
+                for (int k = 0; k < 10; k++) o[k][dst] = h[k][src];
             }
         c->origin[0] = nw[0];
         c->origin[1] = nw[1];
@@ -2162,7 +2379,7 @@ void F(oracle_core_recenter)(Core *c, real x, real z) {
         for (int j = 0; j < WRES; j++)
             for (int i = 0; i < WRES; i++) restore_cell(c, i, j, o);
     }
-    for (int k = 0; k < 9; k++) {
+    for (int k = 0; k < 10; k++) {
         memcpy(h[k], o[k], WN * sizeof(real));
         free(o[k]);
     }

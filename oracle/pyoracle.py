@@ -403,6 +403,10 @@ def _declare_core(L):
         g("oracle_core_seat_rut_water").restype = C.c_int
         g("oracle_core_grain").argtypes = [_P, _P]
         g("oracle_core_grain").restype = None
+        for n, at in (("oracle_core_entrain", [_P, R]), ("oracle_core_suspended_step", [_P, R]),
+                      ("oracle_core_set_susp", [_P, _P]), ("oracle_core_susp", [_P, _P, _P])):
+            g(n).argtypes = at
+            g(n).restype = None
         for n, at in (("oracle_core_set_hydro_solids", [_P, _P, C.c_int]),
                       ("oracle_core_solid_mask", [_P, C.c_int, _P]), ("oracle_core_hydro_load", [_P, _P]),
                       ("oracle_core_bed_drop", [_P, _P]), ("oracle_core_scour", [_P, R]),
@@ -555,6 +559,25 @@ class Core:
         eta = np.zeros(1, self.dtype)
         ok = self._f("oracle_core_seat_rut_water")(self._c, _ptr(p), _ptr(r), _ptr(eta))
         return float(eta[0]) if ok else None
+
+    def entrain(self, dt):
+        """entrain_wake (src/fluid_core.rs:8758-8808) of the pushes given
+        with push_tire, alone."""
+        self._f("oracle_core_entrain")(self._c, dt)
+
+    def suspended_step(self, dt):
+        """step_suspended (src/fluid_core.rs:8750-9003) alone."""
+        self._f("oracle_core_suspended_step")(self._c, dt)
+
+    def set_suspended(self, susp):
+        s = self._arr(susp)
+        self._f("oracle_core_set_susp")(self._c, _ptr(s))
+
+    def suspended(self):
+        """(plane, {column_m3, settled_m3, frozen_m3, entrained_m3})."""
+        s, acc = np.zeros(WINDOW_RES * WINDOW_RES, self.dtype), np.zeros(4, self.dtype)
+        self._f("oracle_core_susp")(self._c, _ptr(s), _ptr(acc))
+        return s, dict(zip(("column_m3", "settled_m3", "frozen_m3", "entrained_m3"), (float(x) for x in acc)))
 
     def grain(self):
         out = np.zeros(WINDOW_RES * WINDOW_RES, np.uint8)
