@@ -34,11 +34,22 @@
 // window rotates by register renaming, not by moves.  Global loads run two
 // rows ahead of their use.
 //
-// A workgroup whose strip lies inside the grid and whose window has one
-// viscosity and open gates everywhere (the common case) runs a lean copy of
-// the row body without the ghost-column stores and per-cell law branches.
-// Three workgroups per CU (168 registers per lane) keep the windows out of
-// scratch memory.
+// A workgroup whose window has one viscosity and open gates everywhere (the
+// common case) runs a lean copy of the row body without the per-cell law
+// branches; a strip inside the grid reads its neighbours' LDS slots
+// directly, and a strip that reaches a grid side reads them at clamped
+// columns (one base address per offset) instead of storing ghost columns,
+// so the edge strips run as fast as the rest.  Three workgroups per CU
+// (168 registers per lane) keep the windows out of scratch memory.
+//
+// The fast path (EXACT = false) also rewrites the arithmetic: the S filter
+// is G / 3 + (2/3) delta, so the pressure 0.45 (0.62 G + 0.38 S) + 0.55 eta
+// and the correction potential 0.62 G + 0.38 S come from one G filter
+// along x and two along z (of the x-filtered rows and of the rows), with
+// the separable products expanded; every central difference keeps its
+// 1 / (2 cell) in the constant that multiplies it; the vorticity is carried
+// as 2 cell w; the finiteness tests that the confinement's magnitude test
+// already implies are dropped; the speed cap is a median.
 //
 // The reference clamps every stencil at the grid edge.  Columns: the lane of
 // the first or last grid column also writes its value into two ghost columns.
@@ -261,6 +272,22 @@ __global__ __launch_bounds__(kLanes, 3) void k_sheet_stream(FusedArgs fa) {
   };
 
   const float inv2c = L.inv2c;
+  // Fast-path constants with the 1 / (2 cell) of each central difference
+  // folded in: the gravity kick, the correction kick, the confinement
+  // (the vorticity is carried as 2 cell w), its 1e-4 floor, and the x / z
+  // transport's half-sum speed.
+  const float k_grav = -kGravity * a.mob0 * a.dt * inv2c;
+  const float k_grav_nu = -kGravity * a.dt * inv2c;
+  const float k_corr = L.kdt * inv2c;
+  const float k_conf = -L.cdt * inv2c * inv2c * inv2c;
+  const float w_floor = 1.0e-4f / inv2c;
+  const float half_dtc = 0.5f * L.dtc;
+  // Pressure p = 0.45 (0.62 G + 0.38 S) + 0.55 eta, and the correction
+  // potential q = 0.62 G + 0.38 S, with S = G/3 + 2/3 in both directions:
+  // a1 GzGx + a2 (Gz + Gx) + a3 for p, b1 .. b3 for q.
+  constexpr float kP1 = (float)(0.45 * (0.62 + 0.38 / 9.0)), kP2 = (float)(0.45 * 0.76 / 9.0),
+                  kP3 = (float)(0.45 * 1.52 / 9.0 + 0.55);
+  constexpr float kQ1 = (float)(0.62 + 0.38 / 9.0), kQ2 = (float)(0.76 / 9.0), kQ3 = (float)(1.52 / 9.0);
   const float two = 2.0f * cell;
   constexpr float kG[5] = {0.06f, 0.24f, 0.40f, 0.24f, 0.06f};
   constexpr float kS[5] = {0.02f, 0.08f, 0.80f, 0.08f, 0.02f};
@@ -298,13 +325,23 @@ __global__ __launch_bounds__(kLanes, 3) void k_sheet_stream(FusedArgs fa) {
 
   // Register windows, oldest row first (rows relative to the iteration's r
   // after its stage has run):
-  float E[4] = {0, 0, 0, 0};                               // eta      r-3 .. r
+  // FF: the fast path.  Its pressure and correction filters use
+  // S = G / 3 + (2/3) delta (the S taps are a third of the G taps plus two
+  // thirds of the centre), so the separable 0.62 G + 0.38 S blend needs the
+  // G filter along x and two G filters along z (of the x-filtered rows and
+  // of the rows themselves, taken as 0 outside the work boxes like the
+  // filter planes), and no S planes.
+  constexpr bool FF = !EXACT;
+  // eta r-3 .. r (r-5 .. r on the FF path); E[EO] is row r-3
+  constexpr int EO = FF ? 2 : 0;
+  float E[EO + 4] = {};
   float GX[5] = {0, 0, 0, 0, 0}, SX[5] = {0, 0, 0, 0, 0};  // x filters of eta  r-5 .. r-1
   float PW[3] = {0, 0, 0};                                 // p        r-5 .. r-3
   float UG[3] = {0, 0, 0}, VG[3] = {0, 0, 0};              // snapshot r-6 .. r-4
   float WW[3] = {0, 0, 0};                                 // w        r-7 .. r-5
   float UC[6] = {0, 0, 0, 0, 0, 0}, VC[6] = {0, 0, 0, 0, 0, 0};  // confined r-11 .. r-6
   float GB[5] = {0, 0, 0, 0, 0}, SB[5] = {0, 0, 0, 0, 0};  // x filters of b    r-12 .. r-8
+  float BW[6] = {0, 0, 0, 0, 0, 0};                        // b (FF)   r-12 .. r-7
   float QW[3] = {0, 0, 0};                                 // q        r-12 .. r-10
   float UF[5] = {0, 0, 0, 0, 0};                           // final u  r-15 .. r-11
   float VF[7] = {0, 0, 0, 0, 0, 0, 0};                     // final v  r-17 .. r-11
@@ -369,7 +406,6 @@ __global__ __launch_bounds__(kLanes, 3) void k_sheet_stream(FusedArgs fa) {
   auto prefetch = [&](auto edge, auto slot, auto lean, int r) __attribute__((always_inline)) {
     constexpr bool EDGE = decltype(edge)::value;
     constexpr int Q = decltype(slot)::value;
-    constexpr bool LEAN = decltype(lean)::value;
     auto ld = [&](const float* p, int y) -> float {
       if constexpr (EDGE) y = y < 0 ? 0 : (y >= res ? res - 1 : y);
       return row_ld(p, y);
@@ -385,12 +421,34 @@ __global__ __launch_bounds__(kLanes, 3) void k_sheet_stream(FusedArgs fa) {
   // One row of the pipeline.  EDGE = false is the body for rows whose
   // stencils and loads all lie inside the grid: no clamps, no edge selects,
   // and with FULL no row masks.
+  // Strips that reach a grid side with one viscosity and open gates (lean
+  // mode 2) read their neighbours at clamped columns instead of ghost slots:
+  // the lane of column c reads column clamp(c + d), d = -2 .. 2, through its
+  // own base address for each d.  Lanes off the grid read the edge column;
+  // no stored value depends on them.
+  auto lane_of = [&](int c) -> int { return (c < 0 ? 0 : (c >= res ? res - 1 : c)) - (x0 - kHaloL); };
+  float* const rb_m2 = lds + 2 + lane_of(col - 2);
+  float* const rb_m1 = lds + 2 + lane_of(col - 1);
+  float* const rb_p1 = lds + 2 + lane_of(col + 1);
+  float* const rb_p2 = lds + 2 + lane_of(col + 2);
+
   auto row = [&](auto par, auto edge, auto lean, int r) __attribute__((always_inline)) {
     constexpr int P = decltype(par)::value;
     constexpr bool EDGE = decltype(edge)::value;
     // LEAN: a strip inside the grid (no ghost columns) with one viscosity and
     // open gates everywhere, decided once per workgroup.
-    constexpr bool LEAN = decltype(lean)::value;
+    constexpr int MODE = decltype(lean)::value;  // 0 general, 1 lean, 2 lean at a grid side
+    constexpr bool LEAN = MODE != 0;
+    constexpr bool CLAMP = MODE == 2;
+    auto at = [&](int rw, int d) -> float {
+      if constexpr (CLAMP) {
+        if (d == -2) return rb_m2[rw * kRow];
+        if (d == -1) return rb_m1[rw * kRow];
+        if (d == 1) return rb_p1[rw * kRow];
+        if (d == 2) return rb_p2[rw * kRow];
+      }
+      return my[rw * kRow + d];
+    };
     const bool nu_u = LEAN || nu_uni;
     const bool gate_u = LEAN || gate_uni;
     auto putv = [&](int rw, float val) {
@@ -471,23 +529,48 @@ __global__ __launch_bounds__(kLanes, 3) void k_sheet_stream(FusedArgs fa) {
         bool inside = (colmask & rmk(1)) != 0;
         // accum5 (interior) and accum5_clamped (edges) differ only in a
         // leading 0.0 +, which leaves the value unchanged.
-        g = inside ? acc5(e0, e1, e2, e3, e4, kG) : 0.0f;
-        s = inside ? acc5(e0, e1, e2, e3, e4, kS) : 0.0f;
+        if constexpr (EXACT) {
+          g = inside ? acc5(e0, e1, e2, e3, e4, kG) : 0.0f;
+          s = inside ? acc5(e0, e1, e2, e3, e4, kS) : 0.0f;
+        } else {
+          const float gv = __builtin_fmaf(0.06f, e0 + e4, __builtin_fmaf(0.24f, e1 + e3, 0.40f * e2));
+          g = inside ? gv : 0.0f;
+          if constexpr (!FF) s = inside ? __builtin_fmaf(1.0f / 3.0f, gv, (2.0f / 3.0f) * e2) : 0.0f;
+        }
       }
       push(GX, g);
-      push(SX, s);
+      if constexpr (!FF) push(SX, s);
       if (PROJECT) {
-        // Rows r and r-2 of this column: E[3], E[1] (S1 loaded clamped rows).
-        float sum = l_e3 + l_e1 + E[3] + E[1] - 4.0f * E[2];
-        my[(R_LW + ((r - 1) & 7)) * kRow] = A::div(sum, cell * cell, L.invc2);
+        // Rows r and r-2 of this column: E[EO + 3], E[EO + 1] (S1 loaded
+        // clamped rows).  The fast path keeps the sum unscaled (1 / cell^2
+        // goes into S7).
+        float sum = l_e3 + l_e1 + E[EO + 3] + E[EO + 1] - 4.0f * E[EO + 2];
+        my[(R_LW + ((r - 1) & 7)) * kRow] = EXACT ? A::div(sum, cell * cell, L.invc2) : sum;
       }
     }
     // ---- S3: pressure at row r-3 ------------------------------------------
     if (!WIND) {
       const int y = r - 3;
-      float gz = zac(GX, y, kG), sz = zac(SX, y, kS);
-      float filtered = A::fma_(0.38f, sz, 0.62f * gz);
-      float pv = (colmask & rmk(3)) != 0 ? A::fma_(0.55f, E[0], 0.45f * filtered) : 0.0f;
+      float pv;
+      if constexpr (FF) {
+        // E holds rows r-5 .. r; outside the boxes the rows count as 0
+        float gze;
+        if constexpr (FULL) {
+          gze = zac(E, y, kG);
+        } else {
+          float em[5];
+#pragma unroll
+          for (int k = 0; k < 5; k++) em[k] = (colmask & rmk(5 - k)) != 0 ? E[k] : 0.0f;
+          gze = zac(em, y, kG);
+        }
+        const float gzg = zac(GX, y, kG);
+        pv = __builtin_fmaf(kP1, gzg, __builtin_fmaf(kP2, gze + GX[2], kP3 * E[2]));
+        pv = (colmask & rmk(3)) != 0 ? pv : 0.0f;
+      } else {
+        float gz = zac(GX, y, kG), sz = zac(SX, y, kS);
+        float filtered = A::fma_(0.38f, sz, 0.62f * gz);
+        pv = (colmask & rmk(3)) != 0 ? A::fma_(0.55f, E[EO], 0.45f * filtered) : 0.0f;
+      }
       putv(R_P + kSlot<P, 3>, pv);
       push(PW, pv);
     }
@@ -505,16 +588,16 @@ __global__ __launch_bounds__(kLanes, 3) void k_sheet_stream(FusedArgs fa) {
         un = A::fma_(a.wx - uu, a.blend, uu);
         vn = A::fma_(a.wz - vv, a.blend, vv);
       } else {
-        float px = A::div(l_pp - l_pm, two, inv2c);
-        float pz = A::div(abv(y, PW[1], PW[2]) - blw(y, PW[0], PW[1]), two, inv2c);
-        float mob = nu_u ? a.mob0 : sheet_mobility(ld(L.nu, y));
         if (EXACT) {
+          float px = A::div(l_pp - l_pm, two, inv2c);
+          float pz = A::div(abv(y, PW[1], PW[2]) - blw(y, PW[0], PW[1]), two, inv2c);
+          float mob = nu_u ? a.mob0 : sheet_mobility(ld(L.nu, y));
           un = uu + -kGravity * px * mob * a.dt;
           vn = vv + -kGravity * pz * mob * a.dt;
         } else {
-          float k = -kGravity * mob * a.dt;
-          un = __builtin_fmaf(k, px, uu);
-          vn = __builtin_fmaf(k, pz, vv);
+          const float k = nu_u ? k_grav : k_grav_nu * sheet_mobility(ld(L.nu, y));
+          un = __builtin_fmaf(k, l_pp - l_pm, uu);
+          vn = __builtin_fmaf(k, abv(y, PW[1], PW[2]) - blw(y, PW[0], PW[1]), vv);
         }
       }
       uu = act ? un : uu;
@@ -526,9 +609,15 @@ __global__ __launch_bounds__(kLanes, 3) void k_sheet_stream(FusedArgs fa) {
     // ---- S5: vorticity of the snapshot at row r-5 -------------------------
     if (vort) {
       const int y = r - 5;
-      float dvdx = A::div(l_vp - l_vm, two, inv2c);
-      float dudz = A::div(abv(y, UG[1], UG[2]) - blw(y, UG[0], UG[1]), two, inv2c);
-      float w = dvdx - dudz;
+      float w;
+      if constexpr (EXACT) {
+        float dvdx = A::div(l_vp - l_vm, two, inv2c);
+        float dudz = A::div(abv(y, UG[1], UG[2]) - blw(y, UG[0], UG[1]), two, inv2c);
+        w = dvdx - dudz;
+      } else {
+        // 2 cell w
+        w = (l_vp - l_vm) - (abv(y, UG[1], UG[2]) - blw(y, UG[0], UG[1]));
+      }
       putv(R_W + kSlot<P, 5>, w);
       push(WW, w);
     }
@@ -542,18 +631,23 @@ __global__ __launch_bounds__(kLanes, 3) void k_sheet_stream(FusedArgs fa) {
         const float w = WW[1];
         float wxp = fabsf(l_wp), wxm = fabsf(l_wm);
         float wzp = fabsf(abv(y, WW[1], WW[2])), wzm = fabsf(blw(y, WW[0], WW[1]));
-        float nx = A::div(wxp - wxm, two, inv2c);
-        float nz = A::div(wzp - wzm, two, inv2c);
         float du, dv;
+        bool ok;
         if (EXACT) {
+          float nx = A::div(wxp - wxm, two, inv2c);
+          float nz = A::div(wzp - wzm, two, inv2c);
           du = a.vort * a.dt * (-nz * w);
           dv = a.vort * a.dt * (nx * w);
+          ok = (colmask & rmk(6)) != 0 && hv > eps && isfinite(w) && !(fabsf(w) < 1.0e-4f) && isfinite(du) &&
+               isfinite(dv) && A::fma_(du, du, dv * dv) <= 0.0625f;
         } else {
-          du = L.cdt * (-nz * w);
-          dv = L.cdt * (nx * w);
+          // w is 2 cell w and the gradients 4 cell^2 |w|'s: k_conf holds the
+          // 1 / (8 cell^3).  A non-finite w, du or dv fails the magnitude test.
+          du = k_conf * ((wzp - wzm) * w);
+          dv = -k_conf * ((wxp - wxm) * w);
+          ok = (colmask & rmk(6)) != 0 && hv > eps && !(fabsf(w) < w_floor) &&
+               __builtin_fmaf(du, du, dv * dv) <= 0.0625f;
         }
-        const bool ok = (colmask & rmk(6)) != 0 && hv > eps && isfinite(w) && !(fabsf(w) < 1.0e-4f) &&
-                        isfinite(du) && isfinite(dv) && A::fma_(du, du, dv * dv) <= 0.0625f;
         uu = ok ? uu + du : uu;
         vv = ok ? vv + dv : vv;
       }
@@ -565,29 +659,42 @@ __global__ __launch_bounds__(kLanes, 3) void k_sheet_stream(FusedArgs fa) {
     // ---- S7: divergence source at row r-7 ---------------------------------
     if (PROJECT) {
       const int y = r - 7;
-      float ux = A::div(l_up - l_um, two, inv2c);
       // VC holds rows r-11..r-6: y-1, y, y+1 at 3, 4, 5.
-      float vz = A::div(abv(y, VC[4], VC[5]) - blw(y, VC[3], VC[4]), two, inv2c);
-      float div = ux + vz;
       float lp = l_lp;
       float ov = nu_u ? a.ov0 : overload_of(ld(L.nu, y));
       float bv;
-      if (EXACT)
+      if (EXACT) {
+        float ux = A::div(l_up - l_um, two, inv2c);
+        float vz = A::div(abv(y, VC[4], VC[5]) - blw(y, VC[3], VC[4]), two, inv2c);
+        float div = ux + vz;
         bv = div * ov - 0.015f * ov * lp;
-      else
-        bv = ov * __builtin_fmaf(-0.015f, lp, div);
+      } else {
+        // 2 cell div and the unscaled lap sum
+        const float div2 = (l_up - l_um) + (abv(y, VC[4], VC[5]) - blw(y, VC[3], VC[4]));
+        bv = __builtin_fmaf(-0.015f * ov * L.invc2, lp, (ov * inv2c) * div2);
+      }
       bv = (colmask & rmk(7)) != 0 ? bv : 0.0f;
       putv(R_B + kSlot<P, 7>, bv);
+      if constexpr (FF) push(BW, bv);
     }
     // ---- S8: x filters of b at row r-8; S9: q at row r-10 ------------------
     if (PROJECT) {
       bool inside = (colmask & rmk(8)) != 0;
       float b0 = l_b0, b1 = l_b1, b2 = l_b2, b3 = l_b3, b4 = l_b4;
-      push(GB, inside ? acc5(b0, b1, b2, b3, b4, kG) : 0.0f);
-      push(SB, inside ? acc5(b0, b1, b2, b3, b4, kS) : 0.0f);
       const int yq = r - 10;
-      float gz = zac(GB, yq, kG), sz = zac(SB, yq, kS);
-      float qv = (colmask & rmk(10)) != 0 ? A::fma_(0.38f, sz, 0.62f * gz) : 0.0f;
+      float qv;
+      if constexpr (FF) {
+        const float gv = __builtin_fmaf(0.06f, b0 + b4, __builtin_fmaf(0.24f, b1 + b3, 0.40f * b2));
+        push(GB, inside ? gv : 0.0f);
+        const float gzg = zac(GB, yq, kG), gzb = zac(BW, yq, kG);
+        qv = __builtin_fmaf(kQ1, gzg, __builtin_fmaf(kQ2, gzb + GB[2], kQ3 * BW[2]));
+        qv = (colmask & rmk(10)) != 0 ? qv : 0.0f;
+      } else {
+        push(GB, inside ? acc5(b0, b1, b2, b3, b4, kG) : 0.0f);
+        push(SB, inside ? acc5(b0, b1, b2, b3, b4, kS) : 0.0f);
+        float gz = zac(GB, yq, kG), sz = zac(SB, yq, kS);
+        qv = (colmask & rmk(10)) != 0 ? A::fma_(0.38f, sz, 0.62f * gz) : 0.0f;
+      }
       putv(R_Q + kSlot<P, 10>, qv);
       push(QW, qv);
     }
@@ -600,19 +707,26 @@ __global__ __launch_bounds__(kLanes, 3) void k_sheet_stream(FusedArgs fa) {
       const bool act = (colmask & rmk(11)) != 0 && hv > eps;
       float un = uu, vn = vv;
       if (PROJECT) {
-        float qx = A::div(l_qp - l_qm, two, inv2c);
-        float qz = A::div(abv(y, QW[1], QW[2]) - blw(y, QW[0], QW[1]), two, inv2c);
         if (EXACT) {
+          float qx = A::div(l_qp - l_qm, two, inv2c);
+          float qz = A::div(abv(y, QW[1], QW[2]) - blw(y, QW[0], QW[1]), two, inv2c);
           un = uu + 0.012f * a.dt * qx;
           vn = vv + 0.012f * a.dt * qz;
         } else {
-          un = __builtin_fmaf(L.kdt, qx, uu);
-          vn = __builtin_fmaf(L.kdt, qz, vv);
+          un = __builtin_fmaf(k_corr, l_qp - l_qm, uu);
+          vn = __builtin_fmaf(k_corr, abv(y, QW[1], QW[2]) - blw(y, QW[0], QW[1]), vv);
         }
       }
       float decay = nu_u ? a.decay0 : expf(-sheet_damp(ld(L.nu, y)) * a.dt);
-      un = finite_speed(un * decay, a.lim);
-      vn = finite_speed(vn * decay, a.lim);
+      if constexpr (EXACT) {
+        un = finite_speed(un * decay, a.lim);
+        vn = finite_speed(vn * decay, a.lim);
+      } else {
+        // finite_speed as one median and one finiteness select
+        const float ud = un * decay, vd = vn * decay;
+        un = isfinite(ud) ? __builtin_amdgcn_fmed3f(ud, -a.lim, a.lim) : 0.0f;
+        vn = isfinite(vd) ? __builtin_amdgcn_fmed3f(vd, -a.lim, a.lim) : 0.0f;
+      }
       uu = act ? un : uu;
       vv = act ? vn : vv;
       putv(R_UF + kSlot<P, 11>, uu);
@@ -628,19 +742,22 @@ __global__ __launch_bounds__(kLanes, 3) void k_sheet_stream(FusedArgs fa) {
     }
     // ---- S11: transport along x over rows r-12 .. r-14 ---------------------
     {
-      const float dtc = L.dtc;
       // X1, row r-12: the left-going flux of face (col, col+1) (donor col+1)
       // goes to LDS; of its right-going flux (donor col) all but the room,
       // which needs the neighbour's left-going flux.
       const bool e_0 = (colmask & colmask_r & rmk(12)) != 0;
-      const float speed = 0.5f * (l_uf0 + l_uf1);
+      // The fast path works with the face's summed speed (twice the mean):
+      // the halving is exact, so the tests and the fraction are the same.
+      const float speed = EXACT ? 0.5f * (l_uf0 + l_uf1) : l_uf0 + l_uf1;
       const float hc = l_hx0, hr = l_hx1;
-      const bool live = !(fabsf(speed) < 1.0e-5f);
-      const float frac = fminf(EXACT ? fabsf(speed) * a.dt / cell : fabsf(speed) * dtc, 0.45f);
+      const bool live = !(fabsf(speed) < (EXACT ? 1.0e-5f : 2.0f * 1.0e-5f));
+      const float frac = fminf(EXACT ? fabsf(speed) * a.dt / cell : fabsf(speed) * half_dtc, 0.45f);
       const bool shut_l = !gate_u && l_g0 == 0.0f && hc <= 1.0e-6f;
       const bool shut_r = !gate_u && l_g1 == 0.0f && hr <= 1.0e-6f;
       const float dl = fminf(hr * frac, hr);
-      const bool ok_l = e_0 && live && !(hr <= 1.0e-8f) && !shut_l && !(dl <= 1.0e-8f) && !(speed > 0.0f);
+      // (dl <= hr, so dl > 1e-8 implies the donor's own hr > 1e-8 test,
+      // NaNs included; the same holds for every flux below.)
+      const bool ok_l = e_0 && live && !shut_l && !(dl <= 1.0e-8f) && !(speed > 0.0f);
       const float fa = ok_l ? dl : 0.0f;
       my[(R_FA + kSlot<P, 12>) * kRow] = fa;
       const bool pre = e_0 && live && !shut_r && speed > 0.0f;
@@ -651,7 +768,7 @@ __global__ __launch_bounds__(kLanes, 3) void k_sheet_stream(FusedArgs fa) {
       const float fl_m1 = e_m1b ? l_fa : 0.0f;
       const float room = xh1 - (0.0f + fl_m1);
       const float dr = fminf(xhdf, room);
-      const float fb = (xpre && !(room <= 1.0e-8f) && !(dr <= 1.0e-8f)) ? dr : 0.0f;
+      const float fb = (xpre && !(dr <= 1.0e-8f)) ? dr : 0.0f;
       my[(R_FB + kSlot<P, 13>) * kRow] = fb;
       // X3, row r-14: the depth after the x pass.
       const int yx = r - 14;
@@ -682,28 +799,27 @@ __global__ __launch_bounds__(kLanes, 3) void k_sheet_stream(FusedArgs fa) {
       // H1, VF and GW hold rows y-2..y+1 at indices 0..3; rmk(15), rmk(14) are
       // the masks of rows y, y+1.
       const bool e_0 = (colmask & rmk(15) & rmk(14)) != 0;  // edge (y, y+1)
-      const float dtc = L.dtc;
       auto gate_ok = [&](float g) -> bool { return gate_u ? true : g != 0.0f; };
       // Up-going flux of edge (k, k+1) (donor k+1 = window index k+1).
       auto fu = [&](int k, bool recv_gate) -> float {
-        float speed = 0.5f * (VF[k] + VF[k + 1]);
+        float speed = EXACT ? 0.5f * (VF[k] + VF[k + 1]) : VF[k] + VF[k + 1];
         float hd = H1[k + 1], hrv = H1[k];
-        bool live = !(fabsf(speed) < 1.0e-5f);
+        bool live = !(fabsf(speed) < (EXACT ? 1.0e-5f : 2.0f * 1.0e-5f));
         bool gate_shut = !recv_gate && hrv <= 1.0e-6f;
-        float frac = fminf(EXACT ? fabsf(speed) * a.dt / cell : fabsf(speed) * dtc, 0.45f);
+        float frac = fminf(EXACT ? fabsf(speed) * a.dt / cell : fabsf(speed) * half_dtc, 0.45f);
         float dh = fminf(hd * frac, hd);
-        bool ok = live && !(hd <= 1.0e-8f) && !gate_shut && !(dh <= 1.0e-8f) && !(speed > 0.0f);
+        bool ok = live && !gate_shut && !(dh <= 1.0e-8f) && !(speed > 0.0f);
         return ok ? dh : 0.0f;
       };
       auto fd = [&](int k, float prev, bool recv_gate) -> float {
-        float speed = 0.5f * (VF[k] + VF[k + 1]);
+        float speed = EXACT ? 0.5f * (VF[k] + VF[k + 1]) : VF[k] + VF[k + 1];
         float hd = H1[k], hrv = H1[k + 1];
-        bool live = !(fabsf(speed) < 1.0e-5f);
+        bool live = !(fabsf(speed) < (EXACT ? 1.0e-5f : 2.0f * 1.0e-5f));
         float room = hd - (0.0f + prev);
         bool gate_shut = !recv_gate && hrv <= 1.0e-6f;
-        float frac = fminf(EXACT ? fabsf(speed) * a.dt / cell : fabsf(speed) * dtc, 0.45f);
+        float frac = fminf(EXACT ? fabsf(speed) * a.dt / cell : fabsf(speed) * half_dtc, 0.45f);
         float dh = fminf(hd * frac, room);
-        bool ok = live && !(room <= 1.0e-8f) && !gate_shut && !(dh <= 1.0e-8f) && speed > 0.0f;
+        bool ok = live && !gate_shut && !(dh <= 1.0e-8f) && speed > 0.0f;
         return ok ? dh : 0.0f;
       };
       // The fluxes of the two edges above come from the previous row, where
@@ -798,10 +914,12 @@ __global__ __launch_bounds__(kLanes, 3) void k_sheet_stream(FusedArgs fa) {
         row(Odd{}, Edge{}, lean, r + 1);
       }
     };
-    if (!edge_strip && nu_uni && gate_uni)
-      run(std::true_type{});
+    if (nu_uni && gate_uni && !edge_strip)
+      run(std::integral_constant<int, 1>{});
+    else if (nu_uni && gate_uni)
+      run(std::integral_constant<int, 2>{});
     else
-      run(std::false_type{});
+      run(std::integral_constant<int, 0>{});
   } else {
     // Every lane walks the rows (the wet ballots count whole waves); only the
     // output columns load and store.
