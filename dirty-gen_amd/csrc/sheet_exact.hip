@@ -258,15 +258,9 @@ __global__ __launch_bounds__(256) void k_force_gauss_seidel_lds(const float* __r
   const int W = b.i1 - b.i0 + 1, H = b.j1 - b.j0 + 1;
   const int tmax = (W - 1) + 2 * (H - 1);
   const float cell = a.cell;
-  // x / (2 cell), correctly rounded without a division: q = x * y with
-  // y = RN(1 / (2 cell)) is within one ulp of the quotient, the remainder
-  // x - 2 cell q is exact in one FMA, and q + remainder * y rounds to the
-  // IEEE quotient (Markstein's theorem) while nothing underflows or
-  // overflows; zeros keep their sign from x * y.  A cell any of whose
-  // quotients lies outside [2^-100, 2^101) or is not finite is redone with
-  // the division itself.
+  // x / (2 cell) is div_by(x, y2c): the IEEE quotient in three instructions.
   const float d2c = 2.0f * cell;
-  const float y2c = 1.0f / d2c;
+  const double y2c = 1.0 / (double)d2c;
   auto col_at = [&](int r, int t) { return b.i0 + t - 2 * (r - b.j0); };
   auto load = [&](int c) {
     if (!live || c < 0 || c >= res) return;
@@ -378,14 +372,7 @@ __global__ __launch_bounds__(256) void k_force_gauss_seidel_lds(const float* __r
             }
           }
         };
-        bool ok = true;
-        auto fast = [&](float x) -> float {
-          const float q = x * y2c;
-          const float q2 = __builtin_fmaf(__builtin_fmaf(-d2c, q, x), y2c, q);
-          const unsigned e = (__float_as_uint(q) >> 23) & 0xffu;
-          ok = ok & ((x == 0.0f) | (e - 27u <= 200u));
-          return x == 0.0f ? q : q2;
-        };
+        auto fast = [&](float x) -> float { return div_by(x, y2c); };
         float uu = u0, vv = v0;
         // Two cells or more from every grid side, no read clamps: every value
         // the cell needs is read from the ring first (none of them changes
@@ -423,15 +410,13 @@ __global__ __launch_bounds__(256) void k_force_gauss_seidel_lds(const float* __r
             vv = take ? vv + dv : vv;
           }
         }
-        if (interior && ok) {
+        if (interior) {
           ru[row][i & (kRing - 1)] = uu;
           rv[row][i & (kRing - 1)] = vv;
         } else {
           // Cells near a grid side (clamped reads may land on the cell
-          // itself), the wind blend, or a quotient outside the fast range.
-          ok = true;
+          // itself) and the wind blend.
           update(fast, uu, vv);
-          if (!ok) update([&](float x) -> float { return x / d2c; }, uu, vv);
         }
         u[idx] = uu;
         v[idx] = vv;

@@ -150,6 +150,12 @@ struct FusedArgs {
   int nwin;
 };
 
+// RN64(1 / d) of a float divisor d, a type of its own so that a divisor
+// cannot be passed where its reciprocal is meant.
+struct Recip {
+  double v;
+};
+
 template <bool EXACT>
 struct Ar {
   static __device__ __forceinline__ float fma_(float a, float b, float c) {
@@ -158,10 +164,11 @@ struct Ar {
     else
       return __builtin_fmaf(a, b, c);
   }
-  // x / d, with r = 1 / d on the fast path.
-  static __device__ __forceinline__ float div(float x, float d, float r) {
+  // x / d: the IEEE quotient from rd = RN64(1 / d) on the exact path
+  // (div_by), x * r with r = RN32(1 / d) on the fast path.
+  static __device__ __forceinline__ float div(float x, Recip rd, float r) {
     if constexpr (EXACT)
-      return x / d;
+      return div_by(x, rd.v);
     else
       return x * r;
   }
@@ -288,7 +295,11 @@ __global__ __launch_bounds__(kLanes, 3) void k_sheet_stream(FusedArgs fa) {
   constexpr float kP1 = (float)(0.45 * (0.62 + 0.38 / 9.0)), kP2 = (float)(0.45 * 0.76 / 9.0),
                   kP3 = (float)(0.45 * 1.52 / 9.0 + 0.55);
   constexpr float kQ1 = (float)(0.62 + 0.38 / 9.0), kQ2 = (float)(0.76 / 9.0), kQ3 = (float)(1.52 / 9.0);
-  const float two = 2.0f * cell;
+  // The exact path's reciprocals of 2 cell, cell and cell^2 (each divisor
+  // rounded to float first, as the quotients it stands for divide by it).
+  const Recip rd2c{EXACT ? 1.0 / (double)(2.0f * cell) : 0.0};
+  const Recip rdcc{EXACT ? 1.0 / (double)(cell * cell) : 0.0};
+  const double rdc = EXACT ? 1.0 / (double)cell : 0.0;
   constexpr float kG[5] = {0.06f, 0.24f, 0.40f, 0.24f, 0.06f};
   constexpr float kS[5] = {0.02f, 0.08f, 0.80f, 0.08f, 0.02f};
   auto acc5 = [&](float a0, float a1, float a2, float a3, float a4, const float* k) -> float {
@@ -545,7 +556,7 @@ __global__ __launch_bounds__(kLanes, 3) void k_sheet_stream(FusedArgs fa) {
         // clamped rows).  The fast path keeps the sum unscaled (1 / cell^2
         // goes into S7).
         float sum = l_e3 + l_e1 + E[EO + 3] + E[EO + 1] - 4.0f * E[EO + 2];
-        my[(R_LW + ((r - 1) & 7)) * kRow] = EXACT ? A::div(sum, cell * cell, L.invc2) : sum;
+        my[(R_LW + ((r - 1) & 7)) * kRow] = EXACT ? A::div(sum, rdcc, L.invc2) : sum;
       }
     }
     // ---- S3: pressure at row r-3 ------------------------------------------
@@ -589,8 +600,8 @@ __global__ __launch_bounds__(kLanes, 3) void k_sheet_stream(FusedArgs fa) {
         vn = A::fma_(a.wz - vv, a.blend, vv);
       } else {
         if (EXACT) {
-          float px = A::div(l_pp - l_pm, two, inv2c);
-          float pz = A::div(abv(y, PW[1], PW[2]) - blw(y, PW[0], PW[1]), two, inv2c);
+          float px = A::div(l_pp - l_pm, rd2c, inv2c);
+          float pz = A::div(abv(y, PW[1], PW[2]) - blw(y, PW[0], PW[1]), rd2c, inv2c);
           float mob = nu_u ? a.mob0 : sheet_mobility(ld(L.nu, y));
           un = uu + -kGravity * px * mob * a.dt;
           vn = vv + -kGravity * pz * mob * a.dt;
@@ -611,8 +622,8 @@ __global__ __launch_bounds__(kLanes, 3) void k_sheet_stream(FusedArgs fa) {
       const int y = r - 5;
       float w;
       if constexpr (EXACT) {
-        float dvdx = A::div(l_vp - l_vm, two, inv2c);
-        float dudz = A::div(abv(y, UG[1], UG[2]) - blw(y, UG[0], UG[1]), two, inv2c);
+        float dvdx = A::div(l_vp - l_vm, rd2c, inv2c);
+        float dudz = A::div(abv(y, UG[1], UG[2]) - blw(y, UG[0], UG[1]), rd2c, inv2c);
         w = dvdx - dudz;
       } else {
         // 2 cell w
@@ -634,8 +645,8 @@ __global__ __launch_bounds__(kLanes, 3) void k_sheet_stream(FusedArgs fa) {
         float du, dv;
         bool ok;
         if (EXACT) {
-          float nx = A::div(wxp - wxm, two, inv2c);
-          float nz = A::div(wzp - wzm, two, inv2c);
+          float nx = A::div(wxp - wxm, rd2c, inv2c);
+          float nz = A::div(wzp - wzm, rd2c, inv2c);
           du = a.vort * a.dt * (-nz * w);
           dv = a.vort * a.dt * (nx * w);
           ok = (colmask & rmk(6)) != 0 && hv > eps && isfinite(w) && !(fabsf(w) < 1.0e-4f) && isfinite(du) &&
@@ -664,8 +675,8 @@ __global__ __launch_bounds__(kLanes, 3) void k_sheet_stream(FusedArgs fa) {
       float ov = nu_u ? a.ov0 : overload_of(ld(L.nu, y));
       float bv;
       if (EXACT) {
-        float ux = A::div(l_up - l_um, two, inv2c);
-        float vz = A::div(abv(y, VC[4], VC[5]) - blw(y, VC[3], VC[4]), two, inv2c);
+        float ux = A::div(l_up - l_um, rd2c, inv2c);
+        float vz = A::div(abv(y, VC[4], VC[5]) - blw(y, VC[3], VC[4]), rd2c, inv2c);
         float div = ux + vz;
         bv = div * ov - 0.015f * ov * lp;
       } else {
@@ -708,8 +719,8 @@ __global__ __launch_bounds__(kLanes, 3) void k_sheet_stream(FusedArgs fa) {
       float un = uu, vn = vv;
       if (PROJECT) {
         if (EXACT) {
-          float qx = A::div(l_qp - l_qm, two, inv2c);
-          float qz = A::div(abv(y, QW[1], QW[2]) - blw(y, QW[0], QW[1]), two, inv2c);
+          float qx = A::div(l_qp - l_qm, rd2c, inv2c);
+          float qz = A::div(abv(y, QW[1], QW[2]) - blw(y, QW[0], QW[1]), rd2c, inv2c);
           un = uu + 0.012f * a.dt * qx;
           vn = vv + 0.012f * a.dt * qz;
         } else {
@@ -751,7 +762,7 @@ __global__ __launch_bounds__(kLanes, 3) void k_sheet_stream(FusedArgs fa) {
       const float speed = EXACT ? 0.5f * (l_uf0 + l_uf1) : l_uf0 + l_uf1;
       const float hc = l_hx0, hr = l_hx1;
       const bool live = !(fabsf(speed) < (EXACT ? 1.0e-5f : 2.0f * 1.0e-5f));
-      const float frac = fminf(EXACT ? fabsf(speed) * a.dt / cell : fabsf(speed) * half_dtc, 0.45f);
+      const float frac = fminf(EXACT ? div_by(fabsf(speed) * a.dt, rdc) : fabsf(speed) * half_dtc, 0.45f);
       const bool shut_l = !gate_u && l_g0 == 0.0f && hc <= 1.0e-6f;
       const bool shut_r = !gate_u && l_g1 == 0.0f && hr <= 1.0e-6f;
       const float dl = fminf(hr * frac, hr);
@@ -806,7 +817,7 @@ __global__ __launch_bounds__(kLanes, 3) void k_sheet_stream(FusedArgs fa) {
         float hd = H1[k + 1], hrv = H1[k];
         bool live = !(fabsf(speed) < (EXACT ? 1.0e-5f : 2.0f * 1.0e-5f));
         bool gate_shut = !recv_gate && hrv <= 1.0e-6f;
-        float frac = fminf(EXACT ? fabsf(speed) * a.dt / cell : fabsf(speed) * half_dtc, 0.45f);
+        float frac = fminf(EXACT ? div_by(fabsf(speed) * a.dt, rdc) : fabsf(speed) * half_dtc, 0.45f);
         float dh = fminf(hd * frac, hd);
         bool ok = live && !gate_shut && !(dh <= 1.0e-8f) && !(speed > 0.0f);
         return ok ? dh : 0.0f;
@@ -817,7 +828,7 @@ __global__ __launch_bounds__(kLanes, 3) void k_sheet_stream(FusedArgs fa) {
         bool live = !(fabsf(speed) < (EXACT ? 1.0e-5f : 2.0f * 1.0e-5f));
         float room = hd - (0.0f + prev);
         bool gate_shut = !recv_gate && hrv <= 1.0e-6f;
-        float frac = fminf(EXACT ? fabsf(speed) * a.dt / cell : fabsf(speed) * half_dtc, 0.45f);
+        float frac = fminf(EXACT ? div_by(fabsf(speed) * a.dt, rdc) : fabsf(speed) * half_dtc, 0.45f);
         float dh = fminf(hd * frac, room);
         bool ok = live && !gate_shut && !(dh <= 1.0e-8f) && speed > 0.0f;
         return ok ? dh : 0.0f;
