@@ -232,23 +232,72 @@ __global__ void k_force_gauss_seidel(const float* __restrict__ h, float* u, floa
   }
 }
 
-// The same sweep with the planes it reads held in LDS.  At wavefront t row
-// jj updates column c(jj, t) = i0 + t - 2 (jj - j0); every cell of that
-// wavefront reads row jj only within c +- 4 (gradient and vorticity stencils
-// reach 2 columns in their own row, 1 column one row away, the same column
-// two rows away).  So each row keeps a ring of 16 columns of u, v, p and h:
-// at step t it loads column c + 4 (not yet updated: its update comes 4
-// wavefronts later), and an updated cell goes to the ring and to memory.
-// One lane per ring row (box rows plus 2 above and below, clamped to the
-// grid).  Same arithmetic in the same order as the
-// global-memory sweep.  Needs the box's rows + 4 <= 256.
-constexpr int kRing = 16;
-constexpr int kRingRows = 256;
+// Kicks of a box's wet cells, gravity or the wind blend (the first half of
+// force_gravity_open :2236-2260 and force_wind_open :2660-2681, per cell),
+// into (ku, kv).  A cell's kick reads p, its viscosity and its own u and v,
+// and the in-place confinement sweep changes none of them before it reaches
+// the cell (a cell's confinement writes only that cell), so every kick can
+// come before the sweep, all cells at once.  ku, kv may be u, v: without
+// confinement the kicks are the whole force pass.  Dry cells are left as
+// they are (ku, kv unwritten).
+__global__ void k_kick_box(const float* __restrict__ h, const float* u, const float* v, const float* __restrict__ nu,
+                           const float* __restrict__ p, float* ku, float* kv, Region r, StepArgs a) {
+  int i, j;
+  if (!region_cell(r, &i, &j)) return;
+  const int res = a.res;
+  const size_t idx = (size_t)j * res + i;
+  if (h[idx] <= a.eps) return;
+  float uu = u[idx], vv = v[idx];
+  if (a.wind) {
+    uu = uu + (a.wx - uu) * a.blend;
+    vv = vv + (a.wz - vv) * a.blend;
+  } else {
+    float px, pz;
+    grad_at(p, res, i, j, a.cell, &px, &pz);
+    const float mob = a.nu_uniform ? a.mob0 : sheet_mobility(nu[idx]);
+    uu = uu + -kGravity * px * mob * a.dt;
+    vv = vv + -kGravity * pz * mob * a.dt;
+  }
+  ku[idx] = uu;
+  kv[idx] = vv;
+}
 
-__global__ __launch_bounds__(256) void k_force_gauss_seidel_lds(const float* __restrict__ h, float* u, float* v,
-                                                                const float* __restrict__ nu,
-                                                                const float* __restrict__ p, Box b, StepArgs a) {
-  __shared__ float ru[kRingRows][kRing], rv[kRingRows][kRing], rp[kRingRows][kRing], rh[kRingRows][kRing];
+// The confinement half of the forward Gauss-Seidel sweep, with the kicks
+// already in (ku, kv) (k_kick_box) and the planes it reads held in LDS.  At
+// wavefront t ring row jj updates column c(jj, t) = i0 + t - 2 (jj - j0);
+// every cell of that wavefront reads row jj only within c +- 4 (the
+// vorticity stencils reach 2 columns in their own row, 1 column one row
+// away, the same column two rows away).  One lane per ring row (box rows
+// plus 2 above and below, clamped to the grid); each row keeps a ring of 32
+// columns of u and v, and the cell's own depth and kicked velocity for the
+// chunk in LDS.
+//
+// Memory moves kChunk wavefronts at a time and coalesced: a lane's row is
+// far from its neighbours' in memory, so a load or store per lane and
+// wavefront costs the CU's memory pipeline one cache line per lane; instead
+// each copy instruction covers 8 rows x 8 consecutive columns of the
+// wave's rows.  Between two chunks, after a barrier: the columns loaded
+// during the chunk (c + 4 .. c + kChunk + 3 of the next chunk's first
+// wavefront, not yet updated: a row's cells change only when the wavefront
+// reaches them) go to the ring, the chunk before the last goes back to
+// memory, and the next chunk's columns are loaded into registers.  The ring
+// slots a chunk's columns land in held columns at least 13 behind the
+// wavefront, which no cell reads any more.  Same arithmetic in the same
+// order as the global-memory sweep: the confinement of a cell reads its own
+// kicked values and its neighbours' values as the raster order leaves them
+// (final before it, unkicked after it).  Needs the box's rows + 4 <= 256
+// and vort > 0.
+constexpr int kRing = 32;
+constexpr int kRingPad = kRing + 1;  // lane = row: an odd row stride spreads the lanes over the banks
+constexpr int kRingRows = 256;
+constexpr int kChunk = 8;
+static_assert(3 * kChunk + 3 < kRing, "the landed, current and flushed columns must fit the ring");
+
+__global__ __launch_bounds__(256) void k_confine_gs_lds(const float* __restrict__ h, float* u, float* v,
+                                                        const float* __restrict__ ku, const float* __restrict__ kv,
+                                                        Box b, StepArgs a) {
+  __shared__ float ru[kRingRows][kRingPad], rv[kRingRows][kRingPad];
+  __shared__ float oh_[kRingRows][kChunk + 1], ou_[kRingRows][kChunk + 1], ov_[kRingRows][kChunk + 1];
   const int res = a.res;
   const int r_lo = b.j0 - 2 > 0 ? b.j0 - 2 : 0;
   const int r_hi = b.j1 + 2 < res - 1 ? b.j1 + 2 : res - 1;
@@ -257,171 +306,185 @@ __global__ __launch_bounds__(256) void k_force_gauss_seidel_lds(const float* __r
   const bool live = jj <= r_hi;
   const int W = b.i1 - b.i0 + 1, H = b.j1 - b.j0 + 1;
   const int tmax = (W - 1) + 2 * (H - 1);
-  const float cell = a.cell;
   // x / (2 cell) is div_by(x, y2c): the IEEE quotient in three instructions.
-  const float d2c = 2.0f * cell;
-  const double y2c = 1.0 / (double)d2c;
+  const double y2c = 1.0 / (double)(2.0f * a.cell);
+  auto fast = [&](float x) -> float { return div_by(x, y2c); };
+  const float kc = a.vort * a.dt;
   auto col_at = [&](int r, int t) { return b.i0 + t - 2 * (r - b.j0); };
-  auto load = [&](int c) {
-    if (!live || c < 0 || c >= res) return;
-    const size_t idx = (size_t)jj * res + c;
-    ru[row][c & (kRing - 1)] = u[idx];
-    rv[row][c & (kRing - 1)] = v[idx];
-    rp[row][c & (kRing - 1)] = p[idx];
-    rh[row][c & (kRing - 1)] = h[idx];
+  auto in_row = [&](int r, int c) { return r_lo + r <= r_hi && c >= 0 && c < res; };
+  auto in_box = [&](int r, int c) {
+    const int y = r_lo + r;
+    return y <= r_hi && y >= b.j0 && y <= b.j1 && c >= b.i0 && c <= b.i1;
+  };
+  // Copy m of a lane covers ring row wbase + 8 m + lr, column l8 of the 8.
+  const int wbase = (int)(threadIdx.x & ~63u), lr = (int)((threadIdx.x & 63u) >> 3), l8 = (int)(threadIdx.x & 7u);
+  float qu[kChunk], qv[kChunk], qh[kChunk], qku[kChunk], qkv[kChunk];
+  // The chunk after the one starting at wavefront t0: ring columns and own
+  // values into registers.
+  auto fetch = [&](int t0) {
+#pragma unroll
+    for (int m = 0; m < kChunk; m++) {
+      const int r = wbase + 8 * m + lr, y = r_lo + r, cb = col_at(y, t0);
+      const int c = cb + kChunk + 4 + l8;
+      if (in_row(r, c)) {
+        const size_t idx = (size_t)y * res + c;
+        qu[m] = u[idx];
+        qv[m] = v[idx];
+      }
+      const int co = cb + kChunk + l8;
+      qh[m] = 0.0f;
+      if (in_box(r, co)) {
+        const size_t idx = (size_t)y * res + co;
+        qh[m] = h[idx];
+        qku[m] = ku[idx];
+        qkv[m] = kv[idx];
+      }
+    }
+  };
+  // The registers to the ring and the own-value rows for the chunk starting
+  // at wavefront t0 (own depth 0 outside the box: eps >= 0 skips the cell).
+  auto land = [&](int t0) {
+#pragma unroll
+    for (int m = 0; m < kChunk; m++) {
+      const int r = wbase + 8 * m + lr, cb = col_at(r_lo + r, t0);
+      const int c = cb + 4 + l8;
+      if (in_row(r, c)) {
+        ru[r][c & (kRing - 1)] = qu[m];
+        rv[r][c & (kRing - 1)] = qv[m];
+      }
+      oh_[r][l8] = qh[m];
+      ou_[r][l8] = qku[m];
+      ov_[r][l8] = qkv[m];
+    }
+  };
+  // The final values of the chunk starting at wavefront t0 to memory.
+  auto flush = [&](int t0) {
+#pragma unroll
+    for (int m = 0; m < kChunk; m++) {
+      const int r = wbase + 8 * m + lr, y = r_lo + r, c = col_at(y, t0) + l8;
+      if (in_box(r, c)) {
+        const size_t idx = (size_t)y * res + c;
+        u[idx] = ru[r][c & (kRing - 1)];
+        v[idx] = rv[r][c & (kRing - 1)];
+      }
+    }
   };
   // Ring slot of grid cell (x, y), the reads clamped to the grid as the
   // global-memory helpers clamp them.
-  auto at = [&](float (*r)[kRing], int x, int y) -> float {
+  auto at = [&](float (*rg)[kRingPad], int x, int y) -> float {
     x = x < 0 ? 0 : (x > res - 1 ? res - 1 : x);
     y = y < 0 ? 0 : (y > res - 1 ? res - 1 : y);
-    return r[y - r_lo][x & (kRing - 1)];
+    return rg[y - r_lo][x & (kRing - 1)];
   };
-  auto vort = [&](int x, int y, auto div2c) -> float {
+  auto vort = [&](int x, int y) -> float {
     const int xp = x + 1 < res ? x + 1 : res - 1, xm = x > 0 ? x - 1 : 0;
     const int yp = y + 1 < res ? y + 1 : res - 1, ym = y > 0 ? y - 1 : 0;
-    const float dvdx = div2c(at(rv, xp, y) - at(rv, xm, y));
-    const float dudz = div2c(at(ru, x, yp) - at(ru, x, ym));
-    return dvdx - dudz;
+    return fast(at(rv, xp, y) - at(rv, xm, y)) - fast(at(ru, x, yp) - at(ru, x, ym));
   };
-  // The column a row needs next comes into registers one wavefront early
-  // and goes to the ring at the start of the next: its memory latency hides
-  // behind this wavefront's update.  The slot it lands in holds column
-  // c - 11, which no cell of the current wavefront reads, so one barrier per
-  // wavefront orders everything.
-  float fu = 0.0f, fv = 0.0f, fp = 0.0f, fh = 0.0f;
-  int fc = -1;
-  auto fetch = [&](int c) {
-    fc = (live && c >= 0 && c < res) ? c : -1;
-    if (fc < 0) return;
-    const size_t idx = (size_t)jj * res + c;
-    fu = u[idx];
-    fv = v[idx];
-    fp = p[idx];
-    fh = h[idx];
-  };
-  // The viscosity of the row's next cell (per-cell viscosity only).
-  const bool cell_nu = !a.nu_uniform;
-  auto nu_of = [&](int c) {
-    return (cell_nu && live && c >= 0 && c < res) ? nu[(size_t)jj * res + c] : 0.0f;
-  };
-  float nu_next;
   {
+    // The first chunk, row by row.
     const int c0 = col_at(jj, 0);
-    for (int c = c0 - 4; c <= c0 + 4; c++) load(c);
-    fetch(c0 + 5);
-    nu_next = nu_of(c0);
+    for (int c = c0 - 4; c <= c0 + kChunk + 3; c++)
+      if (in_row(row, c)) {
+        const size_t idx = (size_t)jj * res + c;
+        ru[row][c & (kRing - 1)] = u[idx];
+        rv[row][c & (kRing - 1)] = v[idx];
+      }
+    for (int k = 0; k < kChunk; k++) {
+      const int c = c0 + k;
+      oh_[row][k] = 0.0f;
+      if (in_box(row, c)) {
+        const size_t idx = (size_t)jj * res + c;
+        oh_[row][k] = h[idx];
+        ou_[row][k] = ku[idx];
+        ov_[row][k] = kv[idx];
+      }
+    }
   }
-  for (int t = 0; t <= tmax; t++) {
-    const int c = live ? col_at(jj, t) : 0;
-    const float nu_c = nu_next;
-    nu_next = nu_of(c + 1);
-    if (t > 0) {
-      if (fc >= 0) {
-        ru[row][fc & (kRing - 1)] = fu;
-        rv[row][fc & (kRing - 1)] = fv;
-        rp[row][fc & (kRing - 1)] = fp;
-        rh[row][fc & (kRing - 1)] = fh;
+  if (kChunk <= tmax) fetch(0);
+  for (int t0 = 0;; t0 += kChunk) {
+    const int t1 = t0 + kChunk - 1 < tmax ? t0 + kChunk - 1 : tmax;
+    float nh = 0.0f, nu_ = 0.0f, nv_ = 0.0f;
+    for (int t = t0; t <= t1; t++) {
+      const int c = live ? col_at(jj, t) : 0;
+      __syncthreads();
+      // The cell's own values: the chunk's first from LDS now, the others
+      // read one wavefront early.
+      float ch = nh, cu = nu_, cv = nv_;
+      if (t == t0) {
+        ch = oh_[row][0];
+        cu = ou_[row][0];
+        cv = ov_[row][0];
       }
-      fetch(c + 5);
-    }
-    __syncthreads();
-    if (live && jj >= b.j0 && jj <= b.j1 && c >= b.i0 && c <= b.i1) {
+      if (t < t1) {
+        nh = oh_[row][t + 1 - t0];
+        nu_ = ou_[row][t + 1 - t0];
+        nv_ = ov_[row][t + 1 - t0];
+      }
+      if (!(ch > a.eps)) continue;
       const int i = c, j = jj;
-      const size_t idx = (size_t)j * res + i;
-      if (at(rh, i, j) > a.eps) {
-        const float u0 = at(ru, i, j), v0 = at(rv, i, j);
-        // The cell's kick and confinement with the quotient div2c; the ring
-        // holds the cell's own values as it goes (no other cell of this
+      float uu = cu, vv = cv;
+      // Two cells or more from every grid side, no read clamps: every value
+      // the cell needs is read from the ring first (none of them changes
+      // during this wavefront), the cell's own values stay in registers, and
+      // the confinement's tests become selects.
+      if (i >= 2 && i <= res - 3 && j >= 2 && j <= res - 3) {
+        const int y0 = j - r_lo;
+        auto U = [&](int dx, int dy) { return ru[y0 + dy][(i + dx) & (kRing - 1)]; };
+        auto Vv = [&](int dx, int dy) { return rv[y0 + dy][(i + dx) & (kRing - 1)]; };
+        const float v_e = Vv(1, 0), v_w = Vv(-1, 0), v_ee = Vv(2, 0), v_ww = Vv(-2, 0);
+        const float v_ne = Vv(1, 1), v_nw = Vv(-1, 1), v_se = Vv(1, -1), v_sw = Vv(-1, -1);
+        const float u_n = U(0, 1), u_s = U(0, -1), u_nn = U(0, 2), u_ss = U(0, -2);
+        const float u_ne = U(1, 1), u_se = U(1, -1), u_nw = U(-1, 1), u_sw = U(-1, -1);
+        const float w = fast(v_e - v_w) - fast(u_n - u_s);
+        const float wxp = fabsf(fast(v_ee - vv) - fast(u_ne - u_se));
+        const float wxm = fabsf(fast(vv - v_ww) - fast(u_nw - u_sw));
+        const float wzp = fabsf(fast(v_ne - v_nw) - fast(u_nn - uu));
+        const float wzm = fabsf(fast(v_se - v_sw) - fast(uu - u_ss));
+        const float nx = fast(wxp - wxm);
+        const float nz = fast(wzp - wzm);
+        const float du = kc * (-nz * w);
+        const float dv = kc * (nx * w);
+        const bool take = isfinite(w) && !(fabsf(w) < 1.0e-4f) && isfinite(du) && isfinite(dv) &&
+                          du * du + dv * dv <= 0.0625f;
+        uu = take ? uu + du : uu;
+        vv = take ? vv + dv : vv;
+      } else {
+        // Near a grid side a clamped read may land on the cell itself: its
+        // kicked values go to the ring first (no other cell of this
         // wavefront reads them).
-        auto update = [&](auto div2c, float& uu, float& vv) {
-          uu = u0;
-          vv = v0;
-          if (a.wind) {
-            uu = uu + (a.wx - uu) * a.blend;
-            vv = vv + (a.wz - vv) * a.blend;
-          } else {
-            const int ip = i + 1 < res ? i + 1 : res - 1, im = i > 0 ? i - 1 : 0;
-            const int jp = j + 1 < res ? j + 1 : res - 1, jm = j > 0 ? j - 1 : 0;
-            const float px = div2c(at(rp, ip, j) - at(rp, im, j));
-            const float pz = div2c(at(rp, i, jp) - at(rp, i, jm));
-            const float mob = a.nu_uniform ? a.mob0 : sheet_mobility(nu_c);
-            uu = uu + -kGravity * px * mob * a.dt;
-            vv = vv + -kGravity * pz * mob * a.dt;
-          }
-          ru[row][i & (kRing - 1)] = uu;
-          rv[row][i & (kRing - 1)] = vv;
-          if (a.vort > 0.0f) {
-            // confine_at on the ring: this cell's own kick is already in it.
-            const float w = vort(i, j, div2c);
-            if (isfinite(w) && !(fabsf(w) < 1.0e-4f)) {
-              const float wxp = fabsf(vort(i + 1 < res ? i + 1 : res - 1, j, div2c));
-              const float wxm = fabsf(vort(i > 0 ? i - 1 : 0, j, div2c));
-              const float wzp = fabsf(vort(i, j + 1 < res ? j + 1 : res - 1, div2c));
-              const float wzm = fabsf(vort(i, j > 0 ? j - 1 : 0, div2c));
-              const float nx = div2c(wxp - wxm);
-              const float nz = div2c(wzp - wzm);
-              const float du = a.vort * a.dt * (-nz * w);
-              const float dv = a.vort * a.dt * (nx * w);
-              if (isfinite(du) && isfinite(dv) && du * du + dv * dv <= 0.0625f) {
-                uu = uu + du;
-                vv = vv + dv;
-                ru[row][i & (kRing - 1)] = uu;
-                rv[row][i & (kRing - 1)] = vv;
-              }
-            }
-          }
-        };
-        auto fast = [&](float x) -> float { return div_by(x, y2c); };
-        float uu = u0, vv = v0;
-        // Two cells or more from every grid side, no read clamps: every value
-        // the cell needs is read from the ring first (none of them changes
-        // during this wavefront), the cell's own new values stay in
-        // registers, and the confinement's tests become selects.
-        const bool interior = !a.wind && i >= 2 && i <= res - 3 && j >= 2 && j <= res - 3;
-        if (interior) {
-          const int y0 = j - r_lo;
-          auto U = [&](int dx, int dy) { return ru[y0 + dy][(i + dx) & (kRing - 1)]; };
-          auto Vv = [&](int dx, int dy) { return rv[y0 + dy][(i + dx) & (kRing - 1)]; };
-          auto P = [&](int dx, int dy) { return rp[y0 + dy][(i + dx) & (kRing - 1)]; };
-          const float p_e = P(1, 0), p_w = P(-1, 0), p_n = P(0, 1), p_s = P(0, -1);
-          const float v_e = Vv(1, 0), v_w = Vv(-1, 0), v_ee = Vv(2, 0), v_ww = Vv(-2, 0);
-          const float v_ne = Vv(1, 1), v_nw = Vv(-1, 1), v_se = Vv(1, -1), v_sw = Vv(-1, -1);
-          const float u_n = U(0, 1), u_s = U(0, -1), u_nn = U(0, 2), u_ss = U(0, -2);
-          const float u_ne = U(1, 1), u_se = U(1, -1), u_nw = U(-1, 1), u_sw = U(-1, -1);
-          const float px = fast(p_e - p_w);
-          const float pz = fast(p_n - p_s);
-          const float mob = a.nu_uniform ? a.mob0 : sheet_mobility(nu_c);
-          uu = uu + -kGravity * px * mob * a.dt;
-          vv = vv + -kGravity * pz * mob * a.dt;
-          if (a.vort > 0.0f) {
-            const float w = fast(v_e - v_w) - fast(u_n - u_s);
-            const float wxp = fabsf(fast(v_ee - vv) - fast(u_ne - u_se));
-            const float wxm = fabsf(fast(vv - v_ww) - fast(u_nw - u_sw));
-            const float wzp = fabsf(fast(v_ne - v_nw) - fast(u_nn - uu));
-            const float wzm = fabsf(fast(v_se - v_sw) - fast(uu - u_ss));
-            const float nx = fast(wxp - wxm);
-            const float nz = fast(wzp - wzm);
-            const float du = a.vort * a.dt * (-nz * w);
-            const float dv = a.vort * a.dt * (nx * w);
-            const bool take = isfinite(w) && !(fabsf(w) < 1.0e-4f) && isfinite(du) && isfinite(dv) &&
-                              du * du + dv * dv <= 0.0625f;
-            uu = take ? uu + du : uu;
-            vv = take ? vv + dv : vv;
+        ru[row][i & (kRing - 1)] = uu;
+        rv[row][i & (kRing - 1)] = vv;
+        const float w = vort(i, j);
+        if (isfinite(w) && !(fabsf(w) < 1.0e-4f)) {
+          const float wxp = fabsf(vort(i + 1 < res ? i + 1 : res - 1, j));
+          const float wxm = fabsf(vort(i > 0 ? i - 1 : 0, j));
+          const float wzp = fabsf(vort(i, j + 1 < res ? j + 1 : res - 1));
+          const float wzm = fabsf(vort(i, j > 0 ? j - 1 : 0));
+          const float nx = fast(wxp - wxm);
+          const float nz = fast(wzp - wzm);
+          const float du = kc * (-nz * w);
+          const float dv = kc * (nx * w);
+          if (isfinite(du) && isfinite(dv) && du * du + dv * dv <= 0.0625f) {
+            uu = uu + du;
+            vv = vv + dv;
           }
         }
-        if (interior) {
-          ru[row][i & (kRing - 1)] = uu;
-          rv[row][i & (kRing - 1)] = vv;
-        } else {
-          // Cells near a grid side (clamped reads may land on the cell
-          // itself) and the wind blend.
-          update(fast, uu, vv);
-        }
-        u[idx] = uu;
-        v[idx] = vv;
       }
+      ru[row][i & (kRing - 1)] = uu;
+      rv[row][i & (kRing - 1)] = vv;
     }
+    // Every cell of the chunk is in the ring before any lane copies rows
+    // that are not its own.
+    __syncthreads();
+    if (t0 + kChunk > tmax) {
+      if (t0 >= kChunk) flush(t0 - kChunk);
+      flush(t0);
+      break;
+    }
+    land(t0 + kChunk);
+    if (t0 >= kChunk) flush(t0 - kChunk);
+    if (t0 + 2 * kChunk <= tmax) fetch(t0 + kChunk);
   }
 }
 
@@ -963,6 +1026,25 @@ inline dim3 grid_for(const Region& r, dim3 blk) {
               (unsigned)((r.j1 - r.j0 + 1 + (int)blk.y - 1) / (int)blk.y));
 }
 
+// The in-place force pass of one box in raster Gauss-Seidel order: the
+// kicks into (su, sv), then the confinement sweep; without confinement the
+// kicks alone, in place.  Boxes over 252 rows take the global-memory sweep.
+void gauss_seidel_force(const SheetPlanes& s, const Box& b, const StepArgs& a, hipStream_t st) {
+  if (b.j1 - b.j0 + 1 + 4 > kRingRows) {
+    hipLaunchKernelGGL(k_force_gauss_seidel, dim3(1), dim3(256), 0, st, s.h, s.u, s.v, s.nu, s.p, b, a);
+    return;
+  }
+  Region box{b.i0, b.i1, b.j0, b.j1};
+  dim3 blk(64, 4);
+  dim3 g_box = grid_for(box, blk);
+  if (a.vort > 0.0f) {
+    hipLaunchKernelGGL(k_kick_box, g_box, blk, 0, st, s.h, s.u, s.v, s.nu, s.p, s.su, s.sv, box, a);
+    hipLaunchKernelGGL(k_confine_gs_lds, dim3(1), dim3(256), 0, st, s.h, s.u, s.v, s.su, s.sv, b, a);
+  } else {
+    hipLaunchKernelGGL(k_kick_box, g_box, blk, 0, st, s.h, s.u, s.v, s.nu, s.p, s.u, s.v, box, a);
+  }
+}
+
 }  // namespace
 
 // One substep on one box (substep, src/fluid_core.rs:1879-2082).  do_flux 0
@@ -981,10 +1063,7 @@ void exact_substep(const SheetPlanes& s, const Box& b, const StepArgs& a, bool d
     hipLaunchKernelGGL(k_zblur_pressure, g_rim, blk, 0, st, s.tg, s.ts, s.eta, s.p, rim, b, res);
   }
   if (gauss_seidel) {
-    if (b.j1 - b.j0 + 1 + 4 <= kRingRows)
-      hipLaunchKernelGGL(k_force_gauss_seidel_lds, dim3(1), dim3(256), 0, st, s.h, s.u, s.v, s.nu, s.p, b, a);
-    else
-      hipLaunchKernelGGL(k_force_gauss_seidel, dim3(1), dim3(256), 0, st, s.h, s.u, s.v, s.nu, s.p, b, a);
+    gauss_seidel_force(s, b, a, st);
   } else {
     hipLaunchKernelGGL(k_force, g_rim, blk, 0, st, s.h, s.u, s.v, s.nu, s.p, s.su, s.sv, rim, b, a);
     if (a.vort > 0.0f)
@@ -1037,10 +1116,7 @@ void exact_substep_walled(const SheetPlanes& s, const Box& b, const StepArgs& a0
   hipLaunchKernelGGL(k_zblur_pressure, g_rim, blk, 0, st, s.tg, s.ts, s.eta, s.p, rim, b, res);
   hipLaunchKernelGGL(k_noslip_ghost, g_box, blk, 0, st, s.u, s.v, s.h, s.solid, box, res, a.eps);
   if (a.wind) {
-    if (b.j1 - b.j0 + 1 + 4 <= kRingRows)
-      hipLaunchKernelGGL(k_force_gauss_seidel_lds, dim3(1), dim3(256), 0, st, s.h, s.u, s.v, s.nu, s.p, b, a);
-    else
-      hipLaunchKernelGGL(k_force_gauss_seidel, dim3(1), dim3(256), 0, st, s.h, s.u, s.v, s.nu, s.p, b, a);
+    gauss_seidel_force(s, b, a, st);  // su, sv are free until the redirect
   } else if (par) {
     hipLaunchKernelGGL(k_kick_walled, g_box, blk, 0, st, s.h, s.u, s.v, s.nu, s.p, s.solid, box, a);
     if (a.vort > 0.0f)
