@@ -192,12 +192,14 @@ __global__ __launch_bounds__(kLanes, 3) void k_sheet_stream(FusedArgs fa) {
   using A = Ar<EXACT>;
   __shared__ float lds[kRows * kRow];
 
-  // XCD-aware tile order: the 8 XCDs take workgroups round robin, so when the
-  // count divides evenly each XCD gets a contiguous run of tiles (neighbouring
-  // strips and chunks share halo rows in that XCD's L2).
+  // XCD-aware tile order: the 8 XCDs take workgroups round robin, so XCD x
+  // holds workgroups x, x + 8, ...; each XCD gets a contiguous run of tiles
+  // (neighbouring strips and chunks share halo rows in that XCD's L2).  With
+  // total = 8 q + rem, XCDs below rem hold q + 1 tiles and the rest q.
   const int total = (int)gridDim.x;
   const int wg = (int)blockIdx.x;
-  const int tile_lin = (total % 8 == 0) ? (wg % 8) * (total / 8) + wg / 8 : wg;
+  const int xq = total >> 3, xrem = total & 7, xcd = wg & 7;
+  const int tile_lin = xcd * xq + min(xcd, xrem) + (wg >> 3);
   const int win =
       fa.nwin <= 2 ? (fa.nwin == 2 && tile_lin >= fa.table[1].hot.tile0 ? 1 : 0) : fa.tile_window[tile_lin];
   ConstLayer& LA = fa.table[win];
