@@ -104,6 +104,8 @@ def traffic_probe(args):
     z = np.zeros(n, np.float32)
     pair = S.SheetPair(sc["res"], sc["cell"], path=args.path)
     pair.upload(sc["bed"], sc["water"], z, z, sc["slurry"], z, z, sc["nu_slurry"])
+    if args.xcd_weights:
+        pair.set_xcd_weights([float(w) for w in args.xcd_weights.split(",")])
     for _ in range(4):
         pair.step(1.0 / 120.0)
     pair.last_kernel_ms()
@@ -130,6 +132,8 @@ def pmc_traffic(args):
         cmd = ["timeout", "-s", "KILL", "180", "rocprofv3", "--output-format", "csv", "--pmc", counter, "-d", d,
                "-o", "pmc", "--", sys.executable, os.path.abspath(__file__), "--traffic-probe", "--res",
                str(args.res), "--path", args.path]
+        if args.xcd_weights:
+            cmd += ["--xcd-weights", args.xcd_weights]
         print(f"[bench] traffic: {counter} pass", file=sys.stderr, flush=True)
         r = subprocess.run(cmd, capture_output=True, text=True, cwd="/tmp", env=dict(os.environ, TMPDIR="/tmp"))
         files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
@@ -558,6 +562,9 @@ def main():
     ap.add_argument("--sur-tol", type=float, default=1.0e-3, help="surrogate: judge tolerance, metres")
     ap.add_argument("--windows", type=int, default=64)
     ap.add_argument("--film", type=float, default=0.0, help="batch: base film under the puddles, metres")
+    ap.add_argument("--no-balance", action="store_true",
+                    help="pair: keep uniform chunks (no XCD balance calibration before the run)")
+    ap.add_argument("--xcd-weights", default="", help=argparse.SUPPRESS)
     ap.add_argument("--autotune", action="store_true",
                     help="time candidate rows per workgroup first and keep the fastest (default: the "
                          "occupancy-derived height, so a kernel trace of the run holds one launch shape)")
@@ -600,6 +607,19 @@ def main():
         chunk, tune_ms = pair.autotune(dt)
         print(f"[bench] autotuned rows per workgroup: {chunk or 'default'} ({tune_ms:.3f} ms)", file=sys.stderr,
               flush=True)
+    balance = None
+    if not args.no_balance and args.path != "exact":
+        # Tuning step before the run (state restored afterwards): the rows
+        # of each XCD's tiles sized to that XCD's measured speed.
+        w, (t_uni, t_bal), us0, us1 = pair.balance(dt)
+        kept = any(x != 1.0 for x in w)
+        balance = {"kept": kept, "weights": [round(x, 4) for x in w], "kernel_ms_uniform": t_uni,
+                   "kernel_ms_balanced": t_bal, "xcd_finish_us_uniform": [round(x, 1) for x in us0],
+                   "xcd_finish_us_last_round": [round(x, 1) for x in us1]}
+        if kept:
+            args.xcd_weights = ",".join(f"{x:.6f}" for x in w)
+        print(f"[bench] XCD balance: uniform {t_uni:.4f} ms, balanced {t_bal:.4f} ms, kept {kept}; XCD finish us "
+              f"{[round(x) for x in us0]} -> {[round(x) for x in us1]}", file=sys.stderr, flush=True)
     print(f"[bench] warmup {args.warmup} steps", file=sys.stderr, flush=True)
     for _ in range(args.warmup):
         pair.step(dt)
@@ -636,7 +656,7 @@ def main():
         "data": "synthetic (seeded fBm bed, Gaussian puddles)",
         "config": {"workload": f"plain two-layer sheet {res}x{res} @ {cell} m, dt 1/120 s, water nu 0.9 vort 0.08, "
                                f"slurry nu 0.5 vort 0.05", "res": res, "cell_m": cell, "path": args.path,
-                   "chunk_rows": chunk or "default",
+                   "chunk_rows": chunk or "default", "xcd_balance": balance,
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,

@@ -10,6 +10,8 @@
 #include <math.h>
 #include <string.h>
 
+#include <algorithm>
+#include <cmath>
 #include <vector>
 
 #include "core.hpp"
@@ -65,6 +67,18 @@ struct silt_pair {
   // moved depth after the step wrote them): recomputed before they are read.
   bool stats_stale = false;
   long long spec_hits = 0, spec_misses = 0;
+  // Back-off after an undone speculation: the next spec_pause steps plan
+  // first and launch after; each miss doubles the pause (1, 2, 4 ... 64
+  // steps), a speculation that holds resets it.
+  int spec_pause = 0, spec_pause_next = 1;
+  // XCD balance of the fused launches (silt_pair_balance): rows of each
+  // tile scaled by its XCD's weight when xcd_on; per-workgroup clocks
+  // recorded into d_clocks while calibrating.
+  float xcd_w[kXcds] = {1, 1, 1, 1, 1, 1, 1, 1};
+  bool xcd_on = false;
+  unsigned long long* d_clocks = nullptr;
+  bool clocks_on = false;
+  std::vector<unsigned long long> h_clocks;  // the last calibration step's records
   int last_nsub[2] = {0, 0};
   int last_fallback = 0;
   PlanLabeller lab;
@@ -225,7 +239,10 @@ int pair_launch(silt_pair* p, float dt, const StepPlan* plans) {
       stats_reset(f.stats, 1, st);
       flip[m - 1] = l;
     }
-    int rc = fused_substep_layers(launch, m, p->bed, res, p->path == SILT_PATH_FUSED_EXACT, st);
+    FusedBalance bal;
+    bal.xcd_w = p->xcd_on ? p->xcd_w : nullptr;
+    bal.clocks = p->clocks_on ? p->d_clocks : nullptr;
+    int rc = fused_substep_layers(launch, m, p->bed, res, p->path == SILT_PATH_FUSED_EXACT, st, &bal);
     if (rc != 0) return rc;
     // The output planes become current only once their launch is enqueued.
     for (int q = 0; q < m; q++) p->cur[flip[q]] = 1 - p->cur[flip[q]];
@@ -324,6 +341,7 @@ void silt_pair_destroy(silt_pair* p) {
       for (int k = 0; k < 3; k++) (void)hipFree(p->planes[l][b][k]);
   (void)hipFree(p->stats);
   (void)hipFree(p->totals);
+  if (p->d_clocks) (void)hipFree(p->d_clocks);
   if (p->totals_save) (void)hipFree(p->totals_save);
   if (p->stats_save) (void)hipFree(p->stats_save);
   if (p->ev_stats) (void)hipEventDestroy(p->ev_stats);
@@ -398,7 +416,11 @@ int silt_pair_step(silt_pair* p, float dt) {
   // a plan that differs undoes the launch (the current planes are untouched,
   // the ledger lines are restored) and launches the true plan.  The GPU
   // then does not wait for the host's planning between steps.
-  const bool spec = p->speculate && p->spec_valid && p->spec_dt == dt;
+  bool spec = p->speculate && p->spec_valid && p->spec_dt == dt;
+  if (spec && p->spec_pause > 0) {
+    p->spec_pause--;
+    spec = false;
+  }
   int cur0[2] = {p->cur[0], p->cur[1]};
   if (spec) {
     HIPCHK(hipMemcpyAsync(p->h_stats, p->stats, 2 * sizeof(SheetStats), hipMemcpyDeviceToHost, st));
@@ -424,6 +446,7 @@ int silt_pair_step(silt_pair* p, float dt) {
   if (spec) {
     if (same_plans(plans, p->spec_plans)) {
       p->spec_hits++;
+      p->spec_pause_next = 1;
       return std::max(p->last_nsub[0], p->last_nsub[1]);
     }
     // Undo: back to the planes the step started from, ledger lines and
@@ -433,6 +456,8 @@ int silt_pair_step(silt_pair* p, float dt) {
     HIPCHK(hipMemcpyAsync(p->totals, p->totals_save, 2 * sizeof(SheetTotals), hipMemcpyDeviceToDevice, st));
     HIPCHK(hipMemcpyAsync(p->stats, p->stats_save, 2 * sizeof(SheetStats), hipMemcpyDeviceToDevice, st));
     p->spec_misses++;
+    p->spec_pause = p->spec_pause_next;
+    p->spec_pause_next = std::min(64, 2 * p->spec_pause_next);
   }
   rc = pair_launch(p, dt, plans);
   if (rc < 0) return rc;
@@ -482,6 +507,8 @@ int silt_pair_set_speculation(silt_pair* p, int on) {
   if (!p) return SILT_EINVAL;
   p->speculate = on != 0;
   p->spec_valid = false;
+  p->spec_pause = 0;
+  p->spec_pause_next = 1;
   return SILT_OK;
 }
 
@@ -491,6 +518,65 @@ int silt_pair_speculation_counts(const silt_pair* p, long long* hits, long long*
   if (misses) *misses = p->spec_misses;
   return SILT_OK;
 }
+
+// The caller's state around tuning steps: the current planes, statistics,
+// ledger lines and speculation state, put back by restore().
+struct PairSnapshot {
+  silt_pair* p = nullptr;
+  float* save = nullptr;
+  int cur0[2] = {0, 0};
+  SheetStats stats0[2];
+  SheetTotals totals0[2];
+  long long hits0 = 0, misses0 = 0;
+  int pause0 = 0, pause_next0 = 1;
+
+  int take(silt_pair* q) {
+    p = q;
+    const size_t fb = p->n * sizeof(float);
+    if (hipMalloc(&save, 6 * fb) != hipSuccess) {
+      save = nullptr;
+      return SILT_ENOMEM;
+    }
+    hipStream_t st = p->stream;
+    cur0[0] = p->cur[0];
+    cur0[1] = p->cur[1];
+    refresh_stats(p);
+    hits0 = p->spec_hits;
+    misses0 = p->spec_misses;
+    pause0 = p->spec_pause;
+    pause_next0 = p->spec_pause_next;
+    for (int l = 0; l < 2; l++)
+      for (int k = 0; k < 3; k++)
+        HIPCHK(hipMemcpyAsync(save + (size_t)(3 * l + k) * p->n, p->planes[l][cur0[l]][k], fb,
+                              hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(stats0, p->stats, sizeof stats0, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(totals0, p->totals, sizeof totals0, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return 0;
+  }
+  int restore() {
+    const size_t fb = p->n * sizeof(float);
+    hipStream_t st = p->stream;
+    p->cur[0] = cur0[0];
+    p->cur[1] = cur0[1];
+    p->spec_valid = false;
+    p->spec_hits = hits0;
+    p->spec_misses = misses0;
+    p->spec_pause = pause0;
+    p->spec_pause_next = pause_next0;
+    for (int l = 0; l < 2; l++)
+      for (int k = 0; k < 3; k++)
+        HIPCHK(hipMemcpyAsync(p->planes[l][cur0[l]][k], save + (size_t)(3 * l + k) * p->n, fb,
+                              hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(p->stats, stats0, sizeof stats0, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(p->totals, totals0, sizeof totals0, hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return 0;
+  }
+  ~PairSnapshot() {
+    if (save) (void)hipFree(save);
+  }
+};
 
 int silt_pair_autotune(silt_pair* p, float dt, const int* rows, size_t n, int reps, int* best_rows, float* best_ms) {
   if (!p || !(dt > 0.0f) || reps < 1 || reps > 1000 || (n && !rows)) return SILT_EINVAL;
@@ -503,42 +589,11 @@ int silt_pair_autotune(silt_pair* p, float dt, const int* rows, size_t n, int re
     cand = {0, 48, 64, 96, 128, 160, 192, 256, 384, 512};
   }
   if (cand.empty()) return SILT_EINVAL;
-  const size_t fb = p->n * sizeof(float);
-  float* save = nullptr;
-  if (hipMalloc(&save, 6 * fb) != hipSuccess) return SILT_ENOMEM;
-  hipStream_t st = p->stream;
-  const int cur0[2] = {p->cur[0], p->cur[1]};
-  refresh_stats(p);
-  SheetStats stats0[2];
-  SheetTotals totals0[2];
-  // The tuning steps are not the caller's: their speculation counts and
-  // plans are dropped with their planes.
-  const long long hits0 = p->spec_hits, misses0 = p->spec_misses;
-  auto restore = [&]() -> int {
-    p->cur[0] = cur0[0];
-    p->cur[1] = cur0[1];
-    p->spec_valid = false;
-    p->spec_hits = hits0;
-    p->spec_misses = misses0;
-    for (int l = 0; l < 2; l++)
-      for (int k = 0; k < 3; k++)
-        HIPCHK(hipMemcpyAsync(p->planes[l][cur0[l]][k], save + (size_t)(3 * l + k) * p->n, fb,
-                              hipMemcpyDeviceToDevice, st));
-    HIPCHK(hipMemcpyAsync(p->stats, stats0, sizeof stats0, hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(p->totals, totals0, sizeof totals0, hipMemcpyHostToDevice, st));
-    HIPCHK(hipStreamSynchronize(st));
-    return 0;
-  };
-  int rc = 0;
-  for (int l = 0; l < 2 && rc == 0; l++)
-    for (int k = 0; k < 3; k++)
-      if (hipMemcpyAsync(save + (size_t)(3 * l + k) * p->n, p->planes[l][cur0[l]][k], fb, hipMemcpyDeviceToDevice,
-                         st) != hipSuccess)
-        rc = SILT_EDEVICE;
-  if (rc == 0 && (hipMemcpyAsync(stats0, p->stats, sizeof stats0, hipMemcpyDeviceToHost, st) != hipSuccess ||
-                  hipMemcpyAsync(totals0, p->totals, sizeof totals0, hipMemcpyDeviceToHost, st) != hipSuccess ||
-                  hipStreamSynchronize(st) != hipSuccess))
-    rc = SILT_EDEVICE;
+  // The tuning steps are not the caller's: their planes, speculation counts
+  // and plans are dropped.
+  PairSnapshot snap;
+  int rc = snap.take(p);
+  if (rc != 0) return rc;
   // Heights are tuned for the joint launch of both layers.
   const int pinned0 = fused_pinned_chunk_rows(p->res, 2);
   float best = INFINITY;
@@ -551,7 +606,7 @@ int silt_pair_autotune(silt_pair* p, float dt, const int* rows, size_t n, int re
       rc = silt_pair_step(p, dt);
       sum += silt_pair_last_kernel_ms(p);
     }
-    int rr = restore();
+    int rr = snap.restore();
     if (rc >= 0) rc = rr;
     if (rc < 0) break;
     rc = 0;
@@ -562,10 +617,174 @@ int silt_pair_autotune(silt_pair* p, float dt, const int* rows, size_t n, int re
     }
   }
   fused_set_chunk_rows_for(p->res, rc == 0 ? best_c : pinned0, 2);
-  (void)hipFree(save);
   if (rc < 0) return rc;
   if (best_rows) *best_rows = best_c;
   if (best_ms) *best_ms = best;
+  return SILT_OK;
+}
+
+int silt_pair_set_xcd_weights(silt_pair* p, const float* w) {
+  if (!p) return SILT_EINVAL;
+  if (!w) {
+    p->xcd_on = false;
+    for (int x = 0; x < kXcds; x++) p->xcd_w[x] = 1.0f;
+    return SILT_OK;
+  }
+  for (int x = 0; x < kXcds; x++)
+    if (!(w[x] >= 0.25f && w[x] <= 4.0f)) return SILT_EINVAL;
+  for (int x = 0; x < kXcds; x++) p->xcd_w[x] = w[x];
+  p->xcd_on = true;
+  return SILT_OK;
+}
+
+namespace {
+
+// One calibration round: reps steps with the current weights, each
+// recording its workgroups' clocks.  Per XCD: when its last workgroup ended
+// (microseconds after the launch's first start: the XCD's finish), the
+// mean pipeline rows of its tiles and the mean tile time; and the mean
+// kernel time of the steps.
+struct XcdRound {
+  double finish[kXcds] = {0}, rows[kXcds] = {0}, tile_us[kXcds] = {0};
+  double kernel_ms = 0.0;
+};
+
+int xcd_round(silt_pair* p, float dt, int reps, size_t nrec, XcdRound* out) {
+  std::vector<unsigned long long>& rec = p->h_clocks;
+  rec.assign(4 * nrec, 0ull);
+  XcdRound r;
+  double cnt[kXcds] = {0};
+  p->clocks_on = true;
+  int rc = silt_pair_step(p, dt);  // first launch of these weights, not counted
+  for (int k = 0; k < reps && rc >= 0; k++) {
+    if (hipMemsetAsync(p->d_clocks, 0, rec.size() * sizeof(unsigned long long), p->stream) != hipSuccess)
+      rc = SILT_EDEVICE;
+    if (rc >= 0) rc = silt_pair_step(p, dt);
+    if (rc >= 0) r.kernel_ms += silt_pair_last_kernel_ms(p) / reps;
+    if (rc >= 0 && (hipMemcpyAsync(rec.data(), p->d_clocks, rec.size() * sizeof(unsigned long long),
+                                   hipMemcpyDeviceToHost, p->stream) != hipSuccess ||
+                    hipStreamSynchronize(p->stream) != hipSuccess))
+      rc = SILT_EDEVICE;
+    if (rc < 0) break;
+    unsigned long long t0 = ~0ull;
+    for (size_t g = 0; g < nrec; g++)
+      if (rec[4 * g + 1] > rec[4 * g]) t0 = std::min(t0, rec[4 * g]);
+    double fin[kXcds] = {0};
+    // Workgroup g ran on XCD g mod 8 (round-robin dispatch).
+    for (size_t g = 0; g < nrec; g++) {
+      const unsigned long long* c = &rec[4 * g];
+      if (c[1] <= c[0]) continue;
+      const int x = (int)(g % kXcds);
+      fin[x] = std::max(fin[x], (double)(c[1] - t0) * 0.01);  // 100 MHz ticks -> us
+      r.tile_us[x] += (double)(c[1] - c[0]) * 0.01;
+      r.rows[x] += (double)(c[2] & 0xffffffffull);
+      cnt[x] += 1.0;
+    }
+    for (int x = 0; x < kXcds; x++) r.finish[x] += fin[x] / reps;
+  }
+  p->clocks_on = false;
+  if (rc < 0) return rc;
+  for (int x = 0; x < kXcds; x++) {
+    if (cnt[x] == 0.0 || r.finish[x] <= 0.0) return SILT_EUNSUPPORTED;
+    r.tile_us[x] /= cnt[x];
+    r.rows[x] /= cnt[x];
+  }
+  *out = r;
+  return 0;
+}
+
+}  // namespace
+
+size_t silt_pair_clock_records(const silt_pair* p, unsigned long long* out, size_t n) {
+  if (!p) return 0;
+  const size_t have = p->h_clocks.size() / 4;
+  if (out) memcpy(out, p->h_clocks.data(), std::min(n, have) * 4 * sizeof(unsigned long long));
+  return have;
+}
+
+int silt_pair_balance(silt_pair* p, float dt, int reps, float* weights, float* ms, float* tile_us) {
+  if (!p || !(dt > 0.0f) || reps < 1 || reps > 1000) return SILT_EINVAL;
+  if (p->path == SILT_PATH_EXACT) return SILT_EUNSUPPORTED;
+  // Record room for the most tiles a launch of this side can have (two
+  // layers, chunks of at least 8 rows).
+  const int strips = (p->res + 234) / 235;
+  const size_t nrec = (size_t)2 * strips * ((p->res + 7) / 8);
+  if (!p->d_clocks && hipMalloc(&p->d_clocks, 4 * nrec * sizeof(unsigned long long)) != hipSuccess) {
+    p->d_clocks = nullptr;
+    return SILT_ENOMEM;
+  }
+  PairSnapshot snap;
+  int rc = snap.take(p);
+  if (rc != 0) return rc;
+  const bool on0 = p->xcd_on;
+  float w0[kXcds];
+  memcpy(w0, p->xcd_w, sizeof w0);
+  // Round 0 runs uniform chunks.  Each later round moves rows toward the
+  // XCDs that finished early: with u[x] the mean pipeline rows of XCD x's
+  // tiles (output rows n[x] plus the fill F around them) and f[x] when its
+  // last workgroup ended, u'[x] is proportional to u[x] / f[x] with the
+  // total rows kept, and the weights are n'[x] = u'[x] - F relative to
+  // their mean (an XCD's rate moves with its rows, so this is a search,
+  // not a solve).  Every
+  // round is scored by the launch's mean kernel time and the fastest
+  // weights are kept, uniform chunks included.
+  const double fill = 2.0 * 15.0;
+  float w[kXcds] = {1, 1, 1, 1, 1, 1, 1, 1}, best_w[kXcds] = {1, 1, 1, 1, 1, 1, 1, 1};
+  double best_ms = INFINITY, t_uni = 0.0;
+  XcdRound first, last;
+  p->xcd_on = true;
+  const int rounds = 8;
+  for (int round = 0; round < rounds && rc >= 0; round++) {
+    memcpy(p->xcd_w, w, sizeof w);
+    XcdRound r;
+    rc = xcd_round(p, dt, reps, nrec, &r);
+    if (rc < 0) break;
+    rc = snap.restore();
+    if (rc < 0) break;
+    if (round == 0) {
+      first = r;
+      t_uni = r.kernel_ms;
+    }
+    last = r;
+    if (r.kernel_ms < best_ms) {
+      best_ms = r.kernel_ms;
+      memcpy(best_w, w, sizeof w);
+    }
+    double rows = 0.0, scaled = 0.0, u[kXcds];
+    for (int x = 0; x < kXcds; x++) {
+      u[x] = r.rows[x] / r.finish[x];
+      rows += r.rows[x];
+      scaled += u[x];
+    }
+    double n[kXcds], nsum = 0.0;
+    for (int x = 0; x < kXcds; x++) {
+      n[x] = std::max(8.0, u[x] * rows / scaled - fill);
+      nsum += n[x];
+    }
+    double wsum = 0.0;
+    for (int x = 0; x < kXcds; x++) {
+      w[x] = (float)std::min(2.5, std::max(0.4, n[x] * kXcds / nsum));
+      wsum += w[x];
+    }
+    for (int x = 0; x < kXcds; x++) w[x] = (float)(w[x] * kXcds / wsum);
+  }
+  if (rc < 0) {
+    p->xcd_on = on0;
+    memcpy(p->xcd_w, w0, sizeof w0);
+    return rc;
+  }
+  const bool keep = std::any_of(best_w, best_w + kXcds, [](float v) { return v != 1.0f; });
+  p->xcd_on = keep;
+  memcpy(p->xcd_w, best_w, sizeof best_w);
+  if (weights) memcpy(weights, p->xcd_w, sizeof p->xcd_w);
+  if (ms) {
+    ms[0] = (float)t_uni;
+    ms[1] = (float)best_ms;
+  }
+  if (tile_us) {
+    for (int x = 0; x < kXcds; x++) tile_us[x] = (float)first.finish[x];
+    for (int x = 0; x < kXcds; x++) tile_us[kXcds + x] = (float)last.finish[x];
+  }
   return SILT_OK;
 }
 

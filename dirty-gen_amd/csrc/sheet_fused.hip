@@ -69,6 +69,9 @@
 
 #include <math.h>
 
+#include <algorithm>
+#include <cmath>
+
 #include <map>
 #include <string.h>
 #include <mutex>
@@ -147,6 +150,12 @@ using ConstInt = const __attribute__((address_space(4))) int;
 struct FusedArgs {
   ConstLayer* table;
   ConstInt* tile_window;
+  // Output rows [y0, y1) of every tile (two ints per tile), or null for
+  // chunks of the window's uniform height.
+  ConstInt* rows;
+  // Per-workgroup record for the XCD balance (start and end of the real-time
+  // clock, rows the pipeline ran | XCC_ID << 32, tile | HW_ID << 32), or null.
+  unsigned long long* clocks;
   int nwin;
 };
 
@@ -211,8 +220,11 @@ __global__ __launch_bounds__(kLanes, 3) void k_sheet_stream(FusedArgs fa) {
   const int strip = tile % L.strips;
   const int chunk = tile / L.strips;
   const int x0 = strip * kOutW;              // first output column
-  const int y0 = chunk * L.chunk;            // first output row
-  const int y1 = min(res, y0 + L.chunk);     // one past the last output row
+  // First output row and one past the last: uniform chunks, or the rows a
+  // table gives each tile (chunk heights sized to each XCD's speed).
+  const int y0 = fa.rows ? fa.rows[2 * tile_lin] : chunk * L.chunk;
+  const int y1 = fa.rows ? fa.rows[2 * tile_lin + 1] : min(res, y0 + L.chunk);
+  const unsigned long long clock0 = fa.clocks ? __builtin_amdgcn_s_memrealtime() : 0ull;
   const int t = (int)threadIdx.x;
   const int col = x0 - kHaloL + t;           // this lane's grid column
   const bool col_in = col >= 0 && col < res;
@@ -990,6 +1002,19 @@ __global__ __launch_bounds__(kLanes, 3) void k_sheet_stream(FusedArgs fa) {
       }
     }
   }
+  if (fa.clocks) {
+    __syncthreads();
+    if (t == 0) {
+      unsigned long long* c = fa.clocks + 4 * wg;
+      c[0] = clock0;
+      c[1] = __builtin_amdgcn_s_memrealtime();
+      // HW_ID (wave, SIMD, CU, shader engine) and XCC_ID hardware registers.
+      const unsigned hw_id = (unsigned)__builtin_amdgcn_s_getreg((31 << 11) | 4);
+      const unsigned xcc_id = (unsigned)__builtin_amdgcn_s_getreg((3 << 11) | 20);
+      c[2] = (unsigned long long)(reached ? r_end - r_begin + 1 : y1 - y0) | ((unsigned long long)xcc_id << 32);
+      c[3] = (unsigned long long)tile_lin | ((unsigned long long)hw_id << 32);
+    }
+  }
 }
 
 namespace {
@@ -1109,13 +1134,14 @@ namespace {
 struct DescBuf {
   LayerArgs* table = nullptr;
   int* tile_window = nullptr;
-  size_t cap_win = 0, cap_tiles = 0;
+  int* rows = nullptr;
+  size_t cap_win = 0, cap_tiles = 0, cap_rows = 0;
 };
 std::mutex g_desc_mu;
 std::map<hipStream_t, DescBuf> g_desc;
 
-int upload_descs(const std::vector<LayerArgs>& wins, const std::vector<int>& tile_window, hipStream_t st,
-                 FusedArgs* fa) {
+int upload_descs(const std::vector<LayerArgs>& wins, const std::vector<int>& tile_window,
+                 const std::vector<int>& rows, hipStream_t st, FusedArgs* fa) {
   std::lock_guard<std::mutex> lock(g_desc_mu);
   DescBuf& b = g_desc[st];
   if (b.cap_win < wins.size()) {
@@ -1132,16 +1158,56 @@ int upload_descs(const std::vector<LayerArgs>& wins, const std::vector<int>& til
     if (hipMalloc(&b.tile_window, tile_window.size() * sizeof(int)) != hipSuccess) return SILT_ENOMEM;
     b.cap_tiles = tile_window.size();
   }
+  if (b.cap_rows < rows.size()) {
+    if (b.rows) (void)hipFree(b.rows);
+    b.rows = nullptr;
+    b.cap_rows = 0;
+    if (hipMalloc(&b.rows, rows.size() * sizeof(int)) != hipSuccess) return SILT_ENOMEM;
+    b.cap_rows = rows.size();
+  }
   // Pageable sources: the copies are staged before the calls return.
   if (hipMemcpyAsync(b.table, wins.data(), wins.size() * sizeof(LayerArgs), hipMemcpyHostToDevice, st) != hipSuccess)
     return SILT_EDEVICE;
   if (!tile_window.empty() && hipMemcpyAsync(b.tile_window, tile_window.data(), tile_window.size() * sizeof(int),
                                              hipMemcpyHostToDevice, st) != hipSuccess)
     return SILT_EDEVICE;
+  if (!rows.empty() &&
+      hipMemcpyAsync(b.rows, rows.data(), rows.size() * sizeof(int), hipMemcpyHostToDevice, st) != hipSuccess)
+    return SILT_EDEVICE;
   fa->table = (ConstLayer*)b.table;
   fa->tile_window = tile_window.empty() ? nullptr : (ConstInt*)b.tile_window;
+  fa->rows = rows.empty() ? nullptr : (ConstInt*)b.rows;
   fa->nwin = (int)wins.size();
   return 0;
+}
+
+// Rows of every tile with each strip's chunks sized by the weight of the XCD
+// that runs them: the boundaries are the strip's cumulative weights scaled
+// to the window's side and rounded.  Empty when a tile would get no row.
+std::vector<int> balanced_rows(const std::vector<LayerArgs>& wins, int tiles, const float* w) {
+  std::vector<int> rows((size_t)2 * tiles);
+  std::vector<double> cum;
+  for (const LayerArgs& L : wins) {
+    const int strips = L.hot.strips, chunks = (L.hot.res + L.hot.chunk - 1) / L.hot.chunk;
+    cum.assign((size_t)chunks + 1, 0.0);
+    for (int s = 0; s < strips; s++) {
+      for (int c = 0; c < chunks; c++) {
+        const int tl = L.hot.tile0 + c * strips + s;
+        cum[(size_t)c + 1] = cum[(size_t)c] + (double)w[fused_tile_xcd(tl, tiles)];
+      }
+      int prev = 0;
+      for (int c = 0; c < chunks; c++) {
+        const int tl = L.hot.tile0 + c * strips + s;
+        const int y1 = c + 1 == chunks ? L.hot.res
+                                       : (int)std::lround((double)L.hot.res * cum[(size_t)c + 1] / cum[(size_t)chunks]);
+        if (y1 <= prev) return {};
+        rows[(size_t)2 * tl] = prev;
+        rows[(size_t)2 * tl + 1] = y1;
+        prev = y1;
+      }
+    }
+  }
+  return rows;
 }
 
 template <bool EXACT, bool FULL>
@@ -1174,9 +1240,17 @@ static int dispatch_any(const FusedArgs* fa, int tiles, bool exact, bool full, b
 
 size_t fused_desc_size() { return sizeof(LayerArgs); }
 
+int fused_tile_xcd(int tile, int tiles) {
+  // Inverse of the kernel's order: XCD x runs tiles [x q + min(x, rem), ...).
+  const int q = tiles / kXcds, rem = tiles % kXcds;
+  for (int x = kXcds - 1; x > 0; x--)
+    if (tile >= x * q + std::min(x, rem)) return x;
+  return 0;
+}
+
 bool fused_boxes_fit(const std::vector<Box>& boxes) { return boxes_fit(boxes); }
 
-int fused_build(const FusedLayer* wins, int n, bool exact, int rows, FusedLaunch* out) {
+int fused_build(const FusedLayer* wins, int n, bool exact, int rows, FusedLaunch* out, const float* xcd_w) {
   if (n < 1 || !out) return SILT_EINVAL;
   bool project = wins[0].a.project != 0, wind = wins[0].a.wind != 0;
   bool full = true;
@@ -1231,33 +1305,46 @@ int fused_build(const FusedLayer* wins, int n, bool exact, int rows, FusedLaunch
     memcpy(out->descs.data() + (size_t)k * sizeof(LayerArgs), &L, sizeof(LayerArgs));
   }
   out->tiles = tiles;
+  out->rows.clear();
+  if (xcd_w) {
+    for (int x = 0; x < kXcds; x++)
+      if (!(xcd_w[x] >= 0.25f && xcd_w[x] <= 4.0f)) return SILT_EINVAL;
+    std::vector<LayerArgs> descs((size_t)n);
+    memcpy(descs.data(), out->descs.data(), out->descs.size());
+    out->rows = balanced_rows(descs, tiles, xcd_w);
+  }
   return 0;
 }
 
-int fused_enqueue(const FusedLaunch& l, const void* dev_table, const int* dev_tile_window, hipStream_t st) {
+int fused_enqueue(const FusedLaunch& l, const void* dev_table, const int* dev_tile_window, hipStream_t st,
+                  const int* dev_rows, unsigned long long* clocks) {
   if (!dev_table || (l.nwin > 2 && !dev_tile_window) || l.tiles < 1) return SILT_EINVAL;
   FusedArgs fa{};
   fa.table = (ConstLayer*)dev_table;
   fa.tile_window = l.nwin > 2 ? (ConstInt*)dev_tile_window : nullptr;
+  fa.rows = (ConstInt*)dev_rows;
+  fa.clocks = clocks;
   fa.nwin = l.nwin;
   dispatch_any(&fa, l.tiles, l.exact, l.full, l.project, l.wind, st);
   return hipGetLastError() == hipSuccess ? 0 : SILT_EDEVICE;
 }
 
-int fused_substep_windows(const FusedLayer* wins, int n, bool exact, int rows, hipStream_t st) {
+int fused_substep_windows(const FusedLayer* wins, int n, bool exact, int rows, hipStream_t st,
+                          const FusedBalance* bal) {
   FusedLaunch l;
-  int rc = fused_build(wins, n, exact, rows, &l);
+  int rc = fused_build(wins, n, exact, rows, &l, bal ? bal->xcd_w : nullptr);
   if (rc != 0) return rc;
   std::vector<LayerArgs> descs((size_t)n);
   memcpy(descs.data(), l.descs.data(), l.descs.size());
   FusedArgs fa{};
-  rc = upload_descs(descs, l.tile_window, st, &fa);
+  rc = upload_descs(descs, l.tile_window, l.rows, st, &fa);
   if (rc != 0) return rc;
-  return fused_enqueue(l, (const void*)fa.table, (const int*)fa.tile_window, st);
+  return fused_enqueue(l, (const void*)fa.table, (const int*)fa.tile_window, st, (const int*)fa.rows,
+                       bal ? bal->clocks : nullptr);
 }
 
 int fused_substep_layers(const FusedLayer* layers, int nlayers, const float* bed, int res, bool exact,
-                         hipStream_t st) {
+                         hipStream_t st, const FusedBalance* bal) {
   if (nlayers < 1 || nlayers > 2) return SILT_EINVAL;
   FusedLayer w[2];
   for (int k = 0; k < nlayers; k++) {
@@ -1265,7 +1352,7 @@ int fused_substep_layers(const FusedLayer* layers, int nlayers, const float* bed
     w[k].in.bed = bed;
     w[k].res = res;
   }
-  return fused_substep_windows(w, nlayers, exact, 0, st);
+  return fused_substep_windows(w, nlayers, exact, 0, st, bal);
 }
 
 // One planned step of one layer: nsub fused substeps ping-ponging between

@@ -115,14 +115,30 @@ int plan_fine_device(const float* h, const float* u, const float* v, const float
                      float eps, float sub_cap, int pace, FineScratch& fs, StepPlan* plan, hipStream_t stream);
 void fine_scratch_free(FineScratch& fs);
 
+// XCD balance of a launch: the 8 XCDs take workgroups round robin and each
+// runs a contiguous run of tiles; weights[x] scales the rows of the tiles XCD
+// x runs (every strip's rows are split in proportion to the weights of the
+// XCDs that run its chunks), so a slower XCD gets shorter tiles.  clocks, when
+// set, receives 4 values per workgroup: real-time clock at start and end
+// (100 MHz), the rows its pipeline ran (XCC_ID register in the high word), and
+// its tile (HW_ID register in the high word).
+struct FusedBalance {
+  const float* xcd_w = nullptr;  // [8], or null for uniform chunks
+  unsigned long long* clocks = nullptr;
+};
+constexpr int kXcds = 8;
+// The XCD that runs linear tile `tile` of a launch of `tiles` workgroups.
+int fused_tile_xcd(int tile, int tiles);
+
 // One substep of one or two layers (same res, drive kind and projection) in
 // a single launch.  exact selects the reference-order arithmetic.
 int fused_substep_layers(const FusedLayer* layers, int nlayers, const float* bed, int res, bool exact,
-                         hipStream_t st);
+                         hipStream_t st, const FusedBalance* bal = nullptr);
 // One substep of n independent windows (each with its own side, bed and
 // planes; one drive kind and projection) in a single launch.  rows > 0 pins
 // the rows per workgroup.
-int fused_substep_windows(const FusedLayer* wins, int n, bool exact, int rows, hipStream_t st);
+int fused_substep_windows(const FusedLayer* wins, int n, bool exact, int rows, hipStream_t st,
+                          const FusedBalance* bal = nullptr);
 
 // A prepared launch over n windows: the packed window descriptors (device
 // layout, fused_desc_size() bytes each) and, for more than two windows, the
@@ -133,14 +149,17 @@ struct FusedLaunch {
   int chunk = 0, nwin = 0, tiles = 0;
   std::vector<unsigned char> descs;
   std::vector<int> tile_window;
+  std::vector<int> rows;  // [2 tiles] output rows of every tile, or empty (uniform chunks)
 };
 size_t fused_desc_size();
 // Whether the fused kernel covers a box list: 1 to 8 boxes that neither
 // overlap nor touch (so two cells in boxes are in one box exactly when a
 // face between them exists).
 bool fused_boxes_fit(const std::vector<Box>& boxes);
-int fused_build(const FusedLayer* wins, int n, bool exact, int rows, FusedLaunch* out);
-int fused_enqueue(const FusedLaunch& l, const void* dev_table, const int* dev_tile_window, hipStream_t st);
+int fused_build(const FusedLayer* wins, int n, bool exact, int rows, FusedLaunch* out,
+                const float* xcd_w = nullptr);
+int fused_enqueue(const FusedLaunch& l, const void* dev_table, const int* dev_tile_window, hipStream_t st,
+                  const int* dev_rows = nullptr, unsigned long long* clocks = nullptr);
 
 // One planned step of one layer on the fused path.  Returns the substep count,
 // SILT_EUNSUPPORTED for plans the fused kernel does not cover (pace-3 slices,

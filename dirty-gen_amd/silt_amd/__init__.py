@@ -120,6 +120,11 @@ def _declare(L):
     L.silt_pair_autotune.argtypes = [_P, _F, C.POINTER(C.c_int), _Z, C.c_int, C.POINTER(C.c_int),
                                      C.POINTER(C.c_float)]
     L.silt_pair_autotune.restype = C.c_int
+    L.silt_pair_set_xcd_weights.argtypes = [_P, C.POINTER(C.c_float)]
+    L.silt_pair_balance.argtypes = [_P, _F, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_float),
+                                    C.POINTER(C.c_float)]
+    L.silt_pair_clock_records.argtypes = [_P, C.POINTER(C.c_ulonglong), _Z]
+    L.silt_pair_clock_records.restype = _Z
     L.silt_pair_set_speculation.argtypes = [_P, C.c_int]
     L.silt_pair_speculation_counts.argtypes = [_P, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
     L.silt_batch_create.argtypes = [_Z, C.POINTER(C.c_size_t), C.POINTER(C.c_float), C.c_int]
@@ -464,6 +469,31 @@ class SheetPair:
         _check_status(lib().silt_pair_autotune(self._h, dt, arr, len(rows) if rows else 0, reps, C.byref(best),
                                                C.byref(ms)))
         return best.value, ms.value
+
+    def set_xcd_weights(self, weights):
+        """Rows of the fused launch's tiles scaled per XCD (8 weights in
+        [0.25, 4]); None restores uniform chunks.  Results do not change."""
+        arr = None if weights is None else (C.c_float * 8)(*[float(w) for w in weights])
+        _check_status(lib().silt_pair_set_xcd_weights(self._h, arr))
+
+    def balance(self, dt, reps=5):
+        """Calibrate the XCD weights on this chip from the workgroups' clock
+        records (state restored afterwards).  Returns (weights, (uniform ms,
+        kept ms), when each XCD finished with uniform chunks (us after the
+        launch's start), the same in the last round); the weights are all 1
+        when no round was faster than uniform chunks."""
+        w, ms, us = (C.c_float * 8)(), (C.c_float * 2)(), (C.c_float * 16)()
+        _check_status(lib().silt_pair_balance(self._h, dt, reps, w, ms, us))
+        return list(w), (ms[0], ms[1]), list(us[:8]), list(us[8:])
+
+    def clock_records(self):
+        """The last calibration step's per-workgroup records as a (n, 4)
+        uint64 array: start and end (100 MHz real-time clock), pipeline rows
+        | XCC_ID << 32, tile | HW_ID << 32; unused rows are zero."""
+        n = lib().silt_pair_clock_records(self._h, None, 0)
+        out = np.zeros((n, 4), np.uint64)
+        lib().silt_pair_clock_records(self._h, out.ctypes.data_as(C.POINTER(C.c_ulonglong)), n)
+        return out
 
 
 class TirePush(C.Structure):

@@ -61,7 +61,9 @@ int silt_pair_download(silt_pair *p, float *water, float *uw, float *vw, float *
  * launched before this step's plan is confirmed from the statistics; a
  * different plan undoes it (the starting planes are untouched, the ledger
  * lines restored) and launches the right one, so results do not depend on
- * it. */
+ * it.  After an undone speculation the next steps plan before they launch,
+ * for 1, 2, 4 ... at most 64 steps as misses repeat; one that holds resets
+ * the pause. */
 int silt_pair_step(silt_pair *p, float dt);
 
 int silt_pair_ledger(silt_pair *p, silt_pair_ledger_t *out);
@@ -91,6 +93,35 @@ int silt_pair_last_fallback(const silt_pair *p);
  * path, run once before a long simulation. */
 int silt_pair_autotune(silt_pair *p, float dt, const int *rows, size_t n, int reps, int *best_rows,
                        float *best_ms);
+
+/* XCD balance of the fused launch.  The 8 XCDs take the launch's workgroups
+ * round robin, and on one chip the same work takes longer on some XCDs than
+ * on others, so the launch waits for the slowest.  weights[x] scales the
+ * rows of the tiles XCD x runs (each strip's rows are split in proportion to
+ * the weights of the XCDs that run its chunks); NULL restores uniform
+ * chunks.  Weights in [0.25, 4].  Results do not depend on the split.
+ *
+ * silt_pair_balance calibrates them on this chip.  Eight rounds of reps steps
+ * record every workgroup's start and end on the real-time clock: round 0
+ * with uniform chunks, each later one moving rows toward the XCDs whose last
+ * workgroup ended early (the oldest workgroups on a CU are issued first, so
+ * one XCD's tiles end at very different times: what counts is when the XCD
+ * is done).  Each round is scored by the launch's mean kernel time and the
+ * fastest weights are kept, uniform chunks included (ms[0] uniform, ms[1]
+ * kept).  tile_us[0..7] is when each XCD finished with uniform chunks
+ * (microseconds after the launch's first workgroup started), tile_us[8..15]
+ * in the last round.
+ * The steps are not the caller's: planes, ledger lines and speculation state
+ * are restored.  weights, ms and tile_us may be NULL.  No counterpart in the
+ * reference: a tuning step of the GPU path. */
+int silt_pair_set_xcd_weights(silt_pair *p, const float *weights);
+int silt_pair_balance(silt_pair *p, float dt, int reps, float *weights, float *ms, float *tile_us);
+/* The per-workgroup records of the last calibration step (4 values each:
+ * real-time clock at start and end at 100 MHz; pipeline rows | XCC_ID << 32;
+ * tile | HW_ID << 32), at most n into out (may be NULL).  Returns how many
+ * there are (workgroups of the largest launch the handle can make; unused
+ * ones are zero). */
+size_t silt_pair_clock_records(const silt_pair *p, unsigned long long *out, size_t n);
 
 /* Tires on the sheet.  No counterpart in the reference, whose tires act on
  * the 40 m FluidCore window only (FluidCore::push_tire and the rut stamp,

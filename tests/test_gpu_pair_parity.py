@@ -117,6 +117,55 @@ def test_benched_launch_at_4096_matches_fp64_per_layer(large):
         p.close()
 
 
+def test_xcd_weighted_rows_step_as_uniform_chunks(large):
+    """Tiles whose rows are scaled per XCD (silt_pair_set_xcd_weights) give
+    the same planes bit for bit as uniform chunks: a cell's arithmetic does
+    not depend on which tile computes it.  Ledger lines agree to FP64
+    summation (their per-tile partial sums move with the split)."""
+    w = [0.6, 1.4, 0.8, 1.2, 1.0, 0.7, 1.3, 1.0]
+    a, b = _pair(large), _pair(large)
+    try:
+        b.set_xcd_weights(w)
+        for _ in range(2):
+            assert a.step(DT) == b.step(DT) == 1
+        for x, y in zip(a.download(), b.download()):
+            assert np.array_equal(x, y)
+        la, lb = a.ledger(), b.ledger()
+        for p_, q_ in ((la.water, lb.water), (la.slurry, lb.slurry)):
+            assert abs(p_.volume_m3 - q_.volume_m3) <= 1e-12 * p_.initial_m3
+            assert abs(p_.residue_m3 - q_.residue_m3) <= 1e-9 * p_.initial_m3
+    finally:
+        a.close()
+        b.close()
+
+
+def test_xcd_balance_restores_state_and_keeps_results():
+    """silt_pair_balance runs calibration steps, restores the caller's planes
+    and ledger, and leaves weights under which the steps equal uniform
+    chunks bit for bit."""
+    sc = scenes.large_scene(res=1000, seed=3, puddles=12)
+    a, b = _pair(sc), _pair(sc)
+    try:
+        before = a.download()
+        led0 = a.ledger()
+        w, (t_uni, t_bal), us0, us1 = a.balance(DT, reps=2)
+        for x, y in zip(before, a.download()):
+            assert np.array_equal(x, y)
+        led1 = a.ledger()
+        assert led1.water.volume_m3 == led0.water.volume_m3 and led1.water.residue_m3 == led0.water.residue_m3
+        assert t_uni > 0.0 and t_bal > 0.0
+        assert all(u > 0.0 for u in us0) and all(u > 0.0 for u in us1)
+        assert all(0.25 <= x <= 4.0 for x in w)
+        assert a.speculation_counts() == (0, 0)
+        for _ in range(2):
+            assert a.step(DT) == b.step(DT) == 1
+        for x, y in zip(a.download(), b.download()):
+            assert np.array_equal(x, y)
+    finally:
+        a.close()
+        b.close()
+
+
 def test_two_layer_launch_at_1000_matches_fp64_per_layer():
     """A side that divides neither the strip width (235) nor the row chunk:
     partial strips and chunks on both layers of one launch."""

@@ -75,6 +75,10 @@ def test_speculation_with_tires_and_changing_substeps():
             for x, y in zip(pairs[0].download(), pairs[1].download()):
                 assert np.array_equal(x, y), k
         assert pairs[0].ledger().water.gap_m3 == pairs[1].ledger().water.gap_m3
+        # Every undone speculation pauses speculation for at least the next
+        # step (the first step has no plan to speculate with).
+        hits, misses = pairs[0].speculation_counts()
+        assert hits + 2 * misses <= 39, (hits, misses)
     finally:
         for p in pairs:
             p.close()
