@@ -418,6 +418,14 @@ def run_large(args):
     pair.upload(sc["bed"], sc["water"], z, z, sc["slurry"], z, z, sc["nu_slurry"])
     pair.set_ground(sc["bed"], origin)
     pair.set_surrogate(params)
+    balance = None
+    if not args.no_balance and args.path != "exact":
+        # XCD balance of the fused launches, calibrated on plain steps from
+        # the starting state (restored afterwards).
+        w, (t_uni, t_bal), _, _ = pair.balance(dt)
+        balance = {"kept": any(x != 1.0 for x in w), "weights": [round(x, 4) for x in w],
+                   "kernel_ms_uniform": t_uni, "kernel_ms_balanced": t_bal}
+        print(f"[bench] XCD balance: uniform {t_uni:.4f} ms, balanced {t_bal:.4f} ms", file=sys.stderr, flush=True)
     fleet = D.Fleet(seed=7, n=16, extent=res * cell, half=25.0)
     for _ in range(args.warmup):
         pair.step_tires(dt, fleet.pushes(), 4, fleet.contacts())
@@ -461,6 +469,7 @@ def run_large(args):
         "speculation": dict(zip(("held", "undone"), pair.speculation_counts())),
         "ledger_gap_rel": {"max": float(np.max(gaps)), "last": float(gaps[-1])},
         "training": info,
+        "xcd_balance": balance,
     }
     print(json.dumps(out), flush=True)
 
