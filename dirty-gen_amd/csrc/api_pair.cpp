@@ -215,8 +215,11 @@ int pair_launch(silt_pair* p, float dt, const StepPlan* plans) {
     stats_compute(cur.h, cur.u, cur.v, res, kEps, p->stats + l, st);
     staged = true;
   }
-  // Time the fused launches alone when they are the step's only kernel work.
-  if (!staged && nfused) HIPCHK(hipEventRecord(p->ev0, st));
+  // Time the fused launches alone when they are the step's only kernel work:
+  // one launch carries the events in its own dispatch (they time the kernel
+  // itself), several are bracketed by event records.
+  const bool one_launch = !staged && nfused && most == 1;
+  if (!staged && nfused && !one_launch) HIPCHK(hipEventRecord(p->ev0, st));
   for (int k = 0; k < most; k++) {
     FusedLayer launch[2];
     int flip[2], m = 0;
@@ -242,12 +245,16 @@ int pair_launch(silt_pair* p, float dt, const StepPlan* plans) {
     FusedBalance bal;
     bal.xcd_w = p->xcd_on ? p->xcd_w : nullptr;
     bal.clocks = p->clocks_on ? p->d_clocks : nullptr;
+    if (one_launch) {
+      bal.ev_start = p->ev0;
+      bal.ev_stop = p->ev1;
+    }
     int rc = fused_substep_layers(launch, m, p->bed, res, p->path == SILT_PATH_FUSED_EXACT, st, &bal);
     if (rc != 0) return rc;
     // The output planes become current only once their launch is enqueued.
     for (int q = 0; q < m; q++) p->cur[flip[q]] = 1 - p->cur[flip[q]];
   }
-  HIPCHK(hipEventRecord(p->ev1, st));
+  if (!one_launch) HIPCHK(hipEventRecord(p->ev1, st));
   p->kernel_ms_pending = true;
   // Staged layers use the process-wide workspace of this side length, which
   // the next call of another handle may use on its own stream: finish them

@@ -67,6 +67,8 @@
 // masks reduce to the grid's own edges.
 #include "sheet_fused.hpp"
 
+#include <hip/hip_ext.h>
+
 #include <math.h>
 
 #include <algorithm>
@@ -1210,8 +1212,17 @@ std::vector<int> balanced_rows(const std::vector<LayerArgs>& wins, int tiles, co
   return rows;
 }
 
+template <typename K>
+static void launch(K kernel, dim3 grid, dim3 blk, hipStream_t st, hipEvent_t e0, hipEvent_t e1, const FusedArgs& fa) {
+  if (e0 || e1)
+    hipExtLaunchKernelGGL(kernel, grid, blk, 0, st, e0, e1, 0, fa);
+  else
+    hipLaunchKernelGGL(kernel, grid, blk, 0, st, fa);
+}
+
 template <bool EXACT, bool FULL>
-static int dispatch(const FusedArgs* fa, int tiles, bool project, bool wind, hipStream_t st) {
+static int dispatch(const FusedArgs* fa, int tiles, bool project, bool wind, hipStream_t st, hipEvent_t e0,
+                    hipEvent_t e1) {
   if (!fa) {
     if (wind) return resident_tiles<EXACT, false, true, FULL>();
     if (project) return resident_tiles<EXACT, true, false, FULL>();
@@ -1220,20 +1231,20 @@ static int dispatch(const FusedArgs* fa, int tiles, bool project, bool wind, hip
   dim3 grid((unsigned)tiles);
   dim3 blk(kLanes);
   if (wind)
-    hipLaunchKernelGGL((k_sheet_stream<EXACT, false, true, FULL>), grid, blk, 0, st, *fa);
+    launch(k_sheet_stream<EXACT, false, true, FULL>, grid, blk, st, e0, e1, *fa);
   else if (project)
-    hipLaunchKernelGGL((k_sheet_stream<EXACT, true, false, FULL>), grid, blk, 0, st, *fa);
+    launch(k_sheet_stream<EXACT, true, false, FULL>, grid, blk, st, e0, e1, *fa);
   else
-    hipLaunchKernelGGL((k_sheet_stream<EXACT, false, false, FULL>), grid, blk, 0, st, *fa);
+    launch(k_sheet_stream<EXACT, false, false, FULL>, grid, blk, st, e0, e1, *fa);
   return 0;
 }
 
 static int dispatch_any(const FusedArgs* fa, int tiles, bool exact, bool full, bool project, bool wind,
-                        hipStream_t st) {
-  if (exact) return full ? dispatch<true, true>(fa, tiles, project, wind, st)
-                         : dispatch<true, false>(fa, tiles, project, wind, st);
-  return full ? dispatch<false, true>(fa, tiles, project, wind, st)
-              : dispatch<false, false>(fa, tiles, project, wind, st);
+                        hipStream_t st, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
+  if (exact) return full ? dispatch<true, true>(fa, tiles, project, wind, st, e0, e1)
+                         : dispatch<true, false>(fa, tiles, project, wind, st, e0, e1);
+  return full ? dispatch<false, true>(fa, tiles, project, wind, st, e0, e1)
+              : dispatch<false, false>(fa, tiles, project, wind, st, e0, e1);
 }
 
 }  // namespace
@@ -1317,7 +1328,7 @@ int fused_build(const FusedLayer* wins, int n, bool exact, int rows, FusedLaunch
 }
 
 int fused_enqueue(const FusedLaunch& l, const void* dev_table, const int* dev_tile_window, hipStream_t st,
-                  const int* dev_rows, unsigned long long* clocks) {
+                  const int* dev_rows, unsigned long long* clocks, hipEvent_t ev_start, hipEvent_t ev_stop) {
   if (!dev_table || (l.nwin > 2 && !dev_tile_window) || l.tiles < 1) return SILT_EINVAL;
   FusedArgs fa{};
   fa.table = (ConstLayer*)dev_table;
@@ -1325,7 +1336,7 @@ int fused_enqueue(const FusedLaunch& l, const void* dev_table, const int* dev_ti
   fa.rows = (ConstInt*)dev_rows;
   fa.clocks = clocks;
   fa.nwin = l.nwin;
-  dispatch_any(&fa, l.tiles, l.exact, l.full, l.project, l.wind, st);
+  dispatch_any(&fa, l.tiles, l.exact, l.full, l.project, l.wind, st, ev_start, ev_stop);
   return hipGetLastError() == hipSuccess ? 0 : SILT_EDEVICE;
 }
 
@@ -1340,7 +1351,7 @@ int fused_substep_windows(const FusedLayer* wins, int n, bool exact, int rows, h
   rc = upload_descs(descs, l.tile_window, l.rows, st, &fa);
   if (rc != 0) return rc;
   return fused_enqueue(l, (const void*)fa.table, (const int*)fa.tile_window, st, (const int*)fa.rows,
-                       bal ? bal->clocks : nullptr);
+                       bal ? bal->clocks : nullptr, bal ? bal->ev_start : nullptr, bal ? bal->ev_stop : nullptr);
 }
 
 int fused_substep_layers(const FusedLayer* layers, int nlayers, const float* bed, int res, bool exact,

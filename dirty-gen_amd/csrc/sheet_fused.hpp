@@ -125,6 +125,9 @@ void fine_scratch_free(FineScratch& fs);
 struct FusedBalance {
   const float* xcd_w = nullptr;  // [8], or null for uniform chunks
   unsigned long long* clocks = nullptr;
+  // When set, the launch carries these events in its own dispatch
+  // (hipExtLaunchKernelGGL): they time the kernel alone.
+  hipEvent_t ev_start = nullptr, ev_stop = nullptr;
 };
 constexpr int kXcds = 8;
 // The XCD that runs linear tile `tile` of a launch of `tiles` workgroups.
@@ -159,7 +162,8 @@ bool fused_boxes_fit(const std::vector<Box>& boxes);
 int fused_build(const FusedLayer* wins, int n, bool exact, int rows, FusedLaunch* out,
                 const float* xcd_w = nullptr);
 int fused_enqueue(const FusedLaunch& l, const void* dev_table, const int* dev_tile_window, hipStream_t st,
-                  const int* dev_rows = nullptr, unsigned long long* clocks = nullptr);
+                  const int* dev_rows = nullptr, unsigned long long* clocks = nullptr,
+                  hipEvent_t ev_start = nullptr, hipEvent_t ev_stop = nullptr);
 
 // One planned step of one layer on the fused path.  Returns the substep count,
 // SILT_EUNSUPPORTED for plans the fused kernel does not cover (pace-3 slices,

@@ -69,7 +69,9 @@ int silt_pair_step(silt_pair *p, float dt);
 int silt_pair_ledger(silt_pair *p, silt_pair_ledger_t *out);
 
 /* GPU time of the last step's kernels, milliseconds (HIP events on the
- * handle's stream; waits for the step's launches). */
+ * handle's stream; a step of one fused launch carries them in the launch's
+ * own dispatch, so they time the kernel alone; waits for the step's
+ * launches). */
 float silt_pair_last_kernel_ms(const silt_pair *p);
 /* Speculative launch of the last plan (default on); no counterpart in the
  * reference: a host-latency hiding switch of the GPU path.  Counts of
