@@ -196,10 +196,32 @@ def run_batch(args):
         n = w["res"] ** 2
         b.upload(k, w["bed"], w["water"], np.zeros(n, np.float32), np.zeros(n, np.float32), 0.9, 0.08)
     rows, tune_ms = 0, None
+    balance = None
+    if not args.no_balance and args.path != "exact":
+        # The chip's XCD weights, calibrated on a 4096^2 pair (the speed of
+        # an XCD is the chip's, not the workload's); the autotuner keeps them
+        # only when the batch's step is faster with them.
+        sc = scenes.large_scene(res=4096)
+        z = np.zeros(sc["res"] ** 2, np.float32)
+        pair = S.SheetPair(sc["res"], sc["cell"], path=args.path)
+        pair.upload(sc["bed"], sc["water"], z, z, sc["slurry"], z, z, sc["nu_slurry"])
+        w, (t_uni, t_bal), _, _ = pair.balance(dt)
+        pair.close()
+        del sc, z
+        balance = {"weights": [round(x, 4) for x in w], "pair_4096_kernel_ms": [t_uni, t_bal]}
+        if any(x != 1.0 for x in w):
+            b.set_xcd_weights(w)
+            if not args.autotune:
+                _, ms_w = b.autotune(dt, rows=[0], reps=10)
+                balance["kept"] = b.xcd_weights_on()
+                balance["step_kernel_ms"] = ms_w
+        print(f"[bench] XCD weights {balance}", file=sys.stderr, flush=True)
     if args.autotune:
         rows, tune_ms = b.autotune(dt)
         print(f"[bench] autotuned rows per workgroup: {rows or 'default'} ({tune_ms:.3f} ms)", file=sys.stderr,
               flush=True)
+        if balance is not None:
+            balance["kept"] = b.xcd_weights_on()
     for _ in range(args.warmup):
         b.step(dt)
     kernel_ms, fallbacks = [], 0
@@ -225,6 +247,7 @@ def run_batch(args):
         "data": "synthetic (seeded fBm beds, 1-3 Gaussian puddles per window)",
         "config": {"workload": f"{len(ws)} windows, sides U[128,1024] (seed 11), {cells} cells, film {args.film} m",
                    "path": args.path, "rows_per_workgroup": rows or "default", "graphs": b.graphs_captured(),
+                   "xcd_balance": balance,
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_sheet_stream (all windows)",

@@ -50,9 +50,11 @@ struct Window {
 struct Slot {
   unsigned char* d_table = nullptr;
   int* d_tiles = nullptr;
+  int* d_rows = nullptr;  // [2 tiles] rows of every tile under XCD weights
   unsigned char* h_table = nullptr;
   int* h_tiles = nullptr;
-  size_t cap_table = 0, cap_tiles = 0;
+  int* h_rows = nullptr;
+  size_t cap_table = 0, cap_tiles = 0, cap_rows = 0;
 };
 
 }  // namespace
@@ -72,6 +74,10 @@ struct silt_batch {
   std::vector<long long> graph_key;
   int graphs = 0;
   int rows = 0;
+  // XCD weights (silt_batch_set_xcd_weights): xcd_have once given, xcd_on
+  // while the launches use them (the autotuner keeps them only when faster).
+  float xcd_w[kXcds] = {1, 1, 1, 1, 1, 1, 1, 1};
+  bool xcd_have = false, xcd_on = false;
   float last_ms = 0.0f;
   int last_fallbacks = 0;
   PlanLabeller lab;
@@ -80,7 +86,7 @@ struct silt_batch {
 
 namespace {
 
-int grow(Slot& s, size_t table_bytes, size_t tiles) {
+int grow(Slot& s, size_t table_bytes, size_t tiles, size_t rows) {
   if (s.cap_table < table_bytes) {
     if (s.d_table) (void)hipFree(s.d_table);
     if (s.h_table) (void)hipHostFree(s.h_table);
@@ -104,6 +110,17 @@ int grow(Slot& s, size_t table_bytes, size_t tiles) {
       return SILT_ENOMEM;
     s.cap_tiles = want;
   }
+  if (s.cap_rows < rows) {
+    if (s.d_rows) (void)hipFree(s.d_rows);
+    if (s.h_rows) (void)hipHostFree(s.h_rows);
+    s.d_rows = nullptr;
+    s.h_rows = nullptr;
+    s.cap_rows = 0;
+    if (hipMalloc(&s.d_rows, rows * sizeof(int)) != hipSuccess ||
+        hipHostMalloc(&s.h_rows, rows * sizeof(int), hipHostMallocDefault) != hipSuccess)
+      return SILT_ENOMEM;
+    s.cap_rows = rows;
+  }
   return 0;
 }
 
@@ -126,7 +143,9 @@ int enqueue_step(silt_batch* b, const std::vector<FusedLaunch>& launches) {
     HIPCHK(hipMemcpyAsync(sl.d_table, sl.h_table, l.descs.size(), hipMemcpyHostToDevice, st));
     if (!l.tile_window.empty())
       HIPCHK(hipMemcpyAsync(sl.d_tiles, sl.h_tiles, l.tile_window.size() * sizeof(int), hipMemcpyHostToDevice, st));
-    int rc = fused_enqueue(l, sl.d_table, sl.d_tiles, st);
+    if (!l.rows.empty())
+      HIPCHK(hipMemcpyAsync(sl.d_rows, sl.h_rows, l.rows.size() * sizeof(int), hipMemcpyHostToDevice, st));
+    int rc = fused_enqueue(l, sl.d_table, sl.d_tiles, st, l.rows.empty() ? nullptr : sl.d_rows);
     if (rc != 0) return rc;
   }
   return 0;
@@ -193,6 +212,8 @@ void silt_batch_destroy(silt_batch* b) {
     (void)hipFree(s.d_tiles);
     if (s.h_table) (void)hipHostFree(s.h_table);
     if (s.h_tiles) (void)hipHostFree(s.h_tiles);
+    if (s.d_rows) (void)hipFree(s.d_rows);
+    if (s.h_rows) (void)hipHostFree(s.h_rows);
   }
   (void)hipFree(b->stats);
   (void)hipFree(b->totals);
@@ -337,13 +358,15 @@ int silt_batch_step(silt_batch* b, float dt) {
       launches.resize((size_t)s);
       break;
     }
-    int rc = fused_build(wins.data(), (int)wins.size(), exact, b->rows, &launches[(size_t)s]);
+    int rc = fused_build(wins.data(), (int)wins.size(), exact, b->rows, &launches[(size_t)s],
+                         b->xcd_on ? b->xcd_w : nullptr);
     if (rc != 0) return rc;
     const FusedLaunch& l = launches[(size_t)s];
     Slot& sl = b->slot[s];
-    if ((rc = grow(sl, l.descs.size(), l.tile_window.size())) != 0) return rc;
+    if ((rc = grow(sl, l.descs.size(), l.tile_window.size(), l.rows.size())) != 0) return rc;
     memcpy(sl.h_table, l.descs.data(), l.descs.size());
     if (!l.tile_window.empty()) memcpy(sl.h_tiles, l.tile_window.data(), l.tile_window.size() * sizeof(int));
+    if (!l.rows.empty()) memcpy(sl.h_rows, l.rows.data(), l.rows.size() * sizeof(int));
   }
   // The launch shape: a new graph when it changes.
   std::vector<long long> key;
@@ -351,9 +374,10 @@ int silt_batch_step(silt_batch* b, float dt) {
     key.push_back(l.tiles);
     key.push_back(l.nwin);
     key.push_back(((long long)l.exact << 3) | ((long long)l.full << 2) | ((long long)l.project << 1) | l.wind);
+    key.push_back((long long)l.rows.size());
     const Slot& sl = b->slot[&l - launches.data()];
     for (const void* ptr : {(const void*)sl.d_table, (const void*)sl.d_tiles, (const void*)sl.h_table,
-                            (const void*)sl.h_tiles})
+                            (const void*)sl.h_tiles, (const void*)sl.d_rows, (const void*)sl.h_rows})
       key.push_back((long long)(size_t)ptr);
   }
   HIPCHK(hipEventRecord(b->ev0, st));
@@ -518,12 +542,52 @@ int silt_batch_autotune(silt_batch* b, float dt, const int* rows, size_t n, int 
     }
   }
   b->rows = rc == 0 ? best_c : rows0;
+  // Second knob: with XCD weights given, the best height with and without
+  // them; the weights stay on only when faster.
+  if (rc == 0 && b->xcd_have) {
+    const bool on0 = b->xcd_on;
+    b->xcd_on = !on0;
+    rc = silt_batch_step(b, dt);
+    double sum = 0.0;
+    for (int k = 0; k < reps && rc >= 0; k++) {
+      rc = silt_batch_step(b, dt);
+      sum += b->last_ms;
+    }
+    int rr = restore();
+    if (rc >= 0) rc = rr;
+    if (rc >= 0) {
+      rc = 0;
+      const float mean = (float)(sum / reps);
+      if (mean < best)
+        best = mean;
+      else
+        b->xcd_on = on0;
+    } else {
+      b->xcd_on = on0;
+    }
+  }
   (void)hipFree(save);
   if (rc < 0) return rc;
   if (best_rows) *best_rows = best_c;
   if (best_ms) *best_ms = best;
   return SILT_OK;
 }
+
+int silt_batch_set_xcd_weights(silt_batch* b, const float* w) {
+  if (!b) return SILT_EINVAL;
+  if (!w) {
+    b->xcd_have = b->xcd_on = false;
+    for (int x = 0; x < kXcds; x++) b->xcd_w[x] = 1.0f;
+    return SILT_OK;
+  }
+  for (int x = 0; x < kXcds; x++)
+    if (!(w[x] >= 0.25f && w[x] <= 4.0f)) return SILT_EINVAL;
+  for (int x = 0; x < kXcds; x++) b->xcd_w[x] = w[x];
+  b->xcd_have = b->xcd_on = true;
+  return SILT_OK;
+}
+
+int silt_batch_xcd_weights_on(const silt_batch* b) { return b ? (b->xcd_on ? 1 : 0) : SILT_EINVAL; }
 
 }  // extern "C"
 

@@ -134,6 +134,8 @@ def _declare(L):
     L.silt_batch_upload.argtypes = [_P, _Z, _P, _P, _P, _P, _F, _F]
     L.silt_batch_download.argtypes = [_P, _Z, _P, _P, _P]
     L.silt_batch_step.argtypes = [_P, _F]
+    L.silt_batch_set_xcd_weights.argtypes = [_P, C.POINTER(C.c_float)]
+    L.silt_batch_xcd_weights_on.argtypes = [_P]
     L.silt_batch_ledger.argtypes = [_P, _Z, C.POINTER(LayerLedger)]
     L.silt_batch_last_kernel_ms.argtypes = [_P]
     L.silt_batch_last_kernel_ms.restype = C.c_float
@@ -327,6 +329,16 @@ class Batch:
         _check_status(lib().silt_batch_autotune(self._h, dt, arr, len(rows) if rows else 0, reps, C.byref(best),
                                                 C.byref(ms)))
         return best.value, ms.value
+
+    def set_xcd_weights(self, weights):
+        """XCD weights for the launches (8 in [0.25, 4], e.g. from
+        SheetPair.balance on this chip); None clears them.  autotune() keeps
+        them only when they make the step faster."""
+        arr = None if weights is None else (C.c_float * 8)(*[float(w) for w in weights])
+        _check_status(lib().silt_batch_set_xcd_weights(self._h, arr))
+
+    def xcd_weights_on(self):
+        return bool(_check_status(lib().silt_batch_xcd_weights_on(self._h)))
 
 
 def plan_boxes(planes, cell, eps=1.0e-6, cap=64):

@@ -62,9 +62,21 @@ int silt_batch_use_graph(silt_batch *b, int on);
 int silt_batch_set_rows(silt_batch *b, int rows);
 /* Time every candidate rows-per-workgroup (n of them; NULL or n = 0: a
  * built-in set) over reps steps from the current state, which is restored
- * after each; keep the fastest.  best_rows / best_ms may be NULL. */
+ * after each; keep the fastest.  With XCD weights given, the fastest height
+ * is also timed with them switched the other way, and the faster setting
+ * is kept.  best_rows / best_ms may be NULL. */
 int silt_batch_autotune(silt_batch *b, float dt, const int *rows, size_t n, int reps, int *best_rows,
                         float *best_ms);
+
+/* XCD weights for the batch's launches (as silt_pair_set_xcd_weights: the
+ * rows of the tiles XCD x runs scaled by weights[x], in [0.25, 4]; NULL
+ * clears them).  Given weights are on until silt_batch_autotune times the
+ * best height with and without them and keeps the faster;
+ * silt_batch_xcd_weights_on says which it kept.  Results do not depend on
+ * them.  No counterpart in the reference: a tuning knob of the GPU path,
+ * usually fed the weights silt_pair_balance calibrated on the same chip. */
+int silt_batch_set_xcd_weights(silt_batch *b, const float *weights);
+int silt_batch_xcd_weights_on(const silt_batch *b);
 
 /* Test hook, no counterpart in the reference (whose sheet_work_boxes,
  * src/fluid_core.rs:1325-1447, it answers for device planes): the work boxes

@@ -118,6 +118,37 @@ def test_batch_autotune_restores_the_state():
         b.close()
 
 
+def test_xcd_weights_leave_the_batch_results_unchanged():
+    """Rows of every launch's tiles scaled per XCD give the same planes bit
+    for bit as uniform chunks, through graph replays and the autotuner's
+    second knob (which restores the state)."""
+    ws, nus, vorts = _windows()
+    a, b = _batch(ws, nus, vorts, "fast"), _batch(ws, nus, vorts, "fast")
+    try:
+        b.set_xcd_weights([0.6, 1.4, 0.8, 1.2, 1.0, 0.7, 1.3, 1.0])
+        assert b.xcd_weights_on()
+        for _ in range(6):
+            a.step(DT)
+            b.step(DT)
+        for k in range(len(ws)):
+            for x, y, name in zip(a.download(k), b.download(k), "huv"):
+                _eq(y, x, f"window {k} {name} with XCD weights")
+        before = [b.download(k) for k in range(len(ws))]
+        b.autotune(DT, rows=[0], reps=2)
+        for k in range(len(ws)):
+            for x, y, name in zip(b.download(k), before[k], "huv"):
+                _eq(x, y, f"window {k} {name} after the XCD knob")
+        for _ in range(3):
+            a.step(DT)
+            b.step(DT)
+        for k in range(len(ws)):
+            for x, y, name in zip(a.download(k), b.download(k), "huv"):
+                _eq(y, x, f"window {k} {name} after tuning")
+    finally:
+        a.close()
+        b.close()
+
+
 def test_wet_bodies_labelled_on_the_gpu_give_the_host_boxes():
     """Scattered puddles: the reference's host scan plans 4 split boxes,
     more than 8 (so the wet bounding box), 7, and bodies whose grown boxes
