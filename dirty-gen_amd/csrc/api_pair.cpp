@@ -729,7 +729,7 @@ int silt_pair_balance(silt_pair* p, float dt, int reps, float* weights, float* m
   // Round 0 runs uniform chunks.  Each later round moves rows toward the
   // XCDs that finished early: with u[x] the mean pipeline rows of XCD x's
   // tiles (output rows n[x] plus the fill F around them) and f[x] when its
-  // last workgroup ended, u'[x] is proportional to u[x] / f[x] with the
+  // last workgroup ended, u'[x] is proportional to u[x] f[x]^-a with the
   // total rows kept, and the weights are n'[x] = u'[x] - F relative to
   // their mean (an XCD's rate moves with its rows, so this is a search,
   // not a solve).  Every
@@ -738,9 +738,13 @@ int silt_pair_balance(silt_pair* p, float dt, int reps, float* weights, float* m
   const double fill = 2.0 * 15.0;
   float w[kXcds] = {1, 1, 1, 1, 1, 1, 1, 1}, best_w[kXcds] = {1, 1, 1, 1, 1, 1, 1, 1};
   double best_ms = INFINITY, t_uni = 0.0;
-  XcdRound first, last;
+  XcdRound first, last, best_r;
   p->xcd_on = true;
-  const int rounds = 8;
+  // A hill climb: each round's weights come from the best round so far,
+  // with the step (the power of the finish-time ratio) halved after a round
+  // that did not beat it.
+  const int rounds = 12;
+  double alpha = 1.0;
   for (int round = 0; round < rounds && rc >= 0; round++) {
     memcpy(p->xcd_w, w, sizeof w);
     XcdRound r;
@@ -756,11 +760,14 @@ int silt_pair_balance(silt_pair* p, float dt, int reps, float* weights, float* m
     if (r.kernel_ms < best_ms) {
       best_ms = r.kernel_ms;
       memcpy(best_w, w, sizeof w);
+      best_r = r;
+    } else {
+      alpha *= 0.5;
     }
     double rows = 0.0, scaled = 0.0, u[kXcds];
     for (int x = 0; x < kXcds; x++) {
-      u[x] = r.rows[x] / r.finish[x];
-      rows += r.rows[x];
+      u[x] = best_r.rows[x] * std::pow(best_r.finish[x], -alpha);
+      rows += best_r.rows[x];
       scaled += u[x];
     }
     double n[kXcds], nsum = 0.0;

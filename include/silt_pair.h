@@ -103,10 +103,11 @@ int silt_pair_autotune(silt_pair *p, float dt, const int *rows, size_t n, int re
  * the weights of the XCDs that run its chunks); NULL restores uniform
  * chunks.  Weights in [0.25, 4].  Results do not depend on the split.
  *
- * silt_pair_balance calibrates them on this chip.  Eight rounds of reps steps
- * record every workgroup's start and end on the real-time clock: round 0
- * with uniform chunks, each later one moving rows toward the XCDs whose last
- * workgroup ended early (the oldest workgroups on a CU are issued first, so
+ * silt_pair_balance calibrates them on this chip.  Twelve rounds of reps
+ * steps record every workgroup's start and end on the real-time clock:
+ * round 0 with uniform chunks, each later one moving rows, from the best
+ * round so far, toward the XCDs whose last workgroup ended early (a smaller
+ * move after a round that was not faster) (the oldest workgroups on a CU are issued first, so
  * one XCD's tiles end at very different times: what counts is when the XCD
  * is done).  Each round is scored by the launch's mean kernel time and the
  * fastest weights are kept, uniform chunks included (ms[0] uniform, ms[1]
