@@ -63,8 +63,10 @@ struct silt_pair {
   SheetTotals* totals_save = nullptr;
   SheetStats* stats_save = nullptr;  // [2] device: stats before a speculative launch
   hipEvent_t ev_stats = nullptr;
-  // The statistics no longer describe the planes (the network's fluxes
-  // moved depth after the step wrote them): recomputed before they are read.
+  // The statistics' plan fields (wet count and box, largest speed) no longer
+  // describe the planes (the network's fluxes moved depth after the step
+  // wrote them): recomputed before a plan reads them.  The volume stays
+  // exact (the flux apply adds its FP64 change), so the ledger needs none.
   bool stats_stale = false;
   long long spec_hits = 0, spec_misses = 0;
   // Back-off after an undone speculation: the next spec_pause steps plan
@@ -479,7 +481,6 @@ int silt_pair_step(silt_pair* p, float dt) {
 
 int silt_pair_ledger(silt_pair* p, silt_pair_ledger_t* out) {
   if (!p || !out) return SILT_EINVAL;
-  refresh_stats(p);
   SheetTotals t[2];
   HIPCHK(hipMemcpyAsync(p->h_stats, p->stats, 2 * sizeof(SheetStats), hipMemcpyDeviceToHost, p->stream));
   HIPCHK(hipMemcpyAsync(t, p->totals, sizeof t, hipMemcpyDeviceToHost, p->stream));
@@ -1023,12 +1024,12 @@ int silt_pair_step_tires(silt_pair* p, float dt, const silt_tire_push* pushes, s
     float* wh = p->planes[0][p->cur[0]][0];
     float* sh = p->planes[1][p->cur[1]][0];
     if (batched) {
-      sur_apply_many(wh, sh, res, p->net.y, d_jobs, (int)groups, p->sur_fac, st, p->totals);
+      sur_apply_many(wh, sh, res, p->net.y, d_jobs, (int)groups, p->sur_fac, st, p->totals, p->stats);
     } else {
       for (size_t v = 0; v < groups; v++) {
         if (!jobs[v].r.nx) continue;
         sur_apply(wh, sh, res, p->net.y + (size_t)jobs[v].row * kMlpOut, jobs[v].r,
-                  p->sur_fac + v * 2 * kSurMax * kSurMax, st, p->totals);
+                  p->sur_fac + v * 2 * kSurMax * kSurMax, st, p->totals, p->stats);
       }
     }
     p->stats_stale = true;

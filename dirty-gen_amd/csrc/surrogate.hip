@@ -279,7 +279,7 @@ __device__ inline int sur_nb(const SurRegion& r, int q, int dir) {
 // and layer and added to that layer's residue line.
 __device__ inline void sur_factor_apply_wg(float* __restrict__ water, float* __restrict__ slurry, int res,
                                            const float* __restrict__ yv, const SurRegion& r, float* fac,
-                                           SheetTotals* totals) {
+                                           SheetTotals* totals, SheetStats* stats) {
   const int t = (int)threadIdx.x;
   const int cells = r.nx * r.nz;
   const bool mine = t < 2 * cells;
@@ -293,7 +293,7 @@ __device__ inline void sur_factor_apply_wg(float* __restrict__ water, float* __r
     fac[t] = out > hv ? 0.999999f * hv / out : 1.0f;
   }
   __syncthreads();
-  double res_w = 0.0, res_s = 0.0;
+  double res_w = 0.0, res_s = 0.0, dv_w = 0.0, dv_s = 0.0;
   if (mine) {
     float net = 0.0f;
     double out = 0.0;
@@ -311,32 +311,40 @@ __device__ inline void sur_factor_apply_wg(float* __restrict__ water, float* __r
     const double rr = ((double)hn - (double)hv) + out;
     res_w = layer == 0 ? rr : 0.0;
     res_s = layer == 1 ? rr : 0.0;
+    // The statistics' volume is the FP64 sum of max(h, 0): its change here.
+    const double dv = (double)fmaxf(hn, 0.0f) - (double)fmaxf(hv, 0.0f);
+    dv_w = layer == 0 ? dv : 0.0;
+    dv_s = layer == 1 ? dv : 0.0;
   }
-  if (!totals) return;
+  if (!totals && !stats) return;
   for (int off = 32; off > 0; off >>= 1) {
     res_w += __shfl_xor(res_w, off);
     res_s += __shfl_xor(res_s, off);
+    dv_w += __shfl_xor(dv_w, off);
+    dv_s += __shfl_xor(dv_s, off);
   }
   if ((t & 63) == 0) {
-    if (res_w != 0.0) atomicAdd(&totals[0].residue, res_w);
-    if (res_s != 0.0) atomicAdd(&totals[1].residue, res_s);
+    if (totals && res_w != 0.0) atomicAdd(&totals[0].residue, res_w);
+    if (totals && res_s != 0.0) atomicAdd(&totals[1].residue, res_s);
+    if (stats && dv_w != 0.0) atomicAdd(&stats[0].vol, dv_w);
+    if (stats && dv_s != 0.0) atomicAdd(&stats[1].vol, dv_s);
   }
 }
 
 __global__ __launch_bounds__(1024) void k_sur_factor_apply(float* __restrict__ water, float* __restrict__ slurry,
                                                            int res, const float* __restrict__ y, SurRegion r,
-                                                           SheetTotals* totals) {
+                                                           SheetTotals* totals, SheetStats* stats) {
   __shared__ float fac[2 * kSurMax * kSurMax];
-  sur_factor_apply_wg(water, slurry, res, y, r, fac, totals);
+  sur_factor_apply_wg(water, slurry, res, y, r, fac, totals, stats);
 }
 
 __global__ __launch_bounds__(1024) void k_sur_factor_apply_many(float* __restrict__ water, float* __restrict__ slurry,
                                                                 int res, const float* __restrict__ y,
                                                                 const TireJob* __restrict__ jobs,
-                                                                SheetTotals* totals) {
+                                                                SheetTotals* totals, SheetStats* stats) {
   __shared__ float fac[2 * kSurMax * kSurMax];
   const TireJob J = jobs[blockIdx.x];
-  sur_factor_apply_wg(water, slurry, res, y + (size_t)J.row * kMlpOut, J.r, fac, totals);
+  sur_factor_apply_wg(water, slurry, res, y + (size_t)J.row * kMlpOut, J.r, fac, totals, stats);
 }
 
 __global__ void k_sur_max_abs_diff(const float* __restrict__ a, const float* __restrict__ b, int n,
@@ -429,11 +437,11 @@ void sur_gather_many(const float* ctx, const TireJob* jobs, int n, uint16_t* xb,
 }
 
 void sur_apply_many(float* water, float* slurry, int res, const float* y, const TireJob* jobs, int n, float* fac,
-                    hipStream_t st, SheetTotals* totals) {
+                    hipStream_t st, SheetTotals* totals, SheetStats* stats) {
   (void)fac;  // the factors live in LDS
   if (n <= 0) return;
   hipLaunchKernelGGL(k_sur_factor_apply_many, dim3((unsigned)n), dim3(1024), 0, st, water, slurry, res, y, jobs,
-                     totals);
+                     totals, stats);
 }
 
 void sur_gather_region(const float* ctx, const SurRegion& r, uint16_t* xb, hipStream_t st) {
@@ -454,10 +462,10 @@ void sur_targets(const float* hn_w, const float* hc_w, const float* hn_s, const 
 }
 
 void sur_apply(float* water, float* slurry, int res, const float* y, const SurRegion& r, float* fac,
-               hipStream_t st, SheetTotals* totals) {
+               hipStream_t st, SheetTotals* totals, SheetStats* stats) {
   (void)fac;  // the factors live in LDS
   if (r.nx <= 0 || r.nz <= 0) return;
-  hipLaunchKernelGGL(k_sur_factor_apply, dim3(1), dim3(1024), 0, st, water, slurry, res, y, r, totals);
+  hipLaunchKernelGGL(k_sur_factor_apply, dim3(1), dim3(1024), 0, st, water, slurry, res, y, r, totals, stats);
 }
 
 void sur_max_abs_diff(const float* a, const float* b, int n, unsigned* out, hipStream_t st) {

@@ -18,6 +18,7 @@
 namespace silt {
 
 struct SheetTotals;
+struct SheetStats;
 
 constexpr int kSurReach = 8;                     // stencil offsets -8 .. +7
 constexpr int kSurMax = 20;                      // largest region side
@@ -70,9 +71,12 @@ void sur_targets(const float* hn_w, const float* hc_w, const float* hn_s, const 
 // faces inside R carry the mean of their two cells' predictions, limited so
 // no donor gives more than it holds.  fac: scratch of 2 * nx * nz floats.
 // totals (water, slurry), when given, book in FP64 what the FP32 update of
-// each cell adds beyond its exchanged fluxes (their residue line).
+// each cell adds beyond its exchanged fluxes (their residue line); stats
+// (water, slurry), when given, get the change of their FP64 volume (the sum
+// of max(h, 0)), so the volume stays that of the planes (the other fields
+// do not follow).
 void sur_apply(float* water, float* slurry, int res, const float* y, const SurRegion& r, float* fac,
-               hipStream_t st, SheetTotals* totals = nullptr);
+               hipStream_t st, SheetTotals* totals = nullptr, SheetStats* stats = nullptr);
 // Many vehicles in one launch per stage: vehicle v's region, its first row
 // of network inputs / outputs, its pushes and their mean velocity, and the
 // box its pushes shove.  Regions (and push boxes) must not overlap.  The
@@ -90,7 +94,7 @@ void sur_context_many(const SurGrid& g, const TireJob* jobs, const CorePush* pus
                       hipStream_t st);
 void sur_gather_many(const float* ctx, const TireJob* jobs, int n, uint16_t* xb, hipStream_t st);
 void sur_apply_many(float* water, float* slurry, int res, const float* y, const TireJob* jobs, int n, float* fac,
-                    hipStream_t st, SheetTotals* totals = nullptr);
+                    hipStream_t st, SheetTotals* totals = nullptr, SheetStats* stats = nullptr);
 // Sum over the region of (a - b)^2 added into *out (FP64).
 void sur_region_sq_diff(const float* a, const float* b, int res, const SurRegion& r, double* out, hipStream_t st);
 // Largest |a - b| over n cells, as float bits, max-ed into *out.

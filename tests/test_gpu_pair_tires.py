@@ -64,6 +64,13 @@ def test_ruts_follow_the_oracle_and_the_ledger_closes_with_the_network():
             led = p.ledger()
             for L in (led.water, led.slurry):
                 assert abs(L.gap_m3) <= 1e-9 * L.initial_m3, (k, L.gap_m3, L.initial_m3)
+            # The ledger's volume is kept by the flux apply (no recount of
+            # the planes): it must equal the planes' own FP64 volume.
+            if k % 4 == 3:
+                h = p.download()
+                for L, plane in ((led.water, h[0]), (led.slurry, h[3])):
+                    vol = float(np.sum(np.maximum(plane, 0.0).astype(np.float64))) * float(np.float32(CELL)) ** 2
+                    assert abs(L.volume_m3 - vol) <= 1e-11 * L.initial_m3, (k, L.volume_m3, vol)
         gb, gp, rut = p.ground()
         assert np.array_equal(gb, ob) and np.array_equal(gp, op)
         assert rut.max() > 0.005
